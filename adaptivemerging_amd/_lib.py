@@ -1,0 +1,218 @@
+"""ctypes binding of libam.so (include/am.h).
+
+The library is loaded from this package directory.  There is no fallback: if
+libam.so is missing or a call fails, an AMError is raised.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libam.so")
+
+AM_OK = 0
+AM_ERR_INVALID = -1
+AM_ERR_HIP = -2
+AM_ERR_CAPACITY = -3
+AM_ERR_STATE = -4
+AM_ERR_NUMERIC = -5
+
+AM_GEOM_BOX = 0
+AM_GEOM_TREE = 1
+AM_GEOM_PLANE = 2
+AM_GEOM_COLLECTION = 3
+
+AM_BODY_PINNED = 1
+AM_BODY_MAGNET_ACTIVE = 2
+AM_BODY_SLEEPING = 4
+
+AM_STATE_BROKEN = 0
+AM_STATE_ONEDGE = 1
+AM_STATE_CLEAR = 2
+
+AM_BV_NONE = -1
+AM_BV_PLANE = -2
+
+# numpy views of the C structs (no padding in either layout)
+BODY_DTYPE = np.dtype(
+    [
+        ("geom", "<i4"), ("flags", "<i4"), ("tree_first", "<i4"), ("tree_count", "<i4"),
+        ("scene", "<i4"), ("reserved", "<i4"),
+        ("size", "<f8", 3), ("radius", "<f8"), ("mass", "<f8"), ("minv", "<f8"),
+        ("jinv0", "<f8", 9), ("friction", "<f8"), ("restitution", "<f8"),
+        ("plane_n", "<f8", 3), ("plane_p", "<f8", 3), ("plane_d", "<f8"), ("bbox", "<f8", 24),
+    ]
+)
+NODE_DTYPE = np.dtype([("c", "<f8", 3), ("r", "<f8"), ("first_child", "<i4"), ("n_children", "<i4")])
+assert BODY_DTYPE.itemsize == 408 and NODE_DTYPE.itemsize == 40
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32), ("iterations_in_collection", C.c_int32),
+        ("tolerance", C.c_double), ("omega", C.c_double), ("feedback_stiffness", C.c_double),
+        ("enable_post_stabilization", C.c_int32), ("enable_compliance", C.c_int32),
+        ("compliance", C.c_double),
+        ("restitution_override", C.c_int32), ("friction_override", C.c_int32),
+        ("restitution", C.c_double), ("friction", C.c_double), ("sliding_threshold", C.c_double),
+        ("warm_start", C.c_int32), ("use_gravity", C.c_int32),
+        ("gravity_amount", C.c_double), ("gravity_angle_deg", C.c_double),
+        ("viscous_linear_decay", C.c_double), ("viscous_angular_decay", C.c_double),
+        ("enable_merging", C.c_int32), ("enable_merge_pinned", C.c_int32),
+        ("enable_merge_stable_contact", C.c_int32), ("enable_merge_let_it_breathe", C.c_int32),
+        ("enable_unmerging", C.c_int32), ("enable_unmerge_friction", C.c_int32),
+        ("enable_unmerge_normal", C.c_int32), ("enable_unmerge_relative_motion", C.c_int32),
+        ("update_contacts_in_collections", C.c_int32), ("step_accum_merging", C.c_int32),
+        ("step_accum_unmerging", C.c_int32), ("steps_between_merge_events", C.c_int32),
+        ("threshold_merge", C.c_double), ("threshold_unmerge", C.c_double), ("threshold_breath", C.c_double),
+        ("pgs_block", C.c_int32), ("pgs_grid", C.c_int32),
+    ]
+
+
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int32)
+
+
+class ContactArrays(C.Structure):
+    _fields_ = [
+        ("body1", _IP), ("body2", _IP), ("bv1", _IP), ("bv2", _IP), ("info", _IP),
+        ("point_b1", _DP), ("normal_b1", _DP), ("tangent1_b1", _DP), ("tangent2_b1", _DP),
+        ("violation", _DP), ("prev_violation", _DP), ("jacobian", _DP),
+        ("lambda_", _DP), ("lambda_warm", _DP), ("b", _DP), ("diag", _DP),
+        ("state", _IP), ("new_this_step", _IP),
+    ]
+
+
+class StepStats(C.Structure):
+    _fields_ = [
+        ("n_contacts", C.c_int32), ("n_pairs", C.c_int32), ("n_groups", C.c_int32), ("n_colours", C.c_int32),
+        ("iterations_used", C.c_int32), ("bad_warm_starts", C.c_int32),
+        ("bad_warm_starts_repaired", C.c_int32), ("reserved", C.c_int32),
+        ("last_change", C.c_double), ("ms_detect", C.c_double), ("ms_warm_start", C.c_double),
+        ("ms_solve", C.c_double), ("ms_integrate", C.c_double), ("ms_total", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+# Exported symbols of include/am.h; the test suite checks this list against
+# the header and the shared object.
+SYMBOLS = [
+    "am_version", "am_ctx_create", "am_ctx_destroy", "am_last_error", "am_params_default", "am_params_set",
+    "am_scene_upload", "am_collections_set", "am_entity_mass_set", "am_jinv_pull", "am_state_push",
+    "am_state_pull", "am_forces_push", "am_forces_pull", "am_clear_bodies", "am_apply_gravity", "am_deltav_pull",
+    "am_update_contacts_map", "am_detect", "am_contacts_push", "am_contacts_pull", "am_lambdas_push",
+    "am_warm_start", "am_redo_warm_start", "am_no_warm_start", "am_solve", "am_solve_order_pull",
+    "am_colours_pull", "am_single_sweep", "am_changes_pull", "am_advance_velocities", "am_advance_positions",
+    "am_step", "am_motion_metric",
+]
+
+
+class AMError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libam error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Loads libam.so from the package directory (raises if it is missing)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise AMError(AM_ERR_STATE, f"{path} is not built; run adaptivemerging_amd/build.py")
+    lib = C.CDLL(path)
+    i32, dbl, vp = C.c_int32, C.c_double, C.c_void_p
+    ctxp = C.c_void_p
+    sig = {
+        "am_version": (C.c_char_p, []),
+        "am_ctx_create": (C.c_int, [i32, C.POINTER(ctxp)]),
+        "am_ctx_destroy": (C.c_int, [ctxp]),
+        "am_last_error": (C.c_char_p, [ctxp]),
+        "am_params_default": (C.c_int, [C.POINTER(Params)]),
+        "am_params_set": (C.c_int, [ctxp, C.POINTER(Params)]),
+        "am_scene_upload": (C.c_int, [ctxp, vp, i32, vp, i32]),
+        "am_collections_set": (C.c_int, [ctxp, vp, vp, i32, vp, vp, vp, i32]),
+        "am_entity_mass_set": (C.c_int, [ctxp, i32, dbl, dbl, vp, i32]),
+        "am_jinv_pull": (C.c_int, [ctxp, vp, i32]),
+        "am_state_push": (C.c_int, [ctxp, vp, vp, vp, vp, i32]),
+        "am_state_pull": (C.c_int, [ctxp, vp, vp, vp, vp, i32]),
+        "am_forces_push": (C.c_int, [ctxp, vp, vp, i32]),
+        "am_forces_pull": (C.c_int, [ctxp, vp, vp, i32]),
+        "am_clear_bodies": (C.c_int, [ctxp]),
+        "am_apply_gravity": (C.c_int, [ctxp]),
+        "am_deltav_pull": (C.c_int, [ctxp, vp, i32]),
+        "am_update_contacts_map": (C.c_int, [ctxp]),
+        "am_detect": (C.c_int, [ctxp, dbl, C.POINTER(i32)]),
+        "am_contacts_push": (C.c_int, [ctxp, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "am_contacts_pull": (C.c_int, [ctxp, C.POINTER(ContactArrays), i32, C.POINTER(i32)]),
+        "am_lambdas_push": (C.c_int, [ctxp, vp, i32]),
+        "am_warm_start": (C.c_int, [ctxp, i32, C.POINTER(i32), C.POINTER(i32)]),
+        "am_redo_warm_start": (C.c_int, [ctxp]),
+        "am_no_warm_start": (C.c_int, [ctxp]),
+        "am_solve": (C.c_int, [ctxp, dbl, i32, C.POINTER(i32), C.POINTER(dbl)]),
+        "am_solve_order_pull": (C.c_int, [ctxp, vp, i32, C.POINTER(i32)]),
+        "am_colours_pull": (C.c_int, [ctxp, vp, i32, C.POINTER(i32)]),
+        "am_single_sweep": (C.c_int, [ctxp, dbl, C.POINTER(i32)]),
+        "am_changes_pull": (C.c_int, [ctxp, vp, i32, C.POINTER(i32)]),
+        "am_advance_velocities": (C.c_int, [ctxp, dbl]),
+        "am_advance_positions": (C.c_int, [ctxp, dbl]),
+        "am_step": (C.c_int, [ctxp, dbl, C.POINTER(StepStats)]),
+        "am_motion_metric": (C.c_int, [ctxp, vp, i32, vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def default_params() -> Params:
+    p = Params()
+    load().am_params_default(C.byref(p))
+    return p
+
+
+def ptr(a):
+    """Raw data pointer of a numpy array (or None)."""
+    if a is None:
+        return None
+    return C.c_void_p(a.ctypes.data)
+
+
+def dptr(a):
+    return None if a is None else a.ctypes.data_as(_DP)
+
+
+def iptr(a):
+    return None if a is None else a.ctypes.data_as(_IP)
+
+
+# Reference defaults (CollisionProcessor.java:1058-1083, Merging.java:31-48,
+# Contact.java:518, RigidBodySystem.java:545-551); equal to am_params_default.
+_DEFAULTS = dict(
+    iterations=30, iterations_in_collection=1, tolerance=1e-5, omega=1.0, feedback_stiffness=0.5,
+    enable_post_stabilization=0, enable_compliance=1, compliance=1e-3, restitution_override=0,
+    friction_override=0, restitution=0.5, friction=0.1, sliding_threshold=0.01, warm_start=1, use_gravity=1,
+    gravity_amount=1.0, gravity_angle_deg=90.0, viscous_linear_decay=1.0, viscous_angular_decay=1.0,
+    enable_merging=1, enable_merge_pinned=1, enable_merge_stable_contact=1, enable_merge_let_it_breathe=1,
+    enable_unmerging=1, enable_unmerge_friction=1, enable_unmerge_normal=1, enable_unmerge_relative_motion=1,
+    update_contacts_in_collections=1, step_accum_merging=3, step_accum_unmerging=3, steps_between_merge_events=10,
+    threshold_merge=1e-2, threshold_unmerge=2e-2, threshold_breath=1e-5, pgs_block=0, pgs_grid=0,
+)
+
+
+def default_params_host() -> Params:
+    """The reference defaults without loading libam."""
+    p = Params()
+    for k, v in _DEFAULTS.items():
+        setattr(p, k, v)
+    return p

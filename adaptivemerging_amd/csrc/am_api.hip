@@ -1,0 +1,1254 @@
+// am_api.hip - host runtime behind the C ABI of include/am.h: context,
+// device buffers, the per-step orchestration of the kernels on one HIP stream.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "am_solve.hpp"
+
+namespace {
+
+using namespace am;
+
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    // Grows (never shrinks); contents are not preserved.
+    hipError_t ensure(size_t want) {
+        if (want <= n && p) return hipSuccess;
+        release();
+        size_t cap = std::max<size_t>(want, 1);
+        hipError_t e = hipMalloc((void**)&p, cap * sizeof(T));
+        if (e == hipSuccess) n = cap;
+        else p = nullptr;
+        return e;
+    }
+};
+
+}  // namespace
+
+struct am_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    am_params params{};
+
+    // scene (host mirrors)
+    int n_ent = 0, n_phys = 0, n_nodes = 0;
+    std::vector<am_body_desc> desc;
+    std::vector<am_sphere_node> nodes;
+    std::vector<int> parent, slot, sub, slot_entity, member_first, member_count, members, flags;
+    std::vector<int> planes, grid_bodies, node_body, top_pinned, top_sleeping, gravity_targets, integrate_targets;
+    double cell_size = 1.0;
+    int hash_mask = 0;
+
+    // body buffers
+    DBuf<am_body_desc> d_desc;
+    DBuf<am_sphere_node> d_nodes;
+    DBuf<double> d_cw, d_x, d_R, d_v, d_w, d_f, d_t, d_jinv, d_dv, d_minv;
+    DBuf<int> d_flags, d_parent, d_slot, d_sub, d_slot_entity, d_member_first, d_members, d_planes, d_grid,
+        d_node_body, d_top_pinned, d_top_sleeping, d_gravity, d_integrate;
+
+    // broad phase
+    DBuf<double> d_sphere;
+    DBuf<int> d_cell, d_bucket_count, d_bucket_start, d_bucket_cursor, d_bucket_items, d_pair_counts, d_pair_offsets;
+    DBuf<uint64_t> d_keys_unsorted, d_keys;
+    DBuf<unsigned char> d_sort_temp, d_scan_temp;
+    size_t sort_temp_size = 0;
+
+    // contacts
+    int cap_c = 0;
+    DBuf<int> c_b1, c_b2, c_bv1, c_bv2, c_info, c_state, c_new;
+    DBuf<double> c_pB1, c_nB1, c_t1B1, c_t2B1, c_viol, c_prev, c_J, c_lam, c_lamw, c_b, c_D;
+    DBuf<int> d_narrow_counts, d_narrow_offsets;
+
+    // old contacts (warm start)
+    int n_old = 0, table_mask = 0;
+    DBuf<int> o_b1, o_b2, o_bv1, o_bv2, o_info, o_table;
+    DBuf<double> o_pB1, o_lam, o_viol, o_prev;
+
+    // groups, colouring, solve layout
+    DBuf<int> g_flags, g_scanned, g_start, g_a, g_b, g_colour_a, g_colour_b, inc_count, inc_start, inc_cursor, inc_items,
+        g_sizes, g_pos_start, s_order, s_colour_start;
+    DBuf<uint64_t> g_keys, g_keys_sorted;
+    DBuf<double> s_rec, s_lam;
+    DBuf<int> s_ab, s_flags, s_state;
+    DBuf<unsigned long long> s_iter_max;
+
+    // scalars on device: 0 n_pairs, 1 n_contacts, 2 n_groups, 3 total scratch, 4 overflow, 5 remaining,
+    // 6 iters_used, 7 bad, 8 repaired, 9 n_sweep
+    DBuf<int> d_scalars;
+    int* h_scalars = nullptr;  // pinned mirror
+
+    int n_pairs = 0, n_contacts = 0, n_groups = 0, n_colours = 0, max_colour_groups = 0;
+    bool layout_valid = false;
+    int last_iters = 0;
+    double last_change = 0.0;
+    std::vector<double> changes;
+    int pgs_max_grid = 0;
+};
+
+namespace {
+
+int fail(am_ctx* c, int code, const std::string& m) {
+    if (c) c->err = m;
+    return code;
+}
+
+#define AM_HIP(ctx, expr)                                                                 \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(ctx, AM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+BodiesView bodies_view(am_ctx* c) {
+    BodiesView B;
+    B.n = c->n_ent;
+    B.n_phys = c->n_phys;
+    B.desc = c->d_desc.p;
+    B.x = c->d_x.p;
+    B.R = c->d_R.p;
+    B.v = c->d_v.p;
+    B.w = c->d_w.p;
+    B.f = c->d_f.p;
+    B.t = c->d_t.p;
+    B.jinv = c->d_jinv.p;
+    B.dv = c->d_dv.p;
+    B.minv = c->d_minv.p;
+    B.flags = c->d_flags.p;
+    B.parent = c->d_parent.p;
+    B.slot = c->d_slot.p;
+    B.sub = c->d_sub.p;
+    return B;
+}
+TreeView tree_view(am_ctx* c) { return TreeView{c->d_nodes.p, c->d_cw.p}; }
+ContactsView contacts_view(am_ctx* c) {
+    ContactsView C;
+    C.cap = c->cap_c;
+    C.b1 = c->c_b1.p;
+    C.b2 = c->c_b2.p;
+    C.bv1 = c->c_bv1.p;
+    C.bv2 = c->c_bv2.p;
+    C.info = c->c_info.p;
+    C.pB1 = c->c_pB1.p;
+    C.nB1 = c->c_nB1.p;
+    C.t1B1 = c->c_t1B1.p;
+    C.t2B1 = c->c_t2B1.p;
+    C.viol = c->c_viol.p;
+    C.prev_viol = c->c_prev.p;
+    C.J = c->c_J.p;
+    C.lam = c->c_lam.p;
+    C.lamw = c->c_lamw.p;
+    C.b = c->c_b.p;
+    C.D = c->c_D.p;
+    C.state = c->c_state.p;
+    C.is_new = c->c_new.p;
+    return C;
+}
+OldContactsView old_view(am_ctx* c) {
+    OldContactsView O;
+    O.n = c->n_old;
+    O.b1 = c->o_b1.p;
+    O.b2 = c->o_b2.p;
+    O.bv1 = c->o_bv1.p;
+    O.bv2 = c->o_bv2.p;
+    O.info = c->o_info.p;
+    O.pB1 = c->o_pB1.p;
+    O.lam = c->o_lam.p;
+    O.viol = c->o_viol.p;
+    O.prev_viol = c->o_prev.p;
+    O.table = c->o_table.p;
+    O.table_mask = c->table_mask;
+    return O;
+}
+
+template <class T>
+hipError_t upload(am_ctx* c, DBuf<T>& b, const std::vector<T>& v) {
+    hipError_t e = b.ensure(std::max<size_t>(v.size(), 1));
+    if (e != hipSuccess) return e;
+    if (!v.empty()) return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream);
+    return hipSuccess;
+}
+
+// Reads n scalars from d_scalars[first..] (synchronises the stream).
+int read_scalars(am_ctx* c, int first, int n) {
+    AM_HIP(c, hipMemcpyAsync(c->h_scalars + first, c->d_scalars.p + first, n * sizeof(int), hipMemcpyDeviceToHost,
+                             c->stream));
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    return AM_OK;
+}
+
+int ensure_scan_temp(am_ctx* c, int n) {
+    size_t want = scan_temp_bytes(n) + 1024;
+    AM_HIP(c, c->d_scan_temp.ensure(std::max(want, c->d_scan_temp.n)));
+    return AM_OK;
+}
+
+int ensure_contact_capacity(am_ctx* c, int want) {
+    if (want <= c->cap_c && c->c_b1.p) return AM_OK;
+    int cap = std::max(want + want / 4, 1024);
+    AM_HIP(c, c->c_b1.ensure(cap));
+    AM_HIP(c, c->c_b2.ensure(cap));
+    AM_HIP(c, c->c_bv1.ensure(cap));
+    AM_HIP(c, c->c_bv2.ensure(cap));
+    AM_HIP(c, c->c_info.ensure(cap));
+    AM_HIP(c, c->c_state.ensure(cap));
+    AM_HIP(c, c->c_new.ensure(cap));
+    AM_HIP(c, c->c_pB1.ensure((size_t)cap * 3));
+    AM_HIP(c, c->c_nB1.ensure((size_t)cap * 3));
+    AM_HIP(c, c->c_t1B1.ensure((size_t)cap * 3));
+    AM_HIP(c, c->c_t2B1.ensure((size_t)cap * 3));
+    AM_HIP(c, c->c_viol.ensure(cap));
+    AM_HIP(c, c->c_prev.ensure(cap));
+    AM_HIP(c, c->c_J.ensure((size_t)cap * 36));
+    AM_HIP(c, c->c_lam.ensure((size_t)cap * 3));
+    AM_HIP(c, c->c_lamw.ensure((size_t)cap * 3));
+    AM_HIP(c, c->c_b.ensure((size_t)cap * 3));
+    AM_HIP(c, c->c_D.ensure((size_t)cap * 3));
+    // groups and solve layout are per contact too
+    AM_HIP(c, c->g_flags.ensure(cap));
+    AM_HIP(c, c->g_scanned.ensure(cap));
+    AM_HIP(c, c->g_start.ensure(cap + 1));
+    AM_HIP(c, c->g_a.ensure(cap));
+    AM_HIP(c, c->g_b.ensure(cap));
+    AM_HIP(c, c->g_colour_a.ensure(cap));
+    AM_HIP(c, c->g_colour_b.ensure(cap));
+    AM_HIP(c, c->g_keys.ensure(cap));
+    AM_HIP(c, c->g_keys_sorted.ensure(cap));
+    AM_HIP(c, c->g_sizes.ensure(cap + 1));
+    AM_HIP(c, c->g_pos_start.ensure(cap + 1));
+    AM_HIP(c, c->s_order.ensure(cap));
+    AM_HIP(c, c->s_rec.ensure((size_t)cap * kRecDoubles));
+    AM_HIP(c, c->s_lam.ensure((size_t)cap * 3));
+    AM_HIP(c, c->s_ab.ensure((size_t)cap * 2));
+    AM_HIP(c, c->s_flags.ensure(cap));
+    AM_HIP(c, c->s_state.ensure(cap));
+    size_t st = sort_temp_bytes(cap);
+    if (st > c->d_sort_temp.n) AM_HIP(c, c->d_sort_temp.ensure(st));
+    int r = ensure_scan_temp(c, cap + 1);
+    if (r) return r;
+    c->cap_c = cap;
+    return AM_OK;
+}
+
+void default_params(am_params* p) {
+    std::memset(p, 0, sizeof *p);
+    p->iterations = 30;
+    p->iterations_in_collection = 1;
+    p->tolerance = 1e-5;
+    p->omega = 1.0;
+    p->feedback_stiffness = 0.5;
+    p->enable_post_stabilization = 0;
+    p->enable_compliance = 1;
+    p->compliance = 1e-3;
+    p->restitution_override = 0;
+    p->friction_override = 0;
+    p->restitution = 0.5;
+    p->friction = 0.1;
+    p->sliding_threshold = 0.01;
+    p->warm_start = 1;
+    p->use_gravity = 1;
+    p->gravity_amount = 1.0;
+    p->gravity_angle_deg = 90.0;
+    p->viscous_linear_decay = 1.0;
+    p->viscous_angular_decay = 1.0;
+    p->enable_merging = 1;
+    p->enable_merge_pinned = 1;
+    p->enable_merge_stable_contact = 1;
+    p->enable_merge_let_it_breathe = 1;
+    p->enable_unmerging = 1;
+    p->enable_unmerge_friction = 1;
+    p->enable_unmerge_normal = 1;
+    p->enable_unmerge_relative_motion = 1;
+    p->update_contacts_in_collections = 1;
+    p->step_accum_merging = 3;
+    p->step_accum_unmerging = 3;
+    p->steps_between_merge_events = 10;
+    p->threshold_merge = 1e-2;
+    p->threshold_unmerge = 2e-2;
+    p->threshold_breath = 1e-5;
+    p->pgs_block = 0;
+    p->pgs_grid = 0;
+}
+
+// Derived host lists from the structure (parent/slot/sub/members).
+int rebuild_structure(am_ctx* c) {
+    const int n = c->n_ent;
+    c->planes.clear();
+    c->grid_bodies.clear();
+    for (int i = 0; i < c->n_phys; i++) {
+        if (c->desc[i].geom == AM_GEOM_PLANE) c->planes.push_back(i);
+        else c->grid_bodies.push_back(i);
+    }
+    int n_slots = (int)c->slot_entity.size();
+    if (n_slots >= kMaxSlots) return fail(c, AM_ERR_INVALID, "too many top-level bodies for the pair key");
+    c->top_pinned.assign(std::max(n_slots, 1), 0);
+    c->top_sleeping.assign(std::max(n_slots, 1), 0);
+    c->gravity_targets.clear();
+    c->integrate_targets.clear();
+    for (int s = 0; s < n_slots; s++) {
+        int e = c->slot_entity[s];
+        c->top_pinned[s] = (c->flags[e] & AM_BODY_PINNED) ? 1 : 0;
+        c->top_sleeping[s] = (c->flags[e] & AM_BODY_SLEEPING) ? 1 : 0;
+        c->gravity_targets.push_back(e);
+        if (c->desc[e].geom == AM_GEOM_COLLECTION)
+            for (int q = 0; q < c->member_count[e]; q++) c->gravity_targets.push_back(c->members[c->member_first[e] + q]);
+        if (!(c->flags[e] & (AM_BODY_PINNED | AM_BODY_SLEEPING))) c->integrate_targets.push_back(e);
+    }
+    for (int i = 0; i < n; i++)
+        if (c->sub[i] >= kMaxSub) return fail(c, AM_ERR_INVALID, "collection too large for the pair key");
+    AM_HIP(c, upload(c, c->d_parent, c->parent));
+    AM_HIP(c, upload(c, c->d_slot, c->slot));
+    AM_HIP(c, upload(c, c->d_sub, c->sub));
+    AM_HIP(c, upload(c, c->d_slot_entity, c->slot_entity));
+    AM_HIP(c, upload(c, c->d_member_first, c->member_first));
+    AM_HIP(c, upload(c, c->d_members, c->members));
+    AM_HIP(c, upload(c, c->d_planes, c->planes));
+    AM_HIP(c, upload(c, c->d_grid, c->grid_bodies));
+    AM_HIP(c, upload(c, c->d_top_pinned, c->top_pinned));
+    AM_HIP(c, upload(c, c->d_top_sleeping, c->top_sleeping));
+    AM_HIP(c, upload(c, c->d_gravity, c->gravity_targets));
+    AM_HIP(c, upload(c, c->d_integrate, c->integrate_targets));
+    AM_HIP(c, upload(c, c->d_flags, c->flags));
+    // grid
+    double rmax = 0.0;
+    for (int b : c->grid_bodies) {
+        const am_body_desc& d = c->desc[b];
+        double r = d.geom == AM_GEOM_BOX ? d.radius : (d.tree_count > 0 ? c->nodes[d.tree_first].r : 0.0);
+        rmax = std::max(rmax, r);
+    }
+    c->cell_size = std::max(2.0 * rmax, 1e-6);
+    int tsize = 16;
+    while (tsize < 2 * (int)c->grid_bodies.size()) tsize *= 2;
+    c->hash_mask = tsize - 1;
+    int ng = (int)c->grid_bodies.size();
+    AM_HIP(c, c->d_sphere.ensure((size_t)std::max(ng, 1) * 4));
+    AM_HIP(c, c->d_cell.ensure((size_t)std::max(ng, 1) * 4));
+    AM_HIP(c, c->d_bucket_count.ensure(tsize));
+    AM_HIP(c, c->d_bucket_start.ensure(tsize + 1));
+    AM_HIP(c, c->d_bucket_cursor.ensure(tsize));
+    AM_HIP(c, c->d_bucket_items.ensure(std::max(ng, 1)));
+    AM_HIP(c, c->d_pair_counts.ensure(std::max(ng, 1) + 1));
+    AM_HIP(c, c->d_pair_offsets.ensure(std::max(ng, 1) + 1));
+    int r = ensure_scan_temp(c, std::max(tsize, ng) + 1);
+    if (r) return r;
+    // incidence arrays are per entity
+    AM_HIP(c, c->inc_count.ensure(n + 1));
+    AM_HIP(c, c->inc_start.ensure(n + 1));
+    AM_HIP(c, c->inc_cursor.ensure(n + 1));
+    c->layout_valid = false;
+    return AM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* am_version(void) { return "libam 0.1 (gfx950, FP64 coloured PGS)"; }
+
+int am_params_default(am_params* out) {
+    if (!out) return AM_ERR_INVALID;
+    default_params(out);
+    return AM_OK;
+}
+
+int am_ctx_create(int32_t device, am_ctx** out) {
+    if (!out) return AM_ERR_INVALID;
+    *out = nullptr;
+    am_ctx* c = new (std::nothrow) am_ctx();
+    if (!c) return AM_ERR_INVALID;
+    int n_dev = 0;
+    if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev <= 0 || device < 0 || device >= n_dev) {
+        delete c;
+        return AM_ERR_HIP;
+    }
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return AM_ERR_HIP;
+    }
+    default_params(&c->params);
+    if (c->d_scalars.ensure(32) != hipSuccess || hipHostMalloc((void**)&c->h_scalars, 32 * sizeof(int)) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return AM_ERR_HIP;
+    }
+    std::memset(c->h_scalars, 0, 32 * sizeof(int));
+    *out = c;
+    return AM_OK;
+}
+
+int am_ctx_destroy(am_ctx* c) {
+    if (!c) return AM_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (c->h_scalars) (void)hipHostFree(c->h_scalars);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return AM_OK;
+}
+
+const char* am_last_error(const am_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int am_params_set(am_ctx* c, const am_params* p) {
+    if (!c || !p) return fail(c, AM_ERR_INVALID, "null argument");
+    if (p->iterations < 0 || p->iterations_in_collection < 0) return fail(c, AM_ERR_INVALID, "negative iterations");
+    c->params = *p;
+    return AM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int check_ctx(am_ctx* c) {
+    if (!c) return AM_ERR_INVALID;
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, AM_ERR_HIP, "hipSetDevice failed");
+    return AM_OK;
+}
+
+// jinv = R jinv0 R^T (zero for pinned) for every entity.
+int refresh_jinv(am_ctx* c) {
+    launch_update_jinv(c->stream, bodies_view(c));
+    AM_HIP(c, hipGetLastError());
+    return AM_OK;
+}
+
+// Colouring and sweep layout for the current groups (depends only on the
+// contact list and the solve bodies).
+int build_layout(am_ctx* c) {
+    if (c->layout_valid) return AM_OK;
+    const int ng = c->n_groups;
+    hipStream_t s = c->stream;
+    int* sc = c->d_scalars.p;
+    fill_int(s, c->inc_count.p, c->n_ent + 1, 0);
+    fill_int(s, c->inc_cursor.p, c->n_ent + 1, 0);
+    if (ng > 0) {
+        launch_incidence(s, false, ng, c->g_a.p, c->g_b.p, c->d_flags.p, c->inc_count.p, nullptr, nullptr, nullptr);
+        exclusive_scan(s, c->inc_count.p, c->inc_start.p, c->n_ent + 1, sc + 3, c->d_scan_temp.p);
+        AM_HIP(c, c->inc_items.ensure(2 * (size_t)ng + 1));
+        launch_incidence(s, true, ng, c->g_a.p, c->g_b.p, c->d_flags.p, c->inc_count.p, c->inc_start.p,
+                         c->inc_cursor.p, c->inc_items.p);
+        fill_int(s, c->g_colour_a.p, ng, -1);
+        int* cin = c->g_colour_a.p;
+        int* cout = c->g_colour_b.p;
+        for (int round = 0; round < 4096; round++) {
+            fill_int(s, sc + 5, 1, 0);
+            launch_colour_round(s, ng, c->g_a.p, c->g_b.p, c->d_flags.p, c->inc_start.p, c->inc_count.p,
+                                c->inc_items.p, cin, cout, sc + 5);
+            std::swap(cin, cout);
+            int r = read_scalars(c, 5, 1);
+            if (r) return r;
+            if (c->h_scalars[5] == 0) break;
+        }
+        launch_colour_keys(s, ng, cin, c->g_keys.p);
+        sort_u64(s, c->g_keys.p, c->g_keys_sorted.p, ng, c->d_sort_temp.p, c->d_sort_temp.n, 0, 64);
+    }
+    AM_HIP(c, c->s_colour_start.ensure(kMaxColours + 2));
+    launch_colour_layout(s, ng, c->g_keys_sorted.p, c->g_start.p, c->s_colour_start.p, c->g_sizes.p,
+                         c->g_pos_start.p, c->s_order.p, sc + 9, c->d_scan_temp.p);
+    AM_HIP(c, hipGetLastError());
+    int cs[kMaxColours + 2];
+    AM_HIP(c, hipMemcpyAsync(cs, c->s_colour_start.p, sizeof cs, hipMemcpyDeviceToHost, s));
+    AM_HIP(c, hipStreamSynchronize(s));
+    int ncol = 0, maxc = 0;
+    for (int q = 0; q <= kMaxColours; q++) {
+        int sz = cs[q + 1] - cs[q];
+        if (sz > 0) ncol++;
+        if (q < kMaxColours) maxc = std::max(maxc, sz);
+    }
+    c->n_colours = ncol;
+    c->max_colour_groups = maxc;
+    c->layout_valid = true;
+    return AM_OK;
+}
+
+int ensure_old_capacity(am_ctx* c, int n) {
+    size_t want = std::max(n, 1);
+    if (c->o_b1.n < want) {
+        size_t cap = want + want / 4;
+        AM_HIP(c, c->o_b1.ensure(cap));
+        AM_HIP(c, c->o_b2.ensure(cap));
+        AM_HIP(c, c->o_bv1.ensure(cap));
+        AM_HIP(c, c->o_bv2.ensure(cap));
+        AM_HIP(c, c->o_info.ensure(cap));
+        AM_HIP(c, c->o_pB1.ensure(cap * 3));
+        AM_HIP(c, c->o_lam.ensure(cap * 3));
+        AM_HIP(c, c->o_viol.ensure(cap));
+        AM_HIP(c, c->o_prev.ensure(cap));
+    }
+    int tsize = 16;
+    while (tsize < 2 * n + 16) tsize *= 2;
+    AM_HIP(c, c->o_table.ensure(tsize));
+    c->table_mask = tsize - 1;
+    return AM_OK;
+}
+
+int groups_after_contacts(am_ctx* c) {
+    if (c->n_contacts > 0) {
+        launch_groups(c->stream, c->n_contacts, c->d_scalars.p + 1, c->c_b1.p, c->c_b2.p, c->d_slot.p, c->d_parent.p,
+                      c->g_flags.p, c->g_scanned.p, c->g_start.p, c->g_a.p, c->g_b.p, c->d_scalars.p + 2,
+                      c->d_scan_temp.p);
+        AM_HIP(c, hipGetLastError());
+        int r = read_scalars(c, 2, 1);
+        if (r) return r;
+        c->n_groups = c->h_scalars[2];
+    } else {
+        c->n_groups = 0;
+    }
+    c->layout_valid = false;
+    return AM_OK;
+}
+
+int do_detect(am_ctx* c) {
+    hipStream_t s = c->stream;
+    BodiesView B = bodies_view(c);
+    TreeView T = tree_view(c);
+    int* sc = c->d_scalars.p;
+    launch_tree_world_centres(s, B, T, c->d_node_body.p, c->n_nodes);
+    const int ng = (int)c->grid_bodies.size();
+    const int tsize = c->hash_mask + 1;
+    launch_bounds(s, B, T, c->d_grid.p, ng, c->d_sphere.p, c->cell_size, c->d_cell.p, c->hash_mask);
+    fill_int(s, c->d_bucket_count.p, tsize, 0);
+    fill_int(s, c->d_bucket_cursor.p, tsize, 0);
+    launch_buckets(s, c->d_cell.p, ng, c->d_bucket_count.p, nullptr, nullptr, nullptr, false);
+    exclusive_scan(s, c->d_bucket_count.p, c->d_bucket_start.p, tsize, sc + 3, c->d_scan_temp.p);
+    launch_buckets(s, c->d_cell.p, ng, c->d_bucket_count.p, c->d_bucket_start.p, c->d_bucket_cursor.p,
+                   c->d_bucket_items.p, true);
+    PairArgs A;
+    A.B = B;
+    A.T = T;
+    A.grid_order = c->d_grid.p;
+    A.n_grid = ng;
+    A.sphere = c->d_sphere.p;
+    A.cell = c->d_cell.p;
+    A.bucket_start = c->d_bucket_start.p;
+    A.bucket_count = c->d_bucket_count.p;
+    A.bucket_items = c->d_bucket_items.p;
+    A.mask = c->hash_mask;
+    A.planes = c->d_planes.p;
+    A.n_planes = (int)c->planes.size();
+    A.top_pinned = c->d_top_pinned.p;
+    A.top_sleeping = c->d_top_sleeping.p;
+    fill_int(s, sc + 4, 1, 0);
+    launch_pairs_impl(s, false, A, c->d_pair_counts.p, nullptr, nullptr, 0, sc + 4);
+    exclusive_scan(s, c->d_pair_counts.p, c->d_pair_offsets.p, ng, sc + 0, c->d_scan_temp.p);
+    AM_HIP(c, hipGetLastError());
+    int r = read_scalars(c, 0, 1);
+    if (r) return r;
+    int np = ng > 0 ? c->h_scalars[0] : 0;
+    c->n_pairs = np;
+    if (np > 0) {
+        if ((int)c->d_keys.n < np) {
+            size_t cap = (size_t)np + np / 4 + 64;
+            AM_HIP(c, c->d_keys.ensure(cap));
+            AM_HIP(c, c->d_keys_unsorted.ensure(cap));
+            AM_HIP(c, c->d_narrow_counts.ensure(cap + 1));
+            AM_HIP(c, c->d_narrow_offsets.ensure(cap + 1));
+            size_t st = sort_temp_bytes((int)cap);
+            if (st > c->d_sort_temp.n) AM_HIP(c, c->d_sort_temp.ensure(st));
+            r = ensure_scan_temp(c, (int)cap + 1);
+            if (r) return r;
+        }
+        launch_pairs_impl(s, true, A, nullptr, c->d_pair_offsets.p, c->d_keys_unsorted.p, (int)c->d_keys.n, sc + 4);
+        sort_u64(s, c->d_keys_unsorted.p, c->d_keys.p, np, c->d_sort_temp.p, c->d_sort_temp.n, 0, 64);
+        fill_int(s, sc + 0, 1, np);
+        launch_narrow(s, false, B, T, c->d_keys.p, sc + 0, np, c->d_slot_entity.p, c->d_member_first.p,
+                      c->d_members.p, c->d_narrow_counts.p, nullptr, contacts_view(c), sc + 4);
+        exclusive_scan(s, c->d_narrow_counts.p, c->d_narrow_offsets.p, np, sc + 1, c->d_scan_temp.p);
+        AM_HIP(c, hipGetLastError());
+        r = read_scalars(c, 1, 1);
+        if (r) return r;
+        int nc = c->h_scalars[1];
+        r = ensure_contact_capacity(c, nc);
+        if (r) return r;
+        c->n_contacts = nc;
+        if (nc > 0)
+            launch_narrow(s, true, B, T, c->d_keys.p, sc + 0, np, c->d_slot_entity.p, c->d_member_first.p,
+                          c->d_members.p, nullptr, c->d_narrow_offsets.p, contacts_view(c), sc + 4);
+    } else {
+        c->n_contacts = 0;
+        fill_int(s, sc + 1, 1, 0);
+    }
+    r = read_scalars(c, 4, 1);
+    if (r) return r;
+    if (c->h_scalars[4] & 2) return fail(c, AM_ERR_CAPACITY, "sphere-tree traversal stack overflow (tree too deep)");
+    if (c->h_scalars[4] & 1) return fail(c, AM_ERR_CAPACITY, "contact or pair buffer overflow");
+    if (c->n_contacts == 0) c->n_old = 0;  // CollisionProcessor.java:99-101
+    return groups_after_contacts(c);
+}
+
+int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double* last) {
+    const am_params& P = c->params;
+    c->changes.clear();
+    if (c->n_contacts == 0) {
+        c->last_iters = 0;
+        c->last_change = 0;
+        if (iters) *iters = 0;
+        if (last) *last = 0;
+        return AM_OK;
+    }
+    int r = build_layout(c);
+    if (r) return r;
+    hipStream_t s = c->stream;
+    SetupArgs S;
+    S.B = bodies_view(c);
+    S.C = contacts_view(c);
+    S.n_contacts = c->d_scalars.p + 1;
+    S.order = c->s_order.p;
+    S.rec = c->s_rec.p;
+    S.slam = c->s_lam.p;
+    S.rec_ab = c->s_ab.p;
+    S.rec_flags = c->s_flags.p;
+    S.sstate = c->s_state.p;
+    S.dt = dt;
+    if (main_solve) S.feedback = (!P.enable_post_stabilization || post) ? P.feedback_stiffness : 0.0;
+    else S.feedback = P.feedback_stiffness;
+    S.compliance = P.enable_compliance ? P.compliance : 0.0;
+    S.rest_val = P.restitution;
+    S.fric_val = P.friction;
+    S.rest_ovr = P.restitution_override;
+    S.fric_ovr = P.friction_override;
+    S.post = main_solve ? post : 0;
+    S.in_collection = main_solve ? 0 : 1;
+    fill_int(s, c->d_scalars.p + 1, 1, c->n_contacts);
+    launch_setup(s, S, c->n_contacts);
+    const int iterations = main_solve ? P.iterations : P.iterations_in_collection;
+    AM_HIP(c, c->s_iter_max.ensure(std::max(iterations, 1)));
+    fill_int(s, (int*)c->s_iter_max.p, 2 * std::max(iterations, 1), 0);
+    PgsArgs G;
+    G.colour_start = c->s_colour_start.p;
+    G.pos_start = c->g_pos_start.p;
+    G.rec_ab = c->s_ab.p;
+    G.rec = c->s_rec.p;
+    G.slam = c->s_lam.p;
+    G.rec_flags = c->s_flags.p;
+    G.sstate = c->s_state.p;
+    G.dv = c->d_dv.p;
+    G.minv = c->d_minv.p;
+    G.jinv = c->d_jinv.p;
+    G.bflags = c->d_flags.p;
+    G.iterations = iterations;
+    G.check_tolerance = main_solve ? 1 : 0;
+    G.warm_start = 1;
+    G.tolerance = P.tolerance;
+    G.omega = P.omega;
+    G.sliding = P.sliding_threshold;
+    G.iter_max = c->s_iter_max.p;
+    G.iters_used = c->d_scalars.p + 6;
+    int block = P.pgs_block > 0 ? P.pgs_block : 256;
+    if (c->pgs_max_grid == 0) {
+        bool ok = false;
+        c->pgs_max_grid = pgs_max_blocks(block, &ok);
+        if (!ok) c->pgs_max_grid = 1;
+    }
+    int grid = P.pgs_grid > 0 ? P.pgs_grid : (c->max_colour_groups + block - 1) / block;
+    grid = std::max(1, std::min(grid, c->pgs_max_grid));
+    AM_HIP(c, launch_pgs(s, G, grid, block));
+    launch_unpack(s, S, c->n_contacts);
+    AM_HIP(c, hipGetLastError());
+    r = read_scalars(c, 6, 1);
+    if (r) return r;
+    int used = c->h_scalars[6];
+    std::vector<unsigned long long> im(std::max(used, 1));
+    if (used > 0)
+        AM_HIP(c, hipMemcpy(im.data(), c->s_iter_max.p, used * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    double lc = 0;
+    for (int i = 0; i < used; i++) {
+        double m;
+        std::memcpy(&m, &im[i], 8);
+        if (main_solve) c->changes.push_back(std::log(m));
+        lc = m;
+    }
+    c->last_iters = used;
+    c->last_change = lc;
+    if (iters) *iters = used;
+    if (last) *last = lc;
+    return AM_OK;
+}
+
+int pull_d(am_ctx* c, double* dst, const double* src, size_t n) {
+    if (!dst || n == 0) return AM_OK;
+    AM_HIP(c, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return AM_OK;
+}
+int pull_i(am_ctx* c, int32_t* dst, const int* src, size_t n) {
+    if (!dst || n == 0) return AM_OK;
+    AM_HIP(c, hipMemcpyAsync(dst, src, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    return AM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int am_scene_upload(am_ctx* c, const am_body_desc* bodies, int32_t n_bodies, const am_sphere_node* nodes,
+                    int32_t n_nodes) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (n_bodies < 0 || n_nodes < 0 || (n_bodies > 0 && !bodies) || (n_nodes > 0 && !nodes))
+        return fail(c, AM_ERR_INVALID, "invalid scene arrays");
+    try {
+        c->desc.assign(bodies, bodies + n_bodies);
+        c->nodes.assign(nodes, nodes + n_nodes);
+        c->n_ent = n_bodies;
+        c->n_nodes = n_nodes;
+        c->n_phys = 0;
+        c->node_body.assign(n_nodes, -1);
+        for (int i = 0; i < n_bodies; i++) {
+            const am_body_desc& d = bodies[i];
+            if (d.geom == AM_GEOM_COLLECTION) continue;
+            c->n_phys = i + 1;
+            if (d.geom == AM_GEOM_TREE) {
+                if (d.tree_first < 0 || d.tree_count <= 0 || d.tree_first + d.tree_count > n_nodes)
+                    return fail(c, AM_ERR_INVALID, "tree node range out of bounds");
+                if (d.tree_count > 0xffff) return fail(c, AM_ERR_INVALID, "tree has more than 65535 nodes");
+                for (int k = 0; k < d.tree_count; k++) {
+                    const am_sphere_node& s = nodes[d.tree_first + k];
+                    if (s.n_children < 0 || (s.n_children > 0 &&
+                                             (s.first_child <= 0 || s.first_child + s.n_children > d.tree_count)))
+                        return fail(c, AM_ERR_INVALID, "tree child range out of bounds");
+                    c->node_body[d.tree_first + k] = i;
+                }
+            }
+            if ((d.geom == AM_GEOM_PLANE) && !(d.flags & AM_BODY_PINNED))
+                return fail(c, AM_ERR_INVALID, "planes must be pinned");
+        }
+        for (int g = 0; g < n_nodes; g++)
+            if (c->node_body[g] < 0) c->node_body[g] = 0;
+        c->flags.resize(n_bodies);
+        std::vector<double> minv(n_bodies);
+        for (int i = 0; i < n_bodies; i++) {
+            c->flags[i] = bodies[i].flags;
+            minv[i] = (bodies[i].flags & AM_BODY_PINNED) ? 0.0 : bodies[i].minv;
+        }
+        c->parent.assign(n_bodies, -1);
+        c->slot.resize(n_bodies);
+        c->sub.assign(n_bodies, 0);
+        c->slot_entity.resize(n_bodies);
+        for (int i = 0; i < n_bodies; i++) {
+            c->slot[i] = i;
+            c->slot_entity[i] = i;
+        }
+        c->member_first.assign(n_bodies, 0);
+        c->member_count.assign(n_bodies, 0);
+        c->members.clear();
+        AM_HIP(c, upload(c, c->d_desc, c->desc));
+        AM_HIP(c, upload(c, c->d_nodes, c->nodes));
+        AM_HIP(c, upload(c, c->d_node_body, c->node_body));
+        AM_HIP(c, upload(c, c->d_minv, minv));
+        size_t n = std::max(n_bodies, 1);
+        AM_HIP(c, c->d_cw.ensure((size_t)std::max(n_nodes, 1) * 3));
+        AM_HIP(c, c->d_x.ensure(n * 3));
+        AM_HIP(c, c->d_R.ensure(n * 9));
+        AM_HIP(c, c->d_v.ensure(n * 3));
+        AM_HIP(c, c->d_w.ensure(n * 3));
+        AM_HIP(c, c->d_f.ensure(n * 3));
+        AM_HIP(c, c->d_t.ensure(n * 3));
+        AM_HIP(c, c->d_jinv.ensure(n * 9));
+        AM_HIP(c, c->d_dv.ensure(n * 6));
+        std::vector<double> zeros(n * 9, 0.0), ident(n * 9, 0.0);
+        for (size_t i = 0; i < n; i++) ident[i * 9 + 0] = ident[i * 9 + 4] = ident[i * 9 + 8] = 1.0;
+        AM_HIP(c, hipMemcpyAsync(c->d_x.p, zeros.data(), n * 24, hipMemcpyHostToDevice, c->stream));
+        AM_HIP(c, hipMemcpyAsync(c->d_R.p, ident.data(), n * 72, hipMemcpyHostToDevice, c->stream));
+        AM_HIP(c, hipMemcpyAsync(c->d_v.p, zeros.data(), n * 24, hipMemcpyHostToDevice, c->stream));
+        AM_HIP(c, hipMemcpyAsync(c->d_w.p, zeros.data(), n * 24, hipMemcpyHostToDevice, c->stream));
+        AM_HIP(c, hipMemcpyAsync(c->d_f.p, zeros.data(), n * 24, hipMemcpyHostToDevice, c->stream));
+        AM_HIP(c, hipMemcpyAsync(c->d_t.p, zeros.data(), n * 24, hipMemcpyHostToDevice, c->stream));
+        AM_HIP(c, hipMemcpyAsync(c->d_dv.p, zeros.data(), n * 48, hipMemcpyHostToDevice, c->stream));
+        r = rebuild_structure(c);
+        if (r) return r;
+        r = refresh_jinv(c);
+        if (r) return r;
+        r = ensure_contact_capacity(c, std::max(1024, 4 * n_bodies));
+        if (r) return r;
+        r = ensure_old_capacity(c, 1024);
+        if (r) return r;
+        c->n_contacts = 0;
+        c->n_old = 0;
+        c->n_pairs = 0;
+        c->n_groups = 0;
+        c->layout_valid = false;
+        AM_HIP(c, hipStreamSynchronize(c->stream));
+    } catch (const std::bad_alloc&) {
+        return fail(c, AM_ERR_INVALID, "host allocation failed");
+    }
+    return AM_OK;
+}
+
+int am_collections_set(am_ctx* c, const int32_t* parent, const int32_t* top, int32_t n_top,
+                       const int32_t* member_first, const int32_t* member_count, const int32_t* members,
+                       int32_t n_members) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!parent || !top || n_top <= 0 || n_top > c->n_ent || n_members < 0)
+        return fail(c, AM_ERR_INVALID, "invalid structure arrays");
+    try {
+        c->parent.assign(parent, parent + c->n_ent);
+        c->slot_entity.assign(top, top + n_top);
+        c->member_first.assign(c->n_ent, 0);
+        c->member_count.assign(c->n_ent, 0);
+        if (member_first) c->member_first.assign(member_first, member_first + c->n_ent);
+        if (member_count) c->member_count.assign(member_count, member_count + c->n_ent);
+        c->members.assign(members ? members : parent, members ? members + n_members : parent);
+        c->slot.assign(c->n_ent, -1);
+        c->sub.assign(c->n_ent, 0);
+        for (int s = 0; s < n_top; s++) {
+            int e = top[s];
+            if (e < 0 || e >= c->n_ent) return fail(c, AM_ERR_INVALID, "top-level entity out of range");
+            c->slot[e] = s;
+            if (c->desc[e].geom == AM_GEOM_COLLECTION) {
+                for (int q = 0; q < c->member_count[e]; q++) {
+                    int idx = c->member_first[e] + q;
+                    if (idx < 0 || idx >= n_members) return fail(c, AM_ERR_INVALID, "member index out of range");
+                    int b = c->members[idx];
+                    if (b < 0 || b >= c->n_ent || c->parent[b] != e)
+                        return fail(c, AM_ERR_INVALID, "member/parent mismatch");
+                    c->slot[b] = s;
+                    c->sub[b] = q;
+                }
+            }
+        }
+        for (int i = 0; i < c->n_phys; i++)
+            if (c->slot[i] < 0) return fail(c, AM_ERR_INVALID, "physical body not reachable from the top-level list");
+        return rebuild_structure(c);
+    } catch (const std::bad_alloc&) {
+        return fail(c, AM_ERR_INVALID, "host allocation failed");
+    }
+}
+
+int am_entity_mass_set(am_ctx* c, int32_t entity, double mass, double minv, const double* jinv, int32_t flags) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (entity < 0 || entity >= c->n_ent) return fail(c, AM_ERR_INVALID, "entity out of range");
+    c->desc[entity].mass = mass;
+    c->desc[entity].minv = minv;
+    c->flags[entity] = flags;
+    c->desc[entity].flags = flags;
+    double mi = (flags & AM_BODY_PINNED) ? 0.0 : minv;
+    AM_HIP(c, hipMemcpyAsync(&c->d_desc.p[entity], &c->desc[entity], sizeof(am_body_desc), hipMemcpyHostToDevice,
+                             c->stream));
+    AM_HIP(c, hipMemcpyAsync(&c->d_minv.p[entity], &mi, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    AM_HIP(c, hipMemcpyAsync(&c->d_flags.p[entity], &flags, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    if (jinv) AM_HIP(c, hipMemcpyAsync(&c->d_jinv.p[entity * 9], jinv, 72, hipMemcpyHostToDevice, c->stream));
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    return rebuild_structure(c);
+}
+
+int am_state_push(am_ctx* c, const double* x, const double* R, const double* v, const double* w, int32_t n) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (n != c->n_ent) return fail(c, AM_ERR_INVALID, "state size does not match the scene");
+    size_t m = (size_t)n;
+    for (size_t i = 0; i < m * 3; i++) {
+        if ((x && !std::isfinite(x[i])) || (v && !std::isfinite(v[i])) || (w && !std::isfinite(w[i])))
+            return fail(c, AM_ERR_NUMERIC, "non-finite body state");
+    }
+    if (R)
+        for (size_t i = 0; i < m * 9; i++)
+            if (!std::isfinite(R[i])) return fail(c, AM_ERR_NUMERIC, "non-finite rotation");
+    if (m == 0) return AM_OK;
+    if (x) AM_HIP(c, hipMemcpyAsync(c->d_x.p, x, m * 24, hipMemcpyHostToDevice, c->stream));
+    if (R) AM_HIP(c, hipMemcpyAsync(c->d_R.p, R, m * 72, hipMemcpyHostToDevice, c->stream));
+    if (v) AM_HIP(c, hipMemcpyAsync(c->d_v.p, v, m * 24, hipMemcpyHostToDevice, c->stream));
+    if (w) AM_HIP(c, hipMemcpyAsync(c->d_w.p, w, m * 24, hipMemcpyHostToDevice, c->stream));
+    r = refresh_jinv(c);
+    if (r) return r;
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    return AM_OK;
+}
+
+int am_state_pull(am_ctx* c, double* x, double* R, double* v, double* w, int32_t n) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (n != c->n_ent) return fail(c, AM_ERR_INVALID, "state size does not match the scene");
+    size_t m = (size_t)n;
+    if ((r = pull_d(c, x, c->d_x.p, m * 3))) return r;
+    if ((r = pull_d(c, R, c->d_R.p, m * 9))) return r;
+    if ((r = pull_d(c, v, c->d_v.p, m * 3))) return r;
+    if ((r = pull_d(c, w, c->d_w.p, m * 3))) return r;
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    return AM_OK;
+}
+
+int am_jinv_pull(am_ctx* c, double* jinv, int32_t n) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (n != c->n_ent || !jinv) return fail(c, AM_ERR_INVALID, "bad jinv buffer");
+    if ((r = pull_d(c, jinv, c->d_jinv.p, (size_t)n * 9))) return r;
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    return AM_OK;
+}
+
+int am_forces_push(am_ctx* c, const double* force, const double* torque, int32_t n) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (n != c->n_ent) return fail(c, AM_ERR_INVALID, "force size does not match the scene");
+    if (force && n) AM_HIP(c, hipMemcpyAsync(c->d_f.p, force, (size_t)n * 24, hipMemcpyHostToDevice, c->stream));
+    if (torque && n) AM_HIP(c, hipMemcpyAsync(c->d_t.p, torque, (size_t)n * 24, hipMemcpyHostToDevice, c->stream));
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    return AM_OK;
+}
+
+int am_forces_pull(am_ctx* c, double* force, double* torque, int32_t n) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (n != c->n_ent) return fail(c, AM_ERR_INVALID, "force size does not match the scene");
+    if ((r = pull_d(c, force, c->d_f.p, (size_t)n * 3))) return r;
+    if ((r = pull_d(c, torque, c->d_t.p, (size_t)n * 3))) return r;
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    return AM_OK;
+}
+
+int am_clear_bodies(am_ctx* c) {
+    int r = check_ctx(c);
+    if (r) return r;
+    launch_clear(c->stream, bodies_view(c));
+    AM_HIP(c, hipGetLastError());
+    return AM_OK;
+}
+
+int am_apply_gravity(am_ctx* c) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!c->params.use_gravity) return AM_OK;
+    double theta = c->params.gravity_angle_deg / 180.0 * M_PI;
+    double g = c->params.gravity_amount;
+    launch_gravity(c->stream, bodies_view(c), c->d_gravity.p, (int)c->gravity_targets.size(), g * std::cos(theta),
+                   g * std::sin(theta), 0.0);
+    AM_HIP(c, hipGetLastError());
+    return AM_OK;
+}
+
+int am_deltav_pull(am_ctx* c, double* dv, int32_t n) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (n != c->n_ent || !dv) return fail(c, AM_ERR_INVALID, "bad deltaV buffer");
+    if ((r = pull_d(c, dv, c->d_dv.p, (size_t)n * 6))) return r;
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    return AM_OK;
+}
+
+int am_update_contacts_map(am_ctx* c) {
+    int r = check_ctx(c);
+    if (r) return r;
+    const int n = c->n_contacts;
+    r = ensure_old_capacity(c, n);
+    if (r) return r;
+    hipStream_t s = c->stream;
+    if (n > 0) {
+        AM_HIP(c, hipMemcpyAsync(c->o_b1.p, c->c_b1.p, n * 4, hipMemcpyDeviceToDevice, s));
+        AM_HIP(c, hipMemcpyAsync(c->o_b2.p, c->c_b2.p, n * 4, hipMemcpyDeviceToDevice, s));
+        AM_HIP(c, hipMemcpyAsync(c->o_bv1.p, c->c_bv1.p, n * 4, hipMemcpyDeviceToDevice, s));
+        AM_HIP(c, hipMemcpyAsync(c->o_bv2.p, c->c_bv2.p, n * 4, hipMemcpyDeviceToDevice, s));
+        AM_HIP(c, hipMemcpyAsync(c->o_info.p, c->c_info.p, n * 4, hipMemcpyDeviceToDevice, s));
+        AM_HIP(c, hipMemcpyAsync(c->o_pB1.p, c->c_pB1.p, (size_t)n * 24, hipMemcpyDeviceToDevice, s));
+        AM_HIP(c, hipMemcpyAsync(c->o_lam.p, c->c_lam.p, (size_t)n * 24, hipMemcpyDeviceToDevice, s));
+        AM_HIP(c, hipMemcpyAsync(c->o_viol.p, c->c_viol.p, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+        AM_HIP(c, hipMemcpyAsync(c->o_prev.p, c->c_prev.p, (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+    }
+    c->n_old = n;
+    fill_int(s, c->o_table.p, c->table_mask + 1, -1);
+    launch_map_insert(s, old_view(c), c->o_table.p);
+    AM_HIP(c, hipGetLastError());
+    return AM_OK;
+}
+
+int am_detect(am_ctx* c, double dt, int32_t* n_contacts) {
+    (void)dt;
+    int r = check_ctx(c);
+    if (r) return r;
+    r = do_detect(c);
+    if (r) return r;
+    if (n_contacts) *n_contacts = c->n_contacts;
+    return AM_OK;
+}
+
+int am_contacts_push(am_ctx* c, int32_t n, const int32_t* body1, const int32_t* body2, const double* point_w,
+                     const double* normal_w, const double* violation, const int32_t* info, const int32_t* bv1,
+                     const int32_t* bv2) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (n < 0 || (n > 0 && (!body1 || !body2 || !point_w || !normal_w || !violation)))
+        return fail(c, AM_ERR_INVALID, "invalid contact arrays");
+    for (int k = 0; k < n; k++) {
+        if (body1[k] < 0 || body1[k] >= c->n_ent || body2[k] < 0 || body2[k] >= c->n_ent || body1[k] == body2[k])
+            return fail(c, AM_ERR_INVALID, "contact body index out of range");
+        if ((c->flags[body1[k]] & AM_BODY_PINNED) && (c->flags[body2[k]] & AM_BODY_PINNED))
+            return fail(c, AM_ERR_INVALID, "contact between two pinned bodies");
+    }
+    r = ensure_contact_capacity(c, n);
+    if (r) return r;
+    std::vector<int32_t> zi(std::max(n, 1), 0), nb(std::max(n, 1), AM_BV_NONE);
+    DBuf<int> t_b1, t_b2, t_info, t_bv1, t_bv2;
+    DBuf<double> t_p, t_n, t_v;
+    hipStream_t s = c->stream;
+    if (n > 0) {
+        AM_HIP(c, t_b1.ensure(n));
+        AM_HIP(c, t_b2.ensure(n));
+        AM_HIP(c, t_info.ensure(n));
+        AM_HIP(c, t_bv1.ensure(n));
+        AM_HIP(c, t_bv2.ensure(n));
+        AM_HIP(c, t_p.ensure((size_t)n * 3));
+        AM_HIP(c, t_n.ensure((size_t)n * 3));
+        AM_HIP(c, t_v.ensure(n));
+        AM_HIP(c, hipMemcpyAsync(t_b1.p, body1, n * 4, hipMemcpyHostToDevice, s));
+        AM_HIP(c, hipMemcpyAsync(t_b2.p, body2, n * 4, hipMemcpyHostToDevice, s));
+        AM_HIP(c, hipMemcpyAsync(t_info.p, info ? info : zi.data(), n * 4, hipMemcpyHostToDevice, s));
+        AM_HIP(c, hipMemcpyAsync(t_bv1.p, bv1 ? bv1 : nb.data(), n * 4, hipMemcpyHostToDevice, s));
+        AM_HIP(c, hipMemcpyAsync(t_bv2.p, bv2 ? bv2 : nb.data(), n * 4, hipMemcpyHostToDevice, s));
+        AM_HIP(c, hipMemcpyAsync(t_p.p, point_w, (size_t)n * 24, hipMemcpyHostToDevice, s));
+        AM_HIP(c, hipMemcpyAsync(t_n.p, normal_w, (size_t)n * 24, hipMemcpyHostToDevice, s));
+        AM_HIP(c, hipMemcpyAsync(t_v.p, violation, (size_t)n * 8, hipMemcpyHostToDevice, s));
+        fill_int(s, c->d_scalars.p + 4, 1, 0);
+        launch_contacts_set(s, bodies_view(c), n, t_b1.p, t_b2.p, t_p.p, t_n.p, t_v.p, t_info.p, t_bv1.p, t_bv2.p,
+                            contacts_view(c), c->d_scalars.p + 4);
+        AM_HIP(c, hipGetLastError());
+    }
+    c->n_contacts = n;
+    fill_int(s, c->d_scalars.p + 1, 1, n);
+    r = groups_after_contacts(c);
+    if (r) return r;
+    AM_HIP(c, hipStreamSynchronize(s));
+    return AM_OK;
+}
+
+int am_contacts_pull(am_ctx* c, am_contact_arrays* o, int32_t capacity, int32_t* n_out) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (n_out) *n_out = c->n_contacts;
+    if (!o) return AM_OK;
+    size_t n = (size_t)std::min<int>(capacity, c->n_contacts);
+    if ((r = pull_i(c, o->body1, c->c_b1.p, n))) return r;
+    if ((r = pull_i(c, o->body2, c->c_b2.p, n))) return r;
+    if ((r = pull_i(c, o->bv1, c->c_bv1.p, n))) return r;
+    if ((r = pull_i(c, o->bv2, c->c_bv2.p, n))) return r;
+    if ((r = pull_i(c, o->info, c->c_info.p, n))) return r;
+    if ((r = pull_d(c, o->point_b1, c->c_pB1.p, n * 3))) return r;
+    if ((r = pull_d(c, o->normal_b1, c->c_nB1.p, n * 3))) return r;
+    if ((r = pull_d(c, o->tangent1_b1, c->c_t1B1.p, n * 3))) return r;
+    if ((r = pull_d(c, o->tangent2_b1, c->c_t2B1.p, n * 3))) return r;
+    if ((r = pull_d(c, o->violation, c->c_viol.p, n))) return r;
+    if ((r = pull_d(c, o->prev_violation, c->c_prev.p, n))) return r;
+    if ((r = pull_d(c, o->jacobian, c->c_J.p, n * 36))) return r;
+    if ((r = pull_d(c, o->lambda, c->c_lam.p, n * 3))) return r;
+    if ((r = pull_d(c, o->lambda_warm, c->c_lamw.p, n * 3))) return r;
+    if ((r = pull_d(c, o->b, c->c_b.p, n * 3))) return r;
+    if ((r = pull_d(c, o->diag, c->c_D.p, n * 3))) return r;
+    if ((r = pull_i(c, o->state, c->c_state.p, n))) return r;
+    if ((r = pull_i(c, o->new_this_step, c->c_new.p, n))) return r;
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    return AM_OK;
+}
+
+int am_lambdas_push(am_ctx* c, const double* lambda, int32_t n) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (n != c->n_contacts || (n > 0 && !lambda)) return fail(c, AM_ERR_INVALID, "lambda count mismatch");
+    if (n > 0) {
+        AM_HIP(c, hipMemcpyAsync(c->c_lam.p, lambda, (size_t)n * 24, hipMemcpyHostToDevice, c->stream));
+        AM_HIP(c, hipMemcpyAsync(c->c_lamw.p, lambda, (size_t)n * 24, hipMemcpyHostToDevice, c->stream));
+    }
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    return AM_OK;
+}
+
+int am_warm_start(am_ctx* c, int32_t post, int32_t* bad, int32_t* repaired) {
+    int r = check_ctx(c);
+    if (r) return r;
+    fill_int(c->stream, c->d_scalars.p + 7, 2, 0);
+    if (c->n_contacts > 0) {
+        fill_int(c->stream, c->d_scalars.p + 2, 1, c->n_groups);
+        launch_warm_start(c->stream, bodies_view(c), contacts_view(c), old_view(c), c->d_scalars.p + 2, c->n_groups,
+                          c->g_start.p, post, c->d_scalars.p + 7);
+        AM_HIP(c, hipGetLastError());
+    }
+    r = read_scalars(c, 7, 2);
+    if (r) return r;
+    if (bad) *bad = c->h_scalars[7];
+    if (repaired) *repaired = c->h_scalars[8];
+    return AM_OK;
+}
+
+int am_redo_warm_start(am_ctx* c) {
+    int r = check_ctx(c);
+    if (r) return r;
+    launch_redo_warm(c->stream, contacts_view(c), c->d_scalars.p + 1, c->n_contacts, 0);
+    AM_HIP(c, hipGetLastError());
+    return AM_OK;
+}
+
+int am_no_warm_start(am_ctx* c) {
+    int r = check_ctx(c);
+    if (r) return r;
+    launch_redo_warm(c->stream, contacts_view(c), c->d_scalars.p + 1, c->n_contacts, 1);
+    AM_HIP(c, hipGetLastError());
+    return AM_OK;
+}
+
+int am_solve(am_ctx* c, double dt, int32_t post, int32_t* iterations_used, double* last_change) {
+    int r = check_ctx(c);
+    if (r) return r;
+    return do_solve(c, dt, post, true, iterations_used, last_change);
+}
+
+int am_single_sweep(am_ctx* c, double dt, int32_t* iterations_used) {
+    int r = check_ctx(c);
+    if (r) return r;
+    return do_solve(c, dt, 0, false, iterations_used, nullptr);
+}
+
+int am_solve_order_pull(am_ctx* c, int32_t* order, int32_t capacity, int32_t* n) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (c->n_contacts > 0) {
+        r = build_layout(c);
+        if (r) return r;
+    }
+    if (n) *n = c->n_contacts;
+    if (order) {
+        if ((r = pull_i(c, order, c->s_order.p, (size_t)std::min(capacity, c->n_contacts)))) return r;
+        AM_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return AM_OK;
+}
+
+int am_colours_pull(am_ctx* c, int32_t* colour_of_contact, int32_t capacity, int32_t* n_colours) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (c->n_contacts > 0) {
+        r = build_layout(c);
+        if (r) return r;
+    }
+    if (n_colours) *n_colours = c->n_colours;
+    if (colour_of_contact && c->n_contacts > 0) {
+        int cs[kMaxColours + 2];
+        std::vector<int> order(c->n_contacts), pos(c->n_groups + 1);
+        AM_HIP(c, hipMemcpy(cs, c->s_colour_start.p, sizeof cs, hipMemcpyDeviceToHost));
+        AM_HIP(c, hipMemcpy(order.data(), c->s_order.p, order.size() * 4, hipMemcpyDeviceToHost));
+        AM_HIP(c, hipMemcpy(pos.data(), c->g_pos_start.p, pos.size() * 4, hipMemcpyDeviceToHost));
+        for (int col = 0; col <= kMaxColours; col++)
+            for (int q = cs[col]; q < cs[col + 1]; q++)
+                for (int k = pos[q]; k < pos[q + 1]; k++)
+                    if (order[k] < capacity) colour_of_contact[order[k]] = col;
+    }
+    return AM_OK;
+}
+
+int am_changes_pull(am_ctx* c, double* changes, int32_t capacity, int32_t* n) {
+    if (!c) return AM_ERR_INVALID;
+    if (n) *n = (int32_t)c->changes.size();
+    if (changes)
+        for (int i = 0; i < (int)c->changes.size() && i < capacity; i++) changes[i] = c->changes[i];
+    return AM_OK;
+}
+
+int am_advance_velocities(am_ctx* c, double dt) {
+    int r = check_ctx(c);
+    if (r) return r;
+    launch_advance(c->stream, bodies_view(c), c->d_integrate.p, (int)c->integrate_targets.size(), dt, false);
+    AM_HIP(c, hipGetLastError());
+    return AM_OK;
+}
+
+int am_advance_positions(am_ctx* c, double dt) {
+    int r = check_ctx(c);
+    if (r) return r;
+    launch_advance(c->stream, bodies_view(c), c->d_integrate.p, (int)c->integrate_targets.size(), dt, true);
+    AM_HIP(c, hipGetLastError());
+    return AM_OK;
+}
+
+int am_step(am_ctx* c, double dt, am_step_stats* st) {
+    int r = check_ctx(c);
+    if (r) return r;
+    hipEvent_t ev[5];
+    for (auto& e : ev) AM_HIP(c, hipEventCreate(&e));
+    struct EvGuard {
+        hipEvent_t* e;
+        ~EvGuard() {
+            for (int i = 0; i < 5; i++) (void)hipEventDestroy(e[i]);
+        }
+    } guard{ev};
+    hipStream_t s = c->stream;
+    AM_HIP(c, hipEventRecord(ev[0], s));
+    launch_clear(s, bodies_view(c));
+    if ((r = am_apply_gravity(c))) return r;
+    if ((r = am_update_contacts_map(c))) return r;
+    if ((r = do_detect(c))) return r;
+    AM_HIP(c, hipEventRecord(ev[1], s));
+    int bad = 0, rep = 0;
+    if ((r = am_warm_start(c, 0, &bad, &rep))) return r;
+    if (c->params.warm_start) r = am_redo_warm_start(c);
+    else r = am_no_warm_start(c);
+    if (r) return r;
+    AM_HIP(c, hipEventRecord(ev[2], s));
+    int iters = 0;
+    double last = 0;
+    if ((r = do_solve(c, dt, 0, true, &iters, &last))) return r;
+    AM_HIP(c, hipEventRecord(ev[3], s));
+    if ((r = am_advance_velocities(c, dt))) return r;
+    if ((r = am_advance_positions(c, dt))) return r;
+    AM_HIP(c, hipEventRecord(ev[4], s));
+    AM_HIP(c, hipEventSynchronize(ev[4]));
+    if (st) {
+        std::memset(st, 0, sizeof *st);
+        st->n_contacts = c->n_contacts;
+        st->n_pairs = c->n_pairs;
+        st->n_groups = c->n_groups;
+        st->n_colours = c->n_colours;
+        st->iterations_used = iters;
+        st->bad_warm_starts = bad;
+        st->bad_warm_starts_repaired = rep;
+        st->last_change = last;
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+        st->ms_detect = ms;
+        (void)hipEventElapsedTime(&ms, ev[1], ev[2]);
+        st->ms_warm_start = ms;
+        (void)hipEventElapsedTime(&ms, ev[2], ev[3]);
+        st->ms_solve = ms;
+        (void)hipEventElapsedTime(&ms, ev[3], ev[4]);
+        st->ms_integrate = ms;
+        (void)hipEventElapsedTime(&ms, ev[0], ev[4]);
+        st->ms_total = ms;
+    }
+    return AM_OK;
+}
+
+int am_motion_metric(am_ctx* c, const int32_t* pairs, int32_t n_pairs, double* out) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (n_pairs < 0 || (n_pairs > 0 && (!pairs || !out))) return fail(c, AM_ERR_INVALID, "bad metric arrays");
+    if (n_pairs == 0) return AM_OK;
+    for (int k = 0; k < n_pairs; k++)
+        if (pairs[2 * k] < 0 || pairs[2 * k] >= c->n_ent || pairs[2 * k + 1] >= c->n_ent)
+            return fail(c, AM_ERR_INVALID, "metric entity out of range");
+    DBuf<int> dp;
+    DBuf<double> dout;
+    AM_HIP(c, dp.ensure((size_t)n_pairs * 2));
+    AM_HIP(c, dout.ensure(n_pairs));
+    AM_HIP(c, hipMemcpyAsync(dp.p, pairs, (size_t)n_pairs * 8, hipMemcpyHostToDevice, c->stream));
+    launch_metric(c->stream, bodies_view(c), dp.p, n_pairs, dout.p);
+    AM_HIP(c, hipGetLastError());
+    AM_HIP(c, hipMemcpyAsync(out, dout.p, (size_t)n_pairs * 8, hipMemcpyDeviceToHost, c->stream));
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    return AM_OK;
+}
+
+}  // extern "C"

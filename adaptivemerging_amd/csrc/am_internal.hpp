@@ -1,0 +1,125 @@
+// am_internal.hpp - device data views and launch entry points shared by the
+// translation units of libam.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/am.h"
+#include "am_math.hpp"
+
+namespace am {
+
+// Body arrays, one slot per entity (physical bodies, then collections).
+struct BodiesView {
+    int n;              // entities
+    int n_phys;         // physical bodies (have geometry)
+    const am_body_desc* desc;
+    double* x;          // 3 per entity
+    double* R;          // 9
+    double* v;          // 3
+    double* w;          // 3
+    double* f;          // 3
+    double* t;          // 3
+    double* jinv;       // 9, world-aligned
+    double* dv;         // 6 (deltaV: v then w)
+    const double* minv; // 1
+    const int* flags;   // am_body_flags
+    const int* parent;  // collection entity or -1
+    const int* slot;    // position of the body's top-level entity in the broad-phase list
+    const int* sub;     // index inside its collection (0 otherwise)
+};
+
+struct TreeView {
+    const am_sphere_node* nodes;
+    double* cw;         // world centre per node, 3 per node
+};
+
+// Contact arrays (structure of arrays), index = position in the contact list.
+struct ContactsView {
+    int cap;
+    int* b1;
+    int* b2;
+    int* bv1;
+    int* bv2;
+    int* info;
+    double* pB1;
+    double* nB1;
+    double* t1B1;
+    double* t2B1;
+    double* viol;
+    double* prev_viol;
+    double* J;          // 36 per contact
+    double* lam;
+    double* lamw;
+    double* b;
+    double* D;
+    int* state;
+    int* is_new;
+};
+
+// Previous step's contacts needed by the warm start.
+struct OldContactsView {
+    int n;
+    const int* b1;
+    const int* b2;
+    const int* bv1;
+    const int* bv2;
+    const int* info;
+    const double* pB1;
+    double* lam;        // zeroed when taken ("take and zero")
+    const double* viol;
+    const double* prev_viol;
+    const int* table;   // open-addressing table of old indices, -1 empty
+    int table_mask;
+};
+
+// Pair keys: slot_lo:20 | slot_hi:20 | sub_lo:12 | sub_hi:12.
+constexpr int kSlotBits = 20;
+constexpr int kSubBits = 12;
+constexpr int kMaxSlots = 1 << kSlotBits;
+constexpr int kMaxSub = 1 << kSubBits;
+constexpr int kTreeStack = 128;
+constexpr int kMaxColours = 64;     // colours 0..63 run in parallel; 64 is the sequential tail
+
+// Solve record: the per-contact data the PGS sweep reads, packed in sweep order.
+// 40 doubles: J 36 (only 30 distinct values are needed by the kernel, but all
+// are kept so the record is the contact's Jacobian verbatim), b 3, D 3 ... see
+// am_solve.hip for the exact layout.
+constexpr int kRecDoubles = 48;
+
+// ---- launchers (am_detect.hip) ----
+struct PairArgs {
+    BodiesView B;
+    TreeView T;
+    const int* grid_order;   // gridded index -> physical body
+    int n_grid;
+    const double* sphere;    // 4 per gridded body: centre, radius
+    const int* cell;         // 4 per gridded body: cx, cy, cz, bucket
+    const int* bucket_start;
+    const int* bucket_count;
+    const int* bucket_items;
+    int mask;
+    const int* planes;
+    int n_planes;
+    const int* top_pinned;   // per slot
+    const int* top_sleeping; // per slot
+};
+void launch_tree_world_centres(hipStream_t s, BodiesView B, TreeView T, const int* node_body, int n_nodes);
+void launch_bounds(hipStream_t s, BodiesView B, TreeView T, const int* order, int n, double* sphere,
+                   double cell, int* cell_key, int hash_mask);
+void launch_pairs_impl(hipStream_t s, bool write, const PairArgs& A, int* counts, const int* offsets, uint64_t* keys,
+                       int cap, int* overflow);
+void launch_narrow(hipStream_t s, bool write, BodiesView B, TreeView T, const uint64_t* keys,
+                   const int* n_pairs, int pair_cap, const int* slot_entity, const int* member_first,
+                   const int* members, int* counts, const int* offsets, ContactsView C, int* overflow);
+
+// ---- launchers (am_scan.hip) ----
+size_t scan_temp_bytes(int n);
+// Exclusive scan of n ints; writes the total to *d_total (device).
+void exclusive_scan(hipStream_t s, const int* in, int* out, int n, int* d_total, void* temp);
+size_t sort_temp_bytes(int n);
+void sort_u64(hipStream_t s, uint64_t* keys_in, uint64_t* keys_out, int n, void* temp, size_t temp_bytes,
+              int begin_bit, int end_bit);
+
+}  // namespace am

@@ -1,0 +1,87 @@
+// am_solve.hpp - argument blocks and launchers of the solve-side kernels.
+#pragma once
+
+#include "am_internal.hpp"
+
+namespace am {
+
+// Inputs of the record set-up (b, D, mu in sweep order) and of the unpack.
+struct SetupArgs {
+    BodiesView B;
+    ContactsView C;
+    const int* n_contacts;
+    const int* order;    // sweep position -> contact index
+    double* rec;         // kRecDoubles per sweep position
+    double* slam;        // lambda per sweep position (3)
+    int* rec_ab;         // solve bodies per sweep position (2)
+    int* rec_flags;      // bit 0: clamp (no active magnet)
+    int* sstate;         // contact state per sweep position
+    double dt;
+    double feedback;
+    double compliance;
+    double rest_val;
+    double fric_val;
+    int rest_ovr;
+    int fric_ovr;
+    int post;
+    int in_collection;
+};
+
+// Inputs of the persistent coloured PGS kernel.
+struct PgsArgs {
+    const int* colour_start;  // kMaxColours + 2 entries over sorted group positions
+    const int* pos_start;     // sweep position of each sorted group, n_groups + 1
+    const int* rec_ab;
+    const double* rec;
+    double* slam;
+    const int* rec_flags;
+    int* sstate;
+    double* dv;
+    const double* minv;
+    const double* jinv;
+    const int* bflags;
+    int iterations;
+    int check_tolerance;      // main solve: early exit and changes
+    int warm_start;           // apply confidentWarmStart first
+    double tolerance;
+    double omega;
+    double sliding;
+    unsigned long long* iter_max;  // per sweep, bit pattern of max |dl|
+    int* iters_used;
+};
+
+void fill_int(hipStream_t s, int* p, int n, int v);
+void fill_double(hipStream_t s, double* p, int n, double v);
+void launch_groups(hipStream_t s, int cap, const int* n_contacts, const int* cb1, const int* cb2, const int* slot,
+                   const int* parent, int* flags, int* scanned, int* group_start, int* group_a, int* group_b,
+                   int* d_n_groups, void* scan_temp);
+void launch_incidence(hipStream_t s, bool fill, int n_groups, const int* ga, const int* gb, const int* bflags,
+                      int* inc_count, const int* inc_start, int* inc_cursor, int* inc_items);
+void launch_colour_round(hipStream_t s, int n_groups, const int* ga, const int* gb, const int* bflags,
+                         const int* inc_start, const int* inc_count, const int* inc_items, const int* col_in,
+                         int* col_out, int* remaining);
+void launch_colour_keys(hipStream_t s, int n_groups, const int* colour, uint64_t* keys);
+void launch_colour_layout(hipStream_t s, int n_groups, const uint64_t* sorted, const int* group_start,
+                          int* colour_start, int* sizes, int* pos_start, int* order, int* d_total, void* scan_temp);
+void launch_setup(hipStream_t s, const SetupArgs& S, int cap);
+void launch_unpack(hipStream_t s, const SetupArgs& S, int cap);
+int pgs_max_blocks(int block, bool* ok);
+hipError_t launch_pgs(hipStream_t s, const PgsArgs& P, int grid, int block);
+void launch_map_insert(hipStream_t s, const OldContactsView& O, int* table);
+void launch_warm_start(hipStream_t s, BodiesView B, ContactsView C, OldContactsView O, const int* n_groups, int cap,
+                       const int* group_start, int post, int* counters);
+void launch_redo_warm(hipStream_t s, ContactsView C, const int* n, int cap, int zero);
+void launch_clear(hipStream_t s, BodiesView B);
+void launch_gravity(hipStream_t s, BodiesView B, const int* targets, int n, double gx, double gy, double gz);
+void launch_advance(hipStream_t s, BodiesView B, const int* targets, int n, double dt, bool positions);
+void launch_update_jinv(hipStream_t s, BodiesView B);
+void launch_metric(hipStream_t s, BodiesView B, const int* pairs, int n, double* out);
+
+// am_detect.hip
+void launch_buckets(hipStream_t s, const int* cell, int n, int* count, const int* start, int* cursor, int* items,
+                    bool scatter);
+void launch_contacts_set(hipStream_t s, BodiesView B, int n, const int* b1, const int* b2, const double* pw,
+                         const double* nw, const double* viol, const int* info, const int* bv1, const int* bv2,
+                         ContactsView C, int* overflow);
+
+}  // namespace am

@@ -1,0 +1,220 @@
+"""Scene description: bodies, sphere trees and initial state as numpy arrays.
+
+Body construction follows XMLParser.createBox / createSphere / createPlane
+(src/mergingBodies3D/XMLParser.java:360-447) and PlaneRigidBody
+(PlaneRigidBody.java:55-62); the .sph sphere-tree format follows
+XMLParser.createMesh (:500-535).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import (AM_BODY_PINNED, AM_GEOM_BOX, AM_GEOM_COLLECTION, AM_GEOM_PLANE, AM_GEOM_TREE, BODY_DTYPE,
+                   NODE_DTYPE)
+
+
+def rot_axis_angle(axis, angle) -> np.ndarray:
+    """Row-major rotation matrix for a unit axis and an angle (radians)."""
+    a = np.asarray(axis, dtype=np.float64)
+    a = a / np.linalg.norm(a)
+    x, y, z = a
+    c, s = math.cos(angle), math.sin(angle)
+    C1 = 1 - c
+    return np.array(
+        [
+            [c + x * x * C1, x * y * C1 - z * s, x * z * C1 + y * s],
+            [y * x * C1 + z * s, c + y * y * C1, y * z * C1 - x * s],
+            [z * x * C1 - y * s, z * y * C1 + x * s, c + z * z * C1],
+        ]
+    )
+
+
+def random_rotation(rng: np.random.Generator) -> np.ndarray:
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array(
+        [
+            [1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+            [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+            [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)],
+        ]
+    )
+
+
+@dataclass
+class Scene:
+    bodies: list = field(default_factory=list)       # list of BODY_DTYPE records
+    nodes: list = field(default_factory=list)        # list of NODE_DTYPE records
+    x: list = field(default_factory=list)
+    R: list = field(default_factory=list)
+    v: list = field(default_factory=list)
+    w: list = field(default_factory=list)
+    names: list = field(default_factory=list)
+
+    # -- body builders -------------------------------------------------
+    def _append(self, rec, x, R, v, w, name):
+        self.bodies.append(rec)
+        self.x.append(np.asarray(x, dtype=np.float64).reshape(3))
+        self.R.append(np.eye(3) if R is None else np.asarray(R, dtype=np.float64).reshape(3, 3))
+        self.v.append(np.zeros(3) if v is None else np.asarray(v, dtype=np.float64).reshape(3))
+        self.w.append(np.zeros(3) if w is None else np.asarray(w, dtype=np.float64).reshape(3))
+        self.names.append(name or f"body{len(self.bodies) - 1}")
+        return len(self.bodies) - 1
+
+    def add_box(self, size, x, R=None, v=None, w=None, density=1.0, friction=0.8, restitution=0.0,
+                pinned=False, scene=0, name=None) -> int:
+        """XMLParser.createBox (XMLParser.java:360-396)."""
+        s = np.asarray(size, dtype=np.float64)
+        rec = np.zeros((), dtype=BODY_DTYPE)
+        rec["geom"] = AM_GEOM_BOX
+        rec["scene"] = scene
+        rec["size"] = s
+        m = s[0] * s[1] * s[2] * density
+        I = np.array([1.0 / 12 * m * (s[1] * s[1] + s[2] * s[2]),
+                      1.0 / 12 * m * (s[0] * s[0] + s[2] * s[2]),
+                      1.0 / 12 * m * (s[0] * s[0] + s[1] * s[1])])
+        h = 0.5 * s
+        bb = []
+        for sx in (-1, 1):
+            for sy in (-1, 1):
+                for sz in (-1, 1):
+                    bb += [sx * h[0], sy * h[1], sz * h[2]]
+        rec["bbox"] = bb
+        rec["radius"] = math.sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2])
+        rec["friction"] = friction
+        rec["restitution"] = restitution
+        if pinned:
+            rec["flags"] = AM_BODY_PINNED
+        else:
+            rec["mass"] = m
+            rec["minv"] = 1.0 / m
+            rec["jinv0"] = [1.0 / I[0], 0, 0, 0, 1.0 / I[1], 0, 0, 0, 1.0 / I[2]]
+        return self._append(rec, x, R, v, w, name)
+
+    def add_plane(self, p, n, friction=0.8, restitution=0.0, scene=0, name=None) -> int:
+        """PlaneRigidBody(p, n) (PlaneRigidBody.java:55-62): pinned, d = -(p.n)."""
+        n = np.asarray(n, dtype=np.float64)
+        n = n * (1.0 / math.sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]))
+        p = np.asarray(p, dtype=np.float64)
+        rec = np.zeros((), dtype=BODY_DTYPE)
+        rec["geom"] = AM_GEOM_PLANE
+        rec["flags"] = AM_BODY_PINNED
+        rec["scene"] = scene
+        rec["plane_n"] = n
+        rec["plane_p"] = p
+        rec["plane_d"] = -(p[0] * n[0] + p[1] * n[1] + p[2] * n[2])
+        rec["friction"] = friction
+        rec["restitution"] = restitution
+        return self._append(rec, np.zeros(3), None, None, None, name)
+
+    def add_sphere(self, r, x, R=None, v=None, w=None, density=1.0, friction=0.8, restitution=0.0, scene=0,
+                   name=None) -> int:
+        """XMLParser.createSphere (XMLParser.java:420-447): a one-leaf sphere
+        tree; RigidBody.radius stays 0 as in the reference."""
+        m = 4.0 / 3 * math.pi * r * r * r * density
+        Ic = 2.0 / 5 * m * r * r
+        rec = np.zeros((), dtype=BODY_DTYPE)
+        rec["geom"] = AM_GEOM_TREE
+        rec["scene"] = scene
+        rec["mass"] = m
+        rec["minv"] = 1.0 / m
+        rec["jinv0"] = [1.0 / Ic, 0, 0, 0, 1.0 / Ic, 0, 0, 0, 1.0 / Ic]
+        rec["friction"] = friction
+        rec["restitution"] = restitution
+        bb = []
+        for sx in (-1, 1):
+            for sy in (-1, 1):
+                for sz in (-1, 1):
+                    bb += [sx * r, sy * r, sz * r]
+        rec["bbox"] = bb
+        rec["tree_first"] = len(self.nodes)
+        rec["tree_count"] = 1
+        node = np.zeros((), dtype=NODE_DTYPE)
+        node["r"] = r
+        node["first_child"] = -1
+        self.nodes.append(node)
+        return self._append(rec, x, R, v, w, name)
+
+    def add_tree(self, centres, radii, children, x, R=None, v=None, w=None, mass=1.0, inertia=(1, 1, 1),
+                 bbox=None, friction=0.8, restitution=0.0, scene=0, name=None) -> int:
+        """A sphere-tree body.  children[i] lists node i's children; node 0 is
+        the root.  Nodes are re-indexed breadth first so every node's children
+        are contiguous (the traversal visits them in the given order)."""
+        n = len(centres)
+        order, q = [0], 0
+        while q < len(order):
+            order += list(children[order[q]])
+            q += 1
+        assert len(order) == n and sorted(order) == list(range(n)), "tree must reach every node once"
+        new_index = {old: k for k, old in enumerate(order)}
+        first = len(self.nodes)
+        for old in order:
+            node = np.zeros((), dtype=NODE_DTYPE)
+            node["c"] = centres[old]
+            node["r"] = radii[old]
+            ch = list(children[old])
+            if ch:
+                node["first_child"] = new_index[ch[0]]
+                node["n_children"] = len(ch)
+                assert [new_index[c] for c in ch] == list(range(new_index[ch[0]], new_index[ch[0]] + len(ch)))
+            else:
+                node["first_child"] = -1
+                node["n_children"] = 0
+            self.nodes.append(node)
+        rec = np.zeros((), dtype=BODY_DTYPE)
+        rec["geom"] = AM_GEOM_TREE
+        rec["scene"] = scene
+        rec["mass"] = mass
+        rec["minv"] = 1.0 / mass
+        rec["jinv0"] = [1.0 / inertia[0], 0, 0, 0, 1.0 / inertia[1], 0, 0, 0, 1.0 / inertia[2]]
+        rec["friction"] = friction
+        rec["restitution"] = restitution
+        if bbox is None:
+            c = np.asarray(centres)
+            r = np.asarray(radii)
+            lo = (c - r[:, None]).min(axis=0)
+            hi = (c + r[:, None]).max(axis=0)
+            bbox = [[lx, ly, lz] for lx in (lo[0], hi[0]) for ly in (lo[1], hi[1]) for lz in (lo[2], hi[2])]
+        rec["bbox"] = np.asarray(bbox, dtype=np.float64).reshape(24)
+        rec["tree_first"] = first
+        rec["tree_count"] = n
+        return self._append(rec, x, R, v, w, name)
+
+    # -- arrays ---------------------------------------------------------
+    def arrays(self):
+        """(bodies, nodes, x, R, v, w) as contiguous numpy arrays."""
+        b = np.array(self.bodies, dtype=BODY_DTYPE) if self.bodies else np.zeros(0, dtype=BODY_DTYPE)
+        nd = np.array(self.nodes, dtype=NODE_DTYPE) if self.nodes else np.zeros(0, dtype=NODE_DTYPE)
+        n = len(self.bodies)
+        x = np.ascontiguousarray(np.array(self.x).reshape(n, 3))
+        R = np.ascontiguousarray(np.array(self.R).reshape(n, 9))
+        v = np.ascontiguousarray(np.array(self.v).reshape(n, 3))
+        w = np.ascontiguousarray(np.array(self.w).reshape(n, 3))
+        return b, nd, x, R, v, w
+
+    @property
+    def n(self) -> int:
+        return len(self.bodies)
+
+
+def load_sph(path: str, scale: float = 1.0, com=(0.0, 0.0, 0.0)):
+    """Reads a .sph sphere tree (XMLParser.java:500-535): first line N, then N
+    lines 'x y z r', then lines 'parent count child...' (1-based).  Returns
+    (centres, radii, children) with centres scaled and shifted by -com."""
+    with open(path) as f:
+        lines = [ln.split() for ln in f if ln.strip()]
+    n = int(lines[0][0])
+    centres, radii = [], []
+    for i in range(n):
+        x, y, z, r = (float(t) for t in lines[1 + i][:4])
+        centres.append([x * scale - com[0], y * scale - com[1], z * scale - com[2]])
+        radii.append(r * scale)
+    children = [[] for _ in range(n)]
+    for ln in lines[1 + n:]:
+        p, cnt = int(ln[0]) - 1, int(ln[1])
+        children[p] = [int(t) - 1 for t in ln[2:2 + cnt]]
+    return centres, radii, children

@@ -1,0 +1,205 @@
+"""Seeded synthetic scenes for the measured configurations.
+
+Each generator draws from numpy's PCG64 (np.random.default_rng(seed)) in a
+fixed field order, so a seed reproduces a scene exactly.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .scene import Scene, random_rotation, rot_axis_angle
+
+
+def jenga_tower(seed: int = 0, layers: int = 33, top: bool = True, size=(1.0, 0.25, 0.15), density: float = 1000.0,
+                jitter: float = 1e-3, overlap: float = 0.0, friction: float = 0.6) -> Scene:
+    """Jenga tower: `layers` layers of 3 blocks, alternating 90 degrees about y,
+    plus one block on top, on a ground plane y = 0 (n = +y).  Block centres are
+    jittered uniformly by +-jitter per axis.  `overlap` lowers every layer by
+    that much per layer so neighbouring layers start interpenetrating."""
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    s.add_plane([0.0, 0.0, 0.0], [0.0, 1.0, 0.0], friction=friction, name="ground")
+    L, W, H = size
+    R90 = rot_axis_angle([0, 1, 0], math.pi / 2)
+    n = 0
+    for k in range(layers + (1 if top else 0)):
+        y = H / 2 + k * (H - overlap)
+        offsets = [0.0] if k == layers else [-W, 0.0, W]
+        for o in offsets:
+            j = rng.uniform(-jitter, jitter, size=3)
+            if k % 2 == 0:
+                x, R = [j[0], y + j[1], o + j[2]], np.eye(3)
+            else:
+                x, R = [o + j[0], y + j[1], j[2]], R90
+            s.add_box([L, H, W], x, R, density=density, friction=friction, name=f"block{n}")
+            n += 1
+    return s
+
+
+def box_pile(seed: int = 0, n: int = 20000, bin_size: float = 4.0, edge=(0.1, 0.5), density=(100.0, 10000.0),
+             gap: float = 0.002, friction: float = 0.5, walls: bool = True, scene_id: int = 0) -> Scene:
+    """A pile of n boxes (edges U(edge), densities log-uniform over `density`,
+    so mass ratios reach density[1]/density[0]) in a bin_size x bin_size bin
+    (floor y = 0 plus four walls).  Boxes are shelf-packed in layers, randomly
+    yawed, with `gap` clearance; they settle under gravity."""
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    half = bin_size / 2
+    s.add_plane([0, 0, 0], [0, 1, 0], friction=friction, scene=scene_id, name="floor")
+    if walls:
+        s.add_plane([-half, 0, 0], [1, 0, 0], friction=friction, scene=scene_id, name="wall-x")
+        s.add_plane([half, 0, 0], [-1, 0, 0], friction=friction, scene=scene_id, name="wall+x")
+        s.add_plane([0, 0, -half], [0, 0, 1], friction=friction, scene=scene_id, name="wall-z")
+        s.add_plane([0, 0, half], [0, 0, -1], friction=friction, scene=scene_id, name="wall+z")
+    sizes = rng.uniform(edge[0], edge[1], size=(n, 3))
+    dens = np.exp(rng.uniform(math.log(density[0]), math.log(density[1]), size=n))
+    yaw = rng.uniform(0, 2 * math.pi, size=n)
+    # footprint of a yawed box (bounding square of its xz rectangle)
+    foot = np.abs(np.cos(yaw))[:, None] * sizes[:, [0, 2]] + np.abs(np.sin(yaw))[:, None] * sizes[:, [2, 0]]
+    y0, i = 0.0, 0
+    while i < n:
+        row_z, layer_h = -half + gap, 0.0
+        while i < n and row_z < half - gap:
+            x, row_d = -half + gap, 0.0
+            placed = False
+            while i < n:
+                fx, fz = foot[i]
+                if x + fx > half - gap or row_z + fz > half - gap:
+                    break
+                cx, cz = x + fx / 2, row_z + fz / 2
+                cy = y0 + sizes[i, 1] / 2 + gap
+                s.add_box(sizes[i], [cx, cy, cz], rot_axis_angle([0, 1, 0], yaw[i]), density=dens[i],
+                          friction=friction, scene=scene_id)
+                x += fx + gap
+                row_d = max(row_d, fz)
+                layer_h = max(layer_h, sizes[i, 1])
+                i += 1
+                placed = True
+            if not placed:
+                break
+            row_z += row_d + gap
+        y0 += layer_h + gap
+    return s
+
+
+def box_cluster(seed: int = 0, n: int = 64, spacing: float = 0.3, edge=(0.1, 0.5), with_plane: bool = True,
+                spheres: int = 0) -> Scene:
+    """Randomly oriented boxes on a loose grid, spaced so that many neighbours
+    interpenetrate (every box-box and box-plane case of the narrow phase is
+    exercised); optional single-sphere bodies mixed in."""
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    if with_plane:
+        s.add_plane([0, 0, 0], [0, 1, 0])
+    side = int(math.ceil(n ** (1 / 3)))
+    k = 0
+    for ix in range(side):
+        for iy in range(side):
+            for iz in range(side):
+                if k >= n:
+                    break
+                c = np.array([ix * spacing, 0.1 + iy * spacing, iz * spacing]) + rng.uniform(-0.03, 0.03, 3)
+                if k < spheres:
+                    s.add_sphere(rng.uniform(0.05, 0.2), c, density=1000.0)
+                else:
+                    s.add_box(rng.uniform(edge[0], edge[1], 3), c, random_rotation(rng), density=1000.0)
+                k += 1
+    return s
+
+
+def sphere_tree_body_shape(rng: np.random.Generator, depth: int = 3, branching: int = 8, leaf_mu=math.log(0.02),
+                           leaf_sigma=0.3, cube=0.3):
+    """A depth-`depth`, branching-`branching` sphere tree: leaf radii
+    lognormal, leaf centres uniform in a cube, leaves grouped by recursive
+    x/y/z median splits, internal spheres enclosing their children.  Returns
+    (centres, radii, children, mass, inertia_diag) about the centre of mass."""
+    n_leaf = branching ** depth
+    leaves = rng.uniform(-cube / 2, cube / 2, size=(n_leaf, 3))
+    lr = np.exp(rng.normal(leaf_mu, leaf_sigma, size=n_leaf))
+    m_leaf = 1000.0 * 4.0 / 3.0 * math.pi * lr ** 3
+    com = (leaves * m_leaf[:, None]).sum(0) / m_leaf.sum()
+    leaves = leaves - com
+
+    def split(idx, parts):
+        # split idx into `parts` equal groups by alternating-axis median splits
+        groups = [idx]
+        axis = 0
+        while len(groups) < parts:
+            new = []
+            for g in groups:
+                order = g[np.argsort(leaves[g, axis], kind="stable")]
+                h = len(order) // 2
+                new += [order[:h], order[h:]]
+            groups = new
+            axis = (axis + 1) % 3
+        return groups
+
+    centres, radii, children = [], [], []
+
+    def build(idx, level):
+        node = len(centres)
+        centres.append(None)
+        radii.append(None)
+        children.append([])
+        if level == depth:
+            leaf = int(idx[0])
+            centres[node] = leaves[leaf].tolist()
+            radii[node] = float(lr[leaf])
+            return node
+        kids = [build(g, level + 1) for g in split(idx, branching)]
+        children[node] = kids
+        cc = np.array([centres[k] for k in kids])
+        rr = np.array([radii[k] for k in kids])
+        c = cc.mean(0)
+        centres[node] = c.tolist()
+        radii[node] = float(np.max(np.linalg.norm(cc - c, axis=1) + rr))
+        return node
+
+    build(np.arange(n_leaf), 0)
+    mass = float(m_leaf.sum())
+    I = np.zeros(3)
+    for p, r, m in zip(leaves, lr, m_leaf):
+        I += 2.0 / 5.0 * m * r * r + m * (np.dot(p, p) - p * p)
+    return centres, radii, children, mass, I
+
+
+def sphere_tree_drop(seed: int = 0, n: int = 512, box=(2.0, 1.0, 2.0), heights=(1.0, 5.0), depth: int = 3,
+                     branching: int = 8) -> Scene:
+    """n sphere-tree bodies dropped from heights U(heights) into an open box of
+    5 static planes (floor + 4 walls), x/z extent box[0] x box[2]."""
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    hx, hz = box[0] / 2, box[2] / 2
+    s.add_plane([0, 0, 0], [0, 1, 0], friction=0.4)
+    s.add_plane([-hx, 0, 0], [1, 0, 0], friction=0.4)
+    s.add_plane([hx, 0, 0], [-1, 0, 0], friction=0.4)
+    s.add_plane([0, 0, -hz], [0, 0, 1], friction=0.4)
+    s.add_plane([0, 0, hz], [0, 0, -1], friction=0.4)
+    for _ in range(n):
+        centres, radii, children, mass, I = sphere_tree_body_shape(rng, depth, branching)
+        x = [rng.uniform(-hx + 0.2, hx - 0.2), rng.uniform(*heights), rng.uniform(-hz + 0.2, hz - 0.2)]
+        s.add_tree(centres, radii, children, x, random_rotation(rng), mass=mass, inertia=I, friction=0.4)
+    return s
+
+
+def batched_piles(seed: int = 0, n_scenes: int = 256, bodies=(50, 2000), box_frac: float = 0.7) -> Scene:
+    """n_scenes independent scenes (scene ids 0..n-1), body counts log-uniform
+    over `bodies`, each a random pile of boxes (edges U(0.1, 0.5)) and spheres
+    (r U(0.05, 0.3)) above its own ground plane."""
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    counts = np.exp(rng.uniform(math.log(bodies[0]), math.log(bodies[1]), size=n_scenes)).astype(int)
+    for sc in range(n_scenes):
+        s.add_plane([0, 0, 0], [0, 1, 0], friction=0.5, scene=sc)
+        nb = int(counts[sc])
+        side = int(math.ceil(math.sqrt(nb / 4)))
+        for k in range(nb):
+            lx, lz, ly = k % side, (k // side) % side, k // (side * side)
+            c = [lx * 0.55, 0.3 + ly * 0.55, lz * 0.55] + rng.uniform(-0.02, 0.02, 3)
+            if rng.uniform() < box_frac:
+                s.add_box(rng.uniform(0.1, 0.5, 3), c, random_rotation(rng), density=1000.0, friction=0.5, scene=sc)
+            else:
+                s.add_sphere(rng.uniform(0.05, 0.25), c, density=1000.0, friction=0.5, scene=sc)
+    return s

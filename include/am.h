@@ -1,0 +1,278 @@
+/*
+ * am.h - C ABI of libam, the MI355X (gfx950) implementation of the per-timestep
+ * 3D contact path of AdaptiveMerging (broad phase, narrow phase, PGS friction-box
+ * solve, warm start, relative-motion metric and merge/unmerge checks).
+ *
+ * Conventions
+ *  - Every entry point returns int: AM_OK (0) or a negative AM_ERR_* code.
+ *    am_last_error(ctx) returns a description of the last failure.  No C++
+ *    exception and no abort crosses this boundary.
+ *  - Host buffers are caller-owned and copied; device memory is owned by the
+ *    context.  One HIP stream per context.  A context is not thread-safe (the
+ *    reference is single-threaded too); independent contexts are.
+ *  - All floating point is IEEE double.  Matrices are row-major 3x3
+ *    (m00 m01 m02 m10 ... m22), the javax.vecmath Matrix3d field order.
+ *  - Body indices are positions in the am_body_desc array passed to
+ *    am_scene_upload ("physical" bodies).  Collections (merged bodies) are
+ *    extra entities appended after them (see am_collections_set).
+ *
+ * Reference interfaces replaced (paths under src/mergingBodies3D/):
+ *   CollisionProcessor.collisionDetection   CollisionProcessor.java:91-102
+ *   CollisionProcessor.updateContactsMap    CollisionProcessor.java:443-448
+ *   CollisionProcessor.warmStart            CollisionProcessor.java:481-640
+ *   CollisionProcessor.redoWarmStart        CollisionProcessor.java:453-459
+ *   CollisionProcessor.noWarmStart          CollisionProcessor.java:464-470
+ *   CollisionProcessor.solveLCP / PGS.solve CollisionProcessor.java:108-137,
+ *                                           PGS.java:73-194
+ *   CollisionProcessor.updateInCollections  CollisionProcessor.java:232-303
+ *   MotionMetricProcessor.getMotionMetric   MotionMetricProcessor.java:15-116
+ *   BodyPairContact.accumulateFor*          BodyPairContact.java:83-145
+ *   RigidBody.advanceVelocities/Positions   RigidBody.java:409-441
+ *   RigidBodySystem.advanceTime (no merge)  RigidBodySystem.java:102-185
+ */
+#ifndef AM_H
+#define AM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AM_OK 0
+#define AM_ERR_INVALID (-1)   /* bad argument or handle */
+#define AM_ERR_HIP (-2)       /* HIP runtime failure */
+#define AM_ERR_CAPACITY (-3)  /* a buffer would overflow; grown, call again */
+#define AM_ERR_STATE (-4)     /* call out of order (e.g. solve before detect) */
+#define AM_ERR_NUMERIC (-5)   /* NaN/Inf in body state */
+
+enum am_geom {
+    AM_GEOM_BOX = 0,        /* RigidBodyGeomBox */
+    AM_GEOM_TREE = 1,       /* sphere tree: RigidBodyGeomSphere (1 leaf) or mesh (.sph) */
+    AM_GEOM_PLANE = 2,      /* PlaneRigidBody, always pinned */
+    AM_GEOM_COLLECTION = 3  /* RigidCollection entity (no own geometry) */
+};
+
+enum am_body_flags {
+    AM_BODY_PINNED = 1,        /* RigidBody.pinned */
+    AM_BODY_MAGNET_ACTIVE = 2, /* magnetic && activateMagnet: no clamping in PGS */
+    AM_BODY_SLEEPING = 4       /* RigidBody.sleeping (broad-phase skip rule only) */
+};
+
+enum am_contact_state { AM_STATE_BROKEN = 0, AM_STATE_ONEDGE = 1, AM_STATE_CLEAR = 2 };
+
+/* bv ids carried by contacts: leaf node index (>=0) within the body's tree,
+ * AM_BV_NONE for primitives with no bounding volume, AM_BV_PLANE for the
+ * plane's dummy bounding volume (CollisionProcessor.java:817). */
+#define AM_BV_NONE (-1)
+#define AM_BV_PLANE (-2)
+
+/* Immutable description of one physical body (RigidBody fields that do not
+ * change per step; XMLParser.createBox/createSphere/createPlane). */
+typedef struct am_body_desc {
+    int32_t geom;        /* am_geom */
+    int32_t flags;       /* am_body_flags */
+    int32_t tree_first;  /* TREE: index of this body's root in the node array */
+    int32_t tree_count;  /* TREE: number of nodes */
+    int32_t scene;       /* independent scene id (batched worlds); 0 for one scene */
+    int32_t reserved;
+    double size[3];      /* BOX: full edge lengths */
+    double radius;       /* RigidBody.radius: box half-diagonal, 0 otherwise */
+    double mass;         /* massLinear (0 when pinned) */
+    double minv;         /* 1/massLinear, 0 when pinned */
+    double jinv0[9];     /* rest-pose inverse rotational inertia */
+    double friction;     /* RigidBody.friction (0.8 default) */
+    double restitution;  /* RigidBody.restitution (0 default) */
+    double plane_n[3];   /* PLANE: unit normal */
+    double plane_p[3];   /* PLANE: point */
+    double plane_d;      /* PLANE: -(p.n) */
+    double bbox[24];     /* boundingBoxB: 8 body-frame points, reference order */
+} am_body_desc;
+
+/* One BVSphere node.  Children of a node are stored contiguously; indices are
+ * relative to the owning body's tree_first. Node 0 of a body is the root. */
+typedef struct am_sphere_node {
+    double c[3];         /* centre in body coordinates (BVSphere.cB) */
+    double r;            /* radius */
+    int32_t first_child; /* -1 for a leaf */
+    int32_t n_children;  /* 0 for a leaf */
+} am_sphere_node;
+
+/* Solver and merge parameters; defaults (am_params_default) are the
+ * reference defaults, CollisionProcessor.java:1058-1083, Merging.java:31-48,
+ * Contact.java:518, RigidBodySystem.java:545-551. */
+typedef struct am_params {
+    int32_t iterations;               /* 30 */
+    int32_t iterations_in_collection; /* 1 */
+    double tolerance;                 /* 1e-5 */
+    double omega;                     /* 1 */
+    double feedback_stiffness;        /* 0.5 */
+    int32_t enable_post_stabilization;/* 0 */
+    int32_t enable_compliance;        /* 1 */
+    double compliance;                /* 1e-3 */
+    int32_t restitution_override;     /* 0 */
+    int32_t friction_override;        /* 0 */
+    double restitution;               /* 0.5 (used only when overridden) */
+    double friction;                  /* 0.1 (used only when overridden) */
+    double sliding_threshold;         /* 0.01 */
+    int32_t warm_start;               /* 1 */
+    int32_t use_gravity;              /* 1 */
+    double gravity_amount;            /* 1 */
+    double gravity_angle_deg;         /* 90 */
+    double viscous_linear_decay;      /* 1 */
+    double viscous_angular_decay;     /* 1 */
+    /* merging (Merging.MergeParameters) */
+    int32_t enable_merging;           /* 1 */
+    int32_t enable_merge_pinned;      /* 1 */
+    int32_t enable_merge_stable_contact; /* 1 */
+    int32_t enable_merge_let_it_breathe; /* 1 */
+    int32_t enable_unmerging;         /* 1 */
+    int32_t enable_unmerge_friction;  /* 1 */
+    int32_t enable_unmerge_normal;    /* 1 */
+    int32_t enable_unmerge_relative_motion; /* 1 */
+    int32_t update_contacts_in_collections; /* 1 */
+    int32_t step_accum_merging;       /* 3 */
+    int32_t step_accum_unmerging;     /* 3 */
+    int32_t steps_between_merge_events; /* 10 */
+    double threshold_merge;           /* 1e-2 */
+    double threshold_unmerge;         /* 2e-2 */
+    double threshold_breath;          /* 1e-5 */
+    /* implementation knobs (no reference counterpart) */
+    int32_t pgs_block;                /* threads per workgroup in the PGS kernel (0 = auto) */
+    int32_t pgs_grid;                 /* workgroups of the PGS kernel (0 = auto) */
+} am_params;
+
+/* Caller-owned destination arrays for am_contacts_pull; any may be NULL.
+ * Vectors are 3 doubles per contact, the Jacobian 36 per contact in the
+ * order jna.v jna.w jnb.v jnb.w jt1a.v jt1a.w jt1b.v jt1b.w jt2a.v jt2a.w
+ * jt2b.v jt2b.w (Contact.java:61-66). */
+typedef struct am_contact_arrays {
+    int32_t* body1;
+    int32_t* body2;
+    int32_t* bv1;
+    int32_t* bv2;
+    int32_t* info;
+    double* point_b1;      /* contactB1 */
+    double* normal_b1;     /* normalB1 */
+    double* tangent1_b1;
+    double* tangent2_b1;
+    double* violation;     /* constraintViolation */
+    double* prev_violation;
+    double* jacobian;
+    double* lambda;        /* lambda0..2 */
+    double* lambda_warm;   /* lambda0warm..2 */
+    double* b;             /* bn bt1 bt2 */
+    double* diag;          /* D00 D11 D22 */
+    int32_t* state;        /* am_contact_state */
+    int32_t* new_this_step;/* newThisTimeStep */
+} am_contact_arrays;
+
+/* Per-step timings (ms, from HIP events) and counters of am_step. */
+typedef struct am_step_stats {
+    int32_t n_contacts;
+    int32_t n_pairs;          /* broad-phase candidate pairs */
+    int32_t n_groups;         /* solve groups (body pairs) */
+    int32_t n_colours;
+    int32_t iterations_used;  /* PGS sweeps done */
+    int32_t bad_warm_starts;
+    int32_t bad_warm_starts_repaired;
+    int32_t reserved;
+    double last_change;       /* max |delta lambda| of the last sweep */
+    double ms_detect;
+    double ms_warm_start;
+    double ms_solve;
+    double ms_integrate;
+    double ms_total;
+} am_step_stats;
+
+typedef struct am_ctx am_ctx;
+
+/* ---- context ---------------------------------------------------------- */
+const char* am_version(void);
+int am_ctx_create(int32_t device, am_ctx** out);
+int am_ctx_destroy(am_ctx* ctx);
+const char* am_last_error(const am_ctx* ctx);
+int am_params_default(am_params* out);
+int am_params_set(am_ctx* ctx, const am_params* params);
+
+/* ---- scene and state (RigidBody fields, XMLParser output) ------------- */
+/* Uploads bodies and sphere-tree nodes; resets contacts and warm-start data.
+ * Replaces XMLParser.parse -> system.bodies (XMLParser.java:40-67). */
+int am_scene_upload(am_ctx* ctx, const am_body_desc* bodies, int32_t n_bodies,
+                    const am_sphere_node* nodes, int32_t n_nodes);
+/* Pose and velocity of every entity (n = bodies + collections).  jinv is
+ * recomputed as R jinv0 R^T (RigidBody.updateRotationalInertiaFromTransformation,
+ * RigidBody.java:311-321).  R may be NULL to keep the current rotation. */
+int am_state_push(am_ctx* ctx, const double* x, const double* R, const double* v,
+                  const double* w, int32_t n);
+int am_state_pull(am_ctx* ctx, double* x, double* R, double* v, double* w, int32_t n);
+/* Merged-body structure (RigidCollection).  parent[i] is the collection
+ * entity of entity i or -1; top[0..n_top) is the broad-phase list
+ * (RigidBodySystem.bodies order, CollisionProcessor.java:683-697); the
+ * members of collection e are members[member_first[e] .. +member_count[e]) in
+ * RigidCollection.bodies order (narrowPhaseCollection, :768-791). */
+int am_collections_set(am_ctx* ctx, const int32_t* parent, const int32_t* top, int32_t n_top,
+                       const int32_t* member_first, const int32_t* member_count, const int32_t* members,
+                       int32_t n_members);
+/* Mass properties and flags of one entity (a collection after
+ * RigidCollection.addBodyInternalMethod / updateInertiaRestAndInvert,
+ * RigidCollection.java:477-514, 674-680); jinv is its world inverse inertia. */
+int am_entity_mass_set(am_ctx* ctx, int32_t entity, double mass, double minv, const double* jinv,
+                       int32_t flags);
+int am_jinv_pull(am_ctx* ctx, double* jinv, int32_t n);
+/* Force/torque accumulators (RigidBody.force/torque). */
+int am_forces_push(am_ctx* ctx, const double* force, const double* torque, int32_t n);
+int am_forces_pull(am_ctx* ctx, double* force, double* torque, int32_t n);
+/* RigidBody.clear for all entities: zero force, torque and deltaV (RigidBody.java:226-230). */
+int am_clear_bodies(am_ctx* ctx);
+/* RigidBodySystem.applyGravityForce (RigidBodySystem.java:276-290). */
+int am_apply_gravity(am_ctx* ctx);
+int am_deltav_pull(am_ctx* ctx, double* dv, int32_t n);   /* 6 per entity: v then w */
+
+/* ---- detection (CollisionProcessor) ----------------------------------- */
+int am_update_contacts_map(am_ctx* ctx);
+int am_detect(am_ctx* ctx, double dt, int32_t* n_contacts);
+/* Installs a contact list directly (as Contact.set would build it), for
+ * callers that bring their own contacts (PGS used on its own). */
+int am_contacts_push(am_ctx* ctx, int32_t n, const int32_t* body1, const int32_t* body2,
+                     const double* point_w, const double* normal_w, const double* violation,
+                     const int32_t* info, const int32_t* bv1, const int32_t* bv2);
+int am_contacts_pull(am_ctx* ctx, am_contact_arrays* out, int32_t capacity, int32_t* n);
+/* Sets lambda (and lambda_warm) of the current contacts, 3 per contact. */
+int am_lambdas_push(am_ctx* ctx, const double* lambda, int32_t n);
+
+/* ---- warm start and solve -------------------------------------------- */
+int am_warm_start(am_ctx* ctx, int32_t post_stabilization, int32_t* bad, int32_t* repaired);
+int am_redo_warm_start(am_ctx* ctx);
+int am_no_warm_start(am_ctx* ctx);
+/* solveLCP: coloured PGS.  The sweep order is the colour-sorted order
+ * returned by am_solve_order_pull; it is the order a sequential Gauss-Seidel
+ * sweep must use to reproduce the result bit for bit. */
+int am_solve(am_ctx* ctx, double dt, int32_t post_stabilization, int32_t* iterations_used,
+             double* last_change);
+int am_solve_order_pull(am_ctx* ctx, int32_t* order, int32_t capacity, int32_t* n);
+/* Colour of every contact in the current layout (64 = sequential tail). */
+int am_colours_pull(am_ctx* ctx, int32_t* colour_of_contact, int32_t capacity, int32_t* n_colours);
+/* The single PGS sweep of updateInCollections (CollisionProcessor.java:232-303)
+ * over the current contact list with computeInCollection semantics: sub-body
+ * Jacobians, no early exit, contact states updated. */
+int am_single_sweep(am_ctx* ctx, double dt, int32_t* iterations_used);
+/* PGS.changes: log(max |delta lambda|) per sweep of the last am_solve. */
+int am_changes_pull(am_ctx* ctx, double* changes, int32_t capacity, int32_t* n);
+
+/* ---- integration (RigidBody.advanceVelocities / advancePositions) ----- */
+int am_advance_velocities(am_ctx* ctx, double dt);
+int am_advance_positions(am_ctx* ctx, double dt);
+/* One advanceTime-equivalent step without merging: clear, gravity, contacts
+ * map, detection, warm start, solve, velocity and position update. */
+int am_step(am_ctx* ctx, double dt, am_step_stats* stats);
+
+/* ---- relative motion (MotionMetricProcessor.getLargestVelocityNorm) --- */
+/* For each pair (a,b) of entity ids: the largest relative velocity over both
+ * bodies' bounding-box corners; b < 0 means the one-body variant. */
+int am_motion_metric(am_ctx* ctx, const int32_t* pairs, int32_t n_pairs, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AM_H */
