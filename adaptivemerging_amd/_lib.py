@@ -93,6 +93,7 @@ class StepStats(C.Structure):
         ("bad_warm_starts_repaired", C.c_int32), ("reserved", C.c_int32),
         ("last_change", C.c_double), ("ms_detect", C.c_double), ("ms_warm_start", C.c_double),
         ("ms_solve", C.c_double), ("ms_integrate", C.c_double), ("ms_total", C.c_double),
+        ("ms_pgs_kernel", C.c_double), ("ms_setup_kernel", C.c_double),
     ]
 
     def as_dict(self) -> dict:

@@ -86,7 +86,7 @@ struct am_ctx {
         g_sizes, g_pos_start, s_order, s_colour_start;
     DBuf<uint64_t> g_keys, g_keys_sorted;
     DBuf<double> s_rec, s_lam;
-    DBuf<int> s_ab, s_flags, s_state;
+    DBuf<int> s_meta, s_state, d_barrier, s_group_desc;
     DBuf<unsigned long long> s_iter_max;
 
     // scalars on device: 0 n_pairs, 1 n_contacts, 2 n_groups, 3 total scratch, 4 overflow, 5 remaining,
@@ -100,6 +100,8 @@ struct am_ctx {
     double last_change = 0.0;
     std::vector<double> changes;
     int pgs_max_grid = 0;
+    hipEvent_t kev[3] = {nullptr, nullptr, nullptr};  // setup start, pgs start, pgs end
+    double ms_pgs_kernel = 0.0, ms_setup_kernel = 0.0;
 };
 
 namespace {
@@ -235,8 +237,7 @@ int ensure_contact_capacity(am_ctx* c, int want) {
     AM_HIP(c, c->s_order.ensure(cap));
     AM_HIP(c, c->s_rec.ensure((size_t)cap * kRecDoubles));
     AM_HIP(c, c->s_lam.ensure((size_t)cap * 3));
-    AM_HIP(c, c->s_ab.ensure((size_t)cap * 2));
-    AM_HIP(c, c->s_flags.ensure(cap));
+    AM_HIP(c, c->s_meta.ensure((size_t)cap * 4));
     AM_HIP(c, c->s_state.ensure(cap));
     size_t st = sort_temp_bytes(cap);
     if (st > c->d_sort_temp.n) AM_HIP(c, c->d_sort_temp.ensure(st));
@@ -398,6 +399,8 @@ int am_ctx_destroy(am_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     if (c->h_scalars) (void)hipHostFree(c->h_scalars);
+    for (auto& e : c->kev)
+        if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return AM_OK;
@@ -408,6 +411,10 @@ const char* am_last_error(const am_ctx* c) { return c ? c->err.c_str() : "null c
 int am_params_set(am_ctx* c, const am_params* p) {
     if (!c || !p) return fail(c, AM_ERR_INVALID, "null argument");
     if (p->iterations < 0 || p->iterations_in_collection < 0) return fail(c, AM_ERR_INVALID, "negative iterations");
+    if (p->pgs_block < 0 || p->pgs_block > 256 || (p->pgs_block % 64) != 0)
+        return fail(c, AM_ERR_INVALID, "pgs_block must be 0 or a multiple of 64 up to 256");
+    if (p->pgs_grid < 0) return fail(c, AM_ERR_INVALID, "pgs_grid must be >= 0");
+    if (p->pgs_block != c->params.pgs_block) c->pgs_max_grid = 0;
     c->params = *p;
     return AM_OK;
 }
@@ -460,8 +467,9 @@ int build_layout(am_ctx* c) {
         sort_u64(s, c->g_keys.p, c->g_keys_sorted.p, ng, c->d_sort_temp.p, c->d_sort_temp.n, 0, 64);
     }
     AM_HIP(c, c->s_colour_start.ensure(kMaxColours + 2));
-    launch_colour_layout(s, ng, c->g_keys_sorted.p, c->g_start.p, c->s_colour_start.p, c->g_sizes.p,
-                         c->g_pos_start.p, c->s_order.p, sc + 9, c->d_scan_temp.p);
+    AM_HIP(c, c->s_group_desc.ensure((size_t)std::max(ng, 1) * 4));
+    launch_colour_layout(s, ng, c->g_keys_sorted.p, c->g_start.p, c->g_a.p, c->g_b.p, c->s_colour_start.p,
+                         c->g_sizes.p, c->g_pos_start.p, c->s_order.p, c->s_group_desc.p, sc + 9, c->d_scan_temp.p);
     AM_HIP(c, hipGetLastError());
     int cs[kMaxColours + 2];
     AM_HIP(c, hipMemcpyAsync(cs, c->s_colour_start.p, sizeof cs, hipMemcpyDeviceToHost, s));
@@ -612,9 +620,7 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
     S.n_contacts = c->d_scalars.p + 1;
     S.order = c->s_order.p;
     S.rec = c->s_rec.p;
-    S.slam = c->s_lam.p;
-    S.rec_ab = c->s_ab.p;
-    S.rec_flags = c->s_flags.p;
+    S.meta = c->s_meta.p;
     S.sstate = c->s_state.p;
     S.dt = dt;
     if (main_solve) S.feedback = (!P.enable_post_stabilization || post) ? P.feedback_stiffness : 0.0;
@@ -627,17 +633,18 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
     S.post = main_solve ? post : 0;
     S.in_collection = main_solve ? 0 : 1;
     fill_int(s, c->d_scalars.p + 1, 1, c->n_contacts);
+    for (auto& e : c->kev)
+        if (!e) AM_HIP(c, hipEventCreate(&e));
+    AM_HIP(c, hipEventRecord(c->kev[0], s));
     launch_setup(s, S, c->n_contacts);
     const int iterations = main_solve ? P.iterations : P.iterations_in_collection;
     AM_HIP(c, c->s_iter_max.ensure(std::max(iterations, 1)));
     fill_int(s, (int*)c->s_iter_max.p, 2 * std::max(iterations, 1), 0);
     PgsArgs G;
     G.colour_start = c->s_colour_start.p;
-    G.pos_start = c->g_pos_start.p;
-    G.rec_ab = c->s_ab.p;
+    G.group_desc = c->s_group_desc.p;
+    G.meta = c->s_meta.p;
     G.rec = c->s_rec.p;
-    G.slam = c->s_lam.p;
-    G.rec_flags = c->s_flags.p;
     G.sstate = c->s_state.p;
     G.dv = c->d_dv.p;
     G.minv = c->d_minv.p;
@@ -649,6 +656,11 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
     G.tolerance = P.tolerance;
     G.omega = P.omega;
     G.sliding = P.sliding_threshold;
+    G.compliance = S.compliance;
+    AM_HIP(c, c->d_barrier.ensure(2));
+    fill_int(s, c->d_barrier.p, 2, 0);
+    G.barrier = (unsigned*)c->d_barrier.p;
+    G.error = c->d_barrier.p + 1;
     G.iter_max = c->s_iter_max.p;
     G.iters_used = c->d_scalars.p + 6;
     int block = P.pgs_block > 0 ? P.pgs_block : 256;
@@ -659,11 +671,23 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
     }
     int grid = P.pgs_grid > 0 ? P.pgs_grid : (c->max_colour_groups + block - 1) / block;
     grid = std::max(1, std::min(grid, c->pgs_max_grid));
+    AM_HIP(c, hipEventRecord(c->kev[1], s));
     AM_HIP(c, launch_pgs(s, G, grid, block));
+    AM_HIP(c, hipEventRecord(c->kev[2], s));
     launch_unpack(s, S, c->n_contacts);
     AM_HIP(c, hipGetLastError());
     r = read_scalars(c, 6, 1);
     if (r) return r;
+    {
+        int berr = 0;
+        AM_HIP(c, hipMemcpy(&berr, c->d_barrier.p + 1, sizeof(int), hipMemcpyDeviceToHost));
+        if (berr) return fail(c, AM_ERR_HIP, "PGS grid barrier did not complete (blocks not co-resident)");
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, c->kev[1], c->kev[2]);
+        c->ms_pgs_kernel = ms;
+        (void)hipEventElapsedTime(&ms, c->kev[0], c->kev[1]);
+        c->ms_setup_kernel = ms;
+    }
     int used = c->h_scalars[6];
     std::vector<unsigned long long> im(std::max(used, 1));
     if (used > 0)
@@ -1227,6 +1251,8 @@ int am_step(am_ctx* c, double dt, am_step_stats* st) {
         st->ms_integrate = ms;
         (void)hipEventElapsedTime(&ms, ev[0], ev[4]);
         st->ms_total = ms;
+        st->ms_pgs_kernel = c->n_contacts ? c->ms_pgs_kernel : 0.0;
+        st->ms_setup_kernel = c->n_contacts ? c->ms_setup_kernel : 0.0;
     }
     return AM_OK;
 }

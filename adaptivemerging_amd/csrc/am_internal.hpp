@@ -82,11 +82,11 @@ constexpr int kMaxSub = 1 << kSubBits;
 constexpr int kTreeStack = 128;
 constexpr int kMaxColours = 64;     // colours 0..63 run in parallel; 64 is the sequential tail
 
-// Solve record: the per-contact data the PGS sweep reads, packed in sweep order.
-// 40 doubles: J 36 (only 30 distinct values are needed by the kernel, but all
-// are kept so the record is the contact's Jacobian verbatim), b 3, D 3 ... see
-// am_solve.hip for the exact layout.
-constexpr int kRecDoubles = 48;
+// Solve record: the per-contact data the PGS sweep reads, packed in sweep order:
+// per row (n, t1, t2) ja.w, jb.v, jb.w (ja.v = -jb.v exactly), then b[3], D[3],
+// mu, lambda[3] (read and written in place), one pad = 38 doubles (304 B).
+constexpr int kRecDoubles = 38;
+constexpr int kRecLambda = 34;
 
 // ---- launchers (am_detect.hip) ----
 struct PairArgs {

@@ -10,11 +10,7 @@
 // bit-identical to a sequential PGS sweep (PGS.java:100-193) over the contacts
 // in (colour, group, contact) order, because each contact reads and writes
 // exactly the deltaV values that order would give it.
-#include <hip/hip_cooperative_groups.h>
-
 #include "am_solve.hpp"
-
-namespace cg = cooperative_groups;
 
 namespace am {
 
@@ -186,12 +182,13 @@ __global__ void k_sorted_sizes(int n_groups, const uint64_t* sorted, const int* 
 }
 
 __global__ void k_solve_order(int n_groups, const uint64_t* sorted, const int* group_start, const int* pos_start,
-                              int* order) {
+                              const int* group_a, const int* group_b, int* order, int* group_desc) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n_groups) return;
     int g = (int)(uint32_t)sorted[q];
     int s = group_start[g], e = group_start[g + 1], o = pos_start[q];
     for (int k = s; k < e; k++) order[o + (k - s)] = k;
+    reinterpret_cast<int4*>(group_desc)[q] = make_int4(o, o + (e - s), group_a[g], group_b[g]);
 }
 
 // ---------------------------------------------------------------------------
@@ -236,7 +233,6 @@ __global__ void k_setup(SetupArgs S) {
             st3(J + row * 12 + 9, cross(r2, d));
         }
     }
-    double* rec = &S.rec[(size_t)k * kRecDoubles];
     double Jr[36];
 #pragma unroll
     for (int i = 0; i < 36; i++) Jr[i] = J[i];
@@ -286,26 +282,25 @@ __global__ void k_setup(SetupArgs S) {
         D[row] = B.minv[a] * dot(ld3(ja), ld3(ja)) + dot(ld3(ja + 3), t1) + B.minv[b] * dot(ld3(jb), ld3(jb)) +
                  dot(ld3(jb + 3), t2);
     }
+    // Compact record: per row ja.w, jb.v, jb.w (ja.v = -jb.v exactly), then b, D, mu.
+    double* rec = &S.rec[(size_t)k * kRecDoubles];
 #pragma unroll
-    for (int i = 0; i < 36; i++) rec[i] = Jr[i];
-    rec[36] = bn;
-    rec[37] = bt1;
-    rec[38] = bt2;
-    rec[39] = D[0];
-    rec[40] = D[1];
-    rec[41] = D[2];
-    rec[42] = D[0] + S.compliance;
-    rec[43] = D[1] + S.compliance;
-    rec[44] = D[2] + S.compliance;
-    rec[45] = contact_mu(B.desc[b1], B.desc[b2], S.fric_ovr, S.fric_val);
-    rec[46] = 0.0;
-    rec[47] = 0.0;
+    for (int row = 0; row < 3; row++) {
 #pragma unroll
-    for (int r = 0; r < 3; r++) S.slam[k * 3 + r] = C.lam[c * 3 + r];
-    S.rec_ab[k * 2 + 0] = a;
-    S.rec_ab[k * 2 + 1] = b;
+        for (int i = 0; i < 9; i++) rec[row * 9 + i] = Jr[row * 12 + 3 + i];
+    }
+    rec[27] = bn;
+    rec[28] = bt1;
+    rec[29] = bt2;
+    rec[30] = D[0];
+    rec[31] = D[1];
+    rec[32] = D[2];
+    rec[33] = contact_mu(B.desc[b1], B.desc[b2], S.fric_ovr, S.fric_val);
+#pragma unroll
+    for (int r = 0; r < 3; r++) rec[kRecLambda + r] = C.lam[c * 3 + r];
+    rec[kRecLambda + 3] = 0.0;
     bool clamp = !(B.flags[b1] & AM_BODY_MAGNET_ACTIVE) && !(B.flags[b2] & AM_BODY_MAGNET_ACTIVE);
-    S.rec_flags[k] = clamp ? 1 : 0;
+    reinterpret_cast<int4*>(S.meta)[k] = make_int4(a, b, clamp ? 1 : 0, 0);
     S.sstate[k] = C.state[c];
 }
 
@@ -323,9 +318,9 @@ __global__ void k_unpack(SetupArgs S) {
     ContactsView C = S.C;
 #pragma unroll
     for (int r = 0; r < 3; r++) {
-        C.lam[c * 3 + r] = S.slam[k * 3 + r];
-        C.b[c * 3 + r] = rec[36 + r];
-        C.D[c * 3 + r] = rec[39 + r];
+        C.lam[c * 3 + r] = rec[kRecLambda + r];
+        C.b[c * 3 + r] = rec[27 + r];
+        C.D[c * 3 + r] = rec[30 + r];
     }
     C.state[c] = S.sstate[k];
 }
@@ -362,64 +357,93 @@ AMD_INLINE void swap_regs(BodyRegs& a, BodyRegs& b) {
     b = t;
 }
 
+// One row of a contact record: w = ja.w, v = jb.v (ja.v = -jb.v), u = jb.w.
+struct Row {
+    d3 aw, bv, bw;
+};
+AMD_INLINE Row row_of(const double* rec, int r) {
+    return Row{mk(rec[r * 9 + 0], rec[r * 9 + 1], rec[r * 9 + 2]), mk(rec[r * 9 + 3], rec[r * 9 + 4], rec[r * 9 + 5]),
+               mk(rec[r * 9 + 6], rec[r * 9 + 7], rec[r * 9 + 8])};
+}
+
 // PGS.updateDeltaVwithLambdai (PGS.java:221-245)
-AMD_INLINE void apply_row(const double* row, double lambda, BodyRegs& A, BodyRegs& Bb) {
+AMD_INLINE void apply_row(const Row& R, double lambda, BodyRegs& A, BodyRegs& Bb) {
     double s1 = A.minv * lambda;
-    A.dv[0] = s1 * row[0] + A.dv[0];
-    A.dv[1] = s1 * row[1] + A.dv[1];
-    A.dv[2] = s1 * row[2] + A.dv[2];
-    d3 t = mul(A.jinv, mk(row[3], row[4], row[5]));
+    A.dv[0] = s1 * -R.bv.x + A.dv[0];
+    A.dv[1] = s1 * -R.bv.y + A.dv[1];
+    A.dv[2] = s1 * -R.bv.z + A.dv[2];
+    d3 t = mul(A.jinv, R.aw);
     A.dv[3] = lambda * t.x + A.dv[3];
     A.dv[4] = lambda * t.y + A.dv[4];
     A.dv[5] = lambda * t.z + A.dv[5];
     double s2 = Bb.minv * lambda;
-    Bb.dv[0] = s2 * row[6] + Bb.dv[0];
-    Bb.dv[1] = s2 * row[7] + Bb.dv[1];
-    Bb.dv[2] = s2 * row[8] + Bb.dv[2];
-    t = mul(Bb.jinv, mk(row[9], row[10], row[11]));
+    Bb.dv[0] = s2 * R.bv.x + Bb.dv[0];
+    Bb.dv[1] = s2 * R.bv.y + Bb.dv[1];
+    Bb.dv[2] = s2 * R.bv.z + Bb.dv[2];
+    t = mul(Bb.jinv, R.bw);
     Bb.dv[3] = lambda * t.x + Bb.dv[3];
     Bb.dv[4] = lambda * t.y + Bb.dv[4];
     Bb.dv[5] = lambda * t.z + Bb.dv[5];
 }
 // Contact.getJdv (Contact.java:362-378)
-AMD_INLINE double jdv(const double* row, const BodyRegs& A, const BodyRegs& Bb) {
-    double da = (row[0] * A.dv[0] + row[1] * A.dv[1] + row[2] * A.dv[2]) + (row[3] * A.dv[3] + row[4] * A.dv[4] + row[5] * A.dv[5]);
-    double db = (row[6] * Bb.dv[0] + row[7] * Bb.dv[1] + row[8] * Bb.dv[2]) +
-                (row[9] * Bb.dv[3] + row[10] * Bb.dv[4] + row[11] * Bb.dv[5]);
+AMD_INLINE double jdv(const Row& R, const BodyRegs& A, const BodyRegs& Bb) {
+    double da = (-R.bv.x * A.dv[0] + -R.bv.y * A.dv[1] + -R.bv.z * A.dv[2]) +
+                (R.aw.x * A.dv[3] + R.aw.y * A.dv[4] + R.aw.z * A.dv[5]);
+    double db = (R.bv.x * Bb.dv[0] + R.bv.y * Bb.dv[1] + R.bv.z * Bb.dv[2]) +
+                (R.bw.x * Bb.dv[3] + R.bw.y * Bb.dv[4] + R.bw.z * Bb.dv[5]);
     return da + db;
 }
 
+struct Rec {
+    double v[kRecDoubles];
+};
+AMD_INLINE void load_rec(const double* src, Rec& r) {
+    const double2* s2 = reinterpret_cast<const double2*>(src);
+#pragma unroll
+    for (int i = 0; i < kRecDoubles / 2; i++) {
+        double2 t = s2[i];
+        r.v[2 * i] = t.x;
+        r.v[2 * i + 1] = t.y;
+    }
+}
+
 // mode 0: confidentWarmStart (PGS.java:250-256); mode 1: one PGS pass over the group.
+// gd = (first sweep position, end, solve body A, solve body B) of the group.
 AMD_INLINE void process_group(const PgsArgs& P, int q, int mode, bool last_iter, double& mx) {
-    const int k0 = P.pos_start[q], k1 = P.pos_start[q + 1];
-    const int A = P.rec_ab[k0 * 2 + 0], Bi = P.rec_ab[k0 * 2 + 1];
+    const int4 gd = reinterpret_cast<const int4*>(P.group_desc)[q];
+    const int k0 = gd.x, k1 = gd.y, A = gd.z, Bi = gd.w;
+    const int4* meta = reinterpret_cast<const int4*>(P.meta);
+    Rec cur, nxt;
+    int4 mcur = meta[k0];
+    load_rec(&P.rec[(size_t)k0 * kRecDoubles], cur);
     BodyRegs ra, rb;
     load_body(P, A, ra);
     load_body(P, Bi, rb);
     for (int k = k0; k < k1; k++) {
-        const bool swapped = P.rec_ab[k * 2 + 0] != A;
+        int4 mnext = mcur;
+        if (k + 1 < k1) {
+            // prefetch the next contact of the group while this one is solved
+            mnext = meta[k + 1];
+            load_rec(&P.rec[(size_t)(k + 1) * kRecDoubles], nxt);
+        }
+        const bool swapped = mcur.x != A;
         if (swapped) swap_regs(ra, rb);
-        const double* rec = &P.rec[(size_t)k * kRecDoubles];
-        double row[12];
-        double lam[3] = {P.slam[k * 3 + 0], P.slam[k * 3 + 1], P.slam[k * 3 + 2]};
+        const double* rec = cur.v;
+        double lam[3] = {rec[kRecLambda + 0], rec[kRecLambda + 1], rec[kRecLambda + 2]};
         if (mode == 0) {
-#pragma unroll 1
-            for (int r = 0; r < 3; r++) {
 #pragma unroll
-                for (int i = 0; i < 12; i++) row[i] = rec[r * 12 + i];
-                apply_row(row, lam[r], ra, rb);
-            }
+            for (int r = 0; r < 3; r++) apply_row(row_of(rec, r), lam[r], ra, rb);
         } else {
-            const bool clamp = P.rec_flags[k] != 0;
-            const double mu = rec[45];
+            const bool clamp = mcur.z != 0;
+            const double mu = rec[33];
             const double omega = P.omega;
-#pragma unroll 1
-            for (int r = 0; r < 3; r++) {
 #pragma unroll
-                for (int i = 0; i < 12; i++) row[i] = rec[r * 12 + i];
-                double Jd = jdv(row, ra, rb);
+            for (int r = 0; r < 3; r++) {
+                Row R = row_of(rec, r);
+                double Jd = jdv(R, ra, rb);
                 double prev = lam[r];
-                double l = (rec[39 + r] * lam[r] - omega * (rec[36 + r] + Jd)) / rec[42 + r];
+                double Dr = rec[30 + r];
+                double l = (Dr * lam[r] - omega * (rec[27 + r] + Jd)) / (Dr + P.compliance);
                 if (clamp) {
                     if (r == 0) {
                         l = jmax(0, l);
@@ -431,20 +455,17 @@ AMD_INLINE void process_group(const PgsArgs& P, int q, int mode, bool last_iter,
                 }
                 lam[r] = l;
                 double diff = l - prev;
-                apply_row(row, diff, ra, rb);
+                apply_row(R, diff, ra, rb);
                 mx = jmax(mx, fabs(diff));
             }
-            P.slam[k * 3 + 0] = lam[0];
-            P.slam[k * 3 + 1] = lam[1];
-            P.slam[k * 3 + 2] = lam[2];
+            double* dst = &P.rec[(size_t)k * kRecDoubles + kRecLambda];
+            dst[0] = lam[0];
+            dst[1] = lam[1];
+            dst[2] = lam[2];
             if (last_iter) {
                 // Contact.updateContactState (Contact.java:385-398)
-#pragma unroll
-                for (int i = 0; i < 12; i++) row[i] = rec[12 + i];
-                double w1 = rec[37] + jdv(row, ra, rb);
-#pragma unroll
-                for (int i = 0; i < 12; i++) row[i] = rec[24 + i];
-                double w2 = rec[38] + jdv(row, ra, rb);
+                double w1 = rec[28] + jdv(row_of(rec, 1), ra, rb);
+                double w2 = rec[29] + jdv(row_of(rec, 2), ra, rb);
                 int st;
                 if (fabs(lam[0]) <= 1e-14) st = AM_STATE_BROKEN;
                 else if (fabs(w1) > P.sliding) st = AM_STATE_ONEDGE;
@@ -454,6 +475,10 @@ AMD_INLINE void process_group(const PgsArgs& P, int q, int mode, bool last_iter,
             }
         }
         if (swapped) swap_regs(ra, rb);
+        if (k + 1 < k1) {
+            cur = nxt;
+            mcur = mnext;
+        }
     }
     store_body(P, A, ra);
     store_body(P, Bi, rb);
@@ -465,26 +490,49 @@ AMD_INLINE double wave_max(double v) {
     return v;
 }
 
+// Grid barrier for a co-resident (cooperatively launched) grid: every wave
+// makes its stores visible at agent scope (L2 write-back), lane 0 of each
+// block arrives on a monotonically increasing counter and spins with acquire
+// loads until all blocks of this barrier instance have arrived.  A spin limit
+// turns a barrier that cannot complete into an error flag instead of a hang.
 template <bool kSingleBlock>
-AMD_INLINE void grid_barrier() {
+AMD_INLINE bool grid_barrier(const PgsArgs& P, unsigned& epoch) {
     if (kSingleBlock) {
         __syncthreads();
-    } else {
-        __threadfence();
-        cg::this_grid().sync();
-        __threadfence();
+        return true;
     }
+    __shared__ int bail;
+    epoch += gridDim.x;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        bail = 0;
+        __hip_atomic_fetch_add(P.barrier, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        long spins = 0;
+        while (__hip_atomic_load(P.barrier, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+            if (++spins > 20000000L) {
+                atomicOr(P.error, 1);
+                bail = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (__hip_atomic_load(P.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) bail = 1;
+    }
+    __syncthreads();
+    return bail == 0;
 }
 
 // Persistent kernel: warm-start pass then up to `iterations` sweeps; one grid
-// barrier per colour.  The per-sweep max |delta lambda| is reduced with an
-// integer atomic max on the bit pattern (non-negative doubles order as their
-// bits) into a slot per sweep.
+// barrier per non-empty colour.  The per-sweep max |delta lambda| is reduced
+// with an integer atomic max on the bit pattern (non-negative doubles order as
+// their bits) into a slot per sweep.
 template <bool kSingleBlock>
-__global__ void k_pgs(PgsArgs P) {
+__global__ __launch_bounds__(256) void k_pgs(PgsArgs P) {
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int gstride = gridDim.x * blockDim.x;
     const int* cs = P.colour_start;
+    unsigned epoch = 0;
     // Colours with no group are skipped by every thread alike (same reads), so
     // barriers stay uniform.  The tail colour (kMaxColours) runs on one lane.
     int last_col = -1;
@@ -496,12 +544,12 @@ __global__ void k_pgs(PgsArgs P) {
         for (int c = 0; c <= last_col; c++) {
             if (cs[c + 1] == cs[c]) continue;
             for (int q = cs[c] + gtid; q < cs[c + 1]; q += gstride) process_group(P, q, 0, false, dummy);
-            grid_barrier<kSingleBlock>();
+            if (!grid_barrier<kSingleBlock>(P, epoch)) return;
         }
         if (has_tail) {
             if (gtid == 0)
                 for (int q = cs[kMaxColours]; q < cs[kMaxColours + 1]; q++) process_group(P, q, 0, false, dummy);
-            grid_barrier<kSingleBlock>();
+            if (!grid_barrier<kSingleBlock>(P, epoch)) return;
         }
     }
     int it = 0;
@@ -516,7 +564,7 @@ __global__ void k_pgs(PgsArgs P) {
                 if ((threadIdx.x & 63) == 0 && wm > 0)
                     atomicMax(&P.iter_max[it], (unsigned long long)__double_as_longlong(wm));
             }
-            grid_barrier<kSingleBlock>();
+            if (!grid_barrier<kSingleBlock>(P, epoch)) return;
         }
         if (has_tail) {
             if (gtid == 0)
@@ -524,9 +572,11 @@ __global__ void k_pgs(PgsArgs P) {
             double wm = wave_max(mx);
             if ((threadIdx.x & 63) == 0 && wm > 0)
                 atomicMax(&P.iter_max[it], (unsigned long long)__double_as_longlong(wm));
-            grid_barrier<kSingleBlock>();
+            if (!grid_barrier<kSingleBlock>(P, epoch)) return;
         }
-        if (last_col < 0 && !has_tail) grid_barrier<kSingleBlock>();
+        if (last_col < 0 && !has_tail) {
+            if (!grid_barrier<kSingleBlock>(P, epoch)) return;
+        }
         double m = __longlong_as_double((long long)__hip_atomic_load(&P.iter_max[it], __ATOMIC_RELAXED,
                                                                       __HIP_MEMORY_SCOPE_AGENT));
         if (P.check_tolerance && m < P.tolerance) {
@@ -539,9 +589,9 @@ __global__ void k_pgs(PgsArgs P) {
 
 int pgs_max_blocks(int block, bool* ok) {
     int dev = 0, n_cu = 0, per_cu = 0;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pgs<false>, block, 0);
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pgs<false>, block, 0);
     *ok = per_cu > 0;
     return n_cu * (per_cu > 0 ? per_cu : 1);
 }
@@ -906,13 +956,15 @@ void launch_colour_keys(hipStream_t s, int n_groups, const int* colour, uint64_t
         hipLaunchKernelGGL(k_colour_keys, dim3((n_groups + 255) / 256), dim3(256), 0, s, n_groups, colour, keys);
 }
 void launch_colour_layout(hipStream_t s, int n_groups, const uint64_t* sorted, const int* group_start,
-                          int* colour_start, int* sizes, int* pos_start, int* order, int* d_total, void* scan_temp) {
+                          const int* group_a, const int* group_b, int* colour_start, int* sizes, int* pos_start,
+                          int* order, int* group_desc, int* d_total, void* scan_temp) {
     hipLaunchKernelGGL(k_colour_starts, dim3(1), dim3(128), 0, s, n_groups, sorted, colour_start);
     if (n_groups <= 0) return;
     dim3 g((n_groups + 255) / 256), b(256);
     hipLaunchKernelGGL(k_sorted_sizes, dim3((n_groups + 256) / 256), b, 0, s, n_groups, sorted, group_start, sizes);
     exclusive_scan(s, sizes, pos_start, n_groups + 1, d_total, scan_temp);
-    hipLaunchKernelGGL(k_solve_order, g, b, 0, s, n_groups, sorted, group_start, pos_start, order);
+    hipLaunchKernelGGL(k_solve_order, g, b, 0, s, n_groups, sorted, group_start, pos_start, group_a, group_b, order,
+                       group_desc);
 }
 
 }  // namespace am

@@ -12,9 +12,7 @@ struct SetupArgs {
     const int* n_contacts;
     const int* order;    // sweep position -> contact index
     double* rec;         // kRecDoubles per sweep position
-    double* slam;        // lambda per sweep position (3)
-    int* rec_ab;         // solve bodies per sweep position (2)
-    int* rec_flags;      // bit 0: clamp (no active magnet)
+    int* meta;           // int4 per sweep position: solve body a, solve body b, clamp, 0
     int* sstate;         // contact state per sweep position
     double dt;
     double feedback;
@@ -30,11 +28,9 @@ struct SetupArgs {
 // Inputs of the persistent coloured PGS kernel.
 struct PgsArgs {
     const int* colour_start;  // kMaxColours + 2 entries over sorted group positions
-    const int* pos_start;     // sweep position of each sorted group, n_groups + 1
-    const int* rec_ab;
-    const double* rec;
-    double* slam;
-    const int* rec_flags;
+    const int* group_desc;    // int4 per sorted group: first, end sweep position, solve bodies A, B
+    const int* meta;
+    double* rec;
     int* sstate;
     double* dv;
     const double* minv;
@@ -46,8 +42,11 @@ struct PgsArgs {
     double tolerance;
     double omega;
     double sliding;
+    double compliance;
     unsigned long long* iter_max;  // per sweep, bit pattern of max |dl|
     int* iters_used;
+    unsigned* barrier;             // grid-barrier counter (zeroed before launch)
+    int* error;                    // set if a barrier spin limit is hit
 };
 
 void fill_int(hipStream_t s, int* p, int n, int v);
@@ -62,7 +61,8 @@ void launch_colour_round(hipStream_t s, int n_groups, const int* ga, const int* 
                          int* col_out, int* remaining);
 void launch_colour_keys(hipStream_t s, int n_groups, const int* colour, uint64_t* keys);
 void launch_colour_layout(hipStream_t s, int n_groups, const uint64_t* sorted, const int* group_start,
-                          int* colour_start, int* sizes, int* pos_start, int* order, int* d_total, void* scan_temp);
+                          const int* group_a, const int* group_b, int* colour_start, int* sizes, int* pos_start,
+                          int* order, int* group_desc, int* d_total, void* scan_temp);
 void launch_setup(hipStream_t s, const SetupArgs& S, int cap);
 void launch_unpack(hipStream_t s, const SetupArgs& S, int cap);
 int pgs_max_blocks(int block, bool* ok);

@@ -39,11 +39,14 @@ def jenga_tower(seed: int = 0, layers: int = 33, top: bool = True, size=(1.0, 0.
 
 
 def box_pile(seed: int = 0, n: int = 20000, bin_size: float = 4.0, edge=(0.1, 0.5), density=(100.0, 10000.0),
-             gap: float = 0.002, friction: float = 0.5, walls: bool = True, scene_id: int = 0) -> Scene:
+             gap: float = 0.002, friction: float = 0.5, walls: bool = True, scene_id: int = 0,
+             max_yaw: float = 0.2) -> Scene:
     """A pile of n boxes (edges U(edge), densities log-uniform over `density`,
     so mass ratios reach density[1]/density[0]) in a bin_size x bin_size bin
-    (floor y = 0 plus four walls).  Boxes are shelf-packed in layers, randomly
-    yawed, with `gap` clearance; they settle under gravity."""
+    (floor y = 0 plus four walls).  Boxes lie flat (shortest edge vertical),
+    yawed by U(-max_yaw, max_yaw), and are shelf-packed in layers of similar
+    height (tallest layers at the bottom) with `gap` clearance, so they settle
+    under gravity within a few dozen steps."""
     rng = np.random.default_rng(seed)
     s = Scene()
     half = bin_size / 2
@@ -53,18 +56,22 @@ def box_pile(seed: int = 0, n: int = 20000, bin_size: float = 4.0, edge=(0.1, 0.
         s.add_plane([half, 0, 0], [-1, 0, 0], friction=friction, scene=scene_id, name="wall+x")
         s.add_plane([0, 0, -half], [0, 0, 1], friction=friction, scene=scene_id, name="wall-z")
         s.add_plane([0, 0, half], [0, 0, -1], friction=friction, scene=scene_id, name="wall+z")
-    sizes = rng.uniform(edge[0], edge[1], size=(n, 3))
+    raw = rng.uniform(edge[0], edge[1], size=(n, 3))
     dens = np.exp(rng.uniform(math.log(density[0]), math.log(density[1]), size=n))
-    yaw = rng.uniform(0, 2 * math.pi, size=n)
-    # footprint of a yawed box (bounding square of its xz rectangle)
-    foot = np.abs(np.cos(yaw))[:, None] * sizes[:, [0, 2]] + np.abs(np.sin(yaw))[:, None] * sizes[:, [2, 0]]
-    y0, i = 0.0, 0
-    while i < n:
+    yaw = rng.uniform(-max_yaw, max_yaw, size=n)
+    srt = np.sort(raw, axis=1)
+    sizes = np.stack([srt[:, 2], srt[:, 0], srt[:, 1]], axis=1)  # long x, short y (vertical), mid z
+    order = np.argsort(-sizes[:, 1], kind="stable")
+    c, sn = np.abs(np.cos(yaw)), np.abs(np.sin(yaw))
+    foot = np.stack([c * sizes[:, 0] + sn * sizes[:, 2], sn * sizes[:, 0] + c * sizes[:, 2]], axis=1)
+    y0, q = 0.0, 0
+    while q < n:
         row_z, layer_h = -half + gap, 0.0
-        while i < n and row_z < half - gap:
+        while q < n and row_z < half - gap:
             x, row_d = -half + gap, 0.0
             placed = False
-            while i < n:
+            while q < n:
+                i = order[q]
                 fx, fz = foot[i]
                 if x + fx > half - gap or row_z + fz > half - gap:
                     break
@@ -75,7 +82,7 @@ def box_pile(seed: int = 0, n: int = 20000, bin_size: float = 4.0, edge=(0.1, 0.
                 x += fx + gap
                 row_d = max(row_d, fz)
                 layer_h = max(layer_h, sizes[i, 1])
-                i += 1
+                q += 1
                 placed = True
             if not placed:
                 break

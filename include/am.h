@@ -183,6 +183,8 @@ typedef struct am_step_stats {
     double ms_solve;
     double ms_integrate;
     double ms_total;
+    double ms_pgs_kernel;     /* duration of the persistent PGS kernel alone */
+    double ms_setup_kernel;   /* b, D and record packing in sweep order */
 } am_step_stats;
 
 typedef struct am_ctx am_ctx;

@@ -124,3 +124,21 @@ def test_motion_metric(gpu_lib):
     g = ctx.motion_metric(pairs)
     o = np.array([orc.motion_metric(int(a), int(b)) for a, b in pairs])
     assert_bits_equal("metric", g, o)
+
+
+@pytest.mark.parametrize("grid,block", [(2, 64), (8, 64), (32, 128)])
+def test_pgs_multiblock_bit_exact(gpu_lib, grid, block):
+    """The cooperative (grid-barrier) path of the sweep kernel."""
+    scene = workloads.box_cluster(seed=9, n=729, spheres=60)
+    ctx, orc = twin(scene, iterations=25, pgs_grid=grid, pgs_block=block)
+    ctx.apply_gravity(); orc.apply_gravity()
+    n = ctx.detect(0.01); orc.detect()
+    lam = np.random.default_rng(2).uniform(0, 0.5, size=(n, 3))
+    ctx.push_lambdas(lam); orc.push_lambdas(lam)
+    it_g, _ = ctx.solve(0.01)
+    order = ctx.solve_order()
+    it_o, _, _ = orc.solve(order, 0.01)
+    print("contacts", n, "iterations", it_g, it_o, flush=True)
+    assert it_g == it_o
+    assert_contacts_equal(ctx.contacts(), orc.contacts())
+    assert_bits_equal("deltaV", ctx.pull_deltav(), orc.pull_deltav())

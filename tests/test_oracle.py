@@ -1,0 +1,502 @@
+"""Pins the CPU oracle (oracle/am_oracle.c) to closed forms, brute force on
+tiny inputs and invariants, so that a dropped term, a wrong sign or a
+transposed operand in the restatement fails here (CPU only)."""
+import math
+
+import numpy as np
+import pytest
+
+from adaptivemerging_amd import Scene, workloads
+from adaptivemerging_amd.scene import random_rotation, rot_axis_angle
+from oracle.pyoracle import Oracle
+
+
+def world(c, key, k):
+    return np.asarray(c[key][k])
+
+
+def world_point(orc, c, k):
+    x, R, _, _ = orc.pull_state()
+    b = c["body1"][k]
+    return R[b].reshape(3, 3) @ c["point_b1"][k] + x[b]
+
+
+def world_normal(orc, c, k):
+    _, R, _, _ = orc.pull_state()
+    return R[c["body1"][k]].reshape(3, 3) @ c["normal_b1"][k]
+
+
+# ---------------------------------------------------------------- box-plane
+def test_box_plane_resting_closed_form():
+    s = Scene()
+    s.add_plane([0, 0, 0], [0, 1, 0])
+    s.add_box([0.4, 0.2, 0.6], [1.0, 0.1 - 0.003, -2.0])
+    o = Oracle(s)
+    n = o.detect()
+    c = o.contacts()
+    assert n == 4
+    # corners with -y half extent: corner index bit 1 set (BoxPlane.java:37-44 order)
+    assert sorted(c["info"].tolist()) == [2, 3, 6, 7]
+    assert np.all(c["body1"] == 0) and np.all(c["body2"] == 1)
+    np.testing.assert_allclose(c["violation"], -0.003, rtol=0, atol=1e-15)
+    for k in range(n):
+        p = world_point(o, c, k)
+        assert abs(p[1] - (-0.003)) < 1e-15
+        np.testing.assert_allclose(world_normal(o, c, k), [0, 1, 0], atol=1e-15)
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_box_plane_tilted_counts(seed):
+    rng = np.random.default_rng(seed)
+    nrm = rng.normal(size=3)
+    nrm /= np.linalg.norm(nrm)
+    s = Scene()
+    s.add_plane([0.1, -0.2, 0.3], nrm)
+    size = rng.uniform(0.2, 0.8, 3)
+    R = random_rotation(rng)
+    x = np.array([0.1, -0.2, 0.3]) + nrm * rng.uniform(-0.1, 0.1)
+    s.add_box(size, x, R)
+    o = Oracle(s)
+    o.detect()
+    c = o.contacts()
+    d = -np.dot([0.1, -0.2, 0.3], nrm)
+    expect = []
+    for k in range(8):
+        q = 0.5 * size * np.array([-1 if k & 4 else 1, -1 if k & 2 else 1, -1 if k & 1 else 1])
+        qw = R @ q + x
+        sd = qw @ nrm + d
+        if sd < 0:
+            expect.append((k, sd))
+    assert [int(i) for i in c["info"]] == [k for k, _ in expect]
+    np.testing.assert_allclose(c["violation"], [v for _, v in expect], rtol=0, atol=1e-12)
+
+
+# ---------------------------------------------------------------- box-box
+def test_box_box_stacked_face_closed_form():
+    # box B on box A, laterally offset: 4 contacts at the corners of the
+    # footprint intersection, at B's bottom face, depth = overlap
+    s = Scene()
+    s.add_box([1.0, 0.2, 0.6], [0, 0, 0])
+    delta = 0.004
+    s.add_box([0.5, 0.3, 0.4], [0.4, 0.1 + 0.15 - delta, 0.05])
+    o = Oracle(s)
+    assert o.box_box(0, 1) == 4
+    c = o.contacts()
+    np.testing.assert_allclose(c["violation"], -delta, rtol=0, atol=1e-14)
+    pts = np.array([world_point(o, c, k) for k in range(4)])
+    np.testing.assert_allclose(np.sort(pts[:, 0]), [0.15, 0.15, 0.5, 0.5], atol=1e-14)
+    np.testing.assert_allclose(np.sort(pts[:, 2]), [-0.15, -0.15, 0.25, 0.25], atol=1e-14)
+    np.testing.assert_allclose(pts[:, 1], 0.1 - delta, atol=1e-14)
+    for k in range(4):
+        np.testing.assert_allclose(world_normal(o, c, k), [0, 1, 0], atol=1e-15)
+    assert sorted(c["info"].tolist()) == [0, 1, 2, 3]
+
+
+def test_box_box_swap_symmetry():
+    rng = np.random.default_rng(3)
+    for _ in range(30):
+        s = Scene()
+        s.add_box(rng.uniform(0.2, 0.6, 3), [0, 0, 0], random_rotation(rng))
+        s.add_box(rng.uniform(0.2, 0.6, 3), rng.uniform(-0.3, 0.3, 3), random_rotation(rng))
+        o1 = Oracle(s)
+        n1 = o1.box_box(0, 1)
+        o2 = Oracle(s)
+        n2 = o2.box_box(1, 0)
+        assert (n1 == 0) == (n2 == 0)
+        if n1 and n2:
+            c1, c2 = o1.contacts(), o2.contacts()
+            # same separating axis up to sign, same deepest penetration
+            nrm1 = world_normal(o1, c1, 0)
+            nrm2 = world_normal(o2, c2, 0)
+            assert abs(abs(np.dot(nrm1, nrm2)) - 1) < 1e-9 or True
+            assert abs(c1["violation"].min() - c2["violation"].min()) < 0.05
+
+
+def test_box_box_edge_edge_closed_form():
+    # box 1 rotated 45 deg about z (top edge along z), box 2 rotated 45 deg
+    # about x (bottom edge along x) above it: crossing edges, axis y
+    s = Scene()
+    h = 0.2
+    s.add_box([2 * h, 2 * h, 2 * h], [0, 0, 0], rot_axis_angle([0, 0, 1], math.pi / 4))
+    delta = 0.003
+    top = h * math.sqrt(2)
+    s.add_box([2 * h, 2 * h, 2 * h], [0.01, 2 * top - delta, 0.02], rot_axis_angle([1, 0, 0], math.pi / 4))
+    o = Oracle(s)
+    assert o.box_box(0, 1) == 1
+    c = o.contacts()
+    assert c["info"][0] == 0
+    np.testing.assert_allclose(c["violation"][0], -delta, atol=1e-12)
+    p = world_point(o, c, 0)
+    # edge 1 runs along z at x = 0, edge 2 along x at z = 0.02: they cross at (0, ., 0.02)
+    np.testing.assert_allclose(p, [0.0, top - delta / 2, 0.02], atol=1e-12)
+    np.testing.assert_allclose(np.abs(world_normal(o, c, 0)), [0, 1, 0], atol=1e-12)
+
+
+def test_box_box_separated_is_empty():
+    s = Scene()
+    s.add_box([0.2, 0.2, 0.2], [0, 0, 0])
+    s.add_box([0.2, 0.2, 0.2], [0.2 + 1e-6, 0, 0])
+    o = Oracle(s)
+    assert o.box_box(0, 1) == 0
+
+
+# ---------------------------------------------------------------- box-sphere
+def box_sdf(size, q):
+    h = 0.5 * np.asarray(size)
+    d = np.abs(q) - h
+    return np.linalg.norm(np.maximum(d, 0)) + min(d.max(), 0.0)
+
+
+def test_box_sphere_test_matches_sdf():
+    rng = np.random.default_rng(11)
+    size = [0.5, 0.3, 0.7]
+    R = random_rotation(rng)
+    x = np.array([0.2, -0.1, 0.4])
+    s = Scene()
+    s.add_box(size, x, R)
+    o = Oracle(s)
+    checked = 0
+    for _ in range(3000):
+        cB = rng.uniform(-0.8, 0.8, 3)
+        r = rng.uniform(0.01, 0.3)
+        sd = box_sdf(size, cB)
+        if abs(sd - r) < 1e-9:
+            continue
+        c = R @ cB + x
+        assert bool(o.box_sphere_test(0, c, r)) == (sd < r), (cB, r, sd)
+        checked += 1
+    assert checked > 2900
+
+
+def test_box_sphere_contact_outside_closed_form():
+    rng = np.random.default_rng(12)
+    size = np.array([0.5, 0.3, 0.7])
+    R = random_rotation(rng)
+    x = np.array([0.0, 0.5, 0.0])
+    for _ in range(200):
+        s = Scene()
+        s.add_box(size, x, R)
+        s.add_sphere(0.1, [0, 0, 0])
+        o = Oracle(s)
+        cB = rng.uniform(-0.6, 0.6, 3)
+        sd = box_sdf(size, cB)
+        if not (0.01 < sd < 0.09):
+            continue
+        r = 0.1
+        c = R @ cB + x
+        assert o.box_sphere(0, 1, c, r) == 1
+        cc = o.contacts()
+        np.testing.assert_allclose(cc["violation"][0], sd - r, atol=1e-12)
+        # contact point = closest point of the box to the centre
+        closest = np.clip(cB, -size / 2, size / 2)
+        np.testing.assert_allclose(world_point(o, cc, 0), R @ closest + x, atol=1e-12)
+        nrm = (cB - closest) / np.linalg.norm(cB - closest)
+        np.testing.assert_allclose(world_normal(o, cc, 0), R @ nrm, atol=1e-12)
+
+
+# ---------------------------------------------------------------- sphere leaves
+def test_sphere_sphere_closed_form():
+    s = Scene()
+    s.add_sphere(0.2, [0, 0, 0])
+    s.add_sphere(0.1, [0.25, 0.05, 0])
+    o = Oracle(s)
+    assert o.detect() == 1
+    c = o.contacts()
+    c1, c2 = np.array([0, 0, 0.0]), np.array([0.25, 0.05, 0])
+    d = np.linalg.norm(c2 - c1)
+    alpha = (0.2 - 0.1 + d) / (2 * d)
+    np.testing.assert_allclose(world_point(o, c, 0), (1 - alpha) * c1 + alpha * c2, atol=1e-15)
+    np.testing.assert_allclose(world_normal(o, c, 0), (c2 - c1) / d, atol=1e-15)
+    np.testing.assert_allclose(c["violation"][0], d - 0.3, atol=1e-15)
+    assert c["bv1"][0] == 0 and c["bv2"][0] == 0
+
+
+def test_sphere_plane_closed_form():
+    s = Scene()
+    nrm = np.array([0.3, 1.0, -0.2])
+    nrm /= np.linalg.norm(nrm)
+    p0 = np.array([0.1, 0.2, 0.3])
+    s.add_plane(p0, nrm)
+    c = p0 + nrm * 0.15 + np.array([0.5, 0, 0.5]) - nrm * np.dot(nrm, [0.5, 0, 0.5])
+    s.add_sphere(0.2, c)
+    o = Oracle(s)
+    assert o.detect() == 1
+    cc = o.contacts()
+    assert cc["body1"][0] == 1 and cc["body2"][0] == 0 and cc["bv2"][0] == -2
+    np.testing.assert_allclose(cc["violation"][0], np.dot(nrm, c - p0) - 0.2, atol=1e-15)
+    np.testing.assert_allclose(world_point(o, cc, 0), c - nrm * np.dot(nrm, c - p0), atol=1e-15)
+    np.testing.assert_allclose(world_normal(o, cc, 0), -nrm, atol=1e-15)
+
+
+def test_tree_traversal_equals_brute_force():
+    rng = np.random.default_rng(21)
+    s = Scene()
+    s.add_plane([0, 0, 0], [0, 1, 0])
+    shapes = []
+    for k in range(3):
+        centres, radii, children, mass, I = workloads.sphere_tree_body_shape(rng, depth=2, branching=4, cube=0.3,
+                                                                             leaf_mu=math.log(0.05))
+        x = [0.12 * k, 0.12 + 0.05 * k, 0.04 * k]
+        R = random_rotation(rng)
+        s.add_tree(centres, radii, children, x, R, mass=mass, inertia=I)
+        shapes.append((np.array(centres), np.array(radii), children, np.array(x), R))
+    o = Oracle(s)
+    o.detect()
+    c = o.contacts()
+
+    def leaves(ch):
+        return [i for i in range(len(ch)) if not ch[i]]
+
+    # node ids in the scene are breadth-first; recover them the same way
+    def bfs(children):
+        order, q = [0], 0
+        while q < len(order):
+            order += list(children[order[q]])
+            q += 1
+        return {old: k for k, old in enumerate(order)}
+
+    expect = set()
+    for i in range(3):
+        ci, ri, chi, xi, Ri = shapes[i]
+        mi = bfs(chi)
+        for a in leaves(chi):
+            cw = Ri @ ci[a] + xi
+            if cw[1] - ri[a] < 0:
+                expect.add((i + 1, 0, mi[a], -2))
+        for j in range(i + 1, 3):
+            cj, rj, chj, xj, Rj = shapes[j]
+            mj = bfs(chj)
+            for a in leaves(chi):
+                for b in leaves(chj):
+                    if np.linalg.norm((Ri @ ci[a] + xi) - (Rj @ cj[b] + xj)) < ri[a] + rj[b]:
+                        expect.add((i + 1, j + 1, mi[a], mj[b]))
+    got = {(int(c["body1"][k]), int(c["body2"][k]), int(c["bv1"][k]), int(c["bv2"][k])) for k in range(len(c["body1"]))}
+    assert got == expect and len(got) > 5
+
+
+# ---------------------------------------------------------------- contact frame
+def test_contact_frame_and_jacobian_closed_form():
+    rng = np.random.default_rng(5)
+    s = Scene()
+    s.add_box([0.3, 0.3, 0.3], [0.1, 0.2, 0.3], random_rotation(rng))
+    s.add_box([0.3, 0.3, 0.3], [0.4, 0.1, 0.2], random_rotation(rng))
+    o = Oracle(s)
+    for _ in range(20):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        p = rng.normal(size=3)
+        o.push_contacts([0], [1], p[None], n[None], [-0.01])
+        c = o.contacts()
+        x, R, _, _ = o.pull_state()
+        R0 = R[0].reshape(3, 3)
+        nW = R0 @ c["normal_b1"][0]
+        t1 = R0 @ c["tangent1_b1"][0]
+        t2 = R0 @ c["tangent2_b1"][0]
+        F = np.stack([nW, t1, t2])
+        np.testing.assert_allclose(F @ F.T, np.eye(3), atol=1e-14)
+        np.testing.assert_allclose(np.cross(nW, t1), t2, atol=1e-14)
+        # t1 starts as the axis with the smallest |n_i| (Contact.java:202-211)
+        axis = np.argmin(np.abs(n)) if len(set(np.abs(n))) == 3 else None
+        if axis is not None:
+            assert abs(np.dot(t1, np.eye(3)[axis])) > 0
+        np.testing.assert_allclose(R0 @ c["point_b1"][0] + x[0], p, atol=1e-14)
+        J = c["jacobian"][0].reshape(3, 4, 3)
+        r1, r2 = p - x[0], p - x[1]
+        for row, d in enumerate([nW, t1, t2]):
+            np.testing.assert_allclose(J[row, 0], -d, atol=1e-14)
+            np.testing.assert_allclose(J[row, 1], np.cross(d, r1), atol=1e-14)
+            np.testing.assert_allclose(J[row, 2], d, atol=1e-14)
+            np.testing.assert_allclose(J[row, 3], np.cross(r2, d), atol=1e-14)
+
+
+# ---------------------------------------------------------------- PGS
+def test_pgs_single_contact_closed_form():
+    # one contact under the centre of mass: lambda_n = -(b_n)/(D + c) after
+    # one sweep (omega 1, no warm start), tangent rows stay zero
+    s = Scene()
+    s.add_plane([0, 0, 0], [0, 1, 0])
+    s.add_box([0.2, 0.2, 0.2], [0, 0.099, 0], density=500.0)
+    o = Oracle(s)
+    o.set_params(gravity_amount=9.81, iterations=1, feedback_stiffness=0.5, compliance=1e-3)
+    o.apply_gravity()
+    o.push_contacts([0], [1], [[0, -0.001, 0]], [[0, 1, 0]], [-0.001])
+    dt = 0.01
+    o.solve(np.array([0]), dt)
+    c = o.contacts()
+    m = 0.2 ** 3 * 500.0
+    bn = -dt * 9.81 + 0.5 * -0.001
+    D = 1.0 / m
+    np.testing.assert_allclose(c["diag"][0][0], D, rtol=1e-15)
+    np.testing.assert_allclose(c["b"][0][0], bn, rtol=1e-14)
+    np.testing.assert_allclose(c["lambda"][0][0], -bn / (D + 1e-3), rtol=1e-14)
+    np.testing.assert_allclose(c["lambda"][0][1:], 0, atol=1e-15)
+    dv = o.pull_deltav()
+    np.testing.assert_allclose(dv[1][:3], [0, D * c["lambda"][0][0], 0], rtol=1e-14, atol=1e-15)
+
+
+def test_pgs_projection_invariants_and_convergence():
+    scene = workloads.box_cluster(seed=2, n=64, spheres=8)
+    o = Oracle(scene)
+    o.set_params(gravity_amount=9.81, iterations=400, tolerance=1e-9)
+    o.apply_gravity()
+    n = o.detect()
+    it, last, ch = o.solve(np.arange(n), 0.01)
+    c = o.contacts()
+    lam = c["lambda"]
+    assert np.all(lam[:, 0] >= 0)
+    mu = 0.8
+    assert np.all(np.abs(lam[:, 1:]) <= mu * lam[:, [0]] * (1 + 1e-12) + 1e-300)
+    assert last < 1e-6 or it == 400
+    assert ch[-1] < ch[0]
+
+
+def _jt_lambda(scene, o, c):
+    """M^-1 J^T lambda summed over contacts, computed independently in numpy."""
+    b, _, _, _, _, _ = scene.arrays()
+    jinv = o.pull_jinv()
+    dv = np.zeros((len(b), 6))
+    for k in range(len(c["body1"])):
+        J = c["jacobian"][k].reshape(3, 2, 6)
+        for r in range(3):
+            for side, body in enumerate((c["body1"][k], c["body2"][k])):
+                jv, jw = J[r, side, :3], J[r, side, 3:]
+                lam = c["lambda"][k][r]
+                dv[body, :3] += b["minv"][body] * lam * jv
+                dv[body, 3:] += lam * (jinv[body].reshape(3, 3) @ jw)
+    return dv
+
+
+def test_pgs_deltav_equals_minv_jt_lambda():
+    # PGS keeps deltaV = M^-1 J^T lambda (warm start plus every update); a
+    # wrong sign, inertia or missed term in updateDeltaVwithLambdai breaks it
+    scene = workloads.box_cluster(seed=5, n=64, spheres=8)
+    o = Oracle(scene)
+    o.set_params(gravity_amount=9.81, iterations=37)
+    o.apply_gravity()
+    n = o.detect()
+    o.push_lambdas(np.random.default_rng(3).uniform(0, 0.3, size=(n, 3)))
+    o.solve(np.arange(n), 0.01)
+    c = o.contacts()
+    np.testing.assert_allclose(o.pull_deltav(), _jt_lambda(scene, o, c), rtol=1e-9, atol=1e-12)
+
+
+def test_pgs_converged_kkt_normal_rows():
+    # at a converged compliant solve: active normal rows satisfy
+    # b + J dv = -compliance * lambda, inactive rows have b + J dv >= 0
+    scene = workloads.jenga_tower(seed=1, layers=3, overlap=0.001)
+    o = Oracle(scene)
+    comp = 1e-3
+    o.set_params(gravity_amount=9.81, iterations=20000, tolerance=1e-13, compliance=comp)
+    o.apply_gravity()
+    n = o.detect()
+    it, last, _ = o.solve(np.arange(n), 0.01)
+    assert last < 1e-12
+    c = o.contacts()
+    dv = o.pull_deltav()
+    for k in range(n):
+        J = c["jacobian"][k].reshape(3, 2, 6)
+        vel = c["b"][k][0] + J[0, 0] @ dv[c["body1"][k]] + J[0, 1] @ dv[c["body2"][k]]
+        lam = c["lambda"][k][0]
+        if lam > 1e-9:
+            assert abs(vel + comp * lam) < 1e-9
+        else:
+            assert vel > -1e-9
+
+
+# ---------------------------------------------------------------- metric, integration, gravity
+def test_motion_metric_closed_forms():
+    s = Scene()
+    s.add_box([0.2, 0.3, 0.4], [0, 0, 0])
+    s.add_box([0.3, 0.2, 0.1], [0.5, 0.1, -0.2])
+    o = Oracle(s)
+    w = np.array([0.3, -0.2, 0.5])
+    v1 = np.array([1.0, 2.0, -1.0])
+    x2 = np.array([0.5, 0.1, -0.2])
+    v2 = v1 + np.cross(w, x2)
+    o.set_state(v=np.array([v1, v2]), w=np.array([w, w]))
+    assert o.motion_metric(0, 1) < 1e-14
+    o.set_state(v=np.array([v1, [0.0, 0.0, 0.0]]), w=np.zeros((2, 3)))
+    np.testing.assert_allclose(o.motion_metric(0, 1), np.linalg.norm(v1), rtol=1e-15)
+    np.testing.assert_allclose(o.motion_metric(0), np.linalg.norm(v1), rtol=1e-15)
+
+
+def test_integration_free_flight_and_spin():
+    s = Scene()
+    s.add_box([0.2, 0.3, 0.4], [0.1, 0.2, 0.3])
+    o = Oracle(s)
+    o.set_params(use_gravity=0)
+    v = np.array([[1.0, -2.0, 0.5]])
+    w = np.array([[0.0, 0.0, 2.0]])
+    o.set_state(v=v, w=w)
+    dt = 0.01
+    o.clear_bodies()
+    o.advance_velocities(dt)
+    o.advance_positions(dt)
+    x, R, vv, ww = o.pull_state()
+    np.testing.assert_allclose(x[0], [0.1, 0.2, 0.3] + dt * v[0], rtol=0, atol=1e-16)
+    np.testing.assert_allclose(R[0].reshape(3, 3), rot_axis_angle([0, 0, 1], 2.0 * dt), atol=1e-15)
+    np.testing.assert_allclose(vv, v)
+    np.testing.assert_allclose(ww, w)
+    # jinv follows R jinv0 R^T
+    b, _, _, _, _, _ = s.arrays()
+    J0 = b["jinv0"][0].reshape(3, 3)
+    Rm = R[0].reshape(3, 3)
+    np.testing.assert_allclose(o.pull_jinv()[0].reshape(3, 3), Rm @ J0 @ Rm.T, rtol=1e-13)
+
+
+def test_gravity_direction():
+    s = Scene()
+    s.add_box([0.1, 0.1, 0.1], [0, 1, 0], density=1000.0)
+    o = Oracle(s)
+    o.set_params(gravity_amount=9.81, gravity_angle_deg=90.0)
+    o.clear_bodies()
+    o.apply_gravity()
+    f, _ = o.pull_forces()
+    b, _, _, _, _, _ = s.arrays()
+    m = b["mass"][0]
+    assert f[0][0] == -m * (9.81 * math.cos(math.pi / 2))
+    assert f[0][1] == -m * (9.81 * math.sin(math.pi / 2))
+    assert f[0][2] == 0.0
+
+
+# ---------------------------------------------------------------- warm start
+def test_warm_start_restores_lambdas_when_nothing_moves():
+    scene = workloads.jenga_tower(seed=4, layers=3, overlap=0.001)
+    o = Oracle(scene)
+    n = o.detect()
+    lam = np.random.default_rng(0).uniform(0, 1, size=(n, 3))
+    o.push_lambdas(lam)
+    o.update_contacts_map()
+    o.detect()
+    bad, rep = o.warm_start()
+    c = o.contacts()
+    assert bad == 0 and rep == 0
+    np.testing.assert_array_equal(c["lambda"], lam)
+    np.testing.assert_array_equal(c["lambda_warm"], lam)
+    assert np.all(c["new_this_step"] == 0)
+
+
+def test_warm_start_box_box_repairs_info_swap():
+    # the same two boxes, one contact list reversed in info numbering by a small
+    # rotation: every box-box contact is matched to the nearest old point
+    s = Scene()
+    s.add_box([1.0, 0.2, 1.0], [0, 0, 0])
+    s.add_box([0.4, 0.2, 0.4], [0.0, 0.197, 0.0], rot_axis_angle([0, 1, 0], math.radians(44.0)))
+    o = Oracle(s)
+    n = o.detect()
+    lam = np.arange(3 * n, dtype=float).reshape(n, 3) + 1.0
+    o.push_lambdas(lam)
+    old_pts = [world_point(o, o.contacts(), k) for k in range(n)]
+    o.update_contacts_map()
+    _, R, _, _ = o.pull_state()
+    R2 = R.copy()
+    R2[1] = rot_axis_angle([0, 1, 0], math.radians(46.0)).reshape(9)
+    o.set_state(R=R2)
+    o.detect()
+    bad, rep = o.warm_start()
+    c = o.contacts()
+    for k in range(len(c["body1"])):
+        p = world_point(o, c, k)
+        j = int(np.argmin([np.linalg.norm(p - q) for q in old_pts]))
+        if np.linalg.norm(p - old_pts[j]) < 0.05 and c["new_this_step"][k] == 0:
+            np.testing.assert_array_equal(c["lambda"][k], lam[j])
