@@ -69,6 +69,7 @@ class Params(C.Structure):
         ("step_accum_unmerging", C.c_int32), ("steps_between_merge_events", C.c_int32),
         ("threshold_merge", C.c_double), ("threshold_unmerge", C.c_double), ("threshold_breath", C.c_double),
         ("pgs_block", C.c_int32), ("pgs_grid", C.c_int32),
+        ("learned_warm_start", C.c_int32), ("collect_training", C.c_int32),
     ]
 
 
@@ -93,11 +94,13 @@ class StepStats(C.Structure):
         ("bad_warm_starts_repaired", C.c_int32), ("reserved", C.c_int32),
         ("last_change", C.c_double), ("ms_detect", C.c_double), ("ms_warm_start", C.c_double),
         ("ms_solve", C.c_double), ("ms_integrate", C.c_double), ("ms_total", C.c_double),
-        ("ms_pgs_kernel", C.c_double), ("ms_setup_kernel", C.c_double),
+        ("ms_pgs_kernel", C.c_double), ("ms_setup_kernel", C.c_double), ("ms_learned", C.c_double),
+        ("start_residual_plain", C.c_double), ("start_residual_learned", C.c_double),
+        ("used_learned", C.c_int32), ("reserved2", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
 
 
 # Exported symbols of include/am.h; the test suite checks this list against
@@ -109,8 +112,12 @@ SYMBOLS = [
     "am_update_contacts_map", "am_detect", "am_contacts_push", "am_contacts_pull", "am_lambdas_push",
     "am_warm_start", "am_redo_warm_start", "am_no_warm_start", "am_solve", "am_solve_order_pull",
     "am_colours_pull", "am_single_sweep", "am_changes_pull", "am_advance_velocities", "am_advance_positions",
-    "am_step", "am_motion_metric",
+    "am_step", "am_motion_metric", "am_mlp_init", "am_mlp_set_params", "am_mlp_get_params", "am_mlp_forward",
+    "am_mlp_gradients", "am_features_pull", "am_train_size", "am_train_clear", "am_train_epochs",
+    "am_start_lambdas_pull",
 ]
+MLP_PARAMS = 10115
+MLP_FEATURES = 24
 
 
 class AMError(RuntimeError):
@@ -167,6 +174,16 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "am_advance_positions": (C.c_int, [ctxp, dbl]),
         "am_step": (C.c_int, [ctxp, dbl, C.POINTER(StepStats)]),
         "am_motion_metric": (C.c_int, [ctxp, vp, i32, vp]),
+        "am_mlp_init": (C.c_int, [ctxp, C.c_uint64]),
+        "am_mlp_set_params": (C.c_int, [ctxp, vp, i32]),
+        "am_mlp_get_params": (C.c_int, [ctxp, vp, i32]),
+        "am_mlp_forward": (C.c_int, [ctxp, vp, i32, vp]),
+        "am_mlp_gradients": (C.c_int, [ctxp, vp, vp, i32, vp, C.POINTER(C.c_float)]),
+        "am_features_pull": (C.c_int, [ctxp, vp, vp, i32, C.POINTER(i32)]),
+        "am_train_size": (C.c_int, [ctxp, C.POINTER(i32)]),
+        "am_train_clear": (C.c_int, [ctxp]),
+        "am_train_epochs": (C.c_int, [ctxp, i32, i32, C.c_float, C.c_uint64, C.POINTER(C.c_float)]),
+        "am_start_lambdas_pull": (C.c_int, [ctxp, vp, i32, C.POINTER(i32)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -208,6 +225,7 @@ _DEFAULTS = dict(
     enable_unmerging=1, enable_unmerge_friction=1, enable_unmerge_normal=1, enable_unmerge_relative_motion=1,
     update_contacts_in_collections=1, step_accum_merging=3, step_accum_unmerging=3, steps_between_merge_events=10,
     threshold_merge=1e-2, threshold_unmerge=2e-2, threshold_breath=1e-5, pgs_block=0, pgs_grid=0,
+    learned_warm_start=0, collect_training=0,
 )
 
 

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "am_mlp.hpp"
 #include "am_solve.hpp"
 
 namespace {
@@ -102,6 +103,20 @@ struct am_ctx {
     int pgs_max_grid = 0;
     hipEvent_t kev[3] = {nullptr, nullptr, nullptr};  // setup start, pgs start, pgs end
     double ms_pgs_kernel = 0.0, ms_setup_kernel = 0.0;
+
+    // learned warm start
+    bool mlp_ready = false;
+    DBuf<float> m_p, m_g, m_m, m_v;  // padded FP32 parameters, gradients, Adam moments
+    DBuf<unsigned short> m_bf;       // padded BF16 weights
+    DBuf<float> f_X, f_Y, d_loss;
+    DBuf<double> f_scale, f_lam, s_start, dv_save;
+    DBuf<unsigned long long> d_resid;
+    DBuf<float> t_X, t_T;
+    DBuf<int> t_idx;
+    int t_n = 0, adam_step = 0;
+    double ms_learned = 0.0, res_plain = 0.0, res_learned = 0.0;
+    int used_learned = 0;
+    hipEvent_t lev[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -401,6 +416,8 @@ int am_ctx_destroy(am_ctx* c) {
     if (c->h_scalars) (void)hipHostFree(c->h_scalars);
     for (auto& e : c->kev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->lev)
+        if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return AM_OK;
@@ -601,6 +618,146 @@ int do_detect(am_ctx* c) {
     return groups_after_contacts(c);
 }
 
+// ---- learned warm start helpers ------------------------------------------
+constexpr int OW1 = 0, OW2 = kPadW1, OW3 = OW2 + kPadW2, OW4 = OW3 + kPadW3;
+constexpr int OB1 = OW4 + kPadW4, OB2 = OB1 + kPadB1, OB3 = OB2 + kPadB2, OB4 = OB3 + kPadB3;
+constexpr int kBfTotal = kPadW1 + kPadW2 + kPadW3 + kPadW4;
+
+// flat (reference layout of include/am.h) <-> padded device layout
+void pack_params(const float* flat, std::vector<float>& pad) {
+    pad.assign(kPadTotal, 0.f);
+    const float* q = flat;
+    for (int i = 0; i < 24; i++)
+        for (int j = 0; j < 64; j++) pad[OW1 + i * 64 + j] = *q++;
+    for (int j = 0; j < 64; j++) pad[OB1 + j] = *q++;
+    for (int i = 0; i < 64 * 64; i++) pad[OW2 + i] = *q++;
+    for (int j = 0; j < 64; j++) pad[OB2 + j] = *q++;
+    for (int i = 0; i < 64 * 64; i++) pad[OW3 + i] = *q++;
+    for (int j = 0; j < 64; j++) pad[OB3 + j] = *q++;
+    for (int i = 0; i < 64; i++)
+        for (int j = 0; j < 3; j++) pad[OW4 + i * 16 + j] = *q++;
+    for (int j = 0; j < 3; j++) pad[OB4 + j] = *q++;
+}
+void unpack_params(const std::vector<float>& pad, float* flat) {
+    float* q = flat;
+    for (int i = 0; i < 24; i++)
+        for (int j = 0; j < 64; j++) *q++ = pad[OW1 + i * 64 + j];
+    for (int j = 0; j < 64; j++) *q++ = pad[OB1 + j];
+    for (int i = 0; i < 64 * 64; i++) *q++ = pad[OW2 + i];
+    for (int j = 0; j < 64; j++) *q++ = pad[OB2 + j];
+    for (int i = 0; i < 64 * 64; i++) *q++ = pad[OW3 + i];
+    for (int j = 0; j < 64; j++) *q++ = pad[OB3 + j];
+    for (int i = 0; i < 64; i++)
+        for (int j = 0; j < 3; j++) *q++ = pad[OW4 + i * 16 + j];
+    for (int j = 0; j < 3; j++) *q++ = pad[OB4 + j];
+}
+
+MlpWeights mlp_weights(am_ctx* c) {
+    const unsigned short* bf = c->m_bf.p;
+    const float* p = c->m_p.p;
+    return MlpWeights{bf + OW1, bf + OW2, bf + OW3, bf + OW4, p + OB1, p + OB2, p + OB3, p + OB4};
+}
+MlpGrads mlp_grads(am_ctx* c) {
+    float* g = c->m_g.p;
+    return MlpGrads{g + OW1, g + OW2, g + OW3, g + OW4, g + OB1, g + OB2, g + OB3, g + OB4};
+}
+
+int mlp_alloc(am_ctx* c) {
+    AM_HIP(c, c->m_p.ensure(kPadTotal));
+    AM_HIP(c, c->m_g.ensure(kPadTotal));
+    AM_HIP(c, c->m_m.ensure(kPadTotal));
+    AM_HIP(c, c->m_v.ensure(kPadTotal));
+    AM_HIP(c, c->m_bf.ensure(kBfTotal));
+    AM_HIP(c, c->d_loss.ensure(4));
+    return AM_OK;
+}
+
+int mlp_upload(am_ctx* c, const float* flat) {
+    int r = mlp_alloc(c);
+    if (r) return r;
+    std::vector<float> pad;
+    pack_params(flat, pad);
+    hipStream_t s = c->stream;
+    AM_HIP(c, hipMemcpyAsync(c->m_p.p, pad.data(), kPadTotal * 4, hipMemcpyHostToDevice, s));
+    std::vector<float> z(kPadTotal, 0.f);
+    AM_HIP(c, hipMemcpyAsync(c->m_g.p, z.data(), kPadTotal * 4, hipMemcpyHostToDevice, s));
+    AM_HIP(c, hipMemcpyAsync(c->m_m.p, z.data(), kPadTotal * 4, hipMemcpyHostToDevice, s));
+    AM_HIP(c, hipMemcpyAsync(c->m_v.p, z.data(), kPadTotal * 4, hipMemcpyHostToDevice, s));
+    launch_to_bf16(s, c->m_p.p, c->m_bf.p, kBfTotal);
+    AM_HIP(c, hipGetLastError());
+    AM_HIP(c, hipStreamSynchronize(s));
+    c->adam_step = 0;
+    c->mlp_ready = true;
+    return AM_OK;
+}
+
+int compute_features(am_ctx* c, double dt) {
+    const int n = c->n_contacts;
+    AM_HIP(c, c->f_X.ensure((size_t)std::max(n, 1) * kMlpIn));
+    AM_HIP(c, c->f_Y.ensure((size_t)std::max(n, 1) * kMlpOut));
+    AM_HIP(c, c->f_scale.ensure(std::max(n, 1)));
+    AM_HIP(c, c->f_lam.ensure((size_t)std::max(n, 1) * 3));
+    FeatureArgs F;
+    F.B = bodies_view(c);
+    F.C = contacts_view(c);
+    F.n = n;
+    F.g = c->params.gravity_amount != 0.0 ? std::fabs(c->params.gravity_amount) : 1.0;
+    F.dt = dt > 0 ? dt : 0.01;
+    F.out = c->f_X.p;
+    F.scale = c->f_scale.p;
+    launch_features(c->stream, F);
+    AM_HIP(c, hipGetLastError());
+    return AM_OK;
+}
+
+int train_append(am_ctx* c) {
+    const int n = c->n_contacts;
+    if (n <= 0) return AM_OK;
+    size_t need = (size_t)c->t_n + n;
+    if (c->t_X.n < need * kMlpIn) {
+        size_t cap = std::max(need + need / 2, (size_t)65536);
+        DBuf<float> nx, nt;
+        AM_HIP(c, nx.ensure(cap * kMlpIn));
+        AM_HIP(c, nt.ensure(cap * kMlpOut));
+        if (c->t_n > 0) {
+            AM_HIP(c, hipMemcpyAsync(nx.p, c->t_X.p, (size_t)c->t_n * kMlpIn * 4, hipMemcpyDeviceToDevice, c->stream));
+            AM_HIP(c, hipMemcpyAsync(nt.p, c->t_T.p, (size_t)c->t_n * kMlpOut * 4, hipMemcpyDeviceToDevice, c->stream));
+        }
+        AM_HIP(c, hipStreamSynchronize(c->stream));
+        std::swap(c->t_X.p, nx.p);
+        std::swap(c->t_X.n, nx.n);
+        std::swap(c->t_T.p, nt.p);
+        std::swap(c->t_T.n, nt.n);
+    }
+    AM_HIP(c, hipMemcpyAsync(c->t_X.p + (size_t)c->t_n * kMlpIn, c->f_X.p, (size_t)n * kMlpIn * 4,
+                             hipMemcpyDeviceToDevice, c->stream));
+    launch_targets(c->stream, c->c_lam.p, c->f_scale.p, n, c->t_T.p + (size_t)c->t_n * kMlpOut);
+    AM_HIP(c, hipGetLastError());
+    c->t_n += n;
+    return AM_OK;
+}
+
+// Start residual of the lambdas currently in the records (sweep order).
+int start_residual(am_ctx* c, PgsArgs G, double* out) {
+    hipStream_t s = c->stream;
+    AM_HIP(c, hipMemcpyAsync(c->d_dv.p, c->dv_save.p, (size_t)c->n_ent * 48, hipMemcpyDeviceToDevice, s));
+    G.iterations = 0;
+    G.warm_start = 1;
+    fill_int(s, c->d_barrier.p, 2, 0);
+    AM_HIP(c, launch_pgs(s, G, 1, 256));
+    AM_HIP(c, c->d_resid.ensure(1));
+    fill_int(s, (int*)c->d_resid.p, 2, 0);
+    launch_start_residual(s, c->s_rec.p, c->s_meta.p, c->d_dv.p, c->d_minv.p, c->d_jinv.p, c->n_contacts, G.omega,
+                          G.compliance, c->d_resid.p);
+    AM_HIP(c, hipGetLastError());
+    unsigned long long bits = 0;
+    AM_HIP(c, hipMemcpyAsync(&bits, c->d_resid.p, 8, hipMemcpyDeviceToHost, s));
+    AM_HIP(c, hipStreamSynchronize(s));
+    std::memcpy(out, &bits, 8);
+    AM_HIP(c, hipMemcpyAsync(c->d_dv.p, c->dv_save.p, (size_t)c->n_ent * 48, hipMemcpyDeviceToDevice, s));
+    return AM_OK;
+}
+
 int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double* last) {
     const am_params& P = c->params;
     c->changes.clear();
@@ -635,6 +792,19 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
     fill_int(s, c->d_scalars.p + 1, 1, c->n_contacts);
     for (auto& e : c->kev)
         if (!e) AM_HIP(c, hipEventCreate(&e));
+    for (auto& e : c->lev)
+        if (!e) AM_HIP(c, hipEventCreate(&e));
+    const bool learned = main_solve && P.learned_warm_start && c->mlp_ready;
+    const bool collect = main_solve && P.collect_training;
+    c->used_learned = 0;
+    c->res_plain = 0;
+    c->res_learned = 0;
+    c->ms_learned = 0;
+    AM_HIP(c, hipEventRecord(c->lev[0], s));
+    if (learned || collect) {
+        r = compute_features(c, dt);
+        if (r) return r;
+    }
     AM_HIP(c, hipEventRecord(c->kev[0], s));
     launch_setup(s, S, c->n_contacts);
     const int iterations = main_solve ? P.iterations : P.iterations_in_collection;
@@ -669,16 +839,49 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
         c->pgs_max_grid = pgs_max_blocks(block, &ok);
         if (!ok) c->pgs_max_grid = 1;
     }
-    int grid = P.pgs_grid > 0 ? P.pgs_grid : (c->max_colour_groups + block - 1) / block;
+    // groups of a colour are dealt block-cyclically; ~16 groups per workgroup
+    // spreads a colour over enough CUs without making the barrier expensive
+    int grid = P.pgs_grid > 0 ? P.pgs_grid : (c->max_colour_groups + 15) / 16;
     grid = std::max(1, std::min(grid, c->pgs_max_grid));
+    if (learned) {
+        // plain start residual, predicted start residual; keep the better
+        AM_HIP(c, c->dv_save.ensure((size_t)c->n_ent * 6));
+        AM_HIP(c, hipMemcpyAsync(c->dv_save.p, c->d_dv.p, (size_t)c->n_ent * 48, hipMemcpyDeviceToDevice, s));
+        r = start_residual(c, G, &c->res_plain);
+        if (r) return r;
+        launch_mlp_forward(s, mlp_weights(c), c->f_X.p, c->n_contacts, c->f_Y.p);
+        launch_apply_prediction(s, contacts_view(c), bodies_view(c), c->f_Y.p, c->f_scale.p, c->n_contacts,
+                                P.friction_override, P.friction, c->f_lam.p);
+        launch_rec_lambda(s, c->s_rec.p, c->s_order.p, c->f_lam.p, c->n_contacts);
+        AM_HIP(c, hipGetLastError());
+        r = start_residual(c, G, &c->res_learned);
+        if (r) return r;
+        if (c->res_learned < c->res_plain) {
+            c->used_learned = 1;
+            AM_HIP(c, hipMemcpyAsync(c->c_lam.p, c->f_lam.p, (size_t)c->n_contacts * 24, hipMemcpyDeviceToDevice, s));
+        } else {
+            launch_rec_lambda(s, c->s_rec.p, c->s_order.p, c->c_lam.p, c->n_contacts);
+        }
+        fill_int(s, c->d_barrier.p, 2, 0);
+    }
+    AM_HIP(c, hipEventRecord(c->lev[1], s));
+    AM_HIP(c, c->s_start.ensure((size_t)std::max(c->n_contacts, 1) * 3));
+    AM_HIP(c, hipMemcpyAsync(c->s_start.p, c->c_lam.p, (size_t)c->n_contacts * 24, hipMemcpyDeviceToDevice, s));
     AM_HIP(c, hipEventRecord(c->kev[1], s));
     AM_HIP(c, launch_pgs(s, G, grid, block));
     AM_HIP(c, hipEventRecord(c->kev[2], s));
     launch_unpack(s, S, c->n_contacts);
     AM_HIP(c, hipGetLastError());
+    if (collect) {
+        r = train_append(c);
+        if (r) return r;
+    }
     r = read_scalars(c, 6, 1);
     if (r) return r;
     {
+        float lms = 0;
+        (void)hipEventElapsedTime(&lms, c->lev[0], c->lev[1]);
+        c->ms_learned = learned ? lms : 0.0;
         int berr = 0;
         AM_HIP(c, hipMemcpy(&berr, c->d_barrier.p + 1, sizeof(int), hipMemcpyDeviceToHost));
         if (berr) return fail(c, AM_ERR_HIP, "PGS grid barrier did not complete (blocks not co-resident)");
@@ -1253,6 +1456,10 @@ int am_step(am_ctx* c, double dt, am_step_stats* st) {
         st->ms_total = ms;
         st->ms_pgs_kernel = c->n_contacts ? c->ms_pgs_kernel : 0.0;
         st->ms_setup_kernel = c->n_contacts ? c->ms_setup_kernel : 0.0;
+        st->ms_learned = c->n_contacts ? c->ms_learned : 0.0;
+        st->start_residual_plain = c->res_plain;
+        st->start_residual_learned = c->res_learned;
+        st->used_learned = c->used_learned;
     }
     return AM_OK;
 }
@@ -1273,6 +1480,177 @@ int am_motion_metric(am_ctx* c, const int32_t* pairs, int32_t n_pairs, double* o
     launch_metric(c->stream, bodies_view(c), dp.p, n_pairs, dout.p);
     AM_HIP(c, hipGetLastError());
     AM_HIP(c, hipMemcpyAsync(out, dout.p, (size_t)n_pairs * 8, hipMemcpyDeviceToHost, c->stream));
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    return AM_OK;
+}
+
+
+int am_mlp_init(am_ctx* c, uint64_t seed) {
+    int r = check_ctx(c);
+    if (r) return r;
+    // He-uniform initialisation, biases 0; a small output layer so the first
+    // predictions stay near zero (a splitmix64 stream seeded by `seed`)
+    std::vector<float> flat(kMlpParams, 0.f);
+    uint64_t st = seed ^ 0x9e3779b97f4a7c15ull;
+    auto rnd = [&st]() {
+        st += 0x9e3779b97f4a7c15ull;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    };
+    size_t q = 0;
+    const int dims[4][2] = {{24, 64}, {64, 64}, {64, 64}, {64, 3}};
+    for (int layer = 0; layer < 4; layer++) {
+        int fan_in = dims[layer][0], fan_out = dims[layer][1];
+        double lim = std::sqrt(6.0 / fan_in) * (layer == 3 ? 0.1 : 1.0);
+        for (int i = 0; i < fan_in * fan_out; i++) flat[q++] = (float)((2.0 * rnd() - 1.0) * lim);
+        q += fan_out;  // biases stay 0
+    }
+    return mlp_upload(c, flat.data());
+}
+
+int am_mlp_set_params(am_ctx* c, const float* params, int32_t n) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!params || n != kMlpParams) return fail(c, AM_ERR_INVALID, "expected AM_MLP_PARAMS floats");
+    return mlp_upload(c, params);
+}
+
+int am_mlp_get_params(am_ctx* c, float* params, int32_t n) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!params || n != kMlpParams) return fail(c, AM_ERR_INVALID, "expected AM_MLP_PARAMS floats");
+    if (!c->mlp_ready) return fail(c, AM_ERR_STATE, "MLP not initialised");
+    std::vector<float> pad(kPadTotal);
+    AM_HIP(c, hipMemcpyAsync(pad.data(), c->m_p.p, kPadTotal * 4, hipMemcpyDeviceToHost, c->stream));
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    unpack_params(pad, params);
+    return AM_OK;
+}
+
+int am_mlp_forward(am_ctx* c, const float* x, int32_t n, float* y) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!c->mlp_ready) return fail(c, AM_ERR_STATE, "MLP not initialised");
+    if (n < 0 || (n > 0 && (!x || !y))) return fail(c, AM_ERR_INVALID, "bad MLP arrays");
+    if (n == 0) return AM_OK;
+    DBuf<float> dx, dy;
+    AM_HIP(c, dx.ensure((size_t)n * kMlpIn));
+    AM_HIP(c, dy.ensure((size_t)n * kMlpOut));
+    AM_HIP(c, hipMemcpyAsync(dx.p, x, (size_t)n * kMlpIn * 4, hipMemcpyHostToDevice, c->stream));
+    launch_mlp_forward(c->stream, mlp_weights(c), dx.p, n, dy.p);
+    AM_HIP(c, hipGetLastError());
+    AM_HIP(c, hipMemcpyAsync(y, dy.p, (size_t)n * kMlpOut * 4, hipMemcpyDeviceToHost, c->stream));
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    return AM_OK;
+}
+
+int am_mlp_gradients(am_ctx* c, const float* x, const float* t, int32_t n, float* grads, float* loss) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!c->mlp_ready) return fail(c, AM_ERR_STATE, "MLP not initialised");
+    if (n <= 0 || !x || !t || !grads) return fail(c, AM_ERR_INVALID, "bad MLP arrays");
+    DBuf<float> dx, dt_;
+    DBuf<int> di;
+    AM_HIP(c, dx.ensure((size_t)n * kMlpIn));
+    AM_HIP(c, dt_.ensure((size_t)n * kMlpOut));
+    AM_HIP(c, di.ensure(n));
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; i++) idx[i] = i;
+    hipStream_t s = c->stream;
+    AM_HIP(c, hipMemcpyAsync(dx.p, x, (size_t)n * kMlpIn * 4, hipMemcpyHostToDevice, s));
+    AM_HIP(c, hipMemcpyAsync(dt_.p, t, (size_t)n * kMlpOut * 4, hipMemcpyHostToDevice, s));
+    AM_HIP(c, hipMemcpyAsync(di.p, idx.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+    fill_int(s, (int*)c->m_g.p, kPadTotal, 0);
+    fill_int(s, (int*)c->d_loss.p, 1, 0);
+    launch_mlp_backward(s, mlp_weights(c), dx.p, dt_.p, di.p, n, 1.0f / (float)n, mlp_grads(c), c->d_loss.p);
+    AM_HIP(c, hipGetLastError());
+    std::vector<float> pad(kPadTotal);
+    float l = 0;
+    AM_HIP(c, hipMemcpyAsync(pad.data(), c->m_g.p, kPadTotal * 4, hipMemcpyDeviceToHost, s));
+    AM_HIP(c, hipMemcpyAsync(&l, c->d_loss.p, 4, hipMemcpyDeviceToHost, s));
+    AM_HIP(c, hipStreamSynchronize(s));
+    fill_int(s, (int*)c->m_g.p, kPadTotal, 0);
+    unpack_params(pad, grads);
+    if (loss) *loss = l;
+    return AM_OK;
+}
+
+int am_features_pull(am_ctx* c, float* x, double* scale, int32_t capacity, int32_t* n) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (n) *n = c->n_contacts;
+    if (c->n_contacts == 0 || (!x && !scale)) return AM_OK;
+    r = compute_features(c, 0.01);
+    if (r) return r;
+    size_t m = (size_t)std::min(capacity, c->n_contacts);
+    if (x) AM_HIP(c, hipMemcpyAsync(x, c->f_X.p, m * kMlpIn * 4, hipMemcpyDeviceToHost, c->stream));
+    if (scale) AM_HIP(c, hipMemcpyAsync(scale, c->f_scale.p, m * 8, hipMemcpyDeviceToHost, c->stream));
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    return AM_OK;
+}
+
+int am_train_size(am_ctx* c, int32_t* n) {
+    if (!c || !n) return AM_ERR_INVALID;
+    *n = c->t_n;
+    return AM_OK;
+}
+
+int am_train_clear(am_ctx* c) {
+    if (!c) return AM_ERR_INVALID;
+    c->t_n = 0;
+    return AM_OK;
+}
+
+int am_train_epochs(am_ctx* c, int32_t epochs, int32_t batch, float lr, uint64_t seed, float* last_loss) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!c->mlp_ready) return fail(c, AM_ERR_STATE, "MLP not initialised");
+    if (epochs < 0 || batch <= 0 || c->t_n <= 0) return fail(c, AM_ERR_INVALID, "no training data or bad batch");
+    const int n = c->t_n;
+    hipStream_t s = c->stream;
+    AM_HIP(c, c->t_idx.ensure(n));
+    std::vector<int> perm(n);
+    for (int i = 0; i < n; i++) perm[i] = i;
+    uint64_t st = seed;
+    float loss = 0;
+    for (int e = 0; e < epochs; e++) {
+        for (int i = n - 1; i > 0; i--) {
+            st = st * 6364136223846793005ull + 1442695040888963407ull;
+            int j = (int)((st >> 33) % (uint64_t)(i + 1));
+            std::swap(perm[i], perm[j]);
+        }
+        AM_HIP(c, hipMemcpyAsync(c->t_idx.p, perm.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+        fill_int(s, (int*)c->d_loss.p, 1, 0);
+        for (int b0 = 0; b0 < n; b0 += batch) {
+            int m = std::min(batch, n - b0);
+            launch_mlp_backward(s, mlp_weights(c), c->t_X.p, c->t_T.p, c->t_idx.p + b0, m, 1.0f / (float)m,
+                                mlp_grads(c), c->d_loss.p);
+            c->adam_step++;
+            launch_adam(s, c->m_p.p, c->m_g.p, c->m_m.p, c->m_v.p, c->m_bf.p, kBfTotal, lr, 0.9f, 0.999f, 1e-8f,
+                        c->adam_step, nullptr);
+            launch_adam(s, c->m_p.p + kBfTotal, c->m_g.p + kBfTotal, c->m_m.p + kBfTotal, c->m_v.p + kBfTotal,
+                        nullptr, kPadTotal - kBfTotal, lr, 0.9f, 0.999f, 1e-8f, c->adam_step, nullptr);
+        }
+        AM_HIP(c, hipGetLastError());
+        AM_HIP(c, hipMemcpyAsync(&loss, c->d_loss.p, 4, hipMemcpyDeviceToHost, s));
+        AM_HIP(c, hipStreamSynchronize(s));
+        int nb = (n + batch - 1) / batch;
+        loss /= (float)nb;
+    }
+    if (last_loss) *last_loss = loss;
+    return AM_OK;
+}
+
+int am_start_lambdas_pull(am_ctx* c, double* lambda, int32_t capacity, int32_t* n) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (n) *n = c->n_contacts;
+    if (!lambda || c->n_contacts == 0 || !c->s_start.p) return AM_OK;
+    size_t m = (size_t)std::min(capacity, c->n_contacts);
+    AM_HIP(c, hipMemcpyAsync(lambda, c->s_start.p, m * 24, hipMemcpyDeviceToHost, c->stream));
     AM_HIP(c, hipStreamSynchronize(c->stream));
     return AM_OK;
 }

@@ -339,9 +339,18 @@ struct BodyRegs {
     bool dyn;
 };
 
+// deltaV crosses workgroups between colours: it is read and written with
+// agent-scope relaxed atomics (coherent at device level, global_load/store
+// ... sc1), so the barrier needs no L2 write-back or invalidation and the
+// read-only contact records stay resident in each XCD's L2 across sweeps.
+AMD_INLINE double ld_coherent(const double* p) {
+    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+AMD_INLINE void st_coherent(double* p, double v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
 AMD_INLINE void load_body(const PgsArgs& P, int e, BodyRegs& r) {
 #pragma unroll
-    for (int i = 0; i < 6; i++) r.dv[i] = P.dv[e * 6 + i];
+    for (int i = 0; i < 6; i++) r.dv[i] = ld_coherent(&P.dv[e * 6 + i]);
     r.minv = P.minv[e];
     r.jinv = ldm(&P.jinv[e * 9]);
     r.dyn = (P.bflags[e] & AM_BODY_PINNED) == 0;
@@ -349,7 +358,7 @@ AMD_INLINE void load_body(const PgsArgs& P, int e, BodyRegs& r) {
 AMD_INLINE void store_body(const PgsArgs& P, int e, const BodyRegs& r) {
     if (!r.dyn) return;
 #pragma unroll
-    for (int i = 0; i < 6; i++) P.dv[e * 6 + i] = r.dv[i];
+    for (int i = 0; i < 6; i++) st_coherent(&P.dv[e * 6 + i], r.dv[i]);
 }
 AMD_INLINE void swap_regs(BodyRegs& a, BodyRegs& b) {
     BodyRegs t = a;
@@ -407,14 +416,24 @@ AMD_INLINE void load_rec(const double* src, Rec& r) {
     }
 }
 
+// The first loads of a group (descriptor and first contact's meta) do not
+// depend on other workgroups' results, so they are issued before the barrier
+// that precedes the group's colour.
+struct GroupHead {
+    int4 gd;  // first, end sweep position, solve bodies A, B
+    int4 m0;
+};
+AMD_INLINE void group_head(const PgsArgs& P, int q, GroupHead& h) {
+    h.gd = reinterpret_cast<const int4*>(P.group_desc)[q];
+    h.m0 = reinterpret_cast<const int4*>(P.meta)[h.gd.x];
+}
+
 // mode 0: confidentWarmStart (PGS.java:250-256); mode 1: one PGS pass over the group.
-// gd = (first sweep position, end, solve body A, solve body B) of the group.
-AMD_INLINE void process_group(const PgsArgs& P, int q, int mode, bool last_iter, double& mx) {
-    const int4 gd = reinterpret_cast<const int4*>(P.group_desc)[q];
-    const int k0 = gd.x, k1 = gd.y, A = gd.z, Bi = gd.w;
+AMD_INLINE void process_group(const PgsArgs& P, const GroupHead& h, int mode, bool last_iter, double& mx) {
+    const int k0 = h.gd.x, k1 = h.gd.y, A = h.gd.z, Bi = h.gd.w;
     const int4* meta = reinterpret_cast<const int4*>(P.meta);
     Rec cur, nxt;
-    int4 mcur = meta[k0];
+    int4 mcur = h.m0;
     load_rec(&P.rec[(size_t)k0 * kRecDoubles], cur);
     BodyRegs ra, rb;
     load_body(P, A, ra);
@@ -491,9 +510,9 @@ AMD_INLINE double wave_max(double v) {
 }
 
 // Grid barrier for a co-resident (cooperatively launched) grid: every wave
-// makes its stores visible at agent scope (L2 write-back), lane 0 of each
-// block arrives on a monotonically increasing counter and spins with acquire
-// loads until all blocks of this barrier instance have arrived.  A spin limit
+// waits until its (device-coherent) deltaV stores are complete, lane 0 of each
+// block arrives on a monotonically increasing counter and spins on a coherent
+// load until all blocks of this barrier instance have arrived.  A spin limit
 // turns a barrier that cannot complete into an error flag instead of a hang.
 template <bool kSingleBlock>
 AMD_INLINE bool grid_barrier(const PgsArgs& P, unsigned& epoch) {
@@ -503,13 +522,13 @@ AMD_INLINE bool grid_barrier(const PgsArgs& P, unsigned& epoch) {
     }
     __shared__ int bail;
     epoch += gridDim.x;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
         bail = 0;
         __hip_atomic_fetch_add(P.barrier, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         long spins = 0;
-        while (__hip_atomic_load(P.barrier, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+        while (__hip_atomic_load(P.barrier, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
             if (++spins > 20000000L) {
                 atomicOr(P.error, 1);
                 bail = 1;
@@ -524,31 +543,69 @@ AMD_INLINE bool grid_barrier(const PgsArgs& P, unsigned& epoch) {
 }
 
 // Persistent kernel: warm-start pass then up to `iterations` sweeps; one grid
-// barrier per non-empty colour.  The per-sweep max |delta lambda| is reduced
-// with an integer atomic max on the bit pattern (non-negative doubles order as
-// their bits) into a slot per sweep.
+// barrier per non-empty colour.  Groups of a colour are dealt block-cyclically
+// (group j of the colour goes to block j % grid, lane j / grid) so every
+// workgroup, hence every CU in use, gets a share.  The per-sweep max |delta
+// lambda| is reduced with an integer atomic max on the bit pattern (non-
+// negative doubles order as their bits) into a slot per sweep.
 template <bool kSingleBlock>
 __global__ __launch_bounds__(256) void k_pgs(PgsArgs P) {
-    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int gstride = gridDim.x * blockDim.x;
     const int* cs = P.colour_start;
-    unsigned epoch = 0;
-    // Colours with no group are skipped by every thread alike (same reads), so
-    // barriers stay uniform.  The tail colour (kMaxColours) runs on one lane.
-    int last_col = -1;
-    for (int c = 0; c < kMaxColours; c++)
-        if (cs[c + 1] > cs[c]) last_col = c;
+    const int slot0 = threadIdx.x * gridDim.x + blockIdx.x;
+    const int stride = gridDim.x * blockDim.x;
+    __shared__ int s_cols[kMaxColours];
+    __shared__ int s_ncols;
+    if (threadIdx.x == 0) {
+        int nc = 0;
+        for (int c = 0; c < kMaxColours; c++)
+            if (cs[c + 1] > cs[c]) s_cols[nc++] = c;
+        s_ncols = nc;
+    }
+    __syncthreads();
+    const int ncols = s_ncols;
     const bool has_tail = cs[kMaxColours + 1] > cs[kMaxColours];
+    const bool lane0 = blockIdx.x == 0 && threadIdx.x == 0;
+    unsigned epoch = 0;
+    GroupHead h;
+    // Runs colour list position ci; prefetches the head of the next position.
+    auto run_colour = [&](int ci, int mode, bool last, double& mx) {
+        const int c = s_cols[ci];
+        int q = cs[c] + slot0;
+        if (q < cs[c + 1]) {
+            process_group(P, h, mode, last, mx);
+            for (q += stride; q < cs[c + 1]; q += stride) {
+                group_head(P, q, h);
+                process_group(P, h, mode, last, mx);
+            }
+        }
+        const int cn = s_cols[(ci + 1) % ncols];
+        const int qn = cs[cn] + slot0;
+        if (qn < cs[cn + 1]) group_head(P, qn, h);
+    };
+    auto run_tail = [&](int mode, bool last, double& mx) {
+        if (lane0)
+            for (int q = cs[kMaxColours]; q < cs[kMaxColours + 1]; q++) {
+                group_head(P, q, h);
+                process_group(P, h, mode, last, mx);
+            }
+        if (ncols > 0) {
+            const int cn = s_cols[0];
+            const int qn = cs[cn] + slot0;
+            if (qn < cs[cn + 1]) group_head(P, qn, h);
+        }
+    };
+    if (ncols > 0) {
+        const int c = s_cols[0];
+        if (cs[c] + slot0 < cs[c + 1]) group_head(P, cs[c] + slot0, h);
+    }
     if (P.warm_start) {
         double dummy = 0;
-        for (int c = 0; c <= last_col; c++) {
-            if (cs[c + 1] == cs[c]) continue;
-            for (int q = cs[c] + gtid; q < cs[c + 1]; q += gstride) process_group(P, q, 0, false, dummy);
+        for (int ci = 0; ci < ncols; ci++) {
+            run_colour(ci, 0, false, dummy);
             if (!grid_barrier<kSingleBlock>(P, epoch)) return;
         }
         if (has_tail) {
-            if (gtid == 0)
-                for (int q = cs[kMaxColours]; q < cs[kMaxColours + 1]; q++) process_group(P, q, 0, false, dummy);
+            run_tail(0, false, dummy);
             if (!grid_barrier<kSingleBlock>(P, epoch)) return;
         }
     }
@@ -556,10 +613,9 @@ __global__ __launch_bounds__(256) void k_pgs(PgsArgs P) {
     for (; it < P.iterations; it++) {
         const bool last = (it == P.iterations - 1);
         double mx = 0;
-        for (int c = 0; c <= last_col; c++) {
-            if (cs[c + 1] == cs[c]) continue;
-            for (int q = cs[c] + gtid; q < cs[c + 1]; q += gstride) process_group(P, q, 1, last, mx);
-            if (c == last_col && !has_tail) {
+        for (int ci = 0; ci < ncols; ci++) {
+            run_colour(ci, 1, last, mx);
+            if (ci == ncols - 1 && !has_tail) {
                 double wm = wave_max(mx);
                 if ((threadIdx.x & 63) == 0 && wm > 0)
                     atomicMax(&P.iter_max[it], (unsigned long long)__double_as_longlong(wm));
@@ -567,14 +623,13 @@ __global__ __launch_bounds__(256) void k_pgs(PgsArgs P) {
             if (!grid_barrier<kSingleBlock>(P, epoch)) return;
         }
         if (has_tail) {
-            if (gtid == 0)
-                for (int q = cs[kMaxColours]; q < cs[kMaxColours + 1]; q++) process_group(P, q, 1, last, mx);
+            run_tail(1, last, mx);
             double wm = wave_max(mx);
             if ((threadIdx.x & 63) == 0 && wm > 0)
                 atomicMax(&P.iter_max[it], (unsigned long long)__double_as_longlong(wm));
             if (!grid_barrier<kSingleBlock>(P, epoch)) return;
         }
-        if (last_col < 0 && !has_tail) {
+        if (ncols == 0 && !has_tail) {
             if (!grid_barrier<kSingleBlock>(P, epoch)) return;
         }
         double m = __longlong_as_double((long long)__hip_atomic_load(&P.iter_max[it], __ATOMIC_RELAXED,
@@ -584,7 +639,7 @@ __global__ __launch_bounds__(256) void k_pgs(PgsArgs P) {
             break;
         }
     }
-    if (gtid == 0) *P.iters_used = it;
+    if (lane0) *P.iters_used = it;
 }
 
 int pgs_max_blocks(int block, bool* ok) {

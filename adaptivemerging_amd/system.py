@@ -1,0 +1,221 @@
+"""Python mirror of the reference's Java call surface over libam.
+
+`RigidBodySystem`, `CollisionProcessor` and `PGS` keep the names, fields and
+call order of src/mergingBodies3D/{RigidBodySystem,CollisionProcessor,PGS}.java
+so code (and tests) written against the reference read the same.  Every
+method runs on the GPU through the C ABI; nothing here computes physics.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from .context import Context
+
+
+class PGS:
+    """m3d/PGS.java:15-63: fields of the projected Gauss-Seidel solver."""
+
+    def __init__(self, iterations: int = 200, ctx: Context | None = None):
+        self.ctx = ctx
+        self.changes: list[float] = []
+        self.init(iterations)
+
+    def init(self, iterations: int):
+        """PGS.init (PGS.java:25-34)."""
+        self.iterations = iterations
+        self.tolerance = 1e-5
+        self.omega = 1.0
+        self.feedbackStiffness = 0.0
+        self.compliance = 0.0
+        self.contacts = None
+        self.warmStart = False
+        self.computeInCollection = False
+        self.postStabilization = False
+
+    def _apply(self, ctx: Context, restitutionOverride, restitutionOverrideVal, frictionOverride, frictionOverrideVal):
+        p = ctx.params
+        if self.computeInCollection:
+            p.iterations_in_collection = self.iterations
+        else:
+            p.iterations = self.iterations
+        p.tolerance = self.tolerance
+        p.omega = self.omega
+        p.feedback_stiffness = self.feedbackStiffness
+        p.enable_post_stabilization = 0
+        p.enable_compliance = 1 if self.compliance != 0.0 else 0
+        p.compliance = self.compliance
+        p.restitution_override = int(restitutionOverride)
+        p.restitution = restitutionOverrideVal
+        p.friction_override = int(frictionOverride)
+        p.friction = frictionOverrideVal
+        ctx.set_params()
+
+    def solve(self, dt: float, restitutionOverride=False, restitutionOverrideVal=0.5, frictionOverride=False,
+              frictionOverrideVal=0.1):
+        """PGS.solve (PGS.java:73-194) over the context's current contacts."""
+        ctx = self.ctx
+        self._apply(ctx, restitutionOverride, restitutionOverrideVal, frictionOverride, frictionOverrideVal)
+        if self.computeInCollection:
+            return ctx.single_sweep(dt)
+        it, _ = ctx.solve(dt, self.postStabilization)
+        self.changes = list(ctx.changes())
+        return it
+
+
+class CollisionProcessor:
+    """m3d/CollisionProcessor.java: detection, warm start, solve drivers and
+    the parameters of :1058-1083 (same names, same defaults)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.solver = PGS(ctx=ctx)
+        self.collisionDetectTime = 0.0
+        self.collisionSolveTime = 0.0
+        self.collectionUpdateTime = 0.0
+        self.contactOrderingTime = 0.0
+        self.singleItPGSTime = 0.0
+        self.badWarmStarts = 0
+        self.badWarmStartsRepaired = 0
+        # parameters (CollisionProcessor.java:1058-1083); the Java field
+        # `warmStart` shares its name with the method, so it is useWarmStart here
+        self.useWarmStart = True
+        self.shuffle = False
+        self.feedbackStiffness = 0.5
+        self.enablePostStabilization = False
+        self.enableCompliance = True
+        self.compliance = 1e-3
+        self.restitutionOverride = False
+        self.restitution = 0.5
+        self.frictionOverride = False
+        self.friction = 0.1
+        self.iterations = 30
+        self.tolerance = 1e-5
+        self.omega = 1.0
+        self.iterationsInCollection = 1
+        self._n = 0
+
+    @property
+    def contacts(self) -> list[dict]:
+        """The current contact list (CollisionProcessor.contacts, :45)."""
+        c = self.ctx.contacts()
+        keys = list(c.keys())
+        return [{k: c[k][i] for k in keys} for i in range(len(c["body1"]))]
+
+    def collisionDetection(self, dt: float):
+        """collisionDetection (:91-102)."""
+        t0 = time.perf_counter()
+        self._n = self.ctx.detect(dt)
+        self.collisionDetectTime = time.perf_counter() - t0
+
+    def updateContactsMap(self):
+        """updateContactsMap (:443-448)."""
+        self.ctx.update_contacts_map()
+
+    def updateBodyPairContacts(self):
+        """updateBodyPairContacts (:145-164): body-pair groups are rebuilt on
+        the device with every detection; nothing to do on the host."""
+
+    def warmStart(self, postStabilization: bool = False):
+        """warmStart (:481-640)."""
+        self.badWarmStarts, self.badWarmStartsRepaired = self.ctx.warm_start(postStabilization)
+
+    def redoWarmStart(self):
+        """redoWarmStart (:453-459)."""
+        self.ctx.redo_warm_start()
+
+    def noWarmStart(self):
+        """noWarmStart (:464-470)."""
+        self.ctx.no_warm_start()
+
+    def solveLCP(self, dt: float, postStabilization: bool = False):
+        """solveLCP (:108-137): configures the solver as the reference does."""
+        if self._n == 0:
+            self.collisionSolveTime = 0.0
+            return 0
+        self.solver.init(self.iterations)
+        self.solver.tolerance = self.tolerance
+        self.solver.omega = self.omega
+        self.solver.feedbackStiffness = (self.feedbackStiffness if (not self.enablePostStabilization or
+                                                                    postStabilization) else 0.0)
+        self.solver.postStabilization = postStabilization
+        self.solver.compliance = self.compliance if self.enableCompliance else 0.0
+        self.solver.warmStart = True
+        t0 = time.perf_counter()
+        it = self.solver.solve(dt, self.restitutionOverride, self.restitution, self.frictionOverride,
+                               self.friction)
+        self.collisionSolveTime = time.perf_counter() - t0
+        return it
+
+    def clearBodyPairContacts(self):
+        """clearBodyPairContacts (:213-226): only merge bookkeeping uses it."""
+
+
+class RigidBodySystem:
+    """m3d/RigidBodySystem.java: the per-step phase order of advanceTime
+    (:102-185) with merging and sleeping disabled."""
+
+    def __init__(self, scene, device: int = 0):
+        self.scene = scene
+        self.ctx = Context(device)
+        self.ctx.upload(scene)
+        self.collision = CollisionProcessor(self.ctx)
+        self.useGravity = True
+        self.gravityAmount = 1.0
+        self.gravityAngle = 90.0
+        self.totalSteps = 0
+        self.simulationTime = 0.0
+        self.computeTime = 0.0
+        self._initial = scene.arrays()[2:]
+
+    @property
+    def bodies(self):
+        return self.scene.bodies
+
+    def applyExternalForces(self, dt: float):
+        """applyExternalForces (:234-257): gravity (:276-290)."""
+        p = self.ctx.params
+        p.use_gravity = int(self.useGravity)
+        p.gravity_amount = self.gravityAmount
+        p.gravity_angle_deg = self.gravityAngle
+        self.ctx.set_params()
+        self.ctx.apply_gravity()
+
+    def advanceTime(self, dt: float):
+        """advanceTime (:102-185)."""
+        t0 = time.perf_counter()
+        self.totalSteps += 1
+        self.ctx.clear_bodies()
+        self.applyExternalForces(dt)
+        self.collision.updateContactsMap()
+        self.collision.collisionDetection(dt)
+        self.collision.updateBodyPairContacts()
+        self.collision.warmStart(False)
+        if self.collision.useWarmStart:
+            self.collision.redoWarmStart()
+        else:
+            self.collision.noWarmStart()
+        self.collision.solveLCP(dt, False)
+        self.collision.clearBodyPairContacts()
+        self.ctx.advance_velocities(dt)
+        self.ctx.advance_positions(dt)
+        self.computeTime = time.perf_counter() - t0
+        self.simulationTime += dt
+
+    def reset(self):
+        """reset (:390-426): initial poses and velocities."""
+        x, R, v, w = self._initial
+        self.ctx.push_state(x, R, v, w)
+        self.ctx.update_contacts_map()
+        self.totalSteps = 0
+        self.simulationTime = 0.0
+
+    def state(self):
+        x, R, v, w = self.ctx.pull_state()
+        return {"x": x, "R": R.reshape(-1, 3, 3), "v": v, "omega": w}
+
+    def kinetic_energy(self) -> float:
+        b, _, _, _, _, _ = self.scene.arrays()
+        x, R, v, w = self.ctx.pull_state()
+        return float(0.5 * np.sum(b["mass"] * np.sum(v * v, axis=1)))

@@ -140,6 +140,11 @@ typedef struct am_params {
     /* implementation knobs (no reference counterpart) */
     int32_t pgs_block;                /* threads per workgroup in the PGS kernel (0 = auto) */
     int32_t pgs_grid;                 /* workgroups of the PGS kernel (0 = auto) */
+    /* learned warm start (no reference counterpart): 0 plain previous-frame
+     * start; 1 the MLP's predicted start, falling back to the plain start
+     * whenever its start residual is not smaller */
+    int32_t learned_warm_start;
+    int32_t collect_training;         /* 1: each main solve appends (features, converged lambda) */
 } am_params;
 
 /* Caller-owned destination arrays for am_contacts_pull; any may be NULL.
@@ -185,6 +190,11 @@ typedef struct am_step_stats {
     double ms_total;
     double ms_pgs_kernel;     /* duration of the persistent PGS kernel alone */
     double ms_setup_kernel;   /* b, D and record packing in sweep order */
+    double ms_learned;        /* features + MLP inference + start selection */
+    double start_residual_plain;   /* start residual of the plain warm start */
+    double start_residual_learned; /* start residual of the predicted start (0 if unused) */
+    int32_t used_learned;     /* 1 if the solve started from the predicted lambdas */
+    int32_t reserved2;
 } am_step_stats;
 
 typedef struct am_ctx am_ctx;
@@ -268,6 +278,28 @@ int am_advance_positions(am_ctx* ctx, double dt);
 /* One advanceTime-equivalent step without merging: clear, gravity, contacts
  * map, detection, warm start, solve, velocity and position update. */
 int am_step(am_ctx* ctx, double dt, am_step_stats* stats);
+
+/* ---- learned warm start (no reference counterpart) --------------------
+ * A per-contact MLP 24 -> 64 -> 64 -> 64 -> 3 (ReLU), BF16 weights on the
+ * matrix cores with FP32 accumulation, predicts lambda / S from 24 contact
+ * features; parameters are a flat FP32 vector W1 (24x64, row-major [in][out]),
+ * b1 (64), W2 (64x64), b2, W3 (64x64), b3, W4 (64x3), b4 (3). */
+#define AM_MLP_PARAMS 10115
+#define AM_MLP_FEATURES 24
+int am_mlp_init(am_ctx* ctx, uint64_t seed);
+int am_mlp_set_params(am_ctx* ctx, const float* params, int32_t n);
+int am_mlp_get_params(am_ctx* ctx, float* params, int32_t n);
+int am_mlp_forward(am_ctx* ctx, const float* x, int32_t n, float* y);
+/* d(mean squared error)/d(params) over n rows, same flat layout */
+int am_mlp_gradients(am_ctx* ctx, const float* x, const float* t, int32_t n, float* grads, float* loss);
+/* features and lambda scale S of the current contacts (list order) */
+int am_features_pull(am_ctx* ctx, float* x, double* scale, int32_t capacity, int32_t* n);
+int am_train_size(am_ctx* ctx, int32_t* n);
+int am_train_clear(am_ctx* ctx);
+/* Adam over the collected (features, lambda / S) pairs */
+int am_train_epochs(am_ctx* ctx, int32_t epochs, int32_t batch, float lr, uint64_t seed, float* last_loss);
+/* lambda each contact started the last main solve from (list order) */
+int am_start_lambdas_pull(am_ctx* ctx, double* lambda, int32_t capacity, int32_t* n);
 
 /* ---- relative motion (MotionMetricProcessor.getLargestVelocityNorm) --- */
 /* For each pair (a,b) of entity ids: the largest relative velocity over both

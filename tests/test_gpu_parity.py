@@ -126,7 +126,7 @@ def test_motion_metric(gpu_lib):
     assert_bits_equal("metric", g, o)
 
 
-@pytest.mark.parametrize("grid,block", [(2, 64), (8, 64), (32, 128)])
+@pytest.mark.parametrize("grid,block", [(2, 64), (8, 64), (32, 128), (64, 64), (128, 256)])
 def test_pgs_multiblock_bit_exact(gpu_lib, grid, block):
     """The cooperative (grid-barrier) path of the sweep kernel."""
     scene = workloads.box_cluster(seed=9, n=729, spheres=60)

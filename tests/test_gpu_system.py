@@ -1,0 +1,34 @@
+"""The Python mirror of the Java surface (RigidBodySystem / CollisionProcessor
+/ PGS) drives libam and reproduces the oracle's step."""
+import numpy as np
+import pytest
+
+from adaptivemerging_amd import workloads
+from adaptivemerging_amd.system import RigidBodySystem
+from oracle.pyoracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def test_system_steps_match_oracle_and_tower_stands(gpu_lib):
+    scene = workloads.jenga_tower(seed=0, layers=6)
+    system = RigidBodySystem(scene)
+    system.gravityAmount = 9.81
+    system.collision.iterations = 60
+    system.collision.frictionOverride = True
+    system.collision.friction = 0.6
+    orc = Oracle(scene)
+    orc.set_params(gravity_amount=9.81, iterations=60, friction_override=1, friction=0.6)
+    for step in range(20):
+        system.advanceTime(0.01)
+        orc.clear_bodies(); orc.apply_gravity(); orc.update_contacts_map(); orc.detect()
+        orc.warm_start(); orc.redo_warm_start()
+        order = system.ctx.solve_order()
+        orc.solve(order, 0.01)
+        orc.advance_velocities(0.01); orc.advance_positions(0.01)
+    st = system.state()
+    xo, _, _, _ = orc.pull_state()
+    np.testing.assert_allclose(st["x"], xo, atol=1e-9)
+    assert len(system.collision.contacts) > 0
+    # the tower is standing: top block within 1 cm of its start height
+    assert abs(st["x"][-1][1] - scene.x[-1][1]) < 0.01
