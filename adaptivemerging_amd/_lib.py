@@ -44,10 +44,11 @@ BODY_DTYPE = np.dtype(
         ("size", "<f8", 3), ("radius", "<f8"), ("mass", "<f8"), ("minv", "<f8"),
         ("jinv0", "<f8", 9), ("friction", "<f8"), ("restitution", "<f8"),
         ("plane_n", "<f8", 3), ("plane_p", "<f8", 3), ("plane_d", "<f8"), ("bbox", "<f8", 24),
+        ("inertia0", "<f8", 9),
     ]
 )
 NODE_DTYPE = np.dtype([("c", "<f8", 3), ("r", "<f8"), ("first_child", "<i4"), ("n_children", "<i4")])
-assert BODY_DTYPE.itemsize == 408 and NODE_DTYPE.itemsize == 40
+assert BODY_DTYPE.itemsize == 480 and NODE_DTYPE.itemsize == 40
 
 
 class Params(C.Structure):
@@ -96,22 +97,35 @@ class StepStats(C.Structure):
         ("ms_solve", C.c_double), ("ms_integrate", C.c_double), ("ms_total", C.c_double),
         ("ms_pgs_kernel", C.c_double), ("ms_setup_kernel", C.c_double), ("ms_learned", C.c_double),
         ("start_residual_plain", C.c_double), ("start_residual_learned", C.c_double),
-        ("used_learned", C.c_int32), ("reserved2", C.c_int32),
+        ("used_learned", C.c_int32), ("n_collections", C.c_int32), ("n_merged", C.c_int32),
+        ("n_unmerged", C.c_int32), ("n_internal_contacts", C.c_int32), ("reserved2", C.c_int32),
+        ("ms_merging", C.c_double),
     ]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
 
 
+class BodyPair(C.Structure):
+    _fields_ = [
+        ("body1", C.c_int32), ("body2", C.c_int32), ("in_collection", C.c_int32), ("n_contacts", C.c_int32),
+        ("hist_n", C.c_int32), ("state_n", C.c_int32), ("flag", C.c_int32), ("collection", C.c_int32),
+        ("hist", C.c_double * 8), ("states", C.c_int32 * 8),
+    ]
+
+
 # Exported symbols of include/am.h; the test suite checks this list against
 # the header and the shared object.
 SYMBOLS = [
     "am_version", "am_ctx_create", "am_ctx_destroy", "am_last_error", "am_params_default", "am_params_set",
-    "am_scene_upload", "am_collections_set", "am_entity_mass_set", "am_jinv_pull", "am_state_push",
+    "am_scene_upload", "am_structure_pull", "am_entities_pull", "am_jinv_pull", "am_state_push",
     "am_state_pull", "am_forces_push", "am_forces_pull", "am_clear_bodies", "am_apply_gravity", "am_deltav_pull",
     "am_update_contacts_map", "am_detect", "am_contacts_push", "am_contacts_pull", "am_lambdas_push",
     "am_warm_start", "am_redo_warm_start", "am_no_warm_start", "am_solve", "am_solve_order_pull",
-    "am_colours_pull", "am_single_sweep", "am_changes_pull", "am_advance_velocities", "am_advance_positions",
+    "am_colours_pull", "am_update_in_collections", "am_changes_pull", "am_advance_velocities",
+    "am_advance_positions", "am_accumulate_for_unmerging", "am_unmerge", "am_clear_top_level",
+    "am_clear_body_pair_contacts", "am_accumulate_for_merging", "am_merge", "am_body_pairs_pull",
+    "am_internal_contacts_pull",
     "am_step", "am_motion_metric", "am_mlp_init", "am_mlp_set_params", "am_mlp_get_params", "am_mlp_forward",
     "am_mlp_gradients", "am_features_pull", "am_train_size", "am_train_clear", "am_train_epochs",
     "am_start_lambdas_pull",
@@ -147,8 +161,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "am_params_default": (C.c_int, [C.POINTER(Params)]),
         "am_params_set": (C.c_int, [ctxp, C.POINTER(Params)]),
         "am_scene_upload": (C.c_int, [ctxp, vp, i32, vp, i32]),
-        "am_collections_set": (C.c_int, [ctxp, vp, vp, i32, vp, vp, vp, i32]),
-        "am_entity_mass_set": (C.c_int, [ctxp, i32, dbl, dbl, vp, i32]),
+        "am_structure_pull": (C.c_int, [ctxp, vp, i32, vp, i32, C.POINTER(i32), C.POINTER(i32)]),
+        "am_entities_pull": (C.c_int, [ctxp, vp, i32]),
         "am_jinv_pull": (C.c_int, [ctxp, vp, i32]),
         "am_state_push": (C.c_int, [ctxp, vp, vp, vp, vp, i32]),
         "am_state_pull": (C.c_int, [ctxp, vp, vp, vp, vp, i32]),
@@ -168,7 +182,15 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "am_solve": (C.c_int, [ctxp, dbl, i32, C.POINTER(i32), C.POINTER(dbl)]),
         "am_solve_order_pull": (C.c_int, [ctxp, vp, i32, C.POINTER(i32)]),
         "am_colours_pull": (C.c_int, [ctxp, vp, i32, C.POINTER(i32)]),
-        "am_single_sweep": (C.c_int, [ctxp, dbl, C.POINTER(i32)]),
+        "am_update_in_collections": (C.c_int, [ctxp, dbl, C.POINTER(i32)]),
+        "am_accumulate_for_unmerging": (C.c_int, [ctxp]),
+        "am_unmerge": (C.c_int, [ctxp, dbl, C.POINTER(i32)]),
+        "am_clear_top_level": (C.c_int, [ctxp]),
+        "am_clear_body_pair_contacts": (C.c_int, [ctxp]),
+        "am_accumulate_for_merging": (C.c_int, [ctxp]),
+        "am_merge": (C.c_int, [ctxp, C.POINTER(i32)]),
+        "am_body_pairs_pull": (C.c_int, [ctxp, vp, i32, C.POINTER(i32), C.POINTER(i32)]),
+        "am_internal_contacts_pull": (C.c_int, [ctxp, C.POINTER(ContactArrays), i32, C.POINTER(i32)]),
         "am_changes_pull": (C.c_int, [ctxp, vp, i32, C.POINTER(i32)]),
         "am_advance_velocities": (C.c_int, [ctxp, dbl]),
         "am_advance_positions": (C.c_int, [ctxp, dbl]),

@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libam.so")
-SOURCES = ["am_api.hip", "am_detect.hip", "am_solve.hip", "am_scan.hip", "am_mlp.hip"]
+SOURCES = ["am_api.hip", "am_detect.hip", "am_solve.hip", "am_scan.hip", "am_mlp.hip", "am_merge.hip"]
 ARCH = os.environ.get("AM_OFFLOAD_ARCH", "gfx950")
 
 

@@ -6,7 +6,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import AMError, ContactArrays, Params, StepStats, dptr, iptr, ptr
+from ._lib import MLP_PARAMS, AMError, ContactArrays, Params, StepStats, dptr, iptr, ptr
 
 
 class Context:
@@ -21,6 +21,7 @@ class Context:
         self.h = h
         self.params = _lib.default_params()
         self.n = 0
+        self.merging_event = False  # Merging.mergingEvent as seen through merge()/unmerge()
 
     def close(self):
         if self.h:
@@ -53,43 +54,63 @@ class Context:
         self._check(self.lib.am_scene_upload(self.h, ptr(b), len(b), ptr(nd), len(nd)))
         self.push_state(x, R, v, w)
 
-    def set_collections(self, parent, top, member_first, member_count, members):
-        parent = np.ascontiguousarray(parent, dtype=np.int32)
-        top = np.ascontiguousarray(top, dtype=np.int32)
-        mf = np.ascontiguousarray(member_first, dtype=np.int32)
-        mc = np.ascontiguousarray(member_count, dtype=np.int32)
-        mem = np.ascontiguousarray(members, dtype=np.int32)
-        self._check(self.lib.am_collections_set(self.h, ptr(parent), ptr(top), len(top), ptr(mf), ptr(mc),
-                                                ptr(mem) if len(mem) else None, len(mem)))
+    def structure(self):
+        """(parent per entity, top-level list) of the merged-body structure."""
+        ne, nt = C.c_int32(), C.c_int32()
+        self._check(self.lib.am_structure_pull(self.h, None, 0, None, 0, C.byref(ne), C.byref(nt)))
+        parent = np.zeros(ne.value, np.int32)
+        top = np.zeros(nt.value, np.int32)
+        self._check(self.lib.am_structure_pull(self.h, ptr(parent), len(parent), ptr(top), len(top), C.byref(ne),
+                                               C.byref(nt)))
+        return parent, top
+
+    def entities(self) -> np.ndarray:
+        """am_body_desc of every entity (collections included)."""
+        parent, _ = self.structure()
+        out = np.zeros(len(parent), dtype=_lib.BODY_DTYPE)
+        self._check(self.lib.am_entities_pull(self.h, ptr(out), len(out)))
+        return out
 
     # -- state ----------------------------------------------------------
     def push_state(self, x=None, R=None, v=None, w=None):
         arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (x, R, v, w)]
         self._check(self.lib.am_state_push(self.h, *[ptr(a) for a in arrs], self.n))
 
-    def pull_state(self):
-        x = np.zeros((self.n, 3)); R = np.zeros((self.n, 9)); v = np.zeros((self.n, 3)); w = np.zeros((self.n, 3))
-        self._check(self.lib.am_state_pull(self.h, ptr(x), ptr(R), ptr(v), ptr(w), self.n))
+    def pull_state(self, n: int | None = None):
+        """Pose and velocity of the first n entities (default: the physical bodies)."""
+        n = self.n if n is None else n
+        x = np.zeros((n, 3)); R = np.zeros((n, 9)); v = np.zeros((n, 3)); w = np.zeros((n, 3))
+        self._check(self.lib.am_state_pull(self.h, ptr(x), ptr(R), ptr(v), ptr(w), n))
         return x, R, v, w
 
-    def pull_jinv(self):
-        j = np.zeros((self.n, 9))
-        self._check(self.lib.am_jinv_pull(self.h, ptr(j), self.n))
+    def n_entities(self) -> int:
+        ne, nt = C.c_int32(), C.c_int32()
+        self._check(self.lib.am_structure_pull(self.h, None, 0, None, 0, C.byref(ne), C.byref(nt)))
+        return ne.value
+
+    def pull_jinv(self, n: int | None = None):
+        """World inverse inertia of the first n entities (default: all)."""
+        n = self.n_entities() if n is None else n
+        j = np.zeros((n, 9))
+        self._check(self.lib.am_jinv_pull(self.h, ptr(j), n))
         return j
 
     def push_forces(self, force=None, torque=None):
         f = None if force is None else np.ascontiguousarray(force, dtype=np.float64)
         t = None if torque is None else np.ascontiguousarray(torque, dtype=np.float64)
-        self._check(self.lib.am_forces_push(self.h, ptr(f), ptr(t), self.n))
+        n = len(f) if f is not None else (len(t) if t is not None else 0)
+        self._check(self.lib.am_forces_push(self.h, ptr(f), ptr(t), n))
 
-    def pull_forces(self):
-        f = np.zeros((self.n, 3)); t = np.zeros((self.n, 3))
-        self._check(self.lib.am_forces_pull(self.h, ptr(f), ptr(t), self.n))
+    def pull_forces(self, n: int | None = None):
+        n = self.n_entities() if n is None else n
+        f = np.zeros((n, 3)); t = np.zeros((n, 3))
+        self._check(self.lib.am_forces_pull(self.h, ptr(f), ptr(t), n))
         return f, t
 
-    def pull_deltav(self):
-        d = np.zeros((self.n, 6))
-        self._check(self.lib.am_deltav_pull(self.h, ptr(d), self.n))
+    def pull_deltav(self, n: int | None = None):
+        n = self.n_entities() if n is None else n
+        d = np.zeros((n, 6))
+        self._check(self.lib.am_deltav_pull(self.h, ptr(d), n))
         return d
 
     def clear_bodies(self):
@@ -126,15 +147,7 @@ class Context:
 
     def contacts(self) -> dict:
         n = self.n_contacts()
-        out = {
-            "body1": np.zeros(n, np.int32), "body2": np.zeros(n, np.int32), "bv1": np.zeros(n, np.int32),
-            "bv2": np.zeros(n, np.int32), "info": np.zeros(n, np.int32),
-            "point_b1": np.zeros((n, 3)), "normal_b1": np.zeros((n, 3)), "tangent1_b1": np.zeros((n, 3)),
-            "tangent2_b1": np.zeros((n, 3)), "violation": np.zeros(n), "prev_violation": np.zeros(n),
-            "jacobian": np.zeros((n, 36)), "lambda": np.zeros((n, 3)), "lambda_warm": np.zeros((n, 3)),
-            "b": np.zeros((n, 3)), "diag": np.zeros((n, 3)), "state": np.zeros(n, np.int32),
-            "new_this_step": np.zeros(n, np.int32),
-        }
+        out = _empty_contacts(n)
         ca = contact_arrays(out)
         m = C.c_int32()
         self._check(self.lib.am_contacts_pull(self.h, C.byref(ca), n, C.byref(m)))
@@ -161,10 +174,59 @@ class Context:
         self._check(self.lib.am_solve(self.h, dt, int(post), C.byref(it), C.byref(last)))
         return it.value, last.value
 
-    def single_sweep(self, dt: float) -> int:
+    def update_in_collections(self, dt: float) -> int:
         it = C.c_int32()
-        self._check(self.lib.am_single_sweep(self.h, dt, C.byref(it)))
+        self._check(self.lib.am_update_in_collections(self.h, dt, C.byref(it)))
         return it.value
+
+    # -- merging --------------------------------------------------------
+    def accumulate_for_unmerging(self):
+        self._check(self.lib.am_accumulate_for_unmerging(self.h))
+
+    def unmerge(self, dt: float) -> int:
+        n = C.c_int32()
+        self._check(self.lib.am_unmerge(self.h, dt, C.byref(n)))
+        self.merging_event = self.merging_event or n.value > 0
+        return n.value
+
+    def clear_top_level(self):
+        self._check(self.lib.am_clear_top_level(self.h))
+
+    def clear_body_pair_contacts(self):
+        self._check(self.lib.am_clear_body_pair_contacts(self.h))
+
+    def accumulate_for_merging(self):
+        self._check(self.lib.am_accumulate_for_merging(self.h))
+
+    def merge(self) -> int:
+        n = C.c_int32()
+        self._check(self.lib.am_merge(self.h, C.byref(n)))
+        self.merging_event = self.merging_event or n.value > 0
+        return n.value
+
+    def body_pairs(self):
+        """(external, internal) body pairs as lists of dicts."""
+        ne, ni = C.c_int32(), C.c_int32()
+        self._check(self.lib.am_body_pairs_pull(self.h, None, 0, C.byref(ne), C.byref(ni)))
+        arr = (_lib.BodyPair * max(ne.value + ni.value, 1))()
+        self._check(self.lib.am_body_pairs_pull(self.h, C.cast(arr, C.c_void_p), ne.value + ni.value, C.byref(ne),
+                                                C.byref(ni)))
+        out = []
+        for k in range(ne.value + ni.value):
+            b = arr[k]
+            out.append(dict(body1=b.body1, body2=b.body2, in_collection=b.in_collection, n_contacts=b.n_contacts,
+                            hist_n=b.hist_n, state_n=b.state_n, flag=b.flag, collection=b.collection,
+                            hist=list(b.hist)[:max(b.hist_n, 0)] if not b.in_collection else [],
+                            states=list(b.states)[:max(b.state_n, 0)]))
+        return out[:ne.value], out[ne.value:]
+
+    def internal_contacts(self) -> dict:
+        n = C.c_int32()
+        self._check(self.lib.am_internal_contacts_pull(self.h, None, 0, C.byref(n)))
+        out = _empty_contacts(n.value)
+        ca = contact_arrays(out)
+        self._check(self.lib.am_internal_contacts_pull(self.h, C.byref(ca), n.value, C.byref(n)))
+        return out
 
     def solve_order(self) -> np.ndarray:
         n = C.c_int32()
@@ -204,6 +266,45 @@ class Context:
         out = np.zeros(len(p))
         self._check(self.lib.am_motion_metric(self.h, ptr(p), len(p), ptr(out)))
         return out
+
+    # -- learned warm start -------------------------------------------------
+    def mlp_init(self, seed: int = 0):
+        self._check(self.lib.am_mlp_init(self.h, C.c_uint64(seed)))
+
+    def mlp_params(self) -> np.ndarray:
+        p = np.zeros(MLP_PARAMS, dtype=np.float32)
+        self._check(self.lib.am_mlp_get_params(self.h, ptr(p), len(p)))
+        return p
+
+    def set_mlp_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._check(self.lib.am_mlp_set_params(self.h, ptr(p), len(p)))
+
+    def train_size(self) -> int:
+        n = C.c_int32(0)
+        self._check(self.lib.am_train_size(self.h, C.byref(n)))
+        return n.value
+
+    def train_clear(self):
+        self._check(self.lib.am_train_clear(self.h))
+
+    def train_epochs(self, epochs: int, batch: int = 256, lr: float = 1e-3, seed: int = 0) -> float:
+        loss = C.c_float(0)
+        self._check(self.lib.am_train_epochs(self.h, epochs, batch, C.c_float(lr), C.c_uint64(seed),
+                                             C.byref(loss)))
+        return loss.value
+
+
+def _empty_contacts(n: int) -> dict:
+    return {
+        "body1": np.zeros(n, np.int32), "body2": np.zeros(n, np.int32), "bv1": np.zeros(n, np.int32),
+        "bv2": np.zeros(n, np.int32), "info": np.zeros(n, np.int32),
+        "point_b1": np.zeros((n, 3)), "normal_b1": np.zeros((n, 3)), "tangent1_b1": np.zeros((n, 3)),
+        "tangent2_b1": np.zeros((n, 3)), "violation": np.zeros(n), "prev_violation": np.zeros(n),
+        "jacobian": np.zeros((n, 36)), "lambda": np.zeros((n, 3)), "lambda_warm": np.zeros((n, 3)),
+        "b": np.zeros((n, 3)), "diag": np.zeros((n, 3)), "state": np.zeros(n, np.int32),
+        "new_this_step": np.zeros(n, np.int32),
+    }
 
 
 def contact_arrays(out: dict) -> ContactArrays:

@@ -8,130 +8,17 @@
 #include <string>
 #include <vector>
 
-#include "am_mlp.hpp"
-#include "am_solve.hpp"
-
-namespace {
+#include "am_ctx.hpp"
 
 using namespace am;
+using namespace am_rt;
 
-template <class T>
-struct DBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    DBuf() = default;
-    DBuf(const DBuf&) = delete;
-    DBuf& operator=(const DBuf&) = delete;
-    ~DBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    // Grows (never shrinks); contents are not preserved.
-    hipError_t ensure(size_t want) {
-        if (want <= n && p) return hipSuccess;
-        release();
-        size_t cap = std::max<size_t>(want, 1);
-        hipError_t e = hipMalloc((void**)&p, cap * sizeof(T));
-        if (e == hipSuccess) n = cap;
-        else p = nullptr;
-        return e;
-    }
-};
-
-}  // namespace
-
-struct am_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    am_params params{};
-
-    // scene (host mirrors)
-    int n_ent = 0, n_phys = 0, n_nodes = 0;
-    std::vector<am_body_desc> desc;
-    std::vector<am_sphere_node> nodes;
-    std::vector<int> parent, slot, sub, slot_entity, member_first, member_count, members, flags;
-    std::vector<int> planes, grid_bodies, node_body, top_pinned, top_sleeping, gravity_targets, integrate_targets;
-    double cell_size = 1.0;
-    int hash_mask = 0;
-
-    // body buffers
-    DBuf<am_body_desc> d_desc;
-    DBuf<am_sphere_node> d_nodes;
-    DBuf<double> d_cw, d_x, d_R, d_v, d_w, d_f, d_t, d_jinv, d_dv, d_minv;
-    DBuf<int> d_flags, d_parent, d_slot, d_sub, d_slot_entity, d_member_first, d_members, d_planes, d_grid,
-        d_node_body, d_top_pinned, d_top_sleeping, d_gravity, d_integrate;
-
-    // broad phase
-    DBuf<double> d_sphere;
-    DBuf<int> d_cell, d_bucket_count, d_bucket_start, d_bucket_cursor, d_bucket_items, d_pair_counts, d_pair_offsets;
-    DBuf<uint64_t> d_keys_unsorted, d_keys;
-    DBuf<unsigned char> d_sort_temp, d_scan_temp;
-    size_t sort_temp_size = 0;
-
-    // contacts
-    int cap_c = 0;
-    DBuf<int> c_b1, c_b2, c_bv1, c_bv2, c_info, c_state, c_new;
-    DBuf<double> c_pB1, c_nB1, c_t1B1, c_t2B1, c_viol, c_prev, c_J, c_lam, c_lamw, c_b, c_D;
-    DBuf<int> d_narrow_counts, d_narrow_offsets;
-
-    // old contacts (warm start)
-    int n_old = 0, table_mask = 0;
-    DBuf<int> o_b1, o_b2, o_bv1, o_bv2, o_info, o_table;
-    DBuf<double> o_pB1, o_lam, o_viol, o_prev;
-
-    // groups, colouring, solve layout
-    DBuf<int> g_flags, g_scanned, g_start, g_a, g_b, g_colour_a, g_colour_b, inc_count, inc_start, inc_cursor, inc_items,
-        g_sizes, g_pos_start, s_order, s_colour_start;
-    DBuf<uint64_t> g_keys, g_keys_sorted;
-    DBuf<double> s_rec, s_lam;
-    DBuf<int> s_meta, s_state, d_barrier, s_group_desc;
-    DBuf<unsigned long long> s_iter_max;
-
-    // scalars on device: 0 n_pairs, 1 n_contacts, 2 n_groups, 3 total scratch, 4 overflow, 5 remaining,
-    // 6 iters_used, 7 bad, 8 repaired, 9 n_sweep
-    DBuf<int> d_scalars;
-    int* h_scalars = nullptr;  // pinned mirror
-
-    int n_pairs = 0, n_contacts = 0, n_groups = 0, n_colours = 0, max_colour_groups = 0;
-    bool layout_valid = false;
-    int last_iters = 0;
-    double last_change = 0.0;
-    std::vector<double> changes;
-    int pgs_max_grid = 0;
-    hipEvent_t kev[3] = {nullptr, nullptr, nullptr};  // setup start, pgs start, pgs end
-    double ms_pgs_kernel = 0.0, ms_setup_kernel = 0.0;
-
-    // learned warm start
-    bool mlp_ready = false;
-    DBuf<float> m_p, m_g, m_m, m_v;  // padded FP32 parameters, gradients, Adam moments
-    DBuf<unsigned short> m_bf;       // padded BF16 weights
-    DBuf<float> f_X, f_Y, d_loss;
-    DBuf<double> f_scale, f_lam, s_start, dv_save;
-    DBuf<unsigned long long> d_resid;
-    DBuf<float> t_X, t_T;
-    DBuf<int> t_idx;
-    int t_n = 0, adam_step = 0;
-    double ms_learned = 0.0, res_plain = 0.0, res_learned = 0.0;
-    int used_learned = 0;
-    hipEvent_t lev[2] = {nullptr, nullptr};
-};
-
-namespace {
+namespace am_rt {
 
 int fail(am_ctx* c, int code, const std::string& m) {
     if (c) c->err = m;
     return code;
 }
-
-#define AM_HIP(ctx, expr)                                                                 \
-    do {                                                                                  \
-        hipError_t e_ = (expr);                                                           \
-        if (e_ != hipSuccess)                                                             \
-            return fail(ctx, AM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
 
 BodiesView bodies_view(am_ctx* c) {
     BodiesView B;
@@ -177,6 +64,7 @@ ContactsView contacts_view(am_ctx* c) {
     C.is_new = c->c_new.p;
     return C;
 }
+ContactsView internal_view(am_ctx* c) { return c->ic.view(); }
 OldContactsView old_view(am_ctx* c) {
     OldContactsView O;
     O.n = c->n_old;
@@ -194,13 +82,6 @@ OldContactsView old_view(am_ctx* c) {
     return O;
 }
 
-template <class T>
-hipError_t upload(am_ctx* c, DBuf<T>& b, const std::vector<T>& v) {
-    hipError_t e = b.ensure(std::max<size_t>(v.size(), 1));
-    if (e != hipSuccess) return e;
-    if (!v.empty()) return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream);
-    return hipSuccess;
-}
 
 // Reads n scalars from d_scalars[first..] (synchronises the stream).
 int read_scalars(am_ctx* c, int first, int n) {
@@ -216,27 +97,35 @@ int ensure_scan_temp(am_ctx* c, int n) {
     return AM_OK;
 }
 
-int ensure_contact_capacity(am_ctx* c, int want) {
+// Capacity for `want` contacts; the first `keep` contacts survive a regrow.
+int ensure_contact_capacity(am_ctx* c, int want, int keep) {
     if (want <= c->cap_c && c->c_b1.p) return AM_OK;
     int cap = std::max(want + want / 4, 1024);
-    AM_HIP(c, c->c_b1.ensure(cap));
-    AM_HIP(c, c->c_b2.ensure(cap));
-    AM_HIP(c, c->c_bv1.ensure(cap));
-    AM_HIP(c, c->c_bv2.ensure(cap));
-    AM_HIP(c, c->c_info.ensure(cap));
-    AM_HIP(c, c->c_state.ensure(cap));
-    AM_HIP(c, c->c_new.ensure(cap));
-    AM_HIP(c, c->c_pB1.ensure((size_t)cap * 3));
-    AM_HIP(c, c->c_nB1.ensure((size_t)cap * 3));
-    AM_HIP(c, c->c_t1B1.ensure((size_t)cap * 3));
-    AM_HIP(c, c->c_t2B1.ensure((size_t)cap * 3));
-    AM_HIP(c, c->c_viol.ensure(cap));
-    AM_HIP(c, c->c_prev.ensure(cap));
-    AM_HIP(c, c->c_J.ensure((size_t)cap * 36));
-    AM_HIP(c, c->c_lam.ensure((size_t)cap * 3));
-    AM_HIP(c, c->c_lamw.ensure((size_t)cap * 3));
-    AM_HIP(c, c->c_b.ensure((size_t)cap * 3));
-    AM_HIP(c, c->c_D.ensure((size_t)cap * 3));
+    size_t k = (size_t)std::min(keep, c->cap_c);
+    AM_HIP(c, grow_keep(c->stream, c->c_b1, cap, 1, k));
+    AM_HIP(c, grow_keep(c->stream, c->c_b2, cap, 1, k));
+    AM_HIP(c, grow_keep(c->stream, c->c_bv1, cap, 1, k));
+    AM_HIP(c, grow_keep(c->stream, c->c_bv2, cap, 1, k));
+    AM_HIP(c, grow_keep(c->stream, c->c_info, cap, 1, k));
+    AM_HIP(c, grow_keep(c->stream, c->c_state, cap, 1, k));
+    AM_HIP(c, grow_keep(c->stream, c->c_new, cap, 1, k));
+    AM_HIP(c, grow_keep(c->stream, c->c_pB1, cap, 3, k));
+    AM_HIP(c, grow_keep(c->stream, c->c_nB1, cap, 3, k));
+    AM_HIP(c, grow_keep(c->stream, c->c_t1B1, cap, 3, k));
+    AM_HIP(c, grow_keep(c->stream, c->c_t2B1, cap, 3, k));
+    AM_HIP(c, grow_keep(c->stream, c->c_viol, cap, 1, k));
+    AM_HIP(c, grow_keep(c->stream, c->c_prev, cap, 1, k));
+    AM_HIP(c, grow_keep(c->stream, c->c_J, cap, 36, k));
+    AM_HIP(c, grow_keep(c->stream, c->c_lam, cap, 3, k));
+    AM_HIP(c, grow_keep(c->stream, c->c_lamw, cap, 3, k));
+    AM_HIP(c, grow_keep(c->stream, c->c_b, cap, 3, k));
+    AM_HIP(c, grow_keep(c->stream, c->c_D, cap, 3, k));
+    AM_HIP(c, c->bp_flags.ensure(cap));
+    AM_HIP(c, c->bp_scanned.ensure(cap));
+    AM_HIP(c, c->bp_start.ensure(cap + 1));
+    AM_HIP(c, c->bp_keep.ensure(cap + 1));
+    AM_HIP(c, c->bp_keep_scan.ensure(cap + 1));
+    AM_HIP(c, c->d_active.ensure(cap));
     // groups and solve layout are per contact too
     AM_HIP(c, c->g_flags.ensure(cap));
     AM_HIP(c, c->g_scanned.ensure(cap));
@@ -341,6 +230,20 @@ int rebuild_structure(am_ctx* c) {
     AM_HIP(c, upload(c, c->d_gravity, c->gravity_targets));
     AM_HIP(c, upload(c, c->d_integrate, c->integrate_targets));
     AM_HIP(c, upload(c, c->d_flags, c->flags));
+    // collection members: all (push-down at clear) and those the single sweep integrates
+    c->member_list.clear();
+    c->member_adv.clear();
+    for (int e : c->slot_entity)
+        if (c->desc[e].geom == AM_GEOM_COLLECTION && !(c->flags[e] & AM_BODY_SLEEPING))
+            for (int q = 0; q < c->member_count[e]; q++) {
+                int b = c->members[c->member_first[e] + q];
+                c->member_list.push_back(b);
+                if (!(c->flags[b] & AM_BODY_PINNED)) c->member_adv.push_back(b);
+            }
+    AM_HIP(c, upload(c, c->d_member_list, c->member_list));
+    AM_HIP(c, upload(c, c->d_member_adv, c->member_adv));
+    if (c->b2c.size() < (size_t)n * 12) c->b2c.resize((size_t)n * 12, 0.0);
+    AM_HIP(c, upload(c, c->d_b2c, c->b2c));
     // grid
     double rmax = 0.0;
     for (int b : c->grid_bodies) {
@@ -371,7 +274,7 @@ int rebuild_structure(am_ctx* c) {
     return AM_OK;
 }
 
-}  // namespace
+}  // namespace am_rt
 
 extern "C" {
 
@@ -418,6 +321,8 @@ int am_ctx_destroy(am_ctx* c) {
         if (e) (void)hipEventDestroy(e);
     for (auto& e : c->lev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->sev)
+        if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return AM_OK;
@@ -438,7 +343,7 @@ int am_params_set(am_ctx* c, const am_params* p) {
 
 }  // extern "C"
 
-namespace {
+namespace am_rt {
 
 int check_ctx(am_ctx* c) {
     if (!c) return AM_ERR_INVALID;
@@ -541,6 +446,7 @@ int groups_after_contacts(am_ctx* c) {
 }
 
 int do_detect(am_ctx* c) {
+    c->sweep_order_n = -1;
     hipStream_t s = c->stream;
     BodiesView B = bodies_view(c);
     TreeView T = tree_view(c);
@@ -759,6 +665,7 @@ int start_residual(am_ctx* c, PgsArgs G, double* out) {
 }
 
 int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double* last) {
+    if (main_solve) c->sweep_order_n = -1;
     const am_params& P = c->params;
     c->changes.clear();
     if (c->n_contacts == 0) {
@@ -920,7 +827,7 @@ int pull_i(am_ctx* c, int32_t* dst, const int* src, size_t n) {
     return AM_OK;
 }
 
-}  // namespace
+}  // namespace am_rt
 
 extern "C" {
 
@@ -937,9 +844,19 @@ int am_scene_upload(am_ctx* c, const am_body_desc* bodies, int32_t n_bodies, con
         c->n_nodes = n_nodes;
         c->n_phys = 0;
         c->node_body.assign(n_nodes, -1);
+        for (int i = 0; i < n_bodies; i++)
+            if (bodies[i].geom == AM_GEOM_COLLECTION)
+                return fail(c, AM_ERR_INVALID, "collections are created by merging, not uploaded");
+        // merging state starts empty
+        c->total_steps = 0;
+        c->merging_event = false;
+        c->free_entities.clear();
+        c->b2c.assign((size_t)n_bodies * 12, 0.0);
+        c->n_bpc = c->n_bpc_prev = 0;
+        c->ibpc.clear();
+        c->n_ic = 0;
         for (int i = 0; i < n_bodies; i++) {
             const am_body_desc& d = bodies[i];
-            if (d.geom == AM_GEOM_COLLECTION) continue;
             c->n_phys = i + 1;
             if (d.geom == AM_GEOM_TREE) {
                 if (d.tree_first < 0 || d.tree_count <= 0 || d.tree_first + d.tree_count > n_nodes)
@@ -1018,69 +935,31 @@ int am_scene_upload(am_ctx* c, const am_body_desc* bodies, int32_t n_bodies, con
     return AM_OK;
 }
 
-int am_collections_set(am_ctx* c, const int32_t* parent, const int32_t* top, int32_t n_top,
-                       const int32_t* member_first, const int32_t* member_count, const int32_t* members,
-                       int32_t n_members) {
+int am_structure_pull(am_ctx* c, int32_t* parent, int32_t cap_entities, int32_t* top, int32_t cap_top,
+                      int32_t* n_entities, int32_t* n_top) {
     int r = check_ctx(c);
     if (r) return r;
-    if (!parent || !top || n_top <= 0 || n_top > c->n_ent || n_members < 0)
-        return fail(c, AM_ERR_INVALID, "invalid structure arrays");
-    try {
-        c->parent.assign(parent, parent + c->n_ent);
-        c->slot_entity.assign(top, top + n_top);
-        c->member_first.assign(c->n_ent, 0);
-        c->member_count.assign(c->n_ent, 0);
-        if (member_first) c->member_first.assign(member_first, member_first + c->n_ent);
-        if (member_count) c->member_count.assign(member_count, member_count + c->n_ent);
-        c->members.assign(members ? members : parent, members ? members + n_members : parent);
-        c->slot.assign(c->n_ent, -1);
-        c->sub.assign(c->n_ent, 0);
-        for (int s = 0; s < n_top; s++) {
-            int e = top[s];
-            if (e < 0 || e >= c->n_ent) return fail(c, AM_ERR_INVALID, "top-level entity out of range");
-            c->slot[e] = s;
-            if (c->desc[e].geom == AM_GEOM_COLLECTION) {
-                for (int q = 0; q < c->member_count[e]; q++) {
-                    int idx = c->member_first[e] + q;
-                    if (idx < 0 || idx >= n_members) return fail(c, AM_ERR_INVALID, "member index out of range");
-                    int b = c->members[idx];
-                    if (b < 0 || b >= c->n_ent || c->parent[b] != e)
-                        return fail(c, AM_ERR_INVALID, "member/parent mismatch");
-                    c->slot[b] = s;
-                    c->sub[b] = q;
-                }
-            }
-        }
-        for (int i = 0; i < c->n_phys; i++)
-            if (c->slot[i] < 0) return fail(c, AM_ERR_INVALID, "physical body not reachable from the top-level list");
-        return rebuild_structure(c);
-    } catch (const std::bad_alloc&) {
-        return fail(c, AM_ERR_INVALID, "host allocation failed");
-    }
+    if (n_entities) *n_entities = c->n_ent;
+    if (n_top) *n_top = (int)c->slot_entity.size();
+    if (parent)
+        for (int e = 0; e < std::min(cap_entities, c->n_ent); e++) parent[e] = c->parent[e];
+    if (top)
+        for (int s_ = 0; s_ < std::min(cap_top, (int)c->slot_entity.size()); s_++) top[s_] = c->slot_entity[s_];
+    return AM_OK;
 }
 
-int am_entity_mass_set(am_ctx* c, int32_t entity, double mass, double minv, const double* jinv, int32_t flags) {
+int am_entities_pull(am_ctx* c, am_body_desc* out, int32_t n) {
     int r = check_ctx(c);
     if (r) return r;
-    if (entity < 0 || entity >= c->n_ent) return fail(c, AM_ERR_INVALID, "entity out of range");
-    c->desc[entity].mass = mass;
-    c->desc[entity].minv = minv;
-    c->flags[entity] = flags;
-    c->desc[entity].flags = flags;
-    double mi = (flags & AM_BODY_PINNED) ? 0.0 : minv;
-    AM_HIP(c, hipMemcpyAsync(&c->d_desc.p[entity], &c->desc[entity], sizeof(am_body_desc), hipMemcpyHostToDevice,
-                             c->stream));
-    AM_HIP(c, hipMemcpyAsync(&c->d_minv.p[entity], &mi, sizeof(double), hipMemcpyHostToDevice, c->stream));
-    AM_HIP(c, hipMemcpyAsync(&c->d_flags.p[entity], &flags, sizeof(int), hipMemcpyHostToDevice, c->stream));
-    if (jinv) AM_HIP(c, hipMemcpyAsync(&c->d_jinv.p[entity * 9], jinv, 72, hipMemcpyHostToDevice, c->stream));
-    AM_HIP(c, hipStreamSynchronize(c->stream));
-    return rebuild_structure(c);
+    if (n < 0 || n > c->n_ent || (n > 0 && !out)) return fail(c, AM_ERR_INVALID, "bad entity buffer");
+    for (int e = 0; e < n; e++) out[e] = c->desc[e];
+    return AM_OK;
 }
 
 int am_state_push(am_ctx* c, const double* x, const double* R, const double* v, const double* w, int32_t n) {
     int r = check_ctx(c);
     if (r) return r;
-    if (n != c->n_ent) return fail(c, AM_ERR_INVALID, "state size does not match the scene");
+    if (n < 0 || n > c->n_ent) return fail(c, AM_ERR_INVALID, "state size exceeds the entity count");
     size_t m = (size_t)n;
     for (size_t i = 0; i < m * 3; i++) {
         if ((x && !std::isfinite(x[i])) || (v && !std::isfinite(v[i])) || (w && !std::isfinite(w[i])))
@@ -1103,7 +982,7 @@ int am_state_push(am_ctx* c, const double* x, const double* R, const double* v, 
 int am_state_pull(am_ctx* c, double* x, double* R, double* v, double* w, int32_t n) {
     int r = check_ctx(c);
     if (r) return r;
-    if (n != c->n_ent) return fail(c, AM_ERR_INVALID, "state size does not match the scene");
+    if (n < 0 || n > c->n_ent) return fail(c, AM_ERR_INVALID, "state size exceeds the entity count");
     size_t m = (size_t)n;
     if ((r = pull_d(c, x, c->d_x.p, m * 3))) return r;
     if ((r = pull_d(c, R, c->d_R.p, m * 9))) return r;
@@ -1116,7 +995,7 @@ int am_state_pull(am_ctx* c, double* x, double* R, double* v, double* w, int32_t
 int am_jinv_pull(am_ctx* c, double* jinv, int32_t n) {
     int r = check_ctx(c);
     if (r) return r;
-    if (n != c->n_ent || !jinv) return fail(c, AM_ERR_INVALID, "bad jinv buffer");
+    if (n < 0 || n > c->n_ent || (n > 0 && !jinv)) return fail(c, AM_ERR_INVALID, "bad jinv buffer");
     if ((r = pull_d(c, jinv, c->d_jinv.p, (size_t)n * 9))) return r;
     AM_HIP(c, hipStreamSynchronize(c->stream));
     return AM_OK;
@@ -1125,7 +1004,7 @@ int am_jinv_pull(am_ctx* c, double* jinv, int32_t n) {
 int am_forces_push(am_ctx* c, const double* force, const double* torque, int32_t n) {
     int r = check_ctx(c);
     if (r) return r;
-    if (n != c->n_ent) return fail(c, AM_ERR_INVALID, "force size does not match the scene");
+    if (n < 0 || n > c->n_ent) return fail(c, AM_ERR_INVALID, "force size exceeds the entity count");
     if (force && n) AM_HIP(c, hipMemcpyAsync(c->d_f.p, force, (size_t)n * 24, hipMemcpyHostToDevice, c->stream));
     if (torque && n) AM_HIP(c, hipMemcpyAsync(c->d_t.p, torque, (size_t)n * 24, hipMemcpyHostToDevice, c->stream));
     AM_HIP(c, hipStreamSynchronize(c->stream));
@@ -1135,7 +1014,7 @@ int am_forces_push(am_ctx* c, const double* force, const double* torque, int32_t
 int am_forces_pull(am_ctx* c, double* force, double* torque, int32_t n) {
     int r = check_ctx(c);
     if (r) return r;
-    if (n != c->n_ent) return fail(c, AM_ERR_INVALID, "force size does not match the scene");
+    if (n < 0 || n > c->n_ent) return fail(c, AM_ERR_INVALID, "force size exceeds the entity count");
     if ((r = pull_d(c, force, c->d_f.p, (size_t)n * 3))) return r;
     if ((r = pull_d(c, torque, c->d_t.p, (size_t)n * 3))) return r;
     AM_HIP(c, hipStreamSynchronize(c->stream));
@@ -1147,12 +1026,21 @@ int am_clear_bodies(am_ctx* c) {
     if (r) return r;
     launch_clear(c->stream, bodies_view(c));
     AM_HIP(c, hipGetLastError());
+    // RigidCollection.clearBodies (RigidCollection.java:98-103)
+    if (has_collections(c)) return merge_push_down(c, false);
     return AM_OK;
 }
 
 int am_apply_gravity(am_ctx* c) {
     int r = check_ctx(c);
     if (r) return r;
+    return apply_gravity(c);
+}
+
+}  // extern "C"
+
+namespace am_rt {
+int apply_gravity(am_ctx* c) {
     if (!c->params.use_gravity) return AM_OK;
     double theta = c->params.gravity_angle_deg / 180.0 * M_PI;
     double g = c->params.gravity_amount;
@@ -1161,11 +1049,14 @@ int am_apply_gravity(am_ctx* c) {
     AM_HIP(c, hipGetLastError());
     return AM_OK;
 }
+}  // namespace am_rt
+
+extern "C" {
 
 int am_deltav_pull(am_ctx* c, double* dv, int32_t n) {
     int r = check_ctx(c);
     if (r) return r;
-    if (n != c->n_ent || !dv) return fail(c, AM_ERR_INVALID, "bad deltaV buffer");
+    if (n < 0 || n > c->n_ent || (n > 0 && !dv)) return fail(c, AM_ERR_INVALID, "bad deltaV buffer");
     if ((r = pull_d(c, dv, c->d_dv.p, (size_t)n * 6))) return r;
     AM_HIP(c, hipStreamSynchronize(c->stream));
     return AM_OK;
@@ -1211,6 +1102,7 @@ int am_contacts_push(am_ctx* c, int32_t n, const int32_t* body1, const int32_t* 
                      const int32_t* bv2) {
     int r = check_ctx(c);
     if (r) return r;
+    c->sweep_order_n = -1;
     if (n < 0 || (n > 0 && (!body1 || !body2 || !point_w || !normal_w || !violation)))
         return fail(c, AM_ERR_INVALID, "invalid contact arrays");
     for (int k = 0; k < n; k++) {
@@ -1255,32 +1147,51 @@ int am_contacts_push(am_ctx* c, int32_t n, const int32_t* body1, const int32_t* 
     return AM_OK;
 }
 
+}  // extern "C"
+
+namespace am_rt {
+int pull_contacts(am_ctx* c, const ContactsView& V, int count, am_contact_arrays* o, int32_t capacity) {
+    int r;
+    size_t n = (size_t)std::max(0, std::min<int>(capacity, count));
+    if ((r = pull_i(c, o->body1, V.b1, n))) return r;
+    if ((r = pull_i(c, o->body2, V.b2, n))) return r;
+    if ((r = pull_i(c, o->bv1, V.bv1, n))) return r;
+    if ((r = pull_i(c, o->bv2, V.bv2, n))) return r;
+    if ((r = pull_i(c, o->info, V.info, n))) return r;
+    if ((r = pull_d(c, o->point_b1, V.pB1, n * 3))) return r;
+    if ((r = pull_d(c, o->normal_b1, V.nB1, n * 3))) return r;
+    if ((r = pull_d(c, o->tangent1_b1, V.t1B1, n * 3))) return r;
+    if ((r = pull_d(c, o->tangent2_b1, V.t2B1, n * 3))) return r;
+    if ((r = pull_d(c, o->violation, V.viol, n))) return r;
+    if ((r = pull_d(c, o->prev_violation, V.prev_viol, n))) return r;
+    if ((r = pull_d(c, o->jacobian, V.J, n * 36))) return r;
+    if ((r = pull_d(c, o->lambda, V.lam, n * 3))) return r;
+    if ((r = pull_d(c, o->lambda_warm, V.lamw, n * 3))) return r;
+    if ((r = pull_d(c, o->b, V.b, n * 3))) return r;
+    if ((r = pull_d(c, o->diag, V.D, n * 3))) return r;
+    if ((r = pull_i(c, o->state, V.state, n))) return r;
+    if ((r = pull_i(c, o->new_this_step, V.is_new, n))) return r;
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    return AM_OK;
+}
+}  // namespace am_rt
+
+extern "C" {
+
 int am_contacts_pull(am_ctx* c, am_contact_arrays* o, int32_t capacity, int32_t* n_out) {
     int r = check_ctx(c);
     if (r) return r;
     if (n_out) *n_out = c->n_contacts;
     if (!o) return AM_OK;
-    size_t n = (size_t)std::min<int>(capacity, c->n_contacts);
-    if ((r = pull_i(c, o->body1, c->c_b1.p, n))) return r;
-    if ((r = pull_i(c, o->body2, c->c_b2.p, n))) return r;
-    if ((r = pull_i(c, o->bv1, c->c_bv1.p, n))) return r;
-    if ((r = pull_i(c, o->bv2, c->c_bv2.p, n))) return r;
-    if ((r = pull_i(c, o->info, c->c_info.p, n))) return r;
-    if ((r = pull_d(c, o->point_b1, c->c_pB1.p, n * 3))) return r;
-    if ((r = pull_d(c, o->normal_b1, c->c_nB1.p, n * 3))) return r;
-    if ((r = pull_d(c, o->tangent1_b1, c->c_t1B1.p, n * 3))) return r;
-    if ((r = pull_d(c, o->tangent2_b1, c->c_t2B1.p, n * 3))) return r;
-    if ((r = pull_d(c, o->violation, c->c_viol.p, n))) return r;
-    if ((r = pull_d(c, o->prev_violation, c->c_prev.p, n))) return r;
-    if ((r = pull_d(c, o->jacobian, c->c_J.p, n * 36))) return r;
-    if ((r = pull_d(c, o->lambda, c->c_lam.p, n * 3))) return r;
-    if ((r = pull_d(c, o->lambda_warm, c->c_lamw.p, n * 3))) return r;
-    if ((r = pull_d(c, o->b, c->c_b.p, n * 3))) return r;
-    if ((r = pull_d(c, o->diag, c->c_D.p, n * 3))) return r;
-    if ((r = pull_i(c, o->state, c->c_state.p, n))) return r;
-    if ((r = pull_i(c, o->new_this_step, c->c_new.p, n))) return r;
-    AM_HIP(c, hipStreamSynchronize(c->stream));
-    return AM_OK;
+    return pull_contacts(c, contacts_view(c), c->n_contacts, o, capacity);
+}
+
+int am_internal_contacts_pull(am_ctx* c, am_contact_arrays* o, int32_t capacity, int32_t* n_out) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (n_out) *n_out = c->n_ic;
+    if (!o || c->n_ic == 0) return AM_OK;
+    return pull_contacts(c, internal_view(c), c->n_ic, o, capacity);
 }
 
 int am_lambdas_push(am_ctx* c, const double* lambda, int32_t n) {
@@ -1334,15 +1245,116 @@ int am_solve(am_ctx* c, double dt, int32_t post, int32_t* iterations_used, doubl
     return do_solve(c, dt, post, true, iterations_used, last_change);
 }
 
-int am_single_sweep(am_ctx* c, double dt, int32_t* iterations_used) {
+int am_update_in_collections(am_ctx* c, double dt, int32_t* iterations_used) {
     int r = check_ctx(c);
     if (r) return r;
-    return do_solve(c, dt, 0, false, iterations_used, nullptr);
+    int it = 0;
+    r = merge_update_in_collections(c, dt, &it);
+    if (iterations_used) *iterations_used = it;
+    return r;
+}
+
+int am_accumulate_for_unmerging(am_ctx* c) {
+    int r = check_ctx(c);
+    if (r) return r;
+    return merge_accumulate_unmerging(c);
+}
+
+int am_unmerge(am_ctx* c, double dt, int32_t* n_cut) {
+    int r = check_ctx(c);
+    if (r) return r;
+    int n = 0;
+    r = merge_unmerge(c, dt, &n);
+    if (n_cut) *n_cut = n;
+    return r;
+}
+
+int am_clear_top_level(am_ctx* c) {
+    int r = check_ctx(c);
+    if (r) return r;
+    launch_clear_targets(c->stream, bodies_view(c), c->d_slot_entity.p, (int)c->slot_entity.size());
+    AM_HIP(c, hipGetLastError());
+    return apply_gravity(c);
+}
+
+int am_clear_body_pair_contacts(am_ctx* c) {
+    int r = check_ctx(c);
+    if (r) return r;
+    return merge_body_pairs(c);
+}
+
+int am_accumulate_for_merging(am_ctx* c) {
+    int r = check_ctx(c);
+    if (r) return r;
+    return merge_accumulate_merging(c);
+}
+
+int am_merge(am_ctx* c, int32_t* n_merged) {
+    int r = check_ctx(c);
+    if (r) return r;
+    int n = 0;
+    r = merge_merge(c, &n);
+    if (n_merged) *n_merged = n;
+    return r;
+}
+
+int am_body_pairs_pull(am_ctx* c, am_body_pair* out, int32_t capacity, int32_t* n_external, int32_t* n_internal) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (n_external) *n_external = c->n_bpc;
+    if (n_internal) *n_internal = (int)c->ibpc.size();
+    if (!out || capacity <= 0) return AM_OK;
+    std::vector<BpcRec> T(c->n_bpc);
+    std::vector<IbpcRec> I(c->ibpc.size());
+    if (!T.empty())
+        AM_HIP(c, hipMemcpyAsync(T.data(), c->d_bpc.p, T.size() * sizeof(BpcRec), hipMemcpyDeviceToHost, c->stream));
+    if (!I.empty())
+        AM_HIP(c, hipMemcpyAsync(I.data(), c->d_ibpc.p, I.size() * sizeof(IbpcRec), hipMemcpyDeviceToHost, c->stream));
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    int k = 0;
+    for (const auto& t : T) {
+        if (k >= capacity) break;
+        am_body_pair& o = out[k++];
+        std::memset(&o, 0, sizeof o);
+        o.body1 = t.b1;
+        o.body2 = t.b2;
+        o.in_collection = 0;
+        o.n_contacts = t.n_active;
+        o.hist_n = t.hist_n;
+        o.state_n = t.state_n;
+        o.flag = t.flag;
+        o.collection = -1;
+        for (int q = 0; q < kMaxAccum; q++) {
+            o.hist[q] = t.hist[q];
+            o.states[q] = q < t.state_n ? (t.states >> (2 * q)) & 3 : 0;
+        }
+    }
+    for (const auto& t : I) {
+        if (k >= capacity) break;
+        am_body_pair& o = out[k++];
+        std::memset(&o, 0, sizeof o);
+        o.body1 = t.b1;
+        o.body2 = t.b2;
+        o.in_collection = 1;
+        o.n_contacts = t.count;
+        o.hist_n = t.hist_n;
+        o.flag = t.flag;
+        o.collection = t.coll;
+    }
+    return AM_OK;
 }
 
 int am_solve_order_pull(am_ctx* c, int32_t* order, int32_t capacity, int32_t* n) {
     int r = check_ctx(c);
     if (r) return r;
+    if (c->sweep_order_n >= 0) {  // the single sweep inside collections ran last
+        if (n) *n = c->sweep_order_n;
+        if (order) {
+            if ((r = pull_i(c, order, c->sweep_order.p, (size_t)std::min(capacity, c->sweep_order_n)))) return r;
+            AM_HIP(c, hipStreamSynchronize(c->stream));
+        }
+        return AM_OK;
+    }
     if (c->n_contacts > 0) {
         r = build_layout(c);
         if (r) return r;
@@ -1390,6 +1402,8 @@ int am_advance_velocities(am_ctx* c, double dt) {
     if (r) return r;
     launch_advance(c->stream, bodies_view(c), c->d_integrate.p, (int)c->integrate_targets.size(), dt, false);
     AM_HIP(c, hipGetLastError());
+    // RigidCollection.advanceVelocities (RigidCollection.java:873-877)
+    if (has_collections(c)) return merge_push_down(c, true);
     return AM_OK;
 }
 
@@ -1398,30 +1412,43 @@ int am_advance_positions(am_ctx* c, double dt) {
     if (r) return r;
     launch_advance(c->stream, bodies_view(c), c->d_integrate.p, (int)c->integrate_targets.size(), dt, true);
     AM_HIP(c, hipGetLastError());
+    // RigidCollection.advancePositions (:879-883)
+    if (has_collections(c)) return merge_member_transforms(c);
     return AM_OK;
 }
 
 int am_step(am_ctx* c, double dt, am_step_stats* st) {
     int r = check_ctx(c);
     if (r) return r;
-    hipEvent_t ev[5];
-    for (auto& e : ev) AM_HIP(c, hipEventCreate(&e));
-    struct EvGuard {
-        hipEvent_t* e;
-        ~EvGuard() {
-            for (int i = 0; i < 5; i++) (void)hipEventDestroy(e[i]);
-        }
-    } guard{ev};
+    for (auto& e : c->sev)
+        if (!e) AM_HIP(c, hipEventCreate(&e));
+    hipEvent_t* ev = c->sev;
     hipStream_t s = c->stream;
+    const am_params& P = c->params;
+    const bool merging = P.enable_merging || P.enable_unmerging || has_collections(c);
+    c->total_steps++;
     AM_HIP(c, hipEventRecord(ev[0], s));
-    launch_clear(s, bodies_view(c));
-    if ((r = am_apply_gravity(c))) return r;
+    // RigidBodySystem.clearBodies (:190-202)
+    if ((r = am_clear_bodies(c))) return r;
+    if ((r = apply_gravity(c))) return r;
     if ((r = am_update_contacts_map(c))) return r;
     if ((r = do_detect(c))) return r;
     AM_HIP(c, hipEventRecord(ev[1], s));
     int bad = 0, rep = 0;
     if ((r = am_warm_start(c, 0, &bad, &rep))) return r;
-    if (c->params.warm_start) r = am_redo_warm_start(c);
+    AM_HIP(c, hipEventRecord(ev[5], s));
+    int sweep_it = 0, cut = 0, merged = 0;
+    if (merging) {
+        if ((r = merge_update_in_collections(c, dt, &sweep_it))) return r;
+        if ((r = merge_accumulate_unmerging(c))) return r;
+        if ((r = merge_unmerge(c, dt, &cut))) return r;
+        if (c->merging_event) {
+            launch_clear_targets(s, bodies_view(c), c->d_slot_entity.p, (int)c->slot_entity.size());
+            if ((r = apply_gravity(c))) return r;
+        }
+    }
+    AM_HIP(c, hipEventRecord(ev[6], s));
+    if (P.warm_start) r = am_redo_warm_start(c);
     else r = am_no_warm_start(c);
     if (r) return r;
     AM_HIP(c, hipEventRecord(ev[2], s));
@@ -1429,10 +1456,18 @@ int am_step(am_ctx* c, double dt, am_step_stats* st) {
     double last = 0;
     if ((r = do_solve(c, dt, 0, true, &iters, &last))) return r;
     AM_HIP(c, hipEventRecord(ev[3], s));
+    if (P.enable_merging && (r = merge_body_pairs(c))) return r;
+    AM_HIP(c, hipEventRecord(ev[7], s));
     if ((r = am_advance_velocities(c, dt))) return r;
+    if (P.enable_merging && (r = merge_accumulate_merging(c))) return r;
     if ((r = am_advance_positions(c, dt))) return r;
+    AM_HIP(c, hipEventRecord(ev[8], s));
+    if (P.enable_merging && P.steps_between_merge_events > 0 && c->total_steps % P.steps_between_merge_events == 0)
+        if ((r = merge_merge(c, &merged))) return r;
     AM_HIP(c, hipEventRecord(ev[4], s));
     AM_HIP(c, hipEventSynchronize(ev[4]));
+    c->last_merged = merged;
+    c->last_unmerged = cut;
     if (st) {
         std::memset(st, 0, sizeof *st);
         st->n_contacts = c->n_contacts;
@@ -1443,23 +1478,30 @@ int am_step(am_ctx* c, double dt, am_step_stats* st) {
         st->bad_warm_starts = bad;
         st->bad_warm_starts_repaired = rep;
         st->last_change = last;
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
-        st->ms_detect = ms;
-        (void)hipEventElapsedTime(&ms, ev[1], ev[2]);
-        st->ms_warm_start = ms;
-        (void)hipEventElapsedTime(&ms, ev[2], ev[3]);
-        st->ms_solve = ms;
-        (void)hipEventElapsedTime(&ms, ev[3], ev[4]);
-        st->ms_integrate = ms;
-        (void)hipEventElapsedTime(&ms, ev[0], ev[4]);
-        st->ms_total = ms;
+        auto el = [&](int a, int b) {
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, ev[a], ev[b]);
+            return (double)ms;
+        };
+        st->ms_detect = el(0, 1);
+        st->ms_warm_start = el(1, 5) + el(6, 2);
+        st->ms_solve = el(2, 3);
+        st->ms_integrate = el(7, 8);
+        st->ms_merging = el(5, 6) + el(3, 7) + el(8, 4);
+        st->ms_total = el(0, 4);
         st->ms_pgs_kernel = c->n_contacts ? c->ms_pgs_kernel : 0.0;
         st->ms_setup_kernel = c->n_contacts ? c->ms_setup_kernel : 0.0;
         st->ms_learned = c->n_contacts ? c->ms_learned : 0.0;
         st->start_residual_plain = c->res_plain;
         st->start_residual_learned = c->res_learned;
         st->used_learned = c->used_learned;
+        int nc = 0;
+        for (int e : c->slot_entity)
+            if (c->desc[e].geom == AM_GEOM_COLLECTION) nc++;
+        st->n_collections = nc;
+        st->n_merged = merged;
+        st->n_unmerged = cut;
+        st->n_internal_contacts = c->n_ic;
     }
     return AM_OK;
 }

@@ -1,0 +1,243 @@
+// am_ctx.hpp - the context behind the C ABI: host mirrors, device buffers
+// and the host-side helpers shared by the runtime's translation units.
+#pragma once
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "am_merge.hpp"
+#include "am_mlp.hpp"
+#include "am_solve.hpp"
+
+namespace am_rt {
+
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    // Grows (never shrinks); contents are not preserved.
+    hipError_t ensure(size_t want) {
+        if (want <= n && p) return hipSuccess;
+        release();
+        size_t cap = std::max<size_t>(want, 1);
+        hipError_t e = hipMalloc((void**)&p, cap * sizeof(T));
+        if (e == hipSuccess) n = cap;
+        else p = nullptr;
+        return e;
+    }
+};
+
+
+// Grows b to cap elements of `stride` values, keeping the first `keep` elements.
+template <class T>
+hipError_t grow_keep(hipStream_t s, DBuf<T>& b, size_t cap, size_t stride, size_t keep) {
+    if (keep == 0 || !b.p) return b.ensure(cap * stride);
+    if (b.n >= cap * stride) return hipSuccess;
+    DBuf<T> nb;
+    hipError_t e = nb.ensure(cap * stride);
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(nb.p, b.p, keep * stride * sizeof(T), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    std::swap(nb.p, b.p);
+    std::swap(nb.n, b.n);
+    return hipSuccess;
+}
+
+// A set of contact arrays (the fields of ContactsView) with its own capacity.
+struct ContactStore {
+    int cap = 0;
+    DBuf<int> b1, b2, bv1, bv2, info, state, is_new;
+    DBuf<double> pB1, nB1, t1B1, t2B1, viol, prev, J, lam, lamw, b, D;
+    hipError_t ensure(int want, int keep, hipStream_t s);
+    void swap(ContactStore& o);
+    am::ContactsView view() {
+        am::ContactsView C;
+        C.cap = cap;
+        C.b1 = b1.p;
+        C.b2 = b2.p;
+        C.bv1 = bv1.p;
+        C.bv2 = bv2.p;
+        C.info = info.p;
+        C.pB1 = pB1.p;
+        C.nB1 = nB1.p;
+        C.t1B1 = t1B1.p;
+        C.t2B1 = t2B1.p;
+        C.viol = viol.p;
+        C.prev_viol = prev.p;
+        C.J = J.p;
+        C.lam = lam.p;
+        C.lamw = lamw.p;
+        C.b = b.p;
+        C.D = D.p;
+        C.state = state.p;
+        C.is_new = is_new.p;
+        return C;
+    }
+};
+
+}  // namespace am_rt
+
+struct am_ctx {
+    template <class T>
+    using DBuf = am_rt::DBuf<T>;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    am_params params{};
+
+    // scene (host mirrors)
+    int n_ent = 0, n_phys = 0, n_nodes = 0;
+    std::vector<am_body_desc> desc;
+    std::vector<am_sphere_node> nodes;
+    std::vector<int> parent, slot, sub, slot_entity, member_first, member_count, members, flags;
+    std::vector<int> planes, grid_bodies, node_body, top_pinned, top_sleeping, gravity_targets, integrate_targets;
+    double cell_size = 1.0;
+    int hash_mask = 0;
+
+    // body buffers
+    DBuf<am_body_desc> d_desc;
+    DBuf<am_sphere_node> d_nodes;
+    DBuf<double> d_cw, d_x, d_R, d_v, d_w, d_f, d_t, d_jinv, d_dv, d_minv;
+    DBuf<int> d_flags, d_parent, d_slot, d_sub, d_slot_entity, d_member_first, d_members, d_planes, d_grid,
+        d_node_body, d_top_pinned, d_top_sleeping, d_gravity, d_integrate;
+
+    // broad phase
+    DBuf<double> d_sphere;
+    DBuf<int> d_cell, d_bucket_count, d_bucket_start, d_bucket_cursor, d_bucket_items, d_pair_counts, d_pair_offsets;
+    DBuf<uint64_t> d_keys_unsorted, d_keys;
+    DBuf<unsigned char> d_sort_temp, d_scan_temp;
+    size_t sort_temp_size = 0;
+
+    // contacts
+    int cap_c = 0;
+    DBuf<int> c_b1, c_b2, c_bv1, c_bv2, c_info, c_state, c_new;
+    DBuf<double> c_pB1, c_nB1, c_t1B1, c_t2B1, c_viol, c_prev, c_J, c_lam, c_lamw, c_b, c_D;
+    DBuf<int> d_narrow_counts, d_narrow_offsets;
+
+    // old contacts (warm start)
+    int n_old = 0, table_mask = 0;
+    DBuf<int> o_b1, o_b2, o_bv1, o_bv2, o_info, o_table;
+    DBuf<double> o_pB1, o_lam, o_viol, o_prev;
+
+    // groups, colouring, solve layout
+    DBuf<int> g_flags, g_scanned, g_start, g_a, g_b, g_colour_a, g_colour_b, inc_count, inc_start, inc_cursor, inc_items,
+        g_sizes, g_pos_start, s_order, s_colour_start;
+    DBuf<uint64_t> g_keys, g_keys_sorted;
+    DBuf<double> s_rec, s_lam;
+    DBuf<int> s_meta, s_state, d_barrier, s_group_desc;
+    DBuf<unsigned long long> s_iter_max;
+
+    // scalars on device: 0 n_pairs, 1 n_contacts, 2 n_groups, 3 total scratch, 4 overflow, 5 remaining,
+    // 6 iters_used, 7 bad, 8 repaired, 9 n_sweep
+    DBuf<int> d_scalars;
+    int* h_scalars = nullptr;  // pinned mirror
+
+    int n_pairs = 0, n_contacts = 0, n_groups = 0, n_colours = 0, max_colour_groups = 0;
+    bool layout_valid = false;
+    int last_iters = 0;
+    double last_change = 0.0;
+    std::vector<double> changes;
+    int pgs_max_grid = 0;
+    hipEvent_t kev[3] = {nullptr, nullptr, nullptr};  // setup start, pgs start, pgs end
+    double ms_pgs_kernel = 0.0, ms_setup_kernel = 0.0;
+
+    // learned warm start
+    bool mlp_ready = false;
+    DBuf<float> m_p, m_g, m_m, m_v;  // padded FP32 parameters, gradients, Adam moments
+    DBuf<unsigned short> m_bf;       // padded BF16 weights
+    DBuf<float> f_X, f_Y, d_loss;
+    DBuf<double> f_scale, f_lam, s_start, dv_save;
+    DBuf<unsigned long long> d_resid;
+    DBuf<float> t_X, t_T;
+    DBuf<int> t_idx;
+    int t_n = 0, adam_step = 0;
+    double ms_learned = 0.0, res_plain = 0.0, res_learned = 0.0;
+    int used_learned = 0;
+    hipEvent_t lev[2] = {nullptr, nullptr};
+
+    // adaptive merging (Merging, RigidCollection, BodyPairContact)
+    int total_steps = 0;
+    bool merging_event = false;         // Merging.mergingEvent: set by the first merge or unmerge
+    std::vector<int> free_entities;     // collection entity ids not in use
+    std::vector<double> b2c;            // per entity: body-to-collection transform, R (9) then t (3)
+    DBuf<double> d_b2c;
+    std::vector<int> member_list;       // every collection member (physical body)
+    std::vector<int> member_adv;        // members that the single sweep integrates (not pinned)
+    DBuf<int> d_member_list, d_member_adv;
+    // external body-pair table, rebuilt after every main solve
+    int n_bpc = 0, n_bpc_prev = 0, bpc_mask = 0;
+    DBuf<am::BpcRec> d_bpc, d_bpc_prev;
+    DBuf<int> d_bpc_hash, bp_flags, bp_scanned, bp_start, bp_keep, bp_keep_scan;
+    DBuf<unsigned char> d_active;
+    // internal body pairs and their contacts (RigidCollection.internalContacts)
+    std::vector<am::IbpcRec> ibpc;
+    DBuf<am::IbpcRec> d_ibpc;
+    int n_ic = 0;
+    am_rt::ContactStore ic, ic_tmp;
+    int last_merged = 0, last_unmerged = 0;
+    DBuf<int> sweep_order;              // order of the last single sweep inside collections
+    int sweep_order_n = -1;             // -1: none since the last solve or detection
+    hipEvent_t sev[9] = {};               // am_step phase events
+    double ms_merging = 0.0;
+};
+
+namespace am_rt {
+
+int fail(am_ctx* c, int code, const std::string& m);
+int check_ctx(am_ctx* c);
+am::BodiesView bodies_view(am_ctx* c);
+am::TreeView tree_view(am_ctx* c);
+am::ContactsView contacts_view(am_ctx* c);
+am::ContactsView internal_view(am_ctx* c);
+am::MergeParams merge_params(const am_params& p);
+int read_scalars(am_ctx* c, int first, int n);
+int ensure_contact_capacity(am_ctx* c, int want, int keep = 0);
+int rebuild_structure(am_ctx* c);
+int refresh_jinv(am_ctx* c);
+int groups_after_contacts(am_ctx* c);
+int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double* last);
+int apply_gravity(am_ctx* c);
+int pull_d(am_ctx* c, double* dst, const double* src, size_t n);
+int pull_i(am_ctx* c, int32_t* dst, const int* src, size_t n);
+
+template <class T>
+hipError_t upload(am_ctx* c, DBuf<T>& b, const std::vector<T>& v) {
+    hipError_t e = b.ensure(std::max<size_t>(v.size(), 1));
+    if (e != hipSuccess) return e;
+    if (!v.empty()) return hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream);
+    return hipSuccess;
+}
+
+// merging phases (am_merge.hip)
+int merge_update_in_collections(am_ctx* c, double dt, int* iters);
+int merge_accumulate_unmerging(am_ctx* c);
+int merge_unmerge(am_ctx* c, double dt, int* n_events);
+int merge_body_pairs(am_ctx* c);
+int merge_accumulate_merging(am_ctx* c);
+int merge_merge(am_ctx* c, int* n_merged);
+int merge_push_down(am_ctx* c, bool integrated_only);
+int merge_member_transforms(am_ctx* c);
+int pull_contacts(am_ctx* c, const am::ContactsView& V, int count, am_contact_arrays* o, int32_t capacity);
+bool has_collections(const am_ctx* c);
+
+}  // namespace am_rt
+
+#define AM_HIP(ctx, expr)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return am_rt::fail(ctx, AM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)

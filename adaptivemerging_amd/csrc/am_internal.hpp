@@ -58,6 +58,40 @@ struct ContactsView {
     int* is_new;
 };
 
+// Velocity of a world point moving with body b (RigidBody.getSpatialVelocity).
+AMD_INLINE d3 spatial_velocity(const BodiesView& B, int b, d3 pW) {
+    d3 r = sub(pW, ld3(&B.x[b * 3]));
+    return add(cross(ld3(&B.w[b * 3]), r), ld3(&B.v[b * 3]));
+}
+// Largest speed of body a's bounding-box corners
+// (MotionMetricProcessor.getLargestVelocityNorm(body), :62-73).
+AMD_INLINE double metric_one(const BodiesView& B, int a) {
+    m33 R = ldm(&B.R[a * 9]);
+    d3 x = ld3(&B.x[a * 3]);
+    double largest = 0;
+    for (int q = 0; q < 8; q++) {
+        d3 pW = xform(R, x, ld3(&B.desc[a].bbox[q * 3]));
+        largest = jmax(length(spatial_velocity(B, a, pW)), largest);
+    }
+    return largest;
+}
+// Largest relative speed over both bodies' bounding-box corners (:39-57).
+AMD_INLINE double metric_two(const BodiesView& B, int a, int b) {
+    double largest = 0;
+    for (int s = 0; s < 2; s++) {
+        int body = s == 0 ? a : b;
+        m33 R = ldm(&B.R[body * 9]);
+        d3 x = ld3(&B.x[body * 3]);
+        for (int q = 0; q < 8; q++) {
+            d3 pW = xform(R, x, ld3(&B.desc[body].bbox[q * 3]));
+            d3 v1 = spatial_velocity(B, a, pW);
+            d3 v2 = spatial_velocity(B, b, pW);
+            largest = jmax(length(sub(v1, v2)), largest);
+        }
+    }
+    return largest;
+}
+
 // Previous step's contacts needed by the warm start.
 struct OldContactsView {
     int n;

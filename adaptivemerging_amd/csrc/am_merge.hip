@@ -1,0 +1,1380 @@
+// am_merge.hip - adaptive merging.
+//
+// Device side: the per-step physics of collections (velocity push-down,
+// member transforms), the external body-pair table with its merge histories,
+// the unmerge histories of internal pairs, and contact gathers.
+// Host side: the graph edits of Merging.merge / Merging.unmerge and the mass
+// properties of RigidCollection, done when a merge or unmerge event happens
+// (rare), on host mirrors downloaded for that event.
+//
+// The reference iterates HashSets here (bodyPairContacts, bpcsToUnmerge,
+// subbodies); this file uses fixed orders instead: body pairs in contact-list
+// order, internal pairs in creation order, connected components in
+// depth-first order from the first member.
+#include <cmath>
+#include <functional>
+#include <new>
+
+#include "am_ctx.hpp"
+
+namespace am {
+
+// ---------------------------------------------------------------------------
+// Collection physics on the device.
+
+// RigidCollection.applyVelocitiesTo (RigidCollection.java:925-937) for every
+// listed member: v = v_c + w_c x (x - x_c), w = w_c.  integrated_only limits it
+// to members of collections that advanceVelocities visits (not pinned).
+__global__ void k_push_down(BodiesView B, const int* members, int n, int integrated_only) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int b = members[k];
+    int c = B.parent[b];
+    if (integrated_only && (B.flags[c] & (AM_BODY_PINNED | AM_BODY_SLEEPING))) return;
+    d3 r = sub(ld3(&B.x[b * 3]), ld3(&B.x[c * 3]));
+    d3 wxr = cross(ld3(&B.w[c * 3]), r);
+    st3(&B.v[b * 3], add(ld3(&B.v[c * 3]), wxr));
+    st3(&B.w[b * 3], ld3(&B.w[c * 3]));
+}
+void launch_push_down(hipStream_t s, BodiesView B, const int* members, int n, int integrated_only) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_push_down, dim3((n + 255) / 256), dim3(256), 0, s, B, members, n, integrated_only);
+}
+
+// RigidCollection.updateBodiesPositionAndTransformations (:898-909):
+// B2W = C2W * B2C, then jinv = R jinv0 R^T, for members of collections that
+// advancePositions visits.
+__global__ void k_member_transforms(BodiesView B, const double* b2c, const int* members, int n) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int b = members[k];
+    int c = B.parent[b];
+    if (B.flags[c] & (AM_BODY_PINNED | AM_BODY_SLEEPING)) return;
+    m33 Rc = ldm(&B.R[c * 9]);
+    m33 Rbc = ldm(&b2c[(size_t)b * 12]);
+    d3 t = add(mul(Rc, ld3(&b2c[(size_t)b * 12 + 9])), ld3(&B.x[c * 3]));
+    m33 R = mmul(Rc, Rbc);
+    st3(&B.x[b * 3], t);
+    stm(&B.R[b * 9], R);
+    stm(&B.jinv[b * 9], rm0rt(R, ldm(B.desc[b].jinv0)));
+}
+void launch_member_transforms(hipStream_t s, BodiesView B, const double* b2c, const int* members, int n) {
+    if (n > 0) hipLaunchKernelGGL(k_member_transforms, dim3((n + 127) / 128), dim3(128), 0, s, B, b2c, members, n);
+}
+
+__global__ void k_zero_dv(BodiesView B, const int* targets, int n) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int i = targets[k];
+#pragma unroll
+    for (int a = 0; a < 6; a++) B.dv[i * 6 + a] = 0.0;
+}
+void launch_zero_dv(hipStream_t s, BodiesView B, const int* targets, int n) {
+    if (n > 0) hipLaunchKernelGGL(k_zero_dv, dim3((n + 255) / 256), dim3(256), 0, s, B, targets, n);
+}
+
+// RigidBody.clear for the listed entities only.
+__global__ void k_clear_targets(BodiesView B, const int* targets, int n) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int i = targets[k];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        B.f[i * 3 + a] = 0.0;
+        B.t[i * 3 + a] = 0.0;
+    }
+#pragma unroll
+    for (int a = 0; a < 6; a++) B.dv[i * 6 + a] = 0.0;
+}
+void launch_clear_targets(hipStream_t s, BodiesView B, const int* targets, int n) {
+    if (n > 0) hipLaunchKernelGGL(k_clear_targets, dim3((n + 255) / 256), dim3(256), 0, s, B, targets, n);
+}
+
+// ---------------------------------------------------------------------------
+// External body pairs.
+
+AMD_INLINE uint64_t pair_key(int a, int b) {
+    uint32_t lo = (uint32_t)min(a, b), hi = (uint32_t)max(a, b);
+    return ((uint64_t)lo << 32) | hi;
+}
+AMD_INLINE uint32_t pair_hash(uint64_t z) {
+    z ^= z >> 33;
+    z *= 0xff51afd7ed558ccdull;
+    z ^= z >> 33;
+    z *= 0xc4ceb9fe1a85ec53ull;
+    z ^= z >> 33;
+    return (uint32_t)z;
+}
+
+// clearBodyPairContacts (CollisionProcessor.java:213-226), pass 1: a
+// contact is active when |lambda0| > 1e-14; a pair survives with one active
+// contact or more.
+__global__ void k_bpc_keep(ContactsView C, const int* n_groups, const int* group_start, unsigned char* active,
+                           int* keep, int cap_groups) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= cap_groups) return;
+    int ng = *n_groups;
+    if (g >= ng) {
+        keep[g] = 0;
+        return;
+    }
+    int any = 0;
+    for (int k = group_start[g]; k < group_start[g + 1]; k++) {
+        unsigned char a = fabs(C.lam[k * 3]) > 1e-14 ? 1 : 0;
+        active[k] = a;
+        any |= a;
+    }
+    keep[g] = any;
+}
+
+// Pass 2: one record per surviving pair, with the histories carried over from
+// the previous step's record of the same pair (BodyPairContact.checkExists).
+__global__ void k_bpc_write(ContactsView C, const int* n_groups, const int* group_start, const unsigned char* active,
+                            const int* keep, const int* keep_scan, const BpcRec* prev, const int* prev_hash,
+                            int prev_mask, BpcRec* out, double threshold_breath) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= *n_groups || !keep[g]) return;
+    BpcRec r;
+    int first = group_start[g], end = group_start[g + 1];
+    r.b1 = C.b1[first];
+    r.b2 = C.b2[first];
+    r.first = first;
+    r.end = end;
+    r.n_active = 0;
+    r.not_on_edge = 0;
+    r.breath = 0;
+    for (int k = first; k < end; k++) {
+        if (!active[k]) continue;
+        r.n_active++;
+        if (C.state[k] != AM_STATE_ONEDGE) r.not_on_edge++;
+        if (fabs(C.prev_viol[k] - C.viol[k]) > threshold_breath) r.breath = 1;
+    }
+    r.hist_n = 0;
+    r.state_n = 0;
+    r.states = 0;
+    r.flag = 0;
+    r.pad = 0;
+    for (int q = 0; q < kMaxAccum; q++) r.hist[q] = 0.0;
+    if (prev_mask > 0) {
+        uint64_t key = pair_key(r.b1, r.b2);
+        uint32_t h = pair_hash(key) & (uint32_t)prev_mask;
+        for (int probe = 0; probe <= prev_mask; probe++) {
+            int idx = prev_hash[h];
+            if (idx < 0) break;
+            const BpcRec& p = prev[idx];
+            if (pair_key(p.b1, p.b2) == key) {
+                r.hist_n = p.hist_n;
+                r.state_n = p.state_n;
+                r.states = p.states;
+                for (int q = 0; q < kMaxAccum; q++) r.hist[q] = p.hist[q];
+                break;
+            }
+            h = (h + 1) & (uint32_t)prev_mask;
+        }
+    }
+    out[keep_scan[g]] = r;
+}
+
+void launch_bpc_build(hipStream_t s, ContactsView C, const int* n_contacts, const int* group_start, int n_groups_cap,
+                      unsigned char* active, int* keep, int* keep_scan, int* d_n_keep, void* scan_temp,
+                      const BpcRec* prev, int n_prev, const int* prev_hash, int prev_mask, BpcRec* out,
+                      double threshold_breath) {
+    (void)n_contacts;
+    (void)n_prev;
+    if (n_groups_cap <= 0) return;
+    // the group count sits in the slot after the scan total written by launch_groups
+    const int* n_groups = d_n_keep + 1;
+    dim3 g((n_groups_cap + 255) / 256), b(256);
+    hipLaunchKernelGGL(k_bpc_keep, g, b, 0, s, C, n_groups, group_start, active, keep, n_groups_cap);
+    exclusive_scan(s, keep, keep_scan, n_groups_cap, d_n_keep, scan_temp);
+    hipLaunchKernelGGL(k_bpc_write, g, b, 0, s, C, n_groups, group_start, active, keep, keep_scan, prev, prev_hash,
+                       prev_mask, out, threshold_breath);
+}
+
+// Open-addressing index over a pair table (pairs with b1 < 0 are dead).
+__global__ void k_bpc_hash(const BpcRec* bpc, int n, int* hash, int mask) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || bpc[i].b1 < 0) return;
+    uint32_t h = pair_hash(pair_key(bpc[i].b1, bpc[i].b2)) & (uint32_t)mask;
+    for (int probe = 0; probe <= mask; probe++) {
+        if (atomicCAS(&hash[h], -1, i) == -1) return;
+        h = (h + 1) & (uint32_t)mask;
+    }
+}
+void launch_bpc_hash(hipStream_t s, const BpcRec* bpc, int n, int* hash, int mask) {
+    if (n > 0) hipLaunchKernelGGL(k_bpc_hash, dim3((n + 255) / 256), dim3(256), 0, s, bpc, n, hash, mask);
+}
+
+AMD_INLINE int top_of(const BodiesView& B, int b) { return B.parent[b] >= 0 ? B.parent[b] : b; }
+AMD_INLINE bool pinned(const BodiesView& B, int e) { return (B.flags[e] & AM_BODY_PINNED) != 0; }
+AMD_INLINE double pair_metric(const BodiesView& B, int a, int b) {
+    if (pinned(B, a)) return metric_one(B, b);
+    if (pinned(B, b)) return metric_one(B, a);
+    return metric_two(B, a, b);
+}
+
+// accumulateForMerging (BodyPairContact.java:83-121) followed by the parts of
+// checkMergeCondition (:151-177) that do not depend on the merges done
+// earlier in the same call: breathing, pinned bodies, the motion-metric
+// history (:187-198) and stable contacts (:212-224).
+__global__ void k_bpc_accumulate(BodiesView B, BpcRec* bpc, int n, MergeParams P) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    BpcRec& r = bpc[i];
+    const int accum = P.step_accum_merging;
+    double metric = pair_metric(B, top_of(B, r.b1), top_of(B, r.b2));
+    // motionMetricHist.add, then drop the oldest beyond stepAccumMerging
+    if (r.hist_n < accum) {
+        r.hist[r.hist_n++] = metric;
+    } else if (accum > 0) {
+        for (int q = 0; q + 1 < accum; q++) r.hist[q] = r.hist[q + 1];
+        r.hist[accum - 1] = metric;
+    }
+    int state;
+    if (r.not_on_edge == 0) state = AM_STATE_ONEDGE;
+    else if (r.not_on_edge >= 2) state = AM_STATE_CLEAR;
+    else if (r.n_active == 1) state = AM_STATE_CLEAR;
+    else state = AM_STATE_ONEDGE;
+    if (r.state_n < accum) {
+        r.states |= state << (2 * r.state_n);
+        r.state_n++;
+    } else if (accum > 0) {
+        r.states = (r.states >> 2) | (state << (2 * (accum - 1)));
+    }
+    int flag = 1;
+    if (P.let_it_breathe && r.breath) flag = 0;
+    if (!P.merge_pinned && (pinned(B, r.b1) || pinned(B, r.b2))) flag = 0;
+    if (r.hist_n != accum) {
+        flag = 0;
+    } else {
+        for (int q = 0; q < accum; q++)
+            if (r.hist[q] > P.threshold_merge) flag = 0;
+    }
+    if (P.stable_contact) {
+        if (r.state_n != accum) {
+            flag = 0;
+        } else {
+            for (int q = 0; q < accum; q++)
+                if (((r.states >> (2 * q)) & 3) == AM_STATE_ONEDGE) flag = 0;
+        }
+    }
+    r.flag = flag;
+}
+void launch_bpc_accumulate(hipStream_t s, BodiesView B, BpcRec* bpc, int n, MergeParams P) {
+    if (n > 0) hipLaunchKernelGGL(k_bpc_accumulate, dim3((n + 127) / 128), dim3(128), 0, s, B, bpc, n, P);
+}
+
+// accumulateForUnmerging (BodyPairContact.java:127-145) on the members'
+// velocities after the single sweep, then the unmerge test of
+// Merging.unmerge: checkContactsState (:255-266) or
+// checkMotionMetricForUnmerging (:200-205).
+__global__ void k_ibpc_accumulate(BodiesView B, IbpcRec* ib, int n, ContactsView IC, MergeParams P) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    IbpcRec& r = ib[i];
+    if (B.flags[r.coll] & AM_BODY_SLEEPING) return;
+    double metric = pair_metric(B, r.b1, r.b2);
+    if (metric > P.threshold_unmerge) r.hist_n = min(r.hist_n + 1, 1 << 30);
+    else r.hist_n = 0;
+    int flag = 0;
+    for (int k = r.first; k < r.first + r.count; k++) {
+        int st = IC.state[k];
+        if (st == AM_STATE_BROKEN && P.unmerge_normal) flag = 1;
+        if (st == AM_STATE_ONEDGE && P.unmerge_friction) flag = 1;
+    }
+    if (!flag && r.hist_n >= P.step_accum_unmerging && r.count < 3) flag = 1;
+    r.flag = flag;
+}
+void launch_ibpc_accumulate(hipStream_t s, BodiesView B, IbpcRec* ib, int n, ContactsView IC, MergeParams P) {
+    if (n > 0) hipLaunchKernelGGL(k_ibpc_accumulate, dim3((n + 127) / 128), dim3(128), 0, s, B, ib, n, IC, P);
+}
+
+// ---------------------------------------------------------------------------
+// Contact gathers: dst[dst_off + k] = src[idx ? idx[k] : src_off + k].
+// mode 1 is the copy of Contact(Contact) (Contact.java:120-150) made when a
+// pair becomes internal: lambda_warm, b and D start at zero and the contact
+// is no longer new.  mode 2 re-exposes internal contacts (Merging.java:346-353):
+// lambda_warm = lambda.
+__global__ void k_gather_contacts(ContactsView dst, int dst_off, ContactsView src, const int* idx, int src_off, int n,
+                                  int mode) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    size_t s = (size_t)(idx ? idx[k] : src_off + k);
+    size_t d = (size_t)(dst_off + k);
+    dst.b1[d] = src.b1[s];
+    dst.b2[d] = src.b2[s];
+    dst.bv1[d] = src.bv1[s];
+    dst.bv2[d] = src.bv2[s];
+    dst.info[d] = src.info[s];
+    dst.state[d] = src.state[s];
+    dst.is_new[d] = mode == 1 ? 0 : src.is_new[s];
+    dst.viol[d] = src.viol[s];
+    dst.prev_viol[d] = src.prev_viol[s];
+    for (int a = 0; a < 3; a++) {
+        dst.pB1[d * 3 + a] = src.pB1[s * 3 + a];
+        dst.nB1[d * 3 + a] = src.nB1[s * 3 + a];
+        dst.t1B1[d * 3 + a] = src.t1B1[s * 3 + a];
+        dst.t2B1[d * 3 + a] = src.t2B1[s * 3 + a];
+        double lam = src.lam[s * 3 + a];
+        dst.lam[d * 3 + a] = lam;
+        dst.lamw[d * 3 + a] = mode == 1 ? 0.0 : (mode == 2 ? lam : src.lamw[s * 3 + a]);
+        dst.b[d * 3 + a] = mode == 1 ? 0.0 : src.b[s * 3 + a];
+        dst.D[d * 3 + a] = mode == 1 ? 0.0 : src.D[s * 3 + a];
+    }
+    for (int a = 0; a < 36; a++) dst.J[d * 36 + a] = src.J[s * 36 + a];
+}
+void launch_gather_contacts(hipStream_t s, ContactsView dst, int dst_off, ContactsView src, const int* idx,
+                            int src_off, int n, int mode) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_gather_contacts, dim3((n + 127) / 128), dim3(128), 0, s, dst, dst_off, src, idx, src_off,
+                           n, mode);
+}
+
+}  // namespace am
+
+// ===========================================================================
+// Host side.
+namespace am_rt {
+
+using namespace am;
+
+namespace {
+
+// ---- javax.vecmath operations in the reference's operation order ----------
+struct V3 {
+    double x, y, z;
+};
+struct M3 {
+    double m[9];
+};
+V3 v3(const double* p) { return V3{p[0], p[1], p[2]}; }
+void put(double* p, V3 v) {
+    p[0] = v.x;
+    p[1] = v.y;
+    p[2] = v.z;
+}
+M3 m3(const double* p) {
+    M3 a;
+    for (int i = 0; i < 9; i++) a.m[i] = p[i];
+    return a;
+}
+M3 zero3() {
+    M3 a;
+    for (int i = 0; i < 9; i++) a.m[i] = 0.0;
+    return a;
+}
+M3 ident3() {
+    M3 a = zero3();
+    a.m[0] = a.m[4] = a.m[8] = 1.0;
+    return a;
+}
+// Matrix3d.transform: m00*x + m01*y + m02*z
+V3 mulv(const M3& A, V3 v) {
+    return V3{A.m[0] * v.x + A.m[1] * v.y + A.m[2] * v.z, A.m[3] * v.x + A.m[4] * v.y + A.m[5] * v.z,
+              A.m[6] * v.x + A.m[7] * v.y + A.m[8] * v.z};
+}
+// Matrix3d.mul(A, B)
+M3 mmul(const M3& A, const M3& B) {
+    M3 C;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            C.m[i * 3 + j] = A.m[i * 3 + 0] * B.m[0 * 3 + j] + A.m[i * 3 + 1] * B.m[1 * 3 + j] +
+                             A.m[i * 3 + 2] * B.m[2 * 3 + j];
+    return C;
+}
+// Matrix3d.mulTransposeRight(A, B) = A B^T
+M3 mmul_bt(const M3& A, const M3& B) {
+    M3 C;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            C.m[i * 3 + j] = A.m[i * 3 + 0] * B.m[j * 3 + 0] + A.m[i * 3 + 1] * B.m[j * 3 + 1] +
+                             A.m[i * 3 + 2] * B.m[j * 3 + 2];
+    return C;
+}
+// Matrix3d.mulTransposeLeft(A, B) = A^T B
+M3 mmul_at(const M3& A, const M3& B) {
+    M3 C;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            C.m[i * 3 + j] = A.m[0 * 3 + i] * B.m[0 * 3 + j] + A.m[1 * 3 + i] * B.m[1 * 3 + j] +
+                             A.m[2 * 3 + i] * B.m[2 * 3 + j];
+    return C;
+}
+M3 transpose(const M3& A) {
+    M3 C;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) C.m[i * 3 + j] = A.m[j * 3 + i];
+    return C;
+}
+// RigidTransform3D.computeRM0RT / computeRTMR (RigidTransform3D.java:225-242)
+M3 rm0rt(const M3& R, const M3& M) { return mmul_bt(mmul(R, M), R); }
+M3 rtmr(const M3& R, const M3& M) { return mmul(mmul_at(R, M), R); }
+// 3x3 inverse by cofactors (Matrix3d.invert; vecmath's own elimination order
+// is not reproduced).
+M3 invert3(const M3& A) {
+    const double* a = A.m;
+    double c00 = a[4] * a[8] - a[5] * a[7];
+    double c01 = a[5] * a[6] - a[3] * a[8];
+    double c02 = a[3] * a[7] - a[4] * a[6];
+    double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    double s = 1.0 / det;
+    M3 r;
+    r.m[0] = c00 * s;
+    r.m[1] = (a[2] * a[7] - a[1] * a[8]) * s;
+    r.m[2] = (a[1] * a[5] - a[2] * a[4]) * s;
+    r.m[3] = c01 * s;
+    r.m[4] = (a[0] * a[8] - a[2] * a[6]) * s;
+    r.m[5] = (a[2] * a[3] - a[0] * a[5]) * s;
+    r.m[6] = c02 * s;
+    r.m[7] = (a[1] * a[6] - a[0] * a[7]) * s;
+    r.m[8] = (a[0] * a[4] - a[1] * a[3]) * s;
+    return r;
+}
+void madd(M3& A, const M3& B) {
+    for (int i = 0; i < 9; i++) A.m[i] = A.m[i] + B.m[i];
+}
+void msub(M3& A, const M3& B) {
+    for (int i = 0; i < 9; i++) A.m[i] = A.m[i] - B.m[i];
+}
+// Java Math.max / Math.min
+double jmax_h(double a, double b) {
+    if (a != a) return a;
+    if (a == 0.0 && b == 0.0) return std::signbit(a) ? b : a;
+    return a >= b ? a : b;
+}
+double jmin_h(double a, double b) {
+    if (a != a) return a;
+    if (a == 0.0 && b == 0.0) return std::signbit(a) ? a : b;
+    return a <= b ? a : b;
+}
+
+// RigidCollection.getOp (:808-830): parallel-axis term of body at distance
+// (x - com), scaled by the body's mass.
+M3 get_op(V3 bx, double mass, V3 com) {
+    double x = bx.x - com.x, y = bx.y - com.y, z = bx.z - com.z;
+    double x2 = x * x, y2 = y * y, z2 = z * z;
+    M3 op;
+    op.m[0] = y2 + z2;
+    op.m[1] = -x * y;
+    op.m[2] = -x * z;
+    op.m[3] = -y * x;
+    op.m[4] = x2 + z2;
+    op.m[5] = -y * z;
+    op.m[6] = -z * x;
+    op.m[7] = -z * y;
+    op.m[8] = x2 + y2;
+    for (int i = 0; i < 9; i++) op.m[i] = op.m[i] * mass;
+    return op;
+}
+
+// Host mirror of one entity for the duration of a merge or unmerge event.
+struct Ent {
+    V3 x, v, w;
+    M3 R, I, I0, jinv, jinv0;  // massAngular, massAngular0, jinv, jinv0
+    double mass, minv;
+    bool pinned, plane;
+    double bb[24];
+};
+
+struct Event {
+    am_ctx* c;
+    std::vector<double> x, R, v, w, jinv;  // device state of every entity
+    std::vector<char> touched;             // entity state changed on the host
+    std::vector<char> dv_zero;             // deltaV to clear (unmergeBody)
+    std::vector<std::vector<int>> members; // per collection entity: RigidCollection.bodies
+
+    Ent load(int e) const {
+        const am_body_desc& d = c->desc[e];
+        Ent E;
+        E.x = v3(&x[e * 3]);
+        E.v = v3(&v[e * 3]);
+        E.w = v3(&w[e * 3]);
+        E.R = m3(&R[e * 9]);
+        E.jinv = m3(&jinv[e * 9]);
+        E.jinv0 = m3(d.jinv0);
+        E.I0 = m3(d.inertia0);
+        E.pinned = (c->flags[e] & AM_BODY_PINNED) != 0;
+        E.plane = d.geom == AM_GEOM_PLANE;
+        // massAngular as RigidBody.updateRotationalInertiaFromTransformation left it
+        E.I = E.pinned ? zero3() : rm0rt(E.R, E.I0);
+        E.mass = E.pinned ? 0.0 : d.mass;
+        E.minv = E.pinned ? 0.0 : d.minv;
+        for (int i = 0; i < 24; i++) E.bb[i] = d.bbox[i];
+        return E;
+    }
+    // Writes a collection's state and mass properties back to the mirrors.
+    void store_collection(int e, const Ent& E) {
+        put(&x[e * 3], E.x);
+        put(&v[e * 3], E.v);
+        put(&w[e * 3], E.w);
+        for (int i = 0; i < 9; i++) {
+            R[e * 9 + i] = E.R.m[i];
+            jinv[e * 9 + i] = E.jinv.m[i];
+        }
+        am_body_desc& d = c->desc[e];
+        std::memset(&d, 0, sizeof d);
+        d.geom = AM_GEOM_COLLECTION;
+        d.flags = E.pinned ? AM_BODY_PINNED : 0;
+        d.mass = E.mass;
+        d.minv = E.minv;
+        for (int i = 0; i < 9; i++) {
+            d.jinv0[i] = E.jinv0.m[i];
+            d.inertia0[i] = E.I0.m[i];
+        }
+        for (int i = 0; i < 24; i++) d.bbox[i] = E.bb[i];
+        d.friction = 0.8;
+        c->flags[e] = d.flags;
+        touched[e] = 1;
+    }
+};
+
+// RigidCollection.updateBB(body) (:612-650), bounding box temporarily in world
+// coordinates; skipped for planes.
+void update_bb_with(Ent& C, const Ent& B) {
+    if (B.plane) return;
+    // Double.MAX_VALUE, not infinity, as the reference initialises them
+    double mn[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
+    double mx[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
+    const Ent* two[2] = {&C, &B};
+    for (int s = 0; s < 2; s++) {
+        const Ent& b = *two[s];
+        for (int q = 0; q < 8; q++) {
+            V3 p = mulv(b.R, v3(&b.bb[q * 3]));
+            p = V3{p.x + b.x.x, p.y + b.x.y, p.z + b.x.z};
+            mn[0] = jmin_h(mn[0], p.x);
+            mn[1] = jmin_h(mn[1], p.y);
+            mn[2] = jmin_h(mn[2], p.z);
+            mx[0] = jmax_h(mx[0], p.x);
+            mx[1] = jmax_h(mx[1], p.y);
+            mx[2] = jmax_h(mx[2], p.z);
+        }
+    }
+    const double pts[8][3] = {{mx[0], mx[1], mx[2]}, {mx[0], mn[1], mn[2]}, {mn[0], mx[1], mn[2]},
+                              {mn[0], mn[1], mx[2]}, {mn[0], mn[1], mn[2]}, {mn[0], mx[1], mx[2]},
+                              {mx[0], mn[1], mx[2]}, {mx[0], mx[1], mn[2]}};
+    for (int q = 0; q < 8; q++)
+        for (int a = 0; a < 3; a++) C.bb[q * 3 + a] = pts[q][a];
+}
+
+// RigidCollection.addBodyInternalMethod (:477-514).
+void add_body_internal(Ent& C, const Ent& B) {
+    C.R = ident3();  // updateTheta: the collection frame is world-aligned
+    if (C.pinned) {
+        C.v = V3{0, 0, 0};
+        C.w = V3{0, 0, 0};
+        C.I = zero3();
+        C.I0 = zero3();
+        C.jinv = zero3();
+        C.jinv0 = zero3();
+        C.mass = 0;
+        C.minv = 0;
+        return;
+    }
+    V3 com = C.x;
+    com = V3{com.x * C.mass, com.y * C.mass, com.z * C.mass};
+    com = V3{B.mass * B.x.x + com.x, B.mass * B.x.y + com.y, B.mass * B.x.z + com.z};
+    double inv = 1. / (B.mass + C.mass);
+    com = V3{com.x * inv, com.y * inv, com.z * inv};
+    // updateVelocitiesFrom (:526-534)
+    V3 v = V3{C.v.x * C.mass, C.v.y * C.mass, C.v.z * C.mass};
+    v = V3{B.mass * B.v.x + v.x, B.mass * B.v.y + v.y, B.mass * B.v.z + v.z};
+    C.v = V3{v.x * inv, v.y * inv, v.z * inv};
+    V3 w = V3{C.w.x * C.mass, C.w.y * C.mass, C.w.z * C.mass};
+    w = V3{B.mass * B.w.x + w.x, B.mass * B.w.y + w.y, B.mass * B.w.z + w.z};
+    C.w = V3{w.x * inv, w.y * inv, w.z * inv};
+    // updateInertia (:791-803)
+    M3 M0 = zero3();
+    madd(M0, C.I);
+    madd(M0, get_op(C.x, C.mass, com));
+    madd(M0, B.I);
+    madd(M0, get_op(B.x, B.mass, com));
+    C.I = M0;
+    update_bb_with(C, B);
+    C.mass = C.mass + B.mass;
+    C.minv = inv;
+    C.x = com;
+    // bounding box back in collection coordinates, R^T (p - x)
+    for (int q = 0; q < 8; q++) {
+        double px = C.bb[q * 3] - C.x.x, py = C.bb[q * 3 + 1] - C.x.y, pz = C.bb[q * 3 + 2] - C.x.z;
+        const double* r = C.R.m;
+        C.bb[q * 3 + 0] = r[0] * px + r[3] * py + r[6] * pz;
+        C.bb[q * 3 + 1] = r[1] * px + r[4] * py + r[7] * pz;
+        C.bb[q * 3 + 2] = r[2] * px + r[5] * py + r[8] * pz;
+    }
+}
+
+// updateInertiaRestAndInvert (:674-680) and updateRotationalInertiaFromTransformation.
+void finish_inertia(Ent& C) {
+    if (C.pinned) return;
+    C.jinv = invert3(C.I);
+    C.I0 = rtmr(C.R, C.I);
+    C.jinv0 = rtmr(C.R, C.jinv);
+    C.I = rm0rt(C.R, C.I0);
+    C.jinv = rm0rt(C.R, C.jinv0);
+}
+
+
+// RigidTransform3D.multAinvB (:201-215): body-to-collection transform of
+// member b, t = R_c^T (x_b - x_c), R = R_c^T R_b.
+void body_to_collection(const Ent& C, V3 xb, const M3& Rb, double* out12) {
+    V3 t = V3{xb.x - C.x.x, xb.y - C.x.y, xb.z - C.x.z};
+    M3 Rt = transpose(C.R);
+    t = mulv(Rt, t);
+    M3 R = mmul(Rt, Rb);
+    for (int i = 0; i < 9; i++) out12[i] = R.m[i];
+    put(out12 + 9, t);
+}
+
+// RigidCollection.updateBodiesTransformations (:589-592).
+void update_bodies_transformations(Event& E, int ce, const Ent& C) {
+    am_ctx* c = E.c;
+    for (int b : E.members[ce]) body_to_collection(C, v3(&E.x[b * 3]), m3(&E.R[b * 9]), &c->b2c[(size_t)b * 12]);
+}
+
+void top_remove(am_ctx* c, int e) {
+    auto& t = c->slot_entity;
+    auto it = std::find(t.begin(), t.end(), e);
+    if (it != t.end()) t.erase(it);
+}
+
+int alloc_entity(Event& E) {
+    am_ctx* c = E.c;
+    if (c->free_entities.empty()) {
+        int old = c->n_ent, add = std::max(16, c->n_phys / 4);
+        int n = old + add;
+        am_body_desc blank;
+        std::memset(&blank, 0, sizeof blank);
+        blank.geom = AM_GEOM_COLLECTION;
+        blank.flags = AM_BODY_PINNED;
+        c->desc.resize(n, blank);
+        c->flags.resize(n, AM_BODY_PINNED);
+        c->parent.resize(n, -1);
+        c->slot.resize(n, -1);
+        c->sub.resize(n, 0);
+        c->b2c.resize((size_t)n * 12, 0.0);
+        E.x.resize((size_t)n * 3, 0.0);
+        E.v.resize((size_t)n * 3, 0.0);
+        E.w.resize((size_t)n * 3, 0.0);
+        E.R.resize((size_t)n * 9, 0.0);
+        E.jinv.resize((size_t)n * 9, 0.0);
+        for (int e = old; e < n; e++) E.R[(size_t)e * 9] = E.R[(size_t)e * 9 + 4] = E.R[(size_t)e * 9 + 8] = 1.0;
+        E.touched.resize(n, 0);
+        E.dv_zero.resize(n, 0);
+        E.members.resize(n);
+        for (int e = n - 1; e >= old; e--) c->free_entities.push_back(e);
+        c->n_ent = n;
+    }
+    int e = c->free_entities.back();
+    c->free_entities.pop_back();
+    E.members[e].clear();
+    return e;
+}
+
+void free_entity(Event& E, int e) {
+    am_ctx* c = E.c;
+    E.members[e].clear();
+    std::memset(&c->desc[e], 0, sizeof(am_body_desc));
+    c->desc[e].geom = AM_GEOM_COLLECTION;
+    c->desc[e].flags = AM_BODY_PINNED;
+    c->flags[e] = AM_BODY_PINNED;
+    c->parent[e] = -1;
+    E.touched[e] = 1;
+    c->free_entities.push_back(e);
+}
+
+// RigidCollection.addBody (:136-147).
+void add_body(Event& E, int ce, Ent& C, int b) {
+    am_ctx* c = E.c;
+    c->parent[b] = ce;
+    E.members[ce].push_back(b);
+    Ent B = E.load(b);
+    C.pinned = C.pinned || B.pinned;  // updateCollectionState (:459-470)
+    add_body_internal(C, B);
+    finish_inertia(C);
+    update_bodies_transformations(E, ce, C);
+}
+
+// RigidCollection.addBodies (:153-166).
+void add_bodies(Event& E, int ce, Ent& C, const std::vector<int>& list) {
+    am_ctx* c = E.c;
+    for (int b : list) {
+        c->parent[b] = ce;
+        E.members[ce].push_back(b);
+        Ent B = E.load(b);
+        C.pinned = C.pinned || B.pinned;
+        add_body_internal(C, B);
+    }
+    finish_inertia(C);
+    update_bodies_transformations(E, ce, C);
+}
+
+// RigidCollection.addCollection (:172-186): the other collection's members
+// join, its state is averaged in as one body.
+void add_collection(Event& E, int ce, Ent& C, int other) {
+    am_ctx* c = E.c;
+    for (int b : E.members[other]) {
+        c->parent[b] = ce;
+        E.members[ce].push_back(b);
+    }
+    Ent O = E.load(other);
+    C.pinned = C.pinned || O.pinned;
+    add_body_internal(C, O);
+    finish_inertia(C);
+    update_bodies_transformations(E, ce, C);
+}
+
+// new RigidCollection(body1, body2) (:55-76): starts as a copy of the first
+// body (RigidBody.set) unless that one is a plane.
+int new_collection(Event& E, int b1, int b2, Ent& C) {
+    am_ctx* c = E.c;
+    int ce = alloc_entity(E);
+    if (c->desc[b1].geom == AM_GEOM_PLANE) std::swap(b1, b2);
+    C = E.load(b1);
+    C.plane = false;
+    c->parent[b1] = ce;
+    E.members[ce].push_back(b1);
+    add_body(E, ce, C, b2);
+    return ce;
+}
+
+// RigidCollection.applyVelocitiesTo + unmergeBody (:925-951).
+void unmerge_body(Event& E, const Ent& C, int b) {
+    am_ctx* c = E.c;
+    V3 xb = v3(&E.x[b * 3]);
+    V3 r = V3{xb.x - C.x.x, xb.y - C.x.y, xb.z - C.x.z};
+    V3 wxr = V3{C.w.y * r.z - C.w.z * r.y, C.w.z * r.x - C.w.x * r.z, C.w.x * r.y - C.w.y * r.x};
+    put(&E.v[b * 3], V3{C.v.x + wxr.x, C.v.y + wxr.y, C.v.z + wxr.z});
+    put(&E.w[b * 3], C.w);
+    E.touched[b] = 1;
+    E.dv_zero[b] = 1;
+    c->parent[b] = -1;
+}
+
+// RigidCollection.updateBB() (:561-606): box of the members in collection
+// coordinates.
+void update_bb_members(Event& E, int ce, Ent& C) {
+    am_ctx* c = E.c;
+    double mn[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
+    double mx[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
+    for (int b : E.members[ce]) {
+        const am_body_desc& d = c->desc[b];
+        if (d.geom == AM_GEOM_PLANE) continue;
+        M3 Rbc = m3(&c->b2c[(size_t)b * 12]);
+        V3 tbc = v3(&c->b2c[(size_t)b * 12 + 9]);
+        for (int q = 0; q < 8; q++) {
+            V3 p = mulv(Rbc, v3(&d.bbox[q * 3]));
+            p = V3{p.x + tbc.x, p.y + tbc.y, p.z + tbc.z};
+            mn[0] = jmin_h(mn[0], p.x);
+            mn[1] = jmin_h(mn[1], p.y);
+            mn[2] = jmin_h(mn[2], p.z);
+            mx[0] = jmax_h(mx[0], p.x);
+            mx[1] = jmax_h(mx[1], p.y);
+            mx[2] = jmax_h(mx[2], p.z);
+        }
+    }
+    const double pts[8][3] = {{mx[0], mx[1], mx[2]}, {mx[0], mn[1], mn[2]}, {mn[0], mx[1], mn[2]},
+                              {mn[0], mn[1], mx[2]}, {mn[0], mn[1], mn[2]}, {mn[0], mx[1], mx[2]},
+                              {mx[0], mn[1], mx[2]}, {mx[0], mx[1], mn[2]}};
+    for (int q = 0; q < 8; q++)
+        for (int a = 0; a < 3; a++) C.bb[q * 3 + a] = pts[q][a];
+}
+
+// RigidCollection.removeBodies (:686-753).
+void remove_bodies(Event& E, int ce, Ent& C, const std::vector<int>& gone) {
+    am_ctx* c = E.c;
+    auto& mem = E.members[ce];
+    std::vector<int> keep;
+    for (int b : mem)
+        if (std::find(gone.begin(), gone.end(), b) == gone.end()) keep.push_back(b);
+    mem = keep;
+    bool was_pinned = C.pinned;
+    C.pinned = false;
+    for (int b : mem) C.pinned = C.pinned || ((c->flags[b] & AM_BODY_PINNED) != 0);
+    C.R = ident3();
+    if (C.pinned) {
+        C.v = V3{0, 0, 0};
+        C.w = V3{0, 0, 0};
+        C.I = zero3();
+        C.I0 = zero3();
+        C.jinv = zero3();
+        C.jinv0 = zero3();
+        C.mass = 0;
+        C.minv = 0;
+    } else if (was_pinned) {
+        C.x = V3{0, 0, 0};
+        C.mass = 0;
+        for (int b : mem) {
+            Ent B = E.load(b);
+            C.x = V3{B.mass * B.x.x + C.x.x, B.mass * B.x.y + C.x.y, B.mass * B.x.z + C.x.z};
+            C.mass = C.mass + B.mass;
+        }
+        C.minv = 1. / C.mass;
+        C.x = V3{C.x.x * C.minv, C.x.y * C.minv, C.x.z * C.minv};
+        // computeInertia (:766-774)
+        M3 I = zero3();
+        for (int b : mem) {
+            Ent B = E.load(b);
+            madd(I, B.I);
+            madd(I, get_op(B.x, B.mass, C.x));
+        }
+        C.I = I;
+        C.jinv = invert3(C.I);
+        C.I0 = rtmr(C.R, C.I);
+        C.jinv0 = rtmr(C.R, C.jinv);
+    } else {
+        for (int b : gone) {
+            Ent B = E.load(b);
+            V3 com = C.x;
+            com = V3{com.x * C.mass, com.y * C.mass, com.z * C.mass};
+            C.x = V3{-B.mass * B.x.x + com.x, -B.mass * B.x.y + com.y, -B.mass * B.x.z + com.z};
+            double s = 1. / C.mass;
+            com = V3{com.x * s, com.y * s, com.z * s};
+            C.mass = C.mass - B.mass;
+            C.minv = 1. / C.mass;
+            C.x = V3{C.x.x * C.minv, C.x.y * C.minv, C.x.z * C.minv};
+            // updateInertiaReverse (:779-789)
+            msub(C.I, B.I);
+            msub(C.I, get_op(C.x, C.mass, com));
+            msub(C.I, get_op(B.x, B.mass, com));
+        }
+        C.jinv = invert3(C.I);
+        C.I0 = rtmr(C.R, C.I);
+        C.jinv0 = rtmr(C.R, C.jinv);
+    }
+    if (!C.pinned) {
+        C.I = rm0rt(C.R, C.I0);
+        C.jinv = rm0rt(C.R, C.jinv0);
+    }
+    update_bodies_transformations(E, ce, C);
+    update_bb_members(E, ce, C);
+}
+
+int begin_event(am_ctx* c, Event& E) {
+    E.c = c;
+    size_t n = (size_t)c->n_ent;
+    E.x.resize(n * 3);
+    E.v.resize(n * 3);
+    E.w.resize(n * 3);
+    E.R.resize(n * 9);
+    E.jinv.resize(n * 9);
+    int r;
+    if ((r = pull_d(c, E.x.data(), c->d_x.p, n * 3))) return r;
+    if ((r = pull_d(c, E.v.data(), c->d_v.p, n * 3))) return r;
+    if ((r = pull_d(c, E.w.data(), c->d_w.p, n * 3))) return r;
+    if ((r = pull_d(c, E.R.data(), c->d_R.p, n * 9))) return r;
+    if ((r = pull_d(c, E.jinv.data(), c->d_jinv.p, n * 9))) return r;
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    E.touched.assign(n, 0);
+    E.dv_zero.assign(n, 0);
+    E.members.assign(n, std::vector<int>());
+    for (int e : c->slot_entity)
+        if (c->desc[e].geom == AM_GEOM_COLLECTION)
+            for (int q = 0; q < c->member_count[e]; q++) E.members[e].push_back(c->members[c->member_first[e] + q]);
+    return AM_OK;
+}
+
+// Flattens the structure, uploads what the event changed.
+int commit_event(am_ctx* c, Event& E) {
+    hipStream_t s = c->stream;
+    const int n = c->n_ent;
+    const int old_n = (int)(c->d_flags.n);
+    // per-entity device arrays sized by n_ent
+    if (c->d_x.n < (size_t)n * 3) {
+        int keep = old_n;
+        AM_HIP(c, grow_keep(s, c->d_f, n, 3, keep));
+        AM_HIP(c, grow_keep(s, c->d_t, n, 3, keep));
+        AM_HIP(c, grow_keep(s, c->d_dv, n, 6, keep));
+        AM_HIP(c, c->d_x.ensure((size_t)n * 3));
+        AM_HIP(c, c->d_v.ensure((size_t)n * 3));
+        AM_HIP(c, c->d_w.ensure((size_t)n * 3));
+        AM_HIP(c, c->d_R.ensure((size_t)n * 9));
+        AM_HIP(c, c->d_jinv.ensure((size_t)n * 9));
+        AM_HIP(c, c->d_minv.ensure(n));
+        // clear the new tail of the accumulators
+        fill_double(s, c->d_f.p + (size_t)keep * 3, (n - keep) * 3, 0.0);
+        fill_double(s, c->d_t.p + (size_t)keep * 3, (n - keep) * 3, 0.0);
+        fill_double(s, c->d_dv.p + (size_t)keep * 6, (n - keep) * 6, 0.0);
+    }
+    AM_HIP(c, hipMemcpyAsync(c->d_x.p, E.x.data(), (size_t)n * 24, hipMemcpyHostToDevice, s));
+    AM_HIP(c, hipMemcpyAsync(c->d_v.p, E.v.data(), (size_t)n * 24, hipMemcpyHostToDevice, s));
+    AM_HIP(c, hipMemcpyAsync(c->d_w.p, E.w.data(), (size_t)n * 24, hipMemcpyHostToDevice, s));
+    AM_HIP(c, hipMemcpyAsync(c->d_R.p, E.R.data(), (size_t)n * 72, hipMemcpyHostToDevice, s));
+    AM_HIP(c, hipMemcpyAsync(c->d_jinv.p, E.jinv.data(), (size_t)n * 72, hipMemcpyHostToDevice, s));
+    std::vector<double> minv(n);
+    for (int i = 0; i < n; i++) minv[i] = (c->flags[i] & AM_BODY_PINNED) ? 0.0 : c->desc[i].minv;
+    AM_HIP(c, upload(c, c->d_minv, minv));
+    AM_HIP(c, upload(c, c->d_desc, c->desc));
+    AM_HIP(c, upload(c, c->d_b2c, c->b2c));
+    std::vector<int> dz;
+    for (int i = 0; i < n; i++)
+        if (E.dv_zero[i]) dz.push_back(i);
+    if (!dz.empty()) {
+        DBuf<int> d_dz;
+        AM_HIP(c, upload(c, d_dz, dz));
+        launch_zero_dv(s, bodies_view(c), d_dz.p, (int)dz.size());
+        AM_HIP(c, hipStreamSynchronize(s));
+    }
+    // structure: top list order, members in RigidCollection.bodies order
+    c->member_first.assign(n, 0);
+    c->member_count.assign(n, 0);
+    c->members.clear();
+    c->slot.assign(n, -1);
+    c->sub.assign(n, 0);
+    for (int e = 0; e < n; e++)
+        if (c->desc[e].geom != AM_GEOM_COLLECTION) c->parent[e] = c->parent[e];
+    for (int s_ = 0; s_ < (int)c->slot_entity.size(); s_++) {
+        int e = c->slot_entity[s_];
+        c->slot[e] = s_;
+        c->parent[e] = -1;
+        if (c->desc[e].geom == AM_GEOM_COLLECTION) {
+            c->member_first[e] = (int)c->members.size();
+            c->member_count[e] = (int)E.members[e].size();
+            int q = 0;
+            for (int b : E.members[e]) {
+                c->members.push_back(b);
+                c->parent[b] = e;
+                c->slot[b] = s_;
+                c->sub[b] = q++;
+            }
+        }
+    }
+    return rebuild_structure(c);
+}
+
+}  // namespace
+
+MergeParams merge_params(const am_params& p) {
+    MergeParams P;
+    P.step_accum_merging = p.step_accum_merging;
+    P.step_accum_unmerging = p.step_accum_unmerging;
+    P.merge_pinned = p.enable_merge_pinned;
+    P.stable_contact = p.enable_merge_stable_contact;
+    P.let_it_breathe = p.enable_merge_let_it_breathe;
+    P.unmerge_friction = p.enable_unmerge_friction;
+    P.unmerge_normal = p.enable_unmerge_normal;
+    P.threshold_merge = p.threshold_merge;
+    P.threshold_unmerge = p.threshold_unmerge;
+    P.threshold_breath = p.threshold_breath;
+    return P;
+}
+
+bool has_collections(const am_ctx* c) { return !c->member_list.empty(); }
+
+hipError_t ContactStore::ensure(int want, int keep, hipStream_t s) {
+    if (want <= cap && b1.p) return hipSuccess;
+    int nc = std::max(want + want / 4, 256);
+    size_t k = (size_t)std::min(keep, cap);
+    hipError_t e;
+#define AM_GROW(buf, stride) \
+    if ((e = grow_keep(s, buf, nc, stride, k)) != hipSuccess) return e;
+    AM_GROW(b1, 1) AM_GROW(b2, 1) AM_GROW(bv1, 1) AM_GROW(bv2, 1) AM_GROW(info, 1) AM_GROW(state, 1)
+    AM_GROW(is_new, 1) AM_GROW(pB1, 3) AM_GROW(nB1, 3) AM_GROW(t1B1, 3) AM_GROW(t2B1, 3) AM_GROW(viol, 1)
+    AM_GROW(prev, 1) AM_GROW(J, 36) AM_GROW(lam, 3) AM_GROW(lamw, 3) AM_GROW(b, 3) AM_GROW(D, 3)
+#undef AM_GROW
+    cap = nc;
+    return hipSuccess;
+}
+
+void ContactStore::swap(ContactStore& o) {
+#define AM_SW(buf)                \
+    std::swap(buf.p, o.buf.p);    \
+    std::swap(buf.n, o.buf.n);
+    AM_SW(b1) AM_SW(b2) AM_SW(bv1) AM_SW(bv2) AM_SW(info) AM_SW(state) AM_SW(is_new) AM_SW(pB1) AM_SW(nB1)
+    AM_SW(t1B1) AM_SW(t2B1) AM_SW(viol) AM_SW(prev) AM_SW(J) AM_SW(lam) AM_SW(lamw) AM_SW(b) AM_SW(D)
+#undef AM_SW
+    std::swap(cap, o.cap);
+}
+
+int merge_push_down(am_ctx* c, bool integrated_only) {
+    launch_push_down(c->stream, bodies_view(c), c->d_member_list.p, (int)c->member_list.size(),
+                     integrated_only ? 1 : 0);
+    AM_HIP(c, hipGetLastError());
+    return AM_OK;
+}
+
+int merge_member_transforms(am_ctx* c) {
+    launch_member_transforms(c->stream, bodies_view(c), c->d_b2c.p, c->d_member_list.p, (int)c->member_list.size());
+    AM_HIP(c, hipGetLastError());
+    return AM_OK;
+}
+
+// CollisionProcessor.updateInCollections (:232-303): one PGS sweep over the
+// external and the internal contacts with sub-body Jacobians, then the members
+// advance their velocities and the top-level deltaV is cleared.  The sweep
+// runs in colour order over physical body pairs (am_solve_order_pull gives
+// the order right after this call).
+int merge_update_in_collections(am_ctx* c, double dt, int* iters) {
+    *iters = 0;
+    const am_params& P = c->params;
+    if (!P.update_contacts_in_collections || !has_collections(c)) return AM_OK;
+    hipStream_t s = c->stream;
+    const int n_ext = c->n_contacts, n_int = c->n_ic;
+    int r = ensure_contact_capacity(c, n_ext + n_int, n_ext);
+    if (r) return r;
+    launch_gather_contacts(s, contacts_view(c), n_ext, internal_view(c), nullptr, 0, n_int, 0);
+    c->n_contacts = n_ext + n_int;
+    fill_int(s, c->d_scalars.p + 1, 1, c->n_contacts);
+    if (c->n_contacts > 0) {
+        launch_groups(s, c->n_contacts, c->d_scalars.p + 1, c->c_b1.p, c->c_b2.p, nullptr, nullptr, c->g_flags.p,
+                      c->g_scanned.p, c->g_start.p, c->g_a.p, c->g_b.p, c->d_scalars.p + 2, c->d_scan_temp.p);
+        AM_HIP(c, hipGetLastError());
+        if ((r = read_scalars(c, 2, 1))) return r;
+        c->n_groups = c->h_scalars[2];
+        c->layout_valid = false;
+        if ((r = do_solve(c, dt, 0, false, iters, nullptr))) return r;
+        launch_gather_contacts(s, internal_view(c), 0, contacts_view(c), nullptr, n_ext, n_int, 0);
+        AM_HIP(c, c->sweep_order.ensure(c->n_contacts));
+        AM_HIP(c, hipMemcpyAsync(c->sweep_order.p, c->s_order.p, (size_t)c->n_contacts * 4, hipMemcpyDeviceToDevice,
+                                 s));
+        c->sweep_order_n = c->n_contacts;
+    }
+    c->n_contacts = n_ext;
+    fill_int(s, c->d_scalars.p + 1, 1, n_ext);
+    if ((r = groups_after_contacts(c))) return r;
+    launch_advance(s, bodies_view(c), c->d_member_adv.p, (int)c->member_adv.size(), dt, false);
+    launch_zero_dv(s, bodies_view(c), c->d_slot_entity.p, (int)c->slot_entity.size());
+    AM_HIP(c, hipGetLastError());
+    return AM_OK;
+}
+
+int merge_accumulate_unmerging(am_ctx* c) {
+    if (c->ibpc.empty()) return AM_OK;
+    launch_ibpc_accumulate(c->stream, bodies_view(c), c->d_ibpc.p, (int)c->ibpc.size(), internal_view(c),
+                           merge_params(c->params));
+    AM_HIP(c, hipGetLastError());
+    return AM_OK;
+}
+
+// clearBodyPairContacts + the BodyPairContact bookkeeping of
+// updateBodyPairContacts (CollisionProcessor.java:145-226): the external pair
+// table of this step, with histories carried over from the previous one.
+int merge_body_pairs(am_ctx* c) {
+    hipStream_t s = c->stream;
+    std::swap(c->d_bpc.p, c->d_bpc_prev.p);
+    std::swap(c->d_bpc.n, c->d_bpc_prev.n);
+    c->n_bpc_prev = c->n_bpc;
+    int mask = 0;
+    if (c->n_bpc_prev > 0) {
+        int ts = 16;
+        while (ts < 2 * c->n_bpc_prev) ts *= 2;
+        mask = ts - 1;
+        AM_HIP(c, c->d_bpc_hash.ensure(ts));
+        fill_int(s, c->d_bpc_hash.p, ts, -1);
+        launch_bpc_hash(s, c->d_bpc_prev.p, c->n_bpc_prev, c->d_bpc_hash.p, mask);
+    }
+    c->n_bpc = 0;
+    const int n = c->n_contacts;
+    if (n == 0) return AM_OK;
+    AM_HIP(c, c->d_bpc.ensure(n + 1));
+    int* sc = c->d_scalars.p;
+    launch_groups(s, n, sc + 1, c->c_b1.p, c->c_b2.p, nullptr, nullptr, c->bp_flags.p, c->bp_scanned.p,
+                  c->bp_start.p, nullptr, nullptr, sc + 11, c->d_scan_temp.p);
+    launch_bpc_build(s, contacts_view(c), sc + 1, c->bp_start.p, n, c->d_active.p, c->bp_keep.p, c->bp_keep_scan.p,
+                     sc + 10, c->d_scan_temp.p, c->d_bpc_prev.p, c->n_bpc_prev, c->d_bpc_hash.p, mask, c->d_bpc.p,
+                     c->params.threshold_breath);
+    AM_HIP(c, hipGetLastError());
+    int r = read_scalars(c, 10, 1);
+    if (r) return r;
+    c->n_bpc = c->h_scalars[10];
+    return AM_OK;
+}
+
+int merge_accumulate_merging(am_ctx* c) {
+    launch_bpc_accumulate(c->stream, bodies_view(c), c->d_bpc.p, c->n_bpc, merge_params(c->params));
+    AM_HIP(c, hipGetLastError());
+    return AM_OK;
+}
+
+namespace {
+
+bool same_collection(const am_ctx* c, int a, int b) { return c->parent[a] >= 0 && c->parent[a] == c->parent[b]; }
+
+// Uploads the internal pair records (after the host changed them).
+int upload_ibpc(am_ctx* c) {
+    for (auto& r : c->ibpc) r.coll = c->parent[r.b1];
+    AM_HIP(c, upload(c, c->d_ibpc, c->ibpc));
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    return AM_OK;
+}
+
+}  // namespace
+
+// Merging.merge (Merging.java:73-163).
+int merge_merge(am_ctx* c, int* n_merged) {
+    *n_merged = 0;
+    if (!c->params.enable_merging || c->n_bpc == 0) return AM_OK;
+    hipStream_t s = c->stream;
+    std::vector<BpcRec> T(c->n_bpc);
+    AM_HIP(c, hipMemcpyAsync(T.data(), c->d_bpc.p, T.size() * sizeof(BpcRec), hipMemcpyDeviceToHost, s));
+    AM_HIP(c, hipStreamSynchronize(s));
+    bool any = false;
+    for (const auto& r : T) any = any || r.flag;
+    if (!any) return AM_OK;
+    std::vector<unsigned char> act(c->n_contacts);
+    AM_HIP(c, hipMemcpyAsync(act.data(), c->d_active.p, act.size(), hipMemcpyDeviceToHost, s));
+    Event E;
+    int r = begin_event(c, E);
+    if (r) return r;
+    // bodyPairContacts of each physical body (external pairs of this step)
+    std::vector<std::vector<int>> adj(c->n_phys);
+    for (int i = 0; i < (int)T.size(); i++) {
+        adj[T[i].b1].push_back(i);
+        if (T[i].b2 != T[i].b1) adj[T[i].b2].push_back(i);
+    }
+    std::vector<char> internal(T.size(), 0);
+    std::vector<int> newly;
+    auto internalize = [&](int i) {
+        internal[i] = 1;
+        newly.push_back(i);
+    };
+    // addIncompleteContacts (RigidCollection.java:1003-1015)
+    auto add_incomplete = [&](int body) {
+        for (int i : adj[body])
+            if (!internal[i] && same_collection(c, T[i].b1, T[i].b2)) internalize(i);
+    };
+    std::vector<Ent> cstate(c->n_ent);
+    std::vector<char> cloaded(c->n_ent, 0);
+    auto coll = [&](int e) -> Ent& {
+        if ((int)cstate.size() < c->n_ent) {
+            cstate.resize(c->n_ent);
+            cloaded.resize(c->n_ent, 0);
+        }
+        if (!cloaded[e]) {
+            cstate[e] = E.load(e);
+            cloaded[e] = 1;
+        }
+        return cstate[e];
+    };
+    auto save = [&](int e) { E.store_collection(e, coll(e)); };
+    for (int i = 0; i < (int)T.size(); i++) {
+        const BpcRec& b = T[i];
+        if (internal[i] || !b.flag) continue;
+        if (same_collection(c, b.b1, b.b2)) continue;
+        c->merging_event = true;
+        (*n_merged)++;
+        internalize(i);
+        int p1 = c->parent[b.b1], p2 = c->parent[b.b2];
+        if (p1 < 0 && p2 < 0) {
+            top_remove(c, b.b1);
+            top_remove(c, b.b2);
+            Ent C;
+            int ce = new_collection(E, b.b1, b.b2, C);
+            if ((int)cstate.size() < c->n_ent) {
+                cstate.resize(c->n_ent);
+                cloaded.resize(c->n_ent, 0);
+            }
+            cstate[ce] = C;
+            cloaded[ce] = 1;
+            save(ce);
+            c->slot_entity.push_back(ce);
+        } else if (p1 >= 0 && p2 >= 0) {
+            int keep = p1, gone = p2;
+            if (!(E.members[p1].size() > E.members[p2].size())) std::swap(keep, gone);
+            top_remove(c, gone);
+            std::vector<int> gone_bodies = E.members[gone];
+            // the absorbed collection's own state is averaged in as one body
+            Ent& K = coll(keep);
+            add_collection(E, keep, K, gone);
+            save(keep);
+            for (int body : gone_bodies) add_incomplete(body);
+            free_entity(E, gone);
+            cloaded[gone] = 0;
+        } else {
+            int host = p1 >= 0 ? p1 : p2, body = p1 >= 0 ? b.b2 : b.b1;
+            top_remove(c, body);
+            Ent& K = coll(host);
+            add_body(E, host, K, body);
+            save(host);
+            add_incomplete(body);
+        }
+    }
+    // RigidCollection.addToInternalContact (:840-850): copies of the active
+    // contacts of every pair that became internal, appended in that order
+    std::vector<int> idx;
+    for (int i : newly) {
+        IbpcRec ib;
+        ib.b1 = T[i].b1;
+        ib.b2 = T[i].b2;
+        ib.first = c->n_ic + (int)idx.size();
+        ib.count = 0;
+        for (int k = T[i].first; k < T[i].end; k++)
+            if (act[k]) {
+                idx.push_back(k);
+                ib.count++;
+            }
+        ib.hist_n = 0;
+        ib.flag = 0;
+        ib.coll = -1;
+        ib.pad = 0;
+        c->ibpc.push_back(ib);
+        T[i].b1 = -1;  // no longer an external pair
+    }
+    if (!idx.empty()) {
+        AM_HIP(c, c->ic.ensure(c->n_ic + (int)idx.size(), c->n_ic, s));
+        DBuf<int> d_idx;
+        AM_HIP(c, upload(c, d_idx, idx));
+        launch_gather_contacts(s, internal_view(c), c->n_ic, contacts_view(c), d_idx.p, 0, (int)idx.size(), 1);
+        AM_HIP(c, hipStreamSynchronize(s));
+        c->n_ic += (int)idx.size();
+    }
+    AM_HIP(c, hipMemcpyAsync(c->d_bpc.p, T.data(), T.size() * sizeof(BpcRec), hipMemcpyHostToDevice, s));
+    if ((r = commit_event(c, E))) return r;
+    return upload_ibpc(c);
+}
+
+// Merging.unmerge (Merging.java:215-273) and unmergeSelectedBpcs (:286-374).
+// Contacts of pairs that leave their collection are appended to the current
+// contact list with lambda_warm = lambda.
+int merge_unmerge(am_ctx* c, double dt, int* n_events) {
+    (void)dt;
+    *n_events = 0;
+    const am_params& P = c->params;
+    if (!P.enable_unmerging || c->ibpc.empty()) return AM_OK;
+    if (!P.enable_unmerge_relative_motion && !P.enable_unmerge_normal && !P.enable_unmerge_friction) return AM_OK;
+    hipStream_t s = c->stream;
+    AM_HIP(c, hipMemcpyAsync(c->ibpc.data(), c->d_ibpc.p, c->ibpc.size() * sizeof(IbpcRec), hipMemcpyDeviceToHost,
+                             s));
+    AM_HIP(c, hipStreamSynchronize(s));
+    bool any = false;
+    for (const auto& r : c->ibpc) any = any || r.flag;
+    if (!any) return AM_OK;
+    Event E;
+    int r = begin_event(c, E);
+    if (r) return r;
+    const int nib = (int)c->ibpc.size();
+    std::vector<char> incoll(nib, 1);
+    std::vector<int> reexpose;  // internal pairs going back to the external list, in order
+    std::vector<int> additions, removals;
+    // internal pairs of each body (BodyPairContact lists of the members)
+    std::vector<std::vector<int>> adj(c->n_phys);
+    for (int i = 0; i < nib; i++) {
+        adj[c->ibpc[i].b1].push_back(i);
+        adj[c->ibpc[i].b2].push_back(i);
+    }
+    const std::vector<int> top0 = c->slot_entity;
+    for (int ce : top0) {
+        if (c->desc[ce].geom != AM_GEOM_COLLECTION || (c->flags[ce] & AM_BODY_SLEEPING)) continue;
+        std::vector<int> cut;
+        for (int i = 0; i < nib; i++)
+            if (c->ibpc[i].coll == ce && c->ibpc[i].flag) cut.push_back(i);
+        if (cut.empty()) continue;
+        for (int i : cut) incoll[i] = 0;
+        Ent C = E.load(ce);
+        const std::vector<int> bodies = E.members[ce];
+        std::vector<char> handled(c->n_phys, 0), remained(c->n_phys, 0);
+        std::vector<int> new_bodies;
+        bool remove_coll = true;
+        for (int body : bodies) {
+            if (handled[body]) continue;
+            // buildNeighborBody (:381-393), depth first over pairs still inside
+            std::vector<int> sub{body};
+            std::vector<char> in_sub(c->n_phys, 0);
+            in_sub[body] = 1;
+            std::vector<int> stack{body};
+            std::vector<int> order;
+            // recursive order: visit neighbours in pair order, descending immediately
+            std::function<void(int)> dfs = [&](int x) {
+                for (int i : adj[x]) {
+                    if (!incoll[i]) continue;
+                    int o = c->ibpc[i].b1 == x ? c->ibpc[i].b2 : c->ibpc[i].b1;
+                    if (!in_sub[o] && !handled[o]) {
+                        in_sub[o] = 1;
+                        sub.push_back(o);
+                        dfs(o);
+                    }
+                }
+            };
+            dfs(body);
+            for (int b : sub) handled[b] = 1;
+            if (bodies.size() != sub.size()) {
+                if (sub.size() < bodies.size() / 2 + 1) {
+                    for (int b : sub) unmerge_body(E, C, b);
+                    if (sub.size() > 1) {
+                        Ent N;
+                        int ne = new_collection(E, sub[0], sub[1], N);
+                        std::vector<int> rest(sub.begin() + 2, sub.end());
+                        if (!rest.empty()) add_bodies(E, ne, N, rest);
+                        E.store_collection(ne, N);
+                        new_bodies.push_back(ne);
+                    } else {
+                        new_bodies.push_back(sub[0]);
+                    }
+                } else {
+                    remove_coll = false;
+                    for (int b : sub) remained[b] = 1;
+                }
+            } else {
+                remove_coll = false;
+                for (int b : sub) remained[b] = 1;
+                break;
+            }
+        }
+        // reconnect (:334-354)
+        for (int i : cut) {
+            if (same_collection(c, c->ibpc[i].b1, c->ibpc[i].b2)) incoll[i] = 1;
+            else reexpose.push_back(i);
+        }
+        if (!remove_coll) {
+            std::vector<int> gone;
+            for (int b : bodies)
+                if (handled[b] && !remained[b]) gone.push_back(b);
+            if (!gone.empty()) {
+                remove_bodies(E, ce, C, gone);
+                E.store_collection(ce, C);
+            }
+        }
+        if (!new_bodies.empty()) {
+            c->merging_event = true;
+            (*n_events)++;
+            for (int e : new_bodies) additions.push_back(e);
+            if (remove_coll) removals.push_back(ce);
+        }
+    }
+    for (int e : additions) c->slot_entity.push_back(e);
+    for (int e : removals) {
+        top_remove(c, e);
+        free_entity(E, e);
+    }
+    // pairs whose bodies no longer share a collection go back to the contact list
+    std::vector<int> ext_idx;
+    for (int i : reexpose)
+        for (int k = 0; k < c->ibpc[i].count; k++) ext_idx.push_back(c->ibpc[i].first + k);
+    if (!ext_idx.empty()) {
+        const int n0 = c->n_contacts, m = (int)ext_idx.size();
+        if ((r = ensure_contact_capacity(c, n0 + m, n0))) return r;
+        DBuf<int> d_idx;
+        AM_HIP(c, upload(c, d_idx, ext_idx));
+        launch_gather_contacts(s, contacts_view(c), n0, internal_view(c), d_idx.p, 0, m, 2);
+        AM_HIP(c, hipStreamSynchronize(s));
+        c->n_contacts = n0 + m;
+        fill_int(s, c->d_scalars.p + 1, 1, c->n_contacts);
+    }
+    // compact the internal pairs still inside a collection
+    std::vector<IbpcRec> kept;
+    std::vector<int> keep_idx;
+    for (int i = 0; i < nib; i++) {
+        if (!incoll[i]) continue;
+        IbpcRec ib = c->ibpc[i];
+        int first = (int)keep_idx.size();
+        for (int k = 0; k < ib.count; k++) keep_idx.push_back(ib.first + k);
+        ib.first = first;
+        kept.push_back(ib);
+    }
+    if ((int)keep_idx.size() != c->n_ic) {
+        AM_HIP(c, c->ic_tmp.ensure((int)keep_idx.size(), 0, s));
+        if (!keep_idx.empty()) {
+            DBuf<int> d_idx;
+            AM_HIP(c, upload(c, d_idx, keep_idx));
+            launch_gather_contacts(s, c->ic_tmp.view(), 0, internal_view(c), d_idx.p, 0, (int)keep_idx.size(), 0);
+            AM_HIP(c, hipStreamSynchronize(s));
+        }
+        c->ic.swap(c->ic_tmp);
+        c->n_ic = (int)keep_idx.size();
+    }
+    c->ibpc = kept;
+    if ((r = commit_event(c, E))) return r;
+    if ((r = upload_ibpc(c))) return r;
+    return groups_after_contacts(c);
+}
+
+}  // namespace am_rt

@@ -32,7 +32,9 @@ void fill_double(hipStream_t s, double* p, int n, double v) {
 }
 
 // ---------------------------------------------------------------------------
-// Groups: runs of contacts with the same unordered slot pair.
+// Groups: runs of contacts with the same unordered slot pair (solve bodies are
+// the top-level entities), or with the same unordered physical body pair when
+// slot and parent are null (the single sweep inside collections).
 __global__ void k_group_flags(int cap, const int* n_contacts, const int* cb1, const int* cb2, const int* slot,
                               int* flags) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -44,8 +46,9 @@ __global__ void k_group_flags(int cap, const int* n_contacts, const int* cb1, co
     }
     int f = 1;
     if (k > 0) {
-        int a = slot[cb1[k]], b = slot[cb2[k]];
-        int pa = slot[cb1[k - 1]], pb = slot[cb2[k - 1]];
+        // slot == nullptr: physical body pairs (computeInCollection)
+        int a = slot ? slot[cb1[k]] : cb1[k], b = slot ? slot[cb2[k]] : cb2[k];
+        int pa = slot ? slot[cb1[k - 1]] : cb1[k - 1], pb = slot ? slot[cb2[k - 1]] : cb2[k - 1];
         int lo = min(a, b), hi = max(a, b), plo = min(pa, pb), phi = max(pa, pb);
         f = (lo != plo || hi != phi) ? 1 : 0;
     }
@@ -61,8 +64,8 @@ __global__ void k_group_build(const int* n_contacts, const int* flags, const int
         int g = scanned[k];
         group_start[g] = k;
         int a = cb1[k], b = cb2[k];
-        group_a[g] = parent[a] >= 0 ? parent[a] : a;
-        group_b[g] = parent[b] >= 0 ? parent[b] : b;
+        if (group_a) group_a[g] = (parent && parent[a] >= 0) ? parent[a] : a;
+        if (group_b) group_b[g] = (parent && parent[b] >= 0) ? parent[b] : b;
     }
     if (k == n - 1) group_start[scanned[k] + flags[k]] = n;
 }
@@ -534,6 +537,9 @@ AMD_INLINE bool grid_barrier(const PgsArgs& P, unsigned& epoch) {
                 bail = 1;
                 break;
             }
+            // another workgroup gave up: leave now instead of at our own limit
+            if ((spins & 1023) == 0 && __hip_atomic_load(P.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                break;
             __builtin_amdgcn_s_sleep(1);
         }
         if (__hip_atomic_load(P.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) bail = 1;
@@ -928,7 +934,7 @@ __global__ void k_advance_positions(BodiesView B, const int* targets, int n, dou
         R.m[8] = R.m[0] * R.m[4] - R.m[1] * R.m[3];
         stm(&B.R[i * 9], R);
     }
-    if (B.desc[i].geom != AM_GEOM_COLLECTION) stm(&B.jinv[i * 9], rm0rt(R, ldm(B.desc[i].jinv0)));
+    stm(&B.jinv[i * 9], rm0rt(R, ldm(B.desc[i].jinv0)));
 }
 
 void launch_advance(hipStream_t s, BodiesView B, const int* targets, int n, double dt, bool positions) {
@@ -937,7 +943,7 @@ void launch_advance(hipStream_t s, BodiesView B, const int* targets, int n, doub
     else hipLaunchKernelGGL(k_advance_velocities, dim3((n + 255) / 256), dim3(256), 0, s, B, targets, n, dt);
 }
 
-// jinv = R jinv0 R^T for non-pinned non-collection entities, zero for pinned.
+// jinv = R jinv0 R^T for non-pinned entities, zero for pinned.
 __global__ void k_update_jinv(BodiesView B) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= B.n) return;
@@ -946,7 +952,6 @@ __global__ void k_update_jinv(BodiesView B) {
         for (int a = 0; a < 9; a++) B.jinv[i * 9 + a] = 0.0;
         return;
     }
-    if (B.desc[i].geom == AM_GEOM_COLLECTION) return;
     stm(&B.jinv[i * 9], rm0rt(ldm(&B.R[i * 9]), ldm(B.desc[i].jinv0)));
 }
 void launch_update_jinv(hipStream_t s, BodiesView B) {
@@ -955,36 +960,11 @@ void launch_update_jinv(hipStream_t s, BodiesView B) {
 
 // ---------------------------------------------------------------------------
 // MotionMetricProcessor.getLargestVelocityNorm (MotionMetricProcessor.java:39-73)
-AMD_INLINE d3 spatial_velocity(const BodiesView& B, int b, d3 pW) {
-    d3 r = sub(pW, ld3(&B.x[b * 3]));
-    return add(cross(ld3(&B.w[b * 3]), r), ld3(&B.v[b * 3]));
-}
 __global__ void k_metric(BodiesView B, const int* pairs, int n, double* out) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     int a = pairs[k * 2], b = pairs[k * 2 + 1];
-    double largest = 0;
-    if (b < 0) {
-        m33 R = ldm(&B.R[a * 9]);
-        d3 x = ld3(&B.x[a * 3]);
-        for (int q = 0; q < 8; q++) {
-            d3 pW = xform(R, x, ld3(&B.desc[a].bbox[q * 3]));
-            largest = jmax(length(spatial_velocity(B, a, pW)), largest);
-        }
-    } else {
-        for (int s = 0; s < 2; s++) {
-            int body = s == 0 ? a : b;
-            m33 R = ldm(&B.R[body * 9]);
-            d3 x = ld3(&B.x[body * 3]);
-            for (int q = 0; q < 8; q++) {
-                d3 pW = xform(R, x, ld3(&B.desc[body].bbox[q * 3]));
-                d3 v1 = spatial_velocity(B, a, pW);
-                d3 v2 = spatial_velocity(B, b, pW);
-                largest = jmax(length(sub(v1, v2)), largest);
-            }
-        }
-    }
-    out[k] = largest;
+    out[k] = b < 0 ? metric_one(B, a) : metric_two(B, a, b);
 }
 void launch_metric(hipStream_t s, BodiesView B, const int* pairs, int n, double* out) {
     if (n > 0) hipLaunchKernelGGL(k_metric, dim3((n + 127) / 128), dim3(128), 0, s, B, pairs, n, out);

@@ -93,6 +93,7 @@ class Scene:
             rec["mass"] = m
             rec["minv"] = 1.0 / m
             rec["jinv0"] = [1.0 / I[0], 0, 0, 0, 1.0 / I[1], 0, 0, 0, 1.0 / I[2]]
+            rec["inertia0"] = [I[0], 0, 0, 0, I[1], 0, 0, 0, I[2]]
         return self._append(rec, x, R, v, w, name)
 
     def add_plane(self, p, n, friction=0.8, restitution=0.0, scene=0, name=None) -> int:
@@ -123,6 +124,7 @@ class Scene:
         rec["mass"] = m
         rec["minv"] = 1.0 / m
         rec["jinv0"] = [1.0 / Ic, 0, 0, 0, 1.0 / Ic, 0, 0, 0, 1.0 / Ic]
+        rec["inertia0"] = [Ic, 0, 0, 0, Ic, 0, 0, 0, Ic]
         rec["friction"] = friction
         rec["restitution"] = restitution
         bb = []
@@ -171,6 +173,7 @@ class Scene:
         rec["mass"] = mass
         rec["minv"] = 1.0 / mass
         rec["jinv0"] = [1.0 / inertia[0], 0, 0, 0, 1.0 / inertia[1], 0, 0, 0, 1.0 / inertia[2]]
+        rec["inertia0"] = [inertia[0], 0, 0, 0, inertia[1], 0, 0, 0, inertia[2]]
         rec["friction"] = friction
         rec["restitution"] = restitution
         if bbox is None:

@@ -58,7 +58,7 @@ class PGS:
         ctx = self.ctx
         self._apply(ctx, restitutionOverride, restitutionOverrideVal, frictionOverride, frictionOverrideVal)
         if self.computeInCollection:
-            return ctx.single_sweep(dt)
+            return ctx.update_in_collections(dt)
         it, _ = ctx.solve(dt, self.postStabilization)
         self.changes = list(ctx.changes())
         return it
@@ -149,18 +149,87 @@ class CollisionProcessor:
         return it
 
     def clearBodyPairContacts(self):
-        """clearBodyPairContacts (:213-226): only merge bookkeeping uses it."""
+        """clearBodyPairContacts (:213-226) and the body-pair histories."""
+        self.ctx.clear_body_pair_contacts()
+
+    def updateInCollections(self, dt: float, mergeParams=None):
+        """updateInCollections (:232-303): one sweep inside collections."""
+        t0 = time.perf_counter()
+        p = self.ctx.params
+        p.iterations_in_collection = self.iterationsInCollection
+        p.feedback_stiffness = self.feedbackStiffness
+        p.enable_compliance = int(self.enableCompliance)
+        p.compliance = self.compliance
+        self.ctx.set_params()
+        self.ctx.update_in_collections(dt)
+        self.collectionUpdateTime = time.perf_counter() - t0
+
+
+class Merging:
+    """m3d/Merging.java: merge / unmerge over the context's body pairs, with
+    the MergeParameters names and defaults (:31-48)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.mergingEvent = False
+        self.enableMerging = True
+        self.enableMergePinned = True
+        self.enableMergeStableContactCondition = True
+        self.enableMergeLetItBreathe = True
+        self.enableUnmerging = True
+        self.enableUnmergeFrictionCondition = True
+        self.enableUnmergeNormalCondition = True
+        self.enableUnmergeRelativeMotionCondition = True
+        self.updateContactsInCollections = True
+        self.stepAccumMerging = 3
+        self.stepAccumUnmerging = 3
+        self.stepsBetweenMergeEvents = 10
+        self.thresholdMerge = 1e-2
+        self.thresholdUnmerge = 2e-2
+        self.thresholdBreath = 1e-5
+
+    def apply(self):
+        p = self.ctx.params
+        p.enable_merging = int(self.enableMerging)
+        p.enable_merge_pinned = int(self.enableMergePinned)
+        p.enable_merge_stable_contact = int(self.enableMergeStableContactCondition)
+        p.enable_merge_let_it_breathe = int(self.enableMergeLetItBreathe)
+        p.enable_unmerging = int(self.enableUnmerging)
+        p.enable_unmerge_friction = int(self.enableUnmergeFrictionCondition)
+        p.enable_unmerge_normal = int(self.enableUnmergeNormalCondition)
+        p.enable_unmerge_relative_motion = int(self.enableUnmergeRelativeMotionCondition)
+        p.update_contacts_in_collections = int(self.updateContactsInCollections)
+        p.step_accum_merging = self.stepAccumMerging
+        p.step_accum_unmerging = self.stepAccumUnmerging
+        p.steps_between_merge_events = self.stepsBetweenMergeEvents
+        p.threshold_merge = self.thresholdMerge
+        p.threshold_unmerge = self.thresholdUnmerge
+        p.threshold_breath = self.thresholdBreath
+        self.ctx.set_params()
+
+    def merge(self) -> int:
+        """Merging.merge (:73-163)."""
+        n = self.ctx.merge()
+        self.mergingEvent = self.mergingEvent or n > 0
+        return n
+
+    def unmerge(self, dt: float) -> int:
+        """Merging.unmerge (:215-273)."""
+        n = self.ctx.unmerge(dt)
+        self.mergingEvent = self.mergingEvent or n > 0
+        return n
 
 
 class RigidBodySystem:
     """m3d/RigidBodySystem.java: the per-step phase order of advanceTime
-    (:102-185) with merging and sleeping disabled."""
+    (:102-185); sleeping is not modelled."""
 
     def __init__(self, scene, device: int = 0):
         self.scene = scene
         self.ctx = Context(device)
         self.ctx.upload(scene)
         self.collision = CollisionProcessor(self.ctx)
+        self.merging = Merging(self.ctx)
         self.useGravity = True
         self.gravityAmount = 1.0
         self.gravityAngle = 90.0
@@ -186,12 +255,19 @@ class RigidBodySystem:
         """advanceTime (:102-185)."""
         t0 = time.perf_counter()
         self.totalSteps += 1
+        self.merging.apply()
         self.ctx.clear_bodies()
         self.applyExternalForces(dt)
         self.collision.updateContactsMap()
         self.collision.collisionDetection(dt)
         self.collision.updateBodyPairContacts()
         self.collision.warmStart(False)
+        if self.merging.updateContactsInCollections:
+            self.collision.updateInCollections(dt)
+        self.ctx.accumulate_for_unmerging()
+        self.merging.unmerge(dt)
+        if self.merging.mergingEvent:
+            self.ctx.clear_top_level()
         if self.collision.useWarmStart:
             self.collision.redoWarmStart()
         else:
@@ -199,7 +275,10 @@ class RigidBodySystem:
         self.collision.solveLCP(dt, False)
         self.collision.clearBodyPairContacts()
         self.ctx.advance_velocities(dt)
+        self.ctx.accumulate_for_merging()
         self.ctx.advance_positions(dt)
+        if self.totalSteps % self.merging.stepsBetweenMergeEvents == 0:
+            self.merging.merge()
         self.computeTime = time.perf_counter() - t0
         self.simulationTime += dt
 

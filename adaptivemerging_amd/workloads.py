@@ -210,3 +210,17 @@ def batched_piles(seed: int = 0, n_scenes: int = 256, bodies=(50, 2000), box_fra
             else:
                 s.add_sphere(rng.uniform(0.05, 0.25), c, density=1000.0, friction=0.5, scene=sc)
     return s
+
+
+def merge_stack(seed: int = 0, layers: int = 4, density: float = 500.0) -> Scene:
+    """A short stack of boards and a lone cube on a ground plane: contacts
+    that settle within a few steps, so body pairs merge (adaptive-merging
+    scenes such as scenes3D/jenga.xml with merging on)."""
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    s.add_plane([0, 0, 0], [0, 1, 0])
+    for k in range(layers):
+        jitter = rng.uniform(-1e-3, 1e-3, size=3)
+        s.add_box([1.0, 0.2, 0.6], [0.02 * k + jitter[0], 0.1 + 0.2 * k, jitter[2]], density=density)
+    s.add_box([0.3, 0.3, 0.3], [2.0, 0.15, 0.0], density=density)
+    return s

@@ -7,9 +7,16 @@ A "step" = clear, gravity, contacts map, broad + narrow phase, warm start,
 coloured FP64 PGS to residual 1e-6 (iteration cap --iterations), velocity and
 position update, all on the GPU with body state resident in HBM.
 
-Prints one JSON line: contact-updates/s (N_contacts x sweeps / solve time),
-ms/step, the PGS kernel's fraction of the HBM roofline, and the CPU oracle
-timed on this host on a bounded sample of the same workload.
+Two contexts replay the same warm-up (the GPU path is deterministic).  The
+second collects converged lambdas as training targets, trains the warm-start
+MLP on them, and then runs the timed steps with the learned start (falling
+back to the plain start whenever its start residual is worse); the first runs
+them with the plain warm start.
+
+Prints one JSON line: contact-updates/s (N_contacts x sweeps / solve time) and
+ms/step of the plain run as the headline, both runs' figures under config,
+the PGS kernel's fraction of the HBM roofline, and the CPU oracle timed on
+this host on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -92,6 +99,45 @@ def run_cpu_baseline(scene, state, params_kw, dt, budget_s: float):
     }
 
 
+def run_steps(ctx, steps: int, dt: float):
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stats.append(ctx.step(dt))
+    return stats, time.perf_counter() - t0
+
+
+def summarize(stats, wall_s: float, n_dyn: int, tolerance: float) -> dict:
+    n_c = np.array([s["n_contacts"] for s in stats], dtype=np.float64)
+    sweeps = np.array([s["iterations_used"] for s in stats], dtype=np.float64)
+    solve_ms = np.array([s["ms_solve"] for s in stats])
+    pgs_ms = np.array([s["ms_pgs_kernel"] for s in stats])
+    updates = float(np.sum(n_c * sweeps))
+    bytes_pgs = float(sum(algorithmic_bytes_per_sweep(int(s["n_contacts"]), n_dyn) * s["iterations_used"]
+                          for s in stats))
+    return {
+        "contact_updates_per_s": updates / (float(np.sum(solve_ms)) * 1e-3) if np.sum(solve_ms) > 0 else 0.0,
+        "ms_per_step": wall_s * 1e3 / max(len(stats), 1),
+        "contacts_mean": float(np.mean(n_c)),
+        "sweeps_mean": float(np.mean(sweeps)),
+        "residual_final_mean": float(np.mean([s["last_change"] for s in stats])),
+        "residual_reached_frac": float(np.mean([s["last_change"] < tolerance for s in stats])),
+        "colours_mean": float(np.mean([s["n_colours"] for s in stats])),
+        "ms_detect": float(np.mean([s["ms_detect"] for s in stats])),
+        "ms_warm_start": float(np.mean([s["ms_warm_start"] for s in stats])),
+        "ms_learned_start": float(np.mean([s["ms_learned"] for s in stats])),
+        "ms_solve": float(np.mean(solve_ms)),
+        "ms_integrate": float(np.mean([s["ms_integrate"] for s in stats])),
+        "ms_step_events": float(np.mean([s["ms_total"] for s in stats])),
+        "learned_used_frac": float(np.mean([s["used_learned"] for s in stats])),
+        "start_residual_plain": float(np.mean([s["start_residual_plain"] for s in stats])),
+        "start_residual_learned": float(np.mean([s["start_residual_learned"] for s in stats])),
+        "_pgs_bytes": bytes_pgs,
+        "_pgs_ms": float(np.sum(pgs_ms)),
+        "_pgs_launches": len(stats),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +148,10 @@ def main() -> int:
     ap.add_argument("--tolerance", type=float, default=1e-6)
     ap.add_argument("--dt", type=float, default=0.01)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--learned", choices=["on", "off"], default="on",
+                    help="also run the learned warm start from the same state")
+    ap.add_argument("--train-steps", type=int, default=40, help="warm-up steps whose solves feed the training set")
+    ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--pgs-grid", type=int, default=0)
@@ -110,7 +160,7 @@ def main() -> int:
 
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards by scene across GPUs, see DESIGN.md)", file=sys.stderr)
+              "(independent scenes would shard across GPUs; one GPU is measured here)", file=sys.stderr)
         return 2
 
     from adaptivemerging_amd import workloads
@@ -120,50 +170,95 @@ def main() -> int:
     scene = workloads.box_pile(seed=args.seed, n=args.bodies)
     b, _, _, _, _, _ = scene.arrays()
     n_dyn = int(np.sum((b["flags"] & 1) == 0))
+    # the pile workload runs without merging (its bodies never settle into
+    # stable stacks within the measured steps)
     params_kw = dict(iterations=args.iterations, tolerance=args.tolerance, gravity_amount=9.81,
-                     pgs_grid=args.pgs_grid, pgs_block=args.pgs_block)
-    ctx = Context()
-    ctx.upload(scene)
-    ctx.set_params(**params_kw)
+                     pgs_grid=args.pgs_grid, pgs_block=args.pgs_block, enable_merging=0, enable_unmerging=0)
+    learned = args.learned == "on"
+    plain = Context()
+    plain.upload(scene)
+    plain.set_params(**params_kw)
+    twin = None
+    if learned:
+        # the twin replays the same warm-up (the GPU path is deterministic),
+        # collects converged lambdas as training targets, then switches to
+        # the learned start for the timed steps
+        twin = Context()
+        twin.upload(scene)
+        twin.set_params(**params_kw)
     print(f"# scene built: {scene.n} bodies in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
 
     for k in range(args.warmup):
-        st = ctx.step(args.dt)
+        if twin is not None and k == args.warmup - args.train_steps:
+            twin.set_params(collect_training=1)
+        st = plain.step(args.dt)
+        if twin is not None:
+            twin.step(args.dt)
         if k % 20 == 0 or k == args.warmup - 1:
             print(f"# warmup {k}: contacts {st['n_contacts']} sweeps {st['iterations_used']} "
-                  f"colours {st['n_colours']} step {st['ms_total']:.2f} ms", file=sys.stderr, flush=True)
+                  f"residual {st['last_change']:.3g} colours {st['n_colours']} step {st['ms_total']:.2f} ms",
+                  file=sys.stderr, flush=True)
 
-    stats = []
-    t_start = time.perf_counter()
-    for k in range(args.steps):
-        stats.append(ctx.step(args.dt))
-    t_total = time.perf_counter() - t_start
-    ms_per_step = t_total * 1e3 / max(args.steps, 1)
+    learned_info = None
+    if twin is not None:
+        sa, sb = plain.pull_state(), twin.pull_state()
+        identical = all(np.array_equal(a, c) for a, c in zip(sa, sb))
+        twin.set_params(collect_training=0)
+        n_train = twin.train_size()
+        t1 = time.perf_counter()
+        twin.mlp_init(args.seed)
+        losses = [twin.train_epochs(1, 256, 1e-3, args.seed + e) for e in range(args.epochs)]
+        train_s = time.perf_counter() - t1
+        twin.train_clear()
+        twin.set_params(learned_warm_start=1)
+        learned_info = dict(replay_identical=identical, train_samples=n_train, epochs=args.epochs,
+                            train_s=train_s, loss_first=losses[0] if losses else None,
+                            loss_last=losses[-1] if losses else None)
+        print(f"# trained on {n_train} contacts in {train_s:.1f} s, loss {losses[0]:.4g} -> {losses[-1]:.4g}; "
+              f"replay identical: {identical}", file=sys.stderr, flush=True)
 
-    n_c = np.array([s["n_contacts"] for s in stats], dtype=np.float64)
-    sweeps = np.array([s["iterations_used"] for s in stats], dtype=np.float64)
-    solve_ms = np.array([s["ms_solve"] for s in stats])
-    pgs_ms = np.array([s["ms_pgs_kernel"] for s in stats])
-    updates = float(np.sum(n_c * sweeps))
-    value = updates / (float(np.sum(solve_ms)) * 1e-3) if np.sum(solve_ms) > 0 else 0.0
-    bytes_pgs = float(sum(algorithmic_bytes_per_sweep(int(s["n_contacts"]), n_dyn) * s["iterations_used"]
-                          for s in stats))
-    achieved = bytes_pgs / (float(np.sum(pgs_ms)) * 1e-3) / 1e9 if np.sum(pgs_ms) > 0 else 0.0
-    reached = float(np.mean([s["last_change"] < args.tolerance for s in stats]))
+    stats_p, wall_p = run_steps(plain, args.steps, args.dt)
+    res_p = summarize(stats_p, wall_p, n_dyn, args.tolerance)
+    print(f"# plain: {res_p['ms_per_step']:.2f} ms/step, sweeps {res_p['sweeps_mean']:.0f}, "
+          f"residual {res_p['residual_final_mean']:.3g}", file=sys.stderr, flush=True)
+    res_l = None
+    if twin is not None:
+        stats_l, wall_l = run_steps(twin, args.steps, args.dt)
+        res_l = summarize(stats_l, wall_l, n_dyn, args.tolerance)
+        res_l.update(learned_info)
+        print(f"# learned: {res_l['ms_per_step']:.2f} ms/step, sweeps {res_l['sweeps_mean']:.0f}, "
+              f"residual {res_l['residual_final_mean']:.3g}, used {res_l['learned_used_frac']:.2f}",
+              file=sys.stderr, flush=True)
 
     cpu = None
     if args.cpu_baseline == "on":
-        state = ctx.pull_state()
+        state = plain.pull_state()
         cpu = run_cpu_baseline(scene, state, params_kw, args.dt, args.cpu_budget)
 
+    achieved = res_p["_pgs_bytes"] / (res_p["_pgs_ms"] * 1e-3) / 1e9 if res_p["_pgs_ms"] > 0 else 0.0
+    roofline = {
+        "bound": "hbm",
+        "kernel": "am::k_pgs<false> (persistent coloured PGS)",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": None,
+        "bytes_per_launch": res_p["_pgs_bytes"] / max(res_p["_pgs_launches"], 1),
+        "ms_per_launch": res_p["_pgs_ms"] / max(res_p["_pgs_launches"], 1),
+    }
+    for r in (res_p, res_l):
+        if r is not None:
+            for k in [k for k in r if k.startswith("_")]:
+                del r[k]
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": res_p["contact_updates_per_s"],
         "unit": "contact-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": res_p["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -172,34 +267,19 @@ def main() -> int:
         "config": {
             "workload": f"{args.bodies}-box pile, edges U(0.1,0.5) m, density log-uniform 100-10000 kg/m^3, "
                         f"4x4 m bin (floor + 4 walls), dt {args.dt}, g 9.81, PGS to residual {args.tolerance} "
-                        f"(cap {args.iterations}), plain warm start",
+                        f"(cap {args.iterations} sweeps); value and ms_per_step: plain warm start",
             "bodies": scene.n,
-            "contacts_mean": float(np.mean(n_c)),
-            "sweeps_mean": float(np.mean(sweeps)),
-            "residual_reached_frac": reached,
-            "colours_mean": float(np.mean([s["n_colours"] for s in stats])),
-            "ms_detect": float(np.mean([s["ms_detect"] for s in stats])),
-            "ms_warm_start": float(np.mean([s["ms_warm_start"] for s in stats])),
-            "ms_solve": float(np.mean(solve_ms)),
-            "ms_integrate": float(np.mean([s["ms_integrate"] for s in stats])),
-            "ms_step_events": float(np.mean([s["ms_total"] for s in stats])),
-            "warm_start": "plain",
+            "plain": res_p,
+            "learned": res_l,
             "parallelism": "none (1 GPU)",
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "am::k_pgs<false> (persistent coloured PGS)",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "bytes_per_launch": bytes_pgs / max(len(stats), 1),
-            "ms_per_launch": float(np.mean(pgs_ms)),
-        },
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    plain.close()
+    if twin is not None:
+        twin.close()
     return 0
 
 

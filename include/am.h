@@ -87,6 +87,7 @@ typedef struct am_body_desc {
     double plane_p[3];   /* PLANE: point */
     double plane_d;      /* PLANE: -(p.n) */
     double bbox[24];     /* boundingBoxB: 8 body-frame points, reference order */
+    double inertia0[9];  /* massAngular0: rest-pose rotational inertia (0 when pinned) */
 } am_body_desc;
 
 /* One BVSphere node.  Children of a node are stored contiguously; indices are
@@ -194,8 +195,26 @@ typedef struct am_step_stats {
     double start_residual_plain;   /* start residual of the plain warm start */
     double start_residual_learned; /* start residual of the predicted start (0 if unused) */
     int32_t used_learned;     /* 1 if the solve started from the predicted lambdas */
+    int32_t n_collections;    /* collections after the step */
+    int32_t n_merged;         /* body pairs merged by this step (Merging.merge) */
+    int32_t n_unmerged;       /* collections cut by this step (Merging.unmerge) */
+    int32_t n_internal_contacts; /* contacts inside collections */
     int32_t reserved2;
+    double ms_merging;        /* single sweep, unmerge, pair histories and merge */
 } am_step_stats;
+
+/* One body pair (BodyPairContact) as am_body_pairs_pull reports it. */
+typedef struct am_body_pair {
+    int32_t body1, body2;     /* physical bodies */
+    int32_t in_collection;    /* BodyPairContact.inCollection */
+    int32_t n_contacts;       /* active contacts (external) or internal contacts */
+    int32_t hist_n;           /* motionMetricHist size */
+    int32_t state_n;          /* contactStateHist size (external pairs) */
+    int32_t flag;             /* merge (external) or unmerge (internal) condition */
+    int32_t collection;       /* owning collection entity (internal pairs), else -1 */
+    double hist[8];           /* motionMetricHist, oldest first (external pairs) */
+    int32_t states[8];        /* contactStateHist, oldest first (external pairs) */
+} am_body_pair;
 
 typedef struct am_ctx am_ctx;
 
@@ -218,19 +237,15 @@ int am_scene_upload(am_ctx* ctx, const am_body_desc* bodies, int32_t n_bodies,
 int am_state_push(am_ctx* ctx, const double* x, const double* R, const double* v,
                   const double* w, int32_t n);
 int am_state_pull(am_ctx* ctx, double* x, double* R, double* v, double* w, int32_t n);
-/* Merged-body structure (RigidCollection).  parent[i] is the collection
- * entity of entity i or -1; top[0..n_top) is the broad-phase list
- * (RigidBodySystem.bodies order, CollisionProcessor.java:683-697); the
- * members of collection e are members[member_first[e] .. +member_count[e]) in
- * RigidCollection.bodies order (narrowPhaseCollection, :768-791). */
-int am_collections_set(am_ctx* ctx, const int32_t* parent, const int32_t* top, int32_t n_top,
-                       const int32_t* member_first, const int32_t* member_count, const int32_t* members,
-                       int32_t n_members);
-/* Mass properties and flags of one entity (a collection after
- * RigidCollection.addBodyInternalMethod / updateInertiaRestAndInvert,
- * RigidCollection.java:477-514, 674-680); jinv is its world inverse inertia. */
-int am_entity_mass_set(am_ctx* ctx, int32_t entity, double mass, double minv, const double* jinv,
-                       int32_t flags);
+/* Merged-body structure.  Entities are the physical bodies followed by
+ * collection slots; n_entities grows when merging needs more slots.
+ * parent[e] is the collection of entity e or -1; top[0..n_top) is the
+ * broad-phase list (RigidBodySystem.bodies order).  Either array may be NULL. */
+int am_structure_pull(am_ctx* ctx, int32_t* parent, int32_t cap_entities, int32_t* top, int32_t cap_top,
+                      int32_t* n_entities, int32_t* n_top);
+/* Descriptions of entities [0, n): for collections the mass properties that
+ * RigidCollection computed (mass, minv, jinv0, inertia0, bbox, pinned). */
+int am_entities_pull(am_ctx* ctx, am_body_desc* out, int32_t n);
 int am_jinv_pull(am_ctx* ctx, double* jinv, int32_t n);
 /* Force/torque accumulators (RigidBody.force/torque). */
 int am_forces_push(am_ctx* ctx, const double* force, const double* torque, int32_t n);
@@ -265,19 +280,48 @@ int am_solve(am_ctx* ctx, double dt, int32_t post_stabilization, int32_t* iterat
 int am_solve_order_pull(am_ctx* ctx, int32_t* order, int32_t capacity, int32_t* n);
 /* Colour of every contact in the current layout (64 = sequential tail). */
 int am_colours_pull(am_ctx* ctx, int32_t* colour_of_contact, int32_t capacity, int32_t* n_colours);
-/* The single PGS sweep of updateInCollections (CollisionProcessor.java:232-303)
- * over the current contact list with computeInCollection semantics: sub-body
- * Jacobians, no early exit, contact states updated. */
-int am_single_sweep(am_ctx* ctx, double dt, int32_t* iterations_used);
+/* CollisionProcessor.updateInCollections (CollisionProcessor.java:232-303):
+ * one PGS sweep over the external and internal contacts with sub-body
+ * Jacobians (computeInCollection), in colour order over physical body pairs
+ * (am_solve_order_pull gives it right after the call); then the members of
+ * collections advance their velocities and top-level deltaV is cleared. */
+int am_update_in_collections(am_ctx* ctx, double dt, int32_t* iterations_used);
 /* PGS.changes: log(max |delta lambda|) per sweep of the last am_solve. */
 int am_changes_pull(am_ctx* ctx, double* changes, int32_t capacity, int32_t* n);
 
 /* ---- integration (RigidBody.advanceVelocities / advancePositions) ----- */
 int am_advance_velocities(am_ctx* ctx, double dt);
 int am_advance_positions(am_ctx* ctx, double dt);
-/* One advanceTime-equivalent step without merging: clear, gravity, contacts
- * map, detection, warm start, solve, velocity and position update. */
+/* One RigidBodySystem.advanceTime (RigidBodySystem.java:102-185): clear,
+ * gravity, contacts map, detection, warm start, the single sweep inside
+ * collections, unmerging, the main solve, body-pair histories, velocity and
+ * position update, and every steps_between_merge_events steps merging.  The
+ * merging phases run only when enable_merging / enable_unmerging are set or
+ * collections exist. */
 int am_step(am_ctx* ctx, double dt, am_step_stats* stats);
+
+/* ---- adaptive merging (Merging, BodyPairContact, RigidCollection) ------
+ * The phases of RigidBodySystem.advanceTime (RigidBodySystem.java:102-185)
+ * that am_step runs when merging is on; callers that drive the phases one by
+ * one call them in this order. */
+/* RigidBodySystem.accumulateForUnmerging (:318-329) */
+int am_accumulate_for_unmerging(am_ctx* ctx);
+/* Merging.unmerge (Merging.java:215-273); n_cut = collections cut */
+int am_unmerge(am_ctx* ctx, double dt, int32_t* n_cut);
+/* RigidBody.clear of the top-level entities and the external forces again,
+ * as advanceTime does after a merging event (RigidBodySystem.java:138-142) */
+int am_clear_top_level(am_ctx* ctx);
+/* CollisionProcessor.clearBodyPairContacts (:213-226) with the body-pair
+ * bookkeeping of updateBodyPairContacts (:145-206) */
+int am_clear_body_pair_contacts(am_ctx* ctx);
+/* RigidBodySystem.accumulateForMerging (:331-334) */
+int am_accumulate_for_merging(am_ctx* ctx);
+/* Merging.merge (Merging.java:73-163); n_merged = body pairs merged */
+int am_merge(am_ctx* ctx, int32_t* n_merged);
+/* External pairs of the last am_clear_body_pair_contacts, then internal pairs. */
+int am_body_pairs_pull(am_ctx* ctx, am_body_pair* out, int32_t capacity, int32_t* n_external, int32_t* n_internal);
+/* RigidCollection.internalContacts of every collection, pair by pair. */
+int am_internal_contacts_pull(am_ctx* ctx, am_contact_arrays* out, int32_t capacity, int32_t* n);
 
 /* ---- learned warm start (no reference counterpart) --------------------
  * A per-contact MLP 24 -> 64 -> 64 -> 64 -> 3 (ReLU), BF16 weights on the

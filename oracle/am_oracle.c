@@ -198,9 +198,33 @@ static int sleeping(const orc_world* w, int32_t i) { return (w->desc[i].flags & 
 /* RigidBody.updateRotationalInertiaFromTransformation (RigidBody.java:311-321) */
 static void update_jinv(orc_world* w, int32_t i) {
     if (pinned(w, i)) return;
-    if (w->desc[i].geom == AM_GEOM_COLLECTION) return; /* collection jinv is set by its builder */
     rm0rt(&w->jinv[i * 9], &w->R[i * 9], w->desc[i].jinv0);
 }
+
+/* Grows the entity arrays to n (new entities: unused collection slots,
+ * pinned, identity rotation).  Collections of a merging run live there. */
+void orc_resize(orc_world* w, int32_t n) {
+    if (n <= w->n) return;
+    size_t o = (size_t)w->n, m = (size_t)n;
+#define GROW(ptr, T, k)                                             \
+    ptr = (T*)realloc(ptr, m * (k) * sizeof(T));                    \
+    memset(ptr + o * (k), 0, (m - o) * (k) * sizeof(T));
+    GROW(w->desc, am_body_desc, 1)
+    GROW(w->x, double, 3) GROW(w->R, double, 9) GROW(w->v, double, 3) GROW(w->w, double, 3)
+    GROW(w->f, double, 3) GROW(w->t, double, 3) GROW(w->jinv, double, 9) GROW(w->dv, double, 6)
+    GROW(w->parent, int32_t, 1) GROW(w->mem_first, int32_t, 1) GROW(w->mem_count, int32_t, 1)
+#undef GROW
+    for (size_t i = o; i < m; i++) {
+        w->desc[i].geom = AM_GEOM_COLLECTION;
+        w->desc[i].flags = AM_BODY_PINNED;
+        w->R[i * 9 + 0] = w->R[i * 9 + 4] = w->R[i * 9 + 8] = 1.0;
+        w->parent[i] = -1;
+    }
+    w->n = n;
+}
+void orc_set_entity(orc_world* w, int32_t e, const am_body_desc* d) { w->desc[e] = *d; }
+void orc_get_entity(const orc_world* w, int32_t e, am_body_desc* d) { *d = w->desc[e]; }
+int32_t orc_entities(const orc_world* w) { return w->n; }
 
 void orc_set_state(orc_world* w, const double* x, const double* R, const double* v, const double* om) {
     for (int32_t i = 0; i < w->n; i++) {
@@ -230,6 +254,7 @@ void orc_get_forces(const orc_world* w, double* f, double* t) {
     if (t) memcpy(t, w->t, (size_t)w->n * 24);
 }
 void orc_get_deltav(const orc_world* w, double* dv) { memcpy(dv, w->dv, (size_t)w->n * 48); }
+void orc_set_deltav(orc_world* w, const double* dv) { memcpy(w->dv, dv, (size_t)w->n * 48); }
 
 /* RigidBody.clear (RigidBody.java:226-230) for every entity */
 void orc_clear_bodies(orc_world* w) {
@@ -875,6 +900,34 @@ void orc_contacts_set(orc_world* w, int32_t n, const int32_t* body1, const int32
     for (int32_t k = 0; k < n; k++)
         emit(w, body1[k], body2[k], &pw[k * 3], &nw[k * 3], bv1 ? bv1[k] : AM_BV_NONE,
              bv2 ? bv2[k] : AM_BV_NONE, info ? info[k] : 0, viol[k]);
+}
+
+/* Installs contacts field by field (the internal contacts of collections
+ * joining the list of the single sweep, or contacts coming back from it). */
+void orc_contacts_set_raw(orc_world* w, int32_t n, const am_contact_arrays* in) {
+    w->nc = 0;
+    for (int32_t k = 0; k < n; k++) {
+        push_contact_slot(w);
+        orc_contact* c = &w->c[w->nc++];
+        c->b1 = in->body1[k];
+        c->b2 = in->body2[k];
+        c->bv1 = in->bv1[k];
+        c->bv2 = in->bv2[k];
+        c->info = in->info[k];
+        memcpy(c->pB1, &in->point_b1[k * 3], 24);
+        memcpy(c->nB1, &in->normal_b1[k * 3], 24);
+        memcpy(c->t1B1, &in->tangent1_b1[k * 3], 24);
+        memcpy(c->t2B1, &in->tangent2_b1[k * 3], 24);
+        c->viol = in->violation[k];
+        c->prev_viol = in->prev_violation[k];
+        memcpy(c->J, &in->jacobian[k * 36], 36 * 8);
+        memcpy(c->lam, &in->lambda[k * 3], 24);
+        memcpy(c->lamw, &in->lambda_warm[k * 3], 24);
+        memcpy(c->b, &in->b[k * 3], 24);
+        memcpy(c->D, &in->diag[k * 3], 24);
+        c->state = in->state[k];
+        c->is_new = in->new_this_step[k];
+    }
 }
 
 int32_t orc_contacts_get(const orc_world* w, am_contact_arrays* o, int32_t cap) {

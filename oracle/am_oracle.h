@@ -53,6 +53,12 @@ void orc_set_structure(orc_world* w, const int32_t* parent, const int32_t* top, 
                        const int32_t* member_first, const int32_t* member_count,
                        const int32_t* members);
 
+/* Grows the entity count (new entities are unused collection slots). */
+void orc_resize(orc_world* w, int32_t n_entities);
+int32_t orc_entities(const orc_world* w);
+void orc_set_entity(orc_world* w, int32_t entity, const am_body_desc* desc);
+void orc_get_entity(const orc_world* w, int32_t entity, am_body_desc* desc);
+
 void orc_set_state(orc_world* w, const double* x, const double* R, const double* v,
                    const double* omega);
 void orc_get_state(const orc_world* w, double* x, double* R, double* v, double* omega);
@@ -61,6 +67,7 @@ void orc_get_jinv(const orc_world* w, double* jinv);
 void orc_set_forces(orc_world* w, const double* force, const double* torque);
 void orc_get_forces(const orc_world* w, double* force, double* torque);
 void orc_get_deltav(const orc_world* w, double* dv);
+void orc_set_deltav(orc_world* w, const double* dv);
 void orc_clear_bodies(orc_world* w);
 void orc_apply_gravity(orc_world* w);
 
@@ -71,6 +78,7 @@ void orc_contacts_set(orc_world* w, int32_t n, const int32_t* body1, const int32
                       const double* point_w, const double* normal_w, const double* violation,
                       const int32_t* info, const int32_t* bv1, const int32_t* bv2);
 int32_t orc_contacts_get(const orc_world* w, am_contact_arrays* out, int32_t capacity);
+void orc_contacts_set_raw(orc_world* w, int32_t n, const am_contact_arrays* in);
 void orc_lambdas_set(orc_world* w, const double* lambda, int32_t n);
 
 /* Primitive narrow-phase entry points on a body pair (results appended to the

@@ -11,7 +11,7 @@ import subprocess
 
 import numpy as np
 
-from adaptivemerging_amd._lib import ContactArrays, Params, default_params_host, ptr  # noqa: F401
+from adaptivemerging_amd._lib import BODY_DTYPE, ContactArrays, Params, default_params_host, ptr  # noqa: F401
 from adaptivemerging_amd.context import contact_arrays
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -46,6 +46,12 @@ def load() -> C.CDLL:
         "orc_set_forces": (None, [vp, vp, vp]),
         "orc_get_forces": (None, [vp, vp, vp]),
         "orc_get_deltav": (None, [vp, vp]),
+        "orc_set_deltav": (None, [vp, vp]),
+        "orc_resize": (None, [vp, i32]),
+        "orc_entities": (i32, [vp]),
+        "orc_set_entity": (None, [vp, i32, vp]),
+        "orc_get_entity": (None, [vp, i32, vp]),
+        "orc_contacts_set_raw": (None, [vp, i32, C.POINTER(ContactArrays)]),
         "orc_clear_bodies": (None, [vp]),
         "orc_apply_gravity": (None, [vp]),
         "orc_update_contacts_map": (None, [vp]),
@@ -116,15 +122,44 @@ class Oracle:
         j = np.ascontiguousarray(jinv, dtype=np.float64)
         self.lib.orc_set_jinv(self.h, entity, ptr(j))
 
-    def pull_state(self):
-        x = np.zeros((self.n, 3)); R = np.zeros((self.n, 9)); v = np.zeros((self.n, 3)); w = np.zeros((self.n, 3))
+    @property
+    def n_entities(self) -> int:
+        return self.lib.orc_entities(self.h)
+
+    def pull_state(self, n: int | None = None):
+        """Pose and velocity of the first n entities (default: physical bodies)."""
+        m = self.n_entities
+        x = np.zeros((m, 3)); R = np.zeros((m, 9)); v = np.zeros((m, 3)); w = np.zeros((m, 3))
         self.lib.orc_get_state(self.h, ptr(x), ptr(R), ptr(v), ptr(w))
-        return x, R, v, w
+        n = self.n if n is None else n
+        return x[:n], R[:n], v[:n], w[:n]
 
     def pull_jinv(self):
-        j = np.zeros((self.n, 9))
+        j = np.zeros((self.n_entities, 9))
         self.lib.orc_get_jinv(self.h, ptr(j))
         return j
+
+    def resize(self, n: int):
+        self.lib.orc_resize(self.h, n)
+
+    def set_entity(self, e: int, rec):
+        r = np.ascontiguousarray(np.asarray(rec, dtype=BODY_DTYPE).reshape(()))
+        self.lib.orc_set_entity(self.h, e, ptr(r))
+
+    def entity(self, e: int):
+        r = np.zeros((), dtype=BODY_DTYPE)
+        self.lib.orc_get_entity(self.h, e, ptr(r))
+        return r
+
+    def push_deltav(self, dv):
+        d = np.ascontiguousarray(dv, dtype=np.float64)
+        self.lib.orc_set_deltav(self.h, ptr(d))
+
+    def push_contacts_raw(self, c: dict):
+        c = {k: np.ascontiguousarray(v) for k, v in c.items()}
+        self._raw = c
+        ca = contact_arrays(c)
+        self.lib.orc_contacts_set_raw(self.h, len(c["body1"]), C.byref(ca))
 
     def push_forces(self, force=None, torque=None):
         f = None if force is None else np.ascontiguousarray(force, dtype=np.float64)
@@ -132,12 +167,12 @@ class Oracle:
         self.lib.orc_set_forces(self.h, ptr(f), ptr(t))
 
     def pull_forces(self):
-        f = np.zeros((self.n, 3)); t = np.zeros((self.n, 3))
+        f = np.zeros((self.n_entities, 3)); t = np.zeros((self.n_entities, 3))
         self.lib.orc_get_forces(self.h, ptr(f), ptr(t))
         return f, t
 
     def pull_deltav(self):
-        d = np.zeros((self.n, 6))
+        d = np.zeros((self.n_entities, 6))
         self.lib.orc_get_deltav(self.h, ptr(d))
         return d
 

@@ -66,7 +66,8 @@ def test_training_reduces_loss(gpu_lib):
     scene = workloads.jenga_tower(seed=3, layers=8, overlap=0.001)
     ctx = Context()
     ctx.upload(scene)
-    ctx.set_params(iterations=200, tolerance=1e-8, gravity_amount=9.81, collect_training=1)
+    ctx.set_params(iterations=200, tolerance=1e-8, gravity_amount=9.81, collect_training=1, enable_merging=0,
+                   enable_unmerging=0)
     init_params(ctx)
     for _ in range(10):
         ctx.step(0.01)
