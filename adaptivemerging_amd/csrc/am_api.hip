@@ -746,9 +746,15 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
         c->pgs_max_grid = pgs_max_blocks(block, &ok);
         if (!ok) c->pgs_max_grid = 1;
     }
-    // groups of a colour are dealt block-cyclically; ~16 groups per workgroup
-    // spreads a colour over enough CUs without making the barrier expensive
-    int grid = P.pgs_grid > 0 ? P.pgs_grid : (c->max_colour_groups + 15) / 16;
+    // groups of a colour are dealt block-cyclically; enough workgroups that a
+    // workgroup's share of the largest colour (at the mean group size) fills
+    // about 60% of its LDS stage
+    int grid = P.pgs_grid;
+    if (grid <= 0) {
+        double per_group = c->n_groups > 0 ? (double)c->n_contacts / c->n_groups : 1.0;
+        double want = c->max_colour_groups * per_group / (0.6 * pgs_stage_capacity(block));
+        grid = (int)std::ceil(want);
+    }
     grid = std::max(1, std::min(grid, c->pgs_max_grid));
     if (learned) {
         // plain start residual, predicted start residual; keep the better

@@ -116,11 +116,20 @@ constexpr int kMaxSub = 1 << kSubBits;
 constexpr int kTreeStack = 128;
 constexpr int kMaxColours = 64;     // colours 0..63 run in parallel; 64 is the sequential tail
 
-// Solve record: the per-contact data the PGS sweep reads, packed in sweep order:
-// per row (n, t1, t2) ja.w, jb.v, jb.w (ja.v = -jb.v exactly), then b[3], D[3],
-// mu, lambda[3] (read and written in place), one pad = 38 doubles (304 B).
-constexpr int kRecDoubles = 38;
-constexpr int kRecLambda = 34;
+// Solve record: the per-contact data the PGS sweep reads, packed in sweep order.
+// Per row (n, t1, t2), 15 doubles: ja.w, jb.v, jb.w (ja.v = -jb.v exactly),
+// then jinv_a ja.w and jinv_b jb.w (constant during a solve, so computed once
+// in setup with the same operations the update would do).  Then b[3], D[3],
+// mu, lambda[3] (read and written in place), the solve bodies a | b << 32 as
+// raw bits, the clamp flag, one pad: 58 doubles (464 B, 16-byte aligned).
+constexpr int kRecDoubles = 58;
+constexpr int kRecRow = 15;
+constexpr int kRecB = 45;
+constexpr int kRecD = 48;
+constexpr int kRecMu = 51;
+constexpr int kRecLambda = 52;
+constexpr int kRecAB = 55;
+constexpr int kRecClamp = 56;
 
 // ---- launchers (am_detect.hip) ----
 struct PairArgs {

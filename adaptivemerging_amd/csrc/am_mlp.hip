@@ -413,14 +413,14 @@ __global__ void k_start_residual(const double* rec, const int* meta, const doubl
         const double* db = &dv[m.y * 6];
         double lam[3] = {R[kRecLambda], R[kRecLambda + 1], R[kRecLambda + 2]};
         for (int r = 0; r < 3; r++) {
-            const double* q = R + r * 9;
+            const double* q = R + r * kRecRow;
             double jd = (-q[3] * da[0] + -q[4] * da[1] + -q[5] * da[2]) + (q[0] * da[3] + q[1] * da[4] + q[2] * da[5]) +
                         (q[3] * db[0] + q[4] * db[1] + q[5] * db[2]) + (q[6] * db[3] + q[7] * db[4] + q[8] * db[5]);
-            double D = R[30 + r];
-            double l = (D * lam[r] - omega * (R[27 + r] + jd)) / (D + compliance);
+            double D = R[kRecD + r];
+            double l = (D * lam[r] - omega * (R[kRecB + r] + jd)) / (D + compliance);
             if (m.z) {
                 if (r == 0) l = fmax(0.0, l);
-                else l = fmin(fmax(l, -R[33] * lam[0]), R[33] * lam[0]);
+                else l = fmin(fmax(l, -R[kRecMu] * lam[0]), R[kRecMu] * lam[0]);
             }
             r_max = fmax(r_max, fabs(l - lam[r]));
         }

@@ -272,37 +272,42 @@ __global__ void k_setup(SetupArgs S) {
         }
         bn += S.feedback * C.viol[c];
     }
-    // D (Contact.computeJMinvJt, Contact.java:340-354)
+    // D (Contact.computeJMinvJt, Contact.java:340-354); the jinv products are
+    // kept for the sweep's deltaV updates (PGS.updateDeltaVwithLambdai).
     m33 ja_inv = ldm(&B.jinv[a * 9]);
     m33 jb_inv = ldm(&B.jinv[b * 9]);
     double D[3];
+    d3 ta[3], tb[3];
 #pragma unroll
     for (int row = 0; row < 3; row++) {
         const double* ja = Jr + row * 12;
         const double* jb = Jr + row * 12 + 6;
-        d3 t1 = mul(ja_inv, ld3(ja + 3));
-        d3 t2 = mul(jb_inv, ld3(jb + 3));
-        D[row] = B.minv[a] * dot(ld3(ja), ld3(ja)) + dot(ld3(ja + 3), t1) + B.minv[b] * dot(ld3(jb), ld3(jb)) +
-                 dot(ld3(jb + 3), t2);
+        ta[row] = mul(ja_inv, ld3(ja + 3));
+        tb[row] = mul(jb_inv, ld3(jb + 3));
+        D[row] = B.minv[a] * dot(ld3(ja), ld3(ja)) + dot(ld3(ja + 3), ta[row]) + B.minv[b] * dot(ld3(jb), ld3(jb)) +
+                 dot(ld3(jb + 3), tb[row]);
     }
-    // Compact record: per row ja.w, jb.v, jb.w (ja.v = -jb.v exactly), then b, D, mu.
     double* rec = &S.rec[(size_t)k * kRecDoubles];
 #pragma unroll
     for (int row = 0; row < 3; row++) {
 #pragma unroll
-        for (int i = 0; i < 9; i++) rec[row * 9 + i] = Jr[row * 12 + 3 + i];
+        for (int i = 0; i < 9; i++) rec[row * kRecRow + i] = Jr[row * 12 + 3 + i];
+        st3(rec + row * kRecRow + 9, ta[row]);
+        st3(rec + row * kRecRow + 12, tb[row]);
     }
-    rec[27] = bn;
-    rec[28] = bt1;
-    rec[29] = bt2;
-    rec[30] = D[0];
-    rec[31] = D[1];
-    rec[32] = D[2];
-    rec[33] = contact_mu(B.desc[b1], B.desc[b2], S.fric_ovr, S.fric_val);
+    rec[kRecB + 0] = bn;
+    rec[kRecB + 1] = bt1;
+    rec[kRecB + 2] = bt2;
+    rec[kRecD + 0] = D[0];
+    rec[kRecD + 1] = D[1];
+    rec[kRecD + 2] = D[2];
+    rec[kRecMu] = contact_mu(B.desc[b1], B.desc[b2], S.fric_ovr, S.fric_val);
 #pragma unroll
     for (int r = 0; r < 3; r++) rec[kRecLambda + r] = C.lam[c * 3 + r];
-    rec[kRecLambda + 3] = 0.0;
     bool clamp = !(B.flags[b1] & AM_BODY_MAGNET_ACTIVE) && !(B.flags[b2] & AM_BODY_MAGNET_ACTIVE);
+    rec[kRecAB] = __longlong_as_double((long long)(((uint64_t)(uint32_t)b << 32) | (uint32_t)a));
+    rec[kRecClamp] = clamp ? 1.0 : 0.0;
+    rec[kRecDoubles - 1] = 0.0;
     reinterpret_cast<int4*>(S.meta)[k] = make_int4(a, b, clamp ? 1 : 0, 0);
     S.sstate[k] = C.state[c];
 }
@@ -322,8 +327,8 @@ __global__ void k_unpack(SetupArgs S) {
 #pragma unroll
     for (int r = 0; r < 3; r++) {
         C.lam[c * 3 + r] = rec[kRecLambda + r];
-        C.b[c * 3 + r] = rec[27 + r];
-        C.D[c * 3 + r] = rec[30 + r];
+        C.b[c * 3 + r] = rec[kRecB + r];
+        C.D[c * 3 + r] = rec[kRecD + r];
     }
     C.state[c] = S.sstate[k];
 }
@@ -338,7 +343,6 @@ void launch_unpack(hipStream_t s, const SetupArgs& S, int cap) {
 struct BodyRegs {
     double dv[6];
     double minv;
-    m33 jinv;
     bool dyn;
 };
 
@@ -355,7 +359,6 @@ AMD_INLINE void load_body(const PgsArgs& P, int e, BodyRegs& r) {
 #pragma unroll
     for (int i = 0; i < 6; i++) r.dv[i] = ld_coherent(&P.dv[e * 6 + i]);
     r.minv = P.minv[e];
-    r.jinv = ldm(&P.jinv[e * 9]);
     r.dyn = (P.bflags[e] & AM_BODY_PINNED) == 0;
 }
 AMD_INLINE void store_body(const PgsArgs& P, int e, const BodyRegs& r) {
@@ -369,13 +372,15 @@ AMD_INLINE void swap_regs(BodyRegs& a, BodyRegs& b) {
     b = t;
 }
 
-// One row of a contact record: w = ja.w, v = jb.v (ja.v = -jb.v), u = jb.w.
+// One row of a contact record: aw = ja.w, bv = jb.v (ja.v = -jb.v),
+// bw = jb.w, ta = jinv_a ja.w, tb = jinv_b jb.w.
 struct Row {
-    d3 aw, bv, bw;
+    d3 aw, bv, bw, ta, tb;
 };
 AMD_INLINE Row row_of(const double* rec, int r) {
-    return Row{mk(rec[r * 9 + 0], rec[r * 9 + 1], rec[r * 9 + 2]), mk(rec[r * 9 + 3], rec[r * 9 + 4], rec[r * 9 + 5]),
-               mk(rec[r * 9 + 6], rec[r * 9 + 7], rec[r * 9 + 8])};
+    const double* q = rec + r * kRecRow;
+    return Row{mk(q[0], q[1], q[2]), mk(q[3], q[4], q[5]), mk(q[6], q[7], q[8]), mk(q[9], q[10], q[11]),
+               mk(q[12], q[13], q[14])};
 }
 
 // PGS.updateDeltaVwithLambdai (PGS.java:221-245)
@@ -384,18 +389,16 @@ AMD_INLINE void apply_row(const Row& R, double lambda, BodyRegs& A, BodyRegs& Bb
     A.dv[0] = s1 * -R.bv.x + A.dv[0];
     A.dv[1] = s1 * -R.bv.y + A.dv[1];
     A.dv[2] = s1 * -R.bv.z + A.dv[2];
-    d3 t = mul(A.jinv, R.aw);
-    A.dv[3] = lambda * t.x + A.dv[3];
-    A.dv[4] = lambda * t.y + A.dv[4];
-    A.dv[5] = lambda * t.z + A.dv[5];
+    A.dv[3] = lambda * R.ta.x + A.dv[3];
+    A.dv[4] = lambda * R.ta.y + A.dv[4];
+    A.dv[5] = lambda * R.ta.z + A.dv[5];
     double s2 = Bb.minv * lambda;
     Bb.dv[0] = s2 * R.bv.x + Bb.dv[0];
     Bb.dv[1] = s2 * R.bv.y + Bb.dv[1];
     Bb.dv[2] = s2 * R.bv.z + Bb.dv[2];
-    t = mul(Bb.jinv, R.bw);
-    Bb.dv[3] = lambda * t.x + Bb.dv[3];
-    Bb.dv[4] = lambda * t.y + Bb.dv[4];
-    Bb.dv[5] = lambda * t.z + Bb.dv[5];
+    Bb.dv[3] = lambda * R.tb.x + Bb.dv[3];
+    Bb.dv[4] = lambda * R.tb.y + Bb.dv[4];
+    Bb.dv[5] = lambda * R.tb.z + Bb.dv[5];
 }
 // Contact.getJdv (Contact.java:362-378)
 AMD_INLINE double jdv(const Row& R, const BodyRegs& A, const BodyRegs& Bb) {
@@ -406,66 +409,50 @@ AMD_INLINE double jdv(const Row& R, const BodyRegs& A, const BodyRegs& Bb) {
     return da + db;
 }
 
+constexpr int kRec2 = kRecDoubles / 2;  // 16-byte words per record
 struct Rec {
     double v[kRecDoubles];
 };
-AMD_INLINE void load_rec(const double* src, Rec& r) {
-    const double2* s2 = reinterpret_cast<const double2*>(src);
+AMD_INLINE void load_rec(const double2* src, Rec& r) {
 #pragma unroll
-    for (int i = 0; i < kRecDoubles / 2; i++) {
-        double2 t = s2[i];
+    for (int i = 0; i < kRec2; i++) {
+        double2 t = src[i];
         r.v[2 * i] = t.x;
         r.v[2 * i + 1] = t.y;
     }
 }
 
-// The first loads of a group (descriptor and first contact's meta) do not
-// depend on other workgroups' results, so they are issued before the barrier
-// that precedes the group's colour.
-struct GroupHead {
-    int4 gd;  // first, end sweep position, solve bodies A, B
-    int4 m0;
-};
-AMD_INLINE void group_head(const PgsArgs& P, int q, GroupHead& h) {
-    h.gd = reinterpret_cast<const int4*>(P.group_desc)[q];
-    h.m0 = reinterpret_cast<const int4*>(P.meta)[h.gd.x];
-}
-
-// mode 0: confidentWarmStart (PGS.java:250-256); mode 1: one PGS pass over the group.
-AMD_INLINE void process_group(const PgsArgs& P, const GroupHead& h, int mode, bool last_iter, double& mx) {
-    const int k0 = h.gd.x, k1 = h.gd.y, A = h.gd.z, Bi = h.gd.w;
-    const int4* meta = reinterpret_cast<const int4*>(P.meta);
-    Rec cur, nxt;
-    int4 mcur = h.m0;
-    load_rec(&P.rec[(size_t)k0 * kRecDoubles], cur);
+// One group (the contacts of one body pair): mode 0 is confidentWarmStart
+// (PGS.java:250-256), mode 1 one PGS pass (PGS.java:100-184).  recs points at
+// the group's first record (in the workgroup's LDS stage or in global memory);
+// lambda and the contact state go back to global memory at sweep position k.
+AMD_INLINE void process_group(const PgsArgs& P, int4 gd, const double2* recs, int mode, bool last_iter, double& mx) {
+    const int k0 = gd.x, k1 = gd.y, A = gd.z, Bi = gd.w;
     BodyRegs ra, rb;
     load_body(P, A, ra);
     load_body(P, Bi, rb);
     for (int k = k0; k < k1; k++) {
-        int4 mnext = mcur;
-        if (k + 1 < k1) {
-            // prefetch the next contact of the group while this one is solved
-            mnext = meta[k + 1];
-            load_rec(&P.rec[(size_t)(k + 1) * kRecDoubles], nxt);
-        }
-        const bool swapped = mcur.x != A;
-        if (swapped) swap_regs(ra, rb);
+        Rec cur;
+        load_rec(recs + (size_t)(k - k0) * kRec2, cur);
         const double* rec = cur.v;
+        const int ca = (int)(uint32_t)(unsigned long long)__double_as_longlong(rec[kRecAB]);
+        const bool swapped = ca != A;
+        if (swapped) swap_regs(ra, rb);
         double lam[3] = {rec[kRecLambda + 0], rec[kRecLambda + 1], rec[kRecLambda + 2]};
         if (mode == 0) {
 #pragma unroll
             for (int r = 0; r < 3; r++) apply_row(row_of(rec, r), lam[r], ra, rb);
         } else {
-            const bool clamp = mcur.z != 0;
-            const double mu = rec[33];
+            const bool clamp = rec[kRecClamp] != 0.0;
+            const double mu = rec[kRecMu];
             const double omega = P.omega;
 #pragma unroll
             for (int r = 0; r < 3; r++) {
                 Row R = row_of(rec, r);
                 double Jd = jdv(R, ra, rb);
                 double prev = lam[r];
-                double Dr = rec[30 + r];
-                double l = (Dr * lam[r] - omega * (rec[27 + r] + Jd)) / (Dr + P.compliance);
+                double Dr = rec[kRecD + r];
+                double l = (Dr * lam[r] - omega * (rec[kRecB + r] + Jd)) / (Dr + P.compliance);
                 if (clamp) {
                     if (r == 0) {
                         l = jmax(0, l);
@@ -486,8 +473,8 @@ AMD_INLINE void process_group(const PgsArgs& P, const GroupHead& h, int mode, bo
             dst[2] = lam[2];
             if (last_iter) {
                 // Contact.updateContactState (Contact.java:385-398)
-                double w1 = rec[28] + jdv(row_of(rec, 1), ra, rb);
-                double w2 = rec[29] + jdv(row_of(rec, 2), ra, rb);
+                double w1 = rec[kRecB + 1] + jdv(row_of(rec, 1), ra, rb);
+                double w2 = rec[kRecB + 2] + jdv(row_of(rec, 2), ra, rb);
                 int st;
                 if (fabs(lam[0]) <= 1e-14) st = AM_STATE_BROKEN;
                 else if (fabs(w1) > P.sliding) st = AM_STATE_ONEDGE;
@@ -497,10 +484,6 @@ AMD_INLINE void process_group(const PgsArgs& P, const GroupHead& h, int mode, bo
             }
         }
         if (swapped) swap_regs(ra, rb);
-        if (k + 1 < k1) {
-            cur = nxt;
-            mcur = mnext;
-        }
     }
     store_body(P, A, ra);
     store_body(P, Bi, rb);
@@ -512,21 +495,17 @@ AMD_INLINE double wave_max(double v) {
     return v;
 }
 
-// Grid barrier for a co-resident (cooperatively launched) grid: every wave
-// waits until its (device-coherent) deltaV stores are complete, lane 0 of each
-// block arrives on a monotonically increasing counter and spins on a coherent
-// load until all blocks of this barrier instance have arrived.  A spin limit
-// turns a barrier that cannot complete into an error flag instead of a hang.
+// Grid barrier for a co-resident (cooperatively launched) grid.  The caller
+// has already waited for its deltaV stores (s_waitcnt) and passed a workgroup
+// barrier, so lane 0 arrives on a monotonically increasing counter right
+// away and spins on a coherent load until all blocks of this instance have
+// arrived; loads issued before (the next colour's records) stay in flight.
+// A spin limit turns a barrier that cannot complete into an error flag.
 template <bool kSingleBlock>
 AMD_INLINE bool grid_barrier(const PgsArgs& P, unsigned& epoch) {
-    if (kSingleBlock) {
-        __syncthreads();
-        return true;
-    }
+    if (kSingleBlock) return true;
     __shared__ int bail;
     epoch += gridDim.x;
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
     if (threadIdx.x == 0) {
         bail = 0;
         __hip_atomic_fetch_add(P.barrier, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -548,6 +527,23 @@ AMD_INLINE bool grid_barrier(const PgsArgs& P, unsigned& epoch) {
     return bail == 0;
 }
 
+// Staging: the records of a workgroup's groups of the next colour are copied
+// into LDS while the grid barrier before that colour is pending (they depend
+// on no other workgroup), so the dependent chain of a group runs on LDS
+// latency.  Each lane loads up to kStage 16-byte words; a group is staged if
+// it fits entirely, otherwise it reads global memory.
+constexpr int kStage = 8;
+constexpr int kArenaMax = 70;  // contacts: 256 * kStage / kRec2 rounded down
+static_assert(kArenaMax * kRec2 <= 256 * kStage, "arena larger than one staging pass");
+
+struct Stage {
+    int4 gd;     // this lane's group of the staged colour (first pass only)
+    bool mine;
+    int off;     // its offset in the arena in contacts, -1 when not staged
+    int nd2;     // staged 16-byte words of the workgroup
+    double2 buf[kStage];
+};
+
 // Persistent kernel: warm-start pass then up to `iterations` sweeps; one grid
 // barrier per non-empty colour.  Groups of a colour are dealt block-cyclically
 // (group j of the colour goes to block j % grid, lane j / grid) so every
@@ -559,8 +555,16 @@ __global__ __launch_bounds__(256) void k_pgs(PgsArgs P) {
     const int* cs = P.colour_start;
     const int slot0 = threadIdx.x * gridDim.x + blockIdx.x;
     const int stride = gridDim.x * blockDim.x;
+    const int4* gdesc = reinterpret_cast<const int4*>(P.group_desc);
+    const double2* grec = reinterpret_cast<const double2*>(P.rec);
+    __shared__ double2 s_arena[kArenaMax * kRec2];
+    __shared__ int s_map[kArenaMax];
+    __shared__ int s_wsum[4];
+    __shared__ int s_staged;
     __shared__ int s_cols[kMaxColours];
     __shared__ int s_ncols;
+    const int cap = min(kArenaMax, (int)(blockDim.x * kStage) / kRec2);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = (blockDim.x + 63) >> 6;
     if (threadIdx.x == 0) {
         int nc = 0;
         for (int c = 0; c < kMaxColours; c++)
@@ -572,47 +576,104 @@ __global__ __launch_bounds__(256) void k_pgs(PgsArgs P) {
     const bool has_tail = cs[kMaxColours + 1] > cs[kMaxColours];
     const bool lane0 = blockIdx.x == 0 && threadIdx.x == 0;
     unsigned epoch = 0;
-    GroupHead h;
-    // Runs colour list position ci; prefetches the head of the next position.
-    auto run_colour = [&](int ci, int mode, bool last, double& mx) {
+    Stage st;
+
+    // Issues the staging loads for colour list position ci (all lanes).
+    auto stage_issue = [&](int ci) {
+        __syncthreads();  // the arena and map of the previous colour are free
         const int c = s_cols[ci];
-        int q = cs[c] + slot0;
-        if (q < cs[c + 1]) {
-            process_group(P, h, mode, last, mx);
-            for (q += stride; q < cs[c + 1]; q += stride) {
-                group_head(P, q, h);
-                process_group(P, h, mode, last, mx);
+        const int q = cs[c] + slot0;
+        st.mine = q < cs[c + 1];
+        st.gd = st.mine ? gdesc[q] : make_int4(0, 0, 0, 0);
+        const int n = st.gd.y - st.gd.x;
+        int incl = n;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            int t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        if (lane == 63 || threadIdx.x == blockDim.x - 1) s_wsum[wave] = incl;
+        if (threadIdx.x == 0) s_staged = 0;
+        __syncthreads();
+        int base = 0;
+        for (int w = 0; w < wave; w++) base += s_wsum[w];
+        const int off = base + incl - n;
+        st.off = (st.mine && off + n <= cap) ? off : -1;
+        if (st.off >= 0) {
+            for (int j = 0; j < n; j++) s_map[off + j] = st.gd.x + j;
+            atomicMax(&s_staged, off + n);
+        }
+        __syncthreads();
+        st.nd2 = s_staged * kRec2;
+#pragma unroll
+        for (int j = 0; j < kStage; j++) {
+            const int i = threadIdx.x + j * blockDim.x;
+            if (i < st.nd2) {
+                const int a = i / kRec2, part = i - a * kRec2;
+                st.buf[j] = grec[(size_t)s_map[a] * kRec2 + part];
             }
         }
-        const int cn = s_cols[(ci + 1) % ncols];
-        const int qn = cs[cn] + slot0;
-        if (qn < cs[cn + 1]) group_head(P, qn, h);
+        (void)nwaves;
+    };
+    // Moves the staged words into LDS once the barrier has passed.
+    auto stage_commit = [&]() {
+#pragma unroll
+        for (int j = 0; j < kStage; j++) {
+            const int i = threadIdx.x + j * blockDim.x;
+            if (i < st.nd2) s_arena[i] = st.buf[j];
+        }
+        __syncthreads();
+    };
+    // Runs colour list position ci (its staging is committed).
+    auto run_colour = [&](int ci, int mode, bool last, double& mx) {
+        const int c = s_cols[ci];
+        if (st.mine) {
+            const double2* recs = st.off >= 0 ? s_arena + st.off * kRec2 : grec + (size_t)st.gd.x * kRec2;
+            process_group(P, st.gd, recs, mode, last, mx);
+        }
+        for (int q = cs[c] + slot0 + stride; q < cs[c + 1]; q += stride) {
+            const int4 gd = gdesc[q];
+            process_group(P, gd, grec + (size_t)gd.x * kRec2, mode, last, mx);
+        }
     };
     auto run_tail = [&](int mode, bool last, double& mx) {
         if (lane0)
             for (int q = cs[kMaxColours]; q < cs[kMaxColours + 1]; q++) {
-                group_head(P, q, h);
-                process_group(P, h, mode, last, mx);
+                const int4 gd = gdesc[q];
+                process_group(P, gd, grec + (size_t)gd.x * kRec2, mode, last, mx);
             }
-        if (ncols > 0) {
-            const int cn = s_cols[0];
-            const int qn = cs[cn] + slot0;
-            if (qn < cs[cn + 1]) group_head(P, qn, h);
-        }
     };
+    // Ends a colour: wait for this lane's deltaV stores, stage the next
+    // colour, meet the other workgroups, commit the stage.
+    auto next = [&](int ci_next) -> bool {
+        __builtin_amdgcn_s_waitcnt(0);
+        if (ci_next >= 0) {
+            stage_issue(ci_next);
+        } else {
+            __syncthreads();
+            st.mine = false;
+            st.nd2 = 0;
+        }
+        if (!grid_barrier<kSingleBlock>(P, epoch)) return false;
+        if (ci_next >= 0) stage_commit();
+        return true;
+    };
+
     if (ncols > 0) {
-        const int c = s_cols[0];
-        if (cs[c] + slot0 < cs[c + 1]) group_head(P, cs[c] + slot0, h);
+        stage_issue(0);
+        stage_commit();
     }
+    const int first_after_tail = ncols > 0 ? 0 : -1;
     if (P.warm_start) {
         double dummy = 0;
         for (int ci = 0; ci < ncols; ci++) {
             run_colour(ci, 0, false, dummy);
-            if (!grid_barrier<kSingleBlock>(P, epoch)) return;
+            const int nx = ci + 1 < ncols ? ci + 1 : (has_tail ? -1 : first_after_tail);
+            if (!next(nx)) return;
         }
         if (has_tail) {
             run_tail(0, false, dummy);
-            if (!grid_barrier<kSingleBlock>(P, epoch)) return;
+            if (!next(first_after_tail)) return;
         }
     }
     int it = 0;
@@ -626,17 +687,18 @@ __global__ __launch_bounds__(256) void k_pgs(PgsArgs P) {
                 if ((threadIdx.x & 63) == 0 && wm > 0)
                     atomicMax(&P.iter_max[it], (unsigned long long)__double_as_longlong(wm));
             }
-            if (!grid_barrier<kSingleBlock>(P, epoch)) return;
+            const int nx = ci + 1 < ncols ? ci + 1 : (has_tail ? -1 : first_after_tail);
+            if (!next(nx)) return;
         }
         if (has_tail) {
             run_tail(1, last, mx);
             double wm = wave_max(mx);
             if ((threadIdx.x & 63) == 0 && wm > 0)
                 atomicMax(&P.iter_max[it], (unsigned long long)__double_as_longlong(wm));
-            if (!grid_barrier<kSingleBlock>(P, epoch)) return;
+            if (!next(first_after_tail)) return;
         }
         if (ncols == 0 && !has_tail) {
-            if (!grid_barrier<kSingleBlock>(P, epoch)) return;
+            if (!next(-1)) return;
         }
         double m = __longlong_as_double((long long)__hip_atomic_load(&P.iter_max[it], __ATOMIC_RELAXED,
                                                                       __HIP_MEMORY_SCOPE_AGENT));
@@ -656,6 +718,8 @@ int pgs_max_blocks(int block, bool* ok) {
     *ok = per_cu > 0;
     return n_cu * (per_cu > 0 ? per_cu : 1);
 }
+
+int pgs_stage_capacity(int block) { return std::min(kArenaMax, block * kStage / kRec2); }
 
 hipError_t launch_pgs(hipStream_t s, const PgsArgs& P, int grid, int block) {
     if (grid <= 1) {
