@@ -14,7 +14,8 @@
  *    (m00 m01 m02 m10 ... m22), the javax.vecmath Matrix3d field order.
  *  - Body indices are positions in the am_body_desc array passed to
  *    am_scene_upload ("physical" bodies).  Collections (merged bodies) are
- *    extra entities appended after them (see am_collections_set).
+ *    extra entities after them, created by am_merge / am_unmerge (see
+ *    am_structure_pull).
  *
  * Reference interfaces replaced (paths under src/mergingBodies3D/):
  *   CollisionProcessor.collisionDetection   CollisionProcessor.java:91-102
@@ -27,8 +28,12 @@
  *   CollisionProcessor.updateInCollections  CollisionProcessor.java:232-303
  *   MotionMetricProcessor.getMotionMetric   MotionMetricProcessor.java:15-116
  *   BodyPairContact.accumulateFor*          BodyPairContact.java:83-145
+ *   BodyPairContact merge/unmerge tests     BodyPairContact.java:151-266
+ *   Merging.merge / Merging.unmerge         Merging.java:73-393
+ *   RigidCollection (mass properties,       RigidCollection.java:55-937
+ *     velocities, member transforms)
  *   RigidBody.advanceVelocities/Positions   RigidBody.java:409-441
- *   RigidBodySystem.advanceTime (no merge)  RigidBodySystem.java:102-185
+ *   RigidBodySystem.advanceTime             RigidBodySystem.java:102-185
  */
 #ifndef AM_H
 #define AM_H
