@@ -9,6 +9,7 @@ import math
 
 import numpy as np
 
+from ._lib import AM_GEOM_TREE, BODY_DTYPE
 from .scene import Scene, random_rotation, rot_axis_angle
 
 
@@ -224,3 +225,19 @@ def merge_stack(seed: int = 0, layers: int = 4, density: float = 500.0) -> Scene
         s.add_box([1.0, 0.2, 0.6], [0.02 * k + jitter[0], 0.1 + 0.2 * k, jitter[2]], density=density)
     s.add_box([0.3, 0.3, 0.3], [2.0, 0.15, 0.0], density=density)
     return s
+
+
+def batch(scenes) -> Scene:
+    """Independent scenes in one context (scene ids 0..n-1, bodies and sphere-
+    tree nodes concatenated): bodies of different scenes never collide."""
+    out = Scene()
+    for k, sc in enumerate(scenes):
+        node_off = len(out.nodes)
+        for i, rec in enumerate(sc.bodies):
+            r = np.array(rec, dtype=BODY_DTYPE).copy()
+            r["scene"] = k
+            if int(r["geom"]) == AM_GEOM_TREE:
+                r["tree_first"] = int(r["tree_first"]) + node_off
+            out._append(r, sc.x[i], sc.R[i], sc.v[i], sc.w[i], f"s{k}_{sc.names[i]}")
+        out.nodes.extend(sc.nodes)
+    return out

@@ -156,7 +156,21 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--pgs-grid", type=int, default=0)
     ap.add_argument("--pgs-block", type=int, default=0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (calibrates the HBM byte counters) and exit")
     args = ap.parse_args()
+
+    if args.copy_probe > 0:
+        import torch
+
+        src = torch.empty(args.copy_probe // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 4, "launches": 3}), flush=True)
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
