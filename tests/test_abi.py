@@ -54,11 +54,15 @@ int main(void) {
   printf("sizeof am_params %zu\n", sizeof(am_params));
   printf("sizeof am_contact_arrays %zu\n", sizeof(am_contact_arrays));
   printf("sizeof am_step_stats %zu\n", sizeof(am_step_stats));
+  printf("sizeof am_body_pair %zu\n", sizeof(am_body_pair));
   F(am_body_desc, size) F(am_body_desc, radius) F(am_body_desc, jinv0) F(am_body_desc, plane_d) F(am_body_desc, bbox)
+  F(am_body_desc, inertia0)
+  F(am_body_pair, flag) F(am_body_pair, collection) F(am_body_pair, hist) F(am_body_pair, states)
   F(am_sphere_node, r) F(am_sphere_node, first_child) F(am_sphere_node, n_children)
   F(am_params, tolerance) F(am_params, compliance) F(am_params, sliding_threshold) F(am_params, gravity_amount)
   F(am_params, threshold_merge) F(am_params, threshold_breath) F(am_params, pgs_block) F(am_params, pgs_grid)
   F(am_step_stats, last_change) F(am_step_stats, ms_total) F(am_step_stats, ms_setup_kernel)
+  F(am_step_stats, used_learned) F(am_step_stats, n_internal_contacts) F(am_step_stats, ms_merging)
   return 0;
 }
 """
@@ -76,6 +80,7 @@ def test_struct_layouts_match_ctypes(tmp_path):
     assert sizes["am_params"] == C.sizeof(_lib.Params)
     assert sizes["am_contact_arrays"] == C.sizeof(_lib.ContactArrays)
     assert sizes["am_step_stats"] == C.sizeof(_lib.StepStats)
+    assert sizes["am_body_pair"] == C.sizeof(_lib.BodyPair)
     for ln in lines:
         if ln.startswith("sizeof"):
             continue
@@ -88,6 +93,8 @@ def test_struct_layouts_match_ctypes(tmp_path):
             assert getattr(_lib.Params, field).offset == int(off), field
         elif struct == "am_step_stats":
             assert getattr(_lib.StepStats, field).offset == int(off), field
+        elif struct == "am_body_pair":
+            assert getattr(_lib.BodyPair, field).offset == int(off), field
 
 
 def test_params_defaults_are_reference_defaults(libam):
