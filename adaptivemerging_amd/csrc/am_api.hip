@@ -139,6 +139,7 @@ int ensure_contact_capacity(am_ctx* c, int want, int keep) {
     AM_HIP(c, c->g_sizes.ensure(cap + 1));
     AM_HIP(c, c->g_pos_start.ensure(cap + 1));
     AM_HIP(c, c->s_order.ensure(cap));
+    AM_HIP(c, c->s_pos_a.ensure(cap));
     AM_HIP(c, c->s_rec.ensure((size_t)cap * kRecDoubles));
     AM_HIP(c, c->s_lam.ensure((size_t)cap * 3));
     AM_HIP(c, c->s_meta.ensure((size_t)cap * 4));
@@ -391,7 +392,8 @@ int build_layout(am_ctx* c) {
     AM_HIP(c, c->s_colour_start.ensure(kMaxColours + 2));
     AM_HIP(c, c->s_group_desc.ensure((size_t)std::max(ng, 1) * 4));
     launch_colour_layout(s, ng, c->g_keys_sorted.p, c->g_start.p, c->g_a.p, c->g_b.p, c->s_colour_start.p,
-                         c->g_sizes.p, c->g_pos_start.p, c->s_order.p, c->s_group_desc.p, sc + 9, c->d_scan_temp.p);
+                         c->g_sizes.p, c->g_pos_start.p, c->s_order.p, c->s_group_desc.p, c->s_pos_a.p, sc + 9,
+                         c->d_scan_temp.p);
     AM_HIP(c, hipGetLastError());
     int cs[kMaxColours + 2];
     AM_HIP(c, hipMemcpyAsync(cs, c->s_colour_start.p, sizeof cs, hipMemcpyDeviceToHost, s));
@@ -637,7 +639,7 @@ int train_append(am_ctx* c) {
     }
     AM_HIP(c, hipMemcpyAsync(c->t_X.p + (size_t)c->t_n * kMlpIn, c->f_X.p, (size_t)n * kMlpIn * 4,
                              hipMemcpyDeviceToDevice, c->stream));
-    launch_targets(c->stream, c->c_lam.p, c->f_scale.p, n, c->t_T.p + (size_t)c->t_n * kMlpOut);
+    launch_targets(c->stream, c->c_lam.p, c->f_scale.p, c->f_X.p, n, c->t_T.p + (size_t)c->t_n * kMlpOut);
     AM_HIP(c, hipGetLastError());
     c->t_n += n;
     return AM_OK;
@@ -685,6 +687,7 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
     S.order = c->s_order.p;
     S.rec = c->s_rec.p;
     S.meta = c->s_meta.p;
+    S.pos_a = c->s_pos_a.p;
     S.sstate = c->s_state.p;
     S.dt = dt;
     if (main_solve) S.feedback = (!P.enable_post_stabilization || post) ? P.feedback_stiffness : 0.0;
@@ -746,15 +749,9 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
         c->pgs_max_grid = pgs_max_blocks(block, &ok);
         if (!ok) c->pgs_max_grid = 1;
     }
-    // groups of a colour are dealt block-cyclically; enough workgroups that a
-    // workgroup's share of the largest colour (at the mean group size) fills
-    // about 60% of its LDS stage
-    int grid = P.pgs_grid;
-    if (grid <= 0) {
-        double per_group = c->n_groups > 0 ? (double)c->n_contacts / c->n_groups : 1.0;
-        double want = c->max_colour_groups * per_group / (0.6 * pgs_stage_capacity(block));
-        grid = (int)std::ceil(want);
-    }
+    // groups of a colour are dealt block-cyclically; about 20 groups of the
+    // largest colour per workgroup (the best measured on the settling box pile)
+    int grid = P.pgs_grid > 0 ? P.pgs_grid : (c->max_colour_groups + 19) / 20;
     grid = std::max(1, std::min(grid, c->pgs_max_grid));
     if (learned) {
         // plain start residual, predicted start residual; keep the better

@@ -137,7 +137,7 @@ struct am_ctx {
         g_sizes, g_pos_start, s_order, s_colour_start;
     DBuf<uint64_t> g_keys, g_keys_sorted;
     DBuf<double> s_rec, s_lam;
-    DBuf<int> s_meta, s_state, d_barrier, s_group_desc;
+    DBuf<int> s_meta, s_state, d_barrier, s_group_desc, s_pos_a;
     DBuf<unsigned long long> s_iter_max;
 
     // scalars on device: 0 n_pairs, 1 n_contacts, 2 n_groups, 3 total scratch, 4 overflow, 5 remaining,

@@ -13,15 +13,16 @@
 #define AMD_INLINE __device__ __forceinline__
 
 // Java Math.max / Math.min: NaN propagates, -0.0 orders below +0.0.
+// Written as selects so divergent lanes do not branch.
 AMD_INLINE double jmax(double a, double b) {
-    if (a != a) return a;
-    if (a == 0.0 && b == 0.0 && __signbit(a)) return b;
-    return (a >= b) ? a : b;
+    double r = (a >= b) ? a : b;
+    r = ((a == 0.0) & (b == 0.0) & (__signbit(a) != 0)) ? b : r;
+    return (a != a) ? a : r;
 }
 AMD_INLINE double jmin(double a, double b) {
-    if (a != a) return a;
-    if (a == 0.0 && b == 0.0 && __signbit(b)) return b;
-    return (a <= b) ? a : b;
+    double r = (a <= b) ? a : b;
+    r = ((a == 0.0) & (b == 0.0) & (__signbit(b) != 0)) ? b : r;
+    return (a != a) ? a : r;
 }
 
 struct d3 {

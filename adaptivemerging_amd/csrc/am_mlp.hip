@@ -355,15 +355,16 @@ void launch_to_bf16(hipStream_t s, const float* p, unsigned short* bf, int n) {
 }
 
 // ---------------------------------------------------------------------------
-// Predicted start: lambda = S * y, projected onto the friction box
-// (lambda_n >= 0, |lambda_t| <= mu lambda_n); contacts the network did not
-// see before (new this step) keep their plain start.
+// Predicted start: the network predicts the change from the plain warm
+// start, lambda = lambda_plain + S * y, projected onto the friction box
+// (lambda_n >= 0, |lambda_t| <= mu lambda_n).  A network that outputs zero
+// reproduces the plain start.
 __global__ void k_apply_prediction(ContactsView C, const BodiesView B, const float* Y, const double* scale, int n,
                                    int fric_ovr, double fric_val, double* lam_out) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     double S = scale[k];
-    double ln = fmax(0.0, (double)Y[k * 3 + 0] * S);
+    double ln = fmax(0.0, C.lam[k * 3 + 0] + (double)Y[k * 3 + 0] * S);
     double mu;
     if (fric_ovr) {
         mu = fric_val;
@@ -374,8 +375,8 @@ __global__ void k_apply_prediction(ContactsView C, const BodiesView B, const flo
         else mu = (f1 + f2) / 2.;
     }
     double lim = mu * ln;
-    double l1 = fmin(fmax((double)Y[k * 3 + 1] * S, -lim), lim);
-    double l2 = fmin(fmax((double)Y[k * 3 + 2] * S, -lim), lim);
+    double l1 = fmin(fmax(C.lam[k * 3 + 1] + (double)Y[k * 3 + 1] * S, -lim), lim);
+    double l2 = fmin(fmax(C.lam[k * 3 + 2] + (double)Y[k * 3 + 2] * S, -lim), lim);
     lam_out[k * 3 + 0] = ln;
     lam_out[k * 3 + 1] = l1;
     lam_out[k * 3 + 2] = l2;
@@ -388,14 +389,16 @@ void launch_apply_prediction(hipStream_t s, ContactsView C, BodiesView B, const 
                            fric_val, lam_out);
 }
 
-// Targets lambda / S of the converged solve.
-__global__ void k_targets(const double* lam, const double* scale, int n, float* T) {
+// Targets: the change from the plain start to the converged solve, over S
+// (features 21..23 hold the plain start over S).
+__global__ void k_targets(const double* lam, const double* scale, const float* X, int n, float* T) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    for (int r = 0; r < 3; r++) T[k * 3 + r] = (float)(lam[k * 3 + r] / scale[k]);
+    for (int r = 0; r < 3; r++)
+        T[k * 3 + r] = (float)(lam[k * 3 + r] / scale[k]) - X[(size_t)k * kMlpIn + 21 + r];
 }
-void launch_targets(hipStream_t s, const double* lam, const double* scale, int n, float* T) {
-    if (n > 0) hipLaunchKernelGGL(k_targets, dim3((n + 255) / 256), dim3(256), 0, s, lam, scale, n, T);
+void launch_targets(hipStream_t s, const double* lam, const double* scale, const float* X, int n, float* T) {
+    if (n > 0) hipLaunchKernelGGL(k_targets, dim3((n + 255) / 256), dim3(256), 0, s, lam, scale, X, n, T);
 }
 
 // Start quality: with deltaV = M^-1 J^T lambda already applied (warm-start

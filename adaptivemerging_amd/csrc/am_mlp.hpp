@@ -63,7 +63,7 @@ void launch_adam(hipStream_t s, float* p, float* g, float* m, float* v, unsigned
 void launch_to_bf16(hipStream_t s, const float* p, unsigned short* bf, int n);
 void launch_apply_prediction(hipStream_t s, ContactsView C, BodiesView B, const float* Y, const double* scale, int n,
                              int fric_ovr, double fric_val, double* lam_out);
-void launch_targets(hipStream_t s, const double* lam, const double* scale, int n, float* T);
+void launch_targets(hipStream_t s, const double* lam, const double* scale, const float* X, int n, float* T);
 void launch_start_residual(hipStream_t s, const double* rec, const int* meta, const double* dv, const double* minv,
                            const double* jinv, int n, double omega, double compliance, unsigned long long* out);
 void launch_rec_lambda(hipStream_t s, double* rec, const int* order, const double* lam, int n);

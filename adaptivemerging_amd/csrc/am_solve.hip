@@ -185,12 +185,15 @@ __global__ void k_sorted_sizes(int n_groups, const uint64_t* sorted, const int* 
 }
 
 __global__ void k_solve_order(int n_groups, const uint64_t* sorted, const int* group_start, const int* pos_start,
-                              const int* group_a, const int* group_b, int* order, int* group_desc) {
+                              const int* group_a, const int* group_b, int* order, int* group_desc, int* pos_a) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n_groups) return;
     int g = (int)(uint32_t)sorted[q];
     int s = group_start[g], e = group_start[g + 1], o = pos_start[q];
-    for (int k = s; k < e; k++) order[o + (k - s)] = k;
+    for (int k = s; k < e; k++) {
+        order[o + (k - s)] = k;
+        pos_a[o + (k - s)] = group_a[g];
+    }
     reinterpret_cast<int4*>(group_desc)[q] = make_int4(o, o + (e - s), group_a[g], group_b[g]);
 }
 
@@ -287,14 +290,32 @@ __global__ void k_setup(SetupArgs S) {
         D[row] = B.minv[a] * dot(ld3(ja), ld3(ja)) + dot(ld3(ja + 3), ta[row]) + B.minv[b] * dot(ld3(jb), ld3(jb)) +
                  dot(ld3(jb + 3), tb[row]);
     }
+    // Records are oriented so that their first body is the group's first
+    // body A; a contact listed the other way round stores jb.w, ja.v (= -jb.v
+    // exactly), ja.w and the swapped jinv products.  Every product of the
+    // update is unchanged and J.dv only swaps its two (commuting) halves, so
+    // the sweep stays bit-exact without swapping body registers per contact.
+    const bool flip = a != S.pos_a[k];
     double* rec = &S.rec[(size_t)k * kRecDoubles];
 #pragma unroll
     for (int row = 0; row < 3; row++) {
-#pragma unroll
-        for (int i = 0; i < 9; i++) rec[row * kRecRow + i] = Jr[row * 12 + 3 + i];
-        st3(rec + row * kRecRow + 9, ta[row]);
-        st3(rec + row * kRecRow + 12, tb[row]);
+        const double* J0 = Jr + row * 12;
+        double* q = rec + row * kRecRow;
+        if (!flip) {
+            st3(q + 0, ld3(J0 + 3));   // ja.w
+            st3(q + 3, ld3(J0 + 6));   // jb.v
+            st3(q + 6, ld3(J0 + 9));   // jb.w
+            st3(q + 9, ta[row]);
+            st3(q + 12, tb[row]);
+        } else {
+            st3(q + 0, ld3(J0 + 9));   // jb.w
+            st3(q + 3, ld3(J0 + 0));   // ja.v = -jb.v
+            st3(q + 6, ld3(J0 + 3));   // ja.w
+            st3(q + 9, tb[row]);
+            st3(q + 12, ta[row]);
+        }
     }
+    const int ea = flip ? b : a, eb = flip ? a : b;
     rec[kRecB + 0] = bn;
     rec[kRecB + 1] = bt1;
     rec[kRecB + 2] = bt2;
@@ -305,10 +326,10 @@ __global__ void k_setup(SetupArgs S) {
 #pragma unroll
     for (int r = 0; r < 3; r++) rec[kRecLambda + r] = C.lam[c * 3 + r];
     bool clamp = !(B.flags[b1] & AM_BODY_MAGNET_ACTIVE) && !(B.flags[b2] & AM_BODY_MAGNET_ACTIVE);
-    rec[kRecAB] = __longlong_as_double((long long)(((uint64_t)(uint32_t)b << 32) | (uint32_t)a));
+    rec[kRecAB] = __longlong_as_double((long long)(((uint64_t)(uint32_t)eb << 32) | (uint32_t)ea));
     rec[kRecClamp] = clamp ? 1.0 : 0.0;
     rec[kRecDoubles - 1] = 0.0;
-    reinterpret_cast<int4*>(S.meta)[k] = make_int4(a, b, clamp ? 1 : 0, 0);
+    reinterpret_cast<int4*>(S.meta)[k] = make_int4(ea, eb, clamp ? 1 : 0, 0);
     S.sstate[k] = C.state[c];
 }
 
@@ -366,11 +387,6 @@ AMD_INLINE void store_body(const PgsArgs& P, int e, const BodyRegs& r) {
 #pragma unroll
     for (int i = 0; i < 6; i++) st_coherent(&P.dv[e * 6 + i], r.dv[i]);
 }
-AMD_INLINE void swap_regs(BodyRegs& a, BodyRegs& b) {
-    BodyRegs t = a;
-    a = b;
-    b = t;
-}
 
 // One row of a contact record: aw = ja.w, bv = jb.v (ja.v = -jb.v),
 // bw = jb.w, ta = jinv_a ja.w, tb = jinv_b jb.w.
@@ -424,20 +440,18 @@ AMD_INLINE void load_rec(const double2* src, Rec& r) {
 
 // One group (the contacts of one body pair): mode 0 is confidentWarmStart
 // (PGS.java:250-256), mode 1 one PGS pass (PGS.java:100-184).  recs points at
-// the group's first record (in the workgroup's LDS stage or in global memory);
-// lambda and the contact state go back to global memory at sweep position k.
+// the group's first record; lambda and the contact state are written back at
+// sweep position k.
 AMD_INLINE void process_group(const PgsArgs& P, int4 gd, const double2* recs, int mode, bool last_iter, double& mx) {
     const int k0 = gd.x, k1 = gd.y, A = gd.z, Bi = gd.w;
     BodyRegs ra, rb;
     load_body(P, A, ra);
     load_body(P, Bi, rb);
+    bool nan = false;
     for (int k = k0; k < k1; k++) {
         Rec cur;
         load_rec(recs + (size_t)(k - k0) * kRec2, cur);
         const double* rec = cur.v;
-        const int ca = (int)(uint32_t)(unsigned long long)__double_as_longlong(rec[kRecAB]);
-        const bool swapped = ca != A;
-        if (swapped) swap_regs(ra, rb);
         double lam[3] = {rec[kRecLambda + 0], rec[kRecLambda + 1], rec[kRecLambda + 2]};
         if (mode == 0) {
 #pragma unroll
@@ -465,7 +479,9 @@ AMD_INLINE void process_group(const PgsArgs& P, int4 gd, const double2* recs, in
                 lam[r] = l;
                 double diff = l - prev;
                 apply_row(R, diff, ra, rb);
-                mx = jmax(mx, fabs(diff));
+                // Math.max over |diff| (PGS.java:125); a NaN is carried separately
+                mx = fmax(mx, fabs(diff));
+                nan = nan || diff != diff;
             }
             double* dst = &P.rec[(size_t)k * kRecDoubles + kRecLambda];
             dst[0] = lam[0];
@@ -483,8 +499,8 @@ AMD_INLINE void process_group(const PgsArgs& P, int4 gd, const double2* recs, in
                 P.sstate[k] = st;
             }
         }
-        if (swapped) swap_regs(ra, rb);
     }
+    if (nan) mx = __longlong_as_double(0x7ff8000000000000ll);
     store_body(P, A, ra);
     store_body(P, Bi, rb);
 }
@@ -499,8 +515,8 @@ AMD_INLINE double wave_max(double v) {
 // has already waited for its deltaV stores (s_waitcnt) and passed a workgroup
 // barrier, so lane 0 arrives on a monotonically increasing counter right
 // away and spins on a coherent load until all blocks of this instance have
-// arrived; loads issued before (the next colour's records) stay in flight.
-// A spin limit turns a barrier that cannot complete into an error flag.
+// arrived.  A spin limit turns a barrier that cannot complete into an error
+// flag.
 template <bool kSingleBlock>
 AMD_INLINE bool grid_barrier(const PgsArgs& P, unsigned& epoch) {
     if (kSingleBlock) return true;
@@ -527,23 +543,6 @@ AMD_INLINE bool grid_barrier(const PgsArgs& P, unsigned& epoch) {
     return bail == 0;
 }
 
-// Staging: the records of a workgroup's groups of the next colour are copied
-// into LDS while the grid barrier before that colour is pending (they depend
-// on no other workgroup), so the dependent chain of a group runs on LDS
-// latency.  Each lane loads up to kStage 16-byte words; a group is staged if
-// it fits entirely, otherwise it reads global memory.
-constexpr int kStage = 8;
-constexpr int kArenaMax = 70;  // contacts: 256 * kStage / kRec2 rounded down
-static_assert(kArenaMax * kRec2 <= 256 * kStage, "arena larger than one staging pass");
-
-struct Stage {
-    int4 gd;     // this lane's group of the staged colour (first pass only)
-    bool mine;
-    int off;     // its offset in the arena in contacts, -1 when not staged
-    int nd2;     // staged 16-byte words of the workgroup
-    double2 buf[kStage];
-};
-
 // Persistent kernel: warm-start pass then up to `iterations` sweeps; one grid
 // barrier per non-empty colour.  Groups of a colour are dealt block-cyclically
 // (group j of the colour goes to block j % grid, lane j / grid) so every
@@ -557,14 +556,8 @@ __global__ __launch_bounds__(256) void k_pgs(PgsArgs P) {
     const int stride = gridDim.x * blockDim.x;
     const int4* gdesc = reinterpret_cast<const int4*>(P.group_desc);
     const double2* grec = reinterpret_cast<const double2*>(P.rec);
-    __shared__ double2 s_arena[kArenaMax * kRec2];
-    __shared__ int s_map[kArenaMax];
-    __shared__ int s_wsum[4];
-    __shared__ int s_staged;
     __shared__ int s_cols[kMaxColours];
     __shared__ int s_ncols;
-    const int cap = min(kArenaMax, (int)(blockDim.x * kStage) / kRec2);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = (blockDim.x + 63) >> 6;
     if (threadIdx.x == 0) {
         int nc = 0;
         for (int c = 0; c < kMaxColours; c++)
@@ -576,61 +569,19 @@ __global__ __launch_bounds__(256) void k_pgs(PgsArgs P) {
     const bool has_tail = cs[kMaxColours + 1] > cs[kMaxColours];
     const bool lane0 = blockIdx.x == 0 && threadIdx.x == 0;
     unsigned epoch = 0;
-    Stage st;
-
-    // Issues the staging loads for colour list position ci (all lanes).
-    auto stage_issue = [&](int ci) {
-        __syncthreads();  // the arena and map of the previous colour are free
+    // this lane's first group of the next colour: its descriptor does not
+    // depend on other workgroups, so it is read before the barrier
+    int4 head = make_int4(0, 0, 0, 0);
+    bool have = false;
+    auto prefetch = [&](int ci) {
         const int c = s_cols[ci];
         const int q = cs[c] + slot0;
-        st.mine = q < cs[c + 1];
-        st.gd = st.mine ? gdesc[q] : make_int4(0, 0, 0, 0);
-        const int n = st.gd.y - st.gd.x;
-        int incl = n;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            int t = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += t;
-        }
-        if (lane == 63 || threadIdx.x == blockDim.x - 1) s_wsum[wave] = incl;
-        if (threadIdx.x == 0) s_staged = 0;
-        __syncthreads();
-        int base = 0;
-        for (int w = 0; w < wave; w++) base += s_wsum[w];
-        const int off = base + incl - n;
-        st.off = (st.mine && off + n <= cap) ? off : -1;
-        if (st.off >= 0) {
-            for (int j = 0; j < n; j++) s_map[off + j] = st.gd.x + j;
-            atomicMax(&s_staged, off + n);
-        }
-        __syncthreads();
-        st.nd2 = s_staged * kRec2;
-#pragma unroll
-        for (int j = 0; j < kStage; j++) {
-            const int i = threadIdx.x + j * blockDim.x;
-            if (i < st.nd2) {
-                const int a = i / kRec2, part = i - a * kRec2;
-                st.buf[j] = grec[(size_t)s_map[a] * kRec2 + part];
-            }
-        }
-        (void)nwaves;
+        have = q < cs[c + 1];
+        if (have) head = gdesc[q];
     };
-    // Moves the staged words into LDS once the barrier has passed.
-    auto stage_commit = [&]() {
-#pragma unroll
-        for (int j = 0; j < kStage; j++) {
-            const int i = threadIdx.x + j * blockDim.x;
-            if (i < st.nd2) s_arena[i] = st.buf[j];
-        }
-        __syncthreads();
-    };
-    // Runs colour list position ci (its staging is committed).
     auto run_colour = [&](int ci, int mode, bool last, double& mx) {
         const int c = s_cols[ci];
-        if (st.mine) {
-            const double2* recs = st.off >= 0 ? s_arena + st.off * kRec2 : grec + (size_t)st.gd.x * kRec2;
-            process_group(P, st.gd, recs, mode, last, mx);
-        }
+        if (have) process_group(P, head, grec + (size_t)head.x * kRec2, mode, last, mx);
         for (int q = cs[c] + slot0 + stride; q < cs[c + 1]; q += stride) {
             const int4 gd = gdesc[q];
             process_group(P, gd, grec + (size_t)gd.x * kRec2, mode, last, mx);
@@ -643,26 +594,16 @@ __global__ __launch_bounds__(256) void k_pgs(PgsArgs P) {
                 process_group(P, gd, grec + (size_t)gd.x * kRec2, mode, last, mx);
             }
     };
-    // Ends a colour: wait for this lane's deltaV stores, stage the next
-    // colour, meet the other workgroups, commit the stage.
+    // Ends a colour: wait for this lane's deltaV stores, read the next
+    // colour's descriptor, meet the other workgroups.
     auto next = [&](int ci_next) -> bool {
         __builtin_amdgcn_s_waitcnt(0);
-        if (ci_next >= 0) {
-            stage_issue(ci_next);
-        } else {
-            __syncthreads();
-            st.mine = false;
-            st.nd2 = 0;
-        }
-        if (!grid_barrier<kSingleBlock>(P, epoch)) return false;
-        if (ci_next >= 0) stage_commit();
-        return true;
+        if (ci_next >= 0) prefetch(ci_next);
+        __syncthreads();
+        return grid_barrier<kSingleBlock>(P, epoch);
     };
 
-    if (ncols > 0) {
-        stage_issue(0);
-        stage_commit();
-    }
+    if (ncols > 0) prefetch(0);
     const int first_after_tail = ncols > 0 ? 0 : -1;
     if (P.warm_start) {
         double dummy = 0;
@@ -718,8 +659,6 @@ int pgs_max_blocks(int block, bool* ok) {
     *ok = per_cu > 0;
     return n_cu * (per_cu > 0 ? per_cu : 1);
 }
-
-int pgs_stage_capacity(int block) { return std::min(kArenaMax, block * kStage / kRec2); }
 
 hipError_t launch_pgs(hipStream_t s, const PgsArgs& P, int grid, int block) {
     if (grid <= 1) {
@@ -1056,14 +995,14 @@ void launch_colour_keys(hipStream_t s, int n_groups, const int* colour, uint64_t
 }
 void launch_colour_layout(hipStream_t s, int n_groups, const uint64_t* sorted, const int* group_start,
                           const int* group_a, const int* group_b, int* colour_start, int* sizes, int* pos_start,
-                          int* order, int* group_desc, int* d_total, void* scan_temp) {
+                          int* order, int* group_desc, int* pos_a, int* d_total, void* scan_temp) {
     hipLaunchKernelGGL(k_colour_starts, dim3(1), dim3(128), 0, s, n_groups, sorted, colour_start);
     if (n_groups <= 0) return;
     dim3 g((n_groups + 255) / 256), b(256);
     hipLaunchKernelGGL(k_sorted_sizes, dim3((n_groups + 256) / 256), b, 0, s, n_groups, sorted, group_start, sizes);
     exclusive_scan(s, sizes, pos_start, n_groups + 1, d_total, scan_temp);
     hipLaunchKernelGGL(k_solve_order, g, b, 0, s, n_groups, sorted, group_start, pos_start, group_a, group_b, order,
-                       group_desc);
+                       group_desc, pos_a);
 }
 
 }  // namespace am

@@ -13,6 +13,7 @@ struct SetupArgs {
     const int* order;    // sweep position -> contact index
     double* rec;         // kRecDoubles per sweep position
     int* meta;           // int4 per sweep position: solve body a, solve body b, clamp, 0
+    const int* pos_a;    // first solve body of the group of each sweep position
     int* sstate;         // contact state per sweep position
     double dt;
     double feedback;
@@ -62,12 +63,10 @@ void launch_colour_round(hipStream_t s, int n_groups, const int* ga, const int* 
 void launch_colour_keys(hipStream_t s, int n_groups, const int* colour, uint64_t* keys);
 void launch_colour_layout(hipStream_t s, int n_groups, const uint64_t* sorted, const int* group_start,
                           const int* group_a, const int* group_b, int* colour_start, int* sizes, int* pos_start,
-                          int* order, int* group_desc, int* d_total, void* scan_temp);
+                          int* order, int* group_desc, int* pos_a, int* d_total, void* scan_temp);
 void launch_setup(hipStream_t s, const SetupArgs& S, int cap);
 void launch_unpack(hipStream_t s, const SetupArgs& S, int cap);
 int pgs_max_blocks(int block, bool* ok);
-// contacts of one colour a workgroup can stage in LDS
-int pgs_stage_capacity(int block);
 hipError_t launch_pgs(hipStream_t s, const PgsArgs& P, int grid, int block);
 void launch_map_insert(hipStream_t s, const OldContactsView& O, int* table);
 void launch_warm_start(hipStream_t s, BodiesView B, ContactsView C, OldContactsView O, const int* n_groups, int cap,

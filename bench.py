@@ -151,7 +151,7 @@ def main() -> int:
     ap.add_argument("--learned", choices=["on", "off"], default="on",
                     help="also run the learned warm start from the same state")
     ap.add_argument("--train-steps", type=int, default=40, help="warm-up steps whose solves feed the training set")
-    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--pgs-grid", type=int, default=0)
