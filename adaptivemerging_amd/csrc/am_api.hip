@@ -652,7 +652,7 @@ int start_residual(am_ctx* c, PgsArgs G, double* out) {
     G.iterations = 0;
     G.warm_start = 1;
     fill_int(s, c->d_barrier.p, 2, 0);
-    AM_HIP(c, launch_pgs(s, G, 1, 256));
+    AM_HIP(c, launch_pgs(s, G, 1, 256, false));
     AM_HIP(c, c->d_resid.ensure(1));
     fill_int(s, (int*)c->d_resid.p, 2, 0);
     launch_start_residual(s, c->s_rec.p, c->s_meta.p, c->d_dv.p, c->d_minv.p, c->d_jinv.p, c->n_contacts, G.omega,
@@ -778,7 +778,7 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
     AM_HIP(c, c->s_start.ensure((size_t)std::max(c->n_contacts, 1) * 3));
     AM_HIP(c, hipMemcpyAsync(c->s_start.p, c->c_lam.p, (size_t)c->n_contacts * 24, hipMemcpyDeviceToDevice, s));
     AM_HIP(c, hipEventRecord(c->kev[1], s));
-    AM_HIP(c, launch_pgs(s, G, grid, block));
+    AM_HIP(c, launch_pgs(s, G, grid, block, P.pgs_plain_launch != 0));
     AM_HIP(c, hipEventRecord(c->kev[2], s));
     launch_unpack(s, S, c->n_contacts);
     AM_HIP(c, hipGetLastError());

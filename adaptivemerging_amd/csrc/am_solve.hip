@@ -660,9 +660,15 @@ int pgs_max_blocks(int block, bool* ok) {
     return n_cu * (per_cu > 0 ? per_cu : 1);
 }
 
-hipError_t launch_pgs(hipStream_t s, const PgsArgs& P, int grid, int block) {
+hipError_t launch_pgs(hipStream_t s, const PgsArgs& P, int grid, int block, bool plain) {
     if (grid <= 1) {
         hipLaunchKernelGGL(k_pgs<true>, dim3(1), dim3(block), 0, s, P);
+        return hipGetLastError();
+    }
+    if (plain) {
+        // the caller clamps grid to the co-resident limit; the barrier's spin
+        // limit turns a grid that is not co-resident into an error
+        hipLaunchKernelGGL(k_pgs<false>, dim3(grid), dim3(block), 0, s, P);
         return hipGetLastError();
     }
     void* args[] = {(void*)&P};

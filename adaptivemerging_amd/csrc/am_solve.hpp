@@ -67,7 +67,7 @@ void launch_colour_layout(hipStream_t s, int n_groups, const uint64_t* sorted, c
 void launch_setup(hipStream_t s, const SetupArgs& S, int cap);
 void launch_unpack(hipStream_t s, const SetupArgs& S, int cap);
 int pgs_max_blocks(int block, bool* ok);
-hipError_t launch_pgs(hipStream_t s, const PgsArgs& P, int grid, int block);
+hipError_t launch_pgs(hipStream_t s, const PgsArgs& P, int grid, int block, bool plain);
 void launch_map_insert(hipStream_t s, const OldContactsView& O, int* table);
 void launch_warm_start(hipStream_t s, BodiesView B, ContactsView C, OldContactsView O, const int* n_groups, int cap,
                        const int* group_start, int post, int* counters);

@@ -156,6 +156,9 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--pgs-grid", type=int, default=0)
     ap.add_argument("--pgs-block", type=int, default=0)
+    ap.add_argument("--pgs-launch", choices=["cooperative", "plain"], default="cooperative",
+                    help="plain: ordinary launch of the persistent kernel (for profilers without cooperative-launch "
+                         "support)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (calibrates the HBM byte counters) and exit")
     args = ap.parse_args()
@@ -187,7 +190,8 @@ def main() -> int:
     # the pile workload runs without merging (its bodies never settle into
     # stable stacks within the measured steps)
     params_kw = dict(iterations=args.iterations, tolerance=args.tolerance, gravity_amount=9.81,
-                     pgs_grid=args.pgs_grid, pgs_block=args.pgs_block, enable_merging=0, enable_unmerging=0)
+                     pgs_grid=args.pgs_grid, pgs_block=args.pgs_block, enable_merging=0, enable_unmerging=0,
+                     pgs_plain_launch=int(args.pgs_launch == "plain"))
     learned = args.learned == "on"
     plain = Context()
     plain.upload(scene)

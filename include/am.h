@@ -146,6 +146,9 @@ typedef struct am_params {
     /* implementation knobs (no reference counterpart) */
     int32_t pgs_block;                /* threads per workgroup in the PGS kernel (0 = auto) */
     int32_t pgs_grid;                 /* workgroups of the PGS kernel (0 = auto) */
+    int32_t pgs_plain_launch;         /* 1: launch the PGS kernel with an ordinary (not cooperative) launch,
+                                         grid clamped to the co-resident limit; for profilers that do not
+                                         support cooperative launches (0 = cooperative) */
     /* learned warm start (no reference counterpart): 0 plain previous-frame
      * start; 1 the MLP's predicted start, falling back to the plain start
      * whenever its start residual is not smaller */
