@@ -43,6 +43,27 @@ def algorithmic_bytes_per_sweep(n_contacts: int, n_dyn: int) -> int:
     return 400 * n_contacts + 176 * n_dyn
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_pgs_pile20k_traffic.md")
+
+
+def measured_traffic(args) -> dict | None:
+    """HBM bytes per PGS launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this same workload (counters scaled by a plain copy
+    of known size); None when the workload arguments differ from the
+    profiled ones."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            line = next(ln for ln in f if ln.startswith('{"workload"'))
+    except (OSError, StopIteration):
+        return None
+    rec = json.loads(line)
+    want = dict(bodies=args.bodies, warmup=args.warmup, steps=args.steps, iterations=args.iterations,
+                tolerance=args.tolerance, dt=args.dt, seed=args.seed)
+    if rec["workload"] != want:
+        return None
+    return rec
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -253,6 +274,7 @@ def main() -> int:
         state = plain.pull_state()
         cpu = run_cpu_baseline(scene, state, params_kw, args.dt, args.cpu_budget)
 
+    traffic = measured_traffic(args)
     achieved = res_p["_pgs_bytes"] / (res_p["_pgs_ms"] * 1e-3) / 1e9 if res_p["_pgs_ms"] > 0 else 0.0
     roofline = {
         "bound": "hbm",
@@ -261,7 +283,8 @@ def main() -> int:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": None,
+        "traffic": traffic["traffic_bytes_per_launch"] if traffic else None,
+        "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
         "bytes_per_launch": res_p["_pgs_bytes"] / max(res_p["_pgs_launches"], 1),
         "ms_per_launch": res_p["_pgs_ms"] / max(res_p["_pgs_launches"], 1),
     }
