@@ -70,6 +70,7 @@ class Params(C.Structure):
         ("step_accum_unmerging", C.c_int32), ("steps_between_merge_events", C.c_int32),
         ("threshold_merge", C.c_double), ("threshold_unmerge", C.c_double), ("threshold_breath", C.c_double),
         ("pgs_block", C.c_int32), ("pgs_grid", C.c_int32), ("pgs_plain_launch", C.c_int32),
+        ("pgs_schedule", C.c_int32),
         ("learned_warm_start", C.c_int32), ("collect_training", C.c_int32),
     ]
 
@@ -246,7 +247,7 @@ _DEFAULTS = dict(
     enable_merging=1, enable_merge_pinned=1, enable_merge_stable_contact=1, enable_merge_let_it_breathe=1,
     enable_unmerging=1, enable_unmerge_friction=1, enable_unmerge_normal=1, enable_unmerge_relative_motion=1,
     update_contacts_in_collections=1, step_accum_merging=3, step_accum_unmerging=3, steps_between_merge_events=10,
-    threshold_merge=1e-2, threshold_unmerge=2e-2, threshold_breath=1e-5, pgs_block=0, pgs_grid=0, pgs_plain_launch=0,
+    threshold_merge=1e-2, threshold_unmerge=2e-2, threshold_breath=1e-5, pgs_block=0, pgs_grid=0, pgs_plain_launch=0, pgs_schedule=0,
     learned_warm_start=0, collect_training=0,
 )
 

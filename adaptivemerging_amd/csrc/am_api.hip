@@ -136,6 +136,7 @@ int ensure_contact_capacity(am_ctx* c, int want, int keep) {
     AM_HIP(c, c->g_colour_b.ensure(cap));
     AM_HIP(c, c->g_keys.ensure(cap));
     AM_HIP(c, c->g_keys_sorted.ensure(cap));
+    AM_HIP(c, c->g_spos.ensure(cap));
     AM_HIP(c, c->g_sizes.ensure(cap + 1));
     AM_HIP(c, c->g_pos_start.ensure(cap + 1));
     AM_HIP(c, c->s_order.ensure(cap));
@@ -190,6 +191,8 @@ void default_params(am_params* p) {
     p->threshold_breath = 1e-5;
     p->pgs_block = 0;
     p->pgs_grid = 0;
+    p->pgs_plain_launch = 0;
+    p->pgs_schedule = 0;
 }
 
 // Derived host lists from the structure (parent/slot/sub/members).
@@ -337,6 +340,7 @@ int am_params_set(am_ctx* c, const am_params* p) {
     if (p->pgs_block < 0 || p->pgs_block > 256 || (p->pgs_block % 64) != 0)
         return fail(c, AM_ERR_INVALID, "pgs_block must be 0 or a multiple of 64 up to 256");
     if (p->pgs_grid < 0) return fail(c, AM_ERR_INVALID, "pgs_grid must be >= 0");
+    if (p->pgs_schedule != 0 && p->pgs_schedule != 1) return fail(c, AM_ERR_INVALID, "pgs_schedule must be 0 or 1");
     if (p->pgs_block != c->params.pgs_block) c->pgs_max_grid = 0;
     c->params = *p;
     return AM_OK;
@@ -392,8 +396,11 @@ int build_layout(am_ctx* c) {
     AM_HIP(c, c->s_colour_start.ensure(kMaxColours + 2));
     AM_HIP(c, c->s_group_desc.ensure((size_t)std::max(ng, 1) * 4));
     launch_colour_layout(s, ng, c->g_keys_sorted.p, c->g_start.p, c->g_a.p, c->g_b.p, c->s_colour_start.p,
-                         c->g_sizes.p, c->g_pos_start.p, c->s_order.p, c->s_group_desc.p, c->s_pos_a.p, sc + 9,
-                         c->d_scan_temp.p);
+                         c->g_sizes.p, c->g_pos_start.p, c->s_order.p, c->s_group_desc.p, c->s_pos_a.p,
+                         c->g_spos.p, sc + 9, c->d_scan_temp.p);
+    AM_HIP(c, c->s_flow.ensure((size_t)std::max(ng, 1) * 4));
+    launch_flow_ranks(s, c->n_ent, ng, c->d_flags.p, c->inc_start.p, c->inc_count.p, c->inc_items.p, c->g_spos.p,
+                      c->s_group_desc.p, c->s_flow.p);
     AM_HIP(c, hipGetLastError());
     int cs[kMaxColours + 2];
     AM_HIP(c, hipMemcpyAsync(cs, c->s_colour_start.p, sizeof cs, hipMemcpyDeviceToHost, s));
@@ -699,6 +706,9 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
     S.fric_ovr = P.friction_override;
     S.post = main_solve ? post : 0;
     S.in_collection = main_solve ? 0 : 1;
+    S.lam_alt = nullptr;
+    S.iters_used = nullptr;
+    const bool flow = P.pgs_schedule == 0;
     fill_int(s, c->d_scalars.p + 1, 1, c->n_contacts);
     for (auto& e : c->kev)
         if (!e) AM_HIP(c, hipEventCreate(&e));
@@ -777,9 +787,47 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
     AM_HIP(c, hipEventRecord(c->lev[1], s));
     AM_HIP(c, c->s_start.ensure((size_t)std::max(c->n_contacts, 1) * 3));
     AM_HIP(c, hipMemcpyAsync(c->s_start.p, c->c_lam.p, (size_t)c->n_contacts * 24, hipMemcpyDeviceToDevice, s));
-    AM_HIP(c, hipEventRecord(c->kev[1], s));
-    AM_HIP(c, launch_pgs(s, G, grid, block, P.pgs_plain_launch != 0));
-    AM_HIP(c, hipEventRecord(c->kev[2], s));
+    if (flow) {
+        const size_t n_dv = (size_t)c->n_ent * 6;
+        AM_HIP(c, c->s_lam_alt.ensure((size_t)c->n_contacts * 3));
+        AM_HIP(c, c->s_lrec.ensure((size_t)c->n_contacts * 72));
+        AM_HIP(c, c->s_urec.ensure((size_t)c->n_contacts * 16));
+        launch_lane_records(s, c->s_rec.p, c->d_scalars.p + 1, c->n_contacts, S.compliance, c->s_lrec.p,
+                            c->s_urec.p);
+        AM_HIP(c, c->dv_alt.ensure(std::max(n_dv, (size_t)6)));
+        AM_HIP(c, c->f_cnt.ensure(std::max(c->n_ent, 1)));
+        AM_HIP(c, c->d_pass_done.ensure(iterations + 2));
+        fill_int(s, (int*)c->f_cnt.p, c->n_ent, 0);
+        fill_int(s, (int*)c->d_pass_done.p, iterations + 2, 0);
+        AM_HIP(c, hipMemcpyAsync(c->dv_alt.p, c->d_dv.p, n_dv * sizeof(double), hipMemcpyDeviceToDevice, s));
+        if (c->flow_max_grid == 0) {
+            bool ok = false;
+            c->flow_max_grid = pgs_flow_max_blocks(&ok);
+            if (!ok) c->flow_max_grid = 1;
+        }
+        // one wave per workgroup, each walking every W-th step of up to 4 groups
+        int fgrid = P.pgs_grid > 0 ? P.pgs_grid : std::min((c->n_groups + 3) / 4, 512);
+        fgrid = std::max(1, std::min(fgrid, c->flow_max_grid));
+        FlowArgs F;
+        F.P = G;
+        F.flow = c->s_flow.p;
+        F.cnt = c->f_cnt.p;
+        F.dv_alt = c->dv_alt.p;
+        F.lam_alt = c->s_lam_alt.p;
+        F.pass_done = c->d_pass_done.p;
+        F.lrec = c->s_lrec.p;
+        F.urec = c->s_urec.p;
+        F.n_waves = (unsigned)fgrid;
+        S.lam_alt = c->s_lam_alt.p;
+        S.iters_used = G.iters_used;
+        AM_HIP(c, hipEventRecord(c->kev[1], s));
+        AM_HIP(c, launch_pgs_flow(s, F, fgrid, P.pgs_plain_launch != 0, (int)n_dv));
+        AM_HIP(c, hipEventRecord(c->kev[2], s));
+    } else {
+        AM_HIP(c, hipEventRecord(c->kev[1], s));
+        AM_HIP(c, launch_pgs(s, G, grid, block, P.pgs_plain_launch != 0));
+        AM_HIP(c, hipEventRecord(c->kev[2], s));
+    }
     launch_unpack(s, S, c->n_contacts);
     AM_HIP(c, hipGetLastError());
     if (collect) {
@@ -794,7 +842,9 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
         c->ms_learned = learned ? lms : 0.0;
         int berr = 0;
         AM_HIP(c, hipMemcpy(&berr, c->d_barrier.p + 1, sizeof(int), hipMemcpyDeviceToHost));
-        if (berr) return fail(c, AM_ERR_HIP, "PGS grid barrier did not complete (blocks not co-resident)");
+        if (berr)
+            return fail(c, AM_ERR_HIP, flow ? "PGS dataflow kernel stopped making progress (workgroups not co-resident?)"
+                                            : "PGS grid barrier did not complete (blocks not co-resident)");
         float ms = 0;
         (void)hipEventElapsedTime(&ms, c->kev[1], c->kev[2]);
         c->ms_pgs_kernel = ms;

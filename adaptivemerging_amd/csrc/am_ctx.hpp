@@ -139,6 +139,11 @@ struct am_ctx {
     DBuf<double> s_rec, s_lam;
     DBuf<int> s_meta, s_state, d_barrier, s_group_desc, s_pos_a;
     DBuf<unsigned long long> s_iter_max;
+    // dataflow schedule: sorted position per group, ranks, counters, odd-pass buffers
+    DBuf<int> g_spos, s_flow;
+    DBuf<unsigned> f_cnt, d_pass_done;
+    DBuf<double> dv_alt, s_lam_alt, s_lrec, s_urec;
+    int flow_max_grid = 0;
 
     // scalars on device: 0 n_pairs, 1 n_contacts, 2 n_groups, 3 total scratch, 4 overflow, 5 remaining,
     // 6 iters_used, 7 bad, 8 repaired, 9 n_sweep

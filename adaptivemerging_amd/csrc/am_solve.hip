@@ -185,16 +185,46 @@ __global__ void k_sorted_sizes(int n_groups, const uint64_t* sorted, const int* 
 }
 
 __global__ void k_solve_order(int n_groups, const uint64_t* sorted, const int* group_start, const int* pos_start,
-                              const int* group_a, const int* group_b, int* order, int* group_desc, int* pos_a) {
+                              const int* group_a, const int* group_b, int* order, int* group_desc, int* pos_a,
+                              int* spos) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n_groups) return;
     int g = (int)(uint32_t)sorted[q];
+    spos[g] = q;
     int s = group_start[g], e = group_start[g + 1], o = pos_start[q];
     for (int k = s; k < e; k++) {
         order[o + (k - s)] = k;
         pos_a[o + (k - s)] = group_a[g];
     }
     reinterpret_cast<int4*>(group_desc)[q] = make_int4(o, o + (e - s), group_a[g], group_b[g]);
+}
+
+// Dataflow ranks: for every dynamic body a with deg_a incident groups, the rank
+// of each of its groups among them in sorted (colour-major) order.  flow[q] =
+// {rank_a, deg_a, rank_b, deg_b} for sorted group q with solve bodies A, B
+// (deg 0 for a pinned body: never waited on, never written).
+__global__ void k_flow_ranks(int n_ent, const int* bflags, const int* inc_start, const int* inc_count,
+                             const int* inc_items, const int* spos, const int* group_desc, int* flow) {
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n_ent || !dynamic_body(bflags, a)) return;
+    const int s = inc_start[a], d = inc_count[a];
+    for (int i = 0; i < d; i++) {
+        const int qi = spos[inc_items[s + i]];
+        int r = 0;
+        for (int j = 0; j < d; j++) r += spos[inc_items[s + j]] < qi ? 1 : 0;
+        const int side = group_desc[4 * qi + 2] == a ? 0 : 2;
+        flow[4 * qi + side] = r;
+        flow[4 * qi + side + 1] = d;
+    }
+}
+
+void launch_flow_ranks(hipStream_t s, int n_ent, int n_groups, const int* bflags, const int* inc_start,
+                       const int* inc_count, const int* inc_items, const int* spos, const int* group_desc,
+                       int* flow) {
+    fill_int(s, flow, 4 * n_groups, 0);
+    if (n_ent > 0 && n_groups > 0)
+        hipLaunchKernelGGL(k_flow_ranks, dim3((n_ent + 255) / 256), dim3(256), 0, s, n_ent, bflags, inc_start,
+                           inc_count, inc_items, spos, group_desc, flow);
 }
 
 // ---------------------------------------------------------------------------
@@ -343,8 +373,13 @@ __global__ void k_unpack(SetupArgs S) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= *S.n_contacts) return;
     int c = S.order[k];
-    const double* rec = &S.rec[(size_t)k * kRecDoubles];
+    double* rec = &S.rec[(size_t)k * kRecDoubles];
     ContactsView C = S.C;
+    if (S.lam_alt && (*S.iters_used & 1)) {
+        // dataflow schedule: an odd last sweep left lambda in the odd-sweep buffer
+#pragma unroll
+        for (int r = 0; r < 3; r++) rec[kRecLambda + r] = S.lam_alt[3 * k + r];
+    }
 #pragma unroll
     for (int r = 0; r < 3; r++) {
         C.lam[c * 3 + r] = rec[kRecLambda + r];
@@ -625,7 +660,7 @@ __global__ __launch_bounds__(256) void k_pgs(PgsArgs P) {
             run_colour(ci, 1, last, mx);
             if (ci == ncols - 1 && !has_tail) {
                 double wm = wave_max(mx);
-                if ((threadIdx.x & 63) == 0 && wm > 0)
+                if ((threadIdx.x & 63) == 0 && !(wm <= 0))
                     atomicMax(&P.iter_max[it], (unsigned long long)__double_as_longlong(wm));
             }
             const int nx = ci + 1 < ncols ? ci + 1 : (has_tail ? -1 : first_after_tail);
@@ -634,7 +669,7 @@ __global__ __launch_bounds__(256) void k_pgs(PgsArgs P) {
         if (has_tail) {
             run_tail(1, last, mx);
             double wm = wave_max(mx);
-            if ((threadIdx.x & 63) == 0 && wm > 0)
+            if ((threadIdx.x & 63) == 0 && !(wm <= 0))
                 atomicMax(&P.iter_max[it], (unsigned long long)__double_as_longlong(wm));
             if (!next(first_after_tail)) return;
         }
@@ -673,6 +708,439 @@ hipError_t launch_pgs(hipStream_t s, const PgsArgs& P, int grid, int block, bool
     }
     void* args[] = {(void*)&P};
     return hipLaunchCooperativeKernel((const void*)k_pgs<false>, dim3(grid), dim3(block), args, 0, s);
+}
+
+// ---------------------------------------------------------------------------
+// Dataflow schedule of the same sweeps.
+//
+// Every dynamic body e has a counter cnt[e] of the group updates applied to it.
+// Sorted group q is the rank_a-th of the deg_a groups on its body A (and the
+// rank_b-th of deg_b on B), so in pass p it may start once cnt[A] == p*deg_a +
+// rank_a and cnt[B] == p*deg_b + rank_b, and it sets them one higher when it is
+// done.  Each body therefore receives its updates in sorted (colour-major)
+// order, the order the coloured schedule and the exported sweep order define,
+// and groups on disjoint bodies commute: the result is bit-identical to the
+// coloured schedule, with no grid-wide barrier.  Pass 0 is confidentWarmStart,
+// passes 1..iterations are sweeps.
+//
+// One wave runs one group at a time.  Lanes 0..11 hold one deltaV component
+// each (lanes 0-5: body A v.xyz w.xyz, lanes 6-11: body B), so a row's J.dv is
+// a product per lane and a fixed-shape DPP reduction in the reference's
+// summation order, ((p0 + p1) + p2) + ((p3 + p4) + p5) per body and then A + B,
+// and the lambda update runs once per wave on uniform values.  Wave w owns the
+// sorted groups w, w + W, ... and walks them pass by pass in order; the global
+// minimum (pass, group) not yet done always has its dependencies met, so with
+// every wave resident the schedule cannot deadlock, and bounded spins turn a
+// wave that stops making progress into an error flag instead of a hang.
+//
+// Early exit (PGS.java:190) needs the max |dl| of sweep t before sweep t+1 is
+// final.  Sweeps double-buffer lambda (odd sweeps in lam_alt) and deltaV (odd
+// passes in dv_alt), and a wave starts sweep p only once sweep p-2 is known not
+// to have converged, so at most one sweep runs speculatively and its buffers
+// are simply not selected.  The last sweep (contact states) waits for the
+// verdict on the one before it, so it never runs speculatively.
+
+AMD_INLINE unsigned ld_cnt(const unsigned* p) {
+    return __hip_atomic_load(const_cast<unsigned*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+AMD_INLINE void st_cnt(unsigned* p, unsigned v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+AMD_INLINE int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+AMD_INLINE double uniform_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)b), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+// DPP move of a double (bound_ctrl: lanes without a source read 0)
+template <int kCtrl>
+AMD_INLINE double dpp_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, kCtrl, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), kCtrl, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+constexpr int kQuadLane0 = 0x00, kQuadLane1 = 0x55, kQuadLane2 = 0xAA;  // quad_perm broadcasts
+constexpr int kRowHalfMirror = 0x141, kRowMirror = 0x140;
+
+// Contact.getJdv (Contact.java:362-378) for the group held by a 16-lane row.
+// Quads of the row hold A.v, A.w, B.v, B.w (lanes 0-2 of each quad, lane 3
+// idle), p = J_i dv_i.  Every lane of the row gets the reference's sum
+// ((p0 + p1) + p2) per quad, (v + w) per body, then A + B; the partner quads
+// come through mirrors, which only swap the operands of an addition, and IEEE
+// addition commutes.
+AMD_INLINE double row_jdv(double p) {
+    const double s = (dpp_d<kQuadLane0>(p) + dpp_d<kQuadLane1>(p)) + dpp_d<kQuadLane2>(p);
+    const double h = s + dpp_d<kRowHalfMirror>(s);
+    return h + dpp_d<kRowMirror>(h);
+}
+
+// The division (D lambda - omega (b + J.dv)) / (D + C) of a row.  The
+// denominator is fixed during a solve, so the denominator-only half of the
+// hardware division sequence (reciprocal estimate and two Newton steps) is
+// done once per row in setup; per sweep the quotient is then q0 = n y,
+// r = fma(-d, q0, n), q = fma(r, y, q0), the same operations the sequence
+// performs when v_div_scale leaves its operands unscaled and v_div_fixup has
+// nothing to fix.  Both |n| and |d| in [2^-250, 2^250] guarantees that (no
+// exponent gap near 768, no denormal quotient or reciprocal, nonzero finite
+// operands); anything else takes the full division.  The result is the
+// correctly rounded quotient either way.
+AMD_INLINE double div_recip(double den) {
+    const double a = fabs(den);
+    if (!(a >= 0x1p-250 && a <= 0x1p250)) return 0.0;  // 0: always divide
+    double y = __builtin_amdgcn_rcp(den);
+    double e = __builtin_fma(-den, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-den, y, 1.0);
+    return __builtin_fma(y, e, y);
+}
+AMD_INLINE double div_row(double num, double den, double y) {
+    const double a = fabs(num);
+    if (y != 0.0 && a >= 0x1p-250 && a <= 0x1p250) {
+        const double q0 = num * y;
+        const double r = __builtin_fma(-den, q0, num);
+        return __builtin_fma(r, y, q0);
+    }
+    return num / den;
+}
+
+// Per sweep position, for the dataflow kernel:
+// lrec: 12 lanes x 3 rows x {J entry, update coefficient} (v lanes: the J
+//       entry, scaled by minv*dl at update time; w lanes: jinv J.w);
+// urec: 16 doubles, the contact's uniform operands: b[3], D[3], D+C [3],
+//       the refined reciprocals of D+C [3], mu, clamp, 2 pad.
+constexpr int kURec = 16;
+__global__ void k_lane_records(const double* rec, const int* n_contacts, double compliance, double* lrec,
+                               double* urec) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = t / 12, i = t % 12;
+    if (k >= *n_contacts) return;
+    const double* r58 = rec + (size_t)k * kRecDoubles;
+    const int c = i % 6;
+    const bool side_b = i >= 6;
+    double* out = lrec + ((size_t)k * 12 + i) * 6;
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const double* q = r58 + r * kRecRow;  // aw, bv, bw, ta, tb
+        double j, u;
+        if (c < 3) {
+            j = side_b ? q[3 + c] : -q[3 + c];
+            u = j;
+        } else {
+            j = side_b ? q[6 + c - 3] : q[c - 3];
+            u = side_b ? q[12 + c - 3] : q[9 + c - 3];
+        }
+        out[2 * r] = j;
+        out[2 * r + 1] = u;
+    }
+    if (i == 0) {
+        double* u = urec + (size_t)k * kURec;
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            const double d = r58[kRecD + r];
+            const double den = d + compliance;
+            u[r] = r58[kRecB + r];
+            u[3 + r] = d;
+            u[6 + r] = den;
+            u[9 + r] = div_recip(den);
+        }
+        u[12] = r58[kRecMu];
+        u[13] = r58[kRecClamp];
+        u[14] = 0.0;
+        u[15] = 0.0;
+    }
+}
+void launch_lane_records(hipStream_t s, const double* rec, const int* n_contacts, int cap, double compliance,
+                         double* lrec, double* urec) {
+    if (cap > 0)
+        hipLaunchKernelGGL(k_lane_records, dim3((cap * 12 + 255) / 256), dim3(256), 0, s, rec, n_contacts,
+                           compliance, lrec, urec);
+}
+
+constexpr long kFlowSpinLimit = 1L << 22;  // polls of one wait (each a memory round trip)
+
+// Per-contact operands of one row: this lane's J entries and coefficients of
+// the three rows, and the contact's uniform block and lambda.
+struct WaveRec {
+    double jc[6];
+    double u[kURec - 2];
+    double lam[3];
+};
+AMD_INLINE void load_wave_rec(const FlowArgs& F, int k, int li, int p, WaveRec& w) {
+    const PgsArgs& P = F.P;
+    const int q = li >> 2, j = li & 3;
+    const int i = q * 3 + (j < 3 ? j : 0);  // index in the 12-lane record
+    const double2* lr = reinterpret_cast<const double2*>(F.lrec + ((size_t)k * 12 + i) * 6);
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        const double2 v = lr[t];
+        w.jc[2 * t] = v.x;
+        w.jc[2 * t + 1] = v.y;
+    }
+    const double2* ur = reinterpret_cast<const double2*>(F.urec + (size_t)k * kURec);
+#pragma unroll
+    for (int t = 0; t < kURec / 2 - 1; t++) {
+        const double2 v = ur[t];
+        w.u[2 * t] = v.x;
+        w.u[2 * t + 1] = v.y;
+    }
+    const bool from_alt = p >= 1 && ((p - 1) & 1);
+    const double* lp = from_alt ? &F.lam_alt[3 * k] : &P.rec[(size_t)k * kRecDoubles + kRecLambda];
+#pragma unroll
+    for (int r = 0; r < 3; r++) w.lam[r] = lp[r];
+}
+
+// One contact of a row-held group in pass p (PGS.java:100-184; pass 0 is
+// confidentWarmStart, PGS.java:250-256).  dv is this lane's deltaV component.
+AMD_INLINE void row_contact(const FlowArgs& F, const WaveRec& c, int k, int p, bool last, bool lead, bool vlane,
+                            double minv, double& dv, double& mx) {
+    const PgsArgs& P = F.P;
+    double lam[3] = {c.lam[0], c.lam[1], c.lam[2]};
+    if (p == 0) {
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            const double m = vlane ? minv * lam[r] : lam[r];
+            dv = m * c.jc[2 * r + 1] + dv;
+        }
+        return;
+    }
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const double jd = row_jdv(c.jc[2 * r] * dv);
+        const double prev = lam[r];
+        const double num = c.u[3 + r] * lam[r] - P.omega * (c.u[r] + jd);
+        double l = div_row(num, c.u[6 + r], c.u[9 + r]);
+        if (c.u[13] != 0.0) {
+            if (r == 0) {
+                l = jmax(0, l);
+            } else {
+                const double limit = c.u[12] * lam[0];
+                l = jmax(l, -limit);
+                l = jmin(l, limit);
+            }
+        }
+        lam[r] = l;
+        const double diff = l - prev;
+        const double m = vlane ? minv * diff : diff;
+        dv = m * c.jc[2 * r + 1] + dv;
+        mx = jmax(mx, fabs(diff));
+    }
+    if (lead) {
+        double* dst = (p & 1) ? &F.lam_alt[3 * k] : &P.rec[(size_t)k * kRecDoubles + kRecLambda];
+        dst[0] = lam[0];
+        dst[1] = lam[1];
+        dst[2] = lam[2];
+    }
+    if (last) {
+        // Contact.updateContactState (Contact.java:385-398)
+        const double w1 = c.u[1] + row_jdv(c.jc[2] * dv);
+        const double w2 = c.u[2] + row_jdv(c.jc[4] * dv);
+        if (lead) {
+            int st;
+            if (fabs(lam[0]) <= 1e-14) st = AM_STATE_BROKEN;
+            else if (fabs(w1) > P.sliding) st = AM_STATE_ONEDGE;
+            else if (fabs(w2) > P.sliding) st = AM_STATE_ONEDGE;
+            else st = AM_STATE_CLEAR;
+            P.sstate[k] = st;
+        }
+    }
+}
+
+// Steps: up to 4 consecutive sorted groups of one parallel colour (no shared
+// dynamic body), one per 16-lane row of a wave; the sequential tail colour
+// gives one group per step.  Wave w runs steps w, w + W, ... of every pass in
+// order.
+constexpr int kRowsPerWave = 4;
+
+__global__ __launch_bounds__(64) void k_pgs_flow(FlowArgs F) {
+    const PgsArgs& P = F.P;
+    const int* cs = P.colour_start;
+    __shared__ int s_sp[kMaxColours + 2];  // first step of each colour, then the total
+    if (threadIdx.x == 0) {
+        int n = 0;
+        for (int c = 0; c <= kMaxColours; c++) {
+            s_sp[c] = n;
+            const int g = cs[c + 1] - cs[c];
+            n += c < kMaxColours ? (g + kRowsPerWave - 1) / kRowsPerWave : g;
+        }
+        s_sp[kMaxColours + 1] = n;
+    }
+    __syncthreads();
+    const int n_steps = s_sp[kMaxColours + 1];
+    const int W = gridDim.x;
+    const int w = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int row = lane >> 4, li = lane & 15, quad = li >> 2, jj = li & 3;
+    const bool lead = li == 0;
+    const bool mine = jj < 3;               // holds a deltaV component
+    const bool side_b = quad >= 2;
+    const bool vlane = (quad & 1) == 0;     // a linear-velocity component
+    const int comp = (quad & 1) * 3 + (jj < 3 ? jj : 0);
+    const int first = P.warm_start ? 0 : 1;
+    const int lastp = P.iterations;
+    const int4* gdesc = reinterpret_cast<const int4*>(P.group_desc);
+    const int4* fdesc = reinterpret_cast<const int4*>(F.flow);
+    int chk = 0;  // sweeps 1..chk are known not to have converged
+    // counters of the last step, released once its deltaV stores are done
+    // (the wait overlaps the next step's first poll); lanes 0 and 1 of a row
+    bool pend = false;
+    unsigned* pend_ctr = nullptr;
+    unsigned pend_val = 0;
+    auto release = [&]() {
+        __builtin_amdgcn_s_waitcnt(0);
+        if (pend_ctr) st_cnt(pend_ctr, pend_val);
+        pend = false;
+    };
+
+    for (int p = first; p <= lastp; p++) {
+        // pass gates: sweep p-2 must be known not converged (p-1 for the last sweep)
+        if (P.check_tolerance) {
+            const int need = (p == lastp && p >= 2) ? p - 1 : p - 2;
+            long spins = 0;
+            while (chk < need) {
+                const unsigned d = (unsigned)uniform((int)ld_cnt(&F.pass_done[chk + 1]));
+                if (d == F.n_waves) {
+                    const double m = uniform_d(__longlong_as_double((long long)__hip_atomic_load(
+                        &P.iter_max[chk], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+                    if (m < P.tolerance) return;  // converged at sweep chk + 1
+                    chk++;
+                    spins = 0;
+                    continue;
+                }
+                if (++spins > kFlowSpinLimit ||
+                    ((spins & 255) == 0 && uniform(__hip_atomic_load(P.error, __ATOMIC_RELAXED,
+                                                                      __HIP_MEMORY_SCOPE_AGENT)))) {
+                    if (lane == 0) atomicOr(P.error, 1);
+                    return;
+                }
+            }
+        }
+        double mx = 0.0;
+        double* cur = (p & 1) ? F.dv_alt : P.dv;
+        double* prv = (p & 1) ? P.dv : F.dv_alt;
+        int c = 0;  // colour of the current step (steps ascend)
+        for (int st = w; st < n_steps; st += W) {
+            while (s_sp[c + 1] <= st) c++;
+            const int per = c < kMaxColours ? kRowsPerWave : 1;
+            const int q0 = cs[c] + (st - s_sp[c]) * per;
+            const int count = min(per, cs[c + 1] - q0);
+            const bool active = row < count;
+            const int q = q0 + (active ? row : 0);
+            const int4 gd = gdesc[q];
+            const int4 fd = fdesc[q];
+            // the group's first operands do not depend on other waves
+            WaveRec r0, r1;
+            load_wave_rec(F, gd.x, li, p, r0);
+            // wait until both bodies of every row's group have received every
+            // earlier update (lanes 0 and 1 of a row poll A's and B's counter)
+            {
+                const bool lb = li == 1;
+                const unsigned tgt = lb ? (unsigned)p * (unsigned)fd.w + (unsigned)fd.z
+                                        : (unsigned)p * (unsigned)fd.y + (unsigned)fd.x;
+                const bool need = active && li < 2 && (lb ? fd.w != 0 : fd.y != 0);
+                const unsigned* ctr = &F.cnt[lb ? gd.w : gd.z];
+                long spins = 0;
+                while (true) {
+                    unsigned v = tgt;
+                    if (need) v = ld_cnt(ctr);
+                    if (pend) release();
+                    if (__all(v == tgt)) break;
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > kFlowSpinLimit ||
+                        ((spins & 255) == 0 && uniform(__hip_atomic_load(P.error, __ATOMIC_RELAXED,
+                                                                          __HIP_MEMORY_SCOPE_AGENT)))) {
+                        if (lane == 0) atomicOr(P.error, 1);
+                        return;
+                    }
+                }
+            }
+            const int e = side_b ? gd.w : gd.z;
+            const int rank = side_b ? fd.z : fd.x;
+            double dv = 0.0, minv = 0.0;
+            bool dyn = false;
+            if (active && mine) {
+                dv = ld_coherent(&(rank == 0 ? prv : cur)[e * 6 + comp]);
+                minv = P.minv[e];
+                dyn = (P.bflags[e] & AM_BODY_PINNED) == 0;
+            }
+            const bool last = p == lastp;
+            const int n = active ? gd.y - gd.x : 0;
+            int nmax = n;
+            nmax = max(nmax, __shfl_xor(nmax, 16, 64));
+            nmax = max(nmax, __shfl_xor(nmax, 32, 64));
+            // records ping-pong between r0 and r1: the next contact's loads are
+            // in flight while this one is solved; rows with fewer contacts idle
+            for (int t = 0; t < nmax; t += 2) {
+                if (t + 1 < n) load_wave_rec(F, gd.x + t + 1, li, p, r1);
+                if (t < n) row_contact(F, r0, gd.x + t, p, last, lead, vlane, minv, dv, mx);
+                if (t + 1 >= nmax) break;
+                if (t + 2 < n) load_wave_rec(F, gd.x + t + 2, li, p, r0);
+                if (t + 1 < n) row_contact(F, r1, gd.x + t + 1, p, last, lead, vlane, minv, dv, mx);
+            }
+            // publish deltaV now, the counters once the stores are done
+            if (active && mine && dyn) st_coherent(&cur[e * 6 + comp], dv);
+            pend = true;
+            pend_ctr = nullptr;
+            if (active && li == 0 && fd.y) {
+                pend_ctr = &F.cnt[gd.z];
+                pend_val = (unsigned)p * (unsigned)fd.y + (unsigned)fd.x + 1u;
+            }
+            if (active && li == 1 && fd.w) {
+                pend_ctr = &F.cnt[gd.w];
+                pend_val = (unsigned)p * (unsigned)fd.w + (unsigned)fd.z + 1u;
+            }
+        }
+        if (pend) release();
+        if (p >= 1) {
+            mx = wave_max(mx);
+            if (lane == 0) {
+                if (!(mx <= 0))  // NaN included
+                    atomicMax(&P.iter_max[p - 1], (unsigned long long)__double_as_longlong(mx));
+                __builtin_amdgcn_s_waitcnt(0);
+                __hip_atomic_fetch_add(&F.pass_done[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+}
+
+// Sweeps used (the first converged sweep, else all), then the buffers of the
+// last pass become the results.
+__global__ void k_flow_finish(PgsArgs P) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int used = P.iterations;
+    if (P.check_tolerance)
+        for (int t = 1; t <= P.iterations; t++)
+            if (__longlong_as_double((long long)P.iter_max[t - 1]) < P.tolerance) {
+                used = t;
+                break;
+            }
+    *P.iters_used = used;
+}
+__global__ void k_flow_select_dv(double* dv, const double* dv_alt, int n, const int* iters_used) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && (*iters_used & 1)) dv[i] = dv_alt[i];
+}
+
+int pgs_flow_max_blocks(bool* ok) {
+    int dev = 0, n_cu = 0, per_cu = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pgs_flow, 64, 0);
+    *ok = per_cu > 0;
+    return n_cu * (per_cu > 0 ? per_cu : 1);
+}
+
+hipError_t launch_pgs_flow(hipStream_t s, const FlowArgs& F, int grid, bool plain, int n_dv) {
+    if (plain) {
+        hipLaunchKernelGGL(k_pgs_flow, dim3(grid), dim3(64), 0, s, F);
+    } else {
+        void* args[] = {(void*)&F};
+        hipError_t e = hipLaunchCooperativeKernel((const void*)k_pgs_flow, dim3(grid), dim3(64), args, 0, s);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_flow_finish, dim3(1), dim3(64), 0, s, F.P);
+    if (n_dv > 0)
+        hipLaunchKernelGGL(k_flow_select_dv, dim3((n_dv + 255) / 256), dim3(256), 0, s, F.P.dv, F.dv_alt, n_dv,
+                           F.P.iters_used);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -1001,14 +1469,14 @@ void launch_colour_keys(hipStream_t s, int n_groups, const int* colour, uint64_t
 }
 void launch_colour_layout(hipStream_t s, int n_groups, const uint64_t* sorted, const int* group_start,
                           const int* group_a, const int* group_b, int* colour_start, int* sizes, int* pos_start,
-                          int* order, int* group_desc, int* pos_a, int* d_total, void* scan_temp) {
+                          int* order, int* group_desc, int* pos_a, int* spos, int* d_total, void* scan_temp) {
     hipLaunchKernelGGL(k_colour_starts, dim3(1), dim3(128), 0, s, n_groups, sorted, colour_start);
     if (n_groups <= 0) return;
     dim3 g((n_groups + 255) / 256), b(256);
     hipLaunchKernelGGL(k_sorted_sizes, dim3((n_groups + 256) / 256), b, 0, s, n_groups, sorted, group_start, sizes);
     exclusive_scan(s, sizes, pos_start, n_groups + 1, d_total, scan_temp);
     hipLaunchKernelGGL(k_solve_order, g, b, 0, s, n_groups, sorted, group_start, pos_start, group_a, group_b, order,
-                       group_desc, pos_a);
+                       group_desc, pos_a, spos);
 }
 
 }  // namespace am

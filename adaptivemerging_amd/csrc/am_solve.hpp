@@ -24,6 +24,8 @@ struct SetupArgs {
     int fric_ovr;
     int post;
     int in_collection;
+    const double* lam_alt;  // unpack: odd-sweep lambda of the dataflow schedule (null: coloured)
+    const int* iters_used;  // unpack: sweeps used (with lam_alt)
 };
 
 // Inputs of the persistent coloured PGS kernel.
@@ -50,6 +52,19 @@ struct PgsArgs {
     int* error;                    // set if a barrier spin limit is hit
 };
 
+// Inputs of the dataflow PGS kernel (the same sweeps without grid barriers).
+struct FlowArgs {
+    PgsArgs P;
+    const int* flow;       // int4 per sorted group: rank_a, deg_a, rank_b, deg_b
+    unsigned* cnt;         // per entity: group updates applied (zeroed before launch)
+    double* dv_alt;        // deltaV of odd passes (a copy of dv at launch)
+    double* lam_alt;       // lambda of odd sweeps, 3 per sweep position
+    unsigned* pass_done;   // per pass: waves that have finished it (zeroed before launch)
+    const double* lrec;    // per sweep position: 12 lanes x 3 rows x {J entry, update coefficient}
+    const double* urec;    // per sweep position: b, D, D+C, refined 1/(D+C), mu, clamp (16 doubles)
+    unsigned n_waves;
+};
+
 void fill_int(hipStream_t s, int* p, int n, int v);
 void fill_double(hipStream_t s, double* p, int n, double v);
 void launch_groups(hipStream_t s, int cap, const int* n_contacts, const int* cb1, const int* cb2, const int* slot,
@@ -63,11 +78,18 @@ void launch_colour_round(hipStream_t s, int n_groups, const int* ga, const int* 
 void launch_colour_keys(hipStream_t s, int n_groups, const int* colour, uint64_t* keys);
 void launch_colour_layout(hipStream_t s, int n_groups, const uint64_t* sorted, const int* group_start,
                           const int* group_a, const int* group_b, int* colour_start, int* sizes, int* pos_start,
-                          int* order, int* group_desc, int* pos_a, int* d_total, void* scan_temp);
+                          int* order, int* group_desc, int* pos_a, int* spos, int* d_total, void* scan_temp);
 void launch_setup(hipStream_t s, const SetupArgs& S, int cap);
 void launch_unpack(hipStream_t s, const SetupArgs& S, int cap);
 int pgs_max_blocks(int block, bool* ok);
 hipError_t launch_pgs(hipStream_t s, const PgsArgs& P, int grid, int block, bool plain);
+int pgs_flow_max_blocks(bool* ok);
+void launch_lane_records(hipStream_t s, const double* rec, const int* n_contacts, int cap, double compliance,
+                         double* lrec, double* urec);
+hipError_t launch_pgs_flow(hipStream_t s, const FlowArgs& F, int grid, bool plain, int n_dv);
+void launch_flow_ranks(hipStream_t s, int n_ent, int n_groups, const int* bflags, const int* inc_start,
+                       const int* inc_count, const int* inc_items, const int* spos, const int* group_desc,
+                       int* flow);
 void launch_map_insert(hipStream_t s, const OldContactsView& O, int* table);
 void launch_warm_start(hipStream_t s, BodiesView B, ContactsView C, OldContactsView O, const int* n_groups, int cap,
                        const int* group_start, int post, int* counters);

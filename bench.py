@@ -58,7 +58,7 @@ def measured_traffic(args) -> dict | None:
         return None
     rec = json.loads(line)
     want = dict(bodies=args.bodies, warmup=args.warmup, steps=args.steps, iterations=args.iterations,
-                tolerance=args.tolerance, dt=args.dt, seed=args.seed)
+                tolerance=args.tolerance, dt=args.dt, seed=args.seed, schedule=args.pgs_schedule)
     if rec["workload"] != want:
         return None
     return rec
@@ -180,6 +180,9 @@ def main() -> int:
     ap.add_argument("--pgs-launch", choices=["cooperative", "plain"], default="cooperative",
                     help="plain: ordinary launch of the persistent kernel (for profilers without cooperative-launch "
                          "support)")
+    ap.add_argument("--pgs-schedule", choices=["dataflow", "coloured"], default="dataflow",
+                    help="dataflow: per-body update counters; coloured: a grid barrier between colours "
+                         "(bit-identical results)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (calibrates the HBM byte counters) and exit")
     args = ap.parse_args()
@@ -212,7 +215,8 @@ def main() -> int:
     # stable stacks within the measured steps)
     params_kw = dict(iterations=args.iterations, tolerance=args.tolerance, gravity_amount=9.81,
                      pgs_grid=args.pgs_grid, pgs_block=args.pgs_block, enable_merging=0, enable_unmerging=0,
-                     pgs_plain_launch=int(args.pgs_launch == "plain"))
+                     pgs_plain_launch=int(args.pgs_launch == "plain"),
+                     pgs_schedule=0 if args.pgs_schedule == "dataflow" else 1)
     learned = args.learned == "on"
     plain = Context()
     plain.upload(scene)
@@ -278,7 +282,8 @@ def main() -> int:
     achieved = res_p["_pgs_bytes"] / (res_p["_pgs_ms"] * 1e-3) / 1e9 if res_p["_pgs_ms"] > 0 else 0.0
     roofline = {
         "bound": "hbm",
-        "kernel": "am::k_pgs<false> (persistent coloured PGS)",
+        "kernel": ("am::k_pgs_flow (persistent dataflow PGS)" if args.pgs_schedule == "dataflow"
+                   else "am::k_pgs<false> (persistent coloured PGS)"),
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",

@@ -44,10 +44,12 @@ def test_detect_sphere_trees(gpu_lib):
     assert_contacts_equal(ctx.contacts(), orc.contacts())
 
 
+@pytest.mark.parametrize("schedule", [0, 1])
 @pytest.mark.parametrize("iterations", [1, 7, 100])
-def test_pgs_jenga_bit_exact(gpu_lib, iterations):
+def test_pgs_jenga_bit_exact(gpu_lib, iterations, schedule):
     scene = workloads.jenga_tower(seed=2, layers=10, overlap=0.002)
-    ctx, orc = twin(scene, iterations=iterations, tolerance=1e-9, friction_override=1, friction=0.6)
+    ctx, orc = twin(scene, iterations=iterations, tolerance=1e-9, friction_override=1, friction=0.6,
+                    pgs_schedule=schedule)
     ctx.apply_gravity(); orc.apply_gravity()
     ctx.detect(0.01); orc.detect()
     it_g, last_g = ctx.solve(0.01)
@@ -128,9 +130,9 @@ def test_motion_metric(gpu_lib):
 
 @pytest.mark.parametrize("grid,block", [(2, 64), (8, 64), (32, 128), (64, 64), (128, 256)])
 def test_pgs_multiblock_bit_exact(gpu_lib, grid, block):
-    """The cooperative (grid-barrier) path of the sweep kernel."""
+    """The cooperative (grid-barrier) path of the coloured sweep kernel."""
     scene = workloads.box_cluster(seed=9, n=729, spheres=60)
-    ctx, orc = twin(scene, iterations=25, pgs_grid=grid, pgs_block=block)
+    ctx, orc = twin(scene, iterations=25, pgs_grid=grid, pgs_block=block, pgs_schedule=1)
     ctx.apply_gravity(); orc.apply_gravity()
     n = ctx.detect(0.01); orc.detect()
     lam = np.random.default_rng(2).uniform(0, 0.5, size=(n, 3))
