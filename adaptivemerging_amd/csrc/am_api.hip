@@ -789,7 +789,7 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
     AM_HIP(c, hipMemcpyAsync(c->s_start.p, c->c_lam.p, (size_t)c->n_contacts * 24, hipMemcpyDeviceToDevice, s));
     if (flow) {
         const size_t n_dv = (size_t)c->n_ent * 6;
-        AM_HIP(c, c->s_lam_alt.ensure((size_t)c->n_contacts * 3));
+        AM_HIP(c, c->s_lam_alt.ensure((size_t)c->n_contacts * 4));
         AM_HIP(c, c->s_lrec.ensure((size_t)c->n_contacts * 72));
         AM_HIP(c, c->s_urec.ensure((size_t)c->n_contacts * 16));
         launch_lane_records(s, c->s_rec.p, c->d_scalars.p + 1, c->n_contacts, S.compliance, c->s_lrec.p,

@@ -378,7 +378,7 @@ __global__ void k_unpack(SetupArgs S) {
     if (S.lam_alt && (*S.iters_used & 1)) {
         // dataflow schedule: an odd last sweep left lambda in the odd-sweep buffer
 #pragma unroll
-        for (int r = 0; r < 3; r++) rec[kRecLambda + r] = S.lam_alt[3 * k + r];
+        for (int r = 0; r < 3; r++) rec[kRecLambda + r] = S.lam_alt[4 * k + r];
     }
 #pragma unroll
     for (int r = 0; r < 3; r++) {
@@ -864,8 +864,11 @@ struct WaveRec {
     double u[kURec - 2];
     double lam[3];
 };
+AMD_INLINE const double* lam_src(const FlowArgs& F, int k, int p) {
+    const bool from_alt = p >= 1 && ((p - 1) & 1);
+    return from_alt ? &F.lam_alt[4 * k] : &F.P.rec[(size_t)k * kRecDoubles + kRecLambda];
+}
 AMD_INLINE void load_wave_rec(const FlowArgs& F, int k, int li, int p, WaveRec& w) {
-    const PgsArgs& P = F.P;
     const int q = li >> 2, j = li & 3;
     const int i = q * 3 + (j < 3 ? j : 0);  // index in the 12-lane record
     const double2* lr = reinterpret_cast<const double2*>(F.lrec + ((size_t)k * 12 + i) * 6);
@@ -882,24 +885,60 @@ AMD_INLINE void load_wave_rec(const FlowArgs& F, int k, int li, int p, WaveRec& 
         w.u[2 * t] = v.x;
         w.u[2 * t + 1] = v.y;
     }
-    const bool from_alt = p >= 1 && ((p - 1) & 1);
-    const double* lp = from_alt ? &F.lam_alt[3 * k] : &P.rec[(size_t)k * kRecDoubles + kRecLambda];
+    const double* lp = lam_src(F, k, p);
 #pragma unroll
     for (int r = 0; r < 3; r++) w.lam[r] = lp[r];
 }
 
+// A step's records are copied into LDS with DMA loads (16 B per lane per
+// block, lane l's bytes at 16 l) before the wave waits for its dependencies:
+// per contact slot, blocks 0-2 hold each lane's J/coefficient pairs and block
+// 3 holds per row the uniform block (lanes 0-7) and lambda (lanes 8-9).
+constexpr int kDmaSlots = 8;
+typedef __attribute__((address_space(3))) void lds_void;
+AMD_INLINE void dma_step(const FlowArgs& F, double2 (*arena)[4][64], int slot, int k, int li, int p) {
+    const int q = li >> 2, j = li & 3;
+    const int i = q * 3 + (j < 3 ? j : 0);
+    const double* lr = F.lrec + ((size_t)k * 12 + i) * 6;
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+        __builtin_amdgcn_global_load_lds((const void*)(lr + 2 * b), (lds_void*)&arena[slot][b][0], 16, 0, 0);
+    const double* u = F.urec + (size_t)k * kURec;
+    const double* src = li < 8 ? u + 2 * li : (li < 10 ? lam_src(F, k, p) + 2 * (li - 8) : u);
+    __builtin_amdgcn_global_load_lds((const void*)src, (lds_void*)&arena[slot][3][0], 16, 0, 0);
+}
+AMD_INLINE void lds_wave_rec(double2 (*arena)[4][64], int slot, int lane, int row, WaveRec& w) {
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+        const double2 v = arena[slot][b][lane];
+        w.jc[2 * b] = v.x;
+        w.jc[2 * b + 1] = v.y;
+    }
+    const double2* ub = &arena[slot][3][row * 16];
+#pragma unroll
+    for (int t = 0; t < kURec / 2 - 1; t++) {
+        const double2 v = ub[t];
+        w.u[2 * t] = v.x;
+        w.u[2 * t + 1] = v.y;
+    }
+    const double2 l01 = ub[8], l2 = ub[9];
+    w.lam[0] = l01.x;
+    w.lam[1] = l01.y;
+    w.lam[2] = l2.x;
+}
+
 // One contact of a row-held group in pass p (PGS.java:100-184; pass 0 is
-// confidentWarmStart, PGS.java:250-256).  dv is this lane's deltaV component.
-AMD_INLINE void row_contact(const FlowArgs& F, const WaveRec& c, int k, int p, bool last, bool lead, bool vlane,
-                            double minv, double& dv, double& mx) {
+// confidentWarmStart, PGS.java:250-256).  dv is this lane's deltaV component,
+// f its body's minv on v lanes and 1 on w lanes (PGS.java:221-245 scales the
+// v part by minv*dl and the w part by dl; 1*dl == dl exactly).  mx is the
+// running max |dl| with NaN tracked apart (Math.max keeps a NaN).
+AMD_INLINE void row_contact(const FlowArgs& F, const WaveRec& c, int k, int p, bool last, bool lead, double f,
+                            double& dv, double& mx, bool& nan) {
     const PgsArgs& P = F.P;
     double lam[3] = {c.lam[0], c.lam[1], c.lam[2]};
     if (p == 0) {
 #pragma unroll
-        for (int r = 0; r < 3; r++) {
-            const double m = vlane ? minv * lam[r] : lam[r];
-            dv = m * c.jc[2 * r + 1] + dv;
-        }
+        for (int r = 0; r < 3; r++) dv = (f * lam[r]) * c.jc[2 * r + 1] + dv;
         return;
     }
 #pragma unroll
@@ -919,12 +958,12 @@ AMD_INLINE void row_contact(const FlowArgs& F, const WaveRec& c, int k, int p, b
         }
         lam[r] = l;
         const double diff = l - prev;
-        const double m = vlane ? minv * diff : diff;
-        dv = m * c.jc[2 * r + 1] + dv;
-        mx = jmax(mx, fabs(diff));
+        dv = (f * diff) * c.jc[2 * r + 1] + dv;
+        mx = fmax(mx, fabs(diff));
+        nan = nan || diff != diff;
     }
     if (lead) {
-        double* dst = (p & 1) ? &F.lam_alt[3 * k] : &P.rec[(size_t)k * kRecDoubles + kRecLambda];
+        double* dst = (p & 1) ? &F.lam_alt[4 * k] : &P.rec[(size_t)k * kRecDoubles + kRecLambda];
         dst[0] = lam[0];
         dst[1] = lam[1];
         dst[2] = lam[2];
@@ -950,7 +989,7 @@ AMD_INLINE void row_contact(const FlowArgs& F, const WaveRec& c, int k, int p, b
 // order.
 constexpr int kRowsPerWave = 4;
 
-__global__ __launch_bounds__(64) void k_pgs_flow(FlowArgs F) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_pgs_flow(FlowArgs F) {
     const PgsArgs& P = F.P;
     const int* cs = P.colour_start;
     __shared__ int s_sp[kMaxColours + 2];  // first step of each colour, then the total
@@ -978,6 +1017,7 @@ __global__ __launch_bounds__(64) void k_pgs_flow(FlowArgs F) {
     const int lastp = P.iterations;
     const int4* gdesc = reinterpret_cast<const int4*>(P.group_desc);
     const int4* fdesc = reinterpret_cast<const int4*>(F.flow);
+    __shared__ double2 s_dma[kDmaSlots][4][64];
     int chk = 0;  // sweeps 1..chk are known not to have converged
     // counters of the last step, released once its deltaV stores are done
     // (the wait overlaps the next step's first poll); lanes 0 and 1 of a row
@@ -1014,6 +1054,7 @@ __global__ __launch_bounds__(64) void k_pgs_flow(FlowArgs F) {
             }
         }
         double mx = 0.0;
+        bool nan = false;
         double* cur = (p & 1) ? F.dv_alt : P.dv;
         double* prv = (p & 1) ? P.dv : F.dv_alt;
         int c = 0;  // colour of the current step (steps ascend)
@@ -1026,9 +1067,14 @@ __global__ __launch_bounds__(64) void k_pgs_flow(FlowArgs F) {
             const int q = q0 + (active ? row : 0);
             const int4 gd = gdesc[q];
             const int4 fd = fdesc[q];
-            // the group's first operands do not depend on other waves
-            WaveRec r0, r1;
-            load_wave_rec(F, gd.x, li, p, r0);
+            const int n = active ? gd.y - gd.x : 0;
+            int nmax = n;
+            nmax = max(nmax, __shfl_xor(nmax, 16, 64));
+            nmax = max(nmax, __shfl_xor(nmax, 32, 64));
+            // the step's records do not depend on other waves: copy them into
+            // LDS while the wave waits
+            const int staged = min(nmax, kDmaSlots);
+            for (int t = 0; t < staged; t++) dma_step(F, s_dma, t, gd.x + min(t, max(n - 1, 0)), li, p);
             // wait until both bodies of every row's group have received every
             // earlier update (lanes 0 and 1 of a row poll A's and B's counter)
             {
@@ -1054,26 +1100,22 @@ __global__ __launch_bounds__(64) void k_pgs_flow(FlowArgs F) {
             }
             const int e = side_b ? gd.w : gd.z;
             const int rank = side_b ? fd.z : fd.x;
-            double dv = 0.0, minv = 0.0;
+            double dv = 0.0, f = 1.0;
             bool dyn = false;
             if (active && mine) {
                 dv = ld_coherent(&(rank == 0 ? prv : cur)[e * 6 + comp]);
-                minv = P.minv[e];
+                if (vlane) f = P.minv[e];
                 dyn = (P.bflags[e] & AM_BODY_PINNED) == 0;
             }
             const bool last = p == lastp;
-            const int n = active ? gd.y - gd.x : 0;
-            int nmax = n;
-            nmax = max(nmax, __shfl_xor(nmax, 16, 64));
-            nmax = max(nmax, __shfl_xor(nmax, 32, 64));
-            // records ping-pong between r0 and r1: the next contact's loads are
-            // in flight while this one is solved; rows with fewer contacts idle
-            for (int t = 0; t < nmax; t += 2) {
-                if (t + 1 < n) load_wave_rec(F, gd.x + t + 1, li, p, r1);
-                if (t < n) row_contact(F, r0, gd.x + t, p, last, lead, vlane, minv, dv, mx);
-                if (t + 1 >= nmax) break;
-                if (t + 2 < n) load_wave_rec(F, gd.x + t + 2, li, p, r0);
-                if (t + 1 < n) row_contact(F, r1, gd.x + t + 1, p, last, lead, vlane, minv, dv, mx);
+            __builtin_amdgcn_s_waitcnt(0);  // LDS copies landed
+            // rows with fewer contacts idle; contacts past the LDS slots
+            // (groups of more than kDmaSlots contacts) read global memory
+            for (int t = 0; t < nmax; t++) {
+                WaveRec rc;
+                if (t < kDmaSlots) lds_wave_rec(s_dma, t, lane, row, rc);
+                else load_wave_rec(F, gd.x + min(t, max(n - 1, 0)), li, p, rc);
+                if (t < n) row_contact(F, rc, gd.x + t, p, last, lead, f, dv, mx, nan);
             }
             // publish deltaV now, the counters once the stores are done
             if (active && mine && dyn) st_coherent(&cur[e * 6 + comp], dv);
@@ -1090,6 +1132,7 @@ __global__ __launch_bounds__(64) void k_pgs_flow(FlowArgs F) {
         }
         if (pend) release();
         if (p >= 1) {
+            if (__any(nan)) mx = __longlong_as_double(0x7ff8000000000000ll);
             mx = wave_max(mx);
             if (lane == 0) {
                 if (!(mx <= 0))  // NaN included

@@ -24,7 +24,7 @@ struct SetupArgs {
     int fric_ovr;
     int post;
     int in_collection;
-    const double* lam_alt;  // unpack: odd-sweep lambda of the dataflow schedule (null: coloured)
+    const double* lam_alt;  // unpack: odd-sweep lambda of the dataflow schedule, stride 4 (null: coloured)
     const int* iters_used;  // unpack: sweeps used (with lam_alt)
 };
 
@@ -58,7 +58,7 @@ struct FlowArgs {
     const int* flow;       // int4 per sorted group: rank_a, deg_a, rank_b, deg_b
     unsigned* cnt;         // per entity: group updates applied (zeroed before launch)
     double* dv_alt;        // deltaV of odd passes (a copy of dv at launch)
-    double* lam_alt;       // lambda of odd sweeps, 3 per sweep position
+    double* lam_alt;       // lambda of odd sweeps, 4 per sweep position (3 used)
     unsigned* pass_done;   // per pass: waves that have finished it (zeroed before launch)
     const double* lrec;    // per sweep position: 12 lanes x 3 rows x {J entry, update coefficient}
     const double* urec;    // per sweep position: b, D, D+C, refined 1/(D+C), mu, clamp (16 doubles)
