@@ -368,6 +368,7 @@ int refresh_jinv(am_ctx* c) {
 int build_layout(am_ctx* c) {
     if (c->layout_valid) return AM_OK;
     const int ng = c->n_groups;
+    if (ng >= (1 << 28)) return fail(c, AM_ERR_INVALID, "more than 2^28 contact groups");
     hipStream_t s = c->stream;
     int* sc = c->d_scalars.p;
     fill_int(s, c->inc_count.p, c->n_ent + 1, 0);
@@ -390,7 +391,7 @@ int build_layout(am_ctx* c) {
             if (r) return r;
             if (c->h_scalars[5] == 0) break;
         }
-        launch_colour_keys(s, ng, cin, c->g_keys.p);
+        launch_colour_keys(s, ng, cin, c->g_start.p, c->g_keys.p);
         sort_u64(s, c->g_keys.p, c->g_keys_sorted.p, ng, c->d_sort_temp.p, c->d_sort_temp.n, 0, 64);
     }
     AM_HIP(c, c->s_colour_start.ensure(kMaxColours + 2));

@@ -157,10 +157,17 @@ __global__ void k_colour_round(int n_groups, const int* ga, const int* gb, const
     col_out[g] = pick;
 }
 
-__global__ void k_colour_keys(int n_groups, const int* colour, uint64_t* keys) {
+// Sort key of a group: colour, then (inside a parallel colour, where groups
+// commute) larger groups first, then group index.  Equal-sized groups end up
+// side by side, so a step of consecutive groups wastes few contact slots.
+constexpr uint32_t kKeyGroupMask = 0x0FFFFFFFu;
+__global__ void k_colour_keys(int n_groups, const int* colour, const int* group_start, uint64_t* keys) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n_groups) return;
-    keys[g] = ((uint64_t)(uint32_t)colour[g] << 32) | (uint32_t)g;
+    const int c = colour[g];
+    const uint32_t size = (uint32_t)min(group_start[g + 1] - group_start[g], 15);
+    const uint32_t rank = c < kMaxColours ? 15u - size : 0u;
+    keys[g] = ((uint64_t)(uint32_t)c << 32) | (rank << 28) | (uint32_t)g;
 }
 
 // colour_start[c] = first sorted position with colour >= c, c = 0..kMaxColours+1
@@ -180,7 +187,7 @@ __global__ void k_sorted_sizes(int n_groups, const uint64_t* sorted, const int* 
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q == n_groups) sizes[q] = 0;
     if (q >= n_groups) return;
-    int g = (int)(uint32_t)sorted[q];
+    int g = (int)((uint32_t)sorted[q] & kKeyGroupMask);
     sizes[q] = group_start[g + 1] - group_start[g];
 }
 
@@ -189,7 +196,7 @@ __global__ void k_solve_order(int n_groups, const uint64_t* sorted, const int* g
                               int* spos) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n_groups) return;
-    int g = (int)(uint32_t)sorted[q];
+    int g = (int)((uint32_t)sorted[q] & kKeyGroupMask);
     spos[g] = q;
     int s = group_start[g], e = group_start[g + 1], o = pos_start[q];
     for (int k = s; k < e; k++) {
@@ -794,12 +801,14 @@ AMD_INLINE double div_recip(double den) {
 }
 AMD_INLINE double div_row(double num, double den, double y) {
     const double a = fabs(num);
-    if (y != 0.0 && a >= 0x1p-250 && a <= 0x1p250) {
-        const double q0 = num * y;
-        const double r = __builtin_fma(-den, q0, num);
-        return __builtin_fma(r, y, q0);
+    const bool fast = y != 0.0 && a >= 0x1p-250 && a <= 0x1p250;
+    const double q0 = num * y;
+    const double r = __builtin_fma(-den, q0, num);
+    double q = __builtin_fma(r, y, q0);
+    if (__builtin_expect(!__all(fast), 0)) {
+        if (!fast) q = num / den;
     }
-    return num / den;
+    return q;
 }
 
 // Per sweep position, for the dataflow kernel:
@@ -947,22 +956,23 @@ AMD_INLINE void row_contact(const FlowArgs& F, const WaveRec& c, int k, int p, b
         const double prev = lam[r];
         const double num = c.u[3 + r] * lam[r] - P.omega * (c.u[r] + jd);
         double l = div_row(num, c.u[6 + r], c.u[9 + r]);
-        if (c.u[13] != 0.0) {
-            if (r == 0) {
-                l = jmax(0, l);
-            } else {
-                const double limit = c.u[12] * lam[0];
-                l = jmax(l, -limit);
-                l = jmin(l, limit);
-            }
+        // projection (PGS.java:111-113, 150-158), selected without a branch
+        double lc;
+        if (r == 0) {
+            lc = jmax(0, l);
+        } else {
+            const double limit = c.u[12] * lam[0];
+            lc = jmin(jmax(l, -limit), limit);
         }
+        l = c.u[13] != 0.0 ? lc : l;
         lam[r] = l;
         const double diff = l - prev;
         dv = (f * diff) * c.jc[2 * r + 1] + dv;
         mx = fmax(mx, fabs(diff));
         nan = nan || diff != diff;
     }
-    if (lead) {
+    {
+        // every lane of the row stores the same values to the same place
         double* dst = (p & 1) ? &F.lam_alt[4 * k] : &P.rec[(size_t)k * kRecDoubles + kRecLambda];
         dst[0] = lam[0];
         dst[1] = lam[1];
@@ -1506,9 +1516,10 @@ void launch_colour_round(hipStream_t s, int n_groups, const int* ga, const int* 
     hipLaunchKernelGGL(k_colour_round, dim3((n_groups + 255) / 256), dim3(256), 0, s, n_groups, ga, gb, bflags,
                        inc_start, inc_count, inc_items, col_in, col_out, remaining);
 }
-void launch_colour_keys(hipStream_t s, int n_groups, const int* colour, uint64_t* keys) {
+void launch_colour_keys(hipStream_t s, int n_groups, const int* colour, const int* group_start, uint64_t* keys) {
     if (n_groups > 0)
-        hipLaunchKernelGGL(k_colour_keys, dim3((n_groups + 255) / 256), dim3(256), 0, s, n_groups, colour, keys);
+        hipLaunchKernelGGL(k_colour_keys, dim3((n_groups + 255) / 256), dim3(256), 0, s, n_groups, colour,
+                           group_start, keys);
 }
 void launch_colour_layout(hipStream_t s, int n_groups, const uint64_t* sorted, const int* group_start,
                           const int* group_a, const int* group_b, int* colour_start, int* sizes, int* pos_start,

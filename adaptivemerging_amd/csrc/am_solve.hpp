@@ -75,7 +75,7 @@ void launch_incidence(hipStream_t s, bool fill, int n_groups, const int* ga, con
 void launch_colour_round(hipStream_t s, int n_groups, const int* ga, const int* gb, const int* bflags,
                          const int* inc_start, const int* inc_count, const int* inc_items, const int* col_in,
                          int* col_out, int* remaining);
-void launch_colour_keys(hipStream_t s, int n_groups, const int* colour, uint64_t* keys);
+void launch_colour_keys(hipStream_t s, int n_groups, const int* colour, const int* group_start, uint64_t* keys);
 void launch_colour_layout(hipStream_t s, int n_groups, const uint64_t* sorted, const int* group_start,
                           const int* group_a, const int* group_b, int* colour_start, int* sizes, int* pos_start,
                           int* order, int* group_desc, int* pos_a, int* spos, int* d_total, void* scan_temp);
