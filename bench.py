@@ -43,7 +43,10 @@ def algorithmic_bytes_per_sweep(n_contacts: int, n_dyn: int) -> int:
     return 400 * n_contacts + 176 * n_dyn
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_pgs_pile20k_traffic.md")
+TRAFFIC_PROFILES = {
+    "dataflow": os.path.join(ROOT, "profiles", "k_pgs_flow_pile20k_traffic.md"),
+    "coloured": os.path.join(ROOT, "profiles", "k_pgs_coloured_pile20k_traffic.md"),
+}
 
 
 def measured_traffic(args) -> dict | None:
@@ -52,7 +55,7 @@ def measured_traffic(args) -> dict | None:
     of known size); None when the workload arguments differ from the
     profiled ones."""
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(TRAFFIC_PROFILES[args.pgs_schedule]) as f:
             line = next(ln for ln in f if ln.startswith('{"workload"'))
     except (OSError, StopIteration):
         return None
@@ -289,7 +292,7 @@ def main() -> int:
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
         "traffic": traffic["traffic_bytes_per_launch"] if traffic else None,
-        "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
+        "traffic_source": os.path.relpath(TRAFFIC_PROFILES[args.pgs_schedule], ROOT) if traffic else None,
         "bytes_per_launch": res_p["_pgs_bytes"] / max(res_p["_pgs_launches"], 1),
         "ms_per_launch": res_p["_pgs_ms"] / max(res_p["_pgs_launches"], 1),
     }
