@@ -1068,15 +1068,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         double* cur = (p & 1) ? F.dv_alt : P.dv;
         double* prv = (p & 1) ? P.dv : F.dv_alt;
         int c = 0;  // colour of the current step (steps ascend)
-        for (int st = w; st < n_steps; st += W) {
+        // this row's group of a step (the colour cursor advances with st)
+        auto step_group = [&](int st, bool& act) {
             while (s_sp[c + 1] <= st) c++;
             const int per = c < kMaxColours ? kRowsPerWave : 1;
             const int q0 = cs[c] + (st - s_sp[c]) * per;
             const int count = min(per, cs[c + 1] - q0);
-            const bool active = row < count;
-            const int q = q0 + (active ? row : 0);
-            const int4 gd = gdesc[q];
-            const int4 fd = fdesc[q];
+            act = row < count;
+            return q0 + (act ? row : 0);
+        };
+        bool nx_active = false;
+        int4 nx_gd = make_int4(0, 0, 0, 0), nx_fd = make_int4(0, 0, 0, 0);
+        if (w < n_steps) {
+            const int q = step_group(w, nx_active);
+            nx_gd = gdesc[q];
+            nx_fd = fdesc[q];
+        }
+        for (int st = w; st < n_steps; st += W) {
+            // descriptors of this step were loaded during the previous one
+            const bool active = nx_active;
+            const int4 gd = nx_gd;
+            const int4 fd = nx_fd;
+            if (st + W < n_steps) {
+                const int q = step_group(st + W, nx_active);
+                nx_gd = gdesc[q];
+                nx_fd = fdesc[q];
+            }
             const int n = active ? gd.y - gd.x : 0;
             int nmax = n;
             nmax = max(nmax, __shfl_xor(nmax, 16, 64));
