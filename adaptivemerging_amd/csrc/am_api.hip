@@ -791,10 +791,6 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
     if (flow) {
         const size_t n_dv = (size_t)c->n_ent * 6;
         AM_HIP(c, c->s_lam_alt.ensure((size_t)c->n_contacts * 4));
-        AM_HIP(c, c->s_lrec.ensure((size_t)c->n_contacts * 72));
-        AM_HIP(c, c->s_urec.ensure((size_t)c->n_contacts * 16));
-        launch_lane_records(s, c->s_rec.p, c->d_scalars.p + 1, c->n_contacts, S.compliance, c->s_lrec.p,
-                            c->s_urec.p);
         AM_HIP(c, c->dv_alt.ensure(std::max(n_dv, (size_t)6)));
         AM_HIP(c, c->f_cnt.ensure(std::max(c->n_ent, 1)));
         AM_HIP(c, c->d_pass_done.ensure(iterations + 2));
@@ -816,8 +812,6 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
         F.dv_alt = c->dv_alt.p;
         F.lam_alt = c->s_lam_alt.p;
         F.pass_done = c->d_pass_done.p;
-        F.lrec = c->s_lrec.p;
-        F.urec = c->s_urec.p;
         F.n_waves = (unsigned)fgrid;
         S.lam_alt = c->s_lam_alt.p;
         S.iters_used = G.iters_used;

@@ -142,7 +142,7 @@ struct am_ctx {
     // dataflow schedule: sorted position per group, ranks, counters, odd-pass buffers
     DBuf<int> g_spos, s_flow;
     DBuf<unsigned> f_cnt, d_pass_done;
-    DBuf<double> dv_alt, s_lam_alt, s_lrec, s_urec;
+    DBuf<double> dv_alt, s_lam_alt;
     int flow_max_grid = 0;
 
     // scalars on device: 0 n_pairs, 1 n_contacts, 2 n_groups, 3 total scratch, 4 overflow, 5 remaining,

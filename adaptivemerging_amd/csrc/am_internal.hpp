@@ -121,8 +121,10 @@ constexpr int kMaxColours = 64;     // colours 0..63 run in parallel; 64 is the 
 // then jinv_a ja.w and jinv_b jb.w (constant during a solve, so computed once
 // in setup with the same operations the update would do).  Then b[3], D[3],
 // mu, lambda[3] (read and written in place), the solve bodies a | b << 32 as
-// raw bits, the clamp flag, one pad: 58 doubles (464 B, 16-byte aligned).
-constexpr int kRecDoubles = 58;
+// raw bits, the clamp flag, then D + C [3] and the refined reciprocals of
+// D + C [3] (the dataflow sweep's division), one pad: 64 doubles (512 B,
+// 16-byte aligned).
+constexpr int kRecDoubles = 64;
 constexpr int kRecRow = 15;
 constexpr int kRecB = 45;
 constexpr int kRecD = 48;
@@ -130,6 +132,8 @@ constexpr int kRecMu = 51;
 constexpr int kRecLambda = 52;
 constexpr int kRecAB = 55;
 constexpr int kRecClamp = 56;
+constexpr int kRecDen = 57;
+constexpr int kRecY = 60;
 
 // ---- launchers (am_detect.hip) ----
 struct PairArgs {

@@ -234,6 +234,37 @@ void launch_flow_ranks(hipStream_t s, int n_ent, int n_groups, const int* bflags
                            inc_count, inc_items, spos, group_desc, flow);
 }
 
+// The division (D lambda - omega (b + J.dv)) / (D + C) of a row.  The
+// denominator is fixed during a solve, so the denominator-only half of the
+// hardware division sequence (reciprocal estimate and two Newton steps) is
+// done once per row in setup; per sweep the quotient is then q0 = n y,
+// r = fma(-d, q0, n), q = fma(r, y, q0), the same operations the sequence
+// performs when v_div_scale leaves its operands unscaled and v_div_fixup has
+// nothing to fix.  Both |n| and |d| in [2^-250, 2^250] guarantees that (no
+// exponent gap near 768, no denormal quotient or reciprocal, nonzero finite
+// operands); anything else takes the full division.  The result is the
+// correctly rounded quotient either way.
+AMD_INLINE double div_recip(double den) {
+    const double a = fabs(den);
+    if (!(a >= 0x1p-250 && a <= 0x1p250)) return 0.0;  // 0: always divide
+    double y = __builtin_amdgcn_rcp(den);
+    double e = __builtin_fma(-den, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-den, y, 1.0);
+    return __builtin_fma(y, e, y);
+}
+AMD_INLINE double div_row(double num, double den, double y) {
+    const double a = fabs(num);
+    const bool fast = y != 0.0 && a >= 0x1p-250 && a <= 0x1p250;
+    const double q0 = num * y;
+    const double r = __builtin_fma(-den, q0, num);
+    double q = __builtin_fma(r, y, q0);
+    if (__builtin_expect(!__all(fast), 0)) {
+        if (!fast) q = num / den;
+    }
+    return q;
+}
+
 // ---------------------------------------------------------------------------
 // Per-contact solve records in sweep order (Contact.computeB,
 // Contact.computeJMinvJt, the friction rule of PGS.java:128-142).
@@ -365,6 +396,12 @@ __global__ void k_setup(SetupArgs S) {
     bool clamp = !(B.flags[b1] & AM_BODY_MAGNET_ACTIVE) && !(B.flags[b2] & AM_BODY_MAGNET_ACTIVE);
     rec[kRecAB] = __longlong_as_double((long long)(((uint64_t)(uint32_t)eb << 32) | (uint32_t)ea));
     rec[kRecClamp] = clamp ? 1.0 : 0.0;
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const double den = rec[kRecD + r] + S.compliance;
+        rec[kRecDen + r] = den;
+        rec[kRecY + r] = div_recip(den);
+    }
     rec[kRecDoubles - 1] = 0.0;
     reinterpret_cast<int4*>(S.meta)[k] = make_int4(ea, eb, clamp ? 1 : 0, 0);
     S.sstate[k] = C.state[c];
@@ -468,12 +505,13 @@ AMD_INLINE double jdv(const Row& R, const BodyRegs& A, const BodyRegs& Bb) {
 }
 
 constexpr int kRec2 = kRecDoubles / 2;  // 16-byte words per record
+constexpr int kRecLoad = 58;            // the coloured sweep reads up to the clamp flag
 struct Rec {
-    double v[kRecDoubles];
+    double v[kRecLoad];
 };
 AMD_INLINE void load_rec(const double2* src, Rec& r) {
 #pragma unroll
-    for (int i = 0; i < kRec2; i++) {
+    for (int i = 0; i < kRecLoad / 2; i++) {
         double2 t = src[i];
         r.v[2 * i] = t.x;
         r.v[2 * i + 1] = t.y;
@@ -780,161 +818,64 @@ AMD_INLINE double row_jdv(double p) {
     return h + dpp_d<kRowMirror>(h);
 }
 
-// The division (D lambda - omega (b + J.dv)) / (D + C) of a row.  The
-// denominator is fixed during a solve, so the denominator-only half of the
-// hardware division sequence (reciprocal estimate and two Newton steps) is
-// done once per row in setup; per sweep the quotient is then q0 = n y,
-// r = fma(-d, q0, n), q = fma(r, y, q0), the same operations the sequence
-// performs when v_div_scale leaves its operands unscaled and v_div_fixup has
-// nothing to fix.  Both |n| and |d| in [2^-250, 2^250] guarantees that (no
-// exponent gap near 768, no denormal quotient or reciprocal, nonzero finite
-// operands); anything else takes the full division.  The result is the
-// correctly rounded quotient either way.
-AMD_INLINE double div_recip(double den) {
-    const double a = fabs(den);
-    if (!(a >= 0x1p-250 && a <= 0x1p250)) return 0.0;  // 0: always divide
-    double y = __builtin_amdgcn_rcp(den);
-    double e = __builtin_fma(-den, y, 1.0);
-    y = __builtin_fma(y, e, y);
-    e = __builtin_fma(-den, y, 1.0);
-    return __builtin_fma(y, e, y);
-}
-AMD_INLINE double div_row(double num, double den, double y) {
-    const double a = fabs(num);
-    const bool fast = y != 0.0 && a >= 0x1p-250 && a <= 0x1p250;
-    const double q0 = num * y;
-    const double r = __builtin_fma(-den, q0, num);
-    double q = __builtin_fma(r, y, q0);
-    if (__builtin_expect(!__all(fast), 0)) {
-        if (!fast) q = num / den;
-    }
-    return q;
-}
-
-// Per sweep position, for the dataflow kernel:
-// lrec: 12 lanes x 3 rows x {J entry, update coefficient} (v lanes: the J
-//       entry, scaled by minv*dl at update time; w lanes: jinv J.w);
-// urec: 16 doubles, the contact's uniform operands: b[3], D[3], D+C [3],
-//       the refined reciprocals of D+C [3], mu, clamp, 2 pad.
-constexpr int kURec = 16;
-__global__ void k_lane_records(const double* rec, const int* n_contacts, double compliance, double* lrec,
-                               double* urec) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = t / 12, i = t % 12;
-    if (k >= *n_contacts) return;
-    const double* r58 = rec + (size_t)k * kRecDoubles;
-    const int c = i % 6;
-    const bool side_b = i >= 6;
-    double* out = lrec + ((size_t)k * 12 + i) * 6;
-#pragma unroll
-    for (int r = 0; r < 3; r++) {
-        const double* q = r58 + r * kRecRow;  // aw, bv, bw, ta, tb
-        double j, u;
-        if (c < 3) {
-            j = side_b ? q[3 + c] : -q[3 + c];
-            u = j;
-        } else {
-            j = side_b ? q[6 + c - 3] : q[c - 3];
-            u = side_b ? q[12 + c - 3] : q[9 + c - 3];
-        }
-        out[2 * r] = j;
-        out[2 * r + 1] = u;
-    }
-    if (i == 0) {
-        double* u = urec + (size_t)k * kURec;
-#pragma unroll
-        for (int r = 0; r < 3; r++) {
-            const double d = r58[kRecD + r];
-            const double den = d + compliance;
-            u[r] = r58[kRecB + r];
-            u[3 + r] = d;
-            u[6 + r] = den;
-            u[9 + r] = div_recip(den);
-        }
-        u[12] = r58[kRecMu];
-        u[13] = r58[kRecClamp];
-        u[14] = 0.0;
-        u[15] = 0.0;
-    }
-}
-void launch_lane_records(hipStream_t s, const double* rec, const int* n_contacts, int cap, double compliance,
-                         double* lrec, double* urec) {
-    if (cap > 0)
-        hipLaunchKernelGGL(k_lane_records, dim3((cap * 12 + 255) / 256), dim3(256), 0, s, rec, n_contacts,
-                           compliance, lrec, urec);
-}
-
 constexpr long kFlowSpinLimit = 1L << 22;  // polls of one wait (each a memory round trip)
 
-// Per-contact operands of one row: this lane's J entries and coefficients of
-// the three rows, and the contact's uniform block and lambda.
+// Per-contact operands of one row: this lane's J entries and update
+// coefficients of the three rows, and the contact's uniform values.
 struct WaveRec {
     double jc[6];
-    double u[kURec - 2];
-    double lam[3];
+    double b[3], d[3], den[3], y[3], mu, clamp, lam[3];
 };
+// Where a lane's entries sit in a record: quads hold A.v, A.w, B.v, B.w;
+// J entries ja.v = -jb.v, ja.w, jb.v, jb.w; coefficients -jb.v, jinv_a ja.w,
+// jb.v, jinv_b jb.w (the v lanes' coefficient is their J entry, scaled by
+// minv*dl at update time).
+struct LaneMap {
+    int jofs, cofs;
+    unsigned sign;  // sign bit flipped on A.v lanes
+};
+AMD_INLINE LaneMap lane_map(int li) {
+    const int q = li >> 2, j = (li & 3) < 3 ? (li & 3) : 0;
+    LaneMap m;
+    m.jofs = (q == 1 ? 0 : q == 3 ? 6 : 3) + j;
+    m.cofs = q == 1 ? 9 + j : q == 3 ? 12 + j : m.jofs;
+    m.sign = q == 0 ? 0x80000000u : 0u;
+    return m;
+}
+AMD_INLINE double flip(double v, unsigned sign) {
+    const long long b = __double_as_longlong(v) ^ ((long long)sign << 32);
+    return __longlong_as_double(b);
+}
 AMD_INLINE const double* lam_src(const FlowArgs& F, int k, int p) {
     const bool from_alt = p >= 1 && ((p - 1) & 1);
     return from_alt ? &F.lam_alt[4 * k] : &F.P.rec[(size_t)k * kRecDoubles + kRecLambda];
 }
-AMD_INLINE void load_wave_rec(const FlowArgs& F, int k, int li, int p, WaveRec& w) {
-    const int q = li >> 2, j = li & 3;
-    const int i = q * 3 + (j < 3 ? j : 0);  // index in the 12-lane record
-    const double2* lr = reinterpret_cast<const double2*>(F.lrec + ((size_t)k * 12 + i) * 6);
+// rec: the row's record (LDS or global); lam: its lambda for this pass
+AMD_INLINE void read_wave_rec(const double* rec, const double* lam, const LaneMap& m, WaveRec& w) {
 #pragma unroll
-    for (int t = 0; t < 3; t++) {
-        const double2 v = lr[t];
-        w.jc[2 * t] = v.x;
-        w.jc[2 * t + 1] = v.y;
+    for (int r = 0; r < 3; r++) {
+        w.jc[2 * r] = flip(rec[r * kRecRow + m.jofs], m.sign);
+        w.jc[2 * r + 1] = flip(rec[r * kRecRow + m.cofs], m.sign);
+        w.b[r] = rec[kRecB + r];
+        w.d[r] = rec[kRecD + r];
+        w.den[r] = rec[kRecDen + r];
+        w.y[r] = rec[kRecY + r];
+        w.lam[r] = lam[r];
     }
-    const double2* ur = reinterpret_cast<const double2*>(F.urec + (size_t)k * kURec);
-#pragma unroll
-    for (int t = 0; t < kURec / 2 - 1; t++) {
-        const double2 v = ur[t];
-        w.u[2 * t] = v.x;
-        w.u[2 * t + 1] = v.y;
-    }
-    const double* lp = lam_src(F, k, p);
-#pragma unroll
-    for (int r = 0; r < 3; r++) w.lam[r] = lp[r];
+    w.mu = rec[kRecMu];
+    w.clamp = rec[kRecClamp];
 }
 
-// A step's records are copied into LDS with DMA loads (16 B per lane per
-// block, lane l's bytes at 16 l) before the wave waits for its dependencies:
-// per contact slot, blocks 0-2 hold each lane's J/coefficient pairs and block
-// 3 holds per row the uniform block (lanes 0-7) and lambda (lanes 8-9).
+// A step's records are copied into LDS with DMA loads (16 B per lane, lane
+// l's bytes at 16 l) before the wave waits for its dependencies: per contact
+// slot, two blocks of two rows' 512-byte records, and a block with each row's
+// lambda when the pass reads the odd-sweep buffer.
 constexpr int kDmaSlots = 8;
 typedef __attribute__((address_space(3))) void lds_void;
-AMD_INLINE void dma_step(const FlowArgs& F, double2 (*arena)[4][64], int slot, int k, int li, int p) {
-    const int q = li >> 2, j = li & 3;
-    const int i = q * 3 + (j < 3 ? j : 0);
-    const double* lr = F.lrec + ((size_t)k * 12 + i) * 6;
-#pragma unroll
-    for (int b = 0; b < 3; b++)
-        __builtin_amdgcn_global_load_lds((const void*)(lr + 2 * b), (lds_void*)&arena[slot][b][0], 16, 0, 0);
-    const double* u = F.urec + (size_t)k * kURec;
-    const double* src = li < 8 ? u + 2 * li : (li < 10 ? lam_src(F, k, p) + 2 * (li - 8) : u);
-    __builtin_amdgcn_global_load_lds((const void*)src, (lds_void*)&arena[slot][3][0], 16, 0, 0);
-}
-AMD_INLINE void lds_wave_rec(double2 (*arena)[4][64], int slot, int lane, int row, WaveRec& w) {
-#pragma unroll
-    for (int b = 0; b < 3; b++) {
-        const double2 v = arena[slot][b][lane];
-        w.jc[2 * b] = v.x;
-        w.jc[2 * b + 1] = v.y;
-    }
-    const double2* ub = &arena[slot][3][row * 16];
-#pragma unroll
-    for (int t = 0; t < kURec / 2 - 1; t++) {
-        const double2 v = ub[t];
-        w.u[2 * t] = v.x;
-        w.u[2 * t + 1] = v.y;
-    }
-    const double2 l01 = ub[8], l2 = ub[9];
-    w.lam[0] = l01.x;
-    w.lam[1] = l01.y;
-    w.lam[2] = l2.x;
-}
+struct DmaSlot {
+    double2 rec[2][64];
+    double2 lam[64];
+};
 
 // One contact of a row-held group in pass p (PGS.java:100-184; pass 0 is
 // confidentWarmStart, PGS.java:250-256).  dv is this lane's deltaV component,
@@ -954,17 +895,17 @@ AMD_INLINE void row_contact(const FlowArgs& F, const WaveRec& c, int k, int p, b
     for (int r = 0; r < 3; r++) {
         const double jd = row_jdv(c.jc[2 * r] * dv);
         const double prev = lam[r];
-        const double num = c.u[3 + r] * lam[r] - P.omega * (c.u[r] + jd);
-        double l = div_row(num, c.u[6 + r], c.u[9 + r]);
+        const double num = c.d[r] * lam[r] - P.omega * (c.b[r] + jd);
+        double l = div_row(num, c.den[r], c.y[r]);
         // projection (PGS.java:111-113, 150-158), selected without a branch
         double lc;
         if (r == 0) {
             lc = jmax(0, l);
         } else {
-            const double limit = c.u[12] * lam[0];
+            const double limit = c.mu * lam[0];
             lc = jmin(jmax(l, -limit), limit);
         }
-        l = c.u[13] != 0.0 ? lc : l;
+        l = c.clamp != 0.0 ? lc : l;
         lam[r] = l;
         const double diff = l - prev;
         dv = (f * diff) * c.jc[2 * r + 1] + dv;
@@ -980,8 +921,8 @@ AMD_INLINE void row_contact(const FlowArgs& F, const WaveRec& c, int k, int p, b
     }
     if (last) {
         // Contact.updateContactState (Contact.java:385-398)
-        const double w1 = c.u[1] + row_jdv(c.jc[2] * dv);
-        const double w2 = c.u[2] + row_jdv(c.jc[4] * dv);
+        const double w1 = c.b[1] + row_jdv(c.jc[2] * dv);
+        const double w2 = c.b[2] + row_jdv(c.jc[4] * dv);
         if (lead) {
             int st;
             if (fabs(lam[0]) <= 1e-14) st = AM_STATE_BROKEN;
@@ -1027,7 +968,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     const int lastp = P.iterations;
     const int4* gdesc = reinterpret_cast<const int4*>(P.group_desc);
     const int4* fdesc = reinterpret_cast<const int4*>(F.flow);
-    __shared__ double2 s_dma[kDmaSlots][4][64];
+    __shared__ DmaSlot s_dma[kDmaSlots];
+    const LaneMap lmap = lane_map(li);
     int chk = 0;  // sweeps 1..chk are known not to have converged
     // counters of the last step, released once its deltaV stores are done
     // (the wait overlaps the next step's first poll); lanes 0 and 1 of a row
@@ -1099,9 +1041,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             nmax = max(nmax, __shfl_xor(nmax, 16, 64));
             nmax = max(nmax, __shfl_xor(nmax, 32, 64));
             // the step's records do not depend on other waves: copy them into
-            // LDS while the wave waits
+            // LDS while the wave waits.  Lane l of a record block copies word
+            // l & 31 of row 2i + (l >> 5)'s record.
             const int staged = min(nmax, kDmaSlots);
-            for (int t = 0; t < staged; t++) dma_step(F, s_dma, t, gd.x + min(t, max(n - 1, 0)), li, p);
+            const bool odd_lam = p >= 1 && ((p - 1) & 1);
+            {
+                int kb[kRowsPerWave], nl[kRowsPerWave];
+#pragma unroll
+                for (int r = 0; r < kRowsPerWave; r++) {
+                    kb[r] = __builtin_amdgcn_readlane(gd.x, r * 16);
+                    nl[r] = max(__builtin_amdgcn_readlane(n, r * 16) - 1, 0);
+                }
+                const bool upper = lane >= 32;
+                for (int t = 0; t < staged; t++) {
+#pragma unroll
+                    for (int i = 0; i < 2; i++) {
+                        const int r0 = 2 * i, r1 = 2 * i + 1;
+                        const int k = upper ? kb[r1] + min(t, nl[r1]) : kb[r0] + min(t, nl[r0]);
+                        const double* src = P.rec + (size_t)k * kRecDoubles + 2 * (lane & 31);
+                        __builtin_amdgcn_global_load_lds((const void*)src, (lds_void*)&s_dma[t].rec[i][0], 16, 0, 0);
+                    }
+                    if (odd_lam) {
+                        const int k = gd.x + min(t, max(n - 1, 0));  // this lane's row
+                        const double* src = &F.lam_alt[4 * k + (li == 1 ? 2 : 0)];
+                        __builtin_amdgcn_global_load_lds((const void*)src, (lds_void*)&s_dma[t].lam[0], 16, 0, 0);
+                    }
+                }
+            }
             // wait until both bodies of every row's group have received every
             // earlier update (lanes 0 and 1 of a row poll A's and B's counter)
             {
@@ -1140,8 +1106,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             // (groups of more than kDmaSlots contacts) read global memory
             for (int t = 0; t < nmax; t++) {
                 WaveRec rc;
-                if (t < kDmaSlots) lds_wave_rec(s_dma, t, lane, row, rc);
-                else load_wave_rec(F, gd.x + min(t, max(n - 1, 0)), li, p, rc);
+                if (t < kDmaSlots) {
+                    const double* rl = reinterpret_cast<const double*>(&s_dma[t].rec[row >> 1][(row & 1) * 32]);
+                    const double* ll = odd_lam ? reinterpret_cast<const double*>(&s_dma[t].lam[row * 16])
+                                               : rl + kRecLambda;
+                    read_wave_rec(rl, ll, lmap, rc);
+                } else {
+                    const int k = gd.x + min(t, max(n - 1, 0));
+                    const double* rg = P.rec + (size_t)k * kRecDoubles;
+                    read_wave_rec(rg, lam_src(F, k, p), lmap, rc);
+                }
                 if (t < n) row_contact(F, rc, gd.x + t, p, last, lead, f, dv, mx, nan);
             }
             // publish deltaV now, the counters once the stores are done

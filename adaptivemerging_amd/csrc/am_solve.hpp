@@ -60,8 +60,6 @@ struct FlowArgs {
     double* dv_alt;        // deltaV of odd passes (a copy of dv at launch)
     double* lam_alt;       // lambda of odd sweeps, 4 per sweep position (3 used)
     unsigned* pass_done;   // per pass: waves that have finished it (zeroed before launch)
-    const double* lrec;    // per sweep position: 12 lanes x 3 rows x {J entry, update coefficient}
-    const double* urec;    // per sweep position: b, D, D+C, refined 1/(D+C), mu, clamp (16 doubles)
     unsigned n_waves;
 };
 
@@ -84,8 +82,6 @@ void launch_unpack(hipStream_t s, const SetupArgs& S, int cap);
 int pgs_max_blocks(int block, bool* ok);
 hipError_t launch_pgs(hipStream_t s, const PgsArgs& P, int grid, int block, bool plain);
 int pgs_flow_max_blocks(bool* ok);
-void launch_lane_records(hipStream_t s, const double* rec, const int* n_contacts, int cap, double compliance,
-                         double* lrec, double* urec);
 hipError_t launch_pgs_flow(hipStream_t s, const FlowArgs& F, int grid, bool plain, int n_dv);
 void launch_flow_ranks(hipStream_t s, int n_ent, int n_groups, const int* bflags, const int* inc_start,
                        const int* inc_count, const int* inc_items, const int* spos, const int* group_desc,
