@@ -74,3 +74,24 @@ def test_flow_equals_coloured_schedule(gpu_lib):
     for a, b in zip(out[0][0], out[1][0]):
         assert_bits_equal("state", a, b)
     assert_bits_equal("lambda", out[0][1], out[1][1])
+
+
+def test_flow_groups_beyond_the_lds_slots(gpu_lib):
+    """Sphere trees resting on planes give body pairs with hundreds of
+    contacts: the records past the kernel's LDS slots come from global memory."""
+    import collections
+
+    scene = workloads.sphere_tree_drop(seed=5, n=16, heights=(0.15, 0.9), box=(1.2, 1.0, 1.2))
+    ctx, orc = twin(scene, iterations=12, tolerance=0.0, gravity_amount=9.81, pgs_schedule=0)
+    ctx.apply_gravity(); orc.apply_gravity()
+    n = ctx.detect(0.01); orc.detect()
+    c = ctx.contacts()
+    sizes = collections.Counter(zip(np.minimum(c["body1"], c["body2"]), np.maximum(c["body1"], c["body2"])))
+    print("contacts", n, "largest group", max(sizes.values()), flush=True)
+    assert max(sizes.values()) > 8
+    it_g, _ = ctx.solve(0.01)
+    order = ctx.solve_order()
+    it_o, _, _ = orc.solve(order, 0.01)
+    assert it_g == it_o
+    assert_contacts_equal(ctx.contacts(), orc.contacts())
+    assert_bits_equal("deltaV", ctx.pull_deltav(), orc.pull_deltav())
