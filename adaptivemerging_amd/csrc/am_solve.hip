@@ -897,13 +897,23 @@ AMD_INLINE void row_contact(const FlowArgs& F, const WaveRec& c, int k, int p, b
         const double prev = lam[r];
         const double num = c.d[r] * lam[r] - P.omega * (c.b[r] + jd);
         double l = div_row(num, c.den[r], c.y[r]);
-        // projection (PGS.java:111-113, 150-158), selected without a branch
+        // projection (PGS.java:111-113, 150-158), selected without a branch.
+        // Math.max(0, l) is (l <= 0) ? +0 : l for every l (NaN and -0
+        // included).  For a limit that is neither NaN nor -0, Math.max(l,
+        // -limit) and Math.min(., limit) are the plain selects below (NaN l
+        // falls through both, and the only zero tie, -0 against +0, resolves
+        // the Java way); other limits take the full Java forms.
         double lc;
         if (r == 0) {
-            lc = jmax(0, l);
+            lc = l <= 0.0 ? 0.0 : l;
         } else {
             const double limit = c.mu * lam[0];
-            lc = jmin(jmax(l, -limit), limit);
+            const bool plain = limit > 0.0 || __double_as_longlong(limit) == 0;
+            const double a = l < -limit ? -limit : l;
+            lc = a > limit ? limit : a;
+            if (__builtin_expect(!__all(plain), 0)) {
+                if (!plain) lc = jmin(jmax(l, -limit), limit);
+            }
         }
         l = c.clamp != 0.0 ? lc : l;
         lam[r] = l;
