@@ -1,0 +1,191 @@
+"""Scene XML ingestion: the reference's scene files drive this build unchanged.
+
+Follows XMLParser (src/mergingBodies3D/XMLParser.java):
+- parse (:41-66): the <system> nodes, then <collision>, then every <body>
+  child of <root> in document order;
+- parseCollision (:91-135): the solver settings common to all bodies;
+- createBox (:360-396), createPlane (:404-413), createSphere (:420-447);
+- setCommonAttributes (:542-626): x, R (axis-angle), v, omega, restitution,
+  friction, pinned and the tags that do not touch the contact step.
+
+Mesh and composite bodies (createMesh :449-538, createComposite :177-353),
+springs and spinners have no counterpart in this build; a scene that uses
+them raises SceneFormatError rather than loading into different dynamics.
+Sphere-tree geometry for hand-built bodies comes from .sph files through
+scene.load_sph (the format of :500-535).
+"""
+from __future__ import annotations
+
+import os
+import xml.etree.ElementTree as ET
+
+import numpy as np
+
+from .scene import Scene, rot_axis_angle
+
+
+class SceneFormatError(ValueError):
+    """A scene file that this build cannot load faithfully."""
+
+
+# <collision> attribute -> (CollisionProcessor field, parser); restitution and
+# friction also switch their override on (XMLParser.java:112-122)
+_COLLISION_ATTRS = {
+    "iterations": ("iterations", int),
+    "feedbackStiffness": ("feedbackStiffness", float),
+    "restitution": ("restitution", float),
+    "friction": ("friction", float),
+    "enablePostStabilization": ("enablePostStabilization", None),
+    "enableCompliance": ("enableCompliance", None),
+}
+
+# CollisionProcessor field -> am_params field
+PARAM_NAMES = {
+    "iterations": "iterations",
+    "feedbackStiffness": "feedback_stiffness",
+    "restitution": "restitution",
+    "restitutionOverride": "restitution_override",
+    "friction": "friction",
+    "frictionOverride": "friction_override",
+    "enablePostStabilization": "enable_post_stabilization",
+    "enableCompliance": "enable_compliance",
+}
+
+
+def _java_bool(s: str) -> bool:
+    """Boolean.parseBoolean: true iff the text is "true", ignoring case."""
+    return s.strip().lower() == "true"
+
+
+def _t3d(s: str) -> list[float]:
+    v = s.split()
+    if len(v) < 3:
+        raise SceneFormatError(f"expected three numbers, got {s!r}")
+    return [float(v[0]), float(v[1]), float(v[2])]
+
+
+def _children(node, tag: str):
+    return [c for c in node if c.tag.lower() == tag]
+
+
+class _Common:
+    """The per-body settings of setCommonAttributes (XMLParser.java:542-626);
+    a later tag overrides an earlier one, as there."""
+
+    def __init__(self, node, name: str, allow_springs: bool):
+        self.x = None
+        self.R = None
+        self.v = None
+        self.w = None
+        self.friction = 0.8      # RigidBody.friction default (RigidBody.java:142)
+        self.restitution = 0.0   # RigidBody.restitution default (RigidBody.java:145)
+        self.pinned = False
+        for c in node:
+            tag = c.tag.lower()
+            text = (c.text or "").strip()
+            if tag == "x":
+                self.x = _t3d(text)
+            elif tag == "r":
+                a = [float(t) for t in text.split()]
+                if len(a) < 4:
+                    raise SceneFormatError(f"body {name}: <R> needs an axis and an angle")
+                n = np.linalg.norm(a[:3])
+                self.R = np.eye(3) if n == 0.0 else rot_axis_angle(a[:3], a[3])
+            elif tag == "v":
+                self.v = _t3d(text)
+            elif tag == "omega":
+                self.w = _t3d(text)
+            elif tag == "restitution":
+                self.restitution = float(text.split()[0])
+            elif tag == "friction":
+                self.friction = float(text.split()[0])
+            elif tag == "pinned":
+                self.pinned = _java_bool(text)
+            elif tag == "spinner":
+                raise SceneFormatError(f"body {name}: spinners are not modelled")
+            elif tag == "spring":
+                if not allow_springs:
+                    raise SceneFormatError(f"body {name}: springs are not modelled (load with ignore_springs=True "
+                                           "to drop them)")
+            # temporarilyPinned is ignored by the reference itself; magnetic
+            # only matters once a magnet is switched on from the UI; factorypart
+            # and col are for the factory feature and drawing
+        self.pos = self.x if self.x is not None else [0.0, 0.0, 0.0]
+
+
+def parse_scene(source: str, *, ignore_springs: bool = False, scene_index: int = 0):
+    """Parses a scene (a file path or the XML text).  Returns (scene,
+    collision), where collision maps CollisionProcessor field names to the
+    values the file sets."""
+    text = source
+    if not source.lstrip().startswith("<"):
+        with open(source) as f:
+            text = f.read()
+    try:
+        root = ET.fromstring(text)
+    except ET.ParseError as e:
+        raise SceneFormatError(f"not an XML scene: {e}") from e
+    if root.tag.lower() != "root":
+        raise SceneFormatError(f"expected a <root> element, got <{root.tag}>")
+
+    collision: dict = {}
+    for node in _children(root, "collision"):
+        for attr, (field, conv) in _COLLISION_ATTRS.items():
+            if attr not in node.attrib:
+                continue
+            raw = node.attrib[attr]
+            collision[field] = _java_bool(raw) if conv is None else conv(raw)
+            if attr == "restitution":
+                collision["restitutionOverride"] = True
+            elif attr == "friction":
+                collision["frictionOverride"] = True
+
+    scene = Scene()
+    for node in _children(root, "body"):
+        kind = node.attrib.get("type", "").lower()
+        name = node.attrib.get("name", "")
+        com = _Common(node, name, ignore_springs)
+        if kind == "box":
+            s = np.array(_t3d(node.attrib["dim"]))
+            density = float(node.attrib.get("density", 1.0))
+            if "scale" in node.attrib:
+                s = s * float(node.attrib["scale"])
+            scene.add_box(s, com.pos, R=com.R, v=com.v, w=com.w, density=density, friction=com.friction,
+                          restitution=com.restitution, pinned=com.pinned, scene=scene_index, name=name)
+        elif kind == "plane":
+            # PlaneRigidBody is always pinned (PlaneRigidBody.java:55-62)
+            scene.add_plane(_t3d(node.attrib["p"]), _t3d(node.attrib["n"]), friction=com.friction,
+                            restitution=com.restitution, scene=scene_index, name=name)
+        elif kind == "sphere":
+            if com.pinned:
+                raise SceneFormatError(f"body {name}: pinned spheres are not supported")
+            density = float(node.attrib.get("density", 1.0))
+            scene.add_sphere(float(node.attrib["r"]), com.pos, R=com.R, v=com.v, w=com.w, density=density,
+                             friction=com.friction, restitution=com.restitution, scene=scene_index, name=name)
+        elif kind in ("mesh", "composite"):
+            raise SceneFormatError(f"body {name}: {kind} bodies are not supported by this build")
+        else:
+            raise SceneFormatError(f"body {name}: unknown type {kind!r}")
+    return scene, collision
+
+
+def apply_collision_settings(collision: dict, params) -> None:
+    """Copies parsed <collision> settings onto an am_params structure."""
+    for field, value in collision.items():
+        setattr(params, PARAM_NAMES[field], int(value) if isinstance(value, bool) else value)
+
+
+class XMLParser:
+    """XMLParser.parse(system, filename) (XMLParser.java:41-66) for this build:
+    builds the scene, then a RigidBodySystem on it with the file's collision
+    settings."""
+
+    def parse(self, filename: str, device: int = 0, ignore_springs: bool = False):
+        from .system import RigidBodySystem
+
+        scene, collision = parse_scene(filename, ignore_springs=ignore_springs)
+        system = RigidBodySystem(scene, device)
+        for field, value in collision.items():
+            setattr(system.collision, field, value)
+        system.name = os.path.basename(filename)
+        return system

@@ -1,0 +1,107 @@
+"""Scene XML ingestion (XMLParser.java): the builders it calls must match the
+direct Scene builders bit for bit, <collision> settings and their overrides
+must map onto the solver parameters, unsupported features must be refused,
+and every box/plane/sphere scene of the reference's scenes3D folder must load
+(when that folder is present)."""
+import glob
+import math
+import os
+
+import numpy as np
+import pytest
+
+from adaptivemerging_amd import xml_scene
+from adaptivemerging_amd._lib import default_params_host
+from adaptivemerging_amd.scene import Scene, rot_axis_angle
+
+SCENE = """
+<root>
+  <system mouseSpringStiffness="10"/>
+  <collision iterations="80" feedbackStiffness="0.25" friction="0.6" enableCompliance="false"/>
+  <body type="plane" name="ground" p="0 -1 0" n="0 2 0"><friction>0.4</friction></body>
+  <body type="box" name="b0" dim="1 0.25 0.15" density="1000">
+    <x> 0.5 0.2 -0.1 </x>
+    <R> 0 1 0 0.7 </R>
+    <v> 1 0 0 </v>
+    <restitution> 0.3 </restitution>
+  </body>
+  <body type="box" name="base" dim="2 2 4" scale="0.5">
+    <x>0 -0.5 0</x>
+    <pinned> TRUE </pinned>
+    <col>0.1 0.2 0.3</col>
+  </body>
+  <body type="sphere" name="s" r="0.2" density="50">
+    <x>0 1 0</x>
+    <omega>0 0 3</omega>
+    <magnetic>true</magnetic>
+  </body>
+</root>
+"""
+
+
+def test_builders_match_direct_construction():
+    scene, coll = xml_scene.parse_scene(SCENE)
+    ref = Scene()
+    ref.add_plane([0, -1, 0], [0, 2, 0], friction=0.4, name="ground")
+    ref.add_box([1, 0.25, 0.15], [0.5, 0.2, -0.1], R=rot_axis_angle([0, 1, 0], 0.7), v=[1, 0, 0], density=1000,
+                restitution=0.3, name="b0")
+    ref.add_box(np.array([2.0, 2.0, 4.0]) * 0.5, [0, -0.5, 0], pinned=True, name="base")
+    ref.add_sphere(0.2, [0, 1, 0], w=[0, 0, 3], density=50, name="s")
+    for a, b in zip(scene.arrays(), ref.arrays()):
+        assert a.dtype == b.dtype and a.shape == b.shape
+        assert a.tobytes() == b.tobytes()
+    assert scene.names == ["ground", "b0", "base", "s"]
+    # box mass properties (createBox): m = w h d rho, I = m/12 (b^2 + c^2)
+    b = scene.arrays()[0][1]
+    assert b["mass"] == 1 * 0.25 * 0.15 * 1000
+    assert math.isclose(1.0 / b["jinv0"][0], 1.0 / 12 * b["mass"] * (0.25 ** 2 + 0.15 ** 2), rel_tol=1e-15)
+    assert scene.arrays()[0][2]["flags"] & 1   # pinned box
+    assert scene.arrays()[0][0]["plane_n"].tolist() == [0.0, 1.0, 0.0]
+
+
+def test_collision_settings_and_overrides():
+    _, coll = xml_scene.parse_scene(SCENE)
+    assert coll == {"iterations": 80, "feedbackStiffness": 0.25, "friction": 0.6, "frictionOverride": True,
+                    "enableCompliance": False}
+    p = default_params_host()
+    xml_scene.apply_collision_settings(coll, p)
+    assert (p.iterations, p.feedback_stiffness, p.friction, p.friction_override, p.enable_compliance) == \
+        (80, 0.25, 0.6, 1, 0)
+    assert p.restitution_override == 0
+
+
+@pytest.mark.parametrize("body,msg", [
+    ('<body type="mesh" name="m" obj="a.obj" st="a.sph" scale="1"/>', "mesh"),
+    ('<body type="composite" name="c"/>', "composite"),
+    ('<body type="box" name="b" dim="1 1 1"><spring pB="0 0 0" pW="0 1 0"/></body>', "springs"),
+    ('<body type="box" name="b" dim="1 1 1"><spinner>0 1 0</spinner></body>', "spinners"),
+    ('<body type="cone" name="b"/>', "unknown type"),
+])
+def test_unsupported_features_are_refused(body, msg):
+    with pytest.raises(xml_scene.SceneFormatError, match=msg):
+        xml_scene.parse_scene(f"<root>{body}</root>")
+
+
+def test_springs_can_be_dropped_explicitly():
+    scene, _ = xml_scene.parse_scene('<root><body type="box" name="b" dim="1 1 1"><spring pB="0 0 0" pW="0 1 0"/>'
+                                     '</body></root>', ignore_springs=True)
+    assert scene.n == 1
+
+
+REF_SCENES = "/root/reference/scenes3D"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_SCENES), reason="the reference's scene folder is not present")
+def test_reference_scene_files_load():
+    loaded = 0
+    for path in sorted(glob.glob(os.path.join(REF_SCENES, "*.xml"))):
+        with open(path) as f:
+            text = f.read()
+        if any(f'type="{t}"' in text for t in ("mesh", "composite")) or "<spring" in text or "<spinner" in text:
+            continue
+        scene, coll = xml_scene.parse_scene(path)
+        b, _, x, R, _, _ = scene.arrays()
+        assert scene.n > 0 and np.all(np.isfinite(x)) and np.all(np.isfinite(R))
+        assert np.allclose(np.einsum("nij,nkj->nik", R.reshape(-1, 3, 3), R.reshape(-1, 3, 3)), np.eye(3), atol=1e-12)
+        loaded += 1
+    assert loaded >= 5
