@@ -48,6 +48,10 @@ BODY_DTYPE = np.dtype(
     ]
 )
 NODE_DTYPE = np.dtype([("c", "<f8", 3), ("r", "<f8"), ("first_child", "<i4"), ("n_children", "<i4")])
+AM_SPRING_ZERO, AM_SPRING_WORLD, AM_SPRING_BODYBODY = 0, 1, 2
+SPRING_DTYPE = np.dtype([("type", "<i4"), ("body1", "<i4"), ("body2", "<i4"), ("pad", "<i4"), ("pb1", "<f8", 3),
+                         ("pb2", "<f8", 3), ("pw", "<f8", 3), ("l0", "<f8"), ("k", "<f8"), ("d", "<f8"),
+                         ("ls", "<f8")])
 assert BODY_DTYPE.itemsize == 480 and NODE_DTYPE.itemsize == 40
 
 
@@ -62,6 +66,7 @@ class Params(C.Structure):
         ("warm_start", C.c_int32), ("use_gravity", C.c_int32),
         ("gravity_amount", C.c_double), ("gravity_angle_deg", C.c_double),
         ("viscous_linear_decay", C.c_double), ("viscous_angular_decay", C.c_double),
+        ("spring_stiffness_mod", C.c_double), ("spring_damping_mod", C.c_double),
         ("enable_merging", C.c_int32), ("enable_merge_pinned", C.c_int32),
         ("enable_merge_stable_contact", C.c_int32), ("enable_merge_let_it_breathe", C.c_int32),
         ("enable_unmerging", C.c_int32), ("enable_unmerge_friction", C.c_int32),
@@ -121,6 +126,7 @@ SYMBOLS = [
     "am_version", "am_ctx_create", "am_ctx_destroy", "am_last_error", "am_params_default", "am_params_set",
     "am_scene_upload", "am_structure_pull", "am_entities_pull", "am_jinv_pull", "am_state_push",
     "am_state_pull", "am_forces_push", "am_forces_pull", "am_clear_bodies", "am_apply_gravity", "am_deltav_pull",
+    "am_springs_set", "am_apply_spring_forces",
     "am_update_contacts_map", "am_detect", "am_contacts_push", "am_contacts_pull", "am_lambdas_push",
     "am_warm_start", "am_redo_warm_start", "am_no_warm_start", "am_solve", "am_solve_order_pull",
     "am_colours_pull", "am_update_in_collections", "am_changes_pull", "am_advance_velocities",
@@ -171,6 +177,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "am_forces_pull": (C.c_int, [ctxp, vp, vp, i32]),
         "am_clear_bodies": (C.c_int, [ctxp]),
         "am_apply_gravity": (C.c_int, [ctxp]),
+        "am_springs_set": (C.c_int, [ctxp, vp, i32]),
+        "am_apply_spring_forces": (C.c_int, [ctxp]),
         "am_deltav_pull": (C.c_int, [ctxp, vp, i32]),
         "am_update_contacts_map": (C.c_int, [ctxp]),
         "am_detect": (C.c_int, [ctxp, dbl, C.POINTER(i32)]),
@@ -244,6 +252,7 @@ _DEFAULTS = dict(
     enable_post_stabilization=0, enable_compliance=1, compliance=1e-3, restitution_override=0,
     friction_override=0, restitution=0.5, friction=0.1, sliding_threshold=0.01, warm_start=1, use_gravity=1,
     gravity_amount=1.0, gravity_angle_deg=90.0, viscous_linear_decay=1.0, viscous_angular_decay=1.0,
+    spring_stiffness_mod=1.0, spring_damping_mod=1.0,
     enable_merging=1, enable_merge_pinned=1, enable_merge_stable_contact=1, enable_merge_let_it_breathe=1,
     enable_unmerging=1, enable_unmerge_friction=1, enable_unmerge_normal=1, enable_unmerge_relative_motion=1,
     update_contacts_in_collections=1, step_accum_merging=3, step_accum_unmerging=3, steps_between_merge_events=10,

@@ -53,6 +53,17 @@ class Context:
         self.n = len(b)
         self._check(self.lib.am_scene_upload(self.h, ptr(b), len(b), ptr(nd), len(nd)))
         self.push_state(x, R, v, w)
+        springs = scene.spring_array() if hasattr(scene, "spring_array") else None
+        if springs is not None and len(springs):
+            self.set_springs(springs)
+
+    def set_springs(self, springs):
+        """RigidBodySystem.springs: a SPRING_DTYPE array (physical body indices)."""
+        self._springs = np.ascontiguousarray(springs, dtype=_lib.SPRING_DTYPE)
+        self._check(self.lib.am_springs_set(self.h, ptr(self._springs), len(self._springs)))
+
+    def apply_spring_forces(self):
+        self._check(self.lib.am_apply_spring_forces(self.h))
 
     def structure(self):
         """(parent per entity, top-level list) of the merged-body structure."""

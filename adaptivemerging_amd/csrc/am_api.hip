@@ -174,6 +174,8 @@ void default_params(am_params* p) {
     p->gravity_angle_deg = 90.0;
     p->viscous_linear_decay = 1.0;
     p->viscous_angular_decay = 1.0;
+    p->spring_stiffness_mod = 1.0;
+    p->spring_damping_mod = 1.0;
     p->enable_merging = 1;
     p->enable_merge_pinned = 1;
     p->enable_merge_stable_contact = 1;
@@ -897,6 +899,7 @@ int am_scene_upload(am_ctx* c, const am_body_desc* bodies, int32_t n_bodies, con
                 return fail(c, AM_ERR_INVALID, "collections are created by merging, not uploaded");
         // merging state starts empty
         c->total_steps = 0;
+        c->n_springs = 0;
         c->merging_event = false;
         c->free_entities.clear();
         c->b2c.assign((size_t)n_bodies * 12, 0.0);
@@ -1079,6 +1082,35 @@ int am_clear_bodies(am_ctx* c) {
     return AM_OK;
 }
 
+int am_springs_set(am_ctx* c, const am_spring* springs, int32_t n) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (n < 0 || (n > 0 && !springs)) return fail(c, AM_ERR_INVALID, "bad spring buffer");
+    const int nb = c->n_phys;
+    for (int32_t i = 0; i < n; i++) {
+        const am_spring& sp = springs[i];
+        if (sp.type < AM_SPRING_ZERO || sp.type > AM_SPRING_BODYBODY)
+            return fail(c, AM_ERR_INVALID, "spring " + std::to_string(i) + ": unknown type");
+        if (sp.body1 < 0 || sp.body1 >= nb)
+            return fail(c, AM_ERR_INVALID, "spring " + std::to_string(i) + ": body1 out of range");
+        if (sp.type == AM_SPRING_BODYBODY && (sp.body2 < 0 || sp.body2 >= nb))
+            return fail(c, AM_ERR_INVALID, "spring " + std::to_string(i) + ": body2 out of range");
+    }
+    AM_HIP(c, c->d_springs.ensure(std::max(n, 1)));
+    if (n > 0)
+        AM_HIP(c, hipMemcpyAsync(c->d_springs.p, springs, (size_t)n * sizeof(am_spring), hipMemcpyHostToDevice,
+                                 c->stream));
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    c->n_springs = n;
+    return AM_OK;
+}
+
+int am_apply_spring_forces(am_ctx* c) {
+    int r = check_ctx(c);
+    if (r) return r;
+    return apply_springs(c);
+}
+
 int am_apply_gravity(am_ctx* c) {
     int r = check_ctx(c);
     if (r) return r;
@@ -1088,6 +1120,21 @@ int am_apply_gravity(am_ctx* c) {
 }  // extern "C"
 
 namespace am_rt {
+int apply_springs(am_ctx* c) {
+    launch_springs(c->stream, bodies_view(c), c->d_springs.p, c->n_springs, c->params.spring_stiffness_mod,
+                   c->params.spring_damping_mod);
+    AM_HIP(c, hipGetLastError());
+    return AM_OK;
+}
+
+// RigidBodySystem.applyExternalForces (RigidBodySystem.java:234-257): gravity,
+// then the springs
+int apply_external_forces(am_ctx* c) {
+    int r = apply_gravity(c);
+    if (r) return r;
+    return apply_springs(c);
+}
+
 int apply_gravity(am_ctx* c) {
     if (!c->params.use_gravity) return AM_OK;
     double theta = c->params.gravity_angle_deg / 180.0 * M_PI;
@@ -1322,7 +1369,7 @@ int am_clear_top_level(am_ctx* c) {
     if (r) return r;
     launch_clear_targets(c->stream, bodies_view(c), c->d_slot_entity.p, (int)c->slot_entity.size());
     AM_HIP(c, hipGetLastError());
-    return apply_gravity(c);
+    return apply_external_forces(c);
 }
 
 int am_clear_body_pair_contacts(am_ctx* c) {
@@ -1478,7 +1525,7 @@ int am_step(am_ctx* c, double dt, am_step_stats* st) {
     AM_HIP(c, hipEventRecord(ev[0], s));
     // RigidBodySystem.clearBodies (:190-202)
     if ((r = am_clear_bodies(c))) return r;
-    if ((r = apply_gravity(c))) return r;
+    if ((r = apply_external_forces(c))) return r;
     if ((r = am_update_contacts_map(c))) return r;
     if ((r = do_detect(c))) return r;
     AM_HIP(c, hipEventRecord(ev[1], s));
@@ -1492,7 +1539,7 @@ int am_step(am_ctx* c, double dt, am_step_stats* st) {
         if ((r = merge_unmerge(c, dt, &cut))) return r;
         if (c->merging_event) {
             launch_clear_targets(s, bodies_view(c), c->d_slot_entity.p, (int)c->slot_entity.size());
-            if ((r = apply_gravity(c))) return r;
+            if ((r = apply_external_forces(c))) return r;
         }
     }
     AM_HIP(c, hipEventRecord(ev[6], s));

@@ -144,6 +144,9 @@ struct am_ctx {
     DBuf<unsigned> f_cnt, d_pass_done;
     DBuf<double> dv_alt, s_lam_alt;
     int flow_max_grid = 0;
+    // springs (RigidBodySystem.springs)
+    DBuf<am_spring> d_springs;
+    int n_springs = 0;
 
     // scalars on device: 0 n_pairs, 1 n_contacts, 2 n_groups, 3 total scratch, 4 overflow, 5 remaining,
     // 6 iters_used, 7 bad, 8 repaired, 9 n_sweep
@@ -202,6 +205,8 @@ struct am_ctx {
 namespace am_rt {
 
 int fail(am_ctx* c, int code, const std::string& m);
+int apply_springs(am_ctx* c);
+int apply_external_forces(am_ctx* c);
 int check_ctx(am_ctx* c);
 am::BodiesView bodies_view(am_ctx* c);
 am::TreeView tree_view(am_ctx* c);

@@ -1409,6 +1409,55 @@ void launch_gravity(hipStream_t s, BodiesView B, const int* targets, int n, doub
     if (n > 0) hipLaunchKernelGGL(k_gravity, dim3((n + 255) / 256), dim3(256), 0, s, B, targets, n, d3{gx, gy, gz});
 }
 
+// RigidBody.applyForceW (RigidBody.java:335-340)
+AMD_INLINE void apply_force_w(BodiesView B, int e, d3 p, d3 F) {
+    st3(&B.f[e * 3], add(ld3(&B.f[e * 3]), F));
+    const d3 r = sub(p, ld3(&B.x[e * 3]));
+    st3(&B.t[e * 3], add(ld3(&B.t[e * 3]), cross(r, F)));
+}
+// RigidBodySystem.applySpringForces (RigidBodySystem.java:309-322) with
+// Spring.applyOneBody / applyTwoBodies (Spring.java:165-214).  One thread walks
+// the springs in list order, so every accumulator receives the reference's
+// sequence of additions.
+__global__ void k_springs(BodiesView B, const am_spring* S, int n, double ks, double ds) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int i = 0; i < n; i++) {
+        const am_spring sp = S[i];
+        const int b1 = sp.body1;
+        const d3 p1w = xform(ldm(&B.R[b1 * 9]), ld3(&B.x[b1 * 3]), ld3(sp.pb1));
+        if (sp.type != AM_SPRING_BODYBODY) {
+            const d3 disp = sub(ld3(sp.pw), p1w);
+            const double len = length(disp);
+            if (len < 1e-3) continue;
+            const d3 vel = spatial_velocity(B, b1, p1w);
+            const double scl = -(sp.k * ks * (len - sp.l0 * sp.ls) - sp.d * ds * (dot(vel, disp) / len)) / len;
+            const d3 F = scale(-scl, disp);
+            apply_force_w(B, b1, p1w, F);
+            if (B.parent[b1] >= 0) apply_force_w(B, B.parent[b1], p1w, F);
+        } else {
+            const int b2 = sp.body2;
+            const d3 p2w = xform(ldm(&B.R[b2 * 9]), ld3(&B.x[b2 * 3]), ld3(sp.pb2));
+            d3 disp = sub(p2w, p1w);
+            const double dist = length(disp);
+            if (dist < 1e-3) continue;
+            disp = scale(1. / dist, disp);
+            const d3 v2 = spatial_velocity(B, b2, p2w);
+            const d3 v1 = spatial_velocity(B, b1, p1w);
+            const d3 rel = sub(v2, v1);
+            const double fi = sp.k * ks * (dist - sp.l0 * sp.ls) + sp.d * ds * dot(rel, disp);
+            d3 F = scale(fi, disp);
+            apply_force_w(B, b1, p1w, F);
+            if (B.parent[b1] >= 0) apply_force_w(B, B.parent[b1], p1w, F);
+            F = scale(-fi, disp);
+            apply_force_w(B, b2, p2w, F);
+            if (B.parent[b2] >= 0) apply_force_w(B, B.parent[b2], p2w, F);
+        }
+    }
+}
+void launch_springs(hipStream_t s, BodiesView B, const am_spring* springs, int n, double ks, double ds) {
+    if (n > 0) hipLaunchKernelGGL(k_springs, dim3(1), dim3(64), 0, s, B, springs, n, ks, ds);
+}
+
 // RigidBody.advanceVelocities (RigidBody.java:409-417)
 __global__ void k_advance_velocities(BodiesView B, const int* targets, int n, double dt) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1425,6 +1474,133 @@ __global__ void k_advance_velocities(BodiesView B, const int* targets, int n, do
     st3(&B.w[i * 3], w);
 }
 
+// Sine and cosine of the rotation angle (expRodrigues calls Math.sin and
+// Math.cos).  Computed by fdlibm 5.3's algorithm (Sun Microsystems, 1993),
+// which defines java.lang.StrictMath.sin/cos: Cody-Waite reduction by pi/2 in
+// up to three steps (e_rem_pio2.c, |x| <= 2^19 pi/2) and the minimax kernels
+// of k_sin.c / k_cos.c.  Larger arguments, which no physical step reaches,
+// take the three-step reduction with a floating-point quotient.  Written out
+// here rather than calling the device library so that every rounding is
+// fixed by the source, whatever the library version.
+namespace fdm {
+__device__ __forceinline__ int hi(double x) { return (int)(__double_as_longlong(x) >> 32); }
+__device__ __forceinline__ double from_hi(int h) { return __longlong_as_double((long long)(unsigned)h << 32); }
+constexpr double S1 = -1.66666666666666324348e-01, S2 = 8.33333333332248946124e-03,
+                 S3 = -1.98412698298579493134e-04, S4 = 2.75573137070700676789e-06,
+                 S5 = -2.50507602534068634195e-08, S6 = 1.58969099521155010221e-10;
+constexpr double C1 = 4.16666666666666019037e-02, C2 = -1.38888888888741095749e-03,
+                 C3 = 2.48015872894767294178e-05, C4 = -2.75573143513906633035e-07,
+                 C5 = 2.08757232129817482790e-09, C6 = -1.13596475577881948265e-11;
+constexpr double invpio2 = 6.36619772367581382433e-01, pio2_1 = 1.57079632673412561417e+00,
+                 pio2_1t = 6.07710050650619224932e-11, pio2_2 = 6.07710050630396597660e-11,
+                 pio2_2t = 2.02226624879595063154e-21, pio2_3 = 2.02226624871116645580e-21,
+                 pio2_3t = 8.47842766036889956997e-32;
+// high words of the doubles nearest n pi/2, n = 1..32
+__constant__ int npio2_hw[32] = {
+    0x3ff921fb, 0x400921fb, 0x4012d97c, 0x401921fb, 0x401f6a7a, 0x4022d97c, 0x4025fdbb, 0x402921fb,
+    0x402c463a, 0x402f6a7a, 0x4031475c, 0x4032d97c, 0x40346b9c, 0x4035fdbb, 0x40378fdb, 0x403921fb,
+    0x403ab41b, 0x403c463a, 0x403dd85a, 0x403f6a7a, 0x40407e4c, 0x4041475c, 0x4042106c, 0x4042d97c,
+    0x4043a28c, 0x40446b9c, 0x404534ac, 0x4045fdbb, 0x4046c6cb, 0x40478fdb, 0x404858eb, 0x404921fb};
+
+__device__ double k_sin(double x, double y, bool tail) {
+    int ix = hi(x) & 0x7fffffff;
+    if (ix < 0x3e400000 && (int)x == 0) return x;
+    double z = x * x, v = z * x;
+    double r = S2 + z * (S3 + z * (S4 + z * (S5 + z * S6)));
+    if (!tail) return x + v * (S1 + z * r);
+    return x - ((z * (0.5 * y - v * r) - y) - v * S1);
+}
+__device__ double k_cos(double x, double y) {
+    int ix = hi(x) & 0x7fffffff;
+    if (ix < 0x3e400000 && (int)x == 0) return 1.0;
+    double z = x * x;
+    double r = z * (C1 + z * (C2 + z * (C3 + z * (C4 + z * (C5 + z * C6)))));
+    if (ix < 0x3fd33333) return 1.0 - (0.5 * z - (z * r - x * y));
+    double qx = ix > 0x3fe90000 ? 0.28125 : from_hi(ix - 0x00200000);
+    double hz = 0.5 * z - qx, a = 1.0 - qx;
+    return a - (hz - (z * r - x * y));
+}
+// x = n pi/2 + y0 + y1 for |x| >= pi/4; returns n
+__device__ int rem_pio2(double x, double& y0, double& y1) {
+    int hx = hi(x), ix = hx & 0x7fffffff;
+    if (ix < 0x4002d97c) {  // |x| < 3pi/4
+        const double s = hx > 0 ? 1.0 : -1.0;
+        double z = x - s * pio2_1;
+        if (ix != 0x3ff921fb) {
+            y0 = z - s * pio2_1t;
+            y1 = (z - y0) - s * pio2_1t;
+        } else {
+            z = z - s * pio2_2;
+            y0 = z - s * pio2_2t;
+            y1 = (z - y0) - s * pio2_2t;
+        }
+        return hx > 0 ? 1 : -1;
+    }
+    double t = fabs(x);
+    const bool medium = ix <= 0x413921fb;
+    double fn;
+    int n;
+    if (medium) {
+        n = (int)(t * invpio2 + 0.5);
+        fn = (double)n;
+    } else {
+        fn = floor(t * invpio2 + 0.5);
+        n = (int)(fn - 4.0 * floor(fn * 0.25));
+    }
+    double r = t - fn * pio2_1;
+    double w = fn * pio2_1t;
+    y0 = r - w;
+    if (!(medium && n < 32 && ix != npio2_hw[n - 1])) {
+        const int j = ix >> 20;
+        int i = j - ((hi(y0) >> 20) & 0x7ff);
+        if (i > 16 || !medium) {
+            t = r;
+            w = fn * pio2_2;
+            r = t - w;
+            w = fn * pio2_2t - ((t - r) - w);
+            y0 = r - w;
+            i = j - ((hi(y0) >> 20) & 0x7ff);
+            if (i > 49 || !medium) {
+                t = r;
+                w = fn * pio2_3;
+                r = t - w;
+                w = fn * pio2_3t - ((t - r) - w);
+                y0 = r - w;
+            }
+        }
+    }
+    y1 = (r - y0) - w;
+    if (hx < 0) {
+        y0 = -y0;
+        y1 = -y1;
+        return -n;
+    }
+    return n;
+}
+// both at once: the reduction is shared
+__device__ void sincos(double x, double& s, double& c) {
+    int ix = hi(x) & 0x7fffffff;
+    if (ix <= 0x3fe921fb) {
+        s = k_sin(x, 0.0, false);
+        c = k_cos(x, 0.0);
+        return;
+    }
+    if (ix >= 0x7ff00000) {
+        s = c = x - x;
+        return;
+    }
+    double y0, y1;
+    const int q = rem_pio2(x, y0, y1) & 3;
+    const double ks = k_sin(y0, y1, true), kc = k_cos(y0, y1);
+    switch (q) {
+        case 0: s = ks; c = kc; break;
+        case 1: s = kc; c = -ks; break;
+        case 2: s = -ks; c = -kc; break;
+        default: s = -kc; c = ks; break;
+    }
+}
+}  // namespace fdm
+
 // RigidBody.advancePositions (RigidBody.java:427-441): Rodrigues rotation
 // (expRodrigues, RigidBody.java:382-401) and Matrix3d.normalizeCP.
 __global__ void k_advance_positions(BodiesView B, const int* targets, int n, double dt) {
@@ -1439,7 +1615,9 @@ __global__ void k_advance_positions(BodiesView B, const int* targets, int n, dou
     d3 d = normalized(om);
     m33 R = ldm(&B.R[i * 9]);
     if (t > 1e-8) {
-        double c = cos(t), s = sin(t), c1 = 1 - c;
+        double s, c;
+        fdm::sincos(t, s, c);
+        const double c1 = 1 - c;
         double wX = d.x, wY = d.y, wZ = d.z;
         m33 dR;
         dR.m[0] = c + wX * wX * c1;

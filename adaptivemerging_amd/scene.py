@@ -12,8 +12,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import (AM_BODY_PINNED, AM_GEOM_BOX, AM_GEOM_COLLECTION, AM_GEOM_PLANE, AM_GEOM_TREE, BODY_DTYPE,
-                   NODE_DTYPE)
+from ._lib import (AM_BODY_PINNED, AM_GEOM_BOX, AM_GEOM_COLLECTION, AM_GEOM_PLANE, AM_GEOM_TREE, AM_SPRING_BODYBODY,
+                   AM_SPRING_WORLD, AM_SPRING_ZERO, BODY_DTYPE, NODE_DTYPE, SPRING_DTYPE)
 
 
 def rot_axis_angle(axis, angle) -> np.ndarray:
@@ -54,6 +54,7 @@ class Scene:
     v: list = field(default_factory=list)
     w: list = field(default_factory=list)
     names: list = field(default_factory=list)
+    springs: list = field(default_factory=list)      # list of SPRING_DTYPE records
 
     # -- body builders -------------------------------------------------
     def _append(self, rec, x, R, v, w, name):
@@ -186,6 +187,51 @@ class Scene:
         rec["tree_first"] = first
         rec["tree_count"] = n
         return self._append(rec, x, R, v, w, name)
+
+    # -- springs (Spring.java:50-104) -----------------------------------
+    def body_point_world(self, body: int, pB, x=None, R=None):
+        """RigidTransform3D.transform: R pB + x, with the Matrix3d.transform
+        evaluation order; x and R default to the body's current pose."""
+        R = self.R[body] if R is None else np.asarray(R, dtype=np.float64).reshape(3, 3)
+        x = self.x[body] if x is None else np.asarray(x, dtype=np.float64).reshape(3)
+        p = [float(t) for t in pB]
+        return [R[i, 0] * p[0] + R[i, 1] * p[1] + R[i, 2] * p[2] + x[i] for i in range(3)]
+
+    def add_spring(self, body1: int, pB, pW=None, body2: int | None = None, pB2=None, k: float = 100.0,
+                   d: float = 10.0, ls: float = 1.0, pose1=None) -> int:
+        """Spring(pB, body), Spring(pB, body, pW) or Spring(pB1, body1, pB2,
+        body2) (Spring.java:50-104).  The attachment points' world positions,
+        hence pw and l0, come from the poses at construction: pose1 = (x, R)
+        overrides body1's current pose (a spring parsed before a body's <x> or
+        <R> sees the pose it had then)."""
+        rec = np.zeros((), dtype=SPRING_DTYPE)
+        rec["body1"] = body1
+        rec["body2"] = -1
+        rec["pb1"] = pB
+        rec["k"], rec["d"], rec["ls"] = k, d, ls
+        x1, R1 = (None, None) if pose1 is None else pose1
+        p1w = self.body_point_world(body1, pB, x1, R1)
+        if body2 is not None:
+            rec["type"] = AM_SPRING_BODYBODY
+            rec["body2"] = body2
+            rec["pb2"] = pB2
+            p2w = self.body_point_world(body2, pB2)
+            disp = [p1w[i] - p2w[i] for i in range(3)]
+            rec["l0"] = math.sqrt(disp[0] * disp[0] + disp[1] * disp[1] + disp[2] * disp[2])
+        elif pW is not None:
+            rec["type"] = AM_SPRING_WORLD
+            rec["pw"] = pW
+            disp = [float(pW[i]) - p1w[i] for i in range(3)]
+            rec["l0"] = math.sqrt(disp[0] * disp[0] + disp[1] * disp[1] + disp[2] * disp[2])
+        else:
+            rec["type"] = AM_SPRING_ZERO
+            rec["pw"] = p1w
+            rec["l0"] = 0.5   # the field's initial value; the constructor leaves it
+        self.springs.append(rec)
+        return len(self.springs) - 1
+
+    def spring_array(self):
+        return np.array(self.springs, dtype=SPRING_DTYPE) if self.springs else np.zeros(0, dtype=SPRING_DTYPE)
 
     # -- arrays ---------------------------------------------------------
     def arrays(self):

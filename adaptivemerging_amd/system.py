@@ -11,6 +11,7 @@ import time
 
 import numpy as np
 
+from ._lib import AM_SPRING_ZERO
 from .context import Context
 
 
@@ -233,23 +234,28 @@ class RigidBodySystem:
         self.useGravity = True
         self.gravityAmount = 1.0
         self.gravityAngle = 90.0
+        self.springStiffnessMod = 1.0   # RigidBodySystem.java:553
+        self.springDampingMod = 1.0     # RigidBodySystem.java:554
         self.totalSteps = 0
         self.simulationTime = 0.0
         self.computeTime = 0.0
-        self._initial = scene.arrays()[2:]
 
     @property
     def bodies(self):
         return self.scene.bodies
 
     def applyExternalForces(self, dt: float):
-        """applyExternalForces (:234-257): gravity (:276-290)."""
+        """applyExternalForces (:234-257): gravity (:276-290), then the
+        scene's springs (applySpringForces :309-322)."""
         p = self.ctx.params
         p.use_gravity = int(self.useGravity)
         p.gravity_amount = self.gravityAmount
         p.gravity_angle_deg = self.gravityAngle
+        p.spring_stiffness_mod = self.springStiffnessMod
+        p.spring_damping_mod = self.springDampingMod
         self.ctx.set_params()
         self.ctx.apply_gravity()
+        self.ctx.apply_spring_forces()
 
     def advanceTime(self, dt: float):
         """advanceTime (:102-185)."""
@@ -283,10 +289,17 @@ class RigidBodySystem:
         self.simulationTime += dt
 
     def reset(self):
-        """reset (:390-426): initial poses and velocities."""
-        x, R, v, w = self._initial
-        self.ctx.push_state(x, R, v, w)
-        self.ctx.update_contacts_map()
+        """reset (:390-426): collections dissolved, bodies back to their
+        initial poses and velocities, the contact history cleared, and each
+        spring reset (Spring.reset, Spring.java:106-130): a zero spring's
+        anchor moves to its point's initial world position."""
+        self.ctx.upload(self.scene)
+        springs = self.scene.spring_array().copy()
+        for sp in springs:
+            if int(sp["type"]) == AM_SPRING_ZERO:
+                sp["pw"] = self.scene.body_point_world(int(sp["body1"]), sp["pb1"])
+        if len(springs):
+            self.ctx.set_springs(springs)
         self.totalSteps = 0
         self.simulationTime = 0.0
 

@@ -227,6 +227,47 @@ def merge_stack(seed: int = 0, layers: int = 4, density: float = 500.0) -> Scene
     return s
 
 
+def hung_platform(seed: int = 0, n_boxes: int = 16, k: float = 5000.0, d: float = 200.0) -> Scene:
+    """A platform hung by four world springs at its corners with a few
+    layers of boxes on it (one of them on a zero-length spring), a chain of
+    spheres tied by body-body springs hanging from a world spring, over a
+    ground plane (the spring scenes of scenes3D such as tower25platform.xml
+    and dominosPlatforms.xml)."""
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    s.add_plane([0, -2.0, 0], [0, 1, 0])
+    plat = s.add_box([3.0, 0.2, 3.0], [0.0, 0.0, 0.0], density=100.0)
+    for cx in (1.5, -1.5):
+        for cz in (1.5, -1.5):
+            s.add_spring(plat, [cx, 0.1, cz], pW=[cx, 1.1, cz], k=k, d=d)
+    side = max(1, int(math.ceil(math.sqrt(n_boxes / 2))))
+    placed = 0
+    layer = 0
+    while placed < n_boxes:
+        for i in range(side):
+            for j in range(side):
+                if placed == n_boxes:
+                    break
+                e = rng.uniform(0.25, 0.4)
+                x = -0.9 + 1.8 * (i + 0.5) / side + rng.uniform(-0.02, 0.02)
+                z = -0.9 + 1.8 * (j + 0.5) / side + rng.uniform(-0.02, 0.02)
+                b = s.add_box([e, e, e], [x, 0.1 + 0.2 + 0.45 * layer + 0.01, z], density=rng.uniform(200.0, 800.0))
+                if placed == 0:
+                    # zero-length spring: its rest length is 0.5, so near its anchor it pushes the box down
+                    s.add_spring(b, [0.0, 0.0, 0.0], k=0.02 * k, d=0.02 * d)
+                placed += 1
+        layer += 1
+    prev = None
+    for q in range(3):
+        b = s.add_sphere(0.2, [2.6, 1.5 - 0.6 * q, 0.0], density=300.0)
+        if prev is None:
+            s.add_spring(b, [0.0, 0.2, 0.0], pW=[2.6, 2.2, 0.0], k=0.1 * k, d=0.05 * d)
+        else:
+            s.add_spring(b, [0.0, 0.2, 0.0], body2=prev, pB2=[0.0, -0.2, 0.0], k=0.1 * k, d=0.05 * d, ls=0.9)
+        prev = b
+    return s
+
+
 def batch(scenes) -> Scene:
     """Independent scenes in one context (scene ids 0..n-1, bodies and sphere-
     tree nodes concatenated): bodies of different scenes never collide."""

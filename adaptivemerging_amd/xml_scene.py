@@ -8,8 +8,10 @@ Follows XMLParser (src/mergingBodies3D/XMLParser.java):
 - setCommonAttributes (:542-626): x, R (axis-angle), v, omega, restitution,
   friction, pinned and the tags that do not touch the contact step.
 
-Mesh and composite bodies (createMesh :449-538, createComposite :177-353),
-springs and spinners have no counterpart in this build; a scene that uses
+Springs (:596-619, Spring.java) load with the attachment geometry their
+constructors compute from the pose the body has when the <spring> tag is
+read.  Mesh and composite bodies (createMesh :449-538, createComposite
+:177-353) and spinners have no counterpart in this build; a scene that uses
 them raises SceneFormatError rather than loading into different dynamics.
 Sphere-tree geometry for hand-built bodies comes from .sph files through
 scene.load_sph (the format of :500-535).
@@ -17,6 +19,7 @@ scene.load_sph (the format of :500-535).
 from __future__ import annotations
 
 import os
+import warnings
 import xml.etree.ElementTree as ET
 
 import numpy as np
@@ -72,7 +75,7 @@ class _Common:
     """The per-body settings of setCommonAttributes (XMLParser.java:542-626);
     a later tag overrides an earlier one, as there."""
 
-    def __init__(self, node, name: str, allow_springs: bool):
+    def __init__(self, node, name: str, drop_springs: bool):
         self.x = None
         self.R = None
         self.v = None
@@ -80,6 +83,7 @@ class _Common:
         self.friction = 0.8      # RigidBody.friction default (RigidBody.java:142)
         self.restitution = 0.0   # RigidBody.restitution default (RigidBody.java:145)
         self.pinned = False
+        self.springs = []        # (attributes, pose (x, R) when the tag was read)
         for c in node:
             tag = c.tag.lower()
             text = (c.text or "").strip()
@@ -104,13 +108,41 @@ class _Common:
             elif tag == "spinner":
                 raise SceneFormatError(f"body {name}: spinners are not modelled")
             elif tag == "spring":
-                if not allow_springs:
-                    raise SceneFormatError(f"body {name}: springs are not modelled (load with ignore_springs=True "
-                                           "to drop them)")
+                if not drop_springs:
+                    pose = (list(self.x) if self.x is not None else [0.0, 0.0, 0.0],
+                            np.array(self.R) if self.R is not None else np.eye(3))
+                    self.springs.append((dict(c.attrib), pose))
             # temporarilyPinned is ignored by the reference itself; magnetic
             # only matters once a magnet is switched on from the UI; factorypart
             # and col are for the factory feature and drawing
         self.pos = self.x if self.x is not None else [0.0, 0.0, 0.0]
+
+
+def _add_springs(scene: Scene, body: int, com: "_Common", name: str, posed_planes: set) -> None:
+    """The springs of one body, as setCommonAttributes builds them
+    (XMLParser.java:596-619): pW gives a world spring, pB2 with body2 a
+    body-body spring (body2 named among the bodies read before), otherwise a
+    spring to the point's initial world position; k, d and ls override the
+    defaults 100, 10 and 1 (Spring.java:27-31)."""
+    for attrs, pose in com.springs:
+        kw = {}
+        for key, default in (("k", 100.0), ("d", 10.0), ("ls", 1.0)):
+            kw[key] = float(attrs[key]) if attrs.get(key, "") != "" else default
+        pB = _t3d(attrs.get("pB", "0 0 0"))
+        if "pW" in attrs:
+            scene.add_spring(body, pB, pW=_t3d(attrs["pW"]), pose1=pose, **kw)
+        elif "pB2" in attrs and "body2" in attrs:
+            target = attrs["body2"]
+            others = [i for i, nm in enumerate(scene.names[:body]) if nm == target]
+            if not others:
+                # the reference prints "[ParseSpring] Something is wrong with the spring." and drops it
+                warnings.warn(f"body {name}: spring to unknown body {target!r} dropped, as the reference does")
+                continue
+            if others[0] in posed_planes:
+                raise SceneFormatError(f"body {name}: spring to plane {target!r}, which has a pose tag")
+            scene.add_spring(body, pB, body2=others[0], pB2=_t3d(attrs["pB2"]), pose1=pose, **kw)
+        else:
+            scene.add_spring(body, pB, pose1=pose, **kw)
 
 
 def parse_scene(source: str, *, ignore_springs: bool = False, scene_index: int = 0):
@@ -141,6 +173,9 @@ def parse_scene(source: str, *, ignore_springs: bool = False, scene_index: int =
                 collision["frictionOverride"] = True
 
     scene = Scene()
+    # planes keep x = 0 and R = I here; the reference moves them with <x>/<R>,
+    # which only matters to a spring attached to one
+    posed_planes: set = set()
     for node in _children(root, "body"):
         kind = node.attrib.get("type", "").lower()
         name = node.attrib.get("name", "")
@@ -150,18 +185,24 @@ def parse_scene(source: str, *, ignore_springs: bool = False, scene_index: int =
             density = float(node.attrib.get("density", 1.0))
             if "scale" in node.attrib:
                 s = s * float(node.attrib["scale"])
-            scene.add_box(s, com.pos, R=com.R, v=com.v, w=com.w, density=density, friction=com.friction,
-                          restitution=com.restitution, pinned=com.pinned, scene=scene_index, name=name)
+            b = scene.add_box(s, com.pos, R=com.R, v=com.v, w=com.w, density=density, friction=com.friction,
+                              restitution=com.restitution, pinned=com.pinned, scene=scene_index, name=name)
+            _add_springs(scene, b, com, name, posed_planes)
         elif kind == "plane":
             # PlaneRigidBody is always pinned (PlaneRigidBody.java:55-62)
-            scene.add_plane(_t3d(node.attrib["p"]), _t3d(node.attrib["n"]), friction=com.friction,
-                            restitution=com.restitution, scene=scene_index, name=name)
+            if com.springs:
+                raise SceneFormatError(f"body {name}: springs on a plane are not supported")
+            b = scene.add_plane(_t3d(node.attrib["p"]), _t3d(node.attrib["n"]), friction=com.friction,
+                                restitution=com.restitution, scene=scene_index, name=name)
+            if com.x is not None or com.R is not None:
+                posed_planes.add(b)
         elif kind == "sphere":
             if com.pinned:
                 raise SceneFormatError(f"body {name}: pinned spheres are not supported")
             density = float(node.attrib.get("density", 1.0))
-            scene.add_sphere(float(node.attrib["r"]), com.pos, R=com.R, v=com.v, w=com.w, density=density,
-                             friction=com.friction, restitution=com.restitution, scene=scene_index, name=name)
+            b = scene.add_sphere(float(node.attrib["r"]), com.pos, R=com.R, v=com.v, w=com.w, density=density,
+                                 friction=com.friction, restitution=com.restitution, scene=scene_index, name=name)
+            _add_springs(scene, b, com, name, posed_planes)
         elif kind in ("mesh", "composite"):
             raise SceneFormatError(f"body {name}: {kind} bodies are not supported by this build")
         else:

@@ -127,6 +127,8 @@ typedef struct am_params {
     double gravity_angle_deg;         /* 90 */
     double viscous_linear_decay;      /* 1 */
     double viscous_angular_decay;     /* 1 */
+    double spring_stiffness_mod;      /* 1 (RigidBodySystem.springStiffnessMod) */
+    double spring_damping_mod;        /* 1 (RigidBodySystem.springDampingMod) */
     /* merging (Merging.MergeParameters) */
     int32_t enable_merging;           /* 1 */
     int32_t enable_merge_pinned;      /* 1 */
@@ -158,6 +160,30 @@ typedef struct am_params {
     int32_t learned_warm_start;
     int32_t collect_training;         /* 1: each main solve appends (features, converged lambda) */
 } am_params;
+
+/* A spring (Spring.java:10-104).  ZERO pulls body1's point pb1 towards the
+ * world point it started at (rest length l0 = 0.5 unless set), WORLD towards
+ * the world point pw, BODYBODY joins pb1 on body1 and pb2 on body2.  pb1/pb2
+ * are in the bodies' frames; pw and l0 are what the constructors computed
+ * from the initial poses.  k stiffness, d damping, ls rest-length scale. */
+enum {
+    AM_SPRING_ZERO = 0,
+    AM_SPRING_WORLD = 1,
+    AM_SPRING_BODYBODY = 2,
+};
+typedef struct am_spring {
+    int32_t type;
+    int32_t body1;
+    int32_t body2;        /* BODYBODY only, else -1 */
+    int32_t pad;
+    double pb1[3];
+    double pb2[3];
+    double pw[3];
+    double l0;
+    double k;
+    double d;
+    double ls;
+} am_spring;
 
 /* Caller-owned destination arrays for am_contacts_pull; any may be NULL.
  * Vectors are 3 doubles per contact, the Jacobian 36 per contact in the
@@ -265,6 +291,14 @@ int am_forces_pull(am_ctx* ctx, double* force, double* torque, int32_t n);
 int am_clear_bodies(am_ctx* ctx);
 /* RigidBodySystem.applyGravityForce (RigidBodySystem.java:276-290). */
 int am_apply_gravity(am_ctx* ctx);
+/* Replaces RigidBodySystem.springs (RigidBodySystem.java:34; filled by
+ * XMLParser.setCommonAttributes :596-619).  Bodies are physical body indices. */
+int am_springs_set(am_ctx* ctx, const am_spring* springs, int32_t n);
+/* RigidBodySystem.applySpringForces (RigidBodySystem.java:309-322) with
+ * Spring.apply (Spring.java:143-214): in list order, each spring adds its force
+ * and torque to its bodies (and to their collections).  am_step calls it right
+ * after the gravity, as applyExternalForces does (:234-257). */
+int am_apply_spring_forces(am_ctx* ctx);
 int am_deltav_pull(am_ctx* ctx, double* dv, int32_t n);   /* 6 per entity: v then w */
 
 /* ---- detection (CollisionProcessor) ----------------------------------- */

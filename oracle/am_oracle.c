@@ -114,6 +114,7 @@ struct orc_world {
     orc_contact* old; int32_t nold, capold;
     /* sorted lookup over old contacts */
     uint64_t* okey; int32_t* oidx; int32_t nokey;
+    am_spring* springs; int32_t n_springs;
     am_params p;
 };
 
@@ -133,6 +134,7 @@ static void default_params(am_params* p) {
     p->compliance = 1e-3; p->restitution = 0.5; p->friction = 0.1; p->sliding_threshold = 0.01;
     p->warm_start = 1; p->use_gravity = 1; p->gravity_amount = 1.0; p->gravity_angle_deg = 90.0;
     p->viscous_linear_decay = 1.0; p->viscous_angular_decay = 1.0;
+    p->spring_stiffness_mod = 1.0; p->spring_damping_mod = 1.0;
 }
 
 orc_world* orc_create(const am_body_desc* bodies, int32_t n, const am_sphere_node* nodes, int32_t n_nodes) {
@@ -170,7 +172,7 @@ void orc_destroy(orc_world* w) {
     free(w->desc); free(w->nodes); free(w->cw);
     free(w->x); free(w->R); free(w->v); free(w->w); free(w->f); free(w->t); free(w->jinv); free(w->dv);
     free(w->parent); free(w->top); free(w->mem_first); free(w->mem_count); free(w->members);
-    free(w->c); free(w->old); free(w->okey); free(w->oidx);
+    free(w->c); free(w->old); free(w->okey); free(w->oidx); free(w->springs);
     free(w);
 }
 
@@ -280,6 +282,77 @@ void orc_apply_gravity(orc_world* w) {
                 double mb = -w->desc[b].mass;
                 for (int a = 0; a < 3; a++) w->f[b * 3 + a] = w->f[b * 3 + a] + mb * dir[a];
             }
+        }
+    }
+}
+
+/* RigidBodySystem.springs (filled by XMLParser.setCommonAttributes :596-619) */
+void orc_set_springs(orc_world* w, const am_spring* springs, int32_t n) {
+    free(w->springs);
+    w->springs = (am_spring*)malloc((size_t)(n ? n : 1) * sizeof(am_spring));
+    if (n) memcpy(w->springs, springs, (size_t)n * sizeof(am_spring));
+    w->n_springs = n;
+}
+
+/* RigidBody.applyForceW (RigidBody.java:335-340) */
+static void apply_force_w(orc_world* w, int32_t e, const double* p, const double* F) {
+    double r[3], tq[3];
+    for (int a = 0; a < 3; a++) w->f[e * 3 + a] = w->f[e * 3 + a] + F[a];
+    for (int a = 0; a < 3; a++) r[a] = p[a] - w->x[e * 3 + a];
+    cross3(tq, r, F);
+    for (int a = 0; a < 3; a++) w->t[e * 3 + a] = w->t[e * 3 + a] + tq[a];
+}
+/* RigidBody.getSpatialVelocity (RigidBody.java:458-462) */
+static void point_velocity(const orc_world* w, int32_t e, const double* p, double* out) {
+    double r[3];
+    for (int a = 0; a < 3; a++) r[a] = p[a] - w->x[e * 3 + a];
+    cross3(out, &w->w[e * 3], r);
+    for (int a = 0; a < 3; a++) out[a] = out[a] + w->v[e * 3 + a];
+}
+/* body point to world: RigidTransform3D.transform (RigidTransform3D.java:107-110) */
+static void body_to_world(const orc_world* w, int32_t e, const double* pb, double* out) {
+    mat_vec(out, &w->R[e * 9], pb);
+    for (int a = 0; a < 3; a++) out[a] = out[a] + w->x[e * 3 + a];
+}
+
+/* RigidBodySystem.applySpringForces (RigidBodySystem.java:309-322) with
+ * Spring.applyOneBody / applyTwoBodies (Spring.java:165-214), springs in order */
+void orc_apply_springs(orc_world* w) {
+    const double ks = w->p.spring_stiffness_mod, ds = w->p.spring_damping_mod;
+    for (int32_t i = 0; i < w->n_springs; i++) {
+        const am_spring* sp = &w->springs[i];
+        const int32_t b1 = sp->body1;
+        double p1w[3], disp[3], F[3];
+        body_to_world(w, b1, sp->pb1, p1w);
+        if (sp->type != AM_SPRING_BODYBODY) {
+            for (int a = 0; a < 3; a++) disp[a] = sp->pw[a] - p1w[a];
+            double len = len3(disp);
+            if (len < 1e-3) continue;
+            double vel[3];
+            point_velocity(w, b1, p1w, vel);
+            double scl = -(sp->k * ks * (len - sp->l0 * sp->ls) - sp->d * ds * (dot3(vel, disp) / len)) / len;
+            for (int a = 0; a < 3; a++) F[a] = -scl * disp[a];
+            apply_force_w(w, b1, p1w, F);
+            if (w->parent[b1] >= 0) apply_force_w(w, w->parent[b1], p1w, F);
+        } else {
+            const int32_t b2 = sp->body2;
+            double p2w[3], v1[3], v2[3], rel[3];
+            body_to_world(w, b2, sp->pb2, p2w);
+            for (int a = 0; a < 3; a++) disp[a] = p2w[a] - p1w[a];
+            double dist = len3(disp);
+            if (dist < 1e-3) continue;
+            double inv = 1. / dist;
+            for (int a = 0; a < 3; a++) disp[a] = inv * disp[a];
+            point_velocity(w, b2, p2w, v2);
+            point_velocity(w, b1, p1w, v1);
+            for (int a = 0; a < 3; a++) rel[a] = v2[a] - v1[a];
+            double fi = sp->k * ks * (dist - sp->l0 * sp->ls) + sp->d * ds * dot3(rel, disp);
+            for (int a = 0; a < 3; a++) F[a] = fi * disp[a];
+            apply_force_w(w, b1, p1w, F);
+            if (w->parent[b1] >= 0) apply_force_w(w, w->parent[b1], p1w, F);
+            for (int a = 0; a < 3; a++) F[a] = -fi * disp[a];
+            apply_force_w(w, b2, p2w, F);
+            if (w->parent[b2] >= 0) apply_force_w(w, w->parent[b2], p2w, F);
         }
     }
 }
@@ -1304,6 +1377,156 @@ static void advance_velocities_one(orc_world* w, int32_t i, double dt) {
     for (int a = 0; a < 3; a++) w->w[i * 3 + a] = w->w[i * 3 + a] + w->dv[i * 6 + 3 + a];
 }
 
+/* Sine and cosine for expRodrigues, which calls Math.cos / Math.sin.  The
+ * algorithm is fdlibm 5.3's (Sun Microsystems, 1993): s_sin.c / s_cos.c,
+ * the kernels k_sin.c / k_cos.c (degree-13 and degree-14 minimax polynomials
+ * on [-pi/4, pi/4]) and the Cody-Waite reduction of e_rem_pio2.c for
+ * |x| <= 2^19 pi/2; java.lang.StrictMath.sin/cos are defined as fdlibm's
+ * results.  Arguments beyond 2^19 pi/2 (fdlibm's Payne-Hanek range, which a
+ * rotation per step never reaches) use the same three-step reduction with a
+ * floating-point quotient. */
+static int32_t fd_hi(double x) {
+    uint64_t b;
+    memcpy(&b, &x, 8);
+    return (int32_t)(b >> 32);
+}
+static double fd_with_hi(int32_t hi) {
+    uint64_t b = (uint64_t)(uint32_t)hi << 32;
+    double x;
+    memcpy(&x, &b, 8);
+    return x;
+}
+static const double fd_S1 = -1.66666666666666324348e-01, fd_S2 = 8.33333333332248946124e-03,
+                    fd_S3 = -1.98412698298579493134e-04, fd_S4 = 2.75573137070700676789e-06,
+                    fd_S5 = -2.50507602534068634195e-08, fd_S6 = 1.58969099521155010221e-10;
+static const double fd_C1 = 4.16666666666666019037e-02, fd_C2 = -1.38888888888741095749e-03,
+                    fd_C3 = 2.48015872894767294178e-05, fd_C4 = -2.75573143513906633035e-07,
+                    fd_C5 = 2.08757232129817482790e-09, fd_C6 = -1.13596475577881948265e-11;
+static const double fd_invpio2 = 6.36619772367581382433e-01, fd_pio2_1 = 1.57079632673412561417e+00,
+                    fd_pio2_1t = 6.07710050650619224932e-11, fd_pio2_2 = 6.07710050630396597660e-11,
+                    fd_pio2_2t = 2.02226624879595063154e-21, fd_pio2_3 = 2.02226624871116645580e-21,
+                    fd_pio2_3t = 8.47842766036889956997e-32;
+/* high words of the doubles nearest n pi/2, n = 1..32 */
+static const int32_t fd_npio2_hw[32] = {
+    0x3ff921fb, 0x400921fb, 0x4012d97c, 0x401921fb, 0x401f6a7a, 0x4022d97c, 0x4025fdbb, 0x402921fb,
+    0x402c463a, 0x402f6a7a, 0x4031475c, 0x4032d97c, 0x40346b9c, 0x4035fdbb, 0x40378fdb, 0x403921fb,
+    0x403ab41b, 0x403c463a, 0x403dd85a, 0x403f6a7a, 0x40407e4c, 0x4041475c, 0x4042106c, 0x4042d97c,
+    0x4043a28c, 0x40446b9c, 0x404534ac, 0x4045fdbb, 0x4046c6cb, 0x40478fdb, 0x404858eb, 0x404921fb};
+
+static double fd_kernel_sin(double x, double y, int iy) {
+    int32_t ix = fd_hi(x) & 0x7fffffff;
+    if (ix < 0x3e400000 && (int)x == 0) return x;
+    double z = x * x, v = z * x;
+    double r = fd_S2 + z * (fd_S3 + z * (fd_S4 + z * (fd_S5 + z * fd_S6)));
+    if (iy == 0) return x + v * (fd_S1 + z * r);
+    return x - ((z * (0.5 * y - v * r) - y) - v * fd_S1);
+}
+static double fd_kernel_cos(double x, double y) {
+    int32_t ix = fd_hi(x) & 0x7fffffff;
+    if (ix < 0x3e400000 && (int)x == 0) return 1.0;
+    double z = x * x;
+    double r = z * (fd_C1 + z * (fd_C2 + z * (fd_C3 + z * (fd_C4 + z * (fd_C5 + z * fd_C6)))));
+    if (ix < 0x3fd33333) return 1.0 - (0.5 * z - (z * r - x * y));
+    double qx = ix > 0x3fe90000 ? 0.28125 : fd_with_hi(ix - 0x00200000);
+    double hz = 0.5 * z - qx, a = 1.0 - qx;
+    return a - (hz - (z * r - x * y));
+}
+/* e_rem_pio2.c for |x| up to 2^19 pi/2: x = n pi/2 + y[0] + y[1] */
+static int fd_rem_pio2(double x, double* y) {
+    int32_t hx = fd_hi(x), ix = hx & 0x7fffffff;
+    if (ix < 0x4002d97c) { /* |x| < 3pi/4: n = +-1 */
+        double z;
+        if (hx > 0) {
+            z = x - fd_pio2_1;
+            if (ix != 0x3ff921fb) {
+                y[0] = z - fd_pio2_1t;
+                y[1] = (z - y[0]) - fd_pio2_1t;
+            } else {
+                z -= fd_pio2_2;
+                y[0] = z - fd_pio2_2t;
+                y[1] = (z - y[0]) - fd_pio2_2t;
+            }
+            return 1;
+        }
+        z = x + fd_pio2_1;
+        if (ix != 0x3ff921fb) {
+            y[0] = z + fd_pio2_1t;
+            y[1] = (z - y[0]) + fd_pio2_1t;
+        } else {
+            z += fd_pio2_2;
+            y[0] = z + fd_pio2_2t;
+            y[1] = (z - y[0]) + fd_pio2_2t;
+        }
+        return -1;
+    }
+    double t = fabs(x);
+    const int medium = ix <= 0x413921fb;
+    double fn;
+    int n;
+    if (medium) {
+        n = (int)(t * fd_invpio2 + 0.5);
+        fn = (double)n;
+    } else {
+        fn = floor(t * fd_invpio2 + 0.5);
+        n = (int)(fn - 4.0 * floor(fn * 0.25));
+    }
+    double r = t - fn * fd_pio2_1;
+    double w = fn * fd_pio2_1t;
+    if (medium && n < 32 && ix != fd_npio2_hw[n - 1]) {
+        y[0] = r - w;
+    } else {
+        int32_t j = ix >> 20;
+        y[0] = r - w;
+        int32_t i = j - ((fd_hi(y[0]) >> 20) & 0x7ff);
+        if (i > 16 || !medium) {
+            t = r;
+            w = fn * fd_pio2_2;
+            r = t - w;
+            w = fn * fd_pio2_2t - ((t - r) - w);
+            y[0] = r - w;
+            i = j - ((fd_hi(y[0]) >> 20) & 0x7ff);
+            if (i > 49 || !medium) {
+                t = r;
+                w = fn * fd_pio2_3;
+                r = t - w;
+                w = fn * fd_pio2_3t - ((t - r) - w);
+                y[0] = r - w;
+            }
+        }
+    }
+    y[1] = (r - y[0]) - w;
+    if (hx < 0) {
+        y[0] = -y[0];
+        y[1] = -y[1];
+        return -n;
+    }
+    return n;
+}
+double orc_sin(double x) {
+    int32_t ix = fd_hi(x) & 0x7fffffff;
+    if (ix <= 0x3fe921fb) return fd_kernel_sin(x, 0.0, 0);
+    if (ix >= 0x7ff00000) return x - x;
+    double y[2];
+    switch (fd_rem_pio2(x, y) & 3) {
+        case 0: return fd_kernel_sin(y[0], y[1], 1);
+        case 1: return fd_kernel_cos(y[0], y[1]);
+        case 2: return -fd_kernel_sin(y[0], y[1], 1);
+        default: return -fd_kernel_cos(y[0], y[1]);
+    }
+}
+double orc_cos(double x) {
+    int32_t ix = fd_hi(x) & 0x7fffffff;
+    if (ix <= 0x3fe921fb) return fd_kernel_cos(x, 0.0);
+    if (ix >= 0x7ff00000) return x - x;
+    double y[2];
+    switch (fd_rem_pio2(x, y) & 3) {
+        case 0: return fd_kernel_cos(y[0], y[1]);
+        case 1: return -fd_kernel_sin(y[0], y[1], 1);
+        case 2: return -fd_kernel_cos(y[0], y[1]);
+        default: return fd_kernel_sin(y[0], y[1], 1);
+    }
+}
+
 /* RigidBody.advancePositions (RigidBody.java:427-441) with expRodrigues
  * (RigidBody.java:382-401) and Matrix3d.normalizeCP */
 static void advance_positions_one(orc_world* w, int32_t i, double dt) {
@@ -1315,7 +1538,7 @@ static void advance_positions_one(orc_world* w, int32_t i, double dt) {
     double d[3] = {om[0], om[1], om[2]};
     normalize3(d);
     if (t > 1e-8) {
-        double c = cos(t), s = sin(t), c1 = 1 - c, dR[9];
+        double c = orc_cos(t), s = orc_sin(t), c1 = 1 - c, dR[9];
         double wX = d[0], wY = d[1], wZ = d[2];
         dR[0] = c + wX * wX * c1;
         dR[3] = wZ * s + wX * wY * c1;
@@ -1392,6 +1615,7 @@ double orc_motion_metric(const orc_world* w, int32_t a, int32_t b) {
 int32_t orc_step(orc_world* w, double dt, int32_t* iters, double* last) {
     orc_clear_bodies(w);
     orc_apply_gravity(w);
+    orc_apply_springs(w);
     orc_update_contacts_map(w);
     orc_detect(w);
     orc_warm_start(w, 0, NULL, NULL);

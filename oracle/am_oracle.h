@@ -70,6 +70,8 @@ void orc_get_deltav(const orc_world* w, double* dv);
 void orc_set_deltav(orc_world* w, const double* dv);
 void orc_clear_bodies(orc_world* w);
 void orc_apply_gravity(orc_world* w);
+void orc_set_springs(orc_world* w, const am_spring* springs, int32_t n);
+void orc_apply_springs(orc_world* w);
 
 /* Detection. */
 void orc_update_contacts_map(orc_world* w);
@@ -102,6 +104,11 @@ int32_t orc_pgs_solve(orc_world* w, const int32_t* order, int32_t n_order, doubl
 
 void orc_advance_velocities(orc_world* w, double dt);
 void orc_advance_positions(orc_world* w, double dt);
+
+/* Math.sin / Math.cos as expRodrigues uses them, by fdlibm 5.3's algorithm
+ * (java.lang.StrictMath's definition). */
+double orc_sin(double x);
+double orc_cos(double x);
 
 /* MotionMetricProcessor.getLargestVelocityNorm; b < 0 -> one-body variant. */
 double orc_motion_metric(const orc_world* w, int32_t a, int32_t b);

@@ -438,6 +438,7 @@ class MergingOracle:
         self.o.push_forces(f, t)
         self.o.push_deltav(dv)
         self.o.apply_gravity()
+        self.o.apply_springs()
 
     def internal_contacts(self):
         rows = [c for p in self.ibpc for c in p["contacts"]]

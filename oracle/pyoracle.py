@@ -11,7 +11,7 @@ import subprocess
 
 import numpy as np
 
-from adaptivemerging_amd._lib import BODY_DTYPE, ContactArrays, Params, default_params_host, ptr  # noqa: F401
+from adaptivemerging_amd._lib import BODY_DTYPE, SPRING_DTYPE, ContactArrays, Params, default_params_host, ptr  # noqa: F401
 from adaptivemerging_amd.context import contact_arrays
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -54,6 +54,8 @@ def load() -> C.CDLL:
         "orc_contacts_set_raw": (None, [vp, i32, C.POINTER(ContactArrays)]),
         "orc_clear_bodies": (None, [vp]),
         "orc_apply_gravity": (None, [vp]),
+        "orc_set_springs": (None, [vp, vp, C.c_int32]),
+        "orc_apply_springs": (None, [vp]),
         "orc_update_contacts_map": (None, [vp]),
         "orc_detect": (i32, [vp]),
         "orc_contacts_set": (None, [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
@@ -71,6 +73,8 @@ def load() -> C.CDLL:
         "orc_advance_positions": (None, [vp, dbl]),
         "orc_motion_metric": (dbl, [vp, i32, i32]),
         "orc_step": (i32, [vp, dbl, C.POINTER(i32), C.POINTER(dbl)]),
+        "orc_sin": (dbl, [dbl]),
+        "orc_cos": (dbl, [dbl]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -92,6 +96,9 @@ class Oracle:
         self.params = params if params is not None else default_params_host()
         self.lib.orc_set_params(self.h, C.byref(self.params))
         self.set_state(x, R, v, w)
+        springs = scene.spring_array() if hasattr(scene, "spring_array") else None
+        if springs is not None and len(springs):
+            self.set_springs(springs)
 
     def __del__(self):
         try:
@@ -181,6 +188,13 @@ class Oracle:
 
     def apply_gravity(self):
         self.lib.orc_apply_gravity(self.h)
+
+    def set_springs(self, springs):
+        self._springs = np.ascontiguousarray(springs, dtype=SPRING_DTYPE)
+        self.lib.orc_set_springs(self.h, ptr(self._springs), len(self._springs))
+
+    def apply_springs(self):
+        self.lib.orc_apply_springs(self.h)
 
     def update_contacts_map(self):
         self.lib.orc_update_contacts_map(self.h)

@@ -44,9 +44,11 @@ def twin(scene, **params):
 
 def oracle_step_like_gpu(ctx, orc, dt):
     """One advanceTime (no merging) on both, the oracle sweeping in the GPU's
-    colour order; returns (iterations gpu, iterations oracle)."""
+    sweep order; external forces are gravity then the scene's springs.
+    Returns (iterations gpu, iterations oracle)."""
     ctx.clear_bodies(); orc.clear_bodies()
     ctx.apply_gravity(); orc.apply_gravity()
+    ctx.apply_spring_forces(); orc.apply_springs()
     ctx.update_contacts_map(); orc.update_contacts_map()
     ctx.detect(dt); orc.detect()
     ctx.warm_start(); orc.warm_start()

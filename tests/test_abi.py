@@ -55,6 +55,7 @@ int main(void) {
   printf("sizeof am_contact_arrays %zu\n", sizeof(am_contact_arrays));
   printf("sizeof am_step_stats %zu\n", sizeof(am_step_stats));
   printf("sizeof am_body_pair %zu\n", sizeof(am_body_pair));
+  printf("sizeof am_spring %zu\n", sizeof(am_spring));
   F(am_body_desc, size) F(am_body_desc, radius) F(am_body_desc, jinv0) F(am_body_desc, plane_d) F(am_body_desc, bbox)
   F(am_body_desc, inertia0)
   F(am_body_pair, flag) F(am_body_pair, collection) F(am_body_pair, hist) F(am_body_pair, states)
@@ -63,6 +64,9 @@ int main(void) {
   F(am_params, threshold_merge) F(am_params, threshold_breath) F(am_params, pgs_block) F(am_params, pgs_grid)
   F(am_step_stats, last_change) F(am_step_stats, ms_total) F(am_step_stats, ms_setup_kernel)
   F(am_step_stats, used_learned) F(am_step_stats, n_internal_contacts) F(am_step_stats, ms_merging)
+  F(am_spring, type) F(am_spring, body2) F(am_spring, pb1) F(am_spring, pb2) F(am_spring, pw) F(am_spring, l0)
+  F(am_spring, k) F(am_spring, d) F(am_spring, ls)
+  F(am_params, spring_stiffness_mod) F(am_params, spring_damping_mod) F(am_params, pgs_schedule)
   return 0;
 }
 """
@@ -81,6 +85,7 @@ def test_struct_layouts_match_ctypes(tmp_path):
     assert sizes["am_contact_arrays"] == C.sizeof(_lib.ContactArrays)
     assert sizes["am_step_stats"] == C.sizeof(_lib.StepStats)
     assert sizes["am_body_pair"] == C.sizeof(_lib.BodyPair)
+    assert sizes["am_spring"] == _lib.SPRING_DTYPE.itemsize
     for ln in lines:
         if ln.startswith("sizeof"):
             continue
@@ -95,6 +100,8 @@ def test_struct_layouts_match_ctypes(tmp_path):
             assert getattr(_lib.StepStats, field).offset == int(off), field
         elif struct == "am_body_pair":
             assert getattr(_lib.BodyPair, field).offset == int(off), field
+        elif struct == "am_spring":
+            assert _lib.SPRING_DTYPE.fields[field][1] == int(off), field
 
 
 def test_params_defaults_are_reference_defaults(libam):
@@ -106,6 +113,8 @@ def test_params_defaults_are_reference_defaults(libam):
     assert p.iterations == 30 and p.tolerance == 1e-5 and p.compliance == 1e-3 and p.feedback_stiffness == 0.5
     assert p.gravity_amount == 1.0 and p.gravity_angle_deg == 90.0 and p.sliding_threshold == 0.01
     assert p.step_accum_merging == 3 and p.steps_between_merge_events == 10 and p.threshold_unmerge == 2e-2
+    # RigidBodySystem.springStiffnessMod / springDampingMod (RigidBodySystem.java:553-554)
+    assert p.spring_stiffness_mod == 1.0 and p.spring_damping_mod == 1.0
 
 
 def test_error_codes_without_gpu(libam):
@@ -114,6 +123,8 @@ def test_error_codes_without_gpu(libam):
     assert libam.am_params_default(None) == _lib.AM_ERR_INVALID
     assert libam.am_detect(None, 0.01, None) == _lib.AM_ERR_INVALID
     assert libam.am_step(None, 0.01, None) == _lib.AM_ERR_INVALID
+    assert libam.am_springs_set(None, None, 0) == _lib.AM_ERR_INVALID
+    assert libam.am_apply_spring_forces(None) == _lib.AM_ERR_INVALID
     h = C.c_void_p()
     rc = libam.am_ctx_create(-1, C.byref(h))
     assert rc != 0 and not h.value

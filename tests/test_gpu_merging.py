@@ -35,6 +35,7 @@ def step_both(ctx, mo, dt, step, kick_body=None):
     mo.total_steps += 1
     ctx.clear_bodies(); mo.clear_bodies()
     ctx.apply_gravity(); mo.o.apply_gravity()
+    ctx.apply_spring_forces(); mo.o.apply_springs()
     if kick_body is not None:
         kick(ctx, kick_body, 2000.0)
         kick(mo.o, kick_body, 2000.0)
@@ -116,7 +117,8 @@ def test_step_equals_phases(gpu_lib):
     dt = 0.01
     for step in range(1, 26):
         st = a.step(dt)
-        b.clear_bodies(); b.apply_gravity(); b.update_contacts_map(); b.detect(dt); b.warm_start()
+        b.clear_bodies(); b.apply_gravity(); b.apply_spring_forces()
+        b.update_contacts_map(); b.detect(dt); b.warm_start()
         b.update_in_collections(dt); b.accumulate_for_unmerging(); b.unmerge(dt)
         if b.merging_event:
             b.clear_top_level()

@@ -1,9 +1,9 @@
 """libam (HIP, gfx950) against the CPU oracle on identical inputs.
 
 Bar: integer, index and floating-point contact fields bit-exact; lambdas and
-deltaV bit-exact against the oracle's sequential sweep in the GPU's colour
-order (coloured Gauss-Seidel equals that sweep); integration within 1e-12
-(sin/cos of the Rodrigues update come from different math libraries).
+deltaV bit-exact against the oracle's sequential sweep in the GPU's sweep
+order; integration bit-exact (both sides take the Rodrigues update's sine and
+cosine from fdlibm's algorithm, not from their math libraries).
 """
 import numpy as np
 import pytest
@@ -108,12 +108,29 @@ def test_steps_jenga(gpu_lib):
     for step in range(5):
         it_g, it_o = oracle_step_like_gpu(ctx, orc, dt)
         assert it_g == it_o
-        xg, Rg, vg, wg = ctx.pull_state()
-        xo, Ro, vo, wo = orc.pull_state()
-        np.testing.assert_allclose(xg, xo, rtol=0, atol=1e-12)
-        np.testing.assert_allclose(Rg, Ro, rtol=0, atol=1e-12)
-        np.testing.assert_allclose(vg, vo, rtol=0, atol=1e-10)
-        np.testing.assert_allclose(wg, wo, rtol=0, atol=1e-10)
+        for name, a, b in zip("xRvw", ctx.pull_state(), orc.pull_state()):
+            assert_bits_equal(f"step {step} {name}", a, b)
+
+
+def test_rotation_update_over_angles(gpu_lib):
+    """advancePositions' Rodrigues update for rotation angles |w| dt from
+    1e-9 to 60 rad, and next to multiples of pi/2, bit for bit."""
+    scene = workloads.box_cluster(seed=9, n=512, spheres=64)
+    ctx, orc = twin(scene)
+    rng = np.random.default_rng(3)
+    n = scene.n
+    dt = 0.01
+    ang = np.exp(rng.uniform(np.log(1e-9), np.log(60.0), n))
+    ang[:96] = [k * np.pi / 2 * (1 + e) for k in range(1, 33) for e in (0.0, 1e-16, -3e-12)]
+    axis = rng.normal(size=(n, 3))
+    axis /= np.linalg.norm(axis, axis=1, keepdims=True)
+    w = axis * (ang / dt)[:, None]
+    v = rng.normal(size=(n, 3))
+    ctx.push_state(v=v, w=w); orc.set_state(v=v, w=w)
+    for rep in range(3):
+        ctx.advance_positions(dt); orc.advance_positions(dt)
+        for name, a, b in zip("xR", ctx.pull_state()[:2], orc.pull_state()[:2]):
+            assert_bits_equal(f"update {rep} {name}", a, b)
 
 
 def test_motion_metric(gpu_lib):

@@ -36,3 +36,34 @@ def test_system_steps_match_oracle_and_tower_stands(gpu_lib):
     # centimetres (penetration under Baumgarte feedback) but does not topple
     assert abs(st["x"][-1][1] - scene.x[-1][1]) < 0.05
     assert np.hypot(st["x"][-1][0] - scene.x[-1][0], st["x"][-1][2] - scene.x[-1][2]) < 0.01
+
+
+def test_reset_restores_the_initial_scene(gpu_lib):
+    """RigidBodySystem.reset (RigidBodySystem.java:389-425): collections
+    dissolve, bodies return to their initial state, springs reset, and the
+    run after a reset repeats the first one bit for bit."""
+    scene = workloads.merge_stack(seed=6)
+    scene.add_spring(scene.n - 2, [0.3, 0.1, 0.0], k=30.0, d=2.0)
+    system = RigidBodySystem(scene)
+    system.gravityAmount = 9.81
+    system.collision.iterations = 60
+
+    def run():
+        out = []
+        for _ in range(25):
+            system.advanceTime(0.01)
+            out.append(system.ctx.pull_state())
+        return out, system.ctx.structure()
+
+    first, (parent1, top1) = run()
+    assert any(p >= 0 for p in parent1[:scene.n]), "the boards should have merged"
+    system.reset()
+    parent, top = system.ctx.structure()
+    assert list(top) == list(range(scene.n)) and all(p < 0 for p in parent[:scene.n])
+    x, R, v, w = system.ctx.pull_state()
+    np.testing.assert_array_equal(x, np.array(scene.x))
+    second, (parent2, top2) = run()
+    assert list(parent1) == list(parent2) and list(top1) == list(top2)
+    for k, (a, b) in enumerate(zip(first, second)):
+        for name, p, q in zip("xRvw", a, b):
+            np.testing.assert_array_equal(p, q, err_msg=f"step {k} {name}")

@@ -73,7 +73,6 @@ def test_collision_settings_and_overrides():
 @pytest.mark.parametrize("body,msg", [
     ('<body type="mesh" name="m" obj="a.obj" st="a.sph" scale="1"/>', "mesh"),
     ('<body type="composite" name="c"/>', "composite"),
-    ('<body type="box" name="b" dim="1 1 1"><spring pB="0 0 0" pW="0 1 0"/></body>', "springs"),
     ('<body type="box" name="b" dim="1 1 1"><spinner>0 1 0</spinner></body>', "spinners"),
     ('<body type="cone" name="b"/>', "unknown type"),
 ])
@@ -85,7 +84,29 @@ def test_unsupported_features_are_refused(body, msg):
 def test_springs_can_be_dropped_explicitly():
     scene, _ = xml_scene.parse_scene('<root><body type="box" name="b" dim="1 1 1"><spring pB="0 0 0" pW="0 1 0"/>'
                                      '</body></root>', ignore_springs=True)
-    assert scene.n == 1
+    assert scene.n == 1 and len(scene.springs) == 0
+
+
+def test_spring_geometry_follows_the_constructors():
+    """Spring.java:50-104: a world spring's rest length is the initial
+    distance; a zero spring anchors at the point's initial world position with
+    l0 = 0.5; a spring read before <x> sees the pose the body had then."""
+    text = """<root>
+      <body type="box" name="a" dim="1 1 1"><x>0 2 0</x><spring pB="0 0.5 0" pW="0 5 0" k="200" ls="0.5"/></body>
+      <body type="box" name="b" dim="1 1 1"><spring pB="0.5 0 0"/><x>3 0 0</x><R>0 0 1 1.5707963267948966</R></body>
+      <body type="box" name="c" dim="1 1 1"><x>0 0 3</x><spring pB="0 0 0" pB2="0 0 0" body2="a" d="1"/></body>
+      <body type="box" name="e" dim="1 1 1"><spring pB="0 0 0" pB2="0 0 0" body2="nobody"/></body>
+    </root>"""
+    with pytest.warns(UserWarning, match="unknown body"):
+        scene, _ = xml_scene.parse_scene(text)
+    sp = scene.spring_array()
+    assert len(sp) == 3
+    a, b, c = sp
+    assert (a["type"], a["body1"], a["k"], a["d"], a["ls"]) == (1, 0, 200.0, 10.0, 0.5)
+    assert a["l0"] == 2.5 and a["pw"].tolist() == [0.0, 5.0, 0.0]
+    assert b["type"] == 0 and b["l0"] == 0.5 and b["pw"].tolist() == [0.5, 0.0, 0.0]   # identity pose at the tag
+    assert (c["type"], c["body1"], c["body2"], c["d"]) == (2, 2, 0, 1.0)
+    assert c["l0"] == math.sqrt(2 * 2 + 3 * 3)
 
 
 REF_SCENES = "/root/reference/scenes3D"
@@ -93,15 +114,18 @@ REF_SCENES = "/root/reference/scenes3D"
 
 @pytest.mark.skipif(not os.path.isdir(REF_SCENES), reason="the reference's scene folder is not present")
 def test_reference_scene_files_load():
-    loaded = 0
+    loaded = with_springs = 0
     for path in sorted(glob.glob(os.path.join(REF_SCENES, "*.xml"))):
         with open(path) as f:
             text = f.read()
-        if any(f'type="{t}"' in text for t in ("mesh", "composite")) or "<spring" in text or "<spinner" in text:
+        if any(f'type="{t}"' in text for t in ("mesh", "composite")) or "<spinner" in text:
             continue
         scene, coll = xml_scene.parse_scene(path)
         b, _, x, R, _, _ = scene.arrays()
         assert scene.n > 0 and np.all(np.isfinite(x)) and np.all(np.isfinite(R))
         assert np.allclose(np.einsum("nij,nkj->nik", R.reshape(-1, 3, 3), R.reshape(-1, 3, 3)), np.eye(3), atol=1e-12)
+        assert len(scene.springs) == text.count("<spring"), os.path.basename(path)
+        if scene.springs:
+            with_springs += 1
         loaded += 1
-    assert loaded >= 5
+    assert loaded >= 12 and with_springs >= 5
