@@ -28,6 +28,7 @@ AM_GEOM_COLLECTION = 3
 AM_BODY_PINNED = 1
 AM_BODY_MAGNET_ACTIVE = 2
 AM_BODY_SLEEPING = 4
+AM_SLEEP_HIST_STRIDE = 201
 
 AM_STATE_BROKEN = 0
 AM_STATE_ONEDGE = 1
@@ -67,6 +68,7 @@ class Params(C.Structure):
         ("gravity_amount", C.c_double), ("gravity_angle_deg", C.c_double),
         ("viscous_linear_decay", C.c_double), ("viscous_angular_decay", C.c_double),
         ("spring_stiffness_mod", C.c_double), ("spring_damping_mod", C.c_double),
+        ("enable_sleeping", C.c_int32), ("sleep_step_accum", C.c_int32), ("sleep_threshold", C.c_double),
         ("enable_merging", C.c_int32), ("enable_merge_pinned", C.c_int32),
         ("enable_merge_stable_contact", C.c_int32), ("enable_merge_let_it_breathe", C.c_int32),
         ("enable_unmerging", C.c_int32), ("enable_unmerge_friction", C.c_int32),
@@ -104,7 +106,7 @@ class StepStats(C.Structure):
         ("ms_pgs_kernel", C.c_double), ("ms_setup_kernel", C.c_double), ("ms_learned", C.c_double),
         ("start_residual_plain", C.c_double), ("start_residual_learned", C.c_double),
         ("used_learned", C.c_int32), ("n_collections", C.c_int32), ("n_merged", C.c_int32),
-        ("n_unmerged", C.c_int32), ("n_internal_contacts", C.c_int32), ("reserved2", C.c_int32),
+        ("n_unmerged", C.c_int32), ("n_internal_contacts", C.c_int32), ("n_sleeping", C.c_int32),
         ("ms_merging", C.c_double),
     ]
 
@@ -126,7 +128,7 @@ SYMBOLS = [
     "am_version", "am_ctx_create", "am_ctx_destroy", "am_last_error", "am_params_default", "am_params_set",
     "am_scene_upload", "am_structure_pull", "am_entities_pull", "am_jinv_pull", "am_state_push",
     "am_state_pull", "am_forces_push", "am_forces_pull", "am_clear_bodies", "am_apply_gravity", "am_deltav_pull",
-    "am_springs_set", "am_apply_spring_forces",
+    "am_springs_set", "am_apply_spring_forces", "am_wake", "am_sleep", "am_wake_all", "am_sleep_state_pull",
     "am_update_contacts_map", "am_detect", "am_contacts_push", "am_contacts_pull", "am_lambdas_push",
     "am_warm_start", "am_redo_warm_start", "am_no_warm_start", "am_solve", "am_solve_order_pull",
     "am_colours_pull", "am_update_in_collections", "am_changes_pull", "am_advance_velocities",
@@ -179,6 +181,10 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "am_apply_gravity": (C.c_int, [ctxp]),
         "am_springs_set": (C.c_int, [ctxp, vp, i32]),
         "am_apply_spring_forces": (C.c_int, [ctxp]),
+        "am_wake": (C.c_int, [ctxp, vp]),
+        "am_sleep": (C.c_int, [ctxp, vp]),
+        "am_wake_all": (C.c_int, [ctxp]),
+        "am_sleep_state_pull": (C.c_int, [ctxp, vp, vp, vp, i32]),
         "am_deltav_pull": (C.c_int, [ctxp, vp, i32]),
         "am_update_contacts_map": (C.c_int, [ctxp]),
         "am_detect": (C.c_int, [ctxp, dbl, C.POINTER(i32)]),
@@ -252,7 +258,7 @@ _DEFAULTS = dict(
     enable_post_stabilization=0, enable_compliance=1, compliance=1e-3, restitution_override=0,
     friction_override=0, restitution=0.5, friction=0.1, sliding_threshold=0.01, warm_start=1, use_gravity=1,
     gravity_amount=1.0, gravity_angle_deg=90.0, viscous_linear_decay=1.0, viscous_angular_decay=1.0,
-    spring_stiffness_mod=1.0, spring_damping_mod=1.0,
+    spring_stiffness_mod=1.0, spring_damping_mod=1.0, enable_sleeping=1, sleep_step_accum=10, sleep_threshold=1e-5,
     enable_merging=1, enable_merge_pinned=1, enable_merge_stable_contact=1, enable_merge_let_it_breathe=1,
     enable_unmerging=1, enable_unmerge_friction=1, enable_unmerge_normal=1, enable_unmerge_relative_motion=1,
     update_contacts_in_collections=1, step_accum_merging=3, step_accum_unmerging=3, steps_between_merge_events=10,

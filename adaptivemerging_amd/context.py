@@ -65,6 +65,30 @@ class Context:
     def apply_spring_forces(self):
         self._check(self.lib.am_apply_spring_forces(self.h))
 
+    # -- sleeping (Sleeping.java) ------------------------------------------------
+    def wake(self) -> int:
+        n = C.c_int32()
+        self._check(self.lib.am_wake(self.h, C.byref(n)))
+        return n.value
+
+    def sleep(self) -> int:
+        n = C.c_int32()
+        self._check(self.lib.am_sleep(self.h, C.byref(n)))
+        return n.value
+
+    def wake_all(self):
+        self._check(self.lib.am_wake_all(self.h))
+
+    def sleep_state(self, n: int | None = None):
+        """(sleeping flags, history lengths, histories (n, AM_SLEEP_HIST_STRIDE))
+        of the first n entities (default: all)."""
+        n = self.n_entities() if n is None else n
+        flags = np.zeros(n, np.uint8)
+        hn = np.zeros(n, np.int32)
+        hist = np.zeros((n, _lib.AM_SLEEP_HIST_STRIDE))
+        self._check(self.lib.am_sleep_state_pull(self.h, ptr(flags), ptr(hn), ptr(hist), n))
+        return flags, hn, hist
+
     def structure(self):
         """(parent per entity, top-level list) of the merged-body structure."""
         ne, nt = C.c_int32(), C.c_int32()

@@ -176,6 +176,9 @@ void default_params(am_params* p) {
     p->viscous_angular_decay = 1.0;
     p->spring_stiffness_mod = 1.0;
     p->spring_damping_mod = 1.0;
+    p->enable_sleeping = 1;
+    p->sleep_step_accum = 10;
+    p->sleep_threshold = 1e-5;
     p->enable_merging = 1;
     p->enable_merge_pinned = 1;
     p->enable_merge_stable_contact = 1;
@@ -236,15 +239,17 @@ int rebuild_structure(am_ctx* c) {
     AM_HIP(c, upload(c, c->d_gravity, c->gravity_targets));
     AM_HIP(c, upload(c, c->d_integrate, c->integrate_targets));
     AM_HIP(c, upload(c, c->d_flags, c->flags));
-    // collection members: all (push-down at clear) and those the single sweep integrates
+    // collection members: all (push-down at clear, RigidCollection.clearBodies
+    // visits every collection) and those the single sweep integrates (members
+    // of awake collections that are not pinned, CollisionProcessor.java:288-296)
     c->member_list.clear();
     c->member_adv.clear();
     for (int e : c->slot_entity)
-        if (c->desc[e].geom == AM_GEOM_COLLECTION && !(c->flags[e] & AM_BODY_SLEEPING))
+        if (c->desc[e].geom == AM_GEOM_COLLECTION)
             for (int q = 0; q < c->member_count[e]; q++) {
                 int b = c->members[c->member_first[e] + q];
                 c->member_list.push_back(b);
-                if (!(c->flags[b] & AM_BODY_PINNED)) c->member_adv.push_back(b);
+                if (!(c->flags[e] & AM_BODY_SLEEPING) && !(c->flags[b] & AM_BODY_PINNED)) c->member_adv.push_back(b);
             }
     AM_HIP(c, upload(c, c->d_member_list, c->member_list));
     AM_HIP(c, upload(c, c->d_member_adv, c->member_adv));
@@ -343,6 +348,8 @@ int am_params_set(am_ctx* c, const am_params* p) {
         return fail(c, AM_ERR_INVALID, "pgs_block must be 0 or a multiple of 64 up to 256");
     if (p->pgs_grid < 0) return fail(c, AM_ERR_INVALID, "pgs_grid must be >= 0");
     if (p->pgs_schedule != 0 && p->pgs_schedule != 1) return fail(c, AM_ERR_INVALID, "pgs_schedule must be 0 or 1");
+    if (p->sleep_step_accum < 0 || p->sleep_step_accum > am::kMaxSleepAccum)
+        return fail(c, AM_ERR_INVALID, "sleep_step_accum must be in 0..200");
     if (p->pgs_block != c->params.pgs_block) c->pgs_max_grid = 0;
     c->params = *p;
     return AM_OK;
@@ -900,7 +907,11 @@ int am_scene_upload(am_ctx* c, const am_body_desc* bodies, int32_t n_bodies, con
         // merging state starts empty
         c->total_steps = 0;
         c->n_springs = 0;
+        c->springs_host.clear();
         c->merging_event = false;
+        // every body starts with an empty metric history
+        c->sleep_cap = 0;
+        c->sleep_fresh.clear();
         c->free_entities.clear();
         c->b2c.assign((size_t)n_bodies * 12, 0.0);
         c->n_bpc = c->n_bpc_prev = 0;
@@ -1101,6 +1112,7 @@ int am_springs_set(am_ctx* c, const am_spring* springs, int32_t n) {
         AM_HIP(c, hipMemcpyAsync(c->d_springs.p, springs, (size_t)n * sizeof(am_spring), hipMemcpyHostToDevice,
                                  c->stream));
     AM_HIP(c, hipStreamSynchronize(c->stream));
+    c->springs_host.assign(springs, springs + n);
     c->n_springs = n;
     return AM_OK;
 }
@@ -1109,6 +1121,45 @@ int am_apply_spring_forces(am_ctx* c) {
     int r = check_ctx(c);
     if (r) return r;
     return apply_springs(c);
+}
+
+int am_wake(am_ctx* c, int32_t* n_woken) {
+    int r = check_ctx(c);
+    if (r) return r;
+    int n = 0;
+    r = sleep_wake(c, &n);
+    if (n_woken) *n_woken = n;
+    return r;
+}
+
+int am_sleep(am_ctx* c, int32_t* n_asleep) {
+    int r = check_ctx(c);
+    if (r) return r;
+    int n = 0;
+    r = sleep_sleep(c, &n);
+    if (n_asleep) *n_asleep = n;
+    return r;
+}
+
+int am_wake_all(am_ctx* c) {
+    int r = check_ctx(c);
+    if (r) return r;
+    return sleep_wake_all(c);
+}
+
+int am_sleep_state_pull(am_ctx* c, uint8_t* sleeping, int32_t* hist_n, double* hist, int32_t n) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (n < 0 || n > c->n_ent) return fail(c, AM_ERR_INVALID, "entity count out of range");
+    if (sleeping)
+        for (int e = 0; e < n; e++) sleeping[e] = (c->flags[e] & AM_BODY_SLEEPING) ? 1 : 0;
+    if (!hist_n && !hist) return AM_OK;
+    if ((r = ensure_sleep_state(c))) return r;
+    if (hist_n) r = pull_i(c, hist_n, c->d_sleep_n.p, n);
+    if (!r && hist) r = pull_d(c, hist, c->d_sleep_hist.p, (size_t)n * am::kSleepHistStride);
+    if (r) return r;
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    return AM_OK;
 }
 
 int am_apply_gravity(am_ctx* c) {
@@ -1531,6 +1582,8 @@ int am_step(am_ctx* c, double dt, am_step_stats* st) {
     AM_HIP(c, hipEventRecord(ev[1], s));
     int bad = 0, rep = 0;
     if ((r = am_warm_start(c, 0, &bad, &rep))) return r;
+    // Sleeping.wake (:128)
+    if ((r = sleep_wake(c, nullptr))) return r;
     AM_HIP(c, hipEventRecord(ev[5], s));
     int sweep_it = 0, cut = 0, merged = 0;
     if (merging) {
@@ -1559,6 +1612,9 @@ int am_step(am_ctx* c, double dt, am_step_stats* st) {
     AM_HIP(c, hipEventRecord(ev[8], s));
     if (P.enable_merging && P.steps_between_merge_events > 0 && c->total_steps % P.steps_between_merge_events == 0)
         if ((r = merge_merge(c, &merged))) return r;
+    // Sleeping.sleep (:170)
+    int asleep = 0;
+    if ((r = sleep_sleep(c, &asleep))) return r;
     AM_HIP(c, hipEventRecord(ev[4], s));
     AM_HIP(c, hipEventSynchronize(ev[4]));
     c->last_merged = merged;
@@ -1597,6 +1653,7 @@ int am_step(am_ctx* c, double dt, am_step_stats* st) {
         st->n_merged = merged;
         st->n_unmerged = cut;
         st->n_internal_contacts = c->n_ic;
+        st->n_sleeping = asleep;
     }
     return AM_OK;
 }

@@ -146,7 +146,13 @@ struct am_ctx {
     int flow_max_grid = 0;
     // springs (RigidBodySystem.springs)
     DBuf<am_spring> d_springs;
+    std::vector<am_spring> springs_host;
     int n_springs = 0;
+    // sleeping (Sleeping.java): per-entity metric histories
+    DBuf<double> d_sleep_hist;          // kSleepHistStride per entity, oldest first
+    DBuf<int> d_sleep_n, d_sleep_ext;   // history length; active external contact this step
+    int sleep_cap = 0;                  // entities the sleep buffers hold
+    std::vector<int> sleep_fresh;       // new collection entities whose history must start empty
 
     // scalars on device: 0 n_pairs, 1 n_contacts, 2 n_groups, 3 total scratch, 4 overflow, 5 remaining,
     // 6 iters_used, 7 bad, 8 repaired, 9 n_sweep
@@ -242,6 +248,11 @@ int merge_push_down(am_ctx* c, bool integrated_only);
 int merge_member_transforms(am_ctx* c);
 int pull_contacts(am_ctx* c, const am::ContactsView& V, int count, am_contact_arrays* o, int32_t capacity);
 bool has_collections(const am_ctx* c);
+// sleeping phases (am_merge.hip)
+int ensure_sleep_state(am_ctx* c);
+int sleep_wake(am_ctx* c, int* n_woken);
+int sleep_sleep(am_ctx* c, int* n_asleep);
+int sleep_wake_all(am_ctx* c);
 
 }  // namespace am_rt
 

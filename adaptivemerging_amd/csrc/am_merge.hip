@@ -299,8 +299,8 @@ __global__ void k_gather_contacts(ContactsView dst, int dst_off, ContactsView sr
                                   int mode) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    size_t s = (size_t)(idx ? idx[k] : src_off + k);
-    size_t d = (size_t)(dst_off + k);
+    size_t s = (size_t)(mode == 3 ? src_off + k : (idx ? idx[k] : src_off + k));
+    size_t d = (size_t)(mode == 3 ? idx[k] : dst_off + k);
     dst.b1[d] = src.b1[s];
     dst.b2[d] = src.b2[s];
     dst.bv1[d] = src.bv1[s];
@@ -323,6 +323,83 @@ __global__ void k_gather_contacts(ContactsView dst, int dst_off, ContactsView sr
     }
     for (int a = 0; a < 36; a++) dst.J[d * 36 + a] = src.J[s * 36 + a];
 }
+// ---------------------------------------------------------------------------
+// Sleeping (Sleeping.java).
+
+// Sleeping.wake's test (:112-121) per contact of this step: both bodies not
+// pinned and one of their top-level bodies asleep.
+__global__ void k_wake_count(BodiesView B, const int* b1, const int* b2, int n, int* count) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int a = b1[k], b = b2[k];
+    if ((B.flags[a] | B.flags[b]) & AM_BODY_PINNED) return;
+    if ((B.flags[top_of(B, a)] | B.flags[top_of(B, b)]) & AM_BODY_SLEEPING) atomicAdd(count, 1);
+}
+void launch_wake_count(hipStream_t s, BodiesView B, const int* b1, const int* b2, int n, int* count) {
+    if (n > 0) hipLaunchKernelGGL(k_wake_count, dim3((n + 255) / 256), dim3(256), 0, s, B, b1, b2, n, count);
+}
+
+// Sleeping.sleep's external-contact test (:64-72): the body pairs left by
+// clearBodyPairContacts (a contact with |lambda0| > 1e-14) that are not inside
+// a collection and join two bodies that are not pinned mark both top-level
+// bodies.
+__global__ void k_sleep_external(BodiesView B, const int* b1, const int* b2, const double* lam, int n, int* ext) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    if (!(fabs(lam[(size_t)k * 3]) > 1e-14)) return;
+    int a = b1[k], b = b2[k];
+    if ((B.flags[a] | B.flags[b]) & AM_BODY_PINNED) return;
+    if (B.parent[a] >= 0 && B.parent[a] == B.parent[b]) return;
+    ext[top_of(B, a)] = 1;
+    ext[top_of(B, b)] = 1;
+}
+void launch_sleep_external(hipStream_t s, BodiesView B, const int* b1, const int* b2, const double* lam, int n,
+                           int* ext) {
+    if (n > 0) hipLaunchKernelGGL(k_sleep_external, dim3((n + 255) / 256), dim3(256), 0, s, B, b1, b2, lam, n, ext);
+}
+
+// Sleeping.sleep (:49-98) and accumulate (:155-160) per top-level body:
+// append the one-body motion metric, drop the oldest value once the history
+// is longer than stepAccum, then sleep if the history is full, every value is
+// at most the threshold and none exceeds its predecessor by more than 5e-5.
+// counts[0] += bodies that fell asleep, counts[1] += bodies asleep afterwards.
+__global__ void k_sleep_decide(BodiesView B, int* flags, const int* slot_entity, int n_slots, const int* ext,
+                               double* hist, int* hist_n, int accum, double threshold, int* counts) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_slots) return;
+    int e = slot_entity[s];
+    if (flags[e] & AM_BODY_SLEEPING) {
+        atomicAdd(&counts[1], 1);
+        return;
+    }
+    if (ext[e]) return;
+    double* h = hist + (size_t)e * kSleepHistStride;
+    int n = hist_n[e];
+    h[n++] = metric_one(B, e);
+    if (n > accum) {
+        for (int i = 1; i < n; i++) h[i - 1] = h[i];
+        n--;
+    }
+    hist_n[e] = n;
+    bool sleep = n >= accum;
+    double prev = 1.7976931348623157e308;
+    for (int i = 0; sleep && i < n; i++) {
+        if (h[i] > prev + 5e-5 || h[i] > threshold) sleep = false;
+        prev = h[i];
+    }
+    if (sleep) {
+        flags[e] |= AM_BODY_SLEEPING;
+        atomicAdd(&counts[0], 1);
+        atomicAdd(&counts[1], 1);
+    }
+}
+void launch_sleep_decide(hipStream_t s, BodiesView B, int* flags, const int* slot_entity, int n_slots, const int* ext,
+                         double* hist, int* hist_n, int accum, double threshold, int* counts) {
+    if (n_slots > 0)
+        hipLaunchKernelGGL(k_sleep_decide, dim3((n_slots + 127) / 128), dim3(128), 0, s, B, flags, slot_entity,
+                           n_slots, ext, hist, hist_n, accum, threshold, counts);
+}
+
 void launch_gather_contacts(hipStream_t s, ContactsView dst, int dst_off, ContactsView src, const int* idx,
                             int src_off, int n, int mode) {
     if (n > 0)
@@ -473,6 +550,7 @@ struct Ent {
     M3 R, I, I0, jinv, jinv0;  // massAngular, massAngular0, jinv, jinv0
     double mass, minv;
     bool pinned, plane;
+    bool sleeping;  // RigidBody.sleeping (a collection's own flag)
     double bb[24];
 };
 
@@ -494,6 +572,7 @@ struct Event {
         E.jinv0 = m3(d.jinv0);
         E.I0 = m3(d.inertia0);
         E.pinned = (c->flags[e] & AM_BODY_PINNED) != 0;
+        E.sleeping = (c->flags[e] & AM_BODY_SLEEPING) != 0;
         E.plane = d.geom == AM_GEOM_PLANE;
         // massAngular as RigidBody.updateRotationalInertiaFromTransformation left it
         E.I = E.pinned ? zero3() : rm0rt(E.R, E.I0);
@@ -514,7 +593,7 @@ struct Event {
         am_body_desc& d = c->desc[e];
         std::memset(&d, 0, sizeof d);
         d.geom = AM_GEOM_COLLECTION;
-        d.flags = E.pinned ? AM_BODY_PINNED : 0;
+        d.flags = (E.pinned ? AM_BODY_PINNED : 0) | (E.sleeping ? AM_BODY_SLEEPING : 0);
         d.mass = E.mass;
         d.minv = E.minv;
         for (int i = 0; i < 9; i++) {
@@ -667,6 +746,7 @@ int alloc_entity(Event& E) {
     int e = c->free_entities.back();
     c->free_entities.pop_back();
     E.members[e].clear();
+    c->sleep_fresh.push_back(e);  // a new RigidCollection has an empty metric history
     return e;
 }
 
@@ -682,13 +762,22 @@ void free_entity(Event& E, int e) {
     c->free_entities.push_back(e);
 }
 
+// RigidCollection.updateCollectionState (:459-470): the collection is pinned
+// if the body is and asleep if the body was; the body's own flag is cleared.
+void collection_state(am_ctx* c, Ent& C, int b, const Ent& B) {
+    C.pinned = C.pinned || B.pinned;
+    C.sleeping = C.sleeping || B.sleeping;
+    c->flags[b] &= ~AM_BODY_SLEEPING;
+    c->desc[b].flags &= ~AM_BODY_SLEEPING;
+}
+
 // RigidCollection.addBody (:136-147).
 void add_body(Event& E, int ce, Ent& C, int b) {
     am_ctx* c = E.c;
     c->parent[b] = ce;
     E.members[ce].push_back(b);
     Ent B = E.load(b);
-    C.pinned = C.pinned || B.pinned;  // updateCollectionState (:459-470)
+    collection_state(c, C, b, B);
     add_body_internal(C, B);
     finish_inertia(C);
     update_bodies_transformations(E, ce, C);
@@ -701,7 +790,7 @@ void add_bodies(Event& E, int ce, Ent& C, const std::vector<int>& list) {
         c->parent[b] = ce;
         E.members[ce].push_back(b);
         Ent B = E.load(b);
-        C.pinned = C.pinned || B.pinned;
+        collection_state(c, C, b, B);
         add_body_internal(C, B);
     }
     finish_inertia(C);
@@ -717,7 +806,7 @@ void add_collection(Event& E, int ce, Ent& C, int other) {
         E.members[ce].push_back(b);
     }
     Ent O = E.load(other);
-    C.pinned = C.pinned || O.pinned;
+    collection_state(c, C, other, O);
     add_body_internal(C, O);
     finish_inertia(C);
     update_bodies_transformations(E, ce, C);
@@ -1008,10 +1097,20 @@ int merge_update_in_collections(am_ctx* c, double dt, int* iters) {
     const am_params& P = c->params;
     if (!P.update_contacts_in_collections || !has_collections(c)) return AM_OK;
     hipStream_t s = c->stream;
-    const int n_ext = c->n_contacts, n_int = c->n_ic;
+    // internal contacts of the collections that are awake, pair by pair
+    std::vector<int> awake_idx;
+    bool all_awake = true;
+    for (const auto& ib : c->ibpc) all_awake = all_awake && !(c->flags[ib.coll] & AM_BODY_SLEEPING);
+    if (!all_awake)
+        for (const auto& ib : c->ibpc)
+            if (!(c->flags[ib.coll] & AM_BODY_SLEEPING))
+                for (int k = 0; k < ib.count; k++) awake_idx.push_back(ib.first + k);
+    const int n_ext = c->n_contacts, n_int = all_awake ? c->n_ic : (int)awake_idx.size();
     int r = ensure_contact_capacity(c, n_ext + n_int, n_ext);
     if (r) return r;
-    launch_gather_contacts(s, contacts_view(c), n_ext, internal_view(c), nullptr, 0, n_int, 0);
+    DBuf<int> d_awake;
+    if (!all_awake) AM_HIP(c, upload(c, d_awake, awake_idx));
+    launch_gather_contacts(s, contacts_view(c), n_ext, internal_view(c), all_awake ? nullptr : d_awake.p, 0, n_int, 0);
     c->n_contacts = n_ext + n_int;
     fill_int(s, c->d_scalars.p + 1, 1, c->n_contacts);
     if (c->n_contacts > 0) {
@@ -1022,7 +1121,9 @@ int merge_update_in_collections(am_ctx* c, double dt, int* iters) {
         c->n_groups = c->h_scalars[2];
         c->layout_valid = false;
         if ((r = do_solve(c, dt, 0, false, iters, nullptr))) return r;
-        launch_gather_contacts(s, internal_view(c), 0, contacts_view(c), nullptr, n_ext, n_int, 0);
+        if (all_awake) launch_gather_contacts(s, internal_view(c), 0, contacts_view(c), nullptr, n_ext, n_int, 0);
+        else launch_gather_contacts(s, internal_view(c), 0, contacts_view(c), d_awake.p, n_ext, n_int, 3);
+        if (!all_awake) AM_HIP(c, hipStreamSynchronize(s));  // d_awake is freed on return
         AM_HIP(c, c->sweep_order.ensure(c->n_contacts));
         AM_HIP(c, hipMemcpyAsync(c->sweep_order.p, c->s_order.p, (size_t)c->n_contacts * 4, hipMemcpyDeviceToDevice,
                                  s));
@@ -1107,8 +1208,11 @@ int merge_merge(am_ctx* c, int* n_merged) {
     std::vector<BpcRec> T(c->n_bpc);
     AM_HIP(c, hipMemcpyAsync(T.data(), c->d_bpc.p, T.size() * sizeof(BpcRec), hipMemcpyDeviceToHost, s));
     AM_HIP(c, hipStreamSynchronize(s));
+    // BodyPairContact.checkMergeCondition (:151-177): a pair whose two bodies
+    // are both asleep merges without the other tests
+    auto asleep = [&](int b) { return (c->flags[b] & AM_BODY_SLEEPING) != 0; };
     bool any = false;
-    for (const auto& r : T) any = any || r.flag;
+    for (const auto& r : T) any = any || r.flag || (asleep(r.b1) && asleep(r.b2));
     if (!any) return AM_OK;
     std::vector<unsigned char> act(c->n_contacts);
     AM_HIP(c, hipMemcpyAsync(act.data(), c->d_active.p, act.size(), hipMemcpyDeviceToHost, s));
@@ -1148,7 +1252,8 @@ int merge_merge(am_ctx* c, int* n_merged) {
     auto save = [&](int e) { E.store_collection(e, coll(e)); };
     for (int i = 0; i < (int)T.size(); i++) {
         const BpcRec& b = T[i];
-        if (internal[i] || !b.flag) continue;
+        if (internal[i]) continue;
+        if (!b.flag && !(asleep(b.b1) && asleep(b.b2))) continue;
         if (same_collection(c, b.b1, b.b2)) continue;
         c->merging_event = true;
         (*n_merged)++;
@@ -1375,6 +1480,175 @@ int merge_unmerge(am_ctx* c, double dt, int* n_events) {
     if ((r = commit_event(c, E))) return r;
     if ((r = upload_ibpc(c))) return r;
     return groups_after_contacts(c);
+}
+
+// ---------------------------------------------------------------------------
+// Sleeping (Sleeping.java).  The flags live in d_flags (authoritative during
+// a step) and in the host mirror c->flags; every change made on the device is
+// pulled back and the derived lists (broad-phase skips, integration targets,
+// members the single sweep advances) are rebuilt.
+
+// Metric histories for every entity: kSleepHistStride doubles and a length.
+int ensure_sleep_state(am_ctx* c) {
+    hipStream_t s = c->stream;
+    const int n = c->n_ent;
+    if (c->sleep_cap < n) {
+        const int keep = c->sleep_cap;
+        AM_HIP(c, grow_keep(s, c->d_sleep_hist, (size_t)n, (size_t)kSleepHistStride, (size_t)keep));
+        AM_HIP(c, grow_keep(s, c->d_sleep_n, (size_t)n, 1, (size_t)keep));
+        AM_HIP(c, c->d_sleep_ext.ensure(n));
+        fill_int(s, c->d_sleep_n.p + keep, n - keep, 0);
+        fill_double(s, c->d_sleep_hist.p + (size_t)keep * kSleepHistStride, (n - keep) * kSleepHistStride, 0.0);
+        c->sleep_cap = n;
+    }
+    if (!c->sleep_fresh.empty()) {
+        std::vector<int> fresh;
+        for (int e : c->sleep_fresh)
+            if (e < n) fresh.push_back(e);
+        c->sleep_fresh.clear();
+        if (!fresh.empty()) {
+            std::vector<int> hn(n);
+            int r = pull_i(c, hn.data(), c->d_sleep_n.p, n);
+            if (r) return r;
+            AM_HIP(c, hipStreamSynchronize(s));
+            for (int e : fresh) hn[e] = 0;
+            AM_HIP(c, hipMemcpyAsync(c->d_sleep_n.p, hn.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+            AM_HIP(c, hipStreamSynchronize(s));
+        }
+    }
+    return AM_OK;
+}
+
+namespace {
+
+// RigidBody.wake (RigidBody.java:298-306): clears the flag (and the history)
+// of the body, then of its collection.
+void wake_flag(am_ctx* c, int e, std::vector<char>& woke) {
+    if (c->flags[e] & AM_BODY_SLEEPING) {
+        c->flags[e] &= ~AM_BODY_SLEEPING;
+        c->desc[e].flags &= ~AM_BODY_SLEEPING;
+        woke[e] = 1;
+    }
+}
+void wake_one(am_ctx* c, int b, std::vector<char>& woke) {
+    wake_flag(c, b, woke);
+    if (c->parent[b] >= 0) wake_flag(c, c->parent[b], woke);
+}
+
+// Uploads flags and clears the histories of the woken entities.
+int commit_wake(am_ctx* c, const std::vector<char>& woke, int* n_woken) {
+    int n = 0;
+    for (char w : woke) n += w != 0;
+    if (n_woken) *n_woken = n;
+    if (n == 0) return AM_OK;
+    int r = ensure_sleep_state(c);
+    if (r) return r;
+    std::vector<int> hn(c->n_ent);
+    if ((r = pull_i(c, hn.data(), c->d_sleep_n.p, c->n_ent))) return r;
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    for (int e = 0; e < c->n_ent; e++)
+        if (woke[e]) hn[e] = 0;
+    AM_HIP(c, hipMemcpyAsync(c->d_sleep_n.p, hn.data(), (size_t)c->n_ent * 4, hipMemcpyHostToDevice, c->stream));
+    return rebuild_structure(c);
+}
+
+bool any_top_asleep(const am_ctx* c) {
+    for (int e : c->slot_entity)
+        if (c->flags[e] & AM_BODY_SLEEPING) return true;
+    return false;
+}
+
+}  // namespace
+
+// Sleeping.wake (Sleeping.java:107-139), after the warm start.
+int sleep_wake(am_ctx* c, int* n_woken) {
+    if (n_woken) *n_woken = 0;
+    if (!c->params.enable_sleeping || !any_top_asleep(c)) return AM_OK;
+    hipStream_t s = c->stream;
+    std::vector<char> woke(c->n_ent, 0);
+    const int n = c->n_contacts;
+    if (n > 0) {
+        fill_int(s, c->d_scalars.p + 20, 1, 0);
+        launch_wake_count(s, bodies_view(c), c->c_b1.p, c->c_b2.p, n, c->d_scalars.p + 20);
+        AM_HIP(c, hipGetLastError());
+        int r = read_scalars(c, 20, 1);
+        if (r) return r;
+        if (c->h_scalars[20] > 0) {
+            std::vector<int> b1(n), b2(n);
+            if ((r = pull_i(c, b1.data(), c->c_b1.p, n))) return r;
+            if ((r = pull_i(c, b2.data(), c->c_b2.p, n))) return r;
+            AM_HIP(c, hipStreamSynchronize(s));
+            // bodyPairContacts of every top-level body: the pairs of this step
+            // between bodies that are not pinned
+            auto top = [&](int b) { return c->parent[b] >= 0 ? c->parent[b] : b; };
+            std::vector<std::vector<int>> pairs(c->n_ent);
+            for (int k = 0; k < n; k++) {
+                if ((c->flags[b1[k]] | c->flags[b2[k]]) & AM_BODY_PINNED) continue;
+                pairs[top(b1[k])].push_back(k);
+                if (top(b2[k]) != top(b1[k])) pairs[top(b2[k])].push_back(k);
+            }
+            const std::vector<int> order = c->slot_entity;
+            for (int e : order) {
+                if (!(c->flags[e] & AM_BODY_SLEEPING)) continue;
+                for (int k : pairs[e]) {
+                    wake_one(c, b1[k], woke);
+                    wake_one(c, b2[k], woke);
+                }
+            }
+        }
+    }
+    // body-body springs with one end asleep and neither end pinned
+    for (const am_spring& sp : c->springs_host) {
+        if (sp.type != AM_SPRING_BODYBODY) continue;
+        const int t1 = c->parent[sp.body1] >= 0 ? c->parent[sp.body1] : sp.body1;
+        const int t2 = c->parent[sp.body2] >= 0 ? c->parent[sp.body2] : sp.body2;
+        const bool s1 = (c->flags[t1] & AM_BODY_SLEEPING) != 0, s2 = (c->flags[t2] & AM_BODY_SLEEPING) != 0;
+        const bool p1 = (c->flags[t1] & AM_BODY_PINNED) != 0, p2 = (c->flags[t2] & AM_BODY_PINNED) != 0;
+        if (s1 != s2 && !(p1 || p2)) {
+            wake_one(c, sp.body1, woke);
+            wake_one(c, sp.body2, woke);
+        }
+    }
+    return commit_wake(c, woke, n_woken);
+}
+
+// Sleeping.wakeAll (Sleeping.java:144-150): every top-level body.
+int sleep_wake_all(am_ctx* c) {
+    std::vector<char> woke(c->n_ent, 0);
+    for (int e : std::vector<int>(c->slot_entity)) wake_one(c, e, woke);
+    return commit_wake(c, woke, nullptr);
+}
+
+// Sleeping.sleep (Sleeping.java:49-98), at the end of the step.
+int sleep_sleep(am_ctx* c, int* n_asleep) {
+    if (!c->params.enable_sleeping) {
+        if (n_asleep) {
+            int k = 0;
+            for (int e : c->slot_entity) k += (c->flags[e] & AM_BODY_SLEEPING) != 0;
+            *n_asleep = k;
+        }
+        return AM_OK;
+    }
+    int r = ensure_sleep_state(c);
+    if (r) return r;
+    hipStream_t s = c->stream;
+    fill_int(s, c->d_sleep_ext.p, c->n_ent, 0);
+    launch_sleep_external(s, bodies_view(c), c->c_b1.p, c->c_b2.p, c->c_lam.p, c->n_contacts, c->d_sleep_ext.p);
+    fill_int(s, c->d_scalars.p + 21, 2, 0);
+    launch_sleep_decide(s, bodies_view(c), c->d_flags.p, c->d_slot_entity.p, (int)c->slot_entity.size(),
+                        c->d_sleep_ext.p, c->d_sleep_hist.p, c->d_sleep_n.p, c->params.sleep_step_accum,
+                        c->params.sleep_threshold, c->d_scalars.p + 21);
+    AM_HIP(c, hipGetLastError());
+    if ((r = read_scalars(c, 21, 2))) return r;
+    if (n_asleep) *n_asleep = c->h_scalars[22];
+    if (c->h_scalars[21] > 0) {
+        if ((r = pull_i(c, c->flags.data(), c->d_flags.p, c->n_ent))) return r;
+        AM_HIP(c, hipStreamSynchronize(s));
+        for (int e = 0; e < c->n_ent; e++)
+            c->desc[e].flags = (c->desc[e].flags & ~AM_BODY_SLEEPING) | (c->flags[e] & AM_BODY_SLEEPING);
+        return rebuild_structure(c);
+    }
+    return AM_OK;
 }
 
 }  // namespace am_rt

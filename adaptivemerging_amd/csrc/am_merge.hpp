@@ -37,6 +37,12 @@ struct IbpcRec {
     int pad;
 };
 
+// Longest motion-metric history of a body for sleeping (the reference's
+// stepAccum slider goes up to 200); one more slot holds the value appended
+// before the oldest is dropped.
+constexpr int kMaxSleepAccum = 200;
+constexpr int kSleepHistStride = kMaxSleepAccum + 1;
+
 struct MergeParams {
     int step_accum_merging, step_accum_unmerging;
     int merge_pinned, stable_contact, let_it_breathe;
@@ -55,7 +61,14 @@ void launch_bpc_build(hipStream_t s, ContactsView C, const int* n_contacts, cons
 void launch_bpc_hash(hipStream_t s, const BpcRec* bpc, int n, int* hash, int mask);
 void launch_bpc_accumulate(hipStream_t s, BodiesView B, BpcRec* bpc, int n, MergeParams P);
 void launch_ibpc_accumulate(hipStream_t s, BodiesView B, IbpcRec* ib, int n, ContactsView IC, MergeParams P);
+// mode 0 copy, 1 fresh copy (lambda_warm, b, D zeroed, not new), 2 lambda_warm
+// = lambda; 3 scatter: dst[idx[k]] = src[src_off + k]
 void launch_gather_contacts(hipStream_t s, ContactsView dst, int dst_off, ContactsView src, const int* idx,
                             int src_off, int n, int mode);
+void launch_wake_count(hipStream_t s, BodiesView B, const int* b1, const int* b2, int n, int* count);
+void launch_sleep_external(hipStream_t s, BodiesView B, const int* b1, const int* b2, const double* lam, int n,
+                           int* ext);
+void launch_sleep_decide(hipStream_t s, BodiesView B, int* flags, const int* slot_entity, int n_slots, const int* ext,
+                         double* hist, int* hist_n, int accum, double threshold, int* counts);
 
 }  // namespace am

@@ -221,9 +221,41 @@ class Merging:
         return n
 
 
+class Sleeping:
+    """m3d/Sleeping.java: sleep / wake over the context's bodies, with the
+    SleepParameters names and defaults (:25-30)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.enableSleeping = True
+        self.stepAccum = 10
+        self.threshold = 1e-5
+        self.wakeAllRequested = False   # SleepParameters.wakeAll (the "wake all" button)
+
+    def apply(self):
+        p = self.ctx.params
+        p.enable_sleeping = int(self.enableSleeping)
+        p.sleep_step_accum = self.stepAccum
+        p.sleep_threshold = self.threshold
+        self.ctx.set_params()
+
+    def wake(self) -> int:
+        """Sleeping.wake (:107-139)."""
+        return self.ctx.wake()
+
+    def sleep(self) -> int:
+        """Sleeping.sleep (:49-98); returns the top-level bodies asleep."""
+        return self.ctx.sleep()
+
+    def wakeAll(self):
+        """Sleeping.wakeAll (:144-150)."""
+        self.ctx.wake_all()
+        self.wakeAllRequested = False
+
+
 class RigidBodySystem:
     """m3d/RigidBodySystem.java: the per-step phase order of advanceTime
-    (:102-185); sleeping is not modelled."""
+    (:102-185)."""
 
     def __init__(self, scene, device: int = 0):
         self.scene = scene
@@ -231,6 +263,7 @@ class RigidBodySystem:
         self.ctx.upload(scene)
         self.collision = CollisionProcessor(self.ctx)
         self.merging = Merging(self.ctx)
+        self.sleeping = Sleeping(self.ctx)
         self.useGravity = True
         self.gravityAmount = 1.0
         self.gravityAngle = 90.0
@@ -262,12 +295,16 @@ class RigidBodySystem:
         t0 = time.perf_counter()
         self.totalSteps += 1
         self.merging.apply()
+        self.sleeping.apply()
         self.ctx.clear_bodies()
+        if self.sleeping.wakeAllRequested:
+            self.sleeping.wakeAll()
         self.applyExternalForces(dt)
         self.collision.updateContactsMap()
         self.collision.collisionDetection(dt)
         self.collision.updateBodyPairContacts()
         self.collision.warmStart(False)
+        self.sleeping.wake()
         if self.merging.updateContactsInCollections:
             self.collision.updateInCollections(dt)
         self.ctx.accumulate_for_unmerging()
@@ -285,6 +322,7 @@ class RigidBodySystem:
         self.ctx.advance_positions(dt)
         if self.totalSteps % self.merging.stepsBetweenMergeEvents == 0:
             self.merging.merge()
+        self.sleeping.sleep()
         self.computeTime = time.perf_counter() - t0
         self.simulationTime += dt
 

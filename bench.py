@@ -215,9 +215,11 @@ def main() -> int:
     b, _, _, _, _, _ = scene.arrays()
     n_dyn = int(np.sum((b["flags"] & 1) == 0))
     # the pile workload runs without merging (its bodies never settle into
-    # stable stacks within the measured steps)
+    # stable stacks within the measured steps) and, like every measured
+    # configuration, without sleeping
     params_kw = dict(iterations=args.iterations, tolerance=args.tolerance, gravity_amount=9.81,
                      pgs_grid=args.pgs_grid, pgs_block=args.pgs_block, enable_merging=0, enable_unmerging=0,
+                     enable_sleeping=0,
                      pgs_plain_launch=int(args.pgs_launch == "plain"),
                      pgs_schedule=0 if args.pgs_schedule == "dataflow" else 1)
     learned = args.learned == "on"

@@ -61,7 +61,7 @@ enum am_geom {
 enum am_body_flags {
     AM_BODY_PINNED = 1,        /* RigidBody.pinned */
     AM_BODY_MAGNET_ACTIVE = 2, /* magnetic && activateMagnet: no clamping in PGS */
-    AM_BODY_SLEEPING = 4       /* RigidBody.sleeping (broad-phase skip rule only) */
+    AM_BODY_SLEEPING = 4       /* RigidBody.sleeping (set and cleared by am_sleep / am_wake) */
 };
 
 enum am_contact_state { AM_STATE_BROKEN = 0, AM_STATE_ONEDGE = 1, AM_STATE_CLEAR = 2 };
@@ -129,6 +129,10 @@ typedef struct am_params {
     double viscous_angular_decay;     /* 1 */
     double spring_stiffness_mod;      /* 1 (RigidBodySystem.springStiffnessMod) */
     double spring_damping_mod;        /* 1 (RigidBodySystem.springDampingMod) */
+    /* sleeping (Sleeping.SleepParameters, Sleeping.java:25-30) */
+    int32_t enable_sleeping;          /* 1 */
+    int32_t sleep_step_accum;         /* 10 (0..200): length of each body's metric history */
+    double sleep_threshold;           /* 1e-5 */
     /* merging (Merging.MergeParameters) */
     int32_t enable_merging;           /* 1 */
     int32_t enable_merge_pinned;      /* 1 */
@@ -236,7 +240,7 @@ typedef struct am_step_stats {
     int32_t n_merged;         /* body pairs merged by this step (Merging.merge) */
     int32_t n_unmerged;       /* collections cut by this step (Merging.unmerge) */
     int32_t n_internal_contacts; /* contacts inside collections */
-    int32_t reserved2;
+    int32_t n_sleeping;       /* top-level entities asleep after the step */
     double ms_merging;        /* single sweep, unmerge, pair histories and merge */
 } am_step_stats;
 
@@ -340,9 +344,10 @@ int am_advance_positions(am_ctx* ctx, double dt);
 /* One RigidBodySystem.advanceTime (RigidBodySystem.java:102-185): clear,
  * gravity, contacts map, detection, warm start, the single sweep inside
  * collections, unmerging, the main solve, body-pair histories, velocity and
- * position update, and every steps_between_merge_events steps merging.  The
- * merging phases run only when enable_merging / enable_unmerging are set or
- * collections exist. */
+ * position update, every steps_between_merge_events steps merging, and
+ * sleeping (am_wake after the warm start, am_sleep last).  The merging phases
+ * run only when enable_merging / enable_unmerging are set or collections
+ * exist; the sleeping phases only when enable_sleeping is set. */
 int am_step(am_ctx* ctx, double dt, am_step_stats* stats);
 
 /* ---- adaptive merging (Merging, BodyPairContact, RigidCollection) ------
@@ -363,6 +368,31 @@ int am_clear_body_pair_contacts(am_ctx* ctx);
 int am_accumulate_for_merging(am_ctx* ctx);
 /* Merging.merge (Merging.java:73-163); n_merged = body pairs merged */
 int am_merge(am_ctx* ctx, int32_t* n_merged);
+/* ---- sleeping (Sleeping.java) ------------------------------------------
+ * am_step calls am_wake right after the warm start and am_sleep at the end
+ * of the step (RigidBodySystem.java:128,170) when enable_sleeping is set.
+ * A sleeping top-level body is not integrated, keeps no contacts with pinned
+ * bodies (CollisionProcessor.java:693), and a sleeping collection is left out
+ * of the single sweep and of unmerging. */
+/* Sleeping.wake (Sleeping.java:107-139): a sleeping body touching a body that
+ * is not pinned wakes, with that body; so do the two ends of a body-body
+ * spring when one end sleeps and the other does not.  n_woken = entities
+ * woken. */
+int am_wake(am_ctx* ctx, int32_t* n_woken);
+/* Sleeping.sleep (Sleeping.java:49-98): every awake top-level body without an
+ * active contact to a body that is not pinned appends its motion metric to
+ * its history; it falls asleep once the last sleep_step_accum values are all
+ * at most sleep_threshold and none exceeds its predecessor by more than 5e-5.
+ * n_asleep = top-level entities asleep afterwards. */
+int am_sleep(am_ctx* ctx, int32_t* n_asleep);
+/* Sleeping.wakeAll (Sleeping.java:144-150) */
+int am_wake_all(am_ctx* ctx);
+/* Per entity: the sleeping flag, the history length and the history, oldest
+ * first, AM_SLEEP_HIST_STRIDE doubles per entity (entries past the length are
+ * stale).  Any of the three arrays may be NULL. */
+#define AM_SLEEP_HIST_STRIDE 201
+int am_sleep_state_pull(am_ctx* ctx, uint8_t* sleeping, int32_t* hist_n, double* hist, int32_t n);
+
 /* External pairs of the last am_clear_body_pair_contacts, then internal pairs. */
 int am_body_pairs_pull(am_ctx* ctx, am_body_pair* out, int32_t capacity, int32_t* n_external, int32_t* n_internal);
 /* RigidCollection.internalContacts of every collection, pair by pair. */

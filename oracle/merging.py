@@ -9,6 +9,7 @@ The phases follow src/mergingBodies3D/ of the reference:
   BodyPairContact (histories, tests)     BodyPairContact.java:83-266
   Merging.merge / unmerge                Merging.java:73-393
   RigidCollection (mass properties)      RigidCollection.java:55-937
+  Sleeping.wake / sleep / wakeAll        Sleeping.java:49-160
 Detection, warm start, the PGS sweeps and the integration of top-level
 entities run in the C oracle; the body-pair bookkeeping, the collection
 physics and the graph edits are restated here in plain Python floats (IEEE
@@ -24,8 +25,9 @@ import math
 
 import numpy as np
 
-from adaptivemerging_amd._lib import (AM_BODY_PINNED, AM_GEOM_COLLECTION, AM_GEOM_PLANE, AM_STATE_BROKEN,
-                                      AM_STATE_CLEAR, AM_STATE_ONEDGE, BODY_DTYPE)
+from adaptivemerging_amd._lib import (AM_BODY_PINNED, AM_BODY_SLEEPING, AM_GEOM_COLLECTION, AM_GEOM_PLANE,
+                                      AM_SPRING_BODYBODY, AM_STATE_BROKEN, AM_STATE_CLEAR, AM_STATE_ONEDGE,
+                                      BODY_DTYPE)
 
 from .pyoracle import Oracle
 
@@ -110,6 +112,7 @@ class Ent:
     """One entity's state and mass properties (RigidBody fields)."""
 
     def __init__(self, **kw):
+        self.sleeping = False
         self.__dict__.update(kw)
 
     def copy(self):
@@ -247,6 +250,9 @@ class MergingOracle:
         self.bpc = []          # external pairs of the last clear_body_pair_contacts
         self.ibpc = []         # internal pairs: dict(b1, b2, contacts (row dicts), hist_n, flag)
         self.n_entities = scene.n
+        self.sleep_hist = {}   # entity -> RigidBody.metricHistory
+        sp = scene.spring_array() if hasattr(scene, "spring_array") else []
+        self.springs = [(int(r["type"]), int(r["body1"]), int(r["body2"])) for r in sp]
 
     # -- helpers --------------------------------------------------------
     def set_params(self, **kw):
@@ -262,6 +268,18 @@ class MergingOracle:
     def pinned(self, e):
         return bool(int(self.desc(e)["flags"]) & AM_BODY_PINNED)
 
+    def sleeping(self, e):
+        return bool(int(self.desc(e)["flags"]) & AM_BODY_SLEEPING)
+
+    def set_sleeping(self, e, value):
+        rec = self.desc(e).copy()
+        f = int(rec["flags"])
+        rec["flags"] = (f | AM_BODY_SLEEPING) if value else (f & ~AM_BODY_SLEEPING)
+        self.o.set_entity(e, rec)
+
+    def top_of(self, b):
+        return self.parent[b] if self.parent[b] >= 0 else b
+
     def state(self):
         x, R, v, w = self.o.pull_state(self.n_entities)
         return [np.array(a) for a in (x, R, v, w)]
@@ -276,6 +294,7 @@ class MergingOracle:
                    jinv=[float(a) for a in jinv[e]], jinv0=[float(a) for a in d["jinv0"]], I0=I0,
                    I=list(ZERO) if pinned else rm0rt(Rl, I0), mass=0.0 if pinned else float(d["mass"]),
                    minv=0.0 if pinned else float(d["minv"]), pinned=pinned,
+                   sleeping=bool(int(d["flags"]) & AM_BODY_SLEEPING),
                    plane=int(d["geom"]) == AM_GEOM_PLANE, bb=[float(a) for a in d["bbox"]])
 
     def store(self, e, C, st, jinv):
@@ -284,7 +303,7 @@ class MergingOracle:
         jinv[e] = C.jinv
         rec = np.zeros((), dtype=BODY_DTYPE)
         rec["geom"] = AM_GEOM_COLLECTION
-        rec["flags"] = AM_BODY_PINNED if C.pinned else 0
+        rec["flags"] = (AM_BODY_PINNED if C.pinned else 0) | (AM_BODY_SLEEPING if C.sleeping else 0)
         rec["mass"] = C.mass
         rec["minv"] = C.minv
         rec["jinv0"] = C.jinv0
@@ -326,6 +345,7 @@ class MergingOracle:
             self.free = list(range(n - 1, old - 1, -1))
         e = self.free.pop()
         self.members[e] = []
+        self.sleep_hist[e] = []   # a new RigidCollection has an empty metric history
         return e
 
     def free_entity(self, e):
@@ -342,11 +362,18 @@ class MergingOracle:
         for b in self.members[ce]:
             self.b2c[b] = body_to_collection(C, [float(a) for a in x[b]], [float(a) for a in R[b]])
 
+    def collection_state(self, C, b, B):
+        """RigidCollection.updateCollectionState (:459-470)."""
+        C.pinned = C.pinned or B.pinned
+        C.sleeping = C.sleeping or B.sleeping
+        if B.sleeping:
+            self.set_sleeping(b, False)
+
     def add_body(self, ce, C, b, st, jinv):
         self.parent[b] = ce
         self.members[ce].append(b)
         B = self.load(b, st, jinv)
-        C.pinned = C.pinned or B.pinned
+        self.collection_state(C, b, B)
         add_body_internal(C, B)
         finish_inertia(C)
         self.update_b2c(ce, C, st)
@@ -356,7 +383,7 @@ class MergingOracle:
             self.parent[b] = ce
             self.members[ce].append(b)
             B = self.load(b, st, jinv)
-            C.pinned = C.pinned or B.pinned
+            self.collection_state(C, b, B)
             add_body_internal(C, B)
         finish_inertia(C)
         self.update_b2c(ce, C, st)
@@ -384,7 +411,7 @@ class MergingOracle:
         for e in self.top:
             if e not in self.members:
                 continue
-            if integrated_only and self.pinned(e):
+            if integrated_only and (self.pinned(e) or self.sleeping(e)):
                 continue
             for b in self.members[e]:
                 r = [x[b][a] - x[e][a] for a in range(3)]
@@ -403,7 +430,7 @@ class MergingOracle:
         x, R, v, w = self.state()
         jinv = self.o.pull_jinv()
         for e in self.top:
-            if e not in self.members or self.pinned(e):
+            if e not in self.members or self.pinned(e) or self.sleeping(e):
                 continue
             Rc = [float(a) for a in R[e]]
             for b in self.members[e]:
@@ -444,6 +471,86 @@ class MergingOracle:
         rows = [c for p in self.ibpc for c in p["contacts"]]
         return _stack(rows)
 
+    # -- Sleeping (Sleeping.java) -------------------------------------------
+    def _wake_body(self, b, woke):
+        """RigidBody.wake (RigidBody.java:298-306): the body, then its collection."""
+        for e in (b, self.parent[b]):
+            if e >= 0 and self.sleeping(e):
+                self.set_sleeping(e, False)
+                self.sleep_hist[e] = []
+                woke.add(e)
+
+    def wake(self):
+        """Sleeping.wake (:107-139), right after the warm start: in the order of
+        the top-level list, each body still asleep wakes both ends of its pairs
+        of this step between bodies that are not pinned; then each body-body
+        spring with exactly one end asleep and neither end pinned wakes both."""
+        if not self.params.enable_sleeping:
+            return 0
+        woke = set()
+        c = self.o.contacts()
+        pairs = {}
+        for k in range(len(c["body1"])):
+            a, b = int(c["body1"][k]), int(c["body2"][k])
+            if self.pinned(a) or self.pinned(b):
+                continue
+            for e in {self.top_of(a), self.top_of(b)}:
+                pairs.setdefault(e, []).append((a, b))
+        for e in list(self.top):
+            if not self.sleeping(e):
+                continue
+            for a, b in pairs.get(e, []):
+                self._wake_body(a, woke)
+                self._wake_body(b, woke)
+        for typ, b1, b2 in self.springs:
+            if typ != AM_SPRING_BODYBODY:
+                continue
+            t1, t2 = self.top_of(b1), self.top_of(b2)
+            if self.sleeping(t1) != self.sleeping(t2) and not (self.pinned(t1) or self.pinned(t2)):
+                self._wake_body(b1, woke)
+                self._wake_body(b2, woke)
+        return len(woke)
+
+    def wake_all(self):
+        """Sleeping.wakeAll (:144-150)."""
+        woke = set()
+        for e in list(self.top):
+            self._wake_body(e, woke)
+
+    def sleep(self):
+        """Sleeping.sleep (:49-98) at the end of the step; returns the number of
+        top-level bodies asleep afterwards."""
+        P = self.params
+        if P.enable_sleeping:
+            c = self.o.contacts()
+            ext = set()
+            for k in range(len(c["body1"])):
+                if not abs(c["lambda"][k][0]) > 1e-14:
+                    continue
+                a, b = int(c["body1"][k]), int(c["body2"][k])
+                if self.pinned(a) or self.pinned(b) or self.same(a, b):
+                    continue
+                ext.add(self.top_of(a))
+                ext.add(self.top_of(b))
+            for e in list(self.top):
+                if self.sleeping(e) or e in ext:
+                    continue
+                h = self.sleep_hist.setdefault(e, [])
+                h.append(self.o.motion_metric(e))
+                if len(h) > P.sleep_step_accum:
+                    h.pop(0)
+                sleep = len(h) >= P.sleep_step_accum
+                prev = MAXD
+                for m in h:
+                    if not sleep:
+                        break
+                    if m > prev + 5e-5 or m > P.sleep_threshold:
+                        sleep = False
+                    prev = m
+                if sleep:
+                    self.set_sleeping(e, True)
+        return sum(1 for e in self.top if self.sleeping(e))
+
     # -- CollisionProcessor.updateInCollections (:232-303) -----------------
     def update_in_collections(self, dt, order_of_sweep):
         """order_of_sweep(): the sweep order of the combined list (external
@@ -452,12 +559,14 @@ class MergingOracle:
             return 0
         ext = self.o.contacts()
         n_ext = len(ext["body1"])
-        comb = _concat(ext, self.internal_contacts())
+        # internal contacts of the collections that are awake (:265-270, :400-408)
+        awake = [p for p in self.ibpc if not self.sleeping(self.parent[p["b1"]])]
+        comb = _concat(ext, _stack([c for p in awake for c in p["contacts"]]))
         self.o.push_contacts_raw(comb)
         used, _, _ = self.o.solve(order_of_sweep(), dt, main=False)
         res = self.o.contacts()
         k = n_ext
-        for p in self.ibpc:
+        for p in awake:
             for j in range(len(p["contacts"])):
                 p["contacts"][j] = _row(res, k)
                 k += 1
@@ -468,6 +577,8 @@ class MergingOracle:
         dv = self.o.pull_deltav()
         jinv = self.o.pull_jinv()
         for e in self.top:
+            if e in self.members and self.sleeping(e):
+                continue
             for b in self.members.get(e, []):
                 if self.pinned(b):
                     continue
@@ -498,6 +609,8 @@ class MergingOracle:
         """RigidBodySystem.accumulateForUnmerging + the unmerge test."""
         P = self.params
         for p in self.ibpc:
+            if self.sleeping(self.parent[p["b1"]]):
+                continue   # RigidBodySystem.accumulateForUnmerging skips sleeping collections (:326-329)
             m = self._metric(p["b1"], p["b2"])
             p["hist_n"] = p["hist_n"] + 1 if m > P.threshold_unmerge else 0
             p["flag"] = unmerge_condition(P, [c["state"] for c in p["contacts"]], p["hist_n"], len(p["contacts"]))
@@ -545,7 +658,11 @@ class MergingOracle:
 
     # -- Merging.merge (:73-163) ---------------------------------------------
     def merge(self):
-        if not self.params.enable_merging or not self.bpc or not any(r["flag"] for r in self.bpc):
+        def asleep(r):
+            # checkMergeCondition (BodyPairContact.java:151-154): both bodies asleep
+            return r["b1"] >= 0 and self.sleeping(r["b1"]) and self.sleeping(r["b2"])
+
+        if not self.params.enable_merging or not self.bpc or not any(r["flag"] or asleep(r) for r in self.bpc):
             return 0
         st = self.state()
         jinv = [list(map(float, j)) for j in self.o.pull_jinv()]
@@ -576,7 +693,7 @@ class MergingOracle:
 
         merged = 0
         for i, r in enumerate(self.bpc):
-            if internal[i] or not r["flag"] or self.same(r["b1"], r["b2"]):
+            if internal[i] or not (r["flag"] or asleep(r)) or self.same(r["b1"], r["b2"]):
                 continue
             self.merging_event = True
             merged += 1
@@ -600,6 +717,7 @@ class MergingOracle:
                     self.parent[b] = keep
                     self.members[keep].append(b)
                 K.pinned = K.pinned or O.pinned
+                K.sleeping = K.sleeping or O.sleeping
                 add_body_internal(K, O)
                 finish_inertia(K)
                 self.update_b2c(keep, K, st)
@@ -651,7 +769,7 @@ class MergingOracle:
         zero_dv = []
         events = 0
         for ce in list(self.top):
-            if ce not in self.members:
+            if ce not in self.members or self.sleeping(ce):
                 continue
             cut = [i for i, p in enumerate(self.ibpc) if p["flag"] and self.parent[p["b1"]] == ce]
             if not cut:

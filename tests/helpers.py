@@ -59,3 +59,44 @@ def oracle_step_like_gpu(ctx, orc, dt):
     ctx.advance_velocities(dt); orc.advance_velocities(dt)
     ctx.advance_positions(dt); orc.advance_positions(dt)
     return it_g, it_o
+
+
+def merging_oracle_step(mo, dt, step, order=None, kick=None):
+    """One RigidBodySystem.advanceTime (RigidBodySystem.java:102-185) on the
+    merging oracle alone: every phase including sleeping, contacts swept in
+    list order unless order(n) gives the sweep order of the single sweep.
+    kick = (body, fy) adds an upward force after the external forces.
+    Returns (merged, cut, asleep)."""
+    import numpy as np
+
+    mo.total_steps += 1
+    mo.clear_bodies()
+    mo.o.apply_gravity()
+    mo.o.apply_springs()
+    if kick is not None:
+        f, t = mo.o.pull_forces()
+        f[kick[0]][1] += kick[1]
+        mo.o.push_forces(f, t)
+    mo.o.update_contacts_map()
+    n = mo.o.detect()
+    mo.o.warm_start()
+    mo.wake()
+    n_int = sum(len(q["contacts"]) for q in mo.ibpc if not mo.sleeping(mo.parent[q["b1"]]))
+    mo.update_in_collections(dt, (lambda: order(n + n_int)) if order else
+                             (lambda: np.arange(n + n_int, dtype=np.int32)))
+    mo.accumulate_for_unmerging()
+    cut = mo.unmerge(dt)
+    if mo.merging_event:
+        mo.clear_top_level()
+    mo.o.redo_warm_start()
+    mo.o.solve(np.arange(len(mo.o.contacts()["body1"]), dtype=np.int32), dt)
+    mo.clear_body_pair_contacts()
+    mo.advance_velocities(dt)
+    mo.accumulate_for_merging()
+    mo.advance_positions(dt)
+    merged = 0
+    steps = mo.params.steps_between_merge_events
+    if steps > 0 and step % steps == 0:
+        merged = mo.merge()
+    asleep = mo.sleep()
+    return merged, cut, asleep

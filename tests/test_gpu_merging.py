@@ -110,7 +110,7 @@ def test_merge_and_unmerge_match_oracle(gpu_lib):
 
 
 def test_step_equals_phases(gpu_lib):
-    """am_step with merging on runs exactly the phases above."""
+    """am_step with merging (and sleeping) on runs exactly the phases above."""
     scene = workloads.merge_stack(seed=4)
     a = Context(); a.upload(scene); a.set_params(**PARAMS)
     b = Context(); b.upload(scene); b.set_params(**PARAMS)
@@ -118,7 +118,7 @@ def test_step_equals_phases(gpu_lib):
     for step in range(1, 26):
         st = a.step(dt)
         b.clear_bodies(); b.apply_gravity(); b.apply_spring_forces()
-        b.update_contacts_map(); b.detect(dt); b.warm_start()
+        b.update_contacts_map(); b.detect(dt); b.warm_start(); b.wake()
         b.update_in_collections(dt); b.accumulate_for_unmerging(); b.unmerge(dt)
         if b.merging_event:
             b.clear_top_level()
@@ -126,6 +126,7 @@ def test_step_equals_phases(gpu_lib):
         b.accumulate_for_merging(); b.advance_positions(dt)
         if step % 10 == 0:
             b.merge()
+        b.sleep()
         pa, ta = a.structure()
         pb, tb = b.structure()
         assert list(ta) == list(tb) and list(pa) == list(pb)
