@@ -173,6 +173,8 @@ class Merging:
     def __init__(self, ctx: Context):
         self.ctx = ctx
         self.mergingEvent = False
+        self.mergingBuildTime = 0.0     # MergeParameters.mergingBuildTime
+        self.unmergingBuildTime = 0.0
         self.enableMerging = True
         self.enableMergePinned = True
         self.enableMergeStableContactCondition = True
@@ -272,6 +274,13 @@ class RigidBodySystem:
         self.totalSteps = 0
         self.simulationTime = 0.0
         self.computeTime = 0.0
+        self.warmStartTime = 0.0
+        self.mergingTime = 0.0
+        self.unmergingTime = 0.0
+        # CSV export (RigidBodySystem.java:495-549): off by default
+        self.saveCSV = False
+        self.sceneName = "scene"
+        self._csv = None
 
     @property
     def bodies(self):
@@ -303,14 +312,21 @@ class RigidBodySystem:
         self.collision.updateContactsMap()
         self.collision.collisionDetection(dt)
         self.collision.updateBodyPairContacts()
+        t1 = time.perf_counter()
         self.collision.warmStart(False)
+        self.warmStartTime = time.perf_counter() - t1
         self.sleeping.wake()
         if self.merging.updateContactsInCollections:
             self.collision.updateInCollections(dt)
+        t1 = time.perf_counter()
         self.ctx.accumulate_for_unmerging()
-        self.merging.unmerge(dt)
+        if self.merging.unmerge(dt):
+            self.merging.unmergingBuildTime = time.perf_counter() - t1
+        else:
+            self.merging.unmergingBuildTime = 0.0
         if self.merging.mergingEvent:
             self.ctx.clear_top_level()
+        self.unmergingTime = time.perf_counter() - t1
         if self.collision.useWarmStart:
             self.collision.redoWarmStart()
         else:
@@ -318,13 +334,43 @@ class RigidBodySystem:
         self.collision.solveLCP(dt, False)
         self.collision.clearBodyPairContacts()
         self.ctx.advance_velocities(dt)
+        t1 = time.perf_counter()
         self.ctx.accumulate_for_merging()
+        self.mergingTime = time.perf_counter() - t1
         self.ctx.advance_positions(dt)
+        self.merging.mergingBuildTime = 0.0
         if self.totalSteps % self.merging.stepsBetweenMergeEvents == 0:
-            self.merging.merge()
+            t1 = time.perf_counter()
+            if self.merging.merge():
+                self.merging.mergingBuildTime = time.perf_counter() - t1
+            self.mergingTime += time.perf_counter() - t1
         self.sleeping.sleep()
         self.computeTime = time.perf_counter() - t0
         self.simulationTime += dt
+        self.exportDataToFile()
+
+    def exportDataToFile(self):
+        """exportDataToFile (:495-542): while saveCSV is on, the first call
+        opens <sceneName>.csv (<sceneName>_merged.csv with merging on) and
+        writes the header, later calls one line per step; turning saveCSV off
+        closes the file.  contactOrdering is 0: contacts are not reordered
+        breadth first here; singleItPGS is the whole single sweep."""
+        from .csv_export import CSVExporter
+
+        if self.saveCSV:
+            if self._csv is None:
+                name = self.sceneName + ("_merged.csv" if self.merging.enableMerging else ".csv")
+                self._csv = CSVExporter(name)
+            else:
+                c = self.collision
+                self._csv.row([len(self.ctx.structure()[1]), self.ctx.n_contacts(), c.collisionDetectTime,
+                               self.warmStartTime, c.collisionSolveTime, c.collectionUpdateTime,
+                               c.contactOrderingTime, c.collectionUpdateTime, self.mergingTime,
+                               self.merging.mergingBuildTime, self.unmergingTime, self.merging.unmergingBuildTime,
+                               self.computeTime])
+        elif self._csv is not None:
+            self._csv.close()
+            self._csv = None
 
     def reset(self):
         """reset (:390-426): collections dissolved, bodies back to their

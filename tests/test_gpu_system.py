@@ -67,3 +67,25 @@ def test_reset_restores_the_initial_scene(gpu_lib):
     for k, (a, b) in enumerate(zip(first, second)):
         for name, p, q in zip("xRvw", a, b):
             np.testing.assert_array_equal(p, q, err_msg=f"step {k} {name}")
+
+
+def test_csv_export_of_the_step_metrics(gpu_lib, tmp_path):
+    """saveCSV on: one header, then a line per step after the first, with the
+    top-level body count and the contact count of each step."""
+    from adaptivemerging_amd.csv_export import COLUMNS, read_csv
+
+    scene = workloads.merge_stack(seed=2)
+    system = RigidBodySystem(scene)
+    system.gravityAmount = 9.81
+    system.sceneName = str(tmp_path / "stack")
+    system.saveCSV = True
+    counts = []
+    for _ in range(12):
+        system.advanceTime(0.01)
+        counts.append((len(system.ctx.structure()[1]), system.ctx.n_contacts()))
+    system.saveCSV = False
+    system.advanceTime(0.01)
+    rows = read_csv(str(tmp_path / "stack_merged.csv"))
+    assert len(rows) == 11 and all(list(r) == list(COLUMNS) for r in rows)
+    assert [(r["#bodies"], r["#contacts"]) for r in rows] == counts[1:]
+    assert all(r["computeTime"] > 0 for r in rows)
