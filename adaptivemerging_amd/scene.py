@@ -143,16 +143,19 @@ class Scene:
         return self._append(rec, x, R, v, w, name)
 
     def add_tree(self, centres, radii, children, x, R=None, v=None, w=None, mass=1.0, inertia=(1, 1, 1),
-                 bbox=None, friction=0.8, restitution=0.0, scene=0, name=None) -> int:
+                 bbox=None, friction=0.8, restitution=0.0, scene=0, name=None, pinned=False) -> int:
         """A sphere-tree body.  children[i] lists node i's children; node 0 is
         the root.  Nodes are re-indexed breadth first so every node's children
-        are contiguous (the traversal visits them in the given order)."""
-        n = len(centres)
+        are contiguous (the traversal visits them in the given order); nodes
+        the root does not reach are dropped, as the reference never visits
+        them.  inertia is the diagonal (3 values) or the full rest-pose tensor
+        (9 values, row-major)."""
         order, q = [0], 0
         while q < len(order):
             order += list(children[order[q]])
             q += 1
-        assert len(order) == n and sorted(order) == list(range(n)), "tree must reach every node once"
+        n = len(order)
+        assert len(set(order)) == n, "a sphere tree node has two parents"
         new_index = {old: k for k, old in enumerate(order)}
         first = len(self.nodes)
         for old in order:
@@ -171,15 +174,24 @@ class Scene:
         rec = np.zeros((), dtype=BODY_DTYPE)
         rec["geom"] = AM_GEOM_TREE
         rec["scene"] = scene
-        rec["mass"] = mass
-        rec["minv"] = 1.0 / mass
-        rec["jinv0"] = [1.0 / inertia[0], 0, 0, 0, 1.0 / inertia[1], 0, 0, 0, 1.0 / inertia[2]]
-        rec["inertia0"] = [inertia[0], 0, 0, 0, inertia[1], 0, 0, 0, inertia[2]]
+        if pinned:
+            # RigidBody with pinned set afterwards (XMLParser.java:577-583)
+            rec["flags"] = AM_BODY_PINNED
+        else:
+            rec["mass"] = mass
+            rec["minv"] = 1.0 / mass
+            if len(inertia) == 3:
+                rec["jinv0"] = [1.0 / inertia[0], 0, 0, 0, 1.0 / inertia[1], 0, 0, 0, 1.0 / inertia[2]]
+                rec["inertia0"] = [inertia[0], 0, 0, 0, inertia[1], 0, 0, 0, inertia[2]]
+            else:
+                I = np.asarray(inertia, dtype=np.float64).reshape(3, 3)
+                rec["inertia0"] = I.reshape(9)
+                rec["jinv0"] = np.linalg.inv(I).reshape(9)
         rec["friction"] = friction
         rec["restitution"] = restitution
         if bbox is None:
-            c = np.asarray(centres)
-            r = np.asarray(radii)
+            c = np.asarray([centres[i] for i in order])
+            r = np.asarray([radii[i] for i in order])
             lo = (c - r[:, None]).min(axis=0)
             hi = (c + r[:, None]).max(axis=0)
             bbox = [[lx, ly, lz] for lx in (lo[0], hi[0]) for ly in (lo[1], hi[1]) for lz in (lo[2], hi[2])]
@@ -187,6 +199,20 @@ class Scene:
         rec["tree_first"] = first
         rec["tree_count"] = n
         return self._append(rec, x, R, v, w, name)
+
+    def add_mesh(self, obj_path: str, sph_path: str, x, scale: float = 1.0, density: float = 1.0, R=None, v=None,
+                 w=None, friction=0.8, restitution=0.0, pinned=False, scene=0, name=None) -> int:
+        """XMLParser.createMesh (XMLParser.java:449-538): mass properties and
+        bounding box from the OBJ mesh (mesh.mesh_body), collision geometry
+        from the .sph sphere tree, both scaled and moved to the centre of
+        mass."""
+        from .mesh import mesh_body
+
+        mb = mesh_body(obj_path, scale, density)
+        centres, radii, children = load_sph(sph_path, scale, mb["com"])
+        return self.add_tree(centres, radii, children, x, R=R, v=v, w=w, mass=mb["mass"], inertia=mb["inertia"],
+                             bbox=mb["bbox"], friction=friction, restitution=restitution, scene=scene, name=name,
+                             pinned=pinned)
 
     # -- springs (Spring.java:50-104) -----------------------------------
     def body_point_world(self, body: int, pB, x=None, R=None):

@@ -10,9 +10,11 @@ Follows XMLParser (src/mergingBodies3D/XMLParser.java):
 
 Springs (:596-619, Spring.java) load with the attachment geometry their
 constructors compute from the pose the body has when the <spring> tag is
-read.  Mesh and composite bodies (createMesh :449-538, createComposite
-:177-353) and spinners have no counterpart in this build; a scene that uses
-them raises SceneFormatError rather than loading into different dynamics.
+read.  Mesh bodies (createMesh :449-538) load their mass properties from
+the OBJ file and their sphere tree from the .sph file.  Composite bodies
+(createComposite :177-353) and spinners have no counterpart in this build;
+a scene that uses them raises SceneFormatError rather than loading into
+different dynamics.
 Sphere-tree geometry for hand-built bodies comes from .sph files through
 scene.load_sph (the format of :500-535).
 """
@@ -145,14 +147,31 @@ def _add_springs(scene: Scene, body: int, com: "_Common", name: str, posed_plane
             scene.add_spring(body, pB, pose1=pose, **kw)
 
 
-def parse_scene(source: str, *, ignore_springs: bool = False, scene_index: int = 0):
+def _find_file(rel: str, search, name: str) -> str:
+    """A data file named in the scene (relative to the directory the
+    reference runs from): tried against each search directory in turn."""
+    if not rel:
+        raise SceneFormatError(f"body {name}: missing file name")
+    for d in search:
+        path = os.path.join(d, rel)
+        if os.path.isfile(path):
+            return path
+    raise SceneFormatError(f"body {name}: file {rel!r} not found under {list(search)}")
+
+
+def parse_scene(source: str, *, ignore_springs: bool = False, scene_index: int = 0, data_root=None):
     """Parses a scene (a file path or the XML text).  Returns (scene,
     collision), where collision maps CollisionProcessor field names to the
-    values the file sets."""
+    values the file sets.  Mesh files (obj=, st=) are looked up under
+    data_root, the current directory, and the scene file's directory and
+    its parent (the reference's scenes name them relative to its root)."""
     text = source
+    search = [d for d in (data_root, os.getcwd()) if d]
     if not source.lstrip().startswith("<"):
         with open(source) as f:
             text = f.read()
+        here = os.path.dirname(os.path.abspath(source))
+        search += [here, os.path.dirname(here)]
     try:
         root = ET.fromstring(text)
     except ET.ParseError as e:
@@ -203,8 +222,18 @@ def parse_scene(source: str, *, ignore_springs: bool = False, scene_index: int =
             b = scene.add_sphere(float(node.attrib["r"]), com.pos, R=com.R, v=com.v, w=com.w, density=density,
                                  friction=com.friction, restitution=com.restitution, scene=scene_index, name=name)
             _add_springs(scene, b, com, name, posed_planes)
-        elif kind in ("mesh", "composite"):
-            raise SceneFormatError(f"body {name}: {kind} bodies are not supported by this build")
+        elif kind == "mesh":
+            if "scale" not in node.attrib:
+                raise SceneFormatError(f"body {name}: a mesh needs a scale")
+            obj = _find_file(node.attrib.get("obj", ""), search, name)
+            sph = _find_file(node.attrib.get("st", ""), search, name)
+            b = scene.add_mesh(obj, sph, com.pos, scale=float(node.attrib["scale"]),
+                               density=float(node.attrib.get("density", 1.0)), R=com.R, v=com.v, w=com.w,
+                               friction=com.friction, restitution=com.restitution, pinned=com.pinned,
+                               scene=scene_index, name=name)
+            _add_springs(scene, b, com, name, posed_planes)
+        elif kind == "composite":
+            raise SceneFormatError(f"body {name}: composite bodies are not supported by this build")
         else:
             raise SceneFormatError(f"body {name}: unknown type {kind!r}")
     return scene, collision
