@@ -28,6 +28,12 @@ AM_GEOM_COLLECTION = 3
 AM_BODY_PINNED = 1
 AM_BODY_MAGNET_ACTIVE = 2
 AM_BODY_SLEEPING = 4
+AM_GEOM_COMPOSITE = 4
+
+
+def AM_BV_SUB(s: int, bv: int) -> int:
+    """The bv id of a contact on sub-body s of a composite (include/am.h)."""
+    return -3 - (s << 16) - (bv + 1)
 AM_SLEEP_HIST_STRIDE = 201
 
 AM_STATE_BROKEN = 0
@@ -126,7 +132,7 @@ class BodyPair(C.Structure):
 # the header and the shared object.
 SYMBOLS = [
     "am_version", "am_ctx_create", "am_ctx_destroy", "am_last_error", "am_params_default", "am_params_set",
-    "am_scene_upload", "am_structure_pull", "am_entities_pull", "am_jinv_pull", "am_state_push",
+    "am_scene_upload", "am_scene_upload_ex", "am_structure_pull", "am_entities_pull", "am_jinv_pull", "am_state_push",
     "am_state_pull", "am_forces_push", "am_forces_pull", "am_clear_bodies", "am_apply_gravity", "am_deltav_pull",
     "am_springs_set", "am_apply_spring_forces", "am_wake", "am_sleep", "am_wake_all", "am_sleep_state_pull",
     "am_update_contacts_map", "am_detect", "am_contacts_push", "am_contacts_pull", "am_lambdas_push",
@@ -170,6 +176,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "am_params_default": (C.c_int, [C.POINTER(Params)]),
         "am_params_set": (C.c_int, [ctxp, C.POINTER(Params)]),
         "am_scene_upload": (C.c_int, [ctxp, vp, i32, vp, i32]),
+        "am_scene_upload_ex": (C.c_int, [ctxp, vp, i32, vp, i32, vp, vp, i32]),
         "am_structure_pull": (C.c_int, [ctxp, vp, i32, vp, i32, C.POINTER(i32), C.POINTER(i32)]),
         "am_entities_pull": (C.c_int, [ctxp, vp, i32]),
         "am_jinv_pull": (C.c_int, [ctxp, vp, i32]),

@@ -51,7 +51,13 @@ class Context:
         b, nd, x, R, v, w = scene.arrays()
         self._bodies, self._nodes = b, nd
         self.n = len(b)
-        self._check(self.lib.am_scene_upload(self.h, ptr(b), len(b), ptr(nd), len(nd)))
+        subs, b2c = scene.sub_arrays() if hasattr(scene, "sub_arrays") else (None, None)
+        if subs is not None and len(subs):
+            self._subs, self._b2c = subs, b2c
+            self._check(self.lib.am_scene_upload_ex(self.h, ptr(b), len(b), ptr(nd), len(nd), ptr(subs), ptr(b2c),
+                                                    len(subs)))
+        else:
+            self._check(self.lib.am_scene_upload(self.h, ptr(b), len(b), ptr(nd), len(nd)))
         self.push_state(x, R, v, w)
         springs = scene.spring_array() if hasattr(scene, "spring_array") else None
         if springs is not None and len(springs):

@@ -38,6 +38,8 @@ BodiesView bodies_view(am_ctx* c) {
     B.parent = c->d_parent.p;
     B.slot = c->d_slot.p;
     B.sub = c->d_sub.p;
+    B.sub_desc = c->d_subs.p;
+    B.sub_b2c = c->d_sub_b2c.p;
     return B;
 }
 TreeView tree_view(am_ctx* c) { return TreeView{c->d_nodes.p, c->d_cw.p}; }
@@ -259,7 +261,9 @@ int rebuild_structure(am_ctx* c) {
     double rmax = 0.0;
     for (int b : c->grid_bodies) {
         const am_body_desc& d = c->desc[b];
-        double r = d.geom == AM_GEOM_BOX ? d.radius : (d.tree_count > 0 ? c->nodes[d.tree_first].r : 0.0);
+        double r = (d.geom == AM_GEOM_BOX || d.geom == AM_GEOM_COMPOSITE)
+                       ? d.radius
+                       : (d.tree_count > 0 ? c->nodes[d.tree_first].r : 0.0);
         rmax = std::max(rmax, r);
     }
     c->cell_size = std::max(2.0 * rmax, 1e-6);
@@ -470,7 +474,7 @@ int do_detect(am_ctx* c) {
     BodiesView B = bodies_view(c);
     TreeView T = tree_view(c);
     int* sc = c->d_scalars.p;
-    launch_tree_world_centres(s, B, T, c->d_node_body.p, c->n_nodes);
+    launch_tree_world_centres(s, B, T, c->d_node_body.p, c->d_node_sub.p, c->n_nodes);
     const int ng = (int)c->grid_bodies.size();
     const int tsize = c->hash_mask + 1;
     launch_bounds(s, B, T, c->d_grid.p, ng, c->d_sphere.p, c->cell_size, c->d_cell.p, c->hash_mask);
@@ -890,13 +894,22 @@ extern "C" {
 
 int am_scene_upload(am_ctx* c, const am_body_desc* bodies, int32_t n_bodies, const am_sphere_node* nodes,
                     int32_t n_nodes) {
+    return am_scene_upload_ex(c, bodies, n_bodies, nodes, n_nodes, nullptr, nullptr, 0);
+}
+
+int am_scene_upload_ex(am_ctx* c, const am_body_desc* bodies, int32_t n_bodies, const am_sphere_node* nodes,
+                       int32_t n_nodes, const am_body_desc* subs, const double* sub_b2c, int32_t n_subs) {
     int r = check_ctx(c);
     if (r) return r;
     if (n_bodies < 0 || n_nodes < 0 || (n_bodies > 0 && !bodies) || (n_nodes > 0 && !nodes))
         return fail(c, AM_ERR_INVALID, "invalid scene arrays");
+    if (n_subs < 0 || (n_subs > 0 && (!subs || !sub_b2c))) return fail(c, AM_ERR_INVALID, "invalid sub-body arrays");
     try {
         c->desc.assign(bodies, bodies + n_bodies);
         c->nodes.assign(nodes, nodes + n_nodes);
+        c->subs.assign(subs, subs + n_subs);
+        c->sub_b2c.assign(sub_b2c, sub_b2c + (size_t)n_subs * 12);
+        c->node_sub.assign(n_nodes, -1);
         c->n_ent = n_bodies;
         c->n_nodes = n_nodes;
         c->n_phys = 0;
@@ -934,6 +947,31 @@ int am_scene_upload(am_ctx* c, const am_body_desc* bodies, int32_t n_bodies, con
             }
             if ((d.geom == AM_GEOM_PLANE) && !(d.flags & AM_BODY_PINNED))
                 return fail(c, AM_ERR_INVALID, "planes must be pinned");
+            if (d.geom == AM_GEOM_COMPOSITE) {
+                if (d.tree_first < 0 || d.tree_count <= 0 || d.tree_first + d.tree_count > n_subs)
+                    return fail(c, AM_ERR_INVALID, "composite sub-body range out of bounds");
+                if (d.tree_count > 0x7fff) return fail(c, AM_ERR_INVALID, "composite has more than 32767 sub-bodies");
+                for (int q = 0; q < d.tree_count; q++) {
+                    const am_body_desc& sd = subs[d.tree_first + q];
+                    for (int k = 0; k < 12; k++)
+                        if (!std::isfinite(sub_b2c[(size_t)(d.tree_first + q) * 12 + k]))
+                            return fail(c, AM_ERR_NUMERIC, "non-finite sub-body transform");
+                    if (sd.geom == AM_GEOM_BOX) continue;
+                    if (sd.geom != AM_GEOM_TREE || sd.tree_first < 0 || sd.tree_count <= 0 ||
+                        sd.tree_first + sd.tree_count > n_nodes || sd.tree_count > 0xffff)
+                        return fail(c, AM_ERR_INVALID, "composite sub-bodies must be boxes or sphere trees");
+                    for (int k = 0; k < sd.tree_count; k++) {
+                        const am_sphere_node& s = nodes[sd.tree_first + k];
+                        if (s.n_children < 0 || (s.n_children > 0 &&
+                                                 (s.first_child <= 0 || s.first_child + s.n_children > sd.tree_count)))
+                            return fail(c, AM_ERR_INVALID, "tree child range out of bounds");
+                        c->node_body[sd.tree_first + k] = i;
+                        c->node_sub[sd.tree_first + k] = q;
+                    }
+                }
+            } else if (d.geom != AM_GEOM_BOX && d.geom != AM_GEOM_TREE && d.geom != AM_GEOM_PLANE) {
+                return fail(c, AM_ERR_INVALID, "unknown body geometry");
+            }
         }
         for (int g = 0; g < n_nodes; g++)
             if (c->node_body[g] < 0) c->node_body[g] = 0;
@@ -957,6 +995,9 @@ int am_scene_upload(am_ctx* c, const am_body_desc* bodies, int32_t n_bodies, con
         AM_HIP(c, upload(c, c->d_desc, c->desc));
         AM_HIP(c, upload(c, c->d_nodes, c->nodes));
         AM_HIP(c, upload(c, c->d_node_body, c->node_body));
+        AM_HIP(c, upload(c, c->d_node_sub, c->node_sub));
+        AM_HIP(c, upload(c, c->d_subs, c->subs));
+        AM_HIP(c, upload(c, c->d_sub_b2c, c->sub_b2c));
         AM_HIP(c, upload(c, c->d_minv, minv));
         size_t n = std::max(n_bodies, 1);
         AM_HIP(c, c->d_cw.ensure((size_t)std::max(n_nodes, 1) * 3));

@@ -102,13 +102,18 @@ struct am_ctx {
     int n_ent = 0, n_phys = 0, n_nodes = 0;
     std::vector<am_body_desc> desc;
     std::vector<am_sphere_node> nodes;
+    std::vector<am_body_desc> subs;     // sub-bodies of composites
+    std::vector<double> sub_b2c;        // 12 per sub-body
+    std::vector<int> node_sub;          // per node: sub-body (inside its composite) owning it, -1 otherwise
     std::vector<int> parent, slot, sub, slot_entity, member_first, member_count, members, flags;
     std::vector<int> planes, grid_bodies, node_body, top_pinned, top_sleeping, gravity_targets, integrate_targets;
     double cell_size = 1.0;
     int hash_mask = 0;
 
     // body buffers
-    DBuf<am_body_desc> d_desc;
+    DBuf<am_body_desc> d_desc, d_subs;
+    DBuf<double> d_sub_b2c;
+    DBuf<int> d_node_sub;
     DBuf<am_sphere_node> d_nodes;
     DBuf<double> d_cw, d_x, d_R, d_v, d_w, d_f, d_t, d_jinv, d_dv, d_minv;
     DBuf<int> d_flags, d_parent, d_slot, d_sub, d_slot_entity, d_member_first, d_members, d_planes, d_grid,

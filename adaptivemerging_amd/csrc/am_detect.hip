@@ -12,17 +12,25 @@ namespace am {
 
 // ---------------------------------------------------------------------------
 // BVSphere.updatecW for every node of every tree body (BVSphere.java:174-176).
-__global__ void k_tree_world_centres(BodiesView B, TreeView T, const int* node_body, int n_nodes) {
+// BVSphere.updatecW (BVSphere.java:174-176) with the owning body's transform;
+// a node of a composite's sphere sub-body uses the sub-body's transform
+// (RigidBodyGeomComposite.updateBodyPositionsFromParent).
+__global__ void k_tree_world_centres(BodiesView B, TreeView T, const int* node_body, const int* node_sub,
+                                     int n_nodes) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n_nodes) return;
     int b = node_body[g];
-    d3 c = xform(ldm(&B.R[b * 9]), ld3(&B.x[b * 3]), ld3(T.nodes[g].c));
+    int s = node_sub[g];
+    BodyPose P = s < 0 ? pose(B, b) : sub_pose(B, b, s);
+    d3 c = xform(P.R, P.x, ld3(T.nodes[g].c));
     st3(&T.cw[g * 3], c);
 }
 
-void launch_tree_world_centres(hipStream_t s, BodiesView B, TreeView T, const int* node_body, int n_nodes) {
+void launch_tree_world_centres(hipStream_t s, BodiesView B, TreeView T, const int* node_body, const int* node_sub,
+                               int n_nodes) {
     if (n_nodes <= 0) return;
-    hipLaunchKernelGGL(k_tree_world_centres, dim3((n_nodes + 255) / 256), dim3(256), 0, s, B, T, node_body, n_nodes);
+    hipLaunchKernelGGL(k_tree_world_centres, dim3((n_nodes + 255) / 256), dim3(256), 0, s, B, T, node_body, node_sub,
+                       n_nodes);
 }
 
 // ---------------------------------------------------------------------------
@@ -49,7 +57,8 @@ __global__ void k_bounds(BodiesView B, TreeView T, const int* order, int n, doub
     const am_body_desc& d = B.desc[b];
     d3 c;
     double r;
-    if (d.geom == AM_GEOM_BOX) {
+    if (d.geom == AM_GEOM_BOX || d.geom == AM_GEOM_COMPOSITE) {
+        // a composite's radius encloses all of its sub-bodies about its centre of mass
         c = ld3(&B.x[b * 3]);
         r = d.radius;
     } else {
@@ -161,8 +170,8 @@ __global__ void k_pairs(PairArgs A, int* counts, const int* offsets, uint64_t* k
         if (slot_p == slot_i) continue;
         if (skip_pair(A, slot_i, slot_p)) continue;
         d3 nrm = ld3(pd.plane_n);
-        if (di.geom == AM_GEOM_BOX) {
-            // first test of BoxPlane.dBoxPlane
+        if (di.geom == AM_GEOM_BOX || di.geom == AM_GEOM_COMPOSITE) {
+            // first test of BoxPlane.dBoxPlane (for a composite: its enclosing sphere)
             if (dot(ld3(&B.x[bi * 3]), nrm) + pd.plane_d > di.radius) continue;
         } else {
             // first (root) test of collideSphereTreeAndPlane

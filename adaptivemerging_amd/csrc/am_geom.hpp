@@ -17,15 +17,38 @@ struct BodyPose {
 
 AMD_INLINE BodyPose pose(const BodiesView& B, int b) { return BodyPose{ld3(&B.x[b * 3]), ldm(&B.R[b * 9])}; }
 
+// What a primitive routine collides: a body's geometry at its pose, or one
+// sub-body of a composite (RigidBodyGeomComposite) at the pose
+// updateBodyPositionsFromParent gives it.  Contacts are reported on the
+// physical body, with the sub-body folded into the bounding-volume id
+// (Contact.set, Contact.java:166-177, keeps it as csb for the warm start).
+struct Shape {
+    const am_body_desc* d;
+    BodyPose P;
+    int body;  // the contact body
+    int sub;   // sub-body inside the composite, -1 for a whole body
+};
+AMD_INLINE Shape body_shape(const BodiesView& B, int b) { return Shape{&B.desc[b], pose(B, b), b, -1}; }
+// RigidTransform3D.mult(A, B) (RigidTransform3D.java:193-202): t = R_A t_B + t_A, R = R_A R_B
+AMD_INLINE BodyPose sub_pose(const BodiesView& B, int b, int s) {
+    const int g = B.desc[b].tree_first + s;
+    const BodyPose Pb = pose(B, b);
+    return BodyPose{xform(Pb.R, Pb.x, ld3(&B.sub_b2c[(size_t)g * 12 + 9])), mmul(Pb.R, ldm(&B.sub_b2c[(size_t)g * 12]))};
+}
+AMD_INLINE Shape sub_shape(const BodiesView& B, int b, int s) {
+    return Shape{&B.sub_desc[B.desc[b].tree_first + s], sub_pose(B, b, s), b, s};
+}
+AMD_INLINE int enc_bv(const Shape& s, int bv) { return s.sub < 0 ? bv : AM_BV_SUB(s.sub, bv); }
+
 // ---------------------------------------------------------------------------
 // Box-plane: BoxPlane.dBoxPlane (collision/BoxPlane.java:27-51).  Contacts have
 // body1 = plane, normal = plane normal, info = corner index.
 template <class Sink>
-AMD_INLINE void box_plane(const BodiesView& B, int box, int plane, Sink& sink) {
-    const am_body_desc& bd = B.desc[box];
-    const am_body_desc& pd = B.desc[plane];
+AMD_INLINE void box_plane(const Shape& box, const Shape& plane, Sink& sink) {
+    const am_body_desc& bd = *box.d;
+    const am_body_desc& pd = *plane.d;
     d3 n = ld3(pd.plane_n);
-    BodyPose P = pose(B, box);
+    const BodyPose& P = box.P;
     if (dot(P.x, n) + pd.plane_d > bd.radius) return;
     d3 h = mk(0.5 * bd.size[0], 0.5 * bd.size[1], 0.5 * bd.size[2]);
 #pragma unroll 1
@@ -33,7 +56,7 @@ AMD_INLINE void box_plane(const BodiesView& B, int box, int plane, Sink& sink) {
         d3 q = mk((k & 4) ? -h.x : h.x, (k & 2) ? -h.y : h.y, (k & 1) ? -h.z : h.z);
         d3 qw = xform(P.R, P.x, q);
         double s = dot(qw, n) + pd.plane_d;
-        if (s < 0) sink.emit(plane, box, qw, n, AM_BV_NONE, AM_BV_NONE, k, s);
+        if (s < 0) sink.emit(plane.body, box.body, qw, n, enc_bv(plane, AM_BV_NONE), enc_bv(box, AM_BV_NONE), k, s);
     }
 }
 
@@ -136,11 +159,13 @@ AMD_INLINE double col_dot(const m33& A, int i, const m33& B, int j) {
 AMD_INLINE double vec_col(d3 v, const m33& M, int j) { return v.x * M.m[j] + v.y * M.m[3 + j] + v.z * M.m[6 + j]; }
 
 template <class Sink>
-AMD_INLINE void box_box(const BodiesView& B, int b1, int b2, Sink& sink) {
-    const am_body_desc& D1 = B.desc[b1];
-    const am_body_desc& D2 = B.desc[b2];
-    const BodyPose P1 = pose(B, b1);
-    const BodyPose P2 = pose(B, b2);
+AMD_INLINE void box_box(const Shape& S1, const Shape& S2, Sink& sink) {
+    const am_body_desc& D1 = *S1.d;
+    const am_body_desc& D2 = *S2.d;
+    const BodyPose& P1 = S1.P;
+    const BodyPose& P2 = S2.P;
+    const int b1 = S1.body, b2 = S2.body;
+    const int bv1 = enc_bv(S1, AM_BV_NONE), bv2 = enc_bv(S2, AM_BV_NONE);
     const m33& R1 = P1.R;
     const m33& R2 = P2.R;
     int info = 0;
@@ -226,7 +251,7 @@ AMD_INLINE void box_box(const BodiesView& B, int b1, int b2, Sink& sink) {
         pa = scale_add(alpha, ua, pa);
         pb = scale_add(beta, ub, pb);
         d3 pos = mk((pa.x + pb.x) * 0.5, (pa.y + pb.y) * 0.5, (pa.z + pb.z) * 0.5);
-        sink.emit(b1, b2, pos, normal, AM_BV_NONE, AM_BV_NONE, info++, -depth);
+        sink.emit(b1, b2, pos, normal, bv1, bv2, info++, -depth);
         return;
     }
 
@@ -310,7 +335,7 @@ AMD_INLINE void box_box(const BodiesView& B, int b1, int b2, Sink& sink) {
         double dep = san - (normal2.x * point.x + normal2.y * point.y + normal2.z * point.z);
         if (dep >= 0) {
             d3 pos = mk(point.x + pa.x, point.y + pa.y, point.z + pa.z);
-            sink.emit(b1, b2, pos, normal, AM_BV_NONE, AM_BV_NONE, info++, -dep);
+            sink.emit(b1, b2, pos, normal, bv1, bv2, info++, -dep);
         }
     }
 }
@@ -340,9 +365,9 @@ AMD_INLINE void bs_test2(double depth, double px, double py, double pz, double n
 }
 
 template <class Sink>
-AMD_INLINE void box_sphere(const BodiesView& B, int box, int other, d3 c, double r, int bv_other, Sink& sink) {
-    const am_body_desc& bd = B.desc[box];
-    BodyPose P = pose(B, box);
+AMD_INLINE void box_sphere(const Shape& box, const Shape& other, d3 c, double r, int bv_other, Sink& sink) {
+    const am_body_desc& bd = *box.d;
+    const BodyPose& P = box.P;
     d3 p = mk(0.5 * bd.size[0], 0.5 * bd.size[1], 0.5 * bd.size[2]);
     d3 cB = inv_xform(P.R, P.x, c);
     BSResult t;
@@ -413,13 +438,13 @@ AMD_INLINE void box_sphere(const BodiesView& B, int box, int other, d3 c, double
         d3 pw = xform(P.R, P.x, t.pos);
         d3 nw = normalized(mul(P.R, t.normal));
         (void)bv_other;
-        sink.emit(box, other, pw, nw, AM_BV_NONE, AM_BV_NONE, 0, t.depth);
+        sink.emit(box.body, other.body, pw, nw, enc_bv(box, AM_BV_NONE), enc_bv(other, AM_BV_NONE), 0, t.depth);
     }
 }
 
-AMD_INLINE bool box_sphere_test(const BodiesView& B, int box, d3 c, double r) {
-    const am_body_desc& bd = B.desc[box];
-    BodyPose P = pose(B, box);
+AMD_INLINE bool box_sphere_test(const Shape& box, d3 c, double r) {
+    const am_body_desc& bd = *box.d;
+    const BodyPose& P = box.P;
     if (distance(c, P.x) > bd.radius + r) return false;
     d3 p = mk(0.5 * bd.size[0], 0.5 * bd.size[1], 0.5 * bd.size[2]);
     d3 cB = inv_xform(P.R, P.x, c);
@@ -500,9 +525,9 @@ AMD_INLINE NodeRef node_ref(const TreeView& T, int g) {
 // collideSphereTreeAndPlane: contacts body1 = tree body, body2 = plane,
 // bv2 = the plane's dummy volume, violation = d.
 template <class Sink>
-AMD_INLINE void tree_plane(const BodiesView& B, const TreeView& T, int body, int plane, Sink& sink, int* overflow) {
-    const am_body_desc& bd = B.desc[body];
-    const am_body_desc& pd = B.desc[plane];
+AMD_INLINE void tree_plane(const TreeView& T, const Shape& body, const Shape& plane, Sink& sink, int* overflow) {
+    const am_body_desc& bd = *body.d;
+    const am_body_desc& pd = *plane.d;
     d3 n = ld3(pd.plane_n);
     d3 pp = ld3(pd.plane_p);
     int stack[kTreeStack];
@@ -519,7 +544,7 @@ AMD_INLINE void tree_plane(const BodiesView& B, const TreeView& T, int body, int
                 nm = scale(val, n);
                 d3 cw = sub(s.c, nm);
                 d3 nn = mk(-1.0 * n.x, -1.0 * n.y, -1.0 * n.z);
-                sink.emit(body, plane, cw, nn, node, AM_BV_PLANE, 0, d);
+                sink.emit(body.body, plane.body, cw, nn, enc_bv(body, node), AM_BV_PLANE, 0, d);
             } else {
                 if (sp + s.count > kTreeStack) {
                     atomicOr(overflow, 2);
@@ -533,15 +558,15 @@ AMD_INLINE void tree_plane(const BodiesView& B, const TreeView& T, int body, int
 
 // collideBoxAndSphereTree: contacts body1 = box, body2 = tree body.
 template <class Sink>
-AMD_INLINE void box_tree(const BodiesView& B, const TreeView& T, int box, int body2, Sink& sink, int* overflow) {
-    const am_body_desc& bd = B.desc[body2];
+AMD_INLINE void box_tree(const TreeView& T, const Shape& box, const Shape& body2, Sink& sink, int* overflow) {
+    const am_body_desc& bd = *body2.d;
     int stack[kTreeStack];
     int sp = 0;
     stack[sp++] = 0;
     while (sp > 0) {
         int node = stack[--sp];
         NodeRef s = node_ref(T, bd.tree_first + node);
-        if (!box_sphere_test(B, box, s.c, s.r)) continue;
+        if (!box_sphere_test(box, s.c, s.r)) continue;
         if (s.count != 0) {
             if (sp + s.count > kTreeStack) {
                 atomicOr(overflow, 2);
@@ -549,15 +574,15 @@ AMD_INLINE void box_tree(const BodiesView& B, const TreeView& T, int box, int bo
             }
             for (int k = s.count - 1; k >= 0; k--) stack[sp++] = s.first + k;
         } else {
-            box_sphere(B, box, body2, s.c, s.r, node, sink);
+            box_sphere(box, body2, s.c, s.r, node, sink);
         }
     }
 }
 
 // collideSphereTrees + processCollision.
 template <class Sink>
-AMD_INLINE void tree_tree(const BodiesView& B, const TreeView& T, int b1, int b2, Sink& sink, int* overflow) {
-    const int f1 = B.desc[b1].tree_first, f2 = B.desc[b2].tree_first;
+AMD_INLINE void tree_tree(const TreeView& T, const Shape& S1, const Shape& S2, Sink& sink, int* overflow) {
+    const int f1 = S1.d->tree_first, f2 = S2.d->tree_first;
     // stack entries pack node1 (low 16 bits) and node2 (high 16 bits)
     int stack[kTreeStack];
     int sp = 0;
@@ -577,7 +602,7 @@ AMD_INLINE void tree_tree(const BodiesView& B, const TreeView& T, int b1, int b2
                 d3 p = mk((1 - alpha) * s1.c.x + alpha * s2.c.x, (1 - alpha) * s1.c.y + alpha * s2.c.y,
                           (1 - alpha) * s1.c.z + alpha * s2.c.z);
                 d3 nrm = normalized(sub(s2.c, s1.c));
-                sink.emit(b1, b2, p, nrm, n1, n2, 0, dist - sum);
+                sink.emit(S1.body, S2.body, p, nrm, enc_bv(S1, n1), enc_bv(S2, n2), 0, dist - sum);
             }
             continue;
         }
@@ -595,22 +620,39 @@ AMD_INLINE void tree_tree(const BodiesView& B, const TreeView& T, int b1, int b2
     }
 }
 
-// narrowPhase dispatch on two physical bodies (CollisionProcessor.java:707-757).
+// narrowPhase on two primitive shapes (CollisionProcessor.java:723-750).
 template <class Sink>
-AMD_INLINE void narrow_dispatch(const BodiesView& B, const TreeView& T, int b1, int b2, Sink& sink, int* overflow) {
-    const int g1 = B.desc[b1].geom, g2 = B.desc[b2].geom;
+AMD_INLINE void shape_dispatch(const TreeView& T, const Shape& s1, const Shape& s2, Sink& sink, int* overflow) {
+    const int g1 = s1.d->geom, g2 = s2.d->geom;
     if (g1 == AM_GEOM_PLANE) {
         if (g2 == AM_GEOM_PLANE) return;
-        if (g2 == AM_GEOM_BOX) box_plane(B, b2, b1, sink);
-        else tree_plane(B, T, b2, b1, sink, overflow);
+        if (g2 == AM_GEOM_BOX) box_plane(s2, s1, sink);
+        else tree_plane(T, s2, s1, sink, overflow);
     } else if (g1 == AM_GEOM_BOX) {
-        if (g2 == AM_GEOM_PLANE) box_plane(B, b1, b2, sink);
-        else if (g2 == AM_GEOM_BOX) box_box(B, b1, b2, sink);
-        else box_tree(B, T, b1, b2, sink, overflow);
+        if (g2 == AM_GEOM_PLANE) box_plane(s1, s2, sink);
+        else if (g2 == AM_GEOM_BOX) box_box(s1, s2, sink);
+        else box_tree(T, s1, s2, sink, overflow);
     } else {
-        if (g2 == AM_GEOM_PLANE) tree_plane(B, T, b1, b2, sink, overflow);
-        else if (g2 == AM_GEOM_BOX) box_tree(B, T, b2, b1, sink, overflow);
-        else tree_tree(B, T, b1, b2, sink, overflow);
+        if (g2 == AM_GEOM_PLANE) tree_plane(T, s1, s2, sink, overflow);
+        else if (g2 == AM_GEOM_BOX) box_tree(T, s2, s1, sink, overflow);
+        else tree_tree(T, s1, s2, sink, overflow);
+    }
+}
+
+// narrowPhase dispatch on two physical bodies (CollisionProcessor.java:707-757):
+// a composite on either side collides each of its sub-bodies in turn, the
+// first body's sub-bodies outermost (:711-722).
+template <class Sink>
+AMD_INLINE void narrow_dispatch(const BodiesView& B, const TreeView& T, int b1, int b2, Sink& sink, int* overflow) {
+    const bool c1 = B.desc[b1].geom == AM_GEOM_COMPOSITE, c2 = B.desc[b2].geom == AM_GEOM_COMPOSITE;
+    if (!c1 && !c2) {
+        shape_dispatch(T, body_shape(B, b1), body_shape(B, b2), sink, overflow);
+        return;
+    }
+    const int n1 = c1 ? B.desc[b1].tree_count : 1, n2 = c2 ? B.desc[b2].tree_count : 1;
+    for (int i = 0; i < n1; i++) {
+        const Shape s1 = c1 ? sub_shape(B, b1, i) : body_shape(B, b1);
+        for (int j = 0; j < n2; j++) shape_dispatch(T, s1, c2 ? sub_shape(B, b2, j) : body_shape(B, b2), sink, overflow);
     }
 }
 

@@ -28,6 +28,8 @@ struct BodiesView {
     const int* parent;  // collection entity or -1
     const int* slot;    // position of the body's top-level entity in the broad-phase list
     const int* sub;     // index inside its collection (0 otherwise)
+    const am_body_desc* sub_desc;  // sub-bodies of composite bodies
+    const double* sub_b2c;         // their transforms into the composite's frame: R (9) then t (3)
 };
 
 struct TreeView {
@@ -152,7 +154,8 @@ struct PairArgs {
     const int* top_pinned;   // per slot
     const int* top_sleeping; // per slot
 };
-void launch_tree_world_centres(hipStream_t s, BodiesView B, TreeView T, const int* node_body, int n_nodes);
+void launch_tree_world_centres(hipStream_t s, BodiesView B, TreeView T, const int* node_body, const int* node_sub,
+                               int n_nodes);
 void launch_bounds(hipStream_t s, BodiesView B, TreeView T, const int* order, int n, double* sphere,
                    double cell, int* cell_key, int hash_mask);
 void launch_pairs_impl(hipStream_t s, bool write, const PairArgs& A, int* counts, const int* offsets, uint64_t* keys,

@@ -12,7 +12,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import (AM_BODY_PINNED, AM_GEOM_BOX, AM_GEOM_COLLECTION, AM_GEOM_PLANE, AM_GEOM_TREE, AM_SPRING_BODYBODY,
+from ._lib import (AM_BODY_PINNED, AM_GEOM_BOX, AM_GEOM_COLLECTION, AM_GEOM_COMPOSITE, AM_GEOM_PLANE, AM_GEOM_TREE, AM_SPRING_BODYBODY,
                    AM_SPRING_WORLD, AM_SPRING_ZERO, BODY_DTYPE, NODE_DTYPE, SPRING_DTYPE)
 
 
@@ -55,6 +55,8 @@ class Scene:
     w: list = field(default_factory=list)
     names: list = field(default_factory=list)
     springs: list = field(default_factory=list)      # list of SPRING_DTYPE records
+    subs: list = field(default_factory=list)         # sub-bodies of composites (BODY_DTYPE records)
+    sub_b2c: list = field(default_factory=list)      # their transforms into the composite frame (12 each)
 
     # -- body builders -------------------------------------------------
     def _append(self, rec, x, R, v, w, name):
@@ -213,6 +215,116 @@ class Scene:
         return self.add_tree(centres, radii, children, x, R=R, v=v, w=w, mass=mb["mass"], inertia=mb["inertia"],
                              bbox=mb["bbox"], friction=friction, restitution=restitution, scene=scene, name=name,
                              pinned=pinned)
+
+    def add_composite(self, parts, x=(0.0, 0.0, 0.0), R=None, v=None, w=None, friction=0.8, restitution=0.0,
+                      pinned=False, scene=0, name=None) -> int:
+        """XMLParser.createComposite (XMLParser.java:180-353): a rigid body
+        made of boxes and spheres.  parts: dicts with kind "box" (size) or
+        "sphere" (r), density (default 1) and the part's pose in the
+        composite's file frame, x and R.  The composite's mass is the sum, its
+        centre of mass the mass-weighted mean, its inertia the sum of the
+        parts' world-aligned inertias plus m (|d|^2 I - d d^T) for each part's
+        offset d; its bounding box spans the parts' boxes (upper corner
+        starting from Double.MIN_VALUE, as in the reference) and its pose is
+        x + centre of mass.  Each part keeps its transform relative to the
+        centre of mass."""
+        if not parts:
+            raise ValueError("a composite needs at least one part")
+        built = []
+        for pt in parts:
+            dens = float(pt.get("density", 1.0))
+            Rb = np.eye(3) if pt.get("R") is None else np.asarray(pt["R"], dtype=np.float64).reshape(3, 3)
+            xb = [float(t) for t in pt.get("x", (0.0, 0.0, 0.0))]
+            rec = np.zeros((), dtype=BODY_DTYPE)
+            if pt["kind"] == "box":
+                sz = np.asarray(pt["size"], dtype=np.float64)
+                m = sz[0] * sz[1] * sz[2] * dens
+                I0 = [1.0 / 12 * m * (sz[1] * sz[1] + sz[2] * sz[2]), 1.0 / 12 * m * (sz[0] * sz[0] + sz[2] * sz[2]),
+                      1.0 / 12 * m * (sz[0] * sz[0] + sz[1] * sz[1])]
+                h = 0.5 * sz
+                bb = [[sx * h[0], sy * h[1], sz_ * h[2]] for sx in (-1, 1) for sy in (-1, 1) for sz_ in (-1, 1)]
+                rec["geom"] = AM_GEOM_BOX
+                rec["size"] = sz
+                rec["radius"] = math.sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2])
+            elif pt["kind"] == "sphere":
+                r = float(pt["r"])
+                m = 4.0 / 3 * math.pi * r * r * r * dens
+                I0 = [2.0 / 5 * m * r * r] * 3
+                bb = [[sx * r, sy * r, sz_ * r] for sx in (-1, 1) for sy in (-1, 1) for sz_ in (-1, 1)]
+                rec["geom"] = AM_GEOM_TREE
+                rec["tree_first"] = len(self.nodes)
+                rec["tree_count"] = 1
+                rec["radius"] = r
+                node = np.zeros((), dtype=NODE_DTYPE)
+                node["r"] = r
+                node["first_child"] = -1
+                self.nodes.append(node)
+            else:
+                raise ValueError(f"composite parts are boxes or spheres, not {pt['kind']!r}")
+            rec["bbox"] = np.asarray(bb).reshape(24)
+            rec["scene"] = scene
+            # massAngular as updateRotationalInertiaFromTransformation leaves it: R I0 R^T
+            Iw = Rb @ np.diag(I0) @ Rb.T
+            built.append((rec, m, xb, Rb, Iw, bb))
+        mass = 0.0
+        com = [0.0, 0.0, 0.0]
+        for _, m, xb, _, _, _ in built:
+            mass += m
+            com = [m * xb[a] + com[a] for a in range(3)]
+        inv = 1.0 / mass
+        com = [com[a] * inv for a in range(3)]
+        I = np.zeros((3, 3))
+        for _, m, xb, _, Iw, _ in built:
+            I = I + Iw
+            d = [xb[0] - com[0], xb[1] - com[1], xb[2] - com[2]]
+            op = np.array([[d[1] * d[1] + d[2] * d[2], -d[0] * d[1], -d[0] * d[2]],
+                           [-d[1] * d[0], d[0] * d[0] + d[2] * d[2], -d[1] * d[2]],
+                           [-d[2] * d[0], -d[2] * d[1], d[0] * d[0] + d[1] * d[1]]]) * m
+            I = I + op
+        ll = [1.7976931348623157e308] * 3
+        ur = [5e-324] * 3
+        for _, _, xb, Rb, _, bb in built:
+            for q in bb:
+                pw = [Rb[i, 0] * q[0] + Rb[i, 1] * q[1] + Rb[i, 2] * q[2] + xb[i] for i in range(3)]
+                for a in range(3):
+                    ll[a] = min(pw[a], ll[a])
+                    ur[a] = max(pw[a], ur[a])
+        ll = [ll[a] - com[a] for a in range(3)]
+        ur = [ur[a] - com[a] for a in range(3)]
+        bbox = [ll, [ll[0], ll[1], ur[2]], [ll[0], ur[1], ll[2]], [ll[0], ur[1], ur[2]], ur, [ur[0], ll[1], ll[2]],
+                [ur[0], ll[1], ur[2]], [ur[0], ur[1], ll[2]]]
+        first = len(self.subs)
+        radius = 0.0
+        for rec, m, xb, Rb, _, _ in built:
+            t = [xb[0] - com[0], xb[1] - com[1], xb[2] - com[2]]
+            self.subs.append(rec)
+            self.sub_b2c.append(list(Rb.reshape(9)) + t)
+            radius = max(radius, math.sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]) + float(rec["radius"]))
+        crec = np.zeros((), dtype=BODY_DTYPE)
+        crec["geom"] = AM_GEOM_COMPOSITE
+        crec["scene"] = scene
+        crec["tree_first"] = first
+        crec["tree_count"] = len(built)
+        # enclosing radius about the centre of mass: the broad phase's bound
+        crec["radius"] = radius * (1 + 1e-12)
+        crec["bbox"] = np.asarray(bbox).reshape(24)
+        crec["friction"] = friction
+        crec["restitution"] = restitution
+        if pinned:
+            crec["flags"] = AM_BODY_PINNED
+        else:
+            crec["mass"] = mass
+            crec["minv"] = 1.0 / mass
+            crec["inertia0"] = I.reshape(9)
+            crec["jinv0"] = np.linalg.inv(I).reshape(9)
+        pos = [float(t) for t in x]
+        return self._append(crec, [pos[a] + com[a] for a in range(3)], R, v, w, name)
+
+    def sub_arrays(self):
+        """(sub-bodies, their transforms (n, 12)) for am_scene_upload_ex."""
+        s = np.array(self.subs, dtype=BODY_DTYPE) if self.subs else np.zeros(0, dtype=BODY_DTYPE)
+        t = np.ascontiguousarray(np.array(self.sub_b2c, dtype=np.float64).reshape(len(self.subs), 12))
+        return s, t
 
     # -- springs (Spring.java:50-104) -----------------------------------
     def body_point_world(self, body: int, pB, x=None, R=None):

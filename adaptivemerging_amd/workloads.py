@@ -9,7 +9,7 @@ import math
 
 import numpy as np
 
-from ._lib import AM_GEOM_TREE, BODY_DTYPE
+from ._lib import AM_GEOM_COMPOSITE, AM_GEOM_TREE, BODY_DTYPE
 from .scene import Scene, random_rotation, rot_axis_angle
 
 
@@ -274,11 +274,21 @@ def batch(scenes) -> Scene:
     out = Scene()
     for k, sc in enumerate(scenes):
         node_off = len(out.nodes)
+        sub_off = len(out.subs)
         for i, rec in enumerate(sc.bodies):
             r = np.array(rec, dtype=BODY_DTYPE).copy()
             r["scene"] = k
             if int(r["geom"]) == AM_GEOM_TREE:
                 r["tree_first"] = int(r["tree_first"]) + node_off
+            elif int(r["geom"]) == AM_GEOM_COMPOSITE:
+                r["tree_first"] = int(r["tree_first"]) + sub_off
             out._append(r, sc.x[i], sc.R[i], sc.v[i], sc.w[i], f"s{k}_{sc.names[i]}")
+        for rec, t in zip(sc.subs, sc.sub_b2c):
+            r = np.array(rec, dtype=BODY_DTYPE).copy()
+            r["scene"] = k
+            if int(r["geom"]) == AM_GEOM_TREE:
+                r["tree_first"] = int(r["tree_first"]) + node_off
+            out.subs.append(r)
+            out.sub_b2c.append(list(t))
         out.nodes.extend(sc.nodes)
     return out

@@ -11,10 +11,10 @@ Follows XMLParser (src/mergingBodies3D/XMLParser.java):
 Springs (:596-619, Spring.java) load with the attachment geometry their
 constructors compute from the pose the body has when the <spring> tag is
 read.  Mesh bodies (createMesh :449-538) load their mass properties from
-the OBJ file and their sphere tree from the .sph file.  Composite bodies
-(createComposite :177-353) and spinners have no counterpart in this build;
-a scene that uses them raises SceneFormatError rather than loading into
-different dynamics.
+the OBJ file and their sphere tree from the .sph file; composite bodies
+(createComposite :180-353) their box and sphere parts.  Spinners have no
+counterpart in this build; a scene that uses them raises SceneFormatError
+rather than loading into different dynamics.
 Sphere-tree geometry for hand-built bodies comes from .sph files through
 scene.load_sph (the format of :500-535).
 """
@@ -233,7 +233,35 @@ def parse_scene(source: str, *, ignore_springs: bool = False, scene_index: int =
                                scene=scene_index, name=name)
             _add_springs(scene, b, com, name, posed_planes)
         elif kind == "composite":
-            raise SceneFormatError(f"body {name}: composite bodies are not supported by this build")
+            parts = []
+            for child in node:
+                if child.tag.lower() != "body":
+                    continue
+                ckind = child.attrib.get("type", "").lower()
+                cname = child.attrib.get("name", "")
+                if ckind in ("plane", "mesh"):
+                    # createComposite prints an error and leaves them out (XMLParser.java:218-226)
+                    warnings.warn(f"composite {name}: {ckind} part {cname!r} left out, as the reference does")
+                    continue
+                part = _Common(child, cname, True)
+                if part.springs or any(c.tag.lower() == "spring" for c in child):
+                    warnings.warn(f"composite {name}: springs on part {cname!r} act on no integrated body; dropped")
+                pd = dict(x=part.pos, R=part.R, density=float(child.attrib.get("density", 1.0)))
+                if ckind == "box":
+                    dim = np.array(_t3d(child.attrib["dim"]))
+                    if "scale" in child.attrib:
+                        dim = dim * float(child.attrib["scale"])
+                    pd.update(kind="box", size=dim)
+                elif ckind == "sphere":
+                    pd.update(kind="sphere", r=float(child.attrib["r"]))
+                else:
+                    continue
+                parts.append(pd)
+            if not parts:
+                raise SceneFormatError(f"body {name}: composite without box or sphere parts")
+            b = scene.add_composite(parts, com.pos, R=com.R, v=com.v, w=com.w, friction=com.friction,
+                                    restitution=com.restitution, pinned=com.pinned, scene=scene_index, name=name)
+            _add_springs(scene, b, com, name, posed_planes)
         else:
             raise SceneFormatError(f"body {name}: unknown type {kind!r}")
     return scene, collision

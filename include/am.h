@@ -55,7 +55,9 @@ enum am_geom {
     AM_GEOM_BOX = 0,        /* RigidBodyGeomBox */
     AM_GEOM_TREE = 1,       /* sphere tree: RigidBodyGeomSphere (1 leaf) or mesh (.sph) */
     AM_GEOM_PLANE = 2,      /* PlaneRigidBody, always pinned */
-    AM_GEOM_COLLECTION = 3  /* RigidCollection entity (no own geometry) */
+    AM_GEOM_COLLECTION = 3, /* RigidCollection entity (no own geometry) */
+    AM_GEOM_COMPOSITE = 4   /* RigidBodyGeomComposite: tree_first/tree_count select its sub-bodies
+                               in the sub-body table of am_scene_upload_ex */
 };
 
 enum am_body_flags {
@@ -71,6 +73,11 @@ enum am_contact_state { AM_STATE_BROKEN = 0, AM_STATE_ONEDGE = 1, AM_STATE_CLEAR
  * plane's dummy bounding volume (CollisionProcessor.java:817). */
 #define AM_BV_NONE (-1)
 #define AM_BV_PLANE (-2)
+/* A contact on sub-body s of a composite (Contact.csb1/csb2) carries that
+ * sub-body in its bv id: AM_BV_SUB(s, bv) with bv the sub-body's own id
+ * (AM_BV_NONE for a box, its leaf node for a sphere); decode with
+ * s = (-3 - id) >> 16, bv = ((-3 - id) & 0xffff) - 1. */
+#define AM_BV_SUB(s, bv) (-3 - ((s) << 16) - ((bv) + 1))
 
 /* Immutable description of one physical body (RigidBody fields that do not
  * change per step; XMLParser.createBox/createSphere/createPlane). */
@@ -272,6 +279,13 @@ int am_params_set(am_ctx* ctx, const am_params* params);
  * Replaces XMLParser.parse -> system.bodies (XMLParser.java:40-67). */
 int am_scene_upload(am_ctx* ctx, const am_body_desc* bodies, int32_t n_bodies,
                     const am_sphere_node* nodes, int32_t n_nodes);
+/* am_scene_upload with composite bodies: subs[n_subs] are the sub-bodies
+ * (boxes, or spheres as one-node trees in the node array) of every
+ * AM_GEOM_COMPOSITE body, sub_b2c[12 * n_subs] their transforms into the
+ * composite's frame (RigidBody.transformB2C, XMLParser.java:336-343): R
+ * row-major, then t. */
+int am_scene_upload_ex(am_ctx* ctx, const am_body_desc* bodies, int32_t n_bodies, const am_sphere_node* nodes,
+                       int32_t n_nodes, const am_body_desc* subs, const double* sub_b2c, int32_t n_subs);
 /* Pose and velocity of every entity (n = bodies + collections).  jinv is
  * recomputed as R jinv0 R^T (RigidBody.updateRotationalInertiaFromTransformation,
  * RigidBody.java:311-321).  R may be NULL to keep the current rotation. */

@@ -115,6 +115,9 @@ struct orc_world {
     /* sorted lookup over old contacts */
     uint64_t* okey; int32_t* oidx; int32_t nokey;
     am_spring* springs; int32_t n_springs;
+    /* sub-bodies of composites (RigidBodyGeomComposite.bodies) and their
+     * transforms into the composite's frame (RigidBody.transformB2C) */
+    am_body_desc* subs; double* sub_b2c; int32_t n_subs;
     am_params p;
 };
 
@@ -138,7 +141,19 @@ static void default_params(am_params* p) {
 }
 
 orc_world* orc_create(const am_body_desc* bodies, int32_t n, const am_sphere_node* nodes, int32_t n_nodes) {
+    return orc_create_ex(bodies, n, nodes, n_nodes, NULL, NULL, 0);
+}
+
+orc_world* orc_create_ex(const am_body_desc* bodies, int32_t n, const am_sphere_node* nodes, int32_t n_nodes,
+                         const am_body_desc* subs, const double* sub_b2c, int32_t n_subs) {
     orc_world* w = (orc_world*)calloc(1, sizeof *w);
+    w->n_subs = n_subs;
+    w->subs = (am_body_desc*)malloc((size_t)(n_subs ? n_subs : 1) * sizeof(am_body_desc));
+    w->sub_b2c = (double*)malloc((size_t)(n_subs ? n_subs : 1) * 12 * sizeof(double));
+    if (n_subs) {
+        memcpy(w->subs, subs, (size_t)n_subs * sizeof(am_body_desc));
+        memcpy(w->sub_b2c, sub_b2c, (size_t)n_subs * 12 * sizeof(double));
+    }
     w->n = n;
     w->desc = (am_body_desc*)malloc((size_t)(n ? n : 1) * sizeof(am_body_desc));
     memcpy(w->desc, bodies, (size_t)n * sizeof(am_body_desc));
@@ -172,7 +187,7 @@ void orc_destroy(orc_world* w) {
     free(w->desc); free(w->nodes); free(w->cw);
     free(w->x); free(w->R); free(w->v); free(w->w); free(w->f); free(w->t); free(w->jinv); free(w->dv);
     free(w->parent); free(w->top); free(w->mem_first); free(w->mem_count); free(w->members);
-    free(w->c); free(w->old); free(w->okey); free(w->oidx); free(w->springs);
+    free(w->c); free(w->old); free(w->okey); free(w->oidx); free(w->springs); free(w->subs); free(w->sub_b2c);
     free(w);
 }
 
@@ -430,15 +445,47 @@ static void emit(orc_world* w, int32_t b1, int32_t b2, const double* pW, const d
     w->nc++;
 }
 
+/* What a primitive collides: a body's geometry at its pose, or a sub-body of
+ * a composite at the pose RigidBodyGeomComposite.updateBodyPositionsFromParent
+ * gives it (RigidTransform3D.mult, RigidTransform3D.java:193-202).  Contacts
+ * go to the physical body with the sub-body in the bv id (Contact.set keeps it
+ * as csb, Contact.java:166-177). */
+typedef struct {
+    const am_body_desc* d;
+    double x[3], R[9];
+    int32_t body, sub;
+} oshape;
+static oshape body_shape(const orc_world* w, int32_t b) {
+    oshape s;
+    s.d = &w->desc[b];
+    memcpy(s.x, &w->x[b * 3], sizeof s.x);
+    memcpy(s.R, &w->R[b * 9], sizeof s.R);
+    s.body = b; s.sub = -1;
+    return s;
+}
+static void sub_pose(const orc_world* w, int32_t b, int32_t q, double* x, double* R) {
+    const double* T = &w->sub_b2c[(size_t)(w->desc[b].tree_first + q) * 12];
+    xform_point(x, &w->R[b * 9], &w->x[b * 3], T + 9);
+    mat_mul(R, &w->R[b * 9], T);
+}
+static oshape sub_shape(const orc_world* w, int32_t b, int32_t q) {
+    oshape s;
+    s.d = &w->subs[w->desc[b].tree_first + q];
+    sub_pose(w, b, q, s.x, s.R);
+    s.body = b; s.sub = q;
+    return s;
+}
+static int32_t enc_bv(const oshape* s, int32_t bv) { return s->sub < 0 ? bv : AM_BV_SUB(s->sub, bv); }
+
 /* ------------------------------------------------------------------ */
 /* Box-plane: BoxPlane.dBoxPlane (collision/BoxPlane.java:27-51)        */
 /* ------------------------------------------------------------------ */
-int32_t orc_box_plane(orc_world* w, int32_t box, int32_t plane) {
-    const am_body_desc* bd = &w->desc[box];
-    const am_body_desc* pd = &w->desc[plane];
+static int32_t box_plane_s(orc_world* w, const oshape* box, const oshape* plane) {
+    const am_body_desc* bd = box->d;
+    const am_body_desc* pd = plane->d;
     const double* n = pd->plane_n;
-    const double* x = &w->x[box * 3];
-    const double* R = &w->R[box * 9];
+    const double* x = box->x;
+    const double* R = box->R;
     if (dot3(x, n) + pd->plane_d > bd->radius) return 0;
     double h[3] = {0.5 * bd->size[0], 0.5 * bd->size[1], 0.5 * bd->size[2]};
     int32_t cnt = 0;
@@ -447,9 +494,16 @@ int32_t orc_box_plane(orc_world* w, int32_t box, int32_t plane) {
         double qw[3];
         xform_point(qw, R, x, q);
         double s = dot3(qw, n) + pd->plane_d;
-        if (s < 0) { emit(w, plane, box, qw, n, AM_BV_NONE, AM_BV_NONE, k, s); cnt++; }
+        if (s < 0) {
+            emit(w, plane->body, box->body, qw, n, enc_bv(plane, AM_BV_NONE), enc_bv(box, AM_BV_NONE), k, s);
+            cnt++;
+        }
     }
     return cnt;
+}
+int32_t orc_box_plane(orc_world* w, int32_t box, int32_t plane) {
+    oshape a = body_shape(w, box), b = body_shape(w, plane);
+    return box_plane_s(w, &a, &b);
 }
 
 /* ------------------------------------------------------------------ */
@@ -525,16 +579,18 @@ static int bb_tst2(double e1, double e2, double n1, double n2, double n3, int cc
     return 1;
 }
 
-int32_t orc_box_box(orc_world* w, int32_t b1, int32_t b2) {
-    const double* p1 = &w->x[b1 * 3];
-    const double* R1 = &w->R[b1 * 9];
-    const double* p2 = &w->x[b2 * 3];
-    const double* R2 = &w->R[b2 * 9];
-    const double* side1 = w->desc[b1].size;
-    const double* side2 = w->desc[b2].size;
+static int32_t box_box_s(orc_world* w, const oshape* S1, const oshape* S2) {
+    const double* p1 = S1->x;
+    const double* R1 = S1->R;
+    const double* p2 = S2->x;
+    const double* R2 = S2->R;
+    const double* side1 = S1->d->size;
+    const double* side2 = S2->d->size;
+    const int32_t b1 = S1->body, b2 = S2->body;
+    const int32_t bv1 = enc_bv(S1, AM_BV_NONE), bv2 = enc_bv(S2, AM_BV_NONE);
     int info = 0;
     double p[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
-    if (len3(p) > w->desc[b1].radius + w->desc[b2].radius) return 0;
+    if (len3(p) > S1->d->radius + S2->d->radius) return 0;
     double pp[3];
     mat_tvec(pp, R1, p);
     double A[3] = {0.5 * side1[0], 0.5 * side1[1], 0.5 * side1[2]};
@@ -606,7 +662,7 @@ int32_t orc_box_box(orc_world* w, int32_t b1, int32_t b2) {
         for (int i = 0; i < 3; i++) pb[i] = beta * ub[i] + pb[i];
         double pos[3];
         for (int i = 0; i < 3; i++) pos[i] = (pa[i] + pb[i]) * 0.5;
-        emit(w, b1, b2, pos, normal, AM_BV_NONE, AM_BV_NONE, info++, -depth);
+        emit(w, b1, b2, pos, normal, bv1, bv2, info++, -depth);
         return 1;
     }
 
@@ -679,9 +735,13 @@ int32_t orc_box_box(orc_world* w, int32_t b1, int32_t b2) {
     for (int j = 0; j < cnum; j++) {
         double pos[3];
         for (int i = 0; i < 3; i++) pos[i] = point[j * 3 + i] + pa[i];
-        emit(w, b1, b2, pos, normal, AM_BV_NONE, AM_BV_NONE, info++, -dep[j]);
+        emit(w, b1, b2, pos, normal, bv1, bv2, info++, -dep[j]);
     }
     return cnum;
+}
+int32_t orc_box_box(orc_world* w, int32_t b1, int32_t b2) {
+    oshape a = body_shape(w, b1), b = body_shape(w, b2);
+    return box_box_s(w, &a, &b);
 }
 
 /* ------------------------------------------------------------------ */
@@ -716,10 +776,10 @@ static void bs_test2(double depth, double px, double py, double pz, double nx, d
 }
 
 /* BoxSphere.dBoxSphere (BoxSphere.java:58-144) */
-int32_t orc_box_sphere(orc_world* w, int32_t box, int32_t other, const double* c, double r) {
-    const am_body_desc* bd = &w->desc[box];
-    const double* R = &w->R[box * 9];
-    const double* x = &w->x[box * 3];
+static int32_t box_sphere_s(orc_world* w, const oshape* box, const oshape* other, const double* c, double r) {
+    const am_body_desc* bd = box->d;
+    const double* R = box->R;
+    const double* x = box->x;
     double p[3] = {0.5 * bd->size[0], 0.5 * bd->size[1], 0.5 * bd->size[2]};
     double cB[3];
     inv_xform_point(cB, R, x, c);
@@ -773,17 +833,21 @@ int32_t orc_box_sphere(orc_world* w, int32_t box, int32_t other, const double* c
         xform_point(pw, R, x, t.pos);
         mat_vec(nw, R, t.normal);
         normalize3(nw);
-        emit(w, box, other, pw, nw, AM_BV_NONE, AM_BV_NONE, 0, t.depth);
+        emit(w, box->body, other->body, pw, nw, enc_bv(box, AM_BV_NONE), enc_bv(other, AM_BV_NONE), 0, t.depth);
         return 1;
     }
     return 0;
 }
+int32_t orc_box_sphere(orc_world* w, int32_t box, int32_t other, const double* c, double r) {
+    oshape a = body_shape(w, box), b = body_shape(w, other);
+    return box_sphere_s(w, &a, &b, c, r);
+}
 
 /* BoxSphere.dBoxSphereTest (BoxSphere.java:156-233) */
-int32_t orc_box_sphere_test(const orc_world* w, int32_t box, const double* c, double r) {
-    const am_body_desc* bd = &w->desc[box];
-    const double* R = &w->R[box * 9];
-    const double* x = &w->x[box * 3];
+static int32_t box_sphere_test_s(const oshape* box, const double* c, double r) {
+    const am_body_desc* bd = box->d;
+    const double* R = box->R;
+    const double* x = box->x;
     if (dist3(c, x) > bd->radius + r) return 0;
     double p[3] = {0.5 * bd->size[0], 0.5 * bd->size[1], 0.5 * bd->size[2]};
     double cB[3];
@@ -833,13 +897,31 @@ int32_t orc_box_sphere_test(const orc_world* w, int32_t box, const double* c, do
     }
     return 0;
 }
+int32_t orc_box_sphere_test(const orc_world* w, int32_t box, const double* c, double r) {
+    oshape a = body_shape(w, box);
+    return box_sphere_test_s(&a, c, r);
+}
 
 /* ------------------------------------------------------------------ */
 /* Sphere trees (CollisionProcessor.java:793-1056, BVSphere.java:174-186)*/
 /* ------------------------------------------------------------------ */
-/* BVSphere.updatecW: every node centre of the body, once per detection */
+/* BVSphere.updatecW: every node centre of the body, once per detection; the
+ * nodes of a composite's sphere sub-bodies with the sub-body's transform */
 static void update_tree_cw(orc_world* w, int32_t b) {
     const am_body_desc* d = &w->desc[b];
+    if (d->geom == AM_GEOM_COMPOSITE) {
+        for (int32_t q = 0; q < d->tree_count; q++) {
+            const am_body_desc* sd = &w->subs[d->tree_first + q];
+            if (sd->geom != AM_GEOM_TREE) continue;
+            double x[3], R[9];
+            sub_pose(w, b, q, x, R);
+            for (int32_t k = 0; k < sd->tree_count; k++) {
+                int32_t g = sd->tree_first + k;
+                xform_point(&w->cw[g * 3], R, x, w->nodes[g].c);
+            }
+        }
+        return;
+    }
     for (int32_t k = 0; k < d->tree_count; k++) {
         int32_t g = d->tree_first + k;
         xform_point(&w->cw[g * 3], &w->R[b * 9], &w->x[b * 3], w->nodes[g].c);
@@ -852,9 +934,9 @@ static double dist2(const double* a, const double* b) {
 }
 
 /* collideSphereTreeAndPlane (CollisionProcessor.java:793-826) */
-static void tree_plane(orc_world* w, int32_t body, int32_t node, int32_t plane) {
-    const am_body_desc* bd = &w->desc[body];
-    const am_body_desc* pd = &w->desc[plane];
+static void tree_plane(orc_world* w, const oshape* body, int32_t node, const oshape* plane) {
+    const am_body_desc* bd = body->d;
+    const am_body_desc* pd = plane->d;
     int32_t g = bd->tree_first + node;
     const double* c = &w->cw[g * 3];
     const double* n = pd->plane_n;
@@ -866,7 +948,7 @@ static void tree_plane(orc_world* w, int32_t body, int32_t node, int32_t plane) 
             nm[0] = val * n[0]; nm[1] = val * n[1]; nm[2] = val * n[2];
             double cw[3] = {c[0] - nm[0], c[1] - nm[1], c[2] - nm[2]};
             double nn[3] = {-1.0 * n[0], -1.0 * n[1], -1.0 * n[2]};
-            emit(w, body, plane, cw, nn, node, AM_BV_PLANE, 0, d);
+            emit(w, body->body, plane->body, cw, nn, enc_bv(body, node), AM_BV_PLANE, 0, d);
         } else {
             for (int32_t k = 0; k < w->nodes[g].n_children; k++)
                 tree_plane(w, body, w->nodes[g].first_child + k, plane);
@@ -875,20 +957,20 @@ static void tree_plane(orc_world* w, int32_t body, int32_t node, int32_t plane) 
 }
 
 /* collideBoxAndSphereTree (CollisionProcessor.java:837-850) */
-static void box_tree(orc_world* w, int32_t box, int32_t body2, int32_t node) {
-    int32_t g = w->desc[body2].tree_first + node;
-    if (!orc_box_sphere_test(w, box, &w->cw[g * 3], w->nodes[g].r)) return;
+static void box_tree(orc_world* w, const oshape* box, const oshape* body2, int32_t node) {
+    int32_t g = body2->d->tree_first + node;
+    if (!box_sphere_test_s(box, &w->cw[g * 3], w->nodes[g].r)) return;
     if (w->nodes[g].n_children != 0) {
         for (int32_t k = 0; k < w->nodes[g].n_children; k++)
             box_tree(w, box, body2, w->nodes[g].first_child + k);
     } else {
-        orc_box_sphere(w, box, body2, &w->cw[g * 3], w->nodes[g].r);
+        box_sphere_s(w, box, body2, &w->cw[g * 3], w->nodes[g].r);
     }
 }
 
 /* processCollision (CollisionProcessor.java:1036-1056) */
-static void leaf_leaf(orc_world* w, int32_t b1, int32_t n1, int32_t b2, int32_t n2) {
-    int32_t g1 = w->desc[b1].tree_first + n1, g2 = w->desc[b2].tree_first + n2;
+static void leaf_leaf(orc_world* w, const oshape* S1, int32_t n1, const oshape* S2, int32_t n2) {
+    int32_t g1 = S1->d->tree_first + n1, g2 = S2->d->tree_first + n2;
     const double* c1 = &w->cw[g1 * 3];
     const double* c2 = &w->cw[g2 * 3];
     double r1 = w->nodes[g1].r, r2 = w->nodes[g2].r;
@@ -900,27 +982,46 @@ static void leaf_leaf(orc_world* w, int32_t b1, int32_t n1, int32_t b2, int32_t 
         for (int i = 0; i < 3; i++) p[i] = (1 - alpha) * c1[i] + alpha * c2[i];
         for (int i = 0; i < 3; i++) n[i] = c2[i] - c1[i];
         normalize3(n);
-        emit(w, b1, b2, p, n, n1, n2, 0, distance - sum);
+        emit(w, S1->body, S2->body, p, n, enc_bv(S1, n1), enc_bv(S2, n2), 0, distance - sum);
     }
 }
 
 /* collideSphereTrees (CollisionProcessor.java:975-1009) */
-static void tree_tree(orc_world* w, int32_t b1, int32_t n1, int32_t b2, int32_t n2) {
-    int32_t g1 = w->desc[b1].tree_first + n1, g2 = w->desc[b2].tree_first + n2;
+static void tree_tree(orc_world* w, const oshape* S1, int32_t n1, const oshape* S2, int32_t n2) {
+    int32_t g1 = S1->d->tree_first + n1, g2 = S2->d->tree_first + n2;
     const am_sphere_node* s1 = &w->nodes[g1];
     const am_sphere_node* s2 = &w->nodes[g2];
     if (dist2(&w->cw[g1 * 3], &w->cw[g2 * 3]) < (s1->r + s2->r) * (s1->r + s2->r)) {
         int l1 = s1->n_children == 0, l2 = s2->n_children == 0;
-        if (l1 && l2) leaf_leaf(w, b1, n1, b2, n2);
-        else if (l1) { for (int32_t k = 0; k < s2->n_children; k++) tree_tree(w, b1, n1, b2, s2->first_child + k); }
-        else if (l2) { for (int32_t k = 0; k < s1->n_children; k++) tree_tree(w, b1, s1->first_child + k, b2, n2); }
-        else if (s1->r <= s2->r) { for (int32_t k = 0; k < s2->n_children; k++) tree_tree(w, b1, n1, b2, s2->first_child + k); }
-        else { for (int32_t k = 0; k < s1->n_children; k++) tree_tree(w, b1, s1->first_child + k, b2, n2); }
+        if (l1 && l2) leaf_leaf(w, S1, n1, S2, n2);
+        else if (l1) { for (int32_t k = 0; k < s2->n_children; k++) tree_tree(w, S1, n1, S2, s2->first_child + k); }
+        else if (l2) { for (int32_t k = 0; k < s1->n_children; k++) tree_tree(w, S1, s1->first_child + k, S2, n2); }
+        else if (s1->r <= s2->r) { for (int32_t k = 0; k < s2->n_children; k++) tree_tree(w, S1, n1, S2, s2->first_child + k); }
+        else { for (int32_t k = 0; k < s1->n_children; k++) tree_tree(w, S1, s1->first_child + k, S2, n2); }
+    }
+}
+
+/* narrowPhase on two primitive shapes (CollisionProcessor.java:723-750) */
+static void shape_phase(orc_world* w, const oshape* s1, const oshape* s2) {
+    const int32_t g1 = s1->d->geom, g2 = s2->d->geom;
+    if (g1 == AM_GEOM_PLANE) {
+        if (g2 == AM_GEOM_PLANE) return; /* plane-plane: reported impossible, no contact */
+        if (g2 == AM_GEOM_BOX) box_plane_s(w, s2, s1);
+        else tree_plane(w, s2, 0, s1);
+    } else if (g1 == AM_GEOM_BOX) {
+        if (g2 == AM_GEOM_PLANE) box_plane_s(w, s1, s2);
+        else if (g2 == AM_GEOM_BOX) box_box_s(w, s1, s2);
+        else box_tree(w, s1, s2, 0);
+    } else {
+        if (g2 == AM_GEOM_PLANE) tree_plane(w, s1, 0, s2);
+        else if (g2 == AM_GEOM_BOX) box_tree(w, s2, s1, 0);
+        else tree_tree(w, s1, 0, s2, 0);
     }
 }
 
 /* narrowPhase type dispatch (CollisionProcessor.java:707-757, 768-791 with
- * collectionCD = 0) */
+ * collectionCD = 0): collections first, then composites, each sub-body in
+ * turn (:711-722) */
 static void narrow_phase(orc_world* w, int32_t b1, int32_t b2) {
     const am_body_desc* d1 = &w->desc[b1];
     const am_body_desc* d2 = &w->desc[b2];
@@ -932,26 +1033,38 @@ static void narrow_phase(orc_world* w, int32_t b1, int32_t b2) {
         for (int32_t q = 0; q < w->mem_count[b2]; q++) narrow_phase(w, b1, w->members[w->mem_first[b2] + q]);
         return;
     }
-    if (d1->geom == AM_GEOM_PLANE) {
-        if (d2->geom == AM_GEOM_PLANE) return; /* plane-plane: reported impossible, no contact */
-        if (d2->geom == AM_GEOM_BOX) orc_box_plane(w, b2, b1);
-        else tree_plane(w, b2, 0, b1);
-    } else if (d1->geom == AM_GEOM_BOX) {
-        if (d2->geom == AM_GEOM_PLANE) orc_box_plane(w, b1, b2);
-        else if (d2->geom == AM_GEOM_BOX) orc_box_box(w, b1, b2);
-        else box_tree(w, b1, b2, 0);
-    } else {
-        if (d2->geom == AM_GEOM_PLANE) tree_plane(w, b1, 0, b2);
-        else if (d2->geom == AM_GEOM_BOX) box_tree(w, b2, b1, 0);
-        else tree_tree(w, b1, 0, b2, 0);
+    if (d1->geom == AM_GEOM_COMPOSITE) {
+        oshape o = body_shape(w, b2);
+        for (int32_t q = 0; q < d1->tree_count; q++) {
+            oshape s = sub_shape(w, b1, q);
+            if (d2->geom == AM_GEOM_COMPOSITE) {
+                for (int32_t q2 = 0; q2 < d2->tree_count; q2++) {
+                    oshape s2 = sub_shape(w, b2, q2);
+                    shape_phase(w, &s, &s2);
+                }
+            } else {
+                shape_phase(w, &s, &o);
+            }
+        }
+        return;
     }
+    oshape a = body_shape(w, b1);
+    if (d2->geom == AM_GEOM_COMPOSITE) {
+        for (int32_t q = 0; q < d2->tree_count; q++) {
+            oshape s2 = sub_shape(w, b2, q);
+            shape_phase(w, &a, &s2);
+        }
+        return;
+    }
+    oshape b = body_shape(w, b2);
+    shape_phase(w, &a, &b);
 }
 
 /* collisionDetection + broadPhase (CollisionProcessor.java:91-102, 683-697) */
 int32_t orc_detect(orc_world* w) {
     w->nc = 0;
     for (int32_t i = 0; i < w->n; i++)
-        if (w->desc[i].geom == AM_GEOM_TREE) update_tree_cw(w, i);
+        if (w->desc[i].geom == AM_GEOM_TREE || w->desc[i].geom == AM_GEOM_COMPOSITE) update_tree_cw(w, i);
     int32_t N = w->n_top;
     for (int32_t i = 0; i < N - 1; i++) {
         int32_t b1 = w->top[i];

@@ -42,6 +42,10 @@ typedef struct orc_world orc_world;
 
 orc_world* orc_create(const am_body_desc* bodies, int32_t n_entities,
                       const am_sphere_node* nodes, int32_t n_nodes);
+/* orc_create with composite bodies (AM_GEOM_COMPOSITE): their sub-bodies and
+ * transforms, as am_scene_upload_ex takes them. */
+orc_world* orc_create_ex(const am_body_desc* bodies, int32_t n_entities, const am_sphere_node* nodes,
+                         int32_t n_nodes, const am_body_desc* subs, const double* sub_b2c, int32_t n_subs);
 void orc_destroy(orc_world* w);
 void orc_set_params(orc_world* w, const am_params* p);
 

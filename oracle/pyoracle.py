@@ -36,6 +36,7 @@ def load() -> C.CDLL:
     vp, i32, dbl = C.c_void_p, C.c_int32, C.c_double
     sig = {
         "orc_create": (vp, [vp, i32, vp, i32]),
+        "orc_create_ex": (vp, [vp, i32, vp, i32, vp, vp, i32]),
         "orc_destroy": (None, [vp]),
         "orc_set_params": (None, [vp, C.POINTER(Params)]),
         "orc_set_structure": (None, [vp, vp, vp, i32, vp, vp, vp]),
@@ -92,7 +93,13 @@ class Oracle:
         b, nd, x, R, v, w = scene.arrays()
         self._b, self._nd = b, nd
         self.n = len(b)
-        self.h = C.c_void_p(self.lib.orc_create(ptr(b), len(b), ptr(nd), len(nd)))
+        subs, b2c = scene.sub_arrays() if hasattr(scene, "sub_arrays") else (None, None)
+        if subs is not None and len(subs):
+            self._subs, self._b2c = subs, b2c
+            self.h = C.c_void_p(self.lib.orc_create_ex(ptr(b), len(b), ptr(nd), len(nd), ptr(subs), ptr(b2c),
+                                                       len(subs)))
+        else:
+            self.h = C.c_void_p(self.lib.orc_create(ptr(b), len(b), ptr(nd), len(nd)))
         self.params = params if params is not None else default_params_host()
         self.lib.orc_set_params(self.h, C.byref(self.params))
         self.set_state(x, R, v, w)

@@ -73,7 +73,7 @@ def test_collision_settings_and_overrides():
 @pytest.mark.parametrize("body,msg", [
     ('<body type="mesh" name="m" obj="a.obj" st="a.sph" scale="1"/>', "not found"),
     ('<body type="mesh" name="m" obj="a.obj" st="a.sph"/>', "needs a scale"),
-    ('<body type="composite" name="c"/>', "composite"),
+    ('<body type="composite" name="c"/>', "without box or sphere parts"),
     ('<body type="box" name="b" dim="1 1 1"><spinner>0 1 0</spinner></body>', "spinners"),
     ('<body type="cone" name="b"/>', "unknown type"),
 ])
@@ -119,7 +119,7 @@ def test_reference_scene_files_load():
     for path in sorted(glob.glob(os.path.join(REF_SCENES, "*.xml"))):
         with open(path) as f:
             text = f.read()
-        if 'type="composite"' in text or "<spinner" in text:
+        if "<spinner" in text:
             continue
         scene, coll = xml_scene.parse_scene(path)
         b, _, x, R, _, _ = scene.arrays()
@@ -129,4 +129,4 @@ def test_reference_scene_files_load():
         if scene.springs:
             with_springs += 1
         loaded += 1
-    assert loaded >= 14 and with_springs >= 6
+    assert loaded >= 24 and with_springs >= 12
