@@ -57,6 +57,7 @@ class Scene:
     springs: list = field(default_factory=list)      # list of SPRING_DTYPE records
     subs: list = field(default_factory=list)         # sub-bodies of composites (BODY_DTYPE records)
     sub_b2c: list = field(default_factory=list)      # their transforms into the composite frame (12 each)
+    spinners: dict = field(default_factory=dict)     # body -> RigidBody.spinner (added to omega by the app's keys)
 
     # -- body builders -------------------------------------------------
     def _append(self, rec, x, R, v, w, name):

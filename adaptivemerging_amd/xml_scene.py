@@ -12,9 +12,9 @@ Springs (:596-619, Spring.java) load with the attachment geometry their
 constructors compute from the pose the body has when the <spring> tag is
 read.  Mesh bodies (createMesh :449-538) load their mass properties from
 the OBJ file and their sphere tree from the .sph file; composite bodies
-(createComposite :180-353) their box and sphere parts.  Spinners have no
-counterpart in this build; a scene that uses them raises SceneFormatError
-rather than loading into different dynamics.
+(createComposite :180-353) their box and sphere parts.  Spinner vectors are
+recorded in scene.spinners: only the app's keyboard applies them, by adding
+them to a body's angular velocity.
 Sphere-tree geometry for hand-built bodies comes from .sph files through
 scene.load_sph (the format of :500-535).
 """
@@ -86,6 +86,7 @@ class _Common:
         self.restitution = 0.0   # RigidBody.restitution default (RigidBody.java:145)
         self.pinned = False
         self.springs = []        # (attributes, pose (x, R) when the tag was read)
+        self.spinner = None
         for c in node:
             tag = c.tag.lower()
             text = (c.text or "").strip()
@@ -108,7 +109,9 @@ class _Common:
             elif tag == "pinned":
                 self.pinned = _java_bool(text)
             elif tag == "spinner":
-                raise SceneFormatError(f"body {name}: spinners are not modelled")
+                # RigidBody.spinner / canSpin: only the app's Q and W keys use
+                # it, adding it to omega (LCPApp3D.java:918-935)
+                self.spinner = _t3d(text)
             elif tag == "spring":
                 if not drop_springs:
                     pose = (list(self.x) if self.x is not None else [0.0, 0.0, 0.0],
@@ -207,6 +210,8 @@ def parse_scene(source: str, *, ignore_springs: bool = False, scene_index: int =
             b = scene.add_box(s, com.pos, R=com.R, v=com.v, w=com.w, density=density, friction=com.friction,
                               restitution=com.restitution, pinned=com.pinned, scene=scene_index, name=name)
             _add_springs(scene, b, com, name, posed_planes)
+            if com.spinner is not None:
+                scene.spinners[b] = com.spinner
         elif kind == "plane":
             # PlaneRigidBody is always pinned (PlaneRigidBody.java:55-62)
             if com.springs:
@@ -222,6 +227,8 @@ def parse_scene(source: str, *, ignore_springs: bool = False, scene_index: int =
             b = scene.add_sphere(float(node.attrib["r"]), com.pos, R=com.R, v=com.v, w=com.w, density=density,
                                  friction=com.friction, restitution=com.restitution, scene=scene_index, name=name)
             _add_springs(scene, b, com, name, posed_planes)
+            if com.spinner is not None:
+                scene.spinners[b] = com.spinner
         elif kind == "mesh":
             if "scale" not in node.attrib:
                 raise SceneFormatError(f"body {name}: a mesh needs a scale")
@@ -232,6 +239,8 @@ def parse_scene(source: str, *, ignore_springs: bool = False, scene_index: int =
                                friction=com.friction, restitution=com.restitution, pinned=com.pinned,
                                scene=scene_index, name=name)
             _add_springs(scene, b, com, name, posed_planes)
+            if com.spinner is not None:
+                scene.spinners[b] = com.spinner
         elif kind == "composite":
             parts = []
             for child in node:
@@ -262,6 +271,8 @@ def parse_scene(source: str, *, ignore_springs: bool = False, scene_index: int =
             b = scene.add_composite(parts, com.pos, R=com.R, v=com.v, w=com.w, friction=com.friction,
                                     restitution=com.restitution, pinned=com.pinned, scene=scene_index, name=name)
             _add_springs(scene, b, com, name, posed_planes)
+            if com.spinner is not None:
+                scene.spinners[b] = com.spinner
         else:
             raise SceneFormatError(f"body {name}: unknown type {kind!r}")
     return scene, collision

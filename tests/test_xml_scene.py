@@ -74,12 +74,17 @@ def test_collision_settings_and_overrides():
     ('<body type="mesh" name="m" obj="a.obj" st="a.sph" scale="1"/>', "not found"),
     ('<body type="mesh" name="m" obj="a.obj" st="a.sph"/>', "needs a scale"),
     ('<body type="composite" name="c"/>', "without box or sphere parts"),
-    ('<body type="box" name="b" dim="1 1 1"><spinner>0 1 0</spinner></body>', "spinners"),
     ('<body type="cone" name="b"/>', "unknown type"),
 ])
 def test_unsupported_features_are_refused(body, msg):
     with pytest.raises(xml_scene.SceneFormatError, match=msg):
         xml_scene.parse_scene(f"<root>{body}</root>")
+
+
+def test_spinner_is_recorded():
+    scene, _ = xml_scene.parse_scene('<root><body type="box" name="b" dim="1 1 1"><spinner>0 2 0</spinner>'
+                                     '</body></root>')
+    assert scene.spinners == {0: [0.0, 2.0, 0.0]} and scene.n == 1
 
 
 def test_springs_can_be_dropped_explicitly():
@@ -119,8 +124,6 @@ def test_reference_scene_files_load():
     for path in sorted(glob.glob(os.path.join(REF_SCENES, "*.xml"))):
         with open(path) as f:
             text = f.read()
-        if "<spinner" in text:
-            continue
         scene, coll = xml_scene.parse_scene(path)
         b, _, x, R, _, _ = scene.arrays()
         assert scene.n > 0 and np.all(np.isfinite(x)) and np.all(np.isfinite(R))
