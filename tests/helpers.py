@@ -100,3 +100,28 @@ def merging_oracle_step(mo, dt, step, order=None, kick=None):
         merged = mo.merge()
     asleep = mo.sleep()
     return merged, cut, asleep
+
+
+def load_scene_fixture(name):
+    """A scene written by tests/golden/make_scene_fixtures.py, as a Scene, and
+    its <collision> settings."""
+    import json
+    import os
+
+    import numpy as np
+
+    from adaptivemerging_amd.scene import Scene
+
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", f"scene_{name}.npz"))
+    s = Scene()
+    s.bodies = list(z["bodies"])
+    s.nodes = list(z["nodes"])
+    s.x = [np.array(a) for a in z["x"]]
+    s.R = [np.array(a).reshape(3, 3) for a in z["R"]]
+    s.v = [np.array(a) for a in z["v"]]
+    s.w = [np.array(a) for a in z["w"]]
+    s.names = [str(a) for a in z["names"]]
+    s.subs = list(z["subs"])
+    s.sub_b2c = [list(a) for a in z["sub_b2c"]]
+    s.springs = list(z["springs"])
+    return s, json.loads(str(z["collision"]))
