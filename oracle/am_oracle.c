@@ -452,14 +452,15 @@ static void emit(orc_world* w, int32_t b1, int32_t b2, const double* pW, const d
  * as csb, Contact.java:166-177). */
 typedef struct {
     const am_body_desc* d;
-    double x[3], R[9];
+    const double* x;  /* the body's state, or the caller's buffer for a sub-body */
+    const double* R;
     int32_t body, sub;
 } oshape;
 static oshape body_shape(const orc_world* w, int32_t b) {
     oshape s;
     s.d = &w->desc[b];
-    memcpy(s.x, &w->x[b * 3], sizeof s.x);
-    memcpy(s.R, &w->R[b * 9], sizeof s.R);
+    s.x = &w->x[b * 3];
+    s.R = &w->R[b * 9];
     s.body = b; s.sub = -1;
     return s;
 }
@@ -468,10 +469,13 @@ static void sub_pose(const orc_world* w, int32_t b, int32_t q, double* x, double
     xform_point(x, &w->R[b * 9], &w->x[b * 3], T + 9);
     mat_mul(R, &w->R[b * 9], T);
 }
-static oshape sub_shape(const orc_world* w, int32_t b, int32_t q) {
+/* buf: 12 doubles that hold the sub-body's pose while the shape is used */
+static oshape sub_shape(const orc_world* w, int32_t b, int32_t q, double* buf) {
     oshape s;
     s.d = &w->subs[w->desc[b].tree_first + q];
-    sub_pose(w, b, q, s.x, s.R);
+    sub_pose(w, b, q, buf, buf + 3);
+    s.x = buf;
+    s.R = buf + 3;
     s.body = b; s.sub = q;
     return s;
 }
@@ -1033,13 +1037,14 @@ static void narrow_phase(orc_world* w, int32_t b1, int32_t b2) {
         for (int32_t q = 0; q < w->mem_count[b2]; q++) narrow_phase(w, b1, w->members[w->mem_first[b2] + q]);
         return;
     }
+    double buf1[12], buf2[12];
     if (d1->geom == AM_GEOM_COMPOSITE) {
         oshape o = body_shape(w, b2);
         for (int32_t q = 0; q < d1->tree_count; q++) {
-            oshape s = sub_shape(w, b1, q);
+            oshape s = sub_shape(w, b1, q, buf1);
             if (d2->geom == AM_GEOM_COMPOSITE) {
                 for (int32_t q2 = 0; q2 < d2->tree_count; q2++) {
-                    oshape s2 = sub_shape(w, b2, q2);
+                    oshape s2 = sub_shape(w, b2, q2, buf2);
                     shape_phase(w, &s, &s2);
                 }
             } else {
@@ -1051,7 +1056,7 @@ static void narrow_phase(orc_world* w, int32_t b1, int32_t b2) {
     oshape a = body_shape(w, b1);
     if (d2->geom == AM_GEOM_COMPOSITE) {
         for (int32_t q = 0; q < d2->tree_count; q++) {
-            oshape s2 = sub_shape(w, b2, q);
+            oshape s2 = sub_shape(w, b2, q, buf2);
             shape_phase(w, &a, &s2);
         }
         return;
