@@ -134,7 +134,7 @@ SYMBOLS = [
     "am_version", "am_ctx_create", "am_ctx_destroy", "am_last_error", "am_params_default", "am_params_set",
     "am_scene_upload", "am_scene_upload_ex", "am_structure_pull", "am_entities_pull", "am_jinv_pull", "am_state_push",
     "am_state_pull", "am_forces_push", "am_forces_pull", "am_clear_bodies", "am_apply_gravity", "am_deltav_pull",
-    "am_springs_set", "am_apply_spring_forces", "am_wake", "am_sleep", "am_wake_all", "am_sleep_state_pull",
+    "am_springs_set", "am_apply_spring_forces", "am_wake", "am_sleep", "am_wake_all", "am_sleep_state_pull", "am_unmerge_all",
     "am_update_contacts_map", "am_detect", "am_contacts_push", "am_contacts_pull", "am_lambdas_push",
     "am_warm_start", "am_redo_warm_start", "am_no_warm_start", "am_solve", "am_solve_order_pull",
     "am_colours_pull", "am_update_in_collections", "am_changes_pull", "am_advance_velocities",
@@ -191,6 +191,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "am_wake": (C.c_int, [ctxp, vp]),
         "am_sleep": (C.c_int, [ctxp, vp]),
         "am_wake_all": (C.c_int, [ctxp]),
+        "am_unmerge_all": (C.c_int, [ctxp, vp]),
         "am_sleep_state_pull": (C.c_int, [ctxp, vp, vp, vp, i32]),
         "am_deltav_pull": (C.c_int, [ctxp, vp, i32]),
         "am_update_contacts_map": (C.c_int, [ctxp]),

@@ -85,6 +85,11 @@ class Context:
     def wake_all(self):
         self._check(self.lib.am_wake_all(self.h))
 
+    def unmerge_all(self) -> int:
+        n = C.c_int32()
+        self._check(self.lib.am_unmerge_all(self.h, C.byref(n)))
+        return n.value
+
     def sleep_state(self, n: int | None = None):
         """(sleeping flags, history lengths, histories (n, AM_SLEEP_HIST_STRIDE))
         of the first n entities (default: all)."""

@@ -1182,6 +1182,15 @@ int am_sleep(am_ctx* c, int32_t* n_asleep) {
     return r;
 }
 
+int am_unmerge_all(am_ctx* c, int32_t* n_dissolved) {
+    int r = check_ctx(c);
+    if (r) return r;
+    int n = 0;
+    r = merge_unmerge_all(c, &n);
+    if (n_dissolved) *n_dissolved = n;
+    return r;
+}
+
 int am_wake_all(am_ctx* c) {
     int r = check_ctx(c);
     if (r) return r;

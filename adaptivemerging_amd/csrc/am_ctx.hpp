@@ -249,6 +249,7 @@ int merge_unmerge(am_ctx* c, double dt, int* n_events);
 int merge_body_pairs(am_ctx* c);
 int merge_accumulate_merging(am_ctx* c);
 int merge_merge(am_ctx* c, int* n_merged);
+int merge_unmerge_all(am_ctx* c, int* n_dissolved);
 int merge_push_down(am_ctx* c, bool integrated_only);
 int merge_member_transforms(am_ctx* c);
 int pull_contacts(am_ctx* c, const am::ContactsView& V, int count, am_contact_arrays* o, int32_t capacity);

@@ -1482,6 +1482,75 @@ int merge_unmerge(am_ctx* c, double dt, int* n_events) {
     return groups_after_contacts(c);
 }
 
+// Merging.unmergeAll (Merging.java:168-206), the "unmerge all" request: every
+// collection that is awake hands its bodies back to the top-level list (after
+// the bodies that stay, in collection order) with the collection's velocity
+// at their position.  Its body pairs become external with an empty motion
+// history; their contacts are dropped (the reference's re-expose branch runs
+// only while inCollection is still set, which it no longer is).
+int merge_unmerge_all(am_ctx* c, int* n_dissolved) {
+    *n_dissolved = 0;
+    std::vector<int> colls;
+    for (int e : c->slot_entity)
+        if (c->desc[e].geom == AM_GEOM_COLLECTION && !(c->flags[e] & AM_BODY_SLEEPING)) colls.push_back(e);
+    if (colls.empty()) return AM_OK;
+    hipStream_t s = c->stream;
+    Event E;
+    int r = begin_event(c, E);
+    if (r) return r;
+    std::vector<char> gone(c->n_ent, 0);
+    for (int ce : colls) gone[ce] = 1;
+    // external pairs touching those collections restart their motion history
+    if (c->n_bpc > 0) {
+        std::vector<BpcRec> T(c->n_bpc);
+        AM_HIP(c, hipMemcpyAsync(T.data(), c->d_bpc.p, T.size() * sizeof(BpcRec), hipMemcpyDeviceToHost, s));
+        AM_HIP(c, hipStreamSynchronize(s));
+        for (auto& t : T)
+            if (t.b1 >= 0 && ((c->parent[t.b1] >= 0 && gone[c->parent[t.b1]]) ||
+                              (c->parent[t.b2] >= 0 && gone[c->parent[t.b2]])))
+                t.hist_n = 0;
+        AM_HIP(c, hipMemcpyAsync(c->d_bpc.p, T.data(), T.size() * sizeof(BpcRec), hipMemcpyHostToDevice, s));
+    }
+    // internal pairs of the dissolved collections leave with their contacts
+    std::vector<IbpcRec> kept;
+    std::vector<int> keep_idx;
+    for (const auto& ib : c->ibpc) {
+        if (gone[ib.coll]) continue;
+        IbpcRec k = ib;
+        k.first = (int)keep_idx.size();
+        for (int q = 0; q < ib.count; q++) keep_idx.push_back(ib.first + q);
+        kept.push_back(k);
+    }
+    if ((int)keep_idx.size() != c->n_ic) {
+        AM_HIP(c, c->ic_tmp.ensure((int)keep_idx.size(), 0, s));
+        if (!keep_idx.empty()) {
+            DBuf<int> d_idx;
+            AM_HIP(c, upload(c, d_idx, keep_idx));
+            launch_gather_contacts(s, c->ic_tmp.view(), 0, internal_view(c), d_idx.p, 0, (int)keep_idx.size(), 0);
+            AM_HIP(c, hipStreamSynchronize(s));
+        }
+        c->ic.swap(c->ic_tmp);
+        c->n_ic = (int)keep_idx.size();
+    }
+    c->ibpc = kept;
+    std::vector<int> additions;
+    for (int ce : colls) {
+        Ent C = E.load(ce);
+        for (int b : E.members[ce]) {
+            unmerge_body(E, C, b);
+            additions.push_back(b);
+        }
+    }
+    for (int ce : colls) {
+        top_remove(c, ce);
+        free_entity(E, ce);
+    }
+    for (int b : additions) c->slot_entity.push_back(b);
+    *n_dissolved = (int)colls.size();
+    if ((r = commit_event(c, E))) return r;
+    return upload_ibpc(c);
+}
+
 // ---------------------------------------------------------------------------
 // Sleeping (Sleeping.java).  The flags live in d_flags (authoritative during
 // a step) and in the host mirror c->flags; every change made on the device is

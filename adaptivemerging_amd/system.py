@@ -174,6 +174,7 @@ class Merging:
         self.ctx = ctx
         self.mergingEvent = False
         self.mergingBuildTime = 0.0     # MergeParameters.mergingBuildTime
+        self.unmergeAllRequested = False  # MergeParameters.unmergeAll (the "unmerge all" request)
         self.unmergingBuildTime = 0.0
         self.enableMerging = True
         self.enableMergePinned = True
@@ -209,6 +210,11 @@ class Merging:
         p.threshold_unmerge = self.thresholdUnmerge
         p.threshold_breath = self.thresholdBreath
         self.ctx.set_params()
+
+    def unmergeAll(self) -> int:
+        """Merging.unmergeAll (:168-206); clears the request."""
+        self.unmergeAllRequested = False
+        return self.ctx.unmerge_all()
 
     def merge(self) -> int:
         """Merging.merge (:73-163)."""
@@ -308,6 +314,8 @@ class RigidBodySystem:
         self.ctx.clear_bodies()
         if self.sleeping.wakeAllRequested:
             self.sleeping.wakeAll()
+        if self.merging.unmergeAllRequested:
+            self.merging.unmergeAll()
         self.applyExternalForces(dt)
         self.collision.updateContactsMap()
         self.collision.collisionDetection(dt)

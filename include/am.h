@@ -407,6 +407,12 @@ int am_wake_all(am_ctx* ctx);
 #define AM_SLEEP_HIST_STRIDE 201
 int am_sleep_state_pull(am_ctx* ctx, uint8_t* sleeping, int32_t* hist_n, double* hist, int32_t n);
 
+/* Merging.unmergeAll (Merging.java:168-206), what the "unmerge all" request
+ * does at the start of a step: every awake collection is dissolved, its
+ * bodies return to the top-level list with the collection's velocity, its
+ * body pairs restart their motion history and its internal contacts are
+ * dropped.  n_dissolved = collections dissolved. */
+int am_unmerge_all(am_ctx* ctx, int32_t* n_dissolved);
 /* External pairs of the last am_clear_body_pair_contacts, then internal pairs. */
 int am_body_pairs_pull(am_ctx* ctx, am_body_pair* out, int32_t capacity, int32_t* n_external, int32_t* n_internal);
 /* RigidCollection.internalContacts of every collection, pair by pair. */

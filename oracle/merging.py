@@ -748,6 +748,42 @@ class MergingOracle:
         self.push_structure(st, jinv)
         return merged
 
+    # -- Merging.unmergeAll (:168-206) ------------------------------------------
+    def unmerge_all(self):
+        """Every awake collection returns its bodies (with its velocity at their
+        position) to the end of the top-level list; its pairs restart their
+        motion history and its internal contacts are dropped."""
+        colls = [e for e in self.top if e in self.members and not self.sleeping(e)]
+        if not colls:
+            return 0
+        st = self.state()
+        jinv = [list(map(float, j)) for j in self.o.pull_jinv()]
+        dv = self.o.pull_deltav()
+        gone = set(colls)
+        for r in self.bpc:
+            if r["b1"] >= 0 and (self.parent[r["b1"]] in gone or self.parent[r["b2"]] in gone):
+                r["hist"] = []
+        self.ibpc = [p for p in self.ibpc if self.parent[p["b1"]] not in gone]
+        x, R, v, w = st
+        additions = []
+        for ce in colls:
+            C = self.load(ce, st, jinv)
+            for b in self.members[ce]:
+                r_ = [x[b][a] - C.x[a] for a in range(3)]
+                wxr = cross(C.w, r_)
+                v[b] = [C.v[a] + wxr[a] for a in range(3)]
+                w[b] = C.w
+                dv[b] = 0.0
+                self.parent[b] = -1
+                additions.append(b)
+        for ce in colls:
+            self.top.remove(ce)
+            self.free_entity(ce)
+        self.top += additions
+        self.push_structure(st, jinv)
+        self.o.push_deltav(dv)
+        return len(colls)
+
     # -- Merging.unmerge (:215-273) and unmergeSelectedBpcs (:286-374) --------
     def unmerge(self, dt):
         P = self.params

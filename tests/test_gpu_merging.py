@@ -134,3 +134,22 @@ def test_step_equals_phases(gpu_lib):
         for name, x, y in zip("xRvw", a.pull_state(n), b.pull_state(n)):
             assert_bits_equal(f"step {step} {name}", x, y)
     assert st["n_collections"] >= 1
+
+
+def test_unmerge_all_matches_oracle(gpu_lib):
+    """The "unmerge all" request (Merging.unmergeAll) between two steps, then
+    the run goes on; the GPU matches the oracle before and after."""
+    scene = workloads.merge_stack(seed=3)
+    ctx, mo = make(scene)
+    ctx.set_params(enable_sleeping=0)
+    mo.set_params(enable_sleeping=0)
+    dt = 0.01
+    for step in range(1, 13):
+        step_both(ctx, mo, dt, step)
+    n_g = ctx.unmerge_all()
+    n_o = mo.unmerge_all()
+    assert n_g == n_o >= 1
+    compare_world(ctx, mo, 12)
+    for step in range(13, 31):
+        step_both(ctx, mo, dt, step)
+        compare_world(ctx, mo, step)

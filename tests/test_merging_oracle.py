@@ -180,3 +180,37 @@ def test_scenario_settles_merges_and_unmerges():
     assert merged_at and merged_at[0] == 10
     assert cut_at == [31]
     assert 4 in mo.top   # the lifted board is a free body again
+
+
+def test_unmerge_all_returns_every_body():
+    """Merging.unmergeAll (Merging.java:168-206): the collections leave the
+    top-level list, their bodies follow the bodies that stayed, in collection
+    order, each moving with its collection's velocity at its position; the
+    internal pairs are gone and merging starts over."""
+    from tests.helpers import merging_oracle_step
+
+    scene = workloads.merge_stack(seed=3)
+    p = params(gravity_amount=9.81, iterations=100, tolerance=1e-10, enable_sleeping=0)
+    mo = M.MergingOracle(scene, p)
+    for step in range(1, 13):
+        merging_oracle_step(mo, 0.01, step)
+    colls = [e for e in mo.top if e in mo.members]
+    assert colls
+    stay = [e for e in mo.top if e not in mo.members]
+    members = [b for e in colls for b in mo.members[e]]
+    x, R, v, w = mo.state()
+    want_v = {}
+    for e in colls:
+        for b in mo.members[e]:
+            want_v[b] = np.array(v[e]) + np.cross(w[e], np.array(x[b]) - np.array(x[e]))
+    assert mo.unmerge_all() == len(colls)
+    assert mo.top == stay + members and not mo.ibpc and not mo.members
+    assert all(mo.parent[b] == -1 for b in members)
+    x2, _, v2, w2 = mo.state()
+    for b, vb in want_v.items():
+        np.testing.assert_allclose(v2[b], vb, rtol=1e-12, atol=1e-14)
+    merged = 0
+    for step in range(13, 31):
+        m, _, _ = merging_oracle_step(mo, 0.01, step)
+        merged += m
+    assert merged >= 1
