@@ -19,6 +19,7 @@ AM_ERR_HIP = -2
 AM_ERR_CAPACITY = -3
 AM_ERR_STATE = -4
 AM_ERR_NUMERIC = -5
+AM_ERR_MISMATCH = -6
 
 AM_GEOM_BOX = 0
 AM_GEOM_TREE = 1
@@ -120,6 +121,17 @@ class StepStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
 
 
+class TuneTrial(C.Structure):
+    _fields_ = [
+        ("pgs_schedule", C.c_int32), ("pgs_grid", C.c_int32), ("pgs_block", C.c_int32),
+        ("grid_used", C.c_int32), ("block_used", C.c_int32), ("reserved", C.c_int32),
+        ("ms_per_sweep", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
+
+
 class BodyPair(C.Structure):
     _fields_ = [
         ("body1", C.c_int32), ("body2", C.c_int32), ("in_collection", C.c_int32), ("n_contacts", C.c_int32),
@@ -143,7 +155,7 @@ SYMBOLS = [
     "am_internal_contacts_pull",
     "am_step", "am_motion_metric", "am_mlp_init", "am_mlp_set_params", "am_mlp_get_params", "am_mlp_forward",
     "am_mlp_gradients", "am_features_pull", "am_train_size", "am_train_clear", "am_train_epochs",
-    "am_start_lambdas_pull",
+    "am_start_lambdas_pull", "am_autotune",
 ]
 MLP_PARAMS = 10115
 MLP_FEATURES = 24
@@ -219,6 +231,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "am_advance_positions": (C.c_int, [ctxp, dbl]),
         "am_step": (C.c_int, [ctxp, dbl, C.POINTER(StepStats)]),
         "am_motion_metric": (C.c_int, [ctxp, vp, i32, vp]),
+        "am_autotune": (C.c_int, [ctxp, dbl, i32, i32, i32, C.POINTER(TuneTrial), i32, C.POINTER(i32),
+                                  C.POINTER(i32)]),
         "am_mlp_init": (C.c_int, [ctxp, C.c_uint64]),
         "am_mlp_set_params": (C.c_int, [ctxp, vp, i32]),
         "am_mlp_get_params": (C.c_int, [ctxp, vp, i32]),

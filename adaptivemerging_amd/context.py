@@ -313,6 +313,23 @@ class Context:
         self._check(self.lib.am_motion_metric(self.h, ptr(p), len(p), ptr(out)))
         return out
 
+    def autotune(self, dt: float, sweeps: int = 20, reps: int = 3, apply: bool = True):
+        """Times the PGS kernel's launch shapes on the contacts currently held
+        (am_autotune); every shape must give the first shape's bits.  Returns
+        (trials as dicts, index of the fastest); with apply the fastest shape
+        becomes this context's pgs_schedule / pgs_grid / pgs_block."""
+        cap = 64
+        trials = (_lib.TuneTrial * cap)()
+        n, best = C.c_int32(), C.c_int32()
+        self._check(self.lib.am_autotune(self.h, dt, sweeps, reps, int(apply), trials, cap, C.byref(n),
+                                         C.byref(best)))
+        out = [trials[k].as_dict() for k in range(min(n.value, cap))]
+        if apply:
+            b = out[best.value]
+            for k in ("pgs_schedule", "pgs_grid", "pgs_block"):
+                setattr(self.params, k, b[k])
+        return out, best.value
+
     # -- learned warm start -------------------------------------------------
     def mlp_init(self, seed: int = 0):
         self._check(self.lib.am_mlp_init(self.h, C.c_uint64(seed)))

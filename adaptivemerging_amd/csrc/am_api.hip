@@ -354,7 +354,6 @@ int am_params_set(am_ctx* c, const am_params* p) {
     if (p->pgs_schedule != 0 && p->pgs_schedule != 1) return fail(c, AM_ERR_INVALID, "pgs_schedule must be 0 or 1");
     if (p->sleep_step_accum < 0 || p->sleep_step_accum > am::kMaxSleepAccum)
         return fail(c, AM_ERR_INVALID, "sleep_step_accum must be in 0..200");
-    if (p->pgs_block != c->params.pgs_block) c->pgs_max_grid = 0;
     c->params = *p;
     return AM_OK;
 }
@@ -687,6 +686,34 @@ int start_residual(am_ctx* c, PgsArgs G, double* out) {
     return AM_OK;
 }
 
+// Launch shape of the PGS kernel under P (after build_layout): workgroups and
+// threads per workgroup, clamped to what can be co-resident.
+void pgs_shape(am_ctx* c, const am_params& P, int* grid, int* block) {
+    bool ok = false;
+    if (P.pgs_schedule == 0) {
+        if (c->flow_max_grid == 0) {
+            c->flow_max_grid = pgs_flow_max_blocks(&ok);
+            if (!ok) c->flow_max_grid = 1;
+        }
+        // one wave per workgroup, each walking every W-th step of up to 4 groups
+        int g = P.pgs_grid > 0 ? P.pgs_grid : std::min((c->n_groups + 3) / 4, 512);
+        *grid = std::max(1, std::min(g, c->flow_max_grid));
+        *block = 64;
+        return;
+    }
+    const int b = P.pgs_block > 0 ? P.pgs_block : 256;
+    int& mx = c->pgs_max_grid[b / 64];
+    if (mx == 0) {
+        mx = pgs_max_blocks(b, &ok);
+        if (!ok) mx = 1;
+    }
+    // groups of a colour are dealt block-cyclically; about 20 groups of the
+    // largest colour per workgroup (the best measured on the settling box pile)
+    int g = P.pgs_grid > 0 ? P.pgs_grid : (c->max_colour_groups + 19) / 20;
+    *grid = std::max(1, std::min(g, mx));
+    *block = b;
+}
+
 int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double* last) {
     if (main_solve) c->sweep_order_n = -1;
     const am_params& P = c->params;
@@ -767,16 +794,8 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
     G.error = c->d_barrier.p + 1;
     G.iter_max = c->s_iter_max.p;
     G.iters_used = c->d_scalars.p + 6;
-    int block = P.pgs_block > 0 ? P.pgs_block : 256;
-    if (c->pgs_max_grid == 0) {
-        bool ok = false;
-        c->pgs_max_grid = pgs_max_blocks(block, &ok);
-        if (!ok) c->pgs_max_grid = 1;
-    }
-    // groups of a colour are dealt block-cyclically; about 20 groups of the
-    // largest colour per workgroup (the best measured on the settling box pile)
-    int grid = P.pgs_grid > 0 ? P.pgs_grid : (c->max_colour_groups + 19) / 20;
-    grid = std::max(1, std::min(grid, c->pgs_max_grid));
+    int grid = 1, block = 64;
+    pgs_shape(c, P, &grid, &block);
     if (learned) {
         // plain start residual, predicted start residual; keep the better
         AM_HIP(c, c->dv_save.ensure((size_t)c->n_ent * 6));
@@ -810,14 +829,7 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
         fill_int(s, (int*)c->f_cnt.p, c->n_ent, 0);
         fill_int(s, (int*)c->d_pass_done.p, iterations + 2, 0);
         AM_HIP(c, hipMemcpyAsync(c->dv_alt.p, c->d_dv.p, n_dv * sizeof(double), hipMemcpyDeviceToDevice, s));
-        if (c->flow_max_grid == 0) {
-            bool ok = false;
-            c->flow_max_grid = pgs_flow_max_blocks(&ok);
-            if (!ok) c->flow_max_grid = 1;
-        }
-        // one wave per workgroup, each walking every W-th step of up to 4 groups
-        int fgrid = P.pgs_grid > 0 ? P.pgs_grid : std::min((c->n_groups + 3) / 4, 512);
-        fgrid = std::max(1, std::min(fgrid, c->flow_max_grid));
+        const int fgrid = grid;
         FlowArgs F;
         F.P = G;
         F.flow = c->s_flow.p;
@@ -1439,6 +1451,163 @@ int am_solve(am_ctx* c, double dt, int32_t post, int32_t* iterations_used, doubl
     int r = check_ctx(c);
     if (r) return r;
     return do_solve(c, dt, post, true, iterations_used, last_change);
+}
+
+int am_autotune(am_ctx* c, double dt, int32_t sweeps, int32_t reps, int32_t apply, am_tune_trial* trials,
+                int32_t capacity, int32_t* n_trials, int32_t* best) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (sweeps < 1 || sweeps > 100000 || reps < 1 || reps > 100)
+        return fail(c, AM_ERR_INVALID, "sweeps must be in 1..100000 and reps in 1..100");
+    if (capacity < 0 || (capacity > 0 && !trials)) return fail(c, AM_ERR_INVALID, "null trials with capacity > 0");
+    if (c->n_contacts == 0) return fail(c, AM_ERR_STATE, "no contacts to tune on: call am_detect or am_step first");
+    if ((r = build_layout(c))) return r;
+    hipStream_t s = c->stream;
+    const size_t nc3 = (size_t)c->n_contacts * 3, nc = (size_t)c->n_contacts, n_dv = (size_t)c->n_ent * 6;
+
+    // what a solve overwrites, restored before every trial and at the end
+    DBuf<double> lam0, b0, D0, dv0, start0;
+    DBuf<int> st0;
+    AM_HIP(c, lam0.ensure(nc3));
+    AM_HIP(c, b0.ensure(nc3));
+    AM_HIP(c, D0.ensure(nc3));
+    AM_HIP(c, start0.ensure(nc3));
+    AM_HIP(c, dv0.ensure(n_dv));
+    AM_HIP(c, st0.ensure(nc));
+    AM_HIP(c, c->s_start.ensure(nc3));
+    auto d2d = [&](void* dst, const void* src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s);
+    };
+    AM_HIP(c, d2d(lam0.p, c->c_lam.p, nc3 * 8));
+    AM_HIP(c, d2d(b0.p, c->c_b.p, nc3 * 8));
+    AM_HIP(c, d2d(D0.p, c->c_D.p, nc3 * 8));
+    AM_HIP(c, d2d(start0.p, c->s_start.p, nc3 * 8));
+    AM_HIP(c, d2d(dv0.p, c->d_dv.p, n_dv * 8));
+    AM_HIP(c, d2d(st0.p, c->c_state.p, nc * 4));
+    const am_params saved = c->params;
+    const int last_iters = c->last_iters, sweep_order_n = c->sweep_order_n, used_learned = c->used_learned;
+    const double last_change = c->last_change, ms_pgs = c->ms_pgs_kernel, ms_setup = c->ms_setup_kernel,
+                 ms_learned = c->ms_learned, res_plain = c->res_plain, res_learned = c->res_learned;
+    const std::vector<double> changes = c->changes;
+    auto restore_inputs = [&]() -> hipError_t {
+        hipError_t e = d2d(c->c_lam.p, lam0.p, nc3 * 8);
+        if (e == hipSuccess) e = d2d(c->d_dv.p, dv0.p, n_dv * 8);
+        return e;
+    };
+
+    // candidate shapes: the dataflow schedule at several wave counts, the
+    // coloured schedule at each workgroup size and several grids; shapes that
+    // clamp to one already listed are timed once
+    struct Cand {
+        int sched, grid, block, grid_used, block_used;
+        double ms;
+    };
+    std::vector<Cand> cands;
+    auto add = [&](int sched, int grid, int block) {
+        am_params P = saved;
+        P.pgs_schedule = sched;
+        P.pgs_grid = grid;
+        P.pgs_block = block;
+        int g = 1, b = 64;
+        pgs_shape(c, P, &g, &b);
+        for (const Cand& k : cands)
+            if (k.sched == sched && k.grid_used == g && k.block_used == b) return;
+        cands.push_back({sched, grid, block, g, b, 0.0});
+    };
+    for (int g : {0, 64, 128, 192, 256, 384, 512, 768, 1024}) add(0, g, 0);
+    for (int b : {0, 64, 128})
+        for (int g : {0, 32, 64, 128, 256, 512}) add(1, g, b);
+
+    std::vector<double> ref_lam(nc3), ref_dv(n_dv), got_lam(nc3), got_dv(n_dv);
+    int err = AM_OK;
+    for (size_t k = 0; k < cands.size() && err == AM_OK; k++) {
+        am_params& P = c->params;
+        P = saved;
+        P.pgs_schedule = cands[k].sched;
+        P.pgs_grid = cands[k].grid;
+        P.pgs_block = cands[k].block;
+        P.iterations = sweeps;
+        P.tolerance = 0.0;  // no early exit: every trial runs the same sweeps
+        P.learned_warm_start = 0;
+        P.collect_training = 0;
+        double best_ms = 1e300;
+        for (int rep = 0; rep < reps && err == AM_OK; rep++) {
+            hipError_t e = restore_inputs();
+            if (e != hipSuccess) {
+                err = fail(c, AM_ERR_HIP, hipGetErrorString(e));
+                break;
+            }
+            int it = 0;
+            if ((err = do_solve(c, dt, 0, true, &it, nullptr))) break;
+            if (it != sweeps) {
+                err = fail(c, AM_ERR_MISMATCH, "a tuning trial did not run the requested sweeps");
+                break;
+            }
+            best_ms = std::min(best_ms, c->ms_pgs_kernel);
+        }
+        if (err) break;
+        cands[k].ms = best_ms / sweeps;
+        std::vector<double>& L = k == 0 ? ref_lam : got_lam;
+        std::vector<double>& V = k == 0 ? ref_dv : got_dv;
+        hipError_t e = hipMemcpyAsync(L.data(), c->c_lam.p, nc3 * 8, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(V.data(), c->d_dv.p, n_dv * 8, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            err = fail(c, AM_ERR_HIP, hipGetErrorString(e));
+            break;
+        }
+        if (k > 0 && (std::memcmp(L.data(), ref_lam.data(), nc3 * 8) != 0 ||
+                      std::memcmp(V.data(), ref_dv.data(), n_dv * 8) != 0)) {
+            char msg[160];
+            std::snprintf(msg, sizeof msg, "launch shape (schedule %d, grid %d, block %d) changed the solve's bits",
+                          cands[k].sched, cands[k].grid_used, cands[k].block_used);
+            err = fail(c, AM_ERR_MISMATCH, msg);
+        }
+    }
+
+    // put back what the trials overwrote, also after a failed trial
+    c->params = saved;
+    hipError_t e = restore_inputs();
+    if (e == hipSuccess) e = d2d(c->c_b.p, b0.p, nc3 * 8);
+    if (e == hipSuccess) e = d2d(c->c_D.p, D0.p, nc3 * 8);
+    if (e == hipSuccess) e = d2d(c->s_start.p, start0.p, nc3 * 8);
+    if (e == hipSuccess) e = d2d(c->c_state.p, st0.p, nc * 4);
+    hipError_t e2 = hipStreamSynchronize(s);  // before the saved copies are freed
+    c->last_iters = last_iters;
+    c->sweep_order_n = sweep_order_n;
+    c->used_learned = used_learned;
+    c->last_change = last_change;
+    c->ms_pgs_kernel = ms_pgs;
+    c->ms_setup_kernel = ms_setup;
+    c->ms_learned = ms_learned;
+    c->res_plain = res_plain;
+    c->res_learned = res_learned;
+    c->changes = changes;
+    if (err) return err;
+    if (e == hipSuccess) e = e2;
+    if (e != hipSuccess) return fail(c, AM_ERR_HIP, hipGetErrorString(e));
+
+    int kb = 0;
+    for (size_t k = 1; k < cands.size(); k++)
+        if (cands[k].ms < cands[kb].ms) kb = (int)k;
+    for (int k = 0; k < std::min(capacity, (int)cands.size()); k++) {
+        am_tune_trial& t = trials[k];
+        t.pgs_schedule = cands[k].sched;
+        t.pgs_grid = cands[k].grid;
+        t.pgs_block = cands[k].block;
+        t.grid_used = cands[k].grid_used;
+        t.block_used = cands[k].block_used;
+        t.reserved = 0;
+        t.ms_per_sweep = cands[k].ms;
+    }
+    if (n_trials) *n_trials = (int)cands.size();
+    if (best) *best = kb;
+    if (apply) {
+        c->params.pgs_schedule = cands[kb].sched;
+        c->params.pgs_grid = cands[kb].grid;
+        c->params.pgs_block = cands[kb].block;
+    }
+    return AM_OK;
 }
 
 int am_update_in_collections(am_ctx* c, double dt, int32_t* iterations_used) {

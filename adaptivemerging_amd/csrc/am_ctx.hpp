@@ -169,7 +169,7 @@ struct am_ctx {
     int last_iters = 0;
     double last_change = 0.0;
     std::vector<double> changes;
-    int pgs_max_grid = 0;
+    int pgs_max_grid[5] = {0, 0, 0, 0, 0};  // co-resident workgroups of the coloured kernel, by block / 64
     hipEvent_t kev[3] = {nullptr, nullptr, nullptr};  // setup start, pgs start, pgs end
     double ms_pgs_kernel = 0.0, ms_setup_kernel = 0.0;
 

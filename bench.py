@@ -186,6 +186,10 @@ def main() -> int:
     ap.add_argument("--pgs-schedule", choices=["dataflow", "coloured"], default="dataflow",
                     help="dataflow: per-body update counters; coloured: a grid barrier between colours "
                          "(bit-identical results)")
+    ap.add_argument("--autotune", choices=["on", "off"], default="on",
+                    help="after the warm-up, time the PGS kernel's launch shapes on the live contacts (both "
+                         "schedules; bit-identical results checked) and run the timed steps with the fastest; "
+                         "overrides --pgs-schedule/--pgs-grid/--pgs-block")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (calibrates the HBM byte counters) and exit")
     args = ap.parse_args()
@@ -246,6 +250,25 @@ def main() -> int:
             print(f"# warmup {k}: contacts {st['n_contacts']} sweeps {st['iterations_used']} "
                   f"residual {st['last_change']:.3g} colours {st['n_colours']} step {st['ms_total']:.2f} ms",
                   file=sys.stderr, flush=True)
+
+    tuned = None
+    if args.autotune == "on":
+        t1 = time.perf_counter()
+        trials, best = plain.autotune(args.dt, sweeps=20, reps=3)
+        chosen = trials[best]
+        auto = next(t for t in trials if t["pgs_schedule"] == 0 and t["pgs_grid"] == 0)
+        tuned = dict(chosen=chosen, trials=len(trials), default_ms_per_sweep=auto["ms_per_sweep"],
+                     tune_s=time.perf_counter() - t1)
+        args.pgs_schedule = "dataflow" if chosen["pgs_schedule"] == 0 else "coloured"
+        shape = {k: chosen[k] for k in ("pgs_schedule", "pgs_grid", "pgs_block")}
+        params_kw.update(shape)
+        if twin is not None:
+            twin.set_params(**shape)
+        for t in trials:
+            print(f"# tune: schedule {t['pgs_schedule']} grid {t['grid_used']} block {t['block_used']}: "
+                  f"{t['ms_per_sweep'] * 1e3:.2f} us/sweep", file=sys.stderr, flush=True)
+        print(f"# tuned in {tuned['tune_s']:.1f} s: schedule {args.pgs_schedule}, grid {chosen['grid_used']}, "
+              f"block {chosen['block_used']}", file=sys.stderr, flush=True)
 
     learned_info = None
     if twin is not None:
@@ -323,6 +346,7 @@ def main() -> int:
             "plain": res_p,
             "learned": res_l,
             "parallelism": "none (1 GPU)",
+            "launch_shape": tuned,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

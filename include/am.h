@@ -50,6 +50,7 @@ extern "C" {
 #define AM_ERR_CAPACITY (-3)  /* a buffer would overflow; grown, call again */
 #define AM_ERR_STATE (-4)     /* call out of order (e.g. solve before detect) */
 #define AM_ERR_NUMERIC (-5)   /* NaN/Inf in body state */
+#define AM_ERR_MISMATCH (-6)  /* two launch shapes of one kernel gave different bits */
 
 enum am_geom {
     AM_GEOM_BOX = 0,        /* RigidBodyGeomBox */
@@ -444,6 +445,29 @@ int am_start_lambdas_pull(am_ctx* ctx, double* lambda, int32_t capacity, int32_t
 /* For each pair (a,b) of entity ids: the largest relative velocity over both
  * bodies' bounding-box corners; b < 0 means the one-body variant. */
 int am_motion_metric(am_ctx* ctx, const int32_t* pairs, int32_t n_pairs, double* out);
+
+/* ---- launch-shape autotuner (no reference counterpart) ----------------
+ * Times the PGS kernel on the contact set currently held (after am_detect
+ * and a warm start, or after an am_step) under each candidate launch shape:
+ * the dataflow schedule at several wave counts and the coloured schedule at
+ * several workgroup sizes and grids.  Every trial runs `sweeps` sweeps with
+ * no early exit from the same lambdas and deltaV; the lambdas and deltaV of
+ * every trial must equal the first trial's bit for bit, else the call fails
+ * with AM_ERR_MISMATCH.  The contacts, lambdas, deltaV and the
+ * last-solve statistics are restored afterwards.  With apply != 0 the
+ * fastest shape is written into the context's pgs_schedule / pgs_grid /
+ * pgs_block; am_params_set overrides it later like any other parameter. */
+typedef struct am_tune_trial {
+    int32_t pgs_schedule;    /* 0 dataflow, 1 coloured */
+    int32_t pgs_grid;        /* as requested (0 = the automatic choice) */
+    int32_t pgs_block;       /* as requested (0 = the automatic choice) */
+    int32_t grid_used;       /* workgroups launched */
+    int32_t block_used;      /* threads per workgroup launched */
+    int32_t reserved;
+    double ms_per_sweep;     /* best of `reps` kernel times / sweeps */
+} am_tune_trial;
+int am_autotune(am_ctx* ctx, double dt, int32_t sweeps, int32_t reps, int32_t apply, am_tune_trial* trials,
+                int32_t capacity, int32_t* n_trials, int32_t* best);
 
 #ifdef __cplusplus
 }

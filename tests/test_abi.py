@@ -56,6 +56,7 @@ int main(void) {
   printf("sizeof am_step_stats %zu\n", sizeof(am_step_stats));
   printf("sizeof am_body_pair %zu\n", sizeof(am_body_pair));
   printf("sizeof am_spring %zu\n", sizeof(am_spring));
+  printf("sizeof am_tune_trial %zu\n", sizeof(am_tune_trial));
   F(am_body_desc, size) F(am_body_desc, radius) F(am_body_desc, jinv0) F(am_body_desc, plane_d) F(am_body_desc, bbox)
   F(am_body_desc, inertia0)
   F(am_body_pair, flag) F(am_body_pair, collection) F(am_body_pair, hist) F(am_body_pair, states)
@@ -67,6 +68,7 @@ int main(void) {
   F(am_spring, type) F(am_spring, body2) F(am_spring, pb1) F(am_spring, pb2) F(am_spring, pw) F(am_spring, l0)
   F(am_spring, k) F(am_spring, d) F(am_spring, ls)
   F(am_params, spring_stiffness_mod) F(am_params, spring_damping_mod) F(am_params, pgs_schedule)
+  F(am_tune_trial, grid_used) F(am_tune_trial, block_used) F(am_tune_trial, ms_per_sweep)
   return 0;
 }
 """
@@ -86,6 +88,7 @@ def test_struct_layouts_match_ctypes(tmp_path):
     assert sizes["am_step_stats"] == C.sizeof(_lib.StepStats)
     assert sizes["am_body_pair"] == C.sizeof(_lib.BodyPair)
     assert sizes["am_spring"] == _lib.SPRING_DTYPE.itemsize
+    assert sizes["am_tune_trial"] == C.sizeof(_lib.TuneTrial)
     for ln in lines:
         if ln.startswith("sizeof"):
             continue
@@ -102,6 +105,8 @@ def test_struct_layouts_match_ctypes(tmp_path):
             assert getattr(_lib.BodyPair, field).offset == int(off), field
         elif struct == "am_spring":
             assert _lib.SPRING_DTYPE.fields[field][1] == int(off), field
+        elif struct == "am_tune_trial":
+            assert getattr(_lib.TuneTrial, field).offset == int(off), field
 
 
 def test_params_defaults_are_reference_defaults(libam):
@@ -125,6 +130,7 @@ def test_error_codes_without_gpu(libam):
     assert libam.am_step(None, 0.01, None) == _lib.AM_ERR_INVALID
     assert libam.am_springs_set(None, None, 0) == _lib.AM_ERR_INVALID
     assert libam.am_apply_spring_forces(None) == _lib.AM_ERR_INVALID
+    assert libam.am_autotune(None, 0.01, 20, 3, 1, None, 0, None, None) == _lib.AM_ERR_INVALID
     h = C.c_void_p()
     rc = libam.am_ctx_create(-1, C.byref(h))
     assert rc != 0 and not h.value
