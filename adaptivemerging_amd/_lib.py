@@ -82,6 +82,7 @@ class Params(C.Structure):
         ("enable_unmerge_normal", C.c_int32), ("enable_unmerge_relative_motion", C.c_int32),
         ("update_contacts_in_collections", C.c_int32), ("step_accum_merging", C.c_int32),
         ("step_accum_unmerging", C.c_int32), ("steps_between_merge_events", C.c_int32),
+        ("metric_position_level", C.c_int32),
         ("threshold_merge", C.c_double), ("threshold_unmerge", C.c_double), ("threshold_breath", C.c_double),
         ("pgs_block", C.c_int32), ("pgs_grid", C.c_int32), ("pgs_plain_launch", C.c_int32),
         ("pgs_schedule", C.c_int32),
@@ -218,11 +219,11 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "am_solve_order_pull": (C.c_int, [ctxp, vp, i32, C.POINTER(i32)]),
         "am_colours_pull": (C.c_int, [ctxp, vp, i32, C.POINTER(i32)]),
         "am_update_in_collections": (C.c_int, [ctxp, dbl, C.POINTER(i32)]),
-        "am_accumulate_for_unmerging": (C.c_int, [ctxp]),
+        "am_accumulate_for_unmerging": (C.c_int, [ctxp, dbl]),
         "am_unmerge": (C.c_int, [ctxp, dbl, C.POINTER(i32)]),
         "am_clear_top_level": (C.c_int, [ctxp]),
         "am_clear_body_pair_contacts": (C.c_int, [ctxp]),
-        "am_accumulate_for_merging": (C.c_int, [ctxp]),
+        "am_accumulate_for_merging": (C.c_int, [ctxp, dbl]),
         "am_merge": (C.c_int, [ctxp, C.POINTER(i32)]),
         "am_body_pairs_pull": (C.c_int, [ctxp, vp, i32, C.POINTER(i32), C.POINTER(i32)]),
         "am_internal_contacts_pull": (C.c_int, [ctxp, C.POINTER(ContactArrays), i32, C.POINTER(i32)]),
@@ -284,6 +285,7 @@ _DEFAULTS = dict(
     enable_merging=1, enable_merge_pinned=1, enable_merge_stable_contact=1, enable_merge_let_it_breathe=1,
     enable_unmerging=1, enable_unmerge_friction=1, enable_unmerge_normal=1, enable_unmerge_relative_motion=1,
     update_contacts_in_collections=1, step_accum_merging=3, step_accum_unmerging=3, steps_between_merge_events=10,
+    metric_position_level=0,
     threshold_merge=1e-2, threshold_unmerge=2e-2, threshold_breath=1e-5, pgs_block=0, pgs_grid=0, pgs_plain_launch=0, pgs_schedule=0,
     learned_warm_start=0, collect_training=0,
 )

@@ -226,8 +226,8 @@ class Context:
         return it.value
 
     # -- merging --------------------------------------------------------
-    def accumulate_for_unmerging(self):
-        self._check(self.lib.am_accumulate_for_unmerging(self.h))
+    def accumulate_for_unmerging(self, dt: float):
+        self._check(self.lib.am_accumulate_for_unmerging(self.h, dt))
 
     def unmerge(self, dt: float) -> int:
         n = C.c_int32()
@@ -241,8 +241,8 @@ class Context:
     def clear_body_pair_contacts(self):
         self._check(self.lib.am_clear_body_pair_contacts(self.h))
 
-    def accumulate_for_merging(self):
-        self._check(self.lib.am_accumulate_for_merging(self.h))
+    def accumulate_for_merging(self, dt: float):
+        self._check(self.lib.am_accumulate_for_merging(self.h, dt))
 
     def merge(self) -> int:
         n = C.c_int32()

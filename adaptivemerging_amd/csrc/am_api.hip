@@ -193,6 +193,7 @@ void default_params(am_params* p) {
     p->step_accum_merging = 3;
     p->step_accum_unmerging = 3;
     p->steps_between_merge_events = 10;
+    p->metric_position_level = 0;
     p->threshold_merge = 1e-2;
     p->threshold_unmerge = 2e-2;
     p->threshold_breath = 1e-5;
@@ -1619,10 +1620,10 @@ int am_update_in_collections(am_ctx* c, double dt, int32_t* iterations_used) {
     return r;
 }
 
-int am_accumulate_for_unmerging(am_ctx* c) {
+int am_accumulate_for_unmerging(am_ctx* c, double dt) {
     int r = check_ctx(c);
     if (r) return r;
-    return merge_accumulate_unmerging(c);
+    return merge_accumulate_unmerging(c, dt);
 }
 
 int am_unmerge(am_ctx* c, double dt, int32_t* n_cut) {
@@ -1648,10 +1649,10 @@ int am_clear_body_pair_contacts(am_ctx* c) {
     return merge_body_pairs(c);
 }
 
-int am_accumulate_for_merging(am_ctx* c) {
+int am_accumulate_for_merging(am_ctx* c, double dt) {
     int r = check_ctx(c);
     if (r) return r;
-    return merge_accumulate_merging(c);
+    return merge_accumulate_merging(c, dt);
 }
 
 int am_merge(am_ctx* c, int32_t* n_merged) {
@@ -1807,7 +1808,7 @@ int am_step(am_ctx* c, double dt, am_step_stats* st) {
     int sweep_it = 0, cut = 0, merged = 0;
     if (merging) {
         if ((r = merge_update_in_collections(c, dt, &sweep_it))) return r;
-        if ((r = merge_accumulate_unmerging(c))) return r;
+        if ((r = merge_accumulate_unmerging(c, dt))) return r;
         if ((r = merge_unmerge(c, dt, &cut))) return r;
         if (c->merging_event) {
             launch_clear_targets(s, bodies_view(c), c->d_slot_entity.p, (int)c->slot_entity.size());
@@ -1826,7 +1827,7 @@ int am_step(am_ctx* c, double dt, am_step_stats* st) {
     if (P.enable_merging && (r = merge_body_pairs(c))) return r;
     AM_HIP(c, hipEventRecord(ev[7], s));
     if ((r = am_advance_velocities(c, dt))) return r;
-    if (P.enable_merging && (r = merge_accumulate_merging(c))) return r;
+    if (P.enable_merging && (r = merge_accumulate_merging(c, dt))) return r;
     if ((r = am_advance_positions(c, dt))) return r;
     AM_HIP(c, hipEventRecord(ev[8], s));
     if (P.enable_merging && P.steps_between_merge_events > 0 && c->total_steps % P.steps_between_merge_events == 0)

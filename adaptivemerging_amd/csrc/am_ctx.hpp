@@ -149,6 +149,8 @@ struct am_ctx {
     DBuf<unsigned> f_cnt, d_pass_done;
     DBuf<double> dv_alt, s_lam_alt;
     int flow_max_grid = 0;
+    // poses advanced by dt for the position-level motion metric
+    DBuf<double> pose_xa, pose_Ra;
     // springs (RigidBodySystem.springs)
     DBuf<am_spring> d_springs;
     std::vector<am_spring> springs_host;
@@ -244,10 +246,10 @@ hipError_t upload(am_ctx* c, DBuf<T>& b, const std::vector<T>& v) {
 
 // merging phases (am_merge.hip)
 int merge_update_in_collections(am_ctx* c, double dt, int* iters);
-int merge_accumulate_unmerging(am_ctx* c);
+int merge_accumulate_unmerging(am_ctx* c, double dt);
 int merge_unmerge(am_ctx* c, double dt, int* n_events);
 int merge_body_pairs(am_ctx* c);
-int merge_accumulate_merging(am_ctx* c);
+int merge_accumulate_merging(am_ctx* c, double dt);
 int merge_merge(am_ctx* c, int* n_merged);
 int merge_unmerge_all(am_ctx* c, int* n_dissolved);
 int merge_push_down(am_ctx* c, bool integrated_only);

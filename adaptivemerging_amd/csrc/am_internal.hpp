@@ -93,6 +93,31 @@ AMD_INLINE double metric_two(const BodiesView& B, int a, int b) {
     }
     return largest;
 }
+// The same measure at the position level (:75-116): each corner of one body
+// goes into the other body's frame, out again through the other body's pose
+// advanced by dt, back through the first body's advanced pose, and its
+// displacement over dt is the speed.  xa/Ra hold every entity's pose after
+// RigidBody.advancePositions(dt) on a copy.
+AMD_INLINE double metric_two_pos(const BodiesView& B, const double* xa, const double* Ra, int a, int b, double dt) {
+    double largest = 0;
+    for (int s = 0; s < 2; s++) {
+        const int self = s == 0 ? a : b, other = s == 0 ? b : a;
+        const m33 R1 = ldm(&B.R[self * 9]), R2 = ldm(&B.R[other * 9]);
+        const m33 R1a = ldm(&Ra[self * 9]), R2a = ldm(&Ra[other * 9]);
+        const d3 x1 = ld3(&B.x[self * 3]), x2 = ld3(&B.x[other * 3]);
+        const d3 x1a = ld3(&xa[self * 3]), x2a = ld3(&xa[other * 3]);
+        for (int q = 0; q < 8; q++) {
+            const d3 p = ld3(&B.desc[self].bbox[q * 3]);
+            const d3 pB = inv_xform(R2, x2, xform(R1, x1, p));
+            const d3 back = inv_xform(R1a, x1a, xform(R2a, x2a, pB));
+            const double s_dt = 1. / dt;
+            d3 v = sub(p, back);
+            v = d3{v.x * s_dt, v.y * s_dt, v.z * s_dt};
+            largest = jmax(length(v), largest);
+        }
+    }
+    return largest;
+}
 
 // Previous step's contacts needed by the warm start.
 struct OldContactsView {

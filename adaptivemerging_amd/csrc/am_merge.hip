@@ -207,22 +207,24 @@ void launch_bpc_hash(hipStream_t s, const BpcRec* bpc, int n, int* hash, int mas
 
 AMD_INLINE int top_of(const BodiesView& B, int b) { return B.parent[b] >= 0 ? B.parent[b] : b; }
 AMD_INLINE bool pinned(const BodiesView& B, int e) { return (B.flags[e] & AM_BODY_PINNED) != 0; }
-AMD_INLINE double pair_metric(const BodiesView& B, int a, int b) {
+// BodyPairContact.java:88-95 and :130-137: a pinned body leaves the other's
+// own motion; otherwise the relative motion at the velocity or position level.
+AMD_INLINE double pair_metric(const BodiesView& B, int a, int b, const PoseLevel& L) {
     if (pinned(B, a)) return metric_one(B, b);
     if (pinned(B, b)) return metric_one(B, a);
-    return metric_two(B, a, b);
+    return L.xa ? metric_two_pos(B, L.xa, L.Ra, a, b, L.dt) : metric_two(B, a, b);
 }
 
 // accumulateForMerging (BodyPairContact.java:83-121) followed by the parts of
 // checkMergeCondition (:151-177) that do not depend on the merges done
 // earlier in the same call: breathing, pinned bodies, the motion-metric
 // history (:187-198) and stable contacts (:212-224).
-__global__ void k_bpc_accumulate(BodiesView B, BpcRec* bpc, int n, MergeParams P) {
+__global__ void k_bpc_accumulate(BodiesView B, BpcRec* bpc, int n, MergeParams P, PoseLevel L) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     BpcRec& r = bpc[i];
     const int accum = P.step_accum_merging;
-    double metric = pair_metric(B, top_of(B, r.b1), top_of(B, r.b2));
+    double metric = pair_metric(B, top_of(B, r.b1), top_of(B, r.b2), L);
     // motionMetricHist.add, then drop the oldest beyond stepAccumMerging
     if (r.hist_n < accum) {
         r.hist[r.hist_n++] = metric;
@@ -260,20 +262,20 @@ __global__ void k_bpc_accumulate(BodiesView B, BpcRec* bpc, int n, MergeParams P
     }
     r.flag = flag;
 }
-void launch_bpc_accumulate(hipStream_t s, BodiesView B, BpcRec* bpc, int n, MergeParams P) {
-    if (n > 0) hipLaunchKernelGGL(k_bpc_accumulate, dim3((n + 127) / 128), dim3(128), 0, s, B, bpc, n, P);
+void launch_bpc_accumulate(hipStream_t s, BodiesView B, BpcRec* bpc, int n, MergeParams P, PoseLevel L) {
+    if (n > 0) hipLaunchKernelGGL(k_bpc_accumulate, dim3((n + 127) / 128), dim3(128), 0, s, B, bpc, n, P, L);
 }
 
 // accumulateForUnmerging (BodyPairContact.java:127-145) on the members'
 // velocities after the single sweep, then the unmerge test of
 // Merging.unmerge: checkContactsState (:255-266) or
 // checkMotionMetricForUnmerging (:200-205).
-__global__ void k_ibpc_accumulate(BodiesView B, IbpcRec* ib, int n, ContactsView IC, MergeParams P) {
+__global__ void k_ibpc_accumulate(BodiesView B, IbpcRec* ib, int n, ContactsView IC, MergeParams P, PoseLevel L) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     IbpcRec& r = ib[i];
     if (B.flags[r.coll] & AM_BODY_SLEEPING) return;
-    double metric = pair_metric(B, r.b1, r.b2);
+    double metric = pair_metric(B, r.b1, r.b2, L);
     if (metric > P.threshold_unmerge) r.hist_n = min(r.hist_n + 1, 1 << 30);
     else r.hist_n = 0;
     int flag = 0;
@@ -285,8 +287,9 @@ __global__ void k_ibpc_accumulate(BodiesView B, IbpcRec* ib, int n, ContactsView
     if (!flag && r.hist_n >= P.step_accum_unmerging && r.count < 3) flag = 1;
     r.flag = flag;
 }
-void launch_ibpc_accumulate(hipStream_t s, BodiesView B, IbpcRec* ib, int n, ContactsView IC, MergeParams P) {
-    if (n > 0) hipLaunchKernelGGL(k_ibpc_accumulate, dim3((n + 127) / 128), dim3(128), 0, s, B, ib, n, IC, P);
+void launch_ibpc_accumulate(hipStream_t s, BodiesView B, IbpcRec* ib, int n, ContactsView IC, MergeParams P,
+                            PoseLevel L) {
+    if (n > 0) hipLaunchKernelGGL(k_ibpc_accumulate, dim3((n + 127) / 128), dim3(128), 0, s, B, ib, n, IC, P, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -1138,10 +1141,29 @@ int merge_update_in_collections(am_ctx* c, double dt, int* iters) {
     return AM_OK;
 }
 
-int merge_accumulate_unmerging(am_ctx* c) {
-    if (c->ibpc.empty()) return AM_OK;
+// The advanced poses the position-level metric reads, computed from the
+// current state; L.xa stays null at the velocity level.
+static int pose_level(am_ctx* c, double dt, PoseLevel* L) {
+    L->xa = nullptr;
+    L->Ra = nullptr;
+    L->dt = dt;
+    if (!c->params.metric_position_level) return AM_OK;
+    if (!(dt > 0)) return fail(c, AM_ERR_INVALID, "the position-level metric needs dt > 0");
+    AM_HIP(c, c->pose_xa.ensure((size_t)std::max(c->n_ent, 1) * 3));
+    AM_HIP(c, c->pose_Ra.ensure((size_t)std::max(c->n_ent, 1) * 9));
+    launch_advanced_poses(c->stream, bodies_view(c), dt, c->pose_xa.p, c->pose_Ra.p);
+    AM_HIP(c, hipGetLastError());
+    L->xa = c->pose_xa.p;
+    L->Ra = c->pose_Ra.p;
+    return AM_OK;
+}
+
+int merge_accumulate_unmerging(am_ctx* c, double dt) {
+    PoseLevel L;
+    int r = pose_level(c, dt, &L);
+    if (r || c->ibpc.empty()) return r;
     launch_ibpc_accumulate(c->stream, bodies_view(c), c->d_ibpc.p, (int)c->ibpc.size(), internal_view(c),
-                           merge_params(c->params));
+                           merge_params(c->params), L);
     AM_HIP(c, hipGetLastError());
     return AM_OK;
 }
@@ -1180,8 +1202,11 @@ int merge_body_pairs(am_ctx* c) {
     return AM_OK;
 }
 
-int merge_accumulate_merging(am_ctx* c) {
-    launch_bpc_accumulate(c->stream, bodies_view(c), c->d_bpc.p, c->n_bpc, merge_params(c->params));
+int merge_accumulate_merging(am_ctx* c, double dt) {
+    PoseLevel L;
+    int r = pose_level(c, dt, &L);
+    if (r) return r;
+    launch_bpc_accumulate(c->stream, bodies_view(c), c->d_bpc.p, c->n_bpc, merge_params(c->params), L);
     AM_HIP(c, hipGetLastError());
     return AM_OK;
 }

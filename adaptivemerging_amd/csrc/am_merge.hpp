@@ -59,8 +59,16 @@ void launch_bpc_build(hipStream_t s, ContactsView C, const int* n_contacts, cons
                       const BpcRec* prev, int n_prev, const int* prev_hash, int prev_mask, BpcRec* out,
                       double threshold_breath);
 void launch_bpc_hash(hipStream_t s, const BpcRec* bpc, int n, int* hash, int mask);
-void launch_bpc_accumulate(hipStream_t s, BodiesView B, BpcRec* bpc, int n, MergeParams P);
-void launch_ibpc_accumulate(hipStream_t s, BodiesView B, IbpcRec* ib, int n, ContactsView IC, MergeParams P);
+// Poses advanced by dt for the position-level metric; xa == nullptr selects
+// the velocity-level metric.
+struct PoseLevel {
+    const double* xa;
+    const double* Ra;
+    double dt;
+};
+void launch_bpc_accumulate(hipStream_t s, BodiesView B, BpcRec* bpc, int n, MergeParams P, PoseLevel L);
+void launch_ibpc_accumulate(hipStream_t s, BodiesView B, IbpcRec* ib, int n, ContactsView IC, MergeParams P,
+                            PoseLevel L);
 // mode 0 copy, 1 fresh copy (lambda_warm, b, D zeroed, not new), 2 lambda_warm
 // = lambda; 3 scatter: dst[idx[k]] = src[src_off + k]
 void launch_gather_contacts(hipStream_t s, ContactsView dst, int dst_off, ContactsView src, const int* idx,

@@ -1601,19 +1601,13 @@ __device__ void sincos(double x, double& s, double& c) {
 }
 }  // namespace fdm
 
-// RigidBody.advancePositions (RigidBody.java:427-441): Rodrigues rotation
-// (expRodrigues, RigidBody.java:382-401) and Matrix3d.normalizeCP.
-__global__ void k_advance_positions(BodiesView B, const int* targets, int n, double dt) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    int i = targets[k];
-    d3 v = ld3(&B.v[i * 3]);
-    d3 x = scale_add(dt, v, ld3(&B.x[i * 3]));
-    st3(&B.x[i * 3], x);
-    d3 om = ld3(&B.w[i * 3]);
+// The pose update of RigidBody.advancePositions (RigidBody.java:427-437):
+// Rodrigues rotation (expRodrigues, RigidBody.java:382-401) and
+// Matrix3d.normalizeCP.  Returns whether R changed.
+__device__ __forceinline__ bool advance_pose(d3& x, m33& R, d3 v, d3 om, double dt) {
+    x = scale_add(dt, v, x);
     double t = length(om) * dt;
     d3 d = normalized(om);
-    m33 R = ldm(&B.R[i * 9]);
     if (t > 1e-8) {
         double s, c;
         fdm::sincos(t, s, c);
@@ -1641,9 +1635,37 @@ __global__ void k_advance_positions(BodiesView B, const int* targets, int n, dou
         R.m[2] = R.m[3] * R.m[7] - R.m[4] * R.m[6];
         R.m[5] = R.m[1] * R.m[6] - R.m[0] * R.m[7];
         R.m[8] = R.m[0] * R.m[4] - R.m[1] * R.m[3];
-        stm(&B.R[i * 9], R);
+        return true;
     }
+    return false;
+}
+
+// RigidBody.advancePositions (RigidBody.java:427-441) of the listed entities.
+__global__ void k_advance_positions(BodiesView B, const int* targets, int n, double dt) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int i = targets[k];
+    d3 x = ld3(&B.x[i * 3]);
+    m33 R = ldm(&B.R[i * 9]);
+    const bool turned = advance_pose(x, R, ld3(&B.v[i * 3]), ld3(&B.w[i * 3]), dt);
+    st3(&B.x[i * 3], x);
+    if (turned) stm(&B.R[i * 9], R);
     stm(&B.jinv[i * 9], rm0rt(R, ldm(B.desc[i].jinv0)));
+}
+
+// Every entity's pose advanced by dt without changing the body state (the
+// copies body1Advanced / body2Advanced of MotionMetricProcessor.java:79-83).
+__global__ void k_advanced_poses(BodiesView B, double dt, double* xa, double* Ra) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.n) return;
+    d3 x = ld3(&B.x[i * 3]);
+    m33 R = ldm(&B.R[i * 9]);
+    advance_pose(x, R, ld3(&B.v[i * 3]), ld3(&B.w[i * 3]), dt);
+    st3(&xa[i * 3], x);
+    stm(&Ra[i * 9], R);
+}
+void launch_advanced_poses(hipStream_t s, BodiesView B, double dt, double* xa, double* Ra) {
+    if (B.n > 0) hipLaunchKernelGGL(k_advanced_poses, dim3((B.n + 127) / 128), dim3(128), 0, s, B, dt, xa, Ra);
 }
 
 void launch_advance(hipStream_t s, BodiesView B, const int* targets, int n, double dt, bool positions) {

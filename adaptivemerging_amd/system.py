@@ -191,6 +191,7 @@ class Merging:
         self.thresholdMerge = 1e-2
         self.thresholdUnmerge = 2e-2
         self.thresholdBreath = 1e-5
+        self.metricPositionLevel = False  # MergeParameters.metricPositionLevel
 
     def apply(self):
         p = self.ctx.params
@@ -209,6 +210,7 @@ class Merging:
         p.threshold_merge = self.thresholdMerge
         p.threshold_unmerge = self.thresholdUnmerge
         p.threshold_breath = self.thresholdBreath
+        p.metric_position_level = int(self.metricPositionLevel)
         self.ctx.set_params()
 
     def unmergeAll(self) -> int:
@@ -327,7 +329,7 @@ class RigidBodySystem:
         if self.merging.updateContactsInCollections:
             self.collision.updateInCollections(dt)
         t1 = time.perf_counter()
-        self.ctx.accumulate_for_unmerging()
+        self.ctx.accumulate_for_unmerging(dt)
         if self.merging.unmerge(dt):
             self.merging.unmergingBuildTime = time.perf_counter() - t1
         else:
@@ -343,7 +345,7 @@ class RigidBodySystem:
         self.collision.clearBodyPairContacts()
         self.ctx.advance_velocities(dt)
         t1 = time.perf_counter()
-        self.ctx.accumulate_for_merging()
+        self.ctx.accumulate_for_merging(dt)
         self.mergingTime = time.perf_counter() - t1
         self.ctx.advance_positions(dt)
         self.merging.mergingBuildTime = 0.0

@@ -154,6 +154,8 @@ typedef struct am_params {
     int32_t step_accum_merging;       /* 3 */
     int32_t step_accum_unmerging;     /* 3 */
     int32_t steps_between_merge_events; /* 10 */
+    int32_t metric_position_level;    /* 0: the merge and unmerge metrics compare the bodies' poses advanced
+                                         by dt instead of their velocities (Merging.metricPositionLevel) */
     double threshold_merge;           /* 1e-2 */
     double threshold_unmerge;         /* 2e-2 */
     double threshold_breath;          /* 1e-5 */
@@ -369,8 +371,9 @@ int am_step(am_ctx* ctx, double dt, am_step_stats* stats);
  * The phases of RigidBodySystem.advanceTime (RigidBodySystem.java:102-185)
  * that am_step runs when merging is on; callers that drive the phases one by
  * one call them in this order. */
-/* RigidBodySystem.accumulateForUnmerging (:318-329) */
-int am_accumulate_for_unmerging(am_ctx* ctx);
+/* RigidBodySystem.accumulateForUnmerging (:318-329); dt is used by the
+ * position-level metric (must be > 0 when metric_position_level is set) */
+int am_accumulate_for_unmerging(am_ctx* ctx, double dt);
 /* Merging.unmerge (Merging.java:215-273); n_cut = collections cut */
 int am_unmerge(am_ctx* ctx, double dt, int32_t* n_cut);
 /* RigidBody.clear of the top-level entities and the external forces again,
@@ -379,8 +382,8 @@ int am_clear_top_level(am_ctx* ctx);
 /* CollisionProcessor.clearBodyPairContacts (:213-226) with the body-pair
  * bookkeeping of updateBodyPairContacts (:145-206) */
 int am_clear_body_pair_contacts(am_ctx* ctx);
-/* RigidBodySystem.accumulateForMerging (:331-334) */
-int am_accumulate_for_merging(am_ctx* ctx);
+/* RigidBodySystem.accumulateForMerging (:331-334); dt as above */
+int am_accumulate_for_merging(am_ctx* ctx, double dt);
 /* Merging.merge (Merging.java:73-163); n_merged = body pairs merged */
 int am_merge(am_ctx* ctx, int32_t* n_merged);
 /* ---- sleeping (Sleeping.java) ------------------------------------------

@@ -1645,13 +1645,12 @@ double orc_cos(double x) {
     }
 }
 
-/* RigidBody.advancePositions (RigidBody.java:427-441) with expRodrigues
- * (RigidBody.java:382-401) and Matrix3d.normalizeCP */
-static void advance_positions_one(orc_world* w, int32_t i, double dt) {
-    double* x = &w->x[i * 3];
-    double* om = &w->w[i * 3];
-    double* R = &w->R[i * 9];
-    for (int a = 0; a < 3; a++) x[a] = dt * w->v[i * 3 + a] + x[a];
+/* The pose update of RigidBody.advancePositions (RigidBody.java:427-437):
+ * x += dt v (Tuple3d.scaleAdd), then for a turn angle |omega| dt above 1e-8
+ * R = normalizeCP(expRodrigues(omega / |omega|, |omega| dt) R)
+ * (RigidBody.java:382-401, Matrix3d.normalizeCP) */
+static void advance_pose(double* x, double* R, const double* v, const double* om, double dt) {
+    for (int a = 0; a < 3; a++) x[a] = dt * v[a] + x[a];
     double t = len3(om) * dt;
     double d[3] = {om[0], om[1], om[2]};
     normalize3(d);
@@ -1676,6 +1675,11 @@ static void advance_positions_one(orc_world* w, int32_t i, double dt) {
         R[5] = R[1] * R[6] - R[0] * R[7];
         R[8] = R[0] * R[4] - R[1] * R[3];
     }
+}
+
+/* RigidBody.advancePositions (RigidBody.java:427-441) */
+static void advance_positions_one(orc_world* w, int32_t i, double dt) {
+    advance_pose(&w->x[i * 3], &w->R[i * 9], &w->v[i * 3], &w->w[i * 3], dt);
     update_jinv(w, i);
 }
 
@@ -1722,6 +1726,38 @@ double orc_motion_metric(const orc_world* w, int32_t a, int32_t b) {
             spatial_velocity(w, b, pW, v2);
             for (int q = 0; q < 3; q++) v1[q] = v1[q] - v2[q];
             largest = jmax(len3(v1), largest);
+        }
+    }
+    return largest;
+}
+
+/* MotionMetricProcessor.getLargestVelocityNorm(body1, body2, dt)
+ * (MotionMetricProcessor.java:75-116): copies of both bodies advanced by dt
+ * (RigidBody.set + advancePositions); each bounding-box corner of one body
+ * goes through the other body's current and advanced poses and back through
+ * its own advanced pose; the largest displacement over dt. */
+double orc_motion_metric_pos(const orc_world* w, int32_t a, int32_t b, double dt) {
+    double xa[2][3], Ra[2][9];
+    const int32_t ids[2] = {a, b};
+    for (int s = 0; s < 2; s++) {
+        memcpy(xa[s], &w->x[ids[s] * 3], sizeof xa[s]);
+        memcpy(Ra[s], &w->R[ids[s] * 9], sizeof Ra[s]);
+        advance_pose(xa[s], Ra[s], &w->v[ids[s] * 3], &w->w[ids[s] * 3], dt);
+    }
+    double largest = 0;
+    for (int s = 0; s < 2; s++) {
+        const int32_t self = ids[s], other = ids[1 - s];
+        for (int k = 0; k < 8; k++) {
+            const double* p = &w->desc[self].bbox[k * 3];
+            double pW[3], pB[3], v[3];
+            xform_point(pW, &w->R[self * 9], &w->x[self * 3], p);
+            inv_xform_point(pB, &w->R[other * 9], &w->x[other * 3], pW);
+            xform_point(pW, Ra[1 - s], xa[1 - s], pB);
+            inv_xform_point(pB, Ra[s], xa[s], pW);
+            for (int q = 0; q < 3; q++) v[q] = p[q] - pB[q];
+            const double inv_dt = 1. / dt;
+            for (int q = 0; q < 3; q++) v[q] = v[q] * inv_dt;
+            largest = jmax(len3(v), largest);
         }
     }
     return largest;

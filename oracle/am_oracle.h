@@ -116,6 +116,9 @@ double orc_cos(double x);
 
 /* MotionMetricProcessor.getLargestVelocityNorm; b < 0 -> one-body variant. */
 double orc_motion_metric(const orc_world* w, int32_t a, int32_t b);
+/* the position-level variant (MotionMetricProcessor.java:75-116), both bodies
+ * unpinned entities */
+double orc_motion_metric_pos(const orc_world* w, int32_t a, int32_t b, double dt);
 
 /* Full step without merging (RigidBodySystem.advanceTime with merging,
  * sleeping and post-stabilization off), sweeping contacts in list order

@@ -597,21 +597,24 @@ class MergingOracle:
         self.o.push_deltav(dv)
         return used
 
-    def _metric(self, a, b):
+    def _metric(self, a, b, dt):
+        """BodyPairContact.java:88-95 / :130-137."""
         if self.pinned(a):
             return self.o.motion_metric(b)
         if self.pinned(b):
             return self.o.motion_metric(a)
+        if self.params.metric_position_level:
+            return self.o.motion_metric_pos(a, b, dt)
         return self.o.motion_metric(a, b)
 
     # -- BodyPairContact histories -----------------------------------------
-    def accumulate_for_unmerging(self):
+    def accumulate_for_unmerging(self, dt):
         """RigidBodySystem.accumulateForUnmerging + the unmerge test."""
         P = self.params
         for p in self.ibpc:
             if self.sleeping(self.parent[p["b1"]]):
                 continue   # RigidBodySystem.accumulateForUnmerging skips sleeping collections (:326-329)
-            m = self._metric(p["b1"], p["b2"])
+            m = self._metric(p["b1"], p["b2"], dt)
             p["hist_n"] = p["hist_n"] + 1 if m > P.threshold_unmerge else 0
             p["flag"] = unmerge_condition(P, [c["state"] for c in p["contacts"]], p["hist_n"], len(p["contacts"]))
 
@@ -640,14 +643,14 @@ class MergingOracle:
         self.bpc = out
         self._contacts = c
 
-    def accumulate_for_merging(self):
+    def accumulate_for_merging(self, dt):
         """RigidBodySystem.accumulateForMerging + static part of checkMergeCondition."""
         P = self.params
         accum = P.step_accum_merging
         for r in self.bpc:
             p1 = self.parent[r["b1"]] if self.parent[r["b1"]] >= 0 else r["b1"]
             p2 = self.parent[r["b2"]] if self.parent[r["b2"]] >= 0 else r["b2"]
-            r["hist"].append(self._metric(p1, p2))
+            r["hist"].append(self._metric(p1, p2, dt))
             if len(r["hist"]) > accum:
                 r["hist"].pop(0)
             r["states"].append(pair_state(r["not_on_edge"], len(r["active"])))

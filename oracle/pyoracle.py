@@ -73,6 +73,7 @@ def load() -> C.CDLL:
         "orc_advance_velocities": (None, [vp, dbl]),
         "orc_advance_positions": (None, [vp, dbl]),
         "orc_motion_metric": (dbl, [vp, i32, i32]),
+        "orc_motion_metric_pos": (dbl, [vp, i32, i32, dbl]),
         "orc_step": (i32, [vp, dbl, C.POINTER(i32), C.POINTER(dbl)]),
         "orc_sin": (dbl, [dbl]),
         "orc_cos": (dbl, [dbl]),
@@ -279,6 +280,9 @@ class Oracle:
 
     def motion_metric(self, a, b=-1) -> float:
         return self.lib.orc_motion_metric(self.h, a, b)
+
+    def motion_metric_pos(self, a, b, dt) -> float:
+        return self.lib.orc_motion_metric_pos(self.h, a, b, dt)
 
     def step(self, dt):
         it, last = C.c_int32(), C.c_double()

@@ -84,7 +84,7 @@ def merging_oracle_step(mo, dt, step, order=None, kick=None):
     n_int = sum(len(q["contacts"]) for q in mo.ibpc if not mo.sleeping(mo.parent[q["b1"]]))
     mo.update_in_collections(dt, (lambda: order(n + n_int)) if order else
                              (lambda: np.arange(n + n_int, dtype=np.int32)))
-    mo.accumulate_for_unmerging()
+    mo.accumulate_for_unmerging(dt)
     cut = mo.unmerge(dt)
     if mo.merging_event:
         mo.clear_top_level()
@@ -92,7 +92,7 @@ def merging_oracle_step(mo, dt, step, order=None, kick=None):
     mo.o.solve(np.arange(len(mo.o.contacts()["body1"]), dtype=np.int32), dt)
     mo.clear_body_pair_contacts()
     mo.advance_velocities(dt)
-    mo.accumulate_for_merging()
+    mo.accumulate_for_merging(dt)
     mo.advance_positions(dt)
     merged = 0
     steps = mo.params.steps_between_merge_events

@@ -14,10 +14,10 @@ pytestmark = pytest.mark.gpu
 PARAMS = dict(gravity_amount=9.81, iterations=100, tolerance=1e-10)
 
 
-def make(scene):
+def make(scene, **extra):
     ctx = Context()
     ctx.upload(scene)
-    ctx.set_params(**PARAMS)
+    ctx.set_params(**{**PARAMS, **extra})
     mo = MergingOracle(scene, type(ctx.params)())
     for k, _ in type(ctx.params)._fields_:
         setattr(mo.params, k, getattr(ctx.params, k))
@@ -45,7 +45,7 @@ def step_both(ctx, mo, dt, step, kick_body=None):
     ctx.update_in_collections(dt)
     mo.update_in_collections(dt, ctx.solve_order)
     assert_contacts_equal(ctx.internal_contacts(), mo.internal_contacts())
-    ctx.accumulate_for_unmerging(); mo.accumulate_for_unmerging()
+    ctx.accumulate_for_unmerging(dt); mo.accumulate_for_unmerging(dt)
     cut_g = ctx.unmerge(dt)
     cut_o = mo.unmerge(dt)
     assert cut_g == cut_o, f"step {step}: unmerge {cut_g} vs {cut_o}"
@@ -57,7 +57,7 @@ def step_both(ctx, mo, dt, step, kick_body=None):
     assert it_g == it_o
     ctx.clear_body_pair_contacts(); mo.clear_body_pair_contacts()
     ctx.advance_velocities(dt); mo.advance_velocities(dt)
-    ctx.accumulate_for_merging(); mo.accumulate_for_merging()
+    ctx.accumulate_for_merging(dt); mo.accumulate_for_merging(dt)
     ctx.advance_positions(dt); mo.advance_positions(dt)
     ext, _ = ctx.body_pairs()
     assert [(p["body1"], p["body2"], p["flag"], p["hist_n"]) for p in ext] == \
@@ -92,12 +92,14 @@ def compare_world(ctx, mo, step):
     assert_contacts_equal(ctx.internal_contacts(), mo.internal_contacts())
 
 
-def test_merge_and_unmerge_match_oracle(gpu_lib):
+@pytest.mark.parametrize("position_level", [0, 1])
+def test_merge_and_unmerge_match_oracle(gpu_lib, position_level):
     """Boards settle and merge (with the ground plane: a pinned collection),
     an upward force on a member breaks its contacts inside the collection and
-    it unmerges, then it merges again."""
+    it unmerges, then it merges again; with the velocity-level metric and
+    with the position-level one (Merging.metricPositionLevel)."""
     scene = workloads.merge_stack(seed=3)
-    ctx, mo = make(scene)
+    ctx, mo = make(scene, metric_position_level=position_level)
     dt = 0.01
     merges = cuts = 0
     for step in range(1, 41):
@@ -109,21 +111,22 @@ def test_merge_and_unmerge_match_oracle(gpu_lib):
     assert merges >= 2 and cuts >= 1
 
 
-def test_step_equals_phases(gpu_lib):
+@pytest.mark.parametrize("position_level", [0, 1])
+def test_step_equals_phases(gpu_lib, position_level):
     """am_step with merging (and sleeping) on runs exactly the phases above."""
     scene = workloads.merge_stack(seed=4)
-    a = Context(); a.upload(scene); a.set_params(**PARAMS)
-    b = Context(); b.upload(scene); b.set_params(**PARAMS)
+    a = Context(); a.upload(scene); a.set_params(**PARAMS, metric_position_level=position_level)
+    b = Context(); b.upload(scene); b.set_params(**PARAMS, metric_position_level=position_level)
     dt = 0.01
     for step in range(1, 26):
         st = a.step(dt)
         b.clear_bodies(); b.apply_gravity(); b.apply_spring_forces()
         b.update_contacts_map(); b.detect(dt); b.warm_start(); b.wake()
-        b.update_in_collections(dt); b.accumulate_for_unmerging(); b.unmerge(dt)
+        b.update_in_collections(dt); b.accumulate_for_unmerging(dt); b.unmerge(dt)
         if b.merging_event:
             b.clear_top_level()
         b.redo_warm_start(); b.solve(dt); b.clear_body_pair_contacts(); b.advance_velocities(dt)
-        b.accumulate_for_merging(); b.advance_positions(dt)
+        b.accumulate_for_merging(dt); b.advance_positions(dt)
         if step % 10 == 0:
             b.merge()
         b.sleep()

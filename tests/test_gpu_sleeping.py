@@ -57,7 +57,7 @@ def step_phases(ctx, mo, dt, step):
     ctx.update_in_collections(dt)
     mo.update_in_collections(dt, ctx.solve_order)
     assert_contacts_equal(ctx.internal_contacts(), mo.internal_contacts())
-    ctx.accumulate_for_unmerging(); mo.accumulate_for_unmerging()
+    ctx.accumulate_for_unmerging(dt); mo.accumulate_for_unmerging(dt)
     cut = ctx.unmerge(dt)
     assert cut == mo.unmerge(dt), f"step {step}: unmerge"
     if mo.merging_event:
@@ -68,7 +68,7 @@ def step_phases(ctx, mo, dt, step):
     assert it_g == it_o
     ctx.clear_body_pair_contacts(); mo.clear_body_pair_contacts()
     ctx.advance_velocities(dt); mo.advance_velocities(dt)
-    ctx.accumulate_for_merging(); mo.accumulate_for_merging()
+    ctx.accumulate_for_merging(dt); mo.accumulate_for_merging(dt)
     ctx.advance_positions(dt); mo.advance_positions(dt)
     merged = 0
     if step % ctx.params.steps_between_merge_events == 0:
@@ -131,11 +131,11 @@ def test_am_step_runs_the_sleeping_phases(gpu_lib):
         st = a.step(dt)
         b.clear_bodies(); b.apply_gravity(); b.apply_spring_forces()
         b.update_contacts_map(); b.detect(dt); b.warm_start(); b.wake()
-        b.update_in_collections(dt); b.accumulate_for_unmerging(); b.unmerge(dt)
+        b.update_in_collections(dt); b.accumulate_for_unmerging(dt); b.unmerge(dt)
         if b.merging_event:
             b.clear_top_level()
         b.redo_warm_start(); b.solve(dt); b.clear_body_pair_contacts(); b.advance_velocities(dt)
-        b.accumulate_for_merging(); b.advance_positions(dt)
+        b.accumulate_for_merging(dt); b.advance_positions(dt)
         if step % 10 == 0:
             b.merge()
         asleep = b.sleep()

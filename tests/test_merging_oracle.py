@@ -160,7 +160,7 @@ def test_scenario_settles_merges_and_unmerges():
         mo.o.warm_start()
         n_int = sum(len(q["contacts"]) for q in mo.ibpc)
         mo.update_in_collections(dt, lambda: np.arange(n + n_int, dtype=np.int32))
-        mo.accumulate_for_unmerging()
+        mo.accumulate_for_unmerging(dt)
         if mo.unmerge(dt):
             cut_at.append(step)
         if mo.merging_event:
@@ -172,8 +172,8 @@ def test_scenario_settles_merges_and_unmerges():
         # right after the push-down, members move rigidly with their collection
         for e, mem in mo.members.items():
             if e in mo.top and len(mem) >= 2 and not mo.pinned(e):
-                assert mo._metric(mem[0], mem[1]) < 1e-12
-        mo.accumulate_for_merging()
+                assert mo._metric(mem[0], mem[1], dt) < 1e-12
+        mo.accumulate_for_merging(dt)
         mo.advance_positions(dt)
         if step % 10 == 0 and mo.merge():
             merged_at.append(step)
@@ -214,3 +214,30 @@ def test_unmerge_all_returns_every_body():
         m, _, _ = merging_oracle_step(mo, 0.01, step)
         merged += m
     assert merged >= 1
+
+
+def test_position_level_metric_dispatch():
+    """Merging.metricPositionLevel (BodyPairContact.java:88-95, :130-137): the
+    relative metric of two unpinned entities is the position-level one, a
+    pair with a pinned body keeps the other body's own velocity metric; the
+    stack still settles and merges with it."""
+    from tests.helpers import merging_oracle_step
+
+    scene = workloads.merge_stack(seed=3)
+    p = params(gravity_amount=9.81, iterations=100, tolerance=1e-10, metric_position_level=1)
+    mo = M.MergingOracle(scene, p)
+    dt = 0.01
+    merged = free_pairs = 0
+    for step in range(1, 13):
+        m, _, _ = merging_oracle_step(mo, dt, step)
+        merged += m
+        if step == 3:
+            free_pairs = sum(1 for r in mo.bpc if not (mo.pinned(r["b1"]) or mo.pinned(r["b2"])))
+            for r in mo.bpc:
+                a, b = r["b1"], r["b2"]
+                if mo.pinned(a) or mo.pinned(b):
+                    assert mo._metric(a, b, dt) == mo.o.motion_metric(b if mo.pinned(a) else a)
+                else:
+                    assert mo._metric(a, b, dt) == mo.o.motion_metric_pos(a, b, dt)
+                    assert mo._metric(a, b, dt) != mo.o.motion_metric(a, b)
+    assert merged >= 1 and free_pairs >= 2

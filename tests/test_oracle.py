@@ -420,6 +420,63 @@ def test_motion_metric_closed_forms():
     np.testing.assert_allclose(o.motion_metric(0), np.linalg.norm(v1), rtol=1e-15)
 
 
+def _two_boxes(x1, x2):
+    s = Scene()
+    s.add_box([0.2, 0.3, 0.4], x1)
+    s.add_box([0.3, 0.2, 0.1], x2)
+    return s
+
+
+def test_position_level_metric_relative_translation():
+    """No spin: whatever the orientations, every corner's displacement over
+    dt is the relative velocity, so the metric is |v1 - v2|; equal
+    velocities give zero."""
+    rng = np.random.default_rng(11)
+    x = np.array([[0.1, 0.2, -0.3], [0.6, 0.1, 0.2]])
+    s = _two_boxes(x[0], x[1])
+    o = Oracle(s)
+    R = np.stack([random_rotation(rng).reshape(9) for _ in range(2)])
+    v = rng.normal(size=(2, 3))
+    o.set_state(x=x, R=R, v=v, w=np.zeros((2, 3)))
+    for dt in (0.01, 0.1, 1.0):
+        np.testing.assert_allclose(o.motion_metric_pos(0, 1, dt), np.linalg.norm(v[0] - v[1]), rtol=1e-12)
+    o.set_state(x=x, R=R, v=np.array([v[0], v[0]]), w=np.zeros((2, 3)))
+    assert o.motion_metric_pos(0, 1, 0.01) < 1e-12
+
+
+def test_position_level_metric_spin_closed_form():
+    """Body 1 at the origin spins at omega, body 2 is at rest: each corner c
+    (world position) turns through |omega| dt about the axis, so its
+    displacement over dt is 2 |c_perp| sin(|omega| dt / 2) / dt, c_perp the
+    part of c across the axis; the metric is the largest over both bodies'
+    corners."""
+    s = _two_boxes([0.0, 0.0, 0.0], [0.45, 0.2, -0.1])
+    o = Oracle(s)
+    om = np.array([0.4, -1.3, 2.1])
+    o.set_state(v=np.zeros((2, 3)), w=np.array([om, [0.0, 0.0, 0.0]]))
+    b = s.arrays()[0]
+    corners = np.concatenate([b["bbox"][0].reshape(8, 3), b["bbox"][1].reshape(8, 3) + [0.45, 0.2, -0.1]])
+    axis = om / np.linalg.norm(om)
+    perp = np.linalg.norm(corners - np.outer(corners @ axis, axis), axis=1)
+    for dt in (0.01, 0.3):
+        want = np.max(2 * perp * np.sin(np.linalg.norm(om) * dt / 2) / dt)
+        np.testing.assert_allclose(o.motion_metric_pos(0, 1, dt), want, rtol=1e-12)
+
+
+def test_position_level_metric_tends_to_velocity_level():
+    """As dt goes to zero the displacement over dt becomes the relative
+    velocity of the corner, so the position-level metric tends to the
+    velocity-level one."""
+    rng = np.random.default_rng(12)
+    x = np.array([[0.1, 0.2, -0.3], [0.6, 0.1, 0.2]])
+    o = Oracle(_two_boxes(x[0], x[1]))
+    R = np.stack([random_rotation(rng).reshape(9) for _ in range(2)])
+    o.set_state(x=x, R=R, v=rng.normal(size=(2, 3)), w=rng.normal(size=(2, 3)))
+    vel = o.motion_metric(0, 1)
+    np.testing.assert_allclose(o.motion_metric_pos(0, 1, 1e-7), vel, rtol=1e-6)
+    assert abs(o.motion_metric_pos(0, 1, 0.05) - vel) > 1e-4 * vel   # a finite dt differs
+
+
 def test_integration_free_flight_and_spin():
     s = Scene()
     s.add_box([0.2, 0.3, 0.4], [0.1, 0.2, 0.3])
