@@ -62,8 +62,8 @@ def measured_traffic(args) -> dict | None:
     rec = json.loads(line)
     want = dict(bodies=args.bodies, warmup=args.warmup, steps=args.steps, iterations=args.iterations,
                 tolerance=args.tolerance, dt=args.dt, seed=args.seed, schedule=args.pgs_schedule)
-    if rec["workload"] != want:
-        return None
+    if rec["workload"] != want or args.pgs_grid != 0 or args.pgs_block != 0:
+        return None  # profiled with the automatic launch shape
     return rec
 
 
@@ -183,16 +183,21 @@ def main() -> int:
     ap.add_argument("--pgs-launch", choices=["cooperative", "plain"], default="cooperative",
                     help="plain: ordinary launch of the persistent kernel (for profilers without cooperative-launch "
                          "support)")
-    ap.add_argument("--pgs-schedule", choices=["dataflow", "coloured"], default="dataflow",
+    ap.add_argument("--pgs-schedule", choices=["dataflow", "coloured"], default=None,
                     help="dataflow: per-body update counters; coloured: a grid barrier between colours "
                          "(bit-identical results)")
-    ap.add_argument("--autotune", choices=["on", "off"], default="on",
+    ap.add_argument("--autotune", choices=["auto", "on", "off"], default="auto",
                     help="after the warm-up, time the PGS kernel's launch shapes on the live contacts (both "
-                         "schedules; bit-identical results checked) and run the timed steps with the fastest; "
-                         "overrides --pgs-schedule/--pgs-grid/--pgs-block")
+                         "schedules; bit-identical results checked) and run the timed steps with the fastest. "
+                         "auto: on unless --pgs-schedule, --pgs-grid or --pgs-block is given")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (calibrates the HBM byte counters) and exit")
     args = ap.parse_args()
+    explicit_shape = args.pgs_schedule is not None or args.pgs_grid != 0 or args.pgs_block != 0
+    if args.pgs_schedule is None:
+        args.pgs_schedule = "dataflow"
+    if args.autotune == "auto":
+        args.autotune = "off" if explicit_shape else "on"
 
     if args.copy_probe > 0:
         import torch
@@ -260,6 +265,7 @@ def main() -> int:
         tuned = dict(chosen=chosen, trials=len(trials), default_ms_per_sweep=auto["ms_per_sweep"],
                      tune_s=time.perf_counter() - t1)
         args.pgs_schedule = "dataflow" if chosen["pgs_schedule"] == 0 else "coloured"
+        args.pgs_grid, args.pgs_block = chosen["pgs_grid"], chosen["pgs_block"]
         shape = {k: chosen[k] for k in ("pgs_schedule", "pgs_grid", "pgs_block")}
         params_kw.update(shape)
         if twin is not None:

@@ -457,9 +457,11 @@ int am_motion_metric(am_ctx* ctx, const int32_t* pairs, int32_t n_pairs, double*
  * no early exit from the same lambdas and deltaV; the lambdas and deltaV of
  * every trial must equal the first trial's bit for bit, else the call fails
  * with AM_ERR_MISMATCH.  The contacts, lambdas, deltaV and the
- * last-solve statistics are restored afterwards.  With apply != 0 the
- * fastest shape is written into the context's pgs_schedule / pgs_grid /
- * pgs_block; am_params_set overrides it later like any other parameter. */
+ * last-solve statistics are restored afterwards.  *best is the fastest
+ * trial, except that the first trial (the automatic dataflow shape) is kept
+ * unless another is at least 2 % faster.  With apply != 0 that shape is
+ * written into the context's pgs_schedule / pgs_grid / pgs_block;
+ * am_params_set overrides it later like any other parameter. */
 typedef struct am_tune_trial {
     int32_t pgs_schedule;    /* 0 dataflow, 1 coloured */
     int32_t pgs_grid;        /* as requested (0 = the automatic choice) */

@@ -35,7 +35,9 @@ def test_autotune_restores_and_solves_like_the_oracle(gpu_lib):
     assert len(trials) >= 4
     assert {t["pgs_schedule"] for t in trials} == {0, 1}
     assert all(t["ms_per_sweep"] > 0 for t in trials)
-    assert trials[best]["ms_per_sweep"] == min(t["ms_per_sweep"] for t in trials)
+    fastest = min(t["ms_per_sweep"] for t in trials)
+    assert trials[best]["ms_per_sweep"] == fastest or (best == 0 and trials[0]["ms_per_sweep"] <= fastest * 1.02)
+    assert (trials[0]["pgs_schedule"], trials[0]["pgs_grid"], trials[0]["pgs_block"]) == (0, 0, 0)
     for k in ("pgs_schedule", "pgs_grid", "pgs_block"):
         assert getattr(ctx.params, k) == trials[best][k]
     assert_contacts_equal(ctx.contacts(), before)
