@@ -1,0 +1,109 @@
+"""Parity at the full sizes of the measured configurations: the 20,000-box
+pile, the 33-layer jenga tower and 512 sphere-tree bodies of 512 leaves
+(about a million contacts).
+
+The GPU runs the scene until it is in contact; its state then goes to the
+oracle, both detect from it (the contacts must agree bit for bit), and both
+step on from there with the oracle sweeping in the GPU's order, comparing
+the contacts, lambdas and body states bit for bit.  Those steps use few
+sweeps so that the CPU side stays within seconds.  On the pile the launch-shape
+tuner then checks that every launch shape of the PGS kernel (both schedules)
+gives the same bits on the full contact set."""
+import numpy as np
+import pytest
+
+from adaptivemerging_amd import workloads
+from adaptivemerging_amd.context import Context
+from oracle.pyoracle import Oracle
+from tests.helpers import assert_bits_equal, assert_contacts_equal, oracle_step_like_gpu
+
+pytestmark = pytest.mark.gpu
+
+GEOM_KEYS = ["body1", "body2", "bv1", "bv2", "info", "point_b1", "normal_b1", "tangent1_b1", "tangent2_b1",
+             "violation", "jacobian"]
+
+
+def settle(scene, steps, dt, **params):
+    ctx = Context()
+    ctx.upload(scene)
+    ctx.set_params(**params)
+    st = None
+    for k in range(steps):
+        st = ctx.step(dt)
+        if k % 25 == 0 or k == steps - 1:
+            print(f"settle step {k}: contacts {st['n_contacts']} sweeps {st['iterations_used']}", flush=True)
+    return ctx, st
+
+
+def sync_and_step(ctx, scene, dt, steps, **params):
+    """Hands the GPU's state to a fresh oracle, detects on both from it, then
+    steps both; returns the contact count of the last step."""
+    ctx.set_params(**params)
+    orc = Oracle(scene)
+    for k, _ in type(ctx.params)._fields_:
+        setattr(orc.params, k, getattr(ctx.params, k))
+    orc.set_params()
+    orc.set_state(*ctx.pull_state())
+    # both maps start from the same contacts: detect once from the synced state
+    ctx.update_contacts_map()
+    n_g = ctx.detect(dt)
+    n_o = orc.detect()
+    print("synced detection:", n_g, "contacts on the GPU,", n_o, "in the oracle", flush=True)
+    assert n_g == n_o > 0
+    assert_contacts_equal(ctx.contacts(), orc.contacts(), keys=GEOM_KEYS)
+    n = 0
+    for step in range(steps):
+        it_g, it_o = oracle_step_like_gpu(ctx, orc, dt)
+        c = ctx.contacts()
+        n = len(c["body1"])
+        print(f"step {step}: {n} contacts, sweeps {it_g} / {it_o}", flush=True)
+        assert it_g == it_o
+        assert_contacts_equal(c, orc.contacts())
+        for name, a, b in zip("xRvw", ctx.pull_state(), orc.pull_state()):
+            assert_bits_equal(f"step {step} {name}", a, b)
+    return n
+
+
+def test_pile_20k(gpu_lib):
+    scene = workloads.box_pile(seed=0, n=20000)
+    dt = 0.01
+    ctx, st = settle(scene, 100, dt, iterations=100, tolerance=1e-6, gravity_amount=9.81, enable_merging=0,
+                     enable_unmerging=0, enable_sleeping=0)
+    assert st["n_contacts"] > 20000
+    n = sync_and_step(ctx, scene, dt, 2, iterations=5, tolerance=0.0)
+    assert n > 20000
+    trials, best = ctx.autotune(dt, sweeps=5, reps=1, apply=False)
+    print("launch shapes with identical bits:", len(trials), flush=True)
+    assert {t["pgs_schedule"] for t in trials} == {0, 1} and len(trials) >= 10
+
+
+def test_jenga_33_layers(gpu_lib):
+    """The small plain configuration at its size: 100 boxes, mu 0.6, 100
+    sweeps, every step bit for bit from the start."""
+    scene = workloads.jenga_tower(seed=0)
+    assert scene.n == 101
+    from tests.helpers import twin
+
+    ctx, orc = twin(scene, iterations=100, tolerance=1e-10, friction_override=1, friction=0.6, gravity_amount=9.81,
+                    enable_merging=0, enable_unmerging=0, enable_sleeping=0)
+    dt = 0.01
+    for step in range(100):
+        it_g, it_o = oracle_step_like_gpu(ctx, orc, dt)
+        assert it_g == it_o
+        if step % 20 == 19:
+            print(f"jenga step {step}: {len(ctx.contacts()['body1'])} contacts, sweeps {it_g}", flush=True)
+            assert_contacts_equal(ctx.contacts(), orc.contacts())
+        for name, a, b in zip("xRvw", ctx.pull_state(), orc.pull_state()):
+            assert_bits_equal(f"step {step} {name}", a, b)
+    assert len(ctx.contacts()["body1"]) > 300
+
+
+def test_sphere_trees_512(gpu_lib):
+    """512 bodies of 512 leaves placed at random in the open box: their trees
+    interpenetrate, about a million leaf contacts."""
+    scene = workloads.sphere_tree_drop(seed=0, n=512)
+    dt = 0.005
+    ctx, st = settle(scene, 4, dt, iterations=20, tolerance=1e-6, gravity_amount=9.81, enable_merging=0,
+                     enable_unmerging=0, enable_sleeping=0)
+    assert st["n_contacts"] > 100000
+    sync_and_step(ctx, scene, dt, 1, iterations=5, tolerance=0.0)
