@@ -192,25 +192,33 @@ def sphere_tree_drop(seed: int = 0, n: int = 512, box=(2.0, 1.0, 2.0), heights=(
     return s
 
 
-def batched_piles(seed: int = 0, n_scenes: int = 256, bodies=(50, 2000), box_frac: float = 0.7) -> Scene:
-    """n_scenes independent scenes (scene ids 0..n-1), body counts log-uniform
+def batched_pile_scenes(seed: int = 0, n_scenes: int = 256, bodies=(50, 2000), box_frac: float = 0.7) -> list:
+    """The scenes of batched_piles, each on its own: body counts log-uniform
     over `bodies`, each a random pile of boxes (edges U(0.1, 0.5)) and spheres
-    (r U(0.05, 0.3)) above its own ground plane."""
+    (r U(0.05, 0.25)) above its own ground plane."""
     rng = np.random.default_rng(seed)
-    s = Scene()
     counts = np.exp(rng.uniform(math.log(bodies[0]), math.log(bodies[1]), size=n_scenes)).astype(int)
+    out = []
     for sc in range(n_scenes):
-        s.add_plane([0, 0, 0], [0, 1, 0], friction=0.5, scene=sc)
+        s = Scene()
+        s.add_plane([0, 0, 0], [0, 1, 0], friction=0.5)
         nb = int(counts[sc])
         side = int(math.ceil(math.sqrt(nb / 4)))
         for k in range(nb):
             lx, lz, ly = k % side, (k // side) % side, k // (side * side)
             c = [lx * 0.55, 0.3 + ly * 0.55, lz * 0.55] + rng.uniform(-0.02, 0.02, 3)
             if rng.uniform() < box_frac:
-                s.add_box(rng.uniform(0.1, 0.5, 3), c, random_rotation(rng), density=1000.0, friction=0.5, scene=sc)
+                s.add_box(rng.uniform(0.1, 0.5, 3), c, random_rotation(rng), density=1000.0, friction=0.5)
             else:
-                s.add_sphere(rng.uniform(0.05, 0.25), c, density=1000.0, friction=0.5, scene=sc)
-    return s
+                s.add_sphere(rng.uniform(0.05, 0.25), c, density=1000.0, friction=0.5)
+        out.append(s)
+    return out
+
+
+def batched_piles(seed: int = 0, n_scenes: int = 256, bodies=(50, 2000), box_frac: float = 0.7) -> Scene:
+    """n_scenes independent scenes (scene ids 0..n-1) of batched_pile_scenes
+    in one context."""
+    return batch(batched_pile_scenes(seed, n_scenes, bodies, box_frac))
 
 
 def merge_stack(seed: int = 0, layers: int = 4, density: float = 500.0) -> Scene:

@@ -107,3 +107,72 @@ def test_sphere_trees_512(gpu_lib):
                      enable_unmerging=0, enable_sleeping=0)
     assert st["n_contacts"] > 100000
     sync_and_step(ctx, scene, dt, 1, iterations=5, tolerance=0.0)
+
+
+def test_batched_256_scenes(gpu_lib):
+    """256 scenes of 50 to 2,000 bodies in one context (156k bodies).  After
+    the GPU has piled them up, a sample of scenes (every 16th and the
+    largest) goes to one oracle each; both detect from the GPU's state and
+    take one step with the oracle sweeping in the GPU's order restricted to
+    its scene.  Merging is off here (the merging oracle is too slow at this
+    size); a fixed sweep count makes the batched solve comparable with one
+    scene solved alone."""
+    scenes = workloads.batched_pile_scenes(seed=0)
+    scene = workloads.batch(scenes)
+    offs = np.cumsum([0] + [s.n for s in scenes])
+    params = dict(iterations=50, tolerance=0.0, gravity_amount=9.81, enable_merging=0, enable_unmerging=0,
+                  enable_sleeping=0)
+    dt = 0.01
+    ctx, st = settle(scene, 60, dt, **params)
+    assert st["n_contacts"] > 100000
+    sample = sorted(set(range(0, len(scenes), 16)) | {int(np.argmax([s.n for s in scenes]))})
+    x, R, v, w = ctx.pull_state()
+    orcs = {}
+    for k in sample:
+        o = Oracle(scenes[k])
+        o.set_params(**params)
+        sl = slice(offs[k], offs[k + 1])
+        o.set_state(x[sl], R[sl], v[sl], w[sl])
+        orcs[k] = o
+
+    def split(g):
+        sc = np.searchsorted(offs, g["body1"], side="right") - 1
+        return sc
+
+    def scene_contacts(g, sc, k):
+        mine = np.nonzero(sc == k)[0]
+        gk = {key: val[mine] for key, val in g.items()}
+        for key in ("body1", "body2"):
+            gk[key] = gk[key] - offs[k]
+        return gk
+
+    ctx.update_contacts_map()
+    ctx.detect(dt)
+    g = ctx.contacts()
+    sc = split(g)
+    for k, o in orcs.items():
+        o.detect()
+        assert_contacts_equal(scene_contacts(g, sc, k), o.contacts(), keys=GEOM_KEYS)
+    print("synced detection:", len(g["body1"]), "contacts; sampled scenes agree", flush=True)
+    ctx.clear_bodies(); ctx.apply_gravity(); ctx.update_contacts_map(); ctx.detect(dt)
+    ctx.warm_start(); ctx.redo_warm_start()
+    it, _ = ctx.solve(dt)
+    assert it == 50
+    order = ctx.solve_order()
+    g = ctx.contacts()
+    sc = split(g)
+    ctx.advance_velocities(dt); ctx.advance_positions(dt)
+    x, R, v, w = ctx.pull_state()
+    for k, o in orcs.items():
+        o.clear_bodies(); o.apply_gravity(); o.update_contacts_map(); o.detect()
+        o.warm_start(); o.redo_warm_start()
+        mine = np.nonzero(sc == k)[0]
+        local = -np.ones(len(sc), dtype=np.int64)
+        local[mine] = np.arange(len(mine))
+        o.solve(local[order[sc[order] == k]].astype(np.int32), dt)
+        assert_contacts_equal(scene_contacts(g, sc, k), o.contacts())
+        o.advance_velocities(dt); o.advance_positions(dt)
+        sl = slice(offs[k], offs[k + 1])
+        for name, a, b in zip("xRvw", (x[sl], R[sl], v[sl], w[sl]), o.pull_state()):
+            assert_bits_equal(f"scene {k} {name}", a, b)
+        print(f"scene {k}: {scenes[k].n} bodies, {len(mine)} contacts bit-exact", flush=True)
