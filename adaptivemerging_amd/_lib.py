@@ -156,7 +156,7 @@ SYMBOLS = [
     "am_internal_contacts_pull",
     "am_step", "am_motion_metric", "am_mlp_init", "am_mlp_set_params", "am_mlp_get_params", "am_mlp_forward",
     "am_mlp_gradients", "am_features_pull", "am_train_size", "am_train_clear", "am_train_epochs",
-    "am_start_lambdas_pull", "am_autotune",
+    "am_start_lambdas_pull", "am_autotune", "am_advance_positions_post",
 ]
 MLP_PARAMS = 10115
 MLP_FEATURES = 24
@@ -230,6 +230,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "am_changes_pull": (C.c_int, [ctxp, vp, i32, C.POINTER(i32)]),
         "am_advance_velocities": (C.c_int, [ctxp, dbl]),
         "am_advance_positions": (C.c_int, [ctxp, dbl]),
+        "am_advance_positions_post": (C.c_int, [ctxp, dbl]),
         "am_step": (C.c_int, [ctxp, dbl, C.POINTER(StepStats)]),
         "am_motion_metric": (C.c_int, [ctxp, vp, i32, vp]),
         "am_autotune": (C.c_int, [ctxp, dbl, i32, i32, i32, C.POINTER(TuneTrial), i32, C.POINTER(i32),

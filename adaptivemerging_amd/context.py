@@ -302,6 +302,9 @@ class Context:
     def advance_positions(self, dt: float):
         self._check(self.lib.am_advance_positions(self.h, dt))
 
+    def advance_positions_post(self, dt: float):
+        self._check(self.lib.am_advance_positions_post(self.h, dt))
+
     def step(self, dt: float) -> dict:
         st = StepStats()
         self._check(self.lib.am_step(self.h, dt, C.byref(st)))

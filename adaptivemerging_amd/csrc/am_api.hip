@@ -1786,9 +1786,22 @@ int am_advance_positions(am_ctx* c, double dt) {
     return AM_OK;
 }
 
+int am_advance_positions_post(am_ctx* c, double dt) {
+    int r = check_ctx(c);
+    if (r) return r;
+    launch_advance_post(c->stream, bodies_view(c), c->d_integrate.p, (int)c->integrate_targets.size(), dt);
+    AM_HIP(c, hipGetLastError());
+    // RigidCollection.advancePositionsPostStabilization (:890-893)
+    if (has_collections(c)) return merge_member_transforms(c);
+    return AM_OK;
+}
+
 int am_step(am_ctx* c, double dt, am_step_stats* st) {
     int r = check_ctx(c);
     if (r) return r;
+    if (c->params.enable_post_stabilization &&
+        (c->params.enable_merging || c->params.enable_unmerging || has_collections(c)))
+        return fail(c, AM_ERR_INVALID, "am_step runs post-stabilization with merging and unmerging off only");
     for (auto& e : c->sev)
         if (!e) AM_HIP(c, hipEventCreate(&e));
     hipEvent_t* ev = c->sev;
@@ -1832,6 +1845,19 @@ int am_step(am_ctx* c, double dt, am_step_stats* st) {
     if ((r = am_advance_velocities(c, dt))) return r;
     if (P.enable_merging && (r = merge_accumulate_merging(c, dt))) return r;
     if ((r = am_advance_positions(c, dt))) return r;
+    // RigidBodySystem.postStabilization (:354-378): a second detection and a
+    // velocity-free solve whose deltaV moves the positions
+    if (P.enable_post_stabilization) {
+        if ((r = am_clear_bodies(c))) return r;
+        if ((r = am_update_contacts_map(c))) return r;
+        if ((r = do_detect(c))) return r;
+        int pb = 0, pr = 0;
+        if ((r = am_warm_start(c, 1, &pb, &pr))) return r;
+        int pit = 0;
+        double plast = 0;
+        if ((r = do_solve(c, dt, 1, true, &pit, &plast))) return r;
+        if ((r = am_advance_positions_post(c, dt))) return r;
+    }
     AM_HIP(c, hipEventRecord(ev[8], s));
     if (P.enable_merging && P.steps_between_merge_events > 0 && c->total_steps % P.steps_between_merge_events == 0)
         if ((r = merge_merge(c, &merged))) return r;

@@ -1640,14 +1640,18 @@ __device__ __forceinline__ bool advance_pose(d3& x, m33& R, d3 v, d3 om, double 
     return false;
 }
 
-// RigidBody.advancePositions (RigidBody.java:427-441) of the listed entities.
-__global__ void k_advance_positions(BodiesView B, const int* targets, int n, double dt) {
+// RigidBody.advancePositions (RigidBody.java:427-441) of the listed entities;
+// with from_dv the velocities are deltaV's parts
+// (advancePositionsPostStabilization, RigidBody.java:423-425).
+__global__ void k_advance_positions(BodiesView B, const int* targets, int n, double dt, int from_dv) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     int i = targets[k];
     d3 x = ld3(&B.x[i * 3]);
     m33 R = ldm(&B.R[i * 9]);
-    const bool turned = advance_pose(x, R, ld3(&B.v[i * 3]), ld3(&B.w[i * 3]), dt);
+    const d3 v = from_dv ? ld3(&B.dv[i * 6]) : ld3(&B.v[i * 3]);
+    const d3 om = from_dv ? ld3(&B.dv[i * 6 + 3]) : ld3(&B.w[i * 3]);
+    const bool turned = advance_pose(x, R, v, om, dt);
     st3(&B.x[i * 3], x);
     if (turned) stm(&B.R[i * 9], R);
     stm(&B.jinv[i * 9], rm0rt(R, ldm(B.desc[i].jinv0)));
@@ -1664,13 +1668,16 @@ __global__ void k_advanced_poses(BodiesView B, double dt, double* xa, double* Ra
     st3(&xa[i * 3], x);
     stm(&Ra[i * 9], R);
 }
+void launch_advance_post(hipStream_t s, BodiesView B, const int* targets, int n, double dt) {
+    if (n > 0) hipLaunchKernelGGL(k_advance_positions, dim3((n + 127) / 128), dim3(128), 0, s, B, targets, n, dt, 1);
+}
 void launch_advanced_poses(hipStream_t s, BodiesView B, double dt, double* xa, double* Ra) {
     if (B.n > 0) hipLaunchKernelGGL(k_advanced_poses, dim3((B.n + 127) / 128), dim3(128), 0, s, B, dt, xa, Ra);
 }
 
 void launch_advance(hipStream_t s, BodiesView B, const int* targets, int n, double dt, bool positions) {
     if (n <= 0) return;
-    if (positions) hipLaunchKernelGGL(k_advance_positions, dim3((n + 127) / 128), dim3(128), 0, s, B, targets, n, dt);
+    if (positions) hipLaunchKernelGGL(k_advance_positions, dim3((n + 127) / 128), dim3(128), 0, s, B, targets, n, dt, 0);
     else hipLaunchKernelGGL(k_advance_velocities, dim3((n + 255) / 256), dim3(256), 0, s, B, targets, n, dt);
 }
 

@@ -93,6 +93,8 @@ void launch_redo_warm(hipStream_t s, ContactsView C, const int* n, int cap, int 
 void launch_clear(hipStream_t s, BodiesView B);
 void launch_gravity(hipStream_t s, BodiesView B, const int* targets, int n, double gx, double gy, double gz);
 void launch_advance(hipStream_t s, BodiesView B, const int* targets, int n, double dt, bool positions);
+// the listed entities' poses advanced by dt with deltaV as the velocities
+void launch_advance_post(hipStream_t s, BodiesView B, const int* targets, int n, double dt);
 // every entity's pose advanced by dt into xa (3 per entity) and Ra (9 per entity)
 void launch_advanced_poses(hipStream_t s, BodiesView B, double dt, double* xa, double* Ra);
 void launch_springs(hipStream_t s, BodiesView B, const am_spring* springs, int n, double ks, double ds);

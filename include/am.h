@@ -358,6 +358,12 @@ int am_changes_pull(am_ctx* ctx, double* changes, int32_t capacity, int32_t* n);
 /* ---- integration (RigidBody.advanceVelocities / advancePositions) ----- */
 int am_advance_velocities(am_ctx* ctx, double dt);
 int am_advance_positions(am_ctx* ctx, double dt);
+/* RigidBody.advancePositionsPostStabilization (RigidBody.java:423-425) over
+ * the top-level entities that are neither pinned nor sleeping
+ * (RigidBodySystem.postStabilization, :374-377): deltaV's parts as the
+ * velocities.  am_step runs the whole post-stabilization pass when
+ * enable_post_stabilization is set, with merging and unmerging off. */
+int am_advance_positions_post(am_ctx* ctx, double dt);
 /* One RigidBodySystem.advanceTime (RigidBodySystem.java:102-185): clear,
  * gravity, contacts map, detection, warm start, the single sweep inside
  * collections, unmerging, the main solve, body-pair histories, velocity and

@@ -1697,6 +1697,17 @@ void orc_advance_positions(orc_world* w, double dt) {
         if (!pinned(w, i) && !sleeping(w, i)) advance_positions_one(w, i, dt);
     }
 }
+/* RigidBody.advancePositionsPostStabilization (RigidBody.java:423-425): the
+ * pose update with deltaV's linear and angular parts as the velocities, over
+ * the loop of RigidBodySystem.postStabilization (:374-377) */
+void orc_advance_positions_post(orc_world* w, double dt) {
+    for (int32_t k = 0; k < w->n_top; k++) {
+        int32_t i = w->top[k];
+        if (pinned(w, i) || sleeping(w, i)) continue;
+        advance_pose(&w->x[i * 3], &w->R[i * 9], &w->dv[i * 6], &w->dv[i * 6 + 3], dt);
+        update_jinv(w, i);
+    }
+}
 
 /* ------------------------------------------------------------------ */
 /* Relative motion (MotionMetricProcessor.java:39-73)                   */

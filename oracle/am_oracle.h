@@ -108,6 +108,7 @@ int32_t orc_pgs_solve(orc_world* w, const int32_t* order, int32_t n_order, doubl
 
 void orc_advance_velocities(orc_world* w, double dt);
 void orc_advance_positions(orc_world* w, double dt);
+void orc_advance_positions_post(orc_world* w, double dt);
 
 /* Math.sin / Math.cos as expRodrigues uses them, by fdlibm 5.3's algorithm
  * (java.lang.StrictMath's definition). */

@@ -72,6 +72,7 @@ def load() -> C.CDLL:
         "orc_pgs_solve": (i32, [vp, vp, i32, dbl, i32, i32, vp, C.POINTER(dbl)]),
         "orc_advance_velocities": (None, [vp, dbl]),
         "orc_advance_positions": (None, [vp, dbl]),
+        "orc_advance_positions_post": (None, [vp, dbl]),
         "orc_motion_metric": (dbl, [vp, i32, i32]),
         "orc_motion_metric_pos": (dbl, [vp, i32, i32, dbl]),
         "orc_step": (i32, [vp, dbl, C.POINTER(i32), C.POINTER(dbl)]),
@@ -277,6 +278,9 @@ class Oracle:
 
     def advance_positions(self, dt):
         self.lib.orc_advance_positions(self.h, dt)
+
+    def advance_positions_post(self, dt):
+        self.lib.orc_advance_positions_post(self.h, dt)
 
     def motion_metric(self, a, b=-1) -> float:
         return self.lib.orc_motion_metric(self.h, a, b)

@@ -557,3 +557,27 @@ def test_warm_start_box_box_repairs_info_swap():
         j = int(np.argmin([np.linalg.norm(p - q) for q in old_pts]))
         if np.linalg.norm(p - old_pts[j]) < 0.05 and c["new_this_step"][k] == 0:
             np.testing.assert_array_equal(c["lambda"][k], lam[j])
+
+
+def test_post_stabilization_advance_uses_deltav():
+    """advancePositionsPostStabilization moves each body as advancePositions
+    would with deltaV's parts in place of v and omega, and leaves v, omega
+    untouched; pinned bodies stay."""
+    rng = np.random.default_rng(21)
+    s = Scene()
+    s.add_plane([0, 0, 0], [0, 1, 0])
+    s.add_box([0.2, 0.3, 0.4], [0.1, 0.5, 0.3])
+    s.add_box([0.3, 0.2, 0.1], [0.6, 0.9, -0.2])
+    x0 = np.array(s.x)
+    R0 = np.stack([np.eye(3).reshape(9)] + [random_rotation(rng).reshape(9) for _ in range(2)])
+    v = rng.normal(size=(3, 3)); w = rng.normal(size=(3, 3))
+    dv = rng.normal(size=(3, 6))
+    a = Oracle(s); a.set_state(x0, R0, v, w); a.push_deltav(dv)
+    a.advance_positions_post(0.01)
+    b = Oracle(s); b.set_state(x0, R0, dv[:, :3], dv[:, 3:])
+    b.advance_positions(0.01)
+    xa, Ra, va, wa = a.pull_state()
+    xb, Rb, _, _ = b.pull_state()
+    assert np.array_equal(xa[1:], xb[1:]) and np.array_equal(Ra[1:], Rb[1:])
+    assert np.array_equal(xa[0], x0[0]) and np.array_equal(va, v) and np.array_equal(wa, w)
+    assert not np.array_equal(xa[1:], x0[1:])
