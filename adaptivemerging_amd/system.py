@@ -348,6 +348,8 @@ class RigidBodySystem:
         self.ctx.accumulate_for_merging(dt)
         self.mergingTime = time.perf_counter() - t1
         self.ctx.advance_positions(dt)
+        if self.collision.enablePostStabilization:
+            self.postStabilization(dt)
         self.merging.mergingBuildTime = 0.0
         if self.totalSteps % self.merging.stepsBetweenMergeEvents == 0:
             t1 = time.perf_counter()
@@ -358,6 +360,19 @@ class RigidBodySystem:
         self.computeTime = time.perf_counter() - t0
         self.simulationTime += dt
         self.exportDataToFile()
+
+    def postStabilization(self, dt: float):
+        """postStabilization (:354-378): clear, detect again, warm start and
+        solve with the post flag, move the positions by deltaV.  Runs with
+        merging and unmerging off, as am_step does."""
+        if self.merging.enableMerging or self.merging.enableUnmerging:
+            raise ValueError("post-stabilization runs with merging and unmerging off")
+        self.ctx.clear_bodies()
+        self.collision.updateContactsMap()
+        self.collision.collisionDetection(dt)
+        self.collision.warmStart(True)
+        self.collision.solveLCP(dt, True)
+        self.ctx.advance_positions_post(dt)
 
     def exportDataToFile(self):
         """exportDataToFile (:495-542): while saveCSV is on, the first call
