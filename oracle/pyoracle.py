@@ -23,16 +23,18 @@ def build() -> str:
     return LIB
 
 
-_lib = None
+_libs: dict = {}
 
 
-def load() -> C.CDLL:
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB):
+def load(path: str | None = None) -> C.CDLL:
+    """The oracle library at `path` (default: the in-tree build).  Another
+    path lets a test load a copy of the oracle built with a planted mistake."""
+    path = path or LIB
+    if path in _libs:
+        return _libs[path]
+    if path == LIB and not os.path.exists(LIB):
         build()
-    lib = C.CDLL(LIB)
+    lib = C.CDLL(path)
     vp, i32, dbl = C.c_void_p, C.c_int32, C.c_double
     sig = {
         "orc_create": (vp, [vp, i32, vp, i32]),
@@ -83,15 +85,15 @@ def load() -> C.CDLL:
         f = getattr(lib, name)
         f.restype = res
         f.argtypes = args
-    _lib = lib
+    _libs[path] = lib
     return lib
 
 
 class Oracle:
     """CPU restatement of the reference path over the same Scene arrays."""
 
-    def __init__(self, scene, params: Params | None = None):
-        self.lib = load()
+    def __init__(self, scene, params: Params | None = None, lib_path: str | None = None):
+        self.lib = load(lib_path)
         b, nd, x, R, v, w = scene.arrays()
         self._b, self._nd = b, nd
         self.n = len(b)
