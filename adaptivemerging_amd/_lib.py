@@ -29,6 +29,7 @@ AM_GEOM_COLLECTION = 3
 AM_BODY_PINNED = 1
 AM_BODY_MAGNET_ACTIVE = 2
 AM_BODY_SLEEPING = 4
+AM_BODY_KINEMATIC = 8
 AM_GEOM_COMPOSITE = 4
 
 
@@ -60,7 +61,9 @@ AM_SPRING_ZERO, AM_SPRING_WORLD, AM_SPRING_BODYBODY = 0, 1, 2
 SPRING_DTYPE = np.dtype([("type", "<i4"), ("body1", "<i4"), ("body2", "<i4"), ("pad", "<i4"), ("pb1", "<f8", 3),
                          ("pb2", "<f8", 3), ("pw", "<f8", 3), ("l0", "<f8"), ("k", "<f8"), ("d", "<f8"),
                          ("ls", "<f8")])
-assert BODY_DTYPE.itemsize == 480 and NODE_DTYPE.itemsize == 40
+KINEMATIC_DTYPE = np.dtype([("body", "<i4"), ("pad", "<i4"), ("x0", "<f8", 3), ("dir", "<f8", 3),
+                            ("amplitude", "<f8"), ("omega", "<f8"), ("phase", "<f8")])
+assert BODY_DTYPE.itemsize == 480 and NODE_DTYPE.itemsize == 40 and KINEMATIC_DTYPE.itemsize == 80
 
 
 class Params(C.Structure):
@@ -156,7 +159,8 @@ SYMBOLS = [
     "am_internal_contacts_pull",
     "am_step", "am_motion_metric", "am_mlp_init", "am_mlp_set_params", "am_mlp_get_params", "am_mlp_forward",
     "am_mlp_gradients", "am_features_pull", "am_train_size", "am_train_clear", "am_train_epochs",
-    "am_start_lambdas_pull", "am_autotune", "am_advance_positions_post",
+    "am_start_lambdas_pull", "am_autotune", "am_advance_positions_post", "am_impulse_add", "am_kinematic_set",
+    "am_kinematic_update", "am_sim_time_get", "am_sim_time_set", "am_apply_impulses", "am_apply_viscous_decay",
 ]
 MLP_PARAMS = 10115
 MLP_FEATURES = 24
@@ -245,6 +249,13 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "am_train_clear": (C.c_int, [ctxp]),
         "am_train_epochs": (C.c_int, [ctxp, i32, i32, C.c_float, C.c_uint64, C.POINTER(C.c_float)]),
         "am_start_lambdas_pull": (C.c_int, [ctxp, vp, i32, C.POINTER(i32)]),
+        "am_impulse_add": (C.c_int, [ctxp, i32, vp, vp]),
+        "am_kinematic_set": (C.c_int, [ctxp, vp, i32]),
+        "am_kinematic_update": (C.c_int, [ctxp, dbl]),
+        "am_sim_time_get": (C.c_int, [ctxp, C.POINTER(dbl)]),
+        "am_sim_time_set": (C.c_int, [ctxp, dbl]),
+        "am_apply_impulses": (C.c_int, [ctxp]),
+        "am_apply_viscous_decay": (C.c_int, [ctxp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -62,6 +62,40 @@ class Context:
         springs = scene.spring_array() if hasattr(scene, "spring_array") else None
         if springs is not None and len(springs):
             self.set_springs(springs)
+        kin = scene.kinematic_array() if hasattr(scene, "kinematic_array") else None
+        if kin is not None and len(kin):
+            self.set_kinematic(kin)
+
+    def set_kinematic(self, table):
+        """Prescribed motions (a KINEMATIC_DTYPE array, physical body indices)."""
+        self._kin = np.ascontiguousarray(table, dtype=_lib.KINEMATIC_DTYPE)
+        self._check(self.lib.am_kinematic_set(self.h, ptr(self._kin), len(self._kin)))
+
+    def kinematic_update(self, t: float):
+        self._check(self.lib.am_kinematic_update(self.h, t))
+
+    @property
+    def sim_time(self) -> float:
+        t = C.c_double()
+        self._check(self.lib.am_sim_time_get(self.h, C.byref(t)))
+        return t.value
+
+    @sim_time.setter
+    def sim_time(self, t: float):
+        self._check(self.lib.am_sim_time_set(self.h, t))
+
+    def impulse_add(self, body: int, point_w, force):
+        """A force at a world point, applied (to the body's top-level entity)
+        by the next external-force pass only (RigidBodySystem.applyImpulse)."""
+        p = np.ascontiguousarray(point_w, dtype=np.float64)
+        f = np.ascontiguousarray(force, dtype=np.float64)
+        self._check(self.lib.am_impulse_add(self.h, body, ptr(p), ptr(f)))
+
+    def apply_impulses(self):
+        self._check(self.lib.am_apply_impulses(self.h))
+
+    def apply_viscous_decay(self):
+        self._check(self.lib.am_apply_viscous_decay(self.h))
 
     def set_springs(self, springs):
         """RigidBodySystem.springs: a SPRING_DTYPE array (physical body indices)."""

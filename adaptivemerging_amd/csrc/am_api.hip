@@ -258,16 +258,34 @@ int rebuild_structure(am_ctx* c) {
     AM_HIP(c, upload(c, c->d_member_adv, c->member_adv));
     if (c->b2c.size() < (size_t)n * 12) c->b2c.resize((size_t)n * 12, 0.0);
     AM_HIP(c, upload(c, c->d_b2c, c->b2c));
-    // grid
-    double rmax = 0.0;
-    for (int b : c->grid_bodies) {
-        const am_body_desc& d = c->desc[b];
-        double r = (d.geom == AM_GEOM_BOX || d.geom == AM_GEOM_COMPOSITE)
+    // grid: cells sized for the bodies up to kLargeRatio times the median
+    // bounding radius; the few larger ones (platforms, walls made of boxes)
+    // stay out of the grid and are paired with every body by the pair kernel
+    const int n_grid = (int)c->grid_bodies.size();
+    std::vector<double> radii(n_grid);
+    for (int k = 0; k < n_grid; k++) {
+        const am_body_desc& d = c->desc[c->grid_bodies[k]];
+        radii[k] = (d.geom == AM_GEOM_BOX || d.geom == AM_GEOM_COMPOSITE)
                        ? d.radius
                        : (d.tree_count > 0 ? c->nodes[d.tree_first].r : 0.0);
-        rmax = std::max(rmax, r);
     }
+    c->grid_is_large.assign(std::max(n_grid, 1), 0);
+    c->grid_large.clear();
+    if (n_grid > 0) {
+        std::vector<double> sorted = radii;
+        std::nth_element(sorted.begin(), sorted.begin() + n_grid / 2, sorted.end());
+        const double limit = kLargeRatio * sorted[n_grid / 2];
+        for (int k = 0; k < n_grid; k++)
+            if (radii[k] > limit) c->grid_large.push_back(k);
+        if ((int)c->grid_large.size() > kMaxLarge) c->grid_large.clear();  // not a few outliers: one grid for all
+        for (int k : c->grid_large) c->grid_is_large[k] = 1;
+    }
+    double rmax = 0.0;
+    for (int k = 0; k < n_grid; k++)
+        if (!c->grid_is_large[k]) rmax = std::max(rmax, radii[k]);
     c->cell_size = std::max(2.0 * rmax, 1e-6);
+    AM_HIP(c, upload(c, c->d_grid_is_large, c->grid_is_large));
+    AM_HIP(c, upload(c, c->d_grid_large, c->grid_large));
     int tsize = 16;
     while (tsize < 2 * (int)c->grid_bodies.size()) tsize *= 2;
     c->hash_mask = tsize - 1;
@@ -477,7 +495,8 @@ int do_detect(am_ctx* c) {
     launch_tree_world_centres(s, B, T, c->d_node_body.p, c->d_node_sub.p, c->n_nodes);
     const int ng = (int)c->grid_bodies.size();
     const int tsize = c->hash_mask + 1;
-    launch_bounds(s, B, T, c->d_grid.p, ng, c->d_sphere.p, c->cell_size, c->d_cell.p, c->hash_mask);
+    launch_bounds(s, B, T, c->d_grid.p, ng, c->d_sphere.p, c->cell_size, c->d_cell.p, c->hash_mask,
+                  c->d_grid_is_large.p);
     fill_int(s, c->d_bucket_count.p, tsize, 0);
     fill_int(s, c->d_bucket_cursor.p, tsize, 0);
     launch_buckets(s, c->d_cell.p, ng, c->d_bucket_count.p, nullptr, nullptr, nullptr, false);
@@ -499,6 +518,8 @@ int do_detect(am_ctx* c) {
     A.n_planes = (int)c->planes.size();
     A.top_pinned = c->d_top_pinned.p;
     A.top_sleeping = c->d_top_sleeping.p;
+    A.large = c->d_grid_large.p;
+    A.n_large = (int)c->grid_large.size();
     fill_int(s, sc + 4, 1, 0);
     launch_pairs_impl(s, false, A, c->d_pair_counts.p, nullptr, nullptr, 0, sc + 4);
     exclusive_scan(s, c->d_pair_counts.p, c->d_pair_offsets.p, ng, sc + 0, c->d_scan_temp.p);
@@ -934,6 +955,9 @@ int am_scene_upload_ex(am_ctx* c, const am_body_desc* bodies, int32_t n_bodies, 
         c->total_steps = 0;
         c->n_springs = 0;
         c->springs_host.clear();
+        c->kin_host.clear();
+        c->impulses.clear();
+        c->sim_time = 0.0;
         c->merging_event = false;
         // every body starts with an empty metric history
         c->sleep_cap = 0;
@@ -960,6 +984,8 @@ int am_scene_upload_ex(am_ctx* c, const am_body_desc* bodies, int32_t n_bodies, 
             }
             if ((d.geom == AM_GEOM_PLANE) && !(d.flags & AM_BODY_PINNED))
                 return fail(c, AM_ERR_INVALID, "planes must be pinned");
+            if ((d.flags & AM_BODY_KINEMATIC) && !(d.flags & AM_BODY_PINNED))
+                return fail(c, AM_ERR_INVALID, "kinematic bodies must be pinned");
             if (d.geom == AM_GEOM_COMPOSITE) {
                 if (d.tree_first < 0 || d.tree_count <= 0 || d.tree_first + d.tree_count > n_subs)
                     return fail(c, AM_ERR_INVALID, "composite sub-body range out of bounds");
@@ -1231,6 +1257,86 @@ int am_apply_gravity(am_ctx* c) {
     return apply_gravity(c);
 }
 
+int am_impulse_add(am_ctx* c, int32_t body, const double* point_w, const double* force) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (body < 0 || body >= c->n_phys || !point_w || !force) return fail(c, AM_ERR_INVALID, "bad impulse");
+    am::ImpulseRec q{};
+    q.body = body;
+    for (int a = 0; a < 3; a++) {
+        q.p[a] = point_w[a];
+        q.f[a] = force[a];
+        if (!std::isfinite(q.p[a]) || !std::isfinite(q.f[a])) return fail(c, AM_ERR_NUMERIC, "non-finite impulse");
+    }
+    c->impulses.push_back(q);
+    return AM_OK;
+}
+
+int am_apply_impulses(am_ctx* c) {
+    int r = check_ctx(c);
+    if (r) return r;
+    return apply_impulses(c);
+}
+
+int am_apply_viscous_decay(am_ctx* c) {
+    int r = check_ctx(c);
+    if (r) return r;
+    return apply_viscous_decay(c);
+}
+
+int am_kinematic_set(am_ctx* c, const am_kinematic* k, int32_t n) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (n < 0 || (n > 0 && !k)) return fail(c, AM_ERR_INVALID, "bad kinematic table");
+    std::vector<char> seen(c->n_phys, 0);
+    for (int i = 0; i < n; i++) {
+        const int b = k[i].body;
+        if (b < 0 || b >= c->n_phys) return fail(c, AM_ERR_INVALID, "kinematic body out of range");
+        const int want = AM_BODY_PINNED | AM_BODY_KINEMATIC;
+        if ((c->desc[b].flags & want) != want)
+            return fail(c, AM_ERR_INVALID, "a kinematic body must be flagged AM_BODY_PINNED | AM_BODY_KINEMATIC");
+        if (seen[b]) return fail(c, AM_ERR_INVALID, "a body has two prescribed motions");
+        seen[b] = 1;
+        const double* vals[] = {k[i].x0, k[i].dir};
+        for (const double* v : vals)
+            for (int a = 0; a < 3; a++)
+                if (!std::isfinite(v[a])) return fail(c, AM_ERR_NUMERIC, "non-finite prescribed motion");
+        if (!std::isfinite(k[i].amplitude) || !std::isfinite(k[i].omega) || !std::isfinite(k[i].phase))
+            return fail(c, AM_ERR_NUMERIC, "non-finite prescribed motion");
+    }
+    try {
+        c->kin_host.assign(k, k + n);
+    } catch (const std::bad_alloc&) {
+        return fail(c, AM_ERR_INVALID, "host allocation failed");
+    }
+    AM_HIP(c, upload(c, c->d_kin, c->kin_host));
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    return AM_OK;
+}
+
+int am_kinematic_update(am_ctx* c, double t) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!std::isfinite(t)) return fail(c, AM_ERR_NUMERIC, "non-finite time");
+    return kinematic_update(c, t);
+}
+
+int am_sim_time_get(am_ctx* c, double* t) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!t) return fail(c, AM_ERR_INVALID, "null time");
+    *t = c->sim_time;
+    return AM_OK;
+}
+
+int am_sim_time_set(am_ctx* c, double t) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!std::isfinite(t)) return fail(c, AM_ERR_NUMERIC, "non-finite time");
+    c->sim_time = t;
+    return AM_OK;
+}
+
 }  // extern "C"
 
 namespace am_rt {
@@ -1246,7 +1352,39 @@ int apply_springs(am_ctx* c) {
 int apply_external_forces(am_ctx* c) {
     int r = apply_gravity(c);
     if (r) return r;
-    return apply_springs(c);
+    r = apply_springs(c);
+    if (r) return r;
+    return apply_impulses(c);
+}
+
+// RigidBodySystem.applyImpulse (:262-267): the held forces, once
+int apply_impulses(am_ctx* c) {
+    if (!c->impulses.empty()) {
+        AM_HIP(c, upload(c, c->d_imp, c->impulses));
+        launch_impulses(c->stream, bodies_view(c), c->d_imp.p, (int)c->impulses.size());
+        AM_HIP(c, hipGetLastError());
+        // the upload reads the host vector asynchronously: wait before it is cleared
+        AM_HIP(c, hipStreamSynchronize(c->stream));
+        c->impulses.clear();
+    }
+    return AM_OK;
+}
+
+// RigidBodySystem.applyViscousDecay (:173, :428-435); scaling by 1 changes no bit
+int apply_viscous_decay(am_ctx* c) {
+    const am_params& P = c->params;
+    if (P.viscous_linear_decay == 1.0 && P.viscous_angular_decay == 1.0) return AM_OK;
+    launch_viscous(c->stream, bodies_view(c), c->d_slot_entity.p, (int)c->slot_entity.size(),
+                   P.viscous_linear_decay, P.viscous_angular_decay);
+    AM_HIP(c, hipGetLastError());
+    return AM_OK;
+}
+
+int kinematic_update(am_ctx* c, double t) {
+    if (c->kin_host.empty()) return AM_OK;
+    launch_kinematic(c->stream, bodies_view(c), c->d_kin.p, (int)c->kin_host.size(), t);
+    AM_HIP(c, hipGetLastError());
+    return AM_OK;
 }
 
 int apply_gravity(am_ctx* c) {
@@ -1810,8 +1948,10 @@ int am_step(am_ctx* c, double dt, am_step_stats* st) {
     const bool merging = P.enable_merging || P.enable_unmerging || has_collections(c);
     c->total_steps++;
     AM_HIP(c, hipEventRecord(ev[0], s));
-    // RigidBodySystem.clearBodies (:190-202)
+    // RigidBodySystem.clearBodies (:190-202); prescribed motions where the
+    // animate tool acts (:109)
     if ((r = am_clear_bodies(c))) return r;
+    if ((r = kinematic_update(c, c->sim_time))) return r;
     if ((r = apply_external_forces(c))) return r;
     if ((r = am_update_contacts_map(c))) return r;
     if ((r = do_detect(c))) return r;
@@ -1864,6 +2004,8 @@ int am_step(am_ctx* c, double dt, am_step_stats* st) {
     // Sleeping.sleep (:170)
     int asleep = 0;
     if ((r = sleep_sleep(c, &asleep))) return r;
+    if ((r = apply_viscous_decay(c))) return r;
+    c->sim_time += dt;
     AM_HIP(c, hipEventRecord(ev[4], s));
     AM_HIP(c, hipEventSynchronize(ev[4]));
     c->last_merged = merged;

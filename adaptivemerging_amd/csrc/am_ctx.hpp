@@ -107,6 +107,7 @@ struct am_ctx {
     std::vector<int> node_sub;          // per node: sub-body (inside its composite) owning it, -1 otherwise
     std::vector<int> parent, slot, sub, slot_entity, member_first, member_count, members, flags;
     std::vector<int> planes, grid_bodies, node_body, top_pinned, top_sleeping, gravity_targets, integrate_targets;
+    std::vector<int> grid_is_large, grid_large;  // per gridded body: too large for the grid; their gridded indices
     double cell_size = 1.0;
     int hash_mask = 0;
 
@@ -117,7 +118,7 @@ struct am_ctx {
     DBuf<am_sphere_node> d_nodes;
     DBuf<double> d_cw, d_x, d_R, d_v, d_w, d_f, d_t, d_jinv, d_dv, d_minv;
     DBuf<int> d_flags, d_parent, d_slot, d_sub, d_slot_entity, d_member_first, d_members, d_planes, d_grid,
-        d_node_body, d_top_pinned, d_top_sleeping, d_gravity, d_integrate;
+        d_node_body, d_top_pinned, d_top_sleeping, d_gravity, d_integrate, d_grid_is_large, d_grid_large;
 
     // broad phase
     DBuf<double> d_sphere;
@@ -155,6 +156,12 @@ struct am_ctx {
     DBuf<am_spring> d_springs;
     std::vector<am_spring> springs_host;
     int n_springs = 0;
+    // prescribed motions, forces held for the next external-force pass, simulation time
+    std::vector<am_kinematic> kin_host;
+    DBuf<am_kinematic> d_kin;
+    std::vector<am::ImpulseRec> impulses;
+    DBuf<am::ImpulseRec> d_imp;
+    double sim_time = 0.0;
     // sleeping (Sleeping.java): per-entity metric histories
     DBuf<double> d_sleep_hist;          // kSleepHistStride per entity, oldest first
     DBuf<int> d_sleep_n, d_sleep_ext;   // history length; active external contact this step
@@ -220,6 +227,9 @@ namespace am_rt {
 int fail(am_ctx* c, int code, const std::string& m);
 int apply_springs(am_ctx* c);
 int apply_external_forces(am_ctx* c);
+int kinematic_update(am_ctx* c, double t);
+int apply_impulses(am_ctx* c);
+int apply_viscous_decay(am_ctx* c);
 int check_ctx(am_ctx* c);
 am::BodiesView bodies_view(am_ctx* c);
 am::TreeView tree_view(am_ctx* c);

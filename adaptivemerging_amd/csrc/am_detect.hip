@@ -48,9 +48,10 @@ AMD_INLINE int cell_hash(int cx, int cy, int cz, int scene, int mask) {
     return (int)(h & (uint32_t)mask);
 }
 
-// sphere: 4 doubles per gridded body (centre, radius); cell_key: 4 ints (cx, cy, cz, bucket)
+// sphere: 4 doubles per gridded body (centre, radius); cell_key: 4 ints (cx, cy, cz, bucket);
+// a body too large for the grid gets bucket -1 and stays out of the buckets.
 __global__ void k_bounds(BodiesView B, TreeView T, const int* order, int n, double* sphere, double inv_h,
-                         int* cell, int mask) {
+                         int* cell, int mask, const int* is_large) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     int b = order[k];
@@ -73,25 +74,27 @@ __global__ void k_bounds(BodiesView B, TreeView T, const int* order, int n, doub
     cell[k * 4 + 0] = cx;
     cell[k * 4 + 1] = cy;
     cell[k * 4 + 2] = cz;
-    cell[k * 4 + 3] = cell_hash(cx, cy, cz, d.scene, mask);
+    cell[k * 4 + 3] = is_large[k] ? -1 : cell_hash(cx, cy, cz, d.scene, mask);
 }
 
 void launch_bounds(hipStream_t s, BodiesView B, TreeView T, const int* order, int n, double* sphere, double cell,
-                   int* cell_key, int hash_mask) {
+                   int* cell_key, int hash_mask, const int* is_large) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_bounds, dim3((n + 255) / 256), dim3(256), 0, s, B, T, order, n, sphere, 1.0 / cell, cell_key,
-                       hash_mask);
+                       hash_mask, is_large);
 }
 
 __global__ void k_bucket_count(const int* cell, int n, int* count) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    atomicAdd(&count[cell[k * 4 + 3]], 1);
+    const int h = cell[k * 4 + 3];
+    if (h >= 0) atomicAdd(&count[h], 1);
 }
 __global__ void k_bucket_scatter(const int* cell, int n, const int* start, int* cursor, int* bucket_items) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     int h = cell[k * 4 + 3];
+    if (h < 0) return;
     int pos = start[h] + atomicAdd(&cursor[h], 1);
     bucket_items[pos] = k;
 }
@@ -120,7 +123,11 @@ AMD_INLINE bool skip_pair(const PairArgs& A, int slot_a, int slot_b) {
 }
 
 // One thread per gridded body: count (write=false) or write its pairs.  A body
-// emits the pairs with bodies of a higher slot, and all its plane pairs.
+// in the grid emits the pairs with grid bodies of a higher slot found in the
+// 27 cells around its own, the pairs with every large body (those stay out of
+// the grid, whose cells are sized for the others), and all its plane pairs; a
+// large body emits its pairs with large bodies of a higher slot and with the
+// planes.  The keys are sorted afterwards, so the emission order is free.
 template <bool kWrite>
 __global__ void k_pairs(PairArgs A, int* counts, const int* offsets, uint64_t* keys, int cap, int* overflow) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -131,9 +138,32 @@ __global__ void k_pairs(PairArgs A, int* counts, const int* offsets, uint64_t* k
     const am_body_desc& di = B.desc[bi];
     double ci[4] = {A.sphere[k * 4 + 0], A.sphere[k * 4 + 1], A.sphere[k * 4 + 2], A.sphere[k * 4 + 3]};
     int cx = A.cell[k * 4 + 0], cy = A.cell[k * 4 + 1], cz = A.cell[k * 4 + 2];
+    const bool large_i = A.cell[k * 4 + 3] < 0;
     int n = 0;
     int base = kWrite ? offsets[k] : 0;
-    for (int dz = -1; dz <= 1; dz++)
+    for (int q = 0; q < A.n_large; q++) {
+        int kj = A.large[q];
+        if (kj == k) continue;
+        int bj = A.grid_order[kj];
+        if (B.desc[bj].scene != di.scene) continue;
+        int slot_j = B.slot[bj];
+        if (slot_j == slot_i || (large_i && slot_j < slot_i)) continue;
+        if (skip_pair(A, slot_i, slot_j)) continue;
+        double ddx = A.sphere[kj * 4 + 0] - ci[0];
+        double ddy = A.sphere[kj * 4 + 1] - ci[1];
+        double ddz = A.sphere[kj * 4 + 2] - ci[2];
+        double rr = A.sphere[kj * 4 + 3] + ci[3];
+        double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        if (d2 > rr * rr * (1.0 + 1e-9) + 1e-300) continue;
+        if (kWrite) {
+            int pos = base + n;
+            if (pos < cap) keys[pos] = pair_key(slot_i, sub_i, slot_j, B.sub[bj]);
+            else atomicOr(overflow, 1);
+        }
+        n++;
+    }
+    // a large body scans no cells: every grid body pairs with it above
+    for (int dz = -1; dz <= (large_i ? -2 : 1); dz++)
         for (int dy = -1; dy <= 1; dy++)
             for (int dx = -1; dx <= 1; dx++) {
                 int nx = cx + dx, ny = cy + dy, nz = cz + dz;

@@ -139,6 +139,11 @@ struct OldContactsView {
 constexpr int kSlotBits = 20;
 constexpr int kSubBits = 12;
 constexpr int kMaxSlots = 1 << kSlotBits;
+// broad phase: bodies whose bounding radius exceeds kLargeRatio times the
+// median stay out of the grid (at most kMaxLarge of them; beyond that the grid
+// takes every body and its cells fit the largest)
+constexpr double kLargeRatio = 4.0;
+constexpr int kMaxLarge = 256;
 constexpr int kMaxSub = 1 << kSubBits;
 constexpr int kTreeStack = 128;
 constexpr int kMaxColours = 64;     // colours 0..63 run in parallel; 64 is the sequential tail
@@ -178,11 +183,13 @@ struct PairArgs {
     int n_planes;
     const int* top_pinned;   // per slot
     const int* top_sleeping; // per slot
+    const int* large;        // gridded indices of the bodies too large for the grid (not in its buckets)
+    int n_large;
 };
 void launch_tree_world_centres(hipStream_t s, BodiesView B, TreeView T, const int* node_body, const int* node_sub,
                                int n_nodes);
 void launch_bounds(hipStream_t s, BodiesView B, TreeView T, const int* order, int n, double* sphere,
-                   double cell, int* cell_key, int hash_mask);
+                   double cell, int* cell_key, int hash_mask, const int* is_large);
 void launch_pairs_impl(hipStream_t s, bool write, const PairArgs& A, int* counts, const int* offsets, uint64_t* keys,
                        int cap, int* overflow);
 void launch_narrow(hipStream_t s, bool write, BodiesView B, TreeView T, const uint64_t* keys,

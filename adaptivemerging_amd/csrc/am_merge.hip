@@ -246,6 +246,8 @@ __global__ void k_bpc_accumulate(BodiesView B, BpcRec* bpc, int n, MergeParams P
     int flag = 1;
     if (P.let_it_breathe && r.breath) flag = 0;
     if (!P.merge_pinned && (pinned(B, r.b1) || pinned(B, r.b2))) flag = 0;
+    // a body on a prescribed motion never merges (its motion would freeze into a pinned collection)
+    if ((B.flags[r.b1] | B.flags[r.b2]) & AM_BODY_KINEMATIC) flag = 0;
     if (r.hist_n != accum) {
         flag = 0;
     } else {
@@ -375,7 +377,8 @@ __global__ void k_sleep_decide(BodiesView B, int* flags, const int* slot_entity,
         atomicAdd(&counts[1], 1);
         return;
     }
-    if (ext[e]) return;
+    // a body on a prescribed motion never sleeps
+    if (ext[e] || (flags[e] & AM_BODY_KINEMATIC)) return;
     double* h = hist + (size_t)e * kSleepHistStride;
     int n = hist_n[e];
     h[n++] = metric_one(B, e);
@@ -1236,6 +1239,7 @@ int merge_merge(am_ctx* c, int* n_merged) {
     // BodyPairContact.checkMergeCondition (:151-177): a pair whose two bodies
     // are both asleep merges without the other tests
     auto asleep = [&](int b) { return (c->flags[b] & AM_BODY_SLEEPING) != 0; };
+    // kinematic bodies never sleep, so this test never pairs one
     bool any = false;
     for (const auto& r : T) any = any || r.flag || (asleep(r.b1) && asleep(r.b2));
     if (!any) return AM_OK;

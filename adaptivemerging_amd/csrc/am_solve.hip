@@ -1601,6 +1601,59 @@ __device__ void sincos(double x, double& s, double& c) {
 }
 }  // namespace fdm
 
+// Prescribed motion of the kinematic bodies at time t:
+// x = x0 + (a sin(w t + phase)) dir, v = (a w cos(w t + phase)) dir, omega = 0.
+__global__ void k_kinematic(BodiesView B, const am_kinematic* K, int n, double t) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const am_kinematic q = K[k];
+    double s, c;
+    fdm::sincos(q.omega * t + q.phase, s, c);
+    const double ds = q.amplitude * s;
+    const double vs = q.amplitude * q.omega * c;
+    const int e = q.body;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        B.x[e * 3 + a] = q.x0[a] + ds * q.dir[a];
+        B.v[e * 3 + a] = vs * q.dir[a];
+        B.w[e * 3 + a] = 0.0;
+    }
+}
+void launch_kinematic(hipStream_t s, BodiesView B, const am_kinematic* k, int n, double t) {
+    if (n > 0) hipLaunchKernelGGL(k_kinematic, dim3((n + 63) / 64), dim3(64), 0, s, B, k, n, t);
+}
+
+// RigidBodySystem.applyImpulse (RigidBodySystem.java:262-267): each held
+// force goes to its body's top-level entity through RigidBody.applyForceW.
+// One thread, so forces on one entity add in the order they were held.
+__global__ void k_impulses(BodiesView B, const ImpulseRec* I, int n) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int k = 0; k < n; k++) {
+        const int b = I[k].body;
+        const int e = B.parent[b] >= 0 ? B.parent[b] : b;
+        apply_force_w(B, e, ld3(I[k].p), ld3(I[k].f));
+    }
+}
+void launch_impulses(hipStream_t s, BodiesView B, const ImpulseRec* I, int n) {
+    if (n > 0) hipLaunchKernelGGL(k_impulses, dim3(1), dim3(64), 0, s, B, I, n);
+}
+
+// RigidBodySystem.applyViscousDecay (RigidBodySystem.java:428-435): v and
+// omega of the listed (top-level) entities scaled (Vector3d.scale).
+__global__ void k_viscous(BodiesView B, const int* targets, int n, double a1, double a2) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int e = targets[k];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        B.v[e * 3 + a] = a1 * B.v[e * 3 + a];
+        B.w[e * 3 + a] = a2 * B.w[e * 3 + a];
+    }
+}
+void launch_viscous(hipStream_t s, BodiesView B, const int* targets, int n, double a1, double a2) {
+    if (n > 0) hipLaunchKernelGGL(k_viscous, dim3((n + 255) / 256), dim3(256), 0, s, B, targets, n, a1, a2);
+}
+
 // The pose update of RigidBody.advancePositions (RigidBody.java:427-437):
 // Rodrigues rotation (expRodrigues, RigidBody.java:382-401) and
 // Matrix3d.normalizeCP.  Returns whether R changed.

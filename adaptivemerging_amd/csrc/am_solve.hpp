@@ -98,6 +98,16 @@ void launch_advance_post(hipStream_t s, BodiesView B, const int* targets, int n,
 // every entity's pose advanced by dt into xa (3 per entity) and Ra (9 per entity)
 void launch_advanced_poses(hipStream_t s, BodiesView B, double dt, double* xa, double* Ra);
 void launch_springs(hipStream_t s, BodiesView B, const am_spring* springs, int n, double ks, double ds);
+// a force held for the next external-force pass (am_impulse_add)
+struct ImpulseRec {
+    int body;
+    int pad;
+    double p[3];
+    double f[3];
+};
+void launch_kinematic(hipStream_t s, BodiesView B, const am_kinematic* k, int n, double t);
+void launch_impulses(hipStream_t s, BodiesView B, const ImpulseRec* I, int n);
+void launch_viscous(hipStream_t s, BodiesView B, const int* targets, int n, double a1, double a2);
 void launch_update_jinv(hipStream_t s, BodiesView B);
 void launch_metric(hipStream_t s, BodiesView B, const int* pairs, int n, double* out);
 

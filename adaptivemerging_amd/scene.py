@@ -12,8 +12,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import (AM_BODY_PINNED, AM_GEOM_BOX, AM_GEOM_COLLECTION, AM_GEOM_COMPOSITE, AM_GEOM_PLANE, AM_GEOM_TREE, AM_SPRING_BODYBODY,
-                   AM_SPRING_WORLD, AM_SPRING_ZERO, BODY_DTYPE, NODE_DTYPE, SPRING_DTYPE)
+from ._lib import (AM_BODY_KINEMATIC, AM_BODY_PINNED, AM_GEOM_BOX, AM_GEOM_COLLECTION, AM_GEOM_COMPOSITE, AM_GEOM_PLANE, AM_GEOM_TREE, AM_SPRING_BODYBODY,
+                   AM_SPRING_WORLD, AM_SPRING_ZERO, BODY_DTYPE, NODE_DTYPE, SPRING_DTYPE, KINEMATIC_DTYPE)
 
 
 def rot_axis_angle(axis, angle) -> np.ndarray:
@@ -58,6 +58,7 @@ class Scene:
     subs: list = field(default_factory=list)         # sub-bodies of composites (BODY_DTYPE records)
     sub_b2c: list = field(default_factory=list)      # their transforms into the composite frame (12 each)
     spinners: dict = field(default_factory=dict)     # body -> RigidBody.spinner (added to omega by the app's keys)
+    kinematics: list = field(default_factory=list)   # prescribed motions (KINEMATIC_DTYPE records)
 
     # -- body builders -------------------------------------------------
     def _append(self, rec, x, R, v, w, name):
@@ -368,6 +369,31 @@ class Scene:
             rec["l0"] = 0.5   # the field's initial value; the constructor leaves it
         self.springs.append(rec)
         return len(self.springs) - 1
+
+    # -- prescribed motion (include/am.h am_kinematic) --------------------
+    def set_kinematic(self, body: int, amplitude: float, omega: float, phase: float, direction=(0.0, 1.0, 0.0)):
+        """Makes `body` kinematic: pinned, and at time t placed at
+        x0 + (amplitude sin(omega t + phase)) direction with velocity
+        (amplitude omega cos(omega t + phase)) direction, x0 its current
+        position."""
+        rec = self.bodies[body]
+        rec["flags"] = int(rec["flags"]) | AM_BODY_PINNED | AM_BODY_KINEMATIC
+        rec["mass"] = 0.0
+        rec["minv"] = 0.0
+        rec["jinv0"] = 0.0
+        rec["inertia0"] = 0.0
+        k = np.zeros((), dtype=KINEMATIC_DTYPE)
+        k["body"] = body
+        k["x0"] = self.x[body]
+        k["dir"] = direction
+        k["amplitude"], k["omega"], k["phase"] = amplitude, omega, phase
+        self.kinematics.append(k)
+        return len(self.kinematics) - 1
+
+    def kinematic_array(self):
+        if not self.kinematics:
+            return np.zeros(0, dtype=KINEMATIC_DTYPE)
+        return np.array(self.kinematics, dtype=KINEMATIC_DTYPE)
 
     def spring_array(self):
         return np.array(self.springs, dtype=SPRING_DTYPE) if self.springs else np.zeros(0, dtype=SPRING_DTYPE)

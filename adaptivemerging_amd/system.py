@@ -279,6 +279,8 @@ class RigidBodySystem:
         self.gravityAngle = 90.0
         self.springStiffnessMod = 1.0   # RigidBodySystem.java:553
         self.springDampingMod = 1.0     # RigidBodySystem.java:554
+        self.globalViscousLinearDecay = 1.0   # RigidBodySystem.java:550
+        self.globalViscousAngularDecay = 1.0  # RigidBodySystem.java:551
         self.totalSteps = 0
         self.simulationTime = 0.0
         self.computeTime = 0.0
@@ -303,9 +305,17 @@ class RigidBodySystem:
         p.gravity_angle_deg = self.gravityAngle
         p.spring_stiffness_mod = self.springStiffnessMod
         p.spring_damping_mod = self.springDampingMod
+        p.viscous_linear_decay = self.globalViscousLinearDecay
+        p.viscous_angular_decay = self.globalViscousAngularDecay
         self.ctx.set_params()
         self.ctx.apply_gravity()
         self.ctx.apply_spring_forces()
+        self.ctx.apply_impulses()
+
+    def applyImpulse(self, body: int, pointW, force):
+        """Holds a force at a world point for the next step's external-force
+        pass (the impulse tool, RigidBodySystem.applyImpulse :262-267)."""
+        self.ctx.impulse_add(body, pointW, force)
 
     def advanceTime(self, dt: float):
         """advanceTime (:102-185)."""
@@ -314,6 +324,8 @@ class RigidBodySystem:
         self.merging.apply()
         self.sleeping.apply()
         self.ctx.clear_bodies()
+        # prescribed motions where the animate tool acts (:109)
+        self.ctx.kinematic_update(self.simulationTime)
         if self.sleeping.wakeAllRequested:
             self.sleeping.wakeAll()
         if self.merging.unmergeAllRequested:
@@ -357,8 +369,10 @@ class RigidBodySystem:
                 self.merging.mergingBuildTime = time.perf_counter() - t1
             self.mergingTime += time.perf_counter() - t1
         self.sleeping.sleep()
+        self.ctx.apply_viscous_decay()
         self.computeTime = time.perf_counter() - t0
         self.simulationTime += dt
+        self.ctx.sim_time = self.simulationTime
         self.exportDataToFile()
 
     def postStabilization(self, dt: float):
@@ -411,6 +425,7 @@ class RigidBodySystem:
             self.ctx.set_springs(springs)
         self.totalSteps = 0
         self.simulationTime = 0.0
+        self.ctx.sim_time = 0.0
 
     def state(self):
         x, R, v, w = self.ctx.pull_state()

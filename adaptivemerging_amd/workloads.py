@@ -92,6 +92,56 @@ def box_pile(seed: int = 0, n: int = 20000, bin_size: float = 4.0, edge=(0.1, 0.
     return s
 
 
+def domino_platforms(seed: int = 0, platforms: int = 10, rows_per_platform: int = 2, per_row: int = 100,
+                     size=(0.05, 0.5, 0.25), spacing: float = 0.2, density: float = 1000.0, friction: float = 0.5,
+                     amplitude: float = 0.02, frequency: float = 0.5, gap: float = 0.001) -> Scene:
+    """Rows of dominoes (size: thickness along the row x height x width,
+    spaced `spacing` apart along x) standing on kinematic platforms that
+    oscillate vertically (amplitude, frequency in Hz, phase U[0, 2 pi) from
+    the seed), over a ground plane; the shape of scenes3D/dominosPlatforms.xml
+    with prescribed platform motion in place of its springs.  The first
+    domino of every row is listed in scene.row_heads (for the push that
+    starts the cascade, see domino_push)."""
+    rng = np.random.default_rng(seed)
+    phases = rng.uniform(0.0, 2.0 * math.pi, size=platforms)
+    s = Scene()
+    s.add_plane([0, 0, 0], [0, 1, 0], friction=friction, name="ground")
+    T, H, W = size
+    row_len = per_row * spacing
+    row_gap = 1.5 * W
+    plat_w = rows_per_platform * row_gap + W
+    plat_h = 0.2
+    y0 = 1.0
+    omega = 2.0 * math.pi * frequency
+    s.row_heads = []
+    for k in range(platforms):
+        zc = k * (plat_w + 1.0)
+        pb = s.add_box([row_len + 1.0, plat_h, plat_w], [row_len / 2, y0, zc], density=density, friction=friction,
+                       name=f"platform{k}")
+        s.set_kinematic(pb, amplitude, omega, float(phases[k]), (0.0, 1.0, 0.0))
+        top = y0 + amplitude * math.sin(float(phases[k])) + plat_h / 2
+        for r in range(rows_per_platform):
+            z = zc + (r - (rows_per_platform - 1) / 2) * row_gap
+            for i in range(per_row):
+                b = s.add_box([T, H, W], [0.5 + i * spacing, top + H / 2 + gap, z], density=density, friction=friction,
+                              name=f"domino{k}_{r}_{i}")
+                if i == 0:
+                    s.row_heads.append(b)
+    return s
+
+
+def domino_push(scene: Scene, x_now, dt: float, speed: float = 5.0):
+    """The push that starts every row: for each row head, (body, point, force)
+    with force m * speed / dt along +x at its current centre (x_now: the
+    bodies' positions), so one step of it adds `speed` to the head's
+    velocity."""
+    out = []
+    for b in scene.row_heads:
+        m = float(scene.bodies[b]["mass"])
+        out.append((b, np.array(x_now[b], dtype=np.float64), np.array([m * speed / dt, 0.0, 0.0])))
+    return out
+
+
 def box_cluster(seed: int = 0, n: int = 64, spacing: float = 0.3, edge=(0.1, 0.5), with_plane: bool = True,
                 spheres: int = 0) -> Scene:
     """Randomly oriented boxes on a loose grid, spaced so that many neighbours

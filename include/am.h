@@ -64,7 +64,10 @@ enum am_geom {
 enum am_body_flags {
     AM_BODY_PINNED = 1,        /* RigidBody.pinned */
     AM_BODY_MAGNET_ACTIVE = 2, /* magnetic && activateMagnet: no clamping in PGS */
-    AM_BODY_SLEEPING = 4       /* RigidBody.sleeping (set and cleared by am_sleep / am_wake) */
+    AM_BODY_SLEEPING = 4,      /* RigidBody.sleeping (set and cleared by am_sleep / am_wake) */
+    AM_BODY_KINEMATIC = 8      /* pinned body whose pose and velocity follow a prescribed motion
+                                  (am_kinematic_set); no reference counterpart.  Such a body never
+                                  merges and never sleeps */
 };
 
 enum am_contact_state { AM_STATE_BROKEN = 0, AM_STATE_ONEDGE = 1, AM_STATE_CLEAR = 2 };
@@ -321,6 +324,49 @@ int am_springs_set(am_ctx* ctx, const am_spring* springs, int32_t n);
  * after the gravity, as applyExternalForces does (:234-257). */
 int am_apply_spring_forces(am_ctx* ctx);
 int am_deltav_pull(am_ctx* ctx, double* dv, int32_t n);   /* 6 per entity: v then w */
+/* A force at a world point, held until the next external-force pass (am_step
+ * runs one after the gravity and the springs), which adds it to the body's
+ * top-level entity as RigidBody.applyForceW does and then drops it: the
+ * impulse tool's stored force, RigidBodySystem.applyImpulse
+ * (RigidBodySystem.java:262-267, called from applyExternalForces :234-257).
+ * Forces added before one pass are applied in the order added. */
+int am_impulse_add(am_ctx* ctx, int32_t body, const double* point_w, const double* force);
+/* RigidBodySystem.applyImpulse (:262-267) on its own: applies the held forces
+ * and drops them (am_step does this inside its external-force pass). */
+int am_apply_impulses(am_ctx* ctx);
+/* RigidBodySystem.applyViscousDecay (:428-435) on its own (am_step does it last). */
+int am_apply_viscous_decay(am_ctx* ctx);
+
+/* ---- prescribed motion (no reference counterpart) ----------------------
+ * A kinematic body is a pinned body (AM_BODY_PINNED | AM_BODY_KINEMATIC in
+ * its am_body_desc) that moves on a sinusoid: at time t its position is
+ * x0 + (amplitude sin(omega t + phase)) dir and its velocity
+ * (amplitude omega cos(omega t + phase)) dir, its rotation stays and its
+ * angular velocity is 0.  Pinned, it has no mass in the solve, but its
+ * velocity enters b = J (v + dt M^-1 f) as every body's does
+ * (Contact.computeB, Contact.java:279-330), so bodies resting on it are
+ * carried along.  The reference's nearest scenes hang platforms on springs
+ * (scenes3D/dominosPlatforms.xml:141-152); its animate tool sets a body's
+ * velocity at the same point of the step (Animation.apply, called from
+ * RigidBodySystem.advanceTime :109). */
+typedef struct am_kinematic {
+    int32_t body;         /* physical body, flagged AM_BODY_PINNED | AM_BODY_KINEMATIC */
+    int32_t pad;
+    double x0[3];
+    double dir[3];
+    double amplitude;
+    double omega;         /* rad/s */
+    double phase;         /* rad */
+} am_kinematic;
+/* Replaces the table of prescribed motions. */
+int am_kinematic_set(am_ctx* ctx, const am_kinematic* k, int32_t n);
+/* Sets every kinematic body's pose and velocity at time t.  am_step calls it
+ * right after clearing the bodies, with t = the simulation time. */
+int am_kinematic_update(am_ctx* ctx, double t);
+/* RigidBodySystem.simulationTime: 0 after am_scene_upload, dt added at the end
+ * of every am_step (RigidBodySystem.java:176). */
+int am_sim_time_get(am_ctx* ctx, double* t);
+int am_sim_time_set(am_ctx* ctx, double t);
 
 /* ---- detection (CollisionProcessor) ----------------------------------- */
 int am_update_contacts_map(am_ctx* ctx);
@@ -370,7 +416,11 @@ int am_advance_positions_post(am_ctx* ctx, double dt);
  * position update, every steps_between_merge_events steps merging, and
  * sleeping (am_wake after the warm start, am_sleep last).  The merging phases
  * run only when enable_merging / enable_unmerging are set or collections
- * exist; the sleeping phases only when enable_sleeping is set. */
+ * exist; the sleeping phases only when enable_sleeping is set.  Kinematic
+ * bodies take their pose at the simulation time right after the bodies are
+ * cleared; the step ends with RigidBodySystem.applyViscousDecay (:173,
+ * :428-435): the velocities of the top-level entities scaled by
+ * viscous_linear_decay and viscous_angular_decay. */
 int am_step(am_ctx* ctx, double dt, am_step_stats* stats);
 
 /* ---- adaptive merging (Merging, BodyPairContact, RigidCollection) ------

@@ -25,7 +25,8 @@ import math
 
 import numpy as np
 
-from adaptivemerging_amd._lib import (AM_BODY_PINNED, AM_BODY_SLEEPING, AM_GEOM_COLLECTION, AM_GEOM_PLANE,
+from adaptivemerging_amd._lib import (AM_BODY_KINEMATIC, AM_BODY_PINNED, AM_BODY_SLEEPING, AM_GEOM_COLLECTION,
+                                      AM_GEOM_PLANE,
                                       AM_SPRING_BODYBODY, AM_STATE_BROKEN, AM_STATE_CLEAR, AM_STATE_ONEDGE,
                                       BODY_DTYPE)
 
@@ -136,14 +137,17 @@ def pair_state(not_on_edge: int, n_active: int) -> int:
     return AM_STATE_ONEDGE
 
 
-def merge_condition(P, hist, states, breath: bool, pinned1: bool, pinned2: bool) -> bool:
+def merge_condition(P, hist, states, breath: bool, pinned1: bool, pinned2: bool, kinematic: bool = False) -> bool:
     """checkMergeCondition (:151-177) without the same-collection test:
     breathing, pinned bodies, checkMotionMetricForMerging (:187-198) and
-    areContactsStable (:212-224)."""
+    areContactsStable (:212-224).  A pair with a body on a prescribed motion
+    (no reference counterpart) never merges."""
     accum = P.step_accum_merging
     if P.enable_merge_let_it_breathe and breath:
         return False
     if not P.enable_merge_pinned and (pinned1 or pinned2):
+        return False
+    if kinematic:
         return False
     if len(hist) != accum or any(m > P.threshold_merge for m in hist):
         return False
@@ -253,6 +257,11 @@ class MergingOracle:
         self.sleep_hist = {}   # entity -> RigidBody.metricHistory
         sp = scene.spring_array() if hasattr(scene, "spring_array") else []
         self.springs = [(int(r["type"]), int(r["body1"]), int(r["body2"])) for r in sp]
+        kin = scene.kinematic_array() if hasattr(scene, "kinematic_array") else []
+        self.kinematics = [(int(r["body"]), [float(a) for a in r["x0"]], [float(a) for a in r["dir"]],
+                            float(r["amplitude"]), float(r["omega"]), float(r["phase"])) for r in kin]
+        self.impulses = []     # forces held for the next external-force pass
+        self.sim_time = 0.0    # RigidBodySystem.simulationTime
 
     # -- helpers --------------------------------------------------------
     def set_params(self, **kw):
@@ -270,6 +279,73 @@ class MergingOracle:
 
     def sleeping(self, e):
         return bool(int(self.desc(e)["flags"]) & AM_BODY_SLEEPING)
+
+    def kinematic(self, e):
+        return bool(int(self.desc(e)["flags"]) & AM_BODY_KINEMATIC)
+
+    # -- prescribed motion, held forces, viscous decay -----------------------
+    def kinematic_update(self, t):
+        """Each kinematic body at time t: x = x0 + (a sin(w t + phase)) dir,
+        v = (a w cos(w t + phase)) dir, omega = 0 (fdlibm sine and cosine)."""
+        if not self.kinematics:
+            return
+        x, R, v, w = self.o.pull_state(self.n_entities)
+        x, v, w = np.array(x), np.array(v), np.array(w)
+        for b, x0, d, amp, om, ph in self.kinematics:
+            arg = om * t + ph
+            ds = amp * self.o.lib.orc_sin(arg)
+            vs = amp * om * self.o.lib.orc_cos(arg)
+            for a in range(3):
+                x[b][a] = x0[a] + ds * d[a]
+                v[b][a] = vs * d[a]
+                w[b][a] = 0.0
+        self._set_state_keep_jinv(x, v, w)
+
+    def _set_state_keep_jinv(self, x, v, w):
+        """Writes x, v, omega and leaves every entity's world inverse inertia
+        as it was (the oracle's set_state recomputes it from R)."""
+        jinv = self.o.pull_jinv()
+        self.o.set_state(x, None, v, w)
+        for i in range(self.n_entities):
+            self.o.set_jinv(i, jinv[i])
+
+    def impulse_add(self, body, point_w, force):
+        self.impulses.append((body, [float(a) for a in point_w], [float(a) for a in force]))
+
+    def apply_external_forces(self):
+        """RigidBodySystem.applyExternalForces (:234-257): gravity, springs,
+        then the held forces once (applyImpulse, :262-267) through
+        RigidBody.applyForceW (RigidBody.java:335-340) on the body's
+        top-level entity."""
+        self.o.apply_gravity()
+        self.o.apply_springs()
+        if not self.impulses:
+            return
+        x, _, _, _ = self.o.pull_state(self.n_entities)
+        f, t = self.o.pull_forces()
+        for b, p, F in self.impulses:
+            e = self.top_of(b)
+            r = [p[a] - float(x[e][a]) for a in range(3)]
+            for a in range(3):
+                f[e][a] = f[e][a] + F[a]
+            tq = cross(r, F)
+            for a in range(3):
+                t[e][a] = t[e][a] + tq[a]
+        self.o.push_forces(f, t)
+        self.impulses = []
+
+    def viscous_decay(self):
+        """RigidBodySystem.applyViscousDecay (:428-435) over the top-level list."""
+        a1, a2 = self.params.viscous_linear_decay, self.params.viscous_angular_decay
+        if a1 == 1.0 and a2 == 1.0:
+            return
+        x, R, v, w = self.o.pull_state(self.n_entities)
+        v, w = np.array(v), np.array(w)
+        for e in self.top:
+            for a in range(3):
+                v[e][a] = a1 * v[e][a]
+                w[e][a] = a2 * w[e][a]
+        self._set_state_keep_jinv(None, v, w)
 
     def set_sleeping(self, e, value):
         rec = self.desc(e).copy()
@@ -464,8 +540,7 @@ class MergingOracle:
             dv[e] = 0.0
         self.o.push_forces(f, t)
         self.o.push_deltav(dv)
-        self.o.apply_gravity()
-        self.o.apply_springs()
+        self.apply_external_forces()
 
     def internal_contacts(self):
         rows = [c for p in self.ibpc for c in p["contacts"]]
@@ -533,7 +608,7 @@ class MergingOracle:
                 ext.add(self.top_of(a))
                 ext.add(self.top_of(b))
             for e in list(self.top):
-                if self.sleeping(e) or e in ext:
+                if self.sleeping(e) or e in ext or self.kinematic(e):
                     continue
                 h = self.sleep_hist.setdefault(e, [])
                 h.append(self.o.motion_metric(e))
@@ -657,7 +732,7 @@ class MergingOracle:
             if len(r["states"]) > accum:
                 r["states"].pop(0)
             r["flag"] = merge_condition(P, r["hist"], r["states"], r["breath"], self.pinned(r["b1"]),
-                                        self.pinned(r["b2"]))
+                                        self.pinned(r["b2"]), self.kinematic(r["b1"]) or self.kinematic(r["b2"]))
 
     # -- Merging.merge (:73-163) ---------------------------------------------
     def merge(self):
