@@ -872,10 +872,6 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
     }
     launch_unpack(s, S, c->n_contacts);
     AM_HIP(c, hipGetLastError());
-    if (collect) {
-        r = train_append(c);
-        if (r) return r;
-    }
     r = read_scalars(c, 6, 1);
     if (r) return r;
     {
@@ -908,6 +904,12 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
     c->last_change = lc;
     if (iters) *iters = used;
     if (last) *last = lc;
+    // training pairs only from solves that reached the tolerance: their
+    // lambdas are the converged solution the network is to predict
+    if (collect && used > 0 && lc < P.tolerance) {
+        r = train_append(c);
+        if (r) return r;
+    }
     return AM_OK;
 }
 

@@ -225,8 +225,11 @@ def sphere_tree_body_shape(rng: np.random.Generator, depth: int = 3, branching: 
 
 def sphere_tree_drop(seed: int = 0, n: int = 512, box=(2.0, 1.0, 2.0), heights=(1.0, 5.0), depth: int = 3,
                      branching: int = 8) -> Scene:
-    """n sphere-tree bodies dropped from heights U(heights) into an open box of
-    5 static planes (floor + 4 walls), x/z extent box[0] x box[2]."""
+    """n sphere-tree bodies placed at random heights U(heights) in an open box
+    of 5 static planes (floor + 4 walls), x/z extent box[0] x box[2], with no
+    overlap check: the trees start interpenetrating, which gives detection
+    tests many tree-tree leaf contacts at once (sphere_tree_pile is the
+    contact-free drop)."""
     rng = np.random.default_rng(seed)
     s = Scene()
     hx, hz = box[0] / 2, box[2] / 2
@@ -239,6 +242,71 @@ def sphere_tree_drop(seed: int = 0, n: int = 512, box=(2.0, 1.0, 2.0), heights=(
         centres, radii, children, mass, I = sphere_tree_body_shape(rng, depth, branching)
         x = [rng.uniform(-hx + 0.2, hx - 0.2), rng.uniform(*heights), rng.uniform(-hz + 0.2, hz - 0.2)]
         s.add_tree(centres, radii, children, x, random_rotation(rng), mass=mass, inertia=I, friction=0.4)
+    return s
+
+
+def sphere_tree_pile(seed: int = 0, n: int = 512, box=(2.0, 1.0, 2.0), bottom: float = 1.0, spacing: float = 0.4,
+                     depth: int = 3, branching: int = 8, max_tries: int = 20) -> Scene:
+    """The sphere-tree configuration: n tree bodies dropped into an open box
+    of 5 static planes, spawned with no two leaf spheres overlapping, so the
+    pile builds up from contact-free drops.  Bodies go to the slots of a
+    lattice (`spacing` apart, lowest layer at `bottom`, filled layer by layer)
+    with a random rotation and a jitter of +-3 cm, retried up to max_tries
+    times while a leaf would overlap a leaf of a body already placed; a slot
+    that stays blocked is skipped.  512 trees of 512 leaves fill up to about 19 m of
+    the 2 x 2 m column this way (their leaves alone are about 13 m^3)."""
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    hx, hz = box[0] / 2, box[2] / 2
+    s.add_plane([0, 0, 0], [0, 1, 0], friction=0.4)
+    s.add_plane([-hx, 0, 0], [1, 0, 0], friction=0.4)
+    s.add_plane([hx, 0, 0], [-1, 0, 0], friction=0.4)
+    s.add_plane([0, 0, -hz], [0, 0, 1], friction=0.4)
+    s.add_plane([0, 0, hz], [0, 0, -1], friction=0.4)
+    # slots keep a body's leaves (within 0.3 m of its centre) off the walls
+    nx = max(1, int((box[0] - 0.6) // spacing) + 1)
+    nz = max(1, int((box[2] - 0.6) // spacing) + 1)
+    xs = (np.arange(nx) - (nx - 1) / 2) * spacing
+    zs = (np.arange(nz) - (nz - 1) / 2) * spacing
+    cen = np.zeros((n, 3))
+    rad = np.zeros(n)
+    leaves = []  # per placed body: (leaf centres world, leaf radii, kd tree)
+    slot = 0
+    for q in range(n):
+        centres, radii, children, mass, I = sphere_tree_body_shape(rng, depth, branching)
+        leaf = [k for k in range(len(centres)) if not children[k]]
+        lc = np.array([centres[k] for k in leaf])
+        lr = np.array([radii[k] for k in leaf])
+        rb = float(radii[0] + np.linalg.norm(centres[0]))
+        while True:
+            layer, rest = divmod(slot, nx * nz)
+            slot += 1
+            base = np.array([xs[rest % nx], bottom + layer * spacing, zs[rest // nx]])
+            found = False
+            for _ in range(max_tries):
+                R = random_rotation(rng)
+                x = base + rng.uniform(-0.03, 0.03, 3)
+                w = lc @ R.T + x
+                near = np.nonzero(np.linalg.norm(cen[:q] - x, axis=1) < rad[:q] + rb)[0]
+                ok = True
+                for j in near:
+                    w2, lr2, tree = leaves[j]
+                    for i, js in enumerate(tree.query_ball_point(w, lr + lr2.max())):
+                        if js and np.any(np.linalg.norm(w2[js] - w[i], axis=1) < lr[i] + lr2[js]):
+                            ok = False
+                            break
+                    if not ok:
+                        break
+                if ok:
+                    found = True
+                    break
+            if found:
+                break
+        cen[q], rad[q] = x, rb
+        leaves.append((w, lr, cKDTree(w)))
+        s.add_tree(centres, radii, children, x, R, mass=mass, inertia=I, friction=0.4)
     return s
 
 

@@ -175,7 +175,8 @@ typedef struct am_params {
      * start; 1 the MLP's predicted start, falling back to the plain start
      * whenever its start residual is not smaller */
     int32_t learned_warm_start;
-    int32_t collect_training;         /* 1: each main solve appends (features, converged lambda) */
+    int32_t collect_training;         /* 1: each main solve that reaches the tolerance appends
+                                         (features, converged lambda) to the training set */
 } am_params;
 
 /* A spring (Spring.java:10-104).  ZERO pulls body1's point pb1 towards the

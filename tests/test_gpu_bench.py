@@ -1,0 +1,31 @@
+"""bench.py end to end at a small size: every phase (settle, warm-up, tuning,
+training on held-out piles, plain and learned timed steps, CPU baseline) runs
+and the one JSON line carries the fields the measurement needs."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_bench_small_end_to_end(gpu_lib):
+    cmd = [sys.executable, os.path.join(ROOT, "bench.py"), "--bodies", "400", "--settle", "40", "--warmup", "1",
+           "--steps", "3", "--iterations", "200", "--train-scenes", "1", "--train-bodies", "120", "--train-settle",
+           "20", "--train-steps", "6", "--epochs", "2", "--cpu-sweeps", "5"]
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=240)
+    assert out.returncode == 0, out.stderr[-3000:]
+    line = [ln for ln in out.stdout.splitlines() if ln.startswith("{")][-1]
+    rec = json.loads(line)
+    for key in ("metric", "value", "unit", "ms_per_step", "roofline", "cpu_baseline", "config"):
+        assert key in rec
+    assert rec["value"] > 0 and rec["roofline"]["frac"] > 0
+    assert rec["cpu_baseline"]["value"] > 0 and rec["cpu_baseline"]["cores"] == 1
+    lrn = rec["config"]["learned"]
+    assert lrn["train_samples"] > 0
+    assert lrn["replay_identical"] is True
+    assert rec["config"]["plain"]["contacts_mean"] > 0

@@ -1,6 +1,7 @@
 """Parity at the full sizes of the measured configurations: the 20,000-box
-pile, the 33-layer jenga tower and 512 sphere-tree bodies of 512 leaves
-(about a million contacts).
+pile, the 33-layer jenga tower, 512 sphere-tree bodies of 512 leaves (dropped
+without overlap, and placed interpenetrating for about a million contacts)
+and the batched scenes.
 
 The GPU runs the scene until it is in contact; its state then goes to the
 oracle, both detect from it (the contacts must agree bit for bit), and both
@@ -98,15 +99,40 @@ def test_jenga_33_layers(gpu_lib):
     assert len(ctx.contacts()["body1"]) > 300
 
 
-def test_sphere_trees_512(gpu_lib):
-    """512 bodies of 512 leaves placed at random in the open box: their trees
-    interpenetrate, about a million leaf contacts."""
+def test_sphere_trees_512_interpenetrating(gpu_lib):
+    """512 bodies of 512 leaves placed at random in the open box with no
+    overlap check: their trees interpenetrate, about a million leaf contacts
+    (the widest tree-tree detection load)."""
     scene = workloads.sphere_tree_drop(seed=0, n=512)
     dt = 0.005
     ctx, st = settle(scene, 4, dt, iterations=20, tolerance=1e-6, gravity_amount=9.81, enable_merging=0,
                      enable_unmerging=0, enable_sleeping=0)
     assert st["n_contacts"] > 100000
     sync_and_step(ctx, scene, dt, 1, iterations=5, tolerance=0.0)
+
+
+def test_sphere_tree_pile_512(gpu_lib):
+    """The sphere-tree configuration: 512 trees of 512 leaves spawned with no
+    leaf overlap and dropped into the open box (dt 0.005, 100 sweeps); once
+    the pile has formed, the GPU's state goes to the oracle and both step on
+    bit for bit."""
+    scene = workloads.sphere_tree_pile(seed=0, n=512)
+    dt = 0.005
+    ctx = Context()
+    ctx.upload(scene)
+    ctx.set_params(iterations=100, tolerance=1e-6, gravity_amount=9.81, friction_override=1, friction=0.4,
+                   enable_merging=0, enable_unmerging=0, enable_sleeping=0)
+    first = ctx.step(dt)
+    assert first["n_contacts"] == 0, "the spawn must be contact free"
+    st = first
+    for k in range(1, 500):
+        st = ctx.step(dt)
+        if k % 100 == 0:
+            print(f"pile step {k}: contacts {st['n_contacts']} sweeps {st['iterations_used']} "
+                  f"{st['ms_total']:.2f} ms", flush=True)
+    assert st["n_contacts"] > 20000
+    n = sync_and_step(ctx, scene, dt, 1, iterations=5, tolerance=0.0)
+    assert n > 20000
 
 
 def test_batched_256_scenes(gpu_lib):
