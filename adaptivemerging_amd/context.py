@@ -91,6 +91,16 @@ class Context:
         f = np.ascontiguousarray(force, dtype=np.float64)
         self._check(self.lib.am_impulse_add(self.h, body, ptr(p), ptr(f)))
 
+    def scene_iterations(self):
+        """(sweeps used, max |delta lambda| of the last sweep) per scene of
+        the last main solve that stopped scenes one by one."""
+        n = C.c_int32()
+        self._check(self.lib.am_scene_iterations_pull(self.h, None, None, 0, C.byref(n)))
+        used = np.zeros(n.value, np.int32)
+        last = np.zeros(n.value)
+        self._check(self.lib.am_scene_iterations_pull(self.h, ptr(used), ptr(last), n.value, C.byref(n)))
+        return used, last
+
     def apply_impulses(self):
         self._check(self.lib.am_apply_impulses(self.h))
 

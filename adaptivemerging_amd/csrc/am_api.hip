@@ -771,6 +771,7 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
     S.in_collection = main_solve ? 0 : 1;
     S.lam_alt = nullptr;
     S.iters_used = nullptr;
+    S.scene_used = nullptr;
     const bool flow = P.pgs_schedule == 0;
     fill_int(s, c->d_scalars.p + 1, 1, c->n_contacts);
     for (auto& e : c->kev)
@@ -816,6 +817,34 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
     G.error = c->d_barrier.p + 1;
     G.iter_max = c->s_iter_max.p;
     G.iters_used = c->d_scalars.p + 6;
+    // several scenes in one context stop one by one, as each would alone
+    G.per_scene = main_solve && c->n_scenes > 1 && P.tolerance > 0 ? 1 : 0;
+    G.n_scenes = c->n_scenes;
+    G.g_scene = nullptr;
+    G.scene_max = nullptr;
+    G.scene_done = nullptr;
+    G.scene_groups = nullptr;
+    G.scene_used = nullptr;
+    if (main_solve) c->scene_solve_valid = false;
+    if (G.per_scene) {
+        const size_t ns = (size_t)c->n_scenes;
+        AM_HIP(c, c->g_scene.ensure(std::max(c->n_groups, 1)));
+        AM_HIP(c, c->scene_used.ensure(ns));
+        AM_HIP(c, c->scene_groups.ensure(ns));
+        AM_HIP(c, c->scene_max.ensure(ns * iterations));
+        AM_HIP(c, c->scene_done.ensure(ns * (iterations + 1)));
+        fill_int(s, (int*)c->scene_groups.p, (int)ns, 0);
+        fill_int(s, (int*)c->scene_max.p, (int)(2 * ns * iterations), 0);
+        fill_int(s, (int*)c->scene_done.p, (int)(ns * (iterations + 1)), 0);
+        launch_group_scene(s, c->s_group_desc.p, c->n_groups, c->s_order.p, c->c_b1.p, c->d_desc.p, c->g_scene.p,
+                           c->scene_groups.p);
+        G.g_scene = c->g_scene.p;
+        G.scene_max = c->scene_max.p;
+        G.scene_done = c->scene_done.p;
+        G.scene_groups = c->scene_groups.p;
+        G.scene_used = c->scene_used.p;
+        S.scene_used = c->scene_used.p;
+    }
     int grid = 1, block = 64;
     pgs_shape(c, P, &grid, &block);
     if (learned) {
@@ -863,12 +892,17 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
         S.lam_alt = c->s_lam_alt.p;
         S.iters_used = G.iters_used;
         AM_HIP(c, hipEventRecord(c->kev[1], s));
-        AM_HIP(c, launch_pgs_flow(s, F, fgrid, P.pgs_plain_launch != 0, (int)n_dv));
+        AM_HIP(c, launch_pgs_flow(s, F, fgrid, P.pgs_plain_launch != 0, (int)n_dv, bodies_view(c)));
         AM_HIP(c, hipEventRecord(c->kev[2], s));
     } else {
         AM_HIP(c, hipEventRecord(c->kev[1], s));
         AM_HIP(c, launch_pgs(s, G, grid, block, P.pgs_plain_launch != 0));
         AM_HIP(c, hipEventRecord(c->kev[2], s));
+        launch_scene_finish(s, G);
+    }
+    if (G.per_scene) {
+        c->scene_solve_valid = true;
+        c->scene_iters = iterations;
     }
     launch_unpack(s, S, c->n_contacts);
     AM_HIP(c, hipGetLastError());
@@ -950,9 +984,15 @@ int am_scene_upload_ex(am_ctx* c, const am_body_desc* bodies, int32_t n_bodies, 
         c->n_nodes = n_nodes;
         c->n_phys = 0;
         c->node_body.assign(n_nodes, -1);
-        for (int i = 0; i < n_bodies; i++)
+        c->n_scenes = 1;
+        c->scene_solve_valid = false;
+        for (int i = 0; i < n_bodies; i++) {
             if (bodies[i].geom == AM_GEOM_COLLECTION)
                 return fail(c, AM_ERR_INVALID, "collections are created by merging, not uploaded");
+            if (bodies[i].scene < 0 || bodies[i].scene >= (1 << 20))
+                return fail(c, AM_ERR_INVALID, "scene ids must be in [0, 2^20)");
+            c->n_scenes = std::max(c->n_scenes, bodies[i].scene + 1);
+        }
         // merging state starts empty
         c->total_steps = 0;
         c->n_springs = 0;
@@ -1284,6 +1324,30 @@ int am_apply_viscous_decay(am_ctx* c) {
     int r = check_ctx(c);
     if (r) return r;
     return apply_viscous_decay(c);
+}
+
+int am_scene_iterations_pull(am_ctx* c, int32_t* used, double* last, int32_t capacity, int32_t* n_scenes) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (n_scenes) *n_scenes = c->n_scenes;
+    if (!used && !last) return AM_OK;
+    if (capacity < c->n_scenes) return fail(c, AM_ERR_CAPACITY, "per-scene arrays too small");
+    if (!c->scene_solve_valid) return fail(c, AM_ERR_STATE, "the last main solve did not stop scenes one by one");
+    const int ns = c->n_scenes;
+    std::vector<int> u(ns);
+    AM_HIP(c, hipMemcpyAsync(u.data(), c->scene_used.p, (size_t)ns * 4, hipMemcpyDeviceToHost, c->stream));
+    AM_HIP(c, hipStreamSynchronize(c->stream));
+    const int iters = c->scene_iters;
+    for (int sc = 0; sc < ns; sc++) {
+        if (used) used[sc] = u[sc];
+        if (last) {
+            unsigned long long b = 0;
+            if (u[sc] > 0)
+                AM_HIP(c, hipMemcpy(&b, c->scene_max.p + (size_t)sc * iters + u[sc] - 1, 8, hipMemcpyDeviceToHost));
+            std::memcpy(&last[sc], &b, 8);
+        }
+    }
+    return AM_OK;
 }
 
 int am_kinematic_set(am_ctx* c, const am_kinematic* k, int32_t n) {

@@ -162,6 +162,13 @@ struct am_ctx {
     std::vector<am::ImpulseRec> impulses;
     DBuf<am::ImpulseRec> d_imp;
     double sim_time = 0.0;
+    // independent scenes (am_body_desc.scene): count, and the per-scene early-exit state of the last main solve
+    int n_scenes = 1;
+    bool scene_solve_valid = false;
+    int scene_iters = 1;  // sweep cap of that solve (the stride of scene_max)
+    DBuf<int> g_scene, scene_used;
+    DBuf<unsigned> scene_done, scene_groups;
+    DBuf<unsigned long long> scene_max;
     // sleeping (Sleeping.java): per-entity metric histories
     DBuf<double> d_sleep_hist;          // kSleepHistStride per entity, oldest first
     DBuf<int> d_sleep_n, d_sleep_ext;   // history length; active external contact this step

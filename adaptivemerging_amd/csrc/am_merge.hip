@@ -600,6 +600,8 @@ struct Event {
         std::memset(&d, 0, sizeof d);
         d.geom = AM_GEOM_COLLECTION;
         d.flags = (E.pinned ? AM_BODY_PINNED : 0) | (E.sleeping ? AM_BODY_SLEEPING : 0);
+        // members share one scene (bodies of different scenes never touch)
+        d.scene = members[e].empty() ? 0 : c->desc[members[e][0]].scene;
         d.mass = E.mass;
         d.minv = E.minv;
         for (int i = 0; i < 9; i++) {

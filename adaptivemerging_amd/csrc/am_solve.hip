@@ -419,7 +419,8 @@ __global__ void k_unpack(SetupArgs S) {
     int c = S.order[k];
     double* rec = &S.rec[(size_t)k * kRecDoubles];
     ContactsView C = S.C;
-    if (S.lam_alt && (*S.iters_used & 1)) {
+    const int used = S.scene_used ? S.scene_used[S.B.desc[S.C.b1[c]].scene] : *S.iters_used;
+    if (S.lam_alt && (used & 1)) {
         // dataflow schedule: an odd last sweep left lambda in the odd-sweep buffer
 #pragma unroll
         for (int r = 0; r < 3; r++) rec[kRecLambda + r] = S.lam_alt[4 * k + r];
@@ -659,20 +660,35 @@ __global__ __launch_bounds__(256) void k_pgs(PgsArgs P) {
         have = q < cs[c + 1];
         if (have) head = gdesc[q];
     };
+    int it = 0;
+    // one group; with per-scene exits a sweep skips the groups of a scene
+    // whose previous sweep was under the tolerance (that sweep is final
+    // after the grid barrier), and each group's max goes to its scene too
+    auto run_group = [&](int q, int4 gd, int mode, bool last, double& mx) {
+        if (!P.per_scene || mode == 0) {
+            process_group(P, gd, grec + (size_t)gd.x * kRec2, mode, last, mx);
+            return;
+        }
+        const int sc = P.g_scene[q];
+        if (it > 0) {
+            const double prev = __longlong_as_double((long long)__hip_atomic_load(
+                &P.scene_max[(size_t)sc * P.iterations + it - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            if (prev < P.tolerance) return;
+        }
+        double gmx = 0.0;
+        process_group(P, gd, grec + (size_t)gd.x * kRec2, mode, last, gmx);
+        if (!(gmx <= 0))
+            atomicMax(&P.scene_max[(size_t)sc * P.iterations + it], (unsigned long long)__double_as_longlong(gmx));
+        mx = gmx != gmx ? gmx : fmax(mx, gmx);
+    };
     auto run_colour = [&](int ci, int mode, bool last, double& mx) {
         const int c = s_cols[ci];
-        if (have) process_group(P, head, grec + (size_t)head.x * kRec2, mode, last, mx);
-        for (int q = cs[c] + slot0 + stride; q < cs[c + 1]; q += stride) {
-            const int4 gd = gdesc[q];
-            process_group(P, gd, grec + (size_t)gd.x * kRec2, mode, last, mx);
-        }
+        if (have) run_group(cs[c] + slot0, head, mode, last, mx);
+        for (int q = cs[c] + slot0 + stride; q < cs[c + 1]; q += stride) run_group(q, gdesc[q], mode, last, mx);
     };
     auto run_tail = [&](int mode, bool last, double& mx) {
         if (lane0)
-            for (int q = cs[kMaxColours]; q < cs[kMaxColours + 1]; q++) {
-                const int4 gd = gdesc[q];
-                process_group(P, gd, grec + (size_t)gd.x * kRec2, mode, last, mx);
-            }
+            for (int q = cs[kMaxColours]; q < cs[kMaxColours + 1]; q++) run_group(q, gdesc[q], mode, last, mx);
     };
     // Ends a colour: wait for this lane's deltaV stores, read the next
     // colour's descriptor, meet the other workgroups.
@@ -697,7 +713,6 @@ __global__ __launch_bounds__(256) void k_pgs(PgsArgs P) {
             if (!next(first_after_tail)) return;
         }
     }
-    int it = 0;
     for (; it < P.iterations; it++) {
         const bool last = (it == P.iterations - 1);
         double mx = 0;
@@ -1031,20 +1046,64 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         };
         bool nx_active = false;
         int4 nx_gd = make_int4(0, 0, 0, 0), nx_fd = make_int4(0, 0, 0, 0);
+        int nx_sc = 0;
         if (w < n_steps) {
             const int q = step_group(w, nx_active);
             nx_gd = gdesc[q];
             nx_fd = fdesc[q];
+            if (P.per_scene) nx_sc = P.g_scene[q];
         }
         for (int st = w; st < n_steps; st += W) {
             // descriptors of this step were loaded during the previous one
-            const bool active = nx_active;
+            const bool has_group = nx_active;
             const int4 gd = nx_gd;
             const int4 fd = nx_fd;
+            const int sc = nx_sc;
             if (st + W < n_steps) {
                 const int q = step_group(st + W, nx_active);
                 nx_gd = gdesc[q];
                 nx_fd = fdesc[q];
+                if (P.per_scene) nx_sc = P.g_scene[q];
+            }
+            // per-scene exits: a row whose scene was under the tolerance at
+            // sweep p-2 (or p-1 before the last sweep) skips this pass; lane
+            // 0 of the row waits until every group of the scene has finished
+            // those sweeps.  Every group they wait for comes earlier in the
+            // (pass, step) order, so the earliest undone step can always run.
+            bool active = has_group;
+            if (P.per_scene && p >= 1 && P.check_tolerance) {
+                const int t1 = p - 2, t2 = (p == lastp) ? p - 1 : 0;
+                if (t1 >= 1 || t2 >= 1) {
+                    if (pend) release();  // a group waited on may need this wave's last counters
+                    const unsigned want = has_group ? P.scene_groups[sc] : 0u;
+                    const unsigned* d1 = &P.scene_done[(size_t)sc * (P.iterations + 1) + (t1 >= 1 ? t1 : t2)];
+                    const unsigned* d2 = &P.scene_done[(size_t)sc * (P.iterations + 1) + (t2 >= 1 ? t2 : t1)];
+                    long spins = 0;
+                    while (true) {
+                        bool ok = true;
+                        if (has_group && li == 0) ok = ld_cnt(d1) == want && ld_cnt(d2) == want;
+                        if (__all(ok)) break;
+                        __builtin_amdgcn_s_sleep(1);
+                        if (++spins > kFlowSpinLimit ||
+                            ((spins & 255) == 0 && uniform(__hip_atomic_load(P.error, __ATOMIC_RELAXED,
+                                                                              __HIP_MEMORY_SCOPE_AGENT)))) {
+                            if (lane == 0) atomicOr(P.error, 1);
+                            return;
+                        }
+                    }
+                    bool skip = false;
+                    if (has_group && li == 0) {
+                        const unsigned long long* m = &P.scene_max[(size_t)sc * P.iterations];
+                        auto under = [&](int t) {
+                            return __longlong_as_double((long long)__hip_atomic_load(
+                                       const_cast<unsigned long long*>(&m[t - 1]), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT)) < P.tolerance;
+                        };
+                        skip = (t1 >= 1 && under(t1)) || (t2 >= 1 && under(t2));
+                    }
+                    skip = __shfl(skip, row * 16, 64);
+                    active = has_group && !skip;
+                }
             }
             const int n = active ? gd.y - gd.x : 0;
             int nmax = n;
@@ -1114,6 +1173,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             __builtin_amdgcn_s_waitcnt(0);  // LDS copies landed
             // rows with fewer contacts idle; contacts past the LDS slots
             // (groups of more than kDmaSlots contacts) read global memory
+            double smx = 0.0;  // this step's max |dl| and NaN flag (row-uniform)
+            bool snan = false;
             for (int t = 0; t < nmax; t++) {
                 WaveRec rc;
                 if (t < kDmaSlots) {
@@ -1126,7 +1187,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                     const double* rg = P.rec + (size_t)k * kRecDoubles;
                     read_wave_rec(rg, lam_src(F, k, p), lmap, rc);
                 }
-                if (t < n) row_contact(F, rc, gd.x + t, p, last, lead, f, dv, mx, nan);
+                if (t < n) row_contact(F, rc, gd.x + t, p, last, lead, f, dv, smx, snan);
+            }
+            mx = fmax(mx, smx);
+            nan = nan || snan;
+            if (P.per_scene && p >= 1 && has_group && lead) {
+                // the scene's max for this sweep, then this group counts as done
+                if (active && !(smx <= 0))
+                    atomicMax(&P.scene_max[(size_t)sc * P.iterations + p - 1],
+                              (unsigned long long)__double_as_longlong(
+                                  snan ? __longlong_as_double(0x7ff8000000000000ll) : smx));
+                __builtin_amdgcn_s_waitcnt(0);
+                __hip_atomic_fetch_add(&P.scene_done[(size_t)sc * (P.iterations + 1) + p], 1u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
             }
             // publish deltaV now, the counters once the stores are done
             if (active && mine && dyn) st_coherent(&cur[e * 6 + comp], dv);
@@ -1168,9 +1241,46 @@ __global__ void k_flow_finish(PgsArgs P) {
             }
     *P.iters_used = used;
 }
-__global__ void k_flow_select_dv(double* dv, const double* dv_alt, int n, const int* iters_used) {
+__global__ void k_flow_select_dv(double* dv, const double* dv_alt, int n, const int* iters_used,
+                                 const int* scene_used, const am_body_desc* desc) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && (*iters_used & 1)) dv[i] = dv_alt[i];
+    if (i >= n) return;
+    const int used = scene_used ? scene_used[desc[i / 6].scene] : *iters_used;
+    if (used & 1) dv[i] = dv_alt[i];
+}
+
+// Sweeps used per scene: the first sweep whose max |dl| over the scene was
+// under the tolerance, else all (a scene's skipped sweeps read 0).
+__global__ void k_scene_finish(PgsArgs P) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.n_scenes) return;
+    int used = P.iterations;
+    for (int t = 1; t <= P.iterations; t++)
+        if (__longlong_as_double((long long)P.scene_max[(size_t)s * P.iterations + t - 1]) < P.tolerance) {
+            used = t;
+            break;
+        }
+    P.scene_used[s] = used;
+}
+void launch_scene_finish(hipStream_t s, const PgsArgs& P) {
+    if (P.per_scene && P.n_scenes > 0)
+        hipLaunchKernelGGL(k_scene_finish, dim3((P.n_scenes + 63) / 64), dim3(64), 0, s, P);
+}
+
+__global__ void k_group_scene(const int* group_desc, int n_groups, const int* order, const int* cb1,
+                              const am_body_desc* desc, int* g_scene, unsigned* scene_groups) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_groups) return;
+    const int first = group_desc[q * 4];
+    const int sc = desc[cb1[order[first]]].scene;
+    g_scene[q] = sc;
+    atomicAdd(&scene_groups[sc], 1u);
+}
+void launch_group_scene(hipStream_t s, const int* group_desc, int n_groups, const int* order, const int* cb1,
+                        const am_body_desc* desc, int* g_scene, unsigned* scene_groups) {
+    if (n_groups > 0)
+        hipLaunchKernelGGL(k_group_scene, dim3((n_groups + 255) / 256), dim3(256), 0, s, group_desc, n_groups, order,
+                           cb1, desc, g_scene, scene_groups);
 }
 
 int pgs_flow_max_blocks(bool* ok) {
@@ -1182,7 +1292,7 @@ int pgs_flow_max_blocks(bool* ok) {
     return n_cu * (per_cu > 0 ? per_cu : 1);
 }
 
-hipError_t launch_pgs_flow(hipStream_t s, const FlowArgs& F, int grid, bool plain, int n_dv) {
+hipError_t launch_pgs_flow(hipStream_t s, const FlowArgs& F, int grid, bool plain, int n_dv, const BodiesView& B) {
     if (plain) {
         hipLaunchKernelGGL(k_pgs_flow, dim3(grid), dim3(64), 0, s, F);
     } else {
@@ -1191,9 +1301,10 @@ hipError_t launch_pgs_flow(hipStream_t s, const FlowArgs& F, int grid, bool plai
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_flow_finish, dim3(1), dim3(64), 0, s, F.P);
+    launch_scene_finish(s, F.P);
     if (n_dv > 0)
         hipLaunchKernelGGL(k_flow_select_dv, dim3((n_dv + 255) / 256), dim3(256), 0, s, F.P.dv, F.dv_alt, n_dv,
-                           F.P.iters_used);
+                           F.P.iters_used, F.P.per_scene ? F.P.scene_used : nullptr, B.desc);
     return hipGetLastError();
 }
 

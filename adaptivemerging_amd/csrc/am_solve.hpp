@@ -26,6 +26,7 @@ struct SetupArgs {
     int in_collection;
     const double* lam_alt;  // unpack: odd-sweep lambda of the dataflow schedule, stride 4 (null: coloured)
     const int* iters_used;  // unpack: sweeps used (with lam_alt)
+    const int* scene_used = nullptr;  // unpack with lam_alt: sweeps used per scene (null: iters_used for all)
 };
 
 // Inputs of the persistent coloured PGS kernel.
@@ -50,6 +51,18 @@ struct PgsArgs {
     int* iters_used;
     unsigned* barrier;             // grid-barrier counter (zeroed before launch)
     int* error;                    // set if a barrier spin limit is hit
+    // Independent scenes that stop one by one (PGS.java:190 per scene): with
+    // per_scene set, a group whose scene has converged skips the remaining
+    // sweeps.  Arrays are indexed by scene s; scene_max[s * iterations + t]
+    // is sweep t+1's max |dl| over the scene's groups, scene_done[s *
+    // (iterations + 1) + p] the scene's groups that finished pass p.
+    int per_scene = 0;
+    int n_scenes = 1;
+    const int* g_scene = nullptr;              // scene of each sorted group
+    unsigned long long* scene_max = nullptr;
+    unsigned* scene_done = nullptr;
+    const unsigned* scene_groups = nullptr;    // groups per scene
+    int* scene_used = nullptr;                 // sweeps each scene used (written after the solve)
 };
 
 // Inputs of the dataflow PGS kernel (the same sweeps without grid barriers).
@@ -78,11 +91,16 @@ void launch_colour_layout(hipStream_t s, int n_groups, const uint64_t* sorted, c
                           const int* group_a, const int* group_b, int* colour_start, int* sizes, int* pos_start,
                           int* order, int* group_desc, int* pos_a, int* spos, int* d_total, void* scan_temp);
 void launch_setup(hipStream_t s, const SetupArgs& S, int cap);
+// scene of every sorted group (from its first contact's body 1) and the group count per scene
+void launch_group_scene(hipStream_t s, const int* group_desc, int n_groups, const int* order, const int* cb1,
+                        const am_body_desc* desc, int* g_scene, unsigned* scene_groups);
+// sweeps used per scene (first sweep under the tolerance, else all)
+void launch_scene_finish(hipStream_t s, const PgsArgs& P);
 void launch_unpack(hipStream_t s, const SetupArgs& S, int cap);
 int pgs_max_blocks(int block, bool* ok);
 hipError_t launch_pgs(hipStream_t s, const PgsArgs& P, int grid, int block, bool plain);
 int pgs_flow_max_blocks(bool* ok);
-hipError_t launch_pgs_flow(hipStream_t s, const FlowArgs& F, int grid, bool plain, int n_dv);
+hipError_t launch_pgs_flow(hipStream_t s, const FlowArgs& F, int grid, bool plain, int n_dv, const BodiesView& B);
 void launch_flow_ranks(hipStream_t s, int n_ent, int n_groups, const int* bflags, const int* inc_start,
                        const int* inc_count, const int* inc_items, const int* spos, const int* group_desc,
                        int* flow);

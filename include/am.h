@@ -338,6 +338,18 @@ int am_apply_impulses(am_ctx* ctx);
 /* RigidBodySystem.applyViscousDecay (:428-435) on its own (am_step does it last). */
 int am_apply_viscous_decay(am_ctx* ctx);
 
+/* ---- independent scenes (no reference counterpart) ----------------------
+ * Bodies carry a scene id (am_body_desc.scene, 0 .. 2^20 - 1); bodies of
+ * different scenes never collide, so one context steps many worlds at once.
+ * With more than one scene and a tolerance > 0, the main solve stops each
+ * scene on its own max |delta lambda| (PGS.java:190): a scene under the
+ * tolerance skips the remaining sweeps, exactly as if it were solved alone
+ * in the exported sweep order restricted to its contacts.  This entry point
+ * reports, per scene of the last such solve, the sweeps used and the max
+ * |delta lambda| of its last sweep (either array may be NULL; capacity >=
+ * n_scenes). */
+int am_scene_iterations_pull(am_ctx* ctx, int32_t* used, double* last, int32_t capacity, int32_t* n_scenes);
+
 /* ---- prescribed motion (no reference counterpart) ----------------------
  * A kinematic body is a pinned body (AM_BODY_PINNED | AM_BODY_KINEMATIC in
  * its am_body_desc) that moves on a sinusoid: at time t its position is

@@ -100,6 +100,7 @@ typedef struct orc_contact {
     int32_t state, is_new;
 } orc_contact;
 
+#define ORC_MAX_SCENES 4096
 struct orc_world {
     int32_t n;                /* entities */
     am_body_desc* desc;
@@ -119,6 +120,10 @@ struct orc_world {
      * transforms into the composite's frame (RigidBody.transformB2C) */
     am_body_desc* subs; double* sub_b2c; int32_t n_subs;
     am_params p;
+    /* per scene of the last main solve that stopped scenes one by one */
+    int32_t n_scene_used;
+    int32_t scene_used[ORC_MAX_SCENES];
+    double scene_last[ORC_MAX_SCENES];
 };
 
 static void push_contact_slot(orc_world* w) {
@@ -1077,6 +1082,8 @@ int32_t orc_detect(orc_world* w) {
             int32_t b2 = w->top[j];
             if (pinned(w, b1) && pinned(w, b2)) continue;
             if ((pinned(w, b1) && sleeping(w, b2)) || (pinned(w, b2) && sleeping(w, b1))) continue;
+            /* bodies of independent scenes (batched worlds) never collide */
+            if (w->desc[b1].scene != w->desc[b2].scene) continue;
             narrow_phase(w, b1, b2);
         }
     }
@@ -1442,12 +1449,32 @@ int32_t orc_pgs_solve(orc_world* w, const int32_t* order, int32_t n_order, doubl
         else compute_b(w, c, dt, fb, in_coll, P->restitution_override, P->restitution);
         compute_d(w, c, in_coll);
     }
+    /* Independent scenes in one list (batched worlds) stop one by one: each
+     * keeps its own max |dl| and leaves the sweeps once it is under the
+     * tolerance (PGS.java:190 per scene), which is the same as sweeping each
+     * scene alone in this order restricted to its contacts. */
+    int32_t n_sc = 1;
+    for (int32_t i = 0; i < w->n; i++)
+        if (w->desc[i].scene + 1 > n_sc) n_sc = w->desc[i].scene + 1;
+    int per_scene = main_solve && n_sc > 1 && P->tolerance > 0;
+    double* sc_max = NULL;
+    int32_t* sc_used = NULL;
+    if (per_scene) {
+        sc_max = calloc((size_t)n_sc, sizeof(double));
+        sc_used = calloc((size_t)n_sc, sizeof(int32_t));
+    }
+    w->n_scene_used = 0;
     int iter = iterations, done = 0;
     double maxabs = 0;
     while (iter > 0) {
         maxabs = 0;
+        if (per_scene)
+            for (int32_t s = 0; s < n_sc; s++) sc_max[s] = 0;
         for (int32_t k = 0; k < n_order; k++) {
             orc_contact* c = &w->c[order[k]];
+            int32_t sc = per_scene ? w->desc[c->b1].scene : 0;
+            if (per_scene && sc_used[sc]) continue;
+            double cmax = 0;
             int cl = clamps(w, c);
             double Jdv = get_jdv(w, c, in_coll, 0);
             double prev = c->lam[0];
@@ -1455,7 +1482,7 @@ int32_t orc_pgs_solve(orc_world* w, const int32_t* order, int32_t n_order, doubl
             if (cl) c->lam[0] = jmax(0, c->lam[0]);
             double diff = c->lam[0] - prev;
             update_dv(w, c, diff, 0, in_coll);
-            maxabs = jmax(maxabs, fabs(diff));
+            cmax = jmax(cmax, fabs(diff));
             double mu = contact_mu(w, c);
             for (int row = 1; row < 3; row++) {
                 Jdv = get_jdv(w, c, in_coll, row);
@@ -1468,17 +1495,48 @@ int32_t orc_pgs_solve(orc_world* w, const int32_t* order, int32_t n_order, doubl
                 }
                 diff = c->lam[row] - prev;
                 update_dv(w, c, diff, row, in_coll);
-                maxabs = jmax(maxabs, fabs(diff));
+                cmax = jmax(cmax, fabs(diff));
             }
+            maxabs = jmax(maxabs, cmax);
+            if (per_scene) sc_max[sc] = jmax(sc_max[sc], cmax);
             if (iter == 1) update_state(w, c, in_coll);
         }
         iter--;
         if (main_solve && changes) changes[done] = log(maxabs);
         done++;
-        if (main_solve && maxabs < P->tolerance) break;
+        if (per_scene) {
+            int all = 1;
+            for (int32_t s = 0; s < n_sc; s++) {
+                if (sc_used[s]) continue;
+                if (sc_max[s] < P->tolerance || iter == 0) {
+                    sc_used[s] = done;
+                    if (s < ORC_MAX_SCENES) w->scene_last[s] = sc_max[s];
+                } else {
+                    all = 0;
+                }
+            }
+            if (all) break;
+        } else if (main_solve && maxabs < P->tolerance) {
+            break;
+        }
+    }
+    if (per_scene) {
+        w->n_scene_used = n_sc < ORC_MAX_SCENES ? n_sc : ORC_MAX_SCENES;
+        for (int32_t s = 0; s < w->n_scene_used; s++) w->scene_used[s] = sc_used[s];
+        free(sc_max);
+        free(sc_used);
     }
     if (last_change) *last_change = maxabs;
     return done;
+}
+
+int32_t orc_scene_iterations(const orc_world* w, int32_t* used, double* last, int32_t capacity) {
+    int32_t n = w->n_scene_used < capacity ? w->n_scene_used : capacity;
+    for (int32_t s = 0; s < n; s++) {
+        if (used) used[s] = w->scene_used[s];
+        if (last) last[s] = w->scene_last[s];
+    }
+    return w->n_scene_used;
 }
 
 /* ------------------------------------------------------------------ */

@@ -106,6 +106,10 @@ int32_t orc_pgs_solve(orc_world* w, const int32_t* order, int32_t n_order, doubl
                       int32_t main_solve, int32_t post_stabilization, double* changes,
                       double* last_change);
 
+/* Sweeps used and the max |dl| of the last sweep per scene of the last main
+ * solve whose contacts spanned several scenes (returns the scene count). */
+int32_t orc_scene_iterations(const orc_world* w, int32_t* used, double* last, int32_t capacity);
+
 void orc_advance_velocities(orc_world* w, double dt);
 void orc_advance_positions(orc_world* w, double dt);
 void orc_advance_positions_post(orc_world* w, double dt);

@@ -386,6 +386,9 @@ class MergingOracle:
         rec["inertia0"] = C.I0
         rec["bbox"] = C.bb
         rec["friction"] = 0.8
+        # members share one scene (bodies of different scenes never touch)
+        mem = self.members.get(e, [])
+        rec["scene"] = int(self.desc(mem[0])["scene"]) if mem else 0
         self.o.set_entity(e, rec)
 
     def push_structure(self, st, jinv):

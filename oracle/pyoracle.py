@@ -78,6 +78,7 @@ def load(path: str | None = None) -> C.CDLL:
         "orc_motion_metric": (dbl, [vp, i32, i32]),
         "orc_motion_metric_pos": (dbl, [vp, i32, i32, dbl]),
         "orc_step": (i32, [vp, dbl, C.POINTER(i32), C.POINTER(dbl)]),
+        "orc_scene_iterations": (i32, [vp, vp, vp, i32]),
         "orc_sin": (dbl, [dbl]),
         "orc_cos": (dbl, [dbl]),
     }
@@ -260,6 +261,15 @@ class Oracle:
         used = self.lib.orc_pgs_solve(self.h, ptr(order), len(order), dt, int(main), int(post), ptr(ch),
                                       C.byref(last))
         return used, last.value, ch[:used] if main else ch[:0]
+
+    def scene_iterations(self):
+        """(sweeps used, max |dl| of the last sweep) per scene of the last
+        main solve that stopped scenes one by one."""
+        n = self.lib.orc_scene_iterations(self.h, None, None, 0)
+        used = np.zeros(n, np.int32)
+        last = np.zeros(n)
+        self.lib.orc_scene_iterations(self.h, ptr(used), ptr(last), n)
+        return used, last
 
     def box_box(self, b1, b2):
         return self.lib.orc_box_box(self.h, b1, b2)

@@ -1,41 +1,51 @@
 """Many independent scenes in one context (the batched workload): every
 scene's contacts, lambdas and states equal those of the oracle running that
-scene alone, sweeping in the GPU's colour order restricted to the scene."""
+scene alone, sweeping in the GPU's colour order restricted to the scene.
+With a tolerance each scene stops on its own max |delta lambda|, as it would
+alone; with merging on, the batch matches the merging oracle run on the same
+batch phase by phase."""
 import numpy as np
 import pytest
 
 from adaptivemerging_amd import workloads
 from adaptivemerging_amd.context import Context
+from oracle.merging import MergingOracle
 from oracle.pyoracle import Oracle
 from tests.helpers import assert_bits_equal, assert_contacts_equal
+from tests.test_gpu_merging import compare_world, step_both
 
 pytestmark = pytest.mark.gpu
 
-# A batched solve stops when every scene has converged, a scene solved alone
-# when it has: a fixed sweep count (tolerance 0) makes the two comparable.
-PARAMS = dict(iterations=40, tolerance=0.0, gravity_amount=9.81, enable_merging=0, enable_unmerging=0)
+PARAMS = dict(iterations=40, gravity_amount=9.81, enable_merging=0, enable_unmerging=0)
 
 
-def test_batched_scenes_equal_separate_runs(gpu_lib):
+@pytest.mark.parametrize("tolerance,schedule", [(0.0, 0), (1e-4, 0), (1e-4, 1), (1e-6, 0)])
+def test_batched_scenes_equal_separate_runs(gpu_lib, tolerance, schedule):
+    """tolerance 0: every scene runs all sweeps; above 0 every scene stops at
+    its own first sweep under the tolerance (checked against the separate
+    oracles' sweep counts) while the others go on."""
     subs = [workloads.jenga_tower(seed=1, layers=5, overlap=0.001),
             workloads.box_cluster(seed=2, n=27, spheres=4),
             workloads.merge_stack(seed=5),
             workloads.box_cluster(seed=6, n=64, spheres=10)]
     scene = workloads.batch(subs)
+    params = dict(PARAMS, tolerance=tolerance, pgs_schedule=schedule)
     ctx = Context()
     ctx.upload(scene)
-    ctx.set_params(**PARAMS)
+    ctx.set_params(**params)
     orcs = []
     for sub in subs:
         o = Oracle(sub)
-        o.set_params(**PARAMS)
+        o.set_params(**params)
         orcs.append(o)
     offs = np.cumsum([0] + [s.n for s in subs])
     dt = 0.01
+    distinct = set()
     for step in range(4):
         ctx.clear_bodies(); ctx.apply_gravity(); ctx.update_contacts_map(); ctx.detect(dt)
         ctx.warm_start(); ctx.redo_warm_start()
-        ctx.solve(dt)
+        it_g, _ = ctx.solve(dt)
+        used_g = ctx.scene_iterations()[0] if tolerance > 0 else None
         order = ctx.solve_order()
         g = ctx.contacts()
         ctx.advance_velocities(dt); ctx.advance_positions(dt)
@@ -47,7 +57,12 @@ def test_batched_scenes_equal_separate_runs(gpu_lib):
             mine = np.nonzero(sc == k)[0]
             local = -np.ones(len(sc), dtype=np.int64)
             local[mine] = np.arange(len(mine))
-            o.solve(np.array([local[c] for c in order if sc[c] == k], dtype=np.int32), dt)
+            it_o, _, _ = o.solve(np.array([local[c] for c in order if sc[c] == k], dtype=np.int32), dt)
+            if used_g is not None:
+                assert used_g[k] == it_o, f"step {step} scene {k}: {used_g[k]} sweeps on the GPU, {it_o} alone"
+                distinct.add(int(it_o))
+            else:
+                assert it_o == it_g == params["iterations"]
             gk = {key: val[mine] for key, val in g.items()}
             for key in ("body1", "body2"):
                 gk[key] = gk[key] - offs[k]
@@ -60,3 +75,36 @@ def test_batched_scenes_equal_separate_runs(gpu_lib):
             sl = slice(offs[k], offs[k + 1])
             for name, a, b in (("x", x[sl], xo), ("R", R[sl], Ro), ("v", v[sl], vo), ("w", w[sl], wo)):
                 assert_bits_equal(f"step {step} scene {k} {name}", a, b)
+    if tolerance > 0:
+        # the scenes stopped at different sweeps
+        assert len(distinct) >= 2, distinct
+        assert it_g == max(used_g)
+
+
+@pytest.mark.parametrize("schedule", [0, 1])
+def test_batched_merging_matches_oracle(gpu_lib, schedule):
+    """The batched configuration at a small size: six piles of 12 to 40 boxes
+    and spheres, merging on at the reference defaults, 50 sweeps at the
+    default tolerance 1e-5 (each scene stops on its own), against the merging
+    oracle on the same batch, phase by phase and bit for bit for 40 steps."""
+    scene = workloads.batched_piles(seed=3, n_scenes=6, bodies=(12, 40))
+    ctx = Context()
+    ctx.upload(scene)
+    ctx.set_params(gravity_amount=9.81, iterations=50, tolerance=1e-5, enable_sleeping=0, pgs_schedule=schedule)
+    mo = MergingOracle(scene, type(ctx.params)())
+    for k, _ in type(ctx.params)._fields_:
+        setattr(mo.params, k, getattr(ctx.params, k))
+    mo.set_params()
+    dt = 0.01
+    merges = 0
+    sweeps = set()
+    for step in range(1, 41):
+        m, _ = step_both(ctx, mo, dt, step)
+        merges += m
+        compare_world(ctx, mo, step)
+        used_g, last_g = ctx.scene_iterations()
+        used_o, last_o = mo.o.scene_iterations()
+        assert list(used_g) == list(used_o[:len(used_g)]), f"step {step}: per-scene sweeps"
+        assert_bits_equal(f"step {step} per-scene residual", last_g, last_o[:len(last_g)])
+        sweeps.update(int(u) for u in used_g)
+    assert merges >= 1 and len(sweeps) >= 2
