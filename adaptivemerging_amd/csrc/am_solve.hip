@@ -1390,6 +1390,15 @@ AMD_INLINE void take_old(ContactsView& C, int c, OldContactsView& O, int o, int 
     C.prev_viol[c] = post ? O.prev_viol[o] : O.viol[o];
 }
 
+// Whether a contact side (body, bv id) is a composite part that is not a box
+// (Contact.csb1 / csb2 set and not RigidBodyGeomBox); AM_BV_SUB carries the part.
+AMD_INLINE bool non_box_part(const BodiesView& B, int body, int bv) {
+    const am_body_desc& d = B.desc[body];
+    if (d.geom != AM_GEOM_COMPOSITE || bv > -3) return false;
+    const int s = (-3 - bv) >> 16;
+    return B.sub_desc[d.tree_first + s].geom != AM_GEOM_BOX;
+}
+
 AMD_INLINE d3 world_point(const BodiesView& B, int body, d3 pB1) {
     return xform(ldm(&B.R[body * 9]), ld3(&B.x[body * 3]), pB1);
 }
@@ -1404,14 +1413,22 @@ __global__ void k_warm_start(BodiesView B, ContactsView C, OldContactsView O, co
     int bad = 0, rep = 0;
     for (int c = group_start[g]; c < group_start[g + 1]; c++) {
         int b1 = C.b1[c], b2 = C.b2[c];
-        bool boxbox = B.desc[b1].geom == AM_GEOM_BOX && B.desc[b2].geom == AM_GEOM_BOX;
+        // box or composite on both sides takes the box-box path (CollisionProcessor.java:489-493)
+        const int g1 = B.desc[b1].geom, g2 = B.desc[b2].geom;
+        const bool boxish = (g1 == AM_GEOM_BOX || g1 == AM_GEOM_COMPOSITE) && (g2 == AM_GEOM_BOX || g2 == AM_GEOM_COMPOSITE);
         CKey key = make_key(b1, b2, C.bv1[c], C.bv2[c], C.info[c]);
-        if (!boxbox) {
+        auto vanilla = [&]() {
             int o = old_lookup(O, key);
             if (o >= 0) take_old(C, c, O, o, post, false);
             else C.is_new[c] = 1;
+        };
+        // a part that is not a box: body 1's takes the vanilla start only
+        // (:497-500), body 2's the vanilla start and then the box-box path (:501-503)
+        if (!boxish || non_box_part(B, b1, C.bv1[c])) {
+            vanilla();
             continue;
         }
+        if (non_box_part(B, b2, C.bv2[c])) vanilla();
         d3 pNew = world_point(B, b1, ld3(&C.pB1[c * 3]));
         int o = old_lookup(O, key);
         if (o >= 0) {

@@ -1255,12 +1255,26 @@ static void contact_world_point(const orc_world* w, const orc_contact* c, double
 /* warmStart (CollisionProcessor.java:481-640).  Body pair contacts are the
  * runs of contacts sharing an unordered body pair; within a pair contacts are
  * visited in list order, which fixes the "take and zero" outcome. */
+/* Whether side `bv` of a contact on `body` is a composite part that is not a
+ * box (Contact.csb1 / csb2 non-null and not RigidBodyGeomBox). */
+static int non_box_part(const orc_world* w, int32_t body, int32_t bv) {
+    if (w->desc[body].geom != AM_GEOM_COMPOSITE || bv > -3) return 0;
+    int32_t s = (-3 - bv) >> 16;
+    return w->subs[w->desc[body].tree_first + s].geom != AM_GEOM_BOX;
+}
+
 void orc_warm_start(orc_world* w, int32_t post, int32_t* bad_out, int32_t* rep_out) {
     int32_t bad = 0, rep = 0;
     for (int32_t k = 0; k < w->nc; k++) {
         orc_contact* c = &w->c[k];
-        int b1box = w->desc[c->b1].geom == AM_GEOM_BOX, b2box = w->desc[c->b2].geom == AM_GEOM_BOX;
-        if (!(b1box && b2box)) { vanilla_warm_start(w, c, post); continue; }
+        /* box or composite on both sides takes the box-box path (:489-493) */
+        int g1 = w->desc[c->b1].geom, g2 = w->desc[c->b2].geom;
+        int s1 = g1 == AM_GEOM_BOX || g1 == AM_GEOM_COMPOSITE, s2 = g2 == AM_GEOM_BOX || g2 == AM_GEOM_COMPOSITE;
+        if (!(s1 && s2)) { vanilla_warm_start(w, c, post); continue; }
+        /* a part of body 1 that is not a box: vanilla, next contact (:497-500);
+         * of body 2: vanilla, then on through the box-box path (:501-503) */
+        if (non_box_part(w, c->b1, c->bv1)) { vanilla_warm_start(w, c, post); continue; }
+        if (non_box_part(w, c->b2, c->bv2)) vanilla_warm_start(w, c, post);
         double pNew[3], pOld[3];
         contact_world_point(w, c, pNew);
         orc_contact* o = old_lookup(w, make_key(c->b1, c->b2, c->bv1, c->bv2, c->info));

@@ -146,3 +146,57 @@ def test_composite_against_composite_order():
     order = [((-3 - int(b1)) >> 16, (-3 - int(b2)) >> 16) for b1, b2 in zip(c["bv1"], c["bv2"])]
     assert order == sorted(order)
     assert all(int(b) == 0 for b in c["body1"]) and all(int(b) == 1 for b in c["body2"])
+
+
+def _warm_pair(comp_first: bool):
+    """A box and a composite (part 0 a box, part 1 a sphere) with three old
+    contacts and three new ones: two on the box part whose info numbers swap
+    between the steps (their points do not move), one on the sphere part
+    whose old point lies 0.2 m from the new one."""
+    s = Scene()
+    s.add_box([2.0, 0.4, 2.0], [0.0, 0.0, 0.0])
+    s.add_composite([dict(kind="box", size=[0.4, 0.2, 0.4], x=[0.0, 0.0, 0.0]),
+                     dict(kind="sphere", r=0.1, x=[0.6, 0.0, 0.0])], x=[0.0, 0.3, 0.0])
+    box, comp = 0, 1
+    o = Oracle(s)
+    b1, b2 = (comp, box) if comp_first else (box, comp)
+    part = lambda q: AM_BV_SUB(q, AM_BV_NONE if q == 0 else 0)
+    bvs = [part(0), part(0), part(1)]
+    bv1 = [bv if comp_first else AM_BV_NONE for bv in bvs]
+    bv2 = [AM_BV_NONE if comp_first else bv for bv in bvs]
+    n = np.tile([0.0, -1.0 if comp_first else 1.0, 0.0], (3, 1))
+    old_p = np.array([[-0.2, 0.2, -0.2], [0.2, 0.2, 0.2], [0.5, 0.2, 0.0]])
+    o.push_contacts([b1] * 3, [b2] * 3, old_p, n, [-1e-3] * 3, info=[0, 1, 0], bv1=bv1, bv2=bv2)
+    o.push_lambdas(np.array([[1.0, 0.1, 0.0], [2.0, 0.0, 0.2], [3.0, 0.0, 0.0]]))
+    o.update_contacts_map()
+    new_p = np.array([[0.2, 0.2, 0.2], [-0.2, 0.2, -0.2], [0.7, 0.2, 0.0]])
+    o.push_contacts([b1] * 3, [b2] * 3, new_p, n, [-1e-3] * 3, info=[0, 1, 0], bv1=bv1, bv2=bv2)
+    bad, rep = o.warm_start()
+    return o.contacts(), bad, rep
+
+
+def test_composite_box_part_warm_start_repairs_info_swap():
+    """CollisionProcessor.warmStart (:489-503): a box-or-composite pair takes
+    the box-box repair on a box part (the info numbers of the two box-part
+    contacts swapped, each takes the lambda of the old contact at its own
+    point); a sphere part of body 1 takes the plain hash match whatever its
+    distance."""
+    c, bad, rep = _warm_pair(comp_first=True)
+    np.testing.assert_array_equal(c["lambda"][0], [2.0, 0.0, 0.2])
+    np.testing.assert_array_equal(c["lambda"][1], [1.0, 0.1, 0.0])
+    np.testing.assert_array_equal(c["lambda"][2], [3.0, 0.0, 0.0])
+    assert list(c["new_this_step"]) == [0, 0, 0]
+    assert (bad, rep) == (2, 2)
+
+
+def test_composite_sphere_part_of_body2_falls_through():
+    """A sphere part of body 2 takes the plain hash match and then goes on
+    through the box-box test (:501-503): its old point is 0.2 m away and no
+    other info matches better, so it keeps the plain match's lambda but is
+    marked new."""
+    c, bad, rep = _warm_pair(comp_first=False)
+    np.testing.assert_array_equal(c["lambda"][0], [2.0, 0.0, 0.2])
+    np.testing.assert_array_equal(c["lambda"][1], [1.0, 0.1, 0.0])
+    np.testing.assert_array_equal(c["lambda"][2], [3.0, 0.0, 0.0])
+    assert list(c["new_this_step"]) == [0, 0, 1]
+    assert bad == 3
