@@ -717,8 +717,10 @@ void pgs_shape(am_ctx* c, const am_params& P, int* grid, int* block) {
             c->flow_max_grid = pgs_flow_max_blocks(&ok);
             if (!ok) c->flow_max_grid = 1;
         }
-        // one wave per workgroup, each walking every W-th step of up to 4 groups
-        int g = P.pgs_grid > 0 ? P.pgs_grid : std::min((c->n_groups + 3) / 4, 512);
+        // one wave per workgroup, each walking every W-th step of up to 4
+        // groups: 512 waves up to ~20k groups, then one per 64 groups (the
+        // settled 20k-box pile, ~78k groups, runs fastest at 1024 or more)
+        int g = P.pgs_grid > 0 ? P.pgs_grid : std::max(std::min((c->n_groups + 3) / 4, 512), c->n_groups / 64);
         *grid = std::max(1, std::min(g, c->flow_max_grid));
         *block = 64;
         return;
@@ -1719,7 +1721,7 @@ int am_autotune(am_ctx* c, double dt, int32_t sweeps, int32_t reps, int32_t appl
             if (k.sched == sched && k.grid_used == g && k.block_used == b) return;
         cands.push_back({sched, grid, block, g, b, 0.0});
     };
-    for (int g : {0, 64, 128, 192, 256, 384, 512, 768, 1024}) add(0, g, 0);
+    for (int g : {0, 64, 128, 192, 256, 384, 512, 768, 1024, 1280, 1536, 2048}) add(0, g, 0);
     for (int b : {0, 64, 128})
         for (int g : {0, 32, 64, 128, 256, 512}) add(1, g, b);
 
