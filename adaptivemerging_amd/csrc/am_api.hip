@@ -201,6 +201,7 @@ void default_params(am_params* p) {
     p->pgs_grid = 0;
     p->pgs_plain_launch = 0;
     p->pgs_schedule = 0;
+    p->organize_contacts = 1;
 }
 
 // Derived host lists from the structure (parent/slot/sub/members).
@@ -416,7 +417,7 @@ int build_layout(am_ctx* c) {
         for (int round = 0; round < 4096; round++) {
             fill_int(s, sc + 5, 1, 0);
             launch_colour_round(s, ng, c->g_a.p, c->g_b.p, c->d_flags.p, c->inc_start.p, c->inc_count.p,
-                                c->inc_items.p, cin, cout, sc + 5);
+                                c->inc_items.p, cin, cout, sc + 5, c->layout_layer);
             std::swap(cin, cout);
             int r = read_scalars(c, 5, 1);
             if (r) return r;

@@ -1100,6 +1100,104 @@ int merge_member_transforms(am_ctx* c) {
 // advance their velocities and the top-level deltaV is cleared.  The sweep
 // runs in colour order over physical body pairs (am_solve_order_pull gives
 // the order right after this call).
+// ---------------------------------------------------------------------------
+// CollisionProcessor.getOrganizedContacts / getNextLayer (:346-441): the
+// single sweep visits the body pairs breadth first from those holding a
+// contact new this step: layer 0 those pairs, layer k+1 the pairs not yet
+// reached that share a body (pinned ones included) with a pair of layer k,
+// then the external pairs never reached, then the internal ones.  Groups
+// (contiguous same-pair runs of the combined list, external first) stand for
+// the pairs; the layers order the colouring of the single sweep.
+__global__ void k_bfs_incidence(int n_groups, const int* ga, const int* gb, int* count, const int* start,
+                                int* cursor, int* items) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_groups) return;
+    const int e2[2] = {ga[g], gb[g]};
+    for (int s = 0; s < 2; s++) {
+        if (s == 1 && e2[1] == e2[0]) break;
+        if (!start) atomicAdd(&count[e2[s]], 1);
+        else items[start[e2[s]] + atomicAdd(&cursor[e2[s]], 1)] = g;
+    }
+}
+__global__ void k_bfs_seed(int n_groups, const int* group_start, const int* is_new, int n_ext, int* layer,
+                           int* n_seeded) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_groups) return;
+    int l = -1;
+    if (group_start[g] < n_ext)
+        for (int k = group_start[g]; k < group_start[g + 1]; k++)
+            if (is_new[k]) {
+                l = 0;
+                break;
+            }
+    layer[g] = l;
+    if (l == 0) atomicAdd(n_seeded, 1);
+}
+// one breadth-first layer: a group not yet reached joins layer k + 1 when a
+// pair of layer k shares one of its bodies (a neighbour set this round reads
+// k + 1, never k, so the result does not depend on scheduling)
+__global__ void k_bfs_round(int n_groups, const int* ga, const int* gb, const int* start, const int* count,
+                            const int* items, int* layer, int k, int* changed) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_groups || layer[g] >= 0) return;
+    const int e2[2] = {ga[g], gb[g]};
+    for (int s = 0; s < 2; s++) {
+        const int e = e2[s];
+        for (int q = start[e]; q < start[e] + count[e]; q++)
+            if (layer[items[q]] == k) {
+                layer[g] = k + 1;
+                atomicAdd(changed, 1);
+                return;
+            }
+    }
+}
+__global__ void k_bfs_rest(int n_groups, const int* group_start, int n_ext, int* layer, int lmax) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_groups || layer[g] >= 0) return;
+    layer[g] = group_start[g] < n_ext ? lmax + 1 : lmax + 2;
+}
+
+// Layers of the groups of the combined list (external contacts [0, n_ext)).
+static int organized_layers(am_ctx* c, int n_ext) {
+    hipStream_t s = c->stream;
+    const int ng = c->n_groups, ne = c->n_ent;
+    AM_HIP(c, c->g_layer.ensure(std::max(ng, 1)));
+    AM_HIP(c, c->bfs_count.ensure(ne + 1));
+    AM_HIP(c, c->bfs_start.ensure(ne + 1));
+    AM_HIP(c, c->bfs_cursor.ensure(ne + 1));
+    AM_HIP(c, c->bfs_items.ensure(2 * (size_t)std::max(ng, 1)));
+    int* sc = c->d_scalars.p;
+    fill_int(s, c->bfs_count.p, ne + 1, 0);
+    fill_int(s, c->bfs_cursor.p, ne + 1, 0);
+    dim3 grid((ng + 255) / 256), block(256);
+    hipLaunchKernelGGL(k_bfs_incidence, grid, block, 0, s, ng, c->g_a.p, c->g_b.p, c->bfs_count.p, nullptr, nullptr,
+                       nullptr);
+    exclusive_scan(s, c->bfs_count.p, c->bfs_start.p, ne + 1, sc + 3, c->d_scan_temp.p);
+    hipLaunchKernelGGL(k_bfs_incidence, grid, block, 0, s, ng, c->g_a.p, c->g_b.p, c->bfs_count.p, c->bfs_start.p,
+                       c->bfs_cursor.p, c->bfs_items.p);
+    fill_int(s, sc + 12, 1, 0);
+    hipLaunchKernelGGL(k_bfs_seed, grid, block, 0, s, ng, c->g_start.p, c->c_new.p, n_ext, c->g_layer.p, sc + 12);
+    AM_HIP(c, hipGetLastError());
+    int r = read_scalars(c, 12, 1);
+    if (r) return r;
+    int lmax = -1;
+    if (c->h_scalars[12] > 0) {
+        lmax = 0;
+        for (int k = 0; k < ng; k++) {
+            fill_int(s, sc + 12, 1, 0);
+            hipLaunchKernelGGL(k_bfs_round, grid, block, 0, s, ng, c->g_a.p, c->g_b.p, c->bfs_start.p,
+                               c->bfs_count.p, c->bfs_items.p, c->g_layer.p, k, sc + 12);
+            AM_HIP(c, hipGetLastError());
+            if ((r = read_scalars(c, 12, 1))) return r;
+            if (c->h_scalars[12] == 0) break;
+            lmax = k + 1;
+        }
+    }
+    hipLaunchKernelGGL(k_bfs_rest, grid, block, 0, s, ng, c->g_start.p, n_ext, c->g_layer.p, lmax);
+    AM_HIP(c, hipGetLastError());
+    return AM_OK;
+}
+
 int merge_update_in_collections(am_ctx* c, double dt, int* iters) {
     *iters = 0;
     const am_params& P = c->params;
@@ -1128,7 +1226,14 @@ int merge_update_in_collections(am_ctx* c, double dt, int* iters) {
         if ((r = read_scalars(c, 2, 1))) return r;
         c->n_groups = c->h_scalars[2];
         c->layout_valid = false;
-        if ((r = do_solve(c, dt, 0, false, iters, nullptr))) return r;
+        // Merging.organizeContacts: the breadth-first order of getOrganizedContacts
+        if (P.organize_contacts) {
+            if ((r = organized_layers(c, n_ext))) return r;
+            c->layout_layer = c->g_layer.p;
+        }
+        r = do_solve(c, dt, 0, false, iters, nullptr);
+        c->layout_layer = nullptr;
+        if (r) return r;
         if (all_awake) launch_gather_contacts(s, internal_view(c), 0, contacts_view(c), nullptr, n_ext, n_int, 0);
         else launch_gather_contacts(s, internal_view(c), 0, contacts_view(c), d_awake.p, n_ext, n_int, 3);
         if (!all_awake) AM_HIP(c, hipStreamSynchronize(s));  // d_awake is freed on return

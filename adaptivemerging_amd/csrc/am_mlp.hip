@@ -4,13 +4,18 @@
 // start-quality residual that decides between the learned and the plain start.
 //
 // Feature vector (24 FP32, dimensionless), per contact with solve bodies a, b:
-//   0-2 normal, 3-5 tangent 1, 6-8 tangent 2 (world),
+//   0-2 normal, 3-5 tangent 1 (world; tangent 2 = n x t1 adds nothing),
+//   6 1 when the plain warm start found no previous contact (new this step),
+//   7-8 0,
 //   9-11 constraint-space velocity J (v_a, w_a, v_b, w_b) / (g dt),
 //   12 violation / (g dt^2),
 //   13 minv_a / (minv_a + minv_b), 14 log10((minv_a + minv_b) * 1 kg) / 3,
 //   15-17 diag(jinv_a) / trace(jinv_a), 18-20 same for b (0 when pinned),
-//   21-23 previous-frame lambda / S,
-// with S = g dt / D_nn (D_nn = J_n M^-1 J_n^T); the network predicts lambda / S.
+//   21-23 slog(previous-frame lambda / S),
+// with S = g dt / D_nn (D_nn = J_n M^-1 J_n^T) and slog(x) = sign(x) ln(1 + |x|),
+// which keeps the impulses of a 100:1 mass-ratio pile in a range BF16 holds.
+// The network predicts the change of slog(lambda / S) from the plain start,
+// bounded by kMaxLogStep.
 #include "am_mlp.hpp"
 
 namespace am {
@@ -20,6 +25,10 @@ typedef float float4v __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
 // features
+AMD_INLINE double slog(double x) { return x < 0 ? -log1p(-x) : log1p(x); }
+AMD_INLINE double sexp(double z) { return z < 0 ? -expm1(-z) : expm1(z); }
+constexpr double kMaxLogStep = 2.0;  // largest change of slog(lambda / S) a prediction makes
+
 __global__ void k_features(FeatureArgs F) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= F.n) return;
@@ -31,9 +40,12 @@ __global__ void k_features(FeatureArgs F) {
     const double* J = &C.J[(size_t)k * 36];
     float* f = &F.out[(size_t)k * kMlpIn];
     double gdt = F.g * F.dt;
-    // frame (jb.v of each row)
-    for (int r = 0; r < 3; r++)
+    // frame (jb.v of the normal and first tangent rows)
+    for (int r = 0; r < 2; r++)
         for (int i = 0; i < 3; i++) f[r * 3 + i] = (float)J[r * 12 + 6 + i];
+    f[6] = C.is_new[k] ? 1.f : 0.f;
+    f[7] = 0.f;
+    f[8] = 0.f;
     d3 va = ld3(&B.v[a * 3]), wa = ld3(&B.w[a * 3]), vb = ld3(&B.v[b * 3]), wb = ld3(&B.w[b * 3]);
     for (int r = 0; r < 3; r++) {
         const double* row = J + r * 12;
@@ -57,7 +69,7 @@ __global__ void k_features(FeatureArgs F) {
     double D = ma * dot(ld3(J), ld3(J)) + dot(ld3(J + 3), t1) + mb * dot(ld3(J + 6), ld3(J + 6)) + dot(ld3(J + 9), t2);
     double S = D > 0 ? gdt / D : 1.0;
     F.scale[k] = S;
-    for (int r = 0; r < 3; r++) f[21 + r] = (float)(C.lam[k * 3 + r] / S);
+    for (int r = 0; r < 3; r++) f[21 + r] = (float)slog(C.lam[k * 3 + r] / S);
 }
 
 void launch_features(hipStream_t s, const FeatureArgs& F) {
@@ -355,16 +367,21 @@ void launch_to_bf16(hipStream_t s, const float* p, unsigned short* bf, int n) {
 }
 
 // ---------------------------------------------------------------------------
-// Predicted start: the network predicts the change from the plain warm
-// start, lambda = lambda_plain + S * y, projected onto the friction box
+// Predicted start: the network predicts the change y of slog(lambda / S)
+// from the plain warm start, clamped to +-kMaxLogStep, so lambda =
+// S sexp(slog(lambda_plain / S) + y), projected onto the friction box
 // (lambda_n >= 0, |lambda_t| <= mu lambda_n).  A network that outputs zero
-// reproduces the plain start.
+// reproduces the plain start (up to the rounding of slog and sexp).
+AMD_INLINE double predicted(double lam_plain, double S, float y) {
+    const double z = slog(lam_plain / S) + fmin(fmax((double)y, -kMaxLogStep), kMaxLogStep);
+    return S * sexp(z);
+}
 __global__ void k_apply_prediction(ContactsView C, const BodiesView B, const float* Y, const double* scale, int n,
                                    int fric_ovr, double fric_val, double* lam_out) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     double S = scale[k];
-    double ln = fmax(0.0, C.lam[k * 3 + 0] + (double)Y[k * 3 + 0] * S);
+    double ln = fmax(0.0, predicted(C.lam[k * 3 + 0], S, Y[k * 3 + 0]));
     double mu;
     if (fric_ovr) {
         mu = fric_val;
@@ -375,8 +392,8 @@ __global__ void k_apply_prediction(ContactsView C, const BodiesView B, const flo
         else mu = (f1 + f2) / 2.;
     }
     double lim = mu * ln;
-    double l1 = fmin(fmax(C.lam[k * 3 + 1] + (double)Y[k * 3 + 1] * S, -lim), lim);
-    double l2 = fmin(fmax(C.lam[k * 3 + 2] + (double)Y[k * 3 + 2] * S, -lim), lim);
+    double l1 = fmin(fmax(predicted(C.lam[k * 3 + 1], S, Y[k * 3 + 1]), -lim), lim);
+    double l2 = fmin(fmax(predicted(C.lam[k * 3 + 2], S, Y[k * 3 + 2]), -lim), lim);
     lam_out[k * 3 + 0] = ln;
     lam_out[k * 3 + 1] = l1;
     lam_out[k * 3 + 2] = l2;
@@ -389,13 +406,16 @@ void launch_apply_prediction(hipStream_t s, ContactsView C, BodiesView B, const 
                            fric_val, lam_out);
 }
 
-// Targets: the change from the plain start to the converged solve, over S
-// (features 21..23 hold the plain start over S).
+// Targets: the change of slog(lambda / S) from the plain start to the
+// converged solve (features 21..23 hold the plain start's), clamped as the
+// prediction is.
 __global__ void k_targets(const double* lam, const double* scale, const float* X, int n, float* T) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    for (int r = 0; r < 3; r++)
-        T[k * 3 + r] = (float)(lam[k * 3 + r] / scale[k]) - X[(size_t)k * kMlpIn + 21 + r];
+    for (int r = 0; r < 3; r++) {
+        const double d = slog(lam[k * 3 + r] / scale[k]) - (double)X[(size_t)k * kMlpIn + 21 + r];
+        T[k * 3 + r] = (float)fmin(fmax(d, -kMaxLogStep), kMaxLogStep);
+    }
 }
 void launch_targets(hipStream_t s, const double* lam, const double* scale, const float* X, int n, float* T) {
     if (n > 0) hipLaunchKernelGGL(k_targets, dim3((n + 255) / 256), dim3(256), 0, s, lam, scale, X, n, T);

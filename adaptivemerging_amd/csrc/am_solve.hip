@@ -116,9 +116,15 @@ AMD_INLINE bool higher(uint32_t h, uint32_t g) {
 // among its uncoloured neighbours takes the smallest colour unused by its
 // coloured neighbours (colours read from the previous round's array, so the
 // result does not depend on scheduling).
+//
+// With layers (the single sweep's breadth-first layers), a group also waits
+// for its neighbours of lower layers and takes a colour above all of theirs,
+// so along every body the sweep visits the groups layer by layer; only
+// neighbours of its own layer compete for colours.  Colours then may pass
+// the parallel ones: those groups run in the sequential tail in colour order.
 __global__ void k_colour_round(int n_groups, const int* ga, const int* gb, const int* bflags, const int* inc_start,
                                const int* inc_count, const int* inc_items, const int* col_in, int* col_out,
-                               int* remaining) {
+                               int* remaining, const int* layer) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n_groups) return;
     int c = col_in[g];
@@ -126,24 +132,43 @@ __global__ void k_colour_round(int n_groups, const int* ga, const int* gb, const
         col_out[g] = c;
         return;
     }
-    uint64_t used = 0;
+    const int lg = layer ? layer[g] : 0;
+    int lo = 0;  // the colour must be at least lo (above every lower-layer neighbour)
     bool ready = true;
     const int bodies[2] = {ga[g], gb[g]};
+    if (layer) {
+        for (int s = 0; s < 2 && ready; s++) {
+            int e = bodies[s];
+            if (!dynamic_body(bflags, e)) continue;
+            int st = inc_start[e], cnt = inc_count[e];
+            for (int q = 0; q < cnt; q++) {
+                int h = inc_items[st + q];
+                if (h == g || layer[h] >= lg) continue;
+                int ch = col_in[h];
+                if (ch < 0) {
+                    ready = false;
+                    break;
+                }
+                lo = max(lo, ch + 1);
+            }
+        }
+    }
+    uint64_t used = 0;  // colours lo .. lo + 63 taken by same-layer neighbours
     for (int s = 0; s < 2 && ready; s++) {
         int e = bodies[s];
         if (!dynamic_body(bflags, e)) continue;
         int st = inc_start[e], cnt = inc_count[e];
         for (int q = 0; q < cnt; q++) {
             int h = inc_items[st + q];
-            if (h == g) continue;
+            if (h == g || (layer && layer[h] != lg)) continue;
             int ch = col_in[h];
             if (ch < 0) {
                 if (higher((uint32_t)h, (uint32_t)g)) {
                     ready = false;
                     break;
                 }
-            } else if (ch < kMaxColours) {
-                used |= (uint64_t)1 << ch;
+            } else if (ch >= lo && ch - lo < 64) {
+                used |= (uint64_t)1 << (ch - lo);
             }
         }
     }
@@ -152,8 +177,13 @@ __global__ void k_colour_round(int n_groups, const int* ga, const int* gb, const
         atomicAdd(remaining, 1);
         return;
     }
-    int pick = kMaxColours;  // sequential tail if all 64 parallel colours are taken
-    if (~used != 0) pick = __ffsll((long long)~used) - 1;
+    int pick;
+    if (!layer) {
+        pick = kMaxColours;  // sequential tail if all 64 parallel colours are taken
+        if (~used != 0) pick = __ffsll((long long)~used) - 1;
+    } else {
+        pick = lo + (~used != 0 ? __ffsll((long long)~used) - 1 : 64);
+    }
     col_out[g] = pick;
 }
 
@@ -170,10 +200,16 @@ __global__ void k_colour_keys(int n_groups, const int* colour, const int* group_
     keys[g] = ((uint64_t)(uint32_t)c << 32) | (rank << 28) | (uint32_t)g;
 }
 
-// colour_start[c] = first sorted position with colour >= c, c = 0..kMaxColours+1
+// colour_start[c] = first sorted position with colour >= c, c = 0..kMaxColours;
+// colour_start[kMaxColours + 1] = n_groups (the tail holds every colour from
+// kMaxColours on, in colour order)
 __global__ void k_colour_starts(int n_groups, const uint64_t* sorted, int* colour_start) {
     int c = threadIdx.x;
     if (c > kMaxColours + 1) return;
+    if (c == kMaxColours + 1) {
+        colour_start[c] = n_groups;
+        return;
+    }
     int lo = 0, hi = n_groups;
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
@@ -1900,10 +1936,10 @@ void launch_incidence(hipStream_t s, bool fill, int n_groups, const int* ga, con
 }
 void launch_colour_round(hipStream_t s, int n_groups, const int* ga, const int* gb, const int* bflags,
                          const int* inc_start, const int* inc_count, const int* inc_items, const int* col_in,
-                         int* col_out, int* remaining) {
+                         int* col_out, int* remaining, const int* layer) {
     if (n_groups <= 0) return;
     hipLaunchKernelGGL(k_colour_round, dim3((n_groups + 255) / 256), dim3(256), 0, s, n_groups, ga, gb, bflags,
-                       inc_start, inc_count, inc_items, col_in, col_out, remaining);
+                       inc_start, inc_count, inc_items, col_in, col_out, remaining, layer);
 }
 void launch_colour_keys(hipStream_t s, int n_groups, const int* colour, const int* group_start, uint64_t* keys) {
     if (n_groups > 0)

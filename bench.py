@@ -196,12 +196,21 @@ def settle_and_warm(ctx, args, params_kw, label):
     return st
 
 
-def train_learned_start(args, params_kw):
-    """Trains the warm-start network on held-out small piles (other seeds,
-    a few hundred boxes, a wide bin) whose solves reach the tolerance: each
-    converged main solve adds (features, converged lambda) pairs.  Returns
-    (flat parameters, info dict)."""
+def small_pile(args, k: int):
+    """Scene k of the small piles the learned start is trained (k >= 0) and
+    evaluated (k < 0) on: the configuration's boxes and densities, few
+    enough in a wide bin to form a shallow pile whose solves reach the
+    tolerance."""
     from adaptivemerging_amd import workloads
+
+    return workloads.box_pile(seed=1000 + args.seed * 97 + k, n=args.train_bodies, bin_size=args.train_bin)
+
+
+def train_learned_start(args, params_kw):
+    """Trains the warm-start network on small piles (other seeds than the
+    measured pile) whose solves reach the tolerance: each converged main
+    solve adds (features, converged lambda) pairs.  Returns (flat parameters,
+    info dict)."""
     from adaptivemerging_amd.context import Context
 
     t0 = time.perf_counter()
@@ -209,7 +218,7 @@ def train_learned_start(args, params_kw):
     n_train = 0
     collected = 0
     for k in range(args.train_scenes):
-        scene = workloads.box_pile(seed=1000 + args.seed * 97 + k, n=args.train_bodies, bin_size=3.0)
+        scene = small_pile(args, k)
         tr.upload(scene)
         tr.set_params(**{**params_kw, "iterations": args.train_iterations, "collect_training": 0,
                          "learned_warm_start": 0})
@@ -238,6 +247,53 @@ def train_learned_start(args, params_kw):
     return params, info
 
 
+def evaluate_learned_start(args, params_kw, mlp):
+    """Sweeps to the tolerance with the plain and with the learned start on
+    held-out small piles (seeds the network was not trained on), where the
+    tolerance is reached: two contexts per pile replay the same settling (the
+    GPU path is deterministic), then run the same steps, one of them with the
+    learned start and its plain-start fallback."""
+    from adaptivemerging_amd.context import Context
+
+    kw = {**params_kw, "iterations": args.train_iterations, "learned_warm_start": 0, "collect_training": 0}
+    out = {"plain": [], "learned": [], "ms_plain": [], "ms_learned": [], "used": [], "converged_plain": [],
+           "converged_learned": []}
+    for k in range(1, args.eval_scenes + 1):
+        scene = small_pile(args, -k)
+        ctxs = []
+        for learned in (False, True):
+            c = Context()
+            c.upload(scene)
+            c.set_params(**kw)
+            for _ in range(args.train_settle):
+                c.step(args.dt)
+            if learned:
+                c.set_mlp_params(mlp)
+                c.set_params(learned_warm_start=1)
+            ctxs.append(c)
+        for name, c in zip(("plain", "learned"), ctxs):
+            for _ in range(args.train_steps):
+                st = c.step(args.dt)
+                out[name].append(st["iterations_used"])
+                out["ms_" + name].append(st["ms_total"])
+                out["converged_" + name].append(st["last_change"] < args.tolerance)
+                if name == "learned":
+                    out["used"].append(st["used_learned"])
+            c.close()
+    res = dict(eval_scenes=args.eval_scenes, steps_per_scene=args.train_steps,
+               sweeps_plain=float(np.mean(out["plain"])), sweeps_learned=float(np.mean(out["learned"])),
+               ms_per_step_plain=float(np.mean(out["ms_plain"])), ms_per_step_learned=float(np.mean(out["ms_learned"])),
+               converged_frac_plain=float(np.mean(out["converged_plain"])),
+               converged_frac_learned=float(np.mean(out["converged_learned"])),
+               learned_used_frac=float(np.mean(out["used"])))
+    res["sweep_ratio"] = res["sweeps_plain"] / res["sweeps_learned"] if res["sweeps_learned"] > 0 else None
+    res["ms_ratio"] = res["ms_per_step_plain"] / res["ms_per_step_learned"] if res["ms_per_step_learned"] > 0 else None
+    print(f"# held-out piles: sweeps to {args.tolerance}: plain {res['sweeps_plain']:.1f}, learned "
+          f"{res['sweeps_learned']:.1f} (learned start used {res['learned_used_frac']:.2f})", file=sys.stderr,
+          flush=True)
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -252,11 +308,13 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--learned", choices=["on", "off"], default="on",
                     help="also run the learned warm start from the same state")
-    ap.add_argument("--train-scenes", type=int, default=4, help="held-out piles the network is trained on")
-    ap.add_argument("--train-bodies", type=int, default=300)
-    ap.add_argument("--train-settle", type=int, default=60)
+    ap.add_argument("--train-scenes", type=int, default=4, help="small piles the network is trained on")
+    ap.add_argument("--eval-scenes", type=int, default=2, help="held-out small piles the gain is measured on")
+    ap.add_argument("--train-bodies", type=int, default=100)
+    ap.add_argument("--train-bin", type=float, default=4.0)
+    ap.add_argument("--train-settle", type=int, default=80)
     ap.add_argument("--train-steps", type=int, default=30)
-    ap.add_argument("--train-iterations", type=int, default=2000)
+    ap.add_argument("--train-iterations", type=int, default=3000)
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
     ap.add_argument("--cpu-sweeps", type=int, default=40, help="sequential sweeps of the CPU baseline sample")
@@ -321,6 +379,8 @@ def main() -> int:
     mlp, train_info = (None, None)
     if learned:
         mlp, train_info = train_learned_start(args, params_kw)
+        if mlp is not None and args.eval_scenes > 0:
+            train_info["held_out"] = evaluate_learned_start(args, params_kw, mlp)
 
     plain = Context()
     plain.upload(scene)

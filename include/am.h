@@ -177,6 +177,10 @@ typedef struct am_params {
     int32_t learned_warm_start;
     int32_t collect_training;         /* 1: each main solve that reaches the tolerance appends
                                          (features, converged lambda) to the training set */
+    int32_t organize_contacts;        /* 1 (Merging.organizeContacts): the single sweep inside collections
+                                         visits the body pairs breadth first from those holding a new
+                                         contact (getOrganizedContacts); 0: colour order only */
+    int32_t reserved_params;
 } am_params;
 
 /* A spring (Spring.java:10-104).  ZERO pulls body1's point pb1 towards the

@@ -143,6 +143,7 @@ static void default_params(am_params* p) {
     p->warm_start = 1; p->use_gravity = 1; p->gravity_amount = 1.0; p->gravity_angle_deg = 90.0;
     p->viscous_linear_decay = 1.0; p->viscous_angular_decay = 1.0;
     p->spring_stiffness_mod = 1.0; p->spring_damping_mod = 1.0;
+    p->organize_contacts = 1;
 }
 
 orc_world* orc_create(const am_body_desc* bodies, int32_t n, const am_sphere_node* nodes, int32_t n_nodes) {

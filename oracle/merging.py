@@ -156,6 +156,53 @@ def merge_condition(P, hist, states, breath: bool, pinned1: bool, pinned2: bool,
     return True
 
 
+def organized_layers(body1, body2, is_new, n_ext: int):
+    """CollisionProcessor.getOrganizedContacts / getNextLayer (:346-441) on a
+    combined contact list (external contacts [0, n_ext), then internal ones):
+    the breadth-first layer of each contact's body pair.  Layer 0 holds the
+    external pairs with a contact new this step; layer k+1 the pairs not yet
+    reached that share a body with a pair of layer k (getNextLayer walks
+    body.bodyPairContacts, pinned bodies included); then come the external
+    pairs never reached, then the internal ones.  The reference visits the
+    pairs of one layer in HashSet order; only the layer is reproducible.
+    Returns the layer of every contact."""
+    n = len(body1)
+    pair = [(min(int(a), int(b)), max(int(a), int(b))) for a, b in zip(body1, body2)]
+    pairs, ext_pair, new_pair = [], {}, {}
+    for k in range(n):
+        p = pair[k]
+        if p not in ext_pair:
+            pairs.append(p)
+            ext_pair[p] = k < n_ext
+            new_pair[p] = False
+        if k < n_ext and is_new[k]:
+            new_pair[p] = True
+    by_body = {}
+    for p in pairs:
+        for b in set(p):
+            by_body.setdefault(b, []).append(p)
+    layer = {p: 0 for p in pairs if ext_pair[p] and new_pair[p]}
+    frontier = list(layer)
+    lmax = 0 if layer else -1
+    k = 0
+    while frontier:
+        nxt = []
+        for p in frontier:
+            for b in p:
+                for q in by_body[b]:
+                    if q not in layer:
+                        layer[q] = k + 1
+                        nxt.append(q)
+        if nxt:
+            lmax = k + 1
+        frontier = nxt
+        k += 1
+    for p in pairs:
+        if p not in layer:
+            layer[p] = lmax + 1 if ext_pair[p] else lmax + 2
+    return [layer[pair[k]] for k in range(n)]
+
+
 def unmerge_condition(P, contact_states, hist_n: int, n_contacts: int) -> bool:
     """checkContactsState (:255-266) or checkMotionMetricForUnmerging (:200-205)."""
     for s in contact_states:
@@ -640,6 +687,10 @@ class MergingOracle:
         # internal contacts of the collections that are awake (:265-270, :400-408)
         awake = [p for p in self.ibpc if not self.sleeping(self.parent[p["b1"]])]
         comb = _concat(ext, _stack([c for p in awake for c in p["contacts"]]))
+        # the single sweep's list and its breadth-first layers (getOrganizedContacts), kept for checks
+        self.last_sweep = dict(body1=np.array(comb["body1"]), body2=np.array(comb["body2"]), n_ext=n_ext,
+                               layers=organized_layers(comb["body1"], comb["body2"], comb["new_this_step"], n_ext)
+                               if self.params.organize_contacts else None)
         self.o.push_contacts_raw(comb)
         used, _, _ = self.o.solve(order_of_sweep(), dt, main=False)
         res = self.o.contacts()

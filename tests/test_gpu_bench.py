@@ -15,8 +15,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def test_bench_small_end_to_end(gpu_lib):
     cmd = [sys.executable, os.path.join(ROOT, "bench.py"), "--bodies", "400", "--settle", "40", "--warmup", "1",
-           "--steps", "3", "--iterations", "200", "--train-scenes", "1", "--train-bodies", "120", "--train-settle",
-           "20", "--train-steps", "6", "--epochs", "2", "--cpu-sweeps", "5"]
+           "--steps", "3", "--iterations", "200", "--train-scenes", "1", "--eval-scenes", "1", "--train-bodies", "40",
+           "--train-settle", "40", "--train-steps", "8", "--epochs", "2", "--cpu-sweeps", "5"]
     out = subprocess.run(cmd, capture_output=True, text=True, timeout=240)
     assert out.returncode == 0, out.stderr[-3000:]
     line = [ln for ln in out.stdout.splitlines() if ln.startswith("{")][-1]
@@ -28,4 +28,5 @@ def test_bench_small_end_to_end(gpu_lib):
     lrn = rec["config"]["learned"]
     assert lrn["train_samples"] > 0
     assert lrn["replay_identical"] is True
+    assert lrn["held_out"]["sweeps_plain"] > 0 and lrn["held_out"]["sweeps_learned"] > 0
     assert rec["config"]["plain"]["contacts_mean"] > 0

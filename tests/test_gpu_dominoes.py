@@ -13,7 +13,7 @@ from adaptivemerging_amd.context import Context
 from oracle.merging import MergingOracle
 from oracle.pyoracle import load as load_oracle
 from tests.helpers import assert_bits_equal, assert_contacts_equal
-from tests.test_gpu_merging import compare_world
+from tests.test_gpu_merging import check_organized, compare_world
 
 pytestmark = pytest.mark.gpu
 
@@ -63,6 +63,7 @@ def step_both(ctx, mo, t, step, push=None):
     ctx.warm_start(); mo.o.warm_start()
     ctx.update_in_collections(DT)
     mo.update_in_collections(DT, ctx.solve_order)
+    check_organized(ctx, mo)
     assert_contacts_equal(ctx.internal_contacts(), mo.internal_contacts())
     ctx.accumulate_for_unmerging(DT); mo.accumulate_for_unmerging(DT)
     cut_g = ctx.unmerge(DT)

@@ -31,6 +31,30 @@ def kick(obj, body, fy):
     obj.push_forces(f, t)
 
 
+# distinct breadth-first layers seen by check_organized (a run can assert it saw some)
+ORGANIZED_SEEN = set()
+
+
+def check_organized(ctx, mo):
+    """The single sweep's exported order visits the body pairs of every
+    dynamic body in non-decreasing breadth-first layer, the layers of
+    getOrganizedContacts as the merging oracle restates them."""
+    ls = getattr(mo, "last_sweep", None)
+    if not mo.members or ls is None or ls["layers"] is None:
+        return
+    order = ctx.solve_order()
+    b1, b2, layers = ls["body1"], ls["body2"], ls["layers"]
+    assert len(order) == len(layers)
+    last = {}
+    for k in order:
+        for b in {int(b1[k]), int(b2[k])}:
+            if mo.pinned(b):
+                continue
+            assert layers[k] >= last.get(b, -1), f"body {b}: layer {layers[k]} after {last[b]}"
+            last[b] = layers[k]
+        ORGANIZED_SEEN.add(layers[k])
+
+
 def step_both(ctx, mo, dt, step, kick_body=None):
     mo.total_steps += 1
     ctx.clear_bodies(); mo.clear_bodies()
@@ -44,6 +68,7 @@ def step_both(ctx, mo, dt, step, kick_body=None):
     ctx.warm_start(); mo.o.warm_start()
     ctx.update_in_collections(dt)
     mo.update_in_collections(dt, ctx.solve_order)
+    check_organized(ctx, mo)
     assert_contacts_equal(ctx.internal_contacts(), mo.internal_contacts())
     ctx.accumulate_for_unmerging(dt); mo.accumulate_for_unmerging(dt)
     cut_g = ctx.unmerge(dt)
@@ -109,6 +134,7 @@ def test_merge_and_unmerge_match_oracle(gpu_lib, position_level):
         compare_world(ctx, mo, step)
         print("step", step, "top", list(ctx.structure()[1]), "merged", m, "cut", c, flush=True)
     assert merges >= 2 and cuts >= 1
+    assert len(ORGANIZED_SEEN) >= 2, ORGANIZED_SEEN
 
 
 @pytest.mark.parametrize("position_level", [0, 1])

@@ -63,14 +63,21 @@ def test_mlp_gradients_match_fp64_reference(gpu_lib):
 
 
 def test_training_reduces_loss(gpu_lib):
-    scene = workloads.jenga_tower(seed=3, layers=8, overlap=0.001)
+    """Pairs come only from solves that reach the tolerance: a shallow pile of
+    100 boxes, whose solves reach 1e-6 within the cap."""
+    scene = workloads.box_pile(seed=1100, n=100, bin_size=4.0)
     ctx = Context()
     ctx.upload(scene)
-    ctx.set_params(iterations=200, tolerance=1e-8, gravity_amount=9.81, collect_training=1, enable_merging=0,
-                   enable_unmerging=0)
+    ctx.set_params(iterations=3000, tolerance=1e-6, gravity_amount=9.81, collect_training=0, enable_merging=0,
+                   enable_unmerging=0, enable_sleeping=0)
     init_params(ctx)
-    for _ in range(10):
+    for _ in range(60):
         ctx.step(0.01)
+    ctx.set_params(collect_training=1)
+    converged = 0
+    for _ in range(12):
+        converged += ctx.step(0.01)["last_change"] < 1e-6
+    assert converged >= 3
     n = C.c_int32()
     mlp_call(ctx, "am_train_size", C.byref(n))
     assert n.value > 500

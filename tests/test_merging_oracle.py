@@ -241,3 +241,30 @@ def test_position_level_metric_dispatch():
                     assert mo._metric(a, b, dt) == mo.o.motion_metric_pos(a, b, dt)
                     assert mo._metric(a, b, dt) != mo.o.motion_metric(a, b)
     assert merged >= 1 and free_pairs >= 2
+
+
+def test_organized_layers_stack_closed_form():
+    """getOrganizedContacts (CollisionProcessor.java:346-441) on a column of
+    boxes 1..5 on the plane 0 with a new contact only between 4 and 5: the
+    layers run down the column, reach box 6 on the plane through the plane,
+    then the external pair never reached, then the internal pair."""
+    from oracle.merging import organized_layers
+
+    pairs = [(0, 1), (1, 2), (2, 3), (3, 4), (4, 5), (0, 6), (9, 10), (7, 8)]
+    b1, b2, new = [], [], []
+    for i, (a, b) in enumerate(pairs):
+        for j in range(2):          # two contacts per pair
+            b1.append(b if i % 2 else a)
+            b2.append(a if i % 2 else b)
+            new.append(1 if (a, b) == (4, 5) and j == 1 else 0)
+    n_ext = 14                      # the last pair (7, 8) is internal
+    layers = organized_layers(b1, b2, new, n_ext)
+    want = {(4, 5): 0, (3, 4): 1, (2, 3): 2, (1, 2): 3, (0, 1): 4, (0, 6): 5, (9, 10): 6, (7, 8): 7}
+    for k, (a, b) in enumerate(zip(b1, b2)):
+        assert layers[k] == want[(min(a, b), max(a, b))]
+    # no new contact: every external pair comes before every internal pair
+    layers = organized_layers(b1, b2, [0] * len(b1), n_ext)
+    assert set(layers[:n_ext]) == {0} and set(layers[n_ext:]) == {1}
+    # a new contact on an internal pair does not seed a layer
+    new_int = [0] * n_ext + [1, 1]
+    assert organized_layers(b1, b2, new_int, n_ext) == layers
