@@ -294,7 +294,7 @@ def evaluate_learned_start(args, params_kw, mlp):
     return res
 
 
-def main() -> int:
+def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -333,7 +333,7 @@ def main() -> int:
                          "auto: on unless --pgs-schedule, --pgs-grid or --pgs-block is given")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (calibrates the HBM byte counters) and exit")
-    args = ap.parse_args()
+    args = ap.parse_args(argv)
     explicit_shape = args.pgs_schedule is not None or args.pgs_grid != 0 or args.pgs_block != 0
     if args.pgs_schedule is None:
         args.pgs_schedule = "dataflow"
