@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -2115,6 +2116,62 @@ int am_step(am_ctx* c, double dt, am_step_stats* st) {
         st->n_internal_contacts = c->n_ic;
         st->n_sleeping = asleep;
     }
+    return AM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// Staging for am_motion_metric_noalloc: the pairs (2 ints each) then the
+// results (1 double each) of up to AM_METRIC_NOALLOC_MAX pairs, 1 MiB in all.
+// Every GPU the library runs on holds its own copy until the process ends.
+constexpr size_t kNoAllocBytes = (size_t)AM_METRIC_NOALLOC_MAX * 16;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of this device's scratch array, looked up once per
+// device with that device current (the array's host-side name is not a
+// device address).
+int scratch_on_device(am_ctx* c, unsigned char** out) {
+    if (c->device < 0 || c->device >= kMaxDevices) return fail(c, AM_ERR_INVALID, "device index too large");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    unsigned char*& p = g_scratch_ptr[c->device];
+    if (!p) {
+        void* addr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !addr)
+            return fail(c, AM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        p = (unsigned char*)addr;
+    }
+    *out = p;
+    return AM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int am_motion_metric_noalloc(am_ctx* c, const int32_t* pairs, int32_t n_pairs, double* out) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (n_pairs < 0 || (n_pairs > 0 && (!pairs || !out))) return fail(c, AM_ERR_INVALID, "bad metric arrays");
+    if (n_pairs > AM_METRIC_NOALLOC_MAX)
+        return fail(c, AM_ERR_CAPACITY, "am_motion_metric_noalloc takes at most AM_METRIC_NOALLOC_MAX (65536) pairs");
+    unsigned char* base = nullptr;
+    if ((r = scratch_on_device(c, &base))) return r;
+    if (n_pairs == 0) return AM_OK;
+    for (int k = 0; k < n_pairs; k++)
+        if (pairs[2 * k] < 0 || pairs[2 * k] >= c->n_ent || pairs[2 * k + 1] >= c->n_ent)
+            return fail(c, AM_ERR_INVALID, "metric entity out of range");
+    int* dp = reinterpret_cast<int*>(base);
+    double* dout = reinterpret_cast<double*>(base + (size_t)AM_METRIC_NOALLOC_MAX * 8);
+    AM_HIP(c, hipMemcpyAsync(dp, pairs, (size_t)n_pairs * 8, hipMemcpyHostToDevice, c->stream));
+    launch_metric(c->stream, bodies_view(c), dp, n_pairs, dout);
+    AM_HIP(c, hipGetLastError());
+    AM_HIP(c, hipMemcpyAsync(out, dout, (size_t)n_pairs * 8, hipMemcpyDeviceToHost, c->stream));
+    AM_HIP(c, hipStreamSynchronize(c->stream));
     return AM_OK;
 }
 

@@ -521,6 +521,15 @@ int am_start_lambdas_pull(am_ctx* ctx, double* lambda, int32_t capacity, int32_t
 /* For each pair (a,b) of entity ids: the largest relative velocity over both
  * bodies' bounding-box corners; b < 0 means the one-body variant. */
 int am_motion_metric(am_ctx* ctx, const int32_t* pairs, int32_t n_pairs, double* out);
+/* am_motion_metric without any device allocation; no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, hosts that forbid hipMalloc): the pairs
+ * and results are staged in a fixed device array of AM_METRIC_NOALLOC_MAX
+ * pairs (1 MiB per GPU, held until the process ends).  More pairs fail with
+ * AM_ERR_CAPACITY.  Every call on one GPU shares that array, so calls that
+ * overlap on one GPU (from several threads or contexts) race. */
+#define AM_METRIC_NOALLOC_MAX 65536
+int am_motion_metric_noalloc(am_ctx* ctx, const int32_t* pairs, int32_t n_pairs, double* out);
 
 /* ---- launch-shape autotuner (no reference counterpart) ----------------
  * Times the PGS kernel on the contact set currently held (after am_detect
