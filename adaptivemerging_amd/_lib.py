@@ -162,7 +162,7 @@ SYMBOLS = [
     "am_mlp_gradients", "am_features_pull", "am_train_size", "am_train_clear", "am_train_epochs",
     "am_start_lambdas_pull", "am_autotune", "am_advance_positions_post", "am_impulse_add", "am_kinematic_set",
     "am_kinematic_update", "am_sim_time_get", "am_sim_time_set", "am_apply_impulses", "am_apply_viscous_decay",
-    "am_scene_iterations_pull", "am_motion_metric_noalloc",
+    "am_scene_iterations_pull", "am_motion_metric_noalloc", "am_launch_shape_pull",
 ]
 AM_METRIC_NOALLOC_MAX = 65536
 MLP_PARAMS = 10115
@@ -261,6 +261,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "am_apply_viscous_decay": (C.c_int, [ctxp]),
         "am_scene_iterations_pull": (C.c_int, [ctxp, vp, vp, i32, C.POINTER(i32)]),
         "am_motion_metric_noalloc": (C.c_int, [ctxp, vp, i32, vp]),
+        "am_launch_shape_pull": (C.c_int, [ctxp, C.POINTER(i32), C.POINTER(i32)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -91,6 +91,12 @@ class Context:
         f = np.ascontiguousarray(force, dtype=np.float64)
         self._check(self.lib.am_impulse_add(self.h, body, ptr(p), ptr(f)))
 
+    def launch_shape(self):
+        """(workgroups, threads per workgroup) of the last main solve's PGS kernel."""
+        g, b = C.c_int32(), C.c_int32()
+        self._check(self.lib.am_launch_shape_pull(self.h, C.byref(g), C.byref(b)))
+        return g.value, b.value
+
     def scene_iterations(self):
         """(sweeps used, max |delta lambda| of the last sweep) per scene of
         the last main solve that stopped scenes one by one."""

@@ -851,6 +851,10 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
     }
     int grid = 1, block = 64;
     pgs_shape(c, P, &grid, &block);
+    if (main_solve) {
+        c->last_grid = grid;
+        c->last_block = block;
+    }
     if (learned) {
         // plain start residual, predicted start residual; keep the better
         AM_HIP(c, c->dv_save.ensure((size_t)c->n_ent * 6));
@@ -1328,6 +1332,15 @@ int am_apply_viscous_decay(am_ctx* c) {
     int r = check_ctx(c);
     if (r) return r;
     return apply_viscous_decay(c);
+}
+
+int am_launch_shape_pull(am_ctx* c, int32_t* grid, int32_t* block) {
+    int r = check_ctx(c);
+    if (r) return r;
+    if (!grid || !block) return fail(c, AM_ERR_INVALID, "null argument");
+    *grid = c->last_grid;
+    *block = c->last_block;
+    return AM_OK;
 }
 
 int am_scene_iterations_pull(am_ctx* c, int32_t* used, double* last, int32_t capacity, int32_t* n_scenes) {

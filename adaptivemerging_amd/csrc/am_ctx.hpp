@@ -188,6 +188,7 @@ struct am_ctx {
     int pgs_max_grid[5] = {0, 0, 0, 0, 0};  // co-resident workgroups of the coloured kernel, by block / 64
     hipEvent_t kev[3] = {nullptr, nullptr, nullptr};  // setup start, pgs start, pgs end
     double ms_pgs_kernel = 0.0, ms_setup_kernel = 0.0;
+    int last_grid = 0, last_block = 0;  // launch shape of the last main solve's PGS kernel
 
     // learned warm start
     bool mlp_ready = false;

@@ -367,11 +367,13 @@ void launch_to_bf16(hipStream_t s, const float* p, unsigned short* bf, int n) {
 }
 
 // ---------------------------------------------------------------------------
-// Predicted start: the network predicts the change y of slog(lambda / S)
-// from the plain warm start, clamped to +-kMaxLogStep, so lambda =
+// Predicted start: for a contact the plain warm start found no previous
+// contact for (new this step, lambda_plain = 0), the network predicts the
+// change y of slog(lambda / S), clamped to +-kMaxLogStep, so lambda =
 // S sexp(slog(lambda_plain / S) + y), projected onto the friction box
-// (lambda_n >= 0, |lambda_t| <= mu lambda_n).  A network that outputs zero
-// reproduces the plain start (up to the rounding of slog and sexp).
+// (lambda_n >= 0, |lambda_t| <= mu lambda_n).  A contact that persists keeps
+// its previous-frame lambda: on a resting pile that start is already close
+// to converged, closer than the network's per-contact accuracy.
 AMD_INLINE double predicted(double lam_plain, double S, float y) {
     const double z = slog(lam_plain / S) + fmin(fmax((double)y, -kMaxLogStep), kMaxLogStep);
     return S * sexp(z);
@@ -380,6 +382,11 @@ __global__ void k_apply_prediction(ContactsView C, const BodiesView B, const flo
                                    int fric_ovr, double fric_val, double* lam_out) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
+    if (!C.is_new[k]) {
+#pragma unroll
+        for (int r = 0; r < 3; r++) lam_out[k * 3 + r] = C.lam[k * 3 + r];
+        return;
+    }
     double S = scale[k];
     double ln = fmax(0.0, predicted(C.lam[k * 3 + 0], S, Y[k * 3 + 0]));
     double mu;

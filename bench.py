@@ -64,10 +64,10 @@ def workload_key(args) -> dict:
                 dt=args.dt, seed=args.seed, schedule=args.pgs_schedule)
 
 
-def measured_traffic(args) -> dict | None:
+def measured_traffic(args, grid_used: int, block_used: int) -> dict | None:
     """HBM bytes per PGS launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this same workload (counters scaled by a plain copy
-    of known size); None when the workload arguments differ from the
+    WRITE_SIZE summary of this same workload and launch shape (counters
+    scaled by a plain copy of known size); None when they differ from the
     profiled ones."""
     try:
         with open(TRAFFIC_PROFILES[args.pgs_schedule]) as f:
@@ -75,7 +75,8 @@ def measured_traffic(args) -> dict | None:
     except (OSError, StopIteration):
         return None
     rec = json.loads(line)
-    if rec["workload"] != workload_key(args) or args.pgs_grid != 0 or args.pgs_block != 0:
+    if rec["workload"] != workload_key(args) or rec.get("grid_used") != grid_used or \
+            rec.get("block_used") != block_used:
         return None
     return rec
 
@@ -250,7 +251,9 @@ def train_learned_start(args, params_kw):
 def evaluate_learned_start(args, params_kw, mlp):
     """Sweeps to the tolerance with the plain and with the learned start on
     held-out small piles (seeds the network was not trained on), where the
-    tolerance is reached: two contexts per pile replay the same settling (the
+    tolerance is reached, while the boxes land (steps --eval-from onwards of
+    the drop: contacts keep appearing, which is where a start for new
+    contacts can matter): two contexts per pile replay the same steps (the
     GPU path is deterministic), then run the same steps, one of them with the
     learned start and its plain-start fallback."""
     from adaptivemerging_amd.context import Context
@@ -265,7 +268,7 @@ def evaluate_learned_start(args, params_kw, mlp):
             c = Context()
             c.upload(scene)
             c.set_params(**kw)
-            for _ in range(args.train_settle):
+            for _ in range(args.eval_from):
                 c.step(args.dt)
             if learned:
                 c.set_mlp_params(mlp)
@@ -310,6 +313,7 @@ def main(argv=None) -> int:
                     help="also run the learned warm start from the same state")
     ap.add_argument("--train-scenes", type=int, default=4, help="small piles the network is trained on")
     ap.add_argument("--eval-scenes", type=int, default=2, help="held-out small piles the gain is measured on")
+    ap.add_argument("--eval-from", type=int, default=10, help="step of the drop the held-out measurement starts at")
     ap.add_argument("--train-bodies", type=int, default=100)
     ap.add_argument("--train-bin", type=float, default=4.0)
     ap.add_argument("--train-settle", type=int, default=80)
@@ -435,7 +439,8 @@ def main(argv=None) -> int:
                   f"{res_l['residual_final_mean']:.3g}, used {res_l['learned_used_frac']:.2f}", file=sys.stderr,
                   flush=True)
 
-    traffic = measured_traffic(args)
+    grid_used, block_used = plain.launch_shape()
+    traffic = measured_traffic(args, grid_used, block_used)
     achieved = res_p["_pgs_bytes"] / (res_p["_pgs_ms"] * 1e-3) / 1e9 if res_p["_pgs_ms"] > 0 else 0.0
     roofline = {
         "bound": "hbm",
@@ -447,6 +452,8 @@ def main(argv=None) -> int:
         "frac": achieved / HBM_PEAK_GBS,
         "traffic": traffic["traffic_bytes_per_launch"] if traffic else None,
         "traffic_source": os.path.relpath(TRAFFIC_PROFILES[args.pgs_schedule], ROOT) if traffic else None,
+        "grid_used": grid_used,
+        "block_used": block_used,
         "bytes_per_launch": res_p["_pgs_bytes"] / max(res_p["_pgs_launches"], 1),
         "ms_per_launch": res_p["_pgs_ms"] / max(res_p["_pgs_launches"], 1),
     }

@@ -555,6 +555,9 @@ typedef struct am_tune_trial {
 } am_tune_trial;
 int am_autotune(am_ctx* ctx, double dt, int32_t sweeps, int32_t reps, int32_t apply, am_tune_trial* trials,
                 int32_t capacity, int32_t* n_trials, int32_t* best);
+/* Workgroups and threads per workgroup of the last main solve's PGS kernel
+ * (0, 0 before the first solve with contacts). */
+int am_launch_shape_pull(am_ctx* ctx, int32_t* grid, int32_t* block);
 
 #ifdef __cplusplus
 }
