@@ -928,6 +928,32 @@ struct DmaSlot {
     double2 lam[64];
 };
 
+// A contact's new lambdas go to this pass's buffer; the last sweep also sets
+// its state (Contact.updateContactState, Contact.java:385-398).
+AMD_INLINE void contact_done(const FlowArgs& F, const WaveRec& c, int k, int p, bool last, bool lead,
+                             const double (&lam)[3], double dv) {
+    const PgsArgs& P = F.P;
+    {
+        // every lane of the row stores the same values to the same place
+        double* dst = (p & 1) ? &F.lam_alt[4 * k] : &P.rec[(size_t)k * kRecDoubles + kRecLambda];
+        dst[0] = lam[0];
+        dst[1] = lam[1];
+        dst[2] = lam[2];
+    }
+    if (last) {
+        const double w1 = c.b[1] + row_jdv(c.jc[2] * dv);
+        const double w2 = c.b[2] + row_jdv(c.jc[4] * dv);
+        if (lead) {
+            int st;
+            if (fabs(lam[0]) <= 1e-14) st = AM_STATE_BROKEN;
+            else if (fabs(w1) > P.sliding) st = AM_STATE_ONEDGE;
+            else if (fabs(w2) > P.sliding) st = AM_STATE_ONEDGE;
+            else st = AM_STATE_CLEAR;
+            P.sstate[k] = st;
+        }
+    }
+}
+
 // One contact of a row-held group in pass p (PGS.java:100-184; pass 0 is
 // confidentWarmStart, PGS.java:250-256).  dv is this lane's deltaV component,
 // f its body's minv on v lanes and 1 on w lanes (PGS.java:221-245 scales the
@@ -973,26 +999,59 @@ AMD_INLINE void row_contact(const FlowArgs& F, const WaveRec& c, int k, int p, b
         mx = fmax(mx, fabs(diff));
         nan = nan || diff != diff;
     }
-    {
-        // every lane of the row stores the same values to the same place
-        double* dst = (p & 1) ? &F.lam_alt[4 * k] : &P.rec[(size_t)k * kRecDoubles + kRecLambda];
-        dst[0] = lam[0];
-        dst[1] = lam[1];
-        dst[2] = lam[2];
+    contact_done(F, c, k, p, last, lead, lam, dv);
+}
+
+// The same update where the step has been found to need none of the general
+// forms: every row's |numerator| in [2^-250, 2^250) with a nonzero stored
+// reciprocal (the quotient is then q0 = n y, r = fma(-d, q0, n), q = fma(r, y,
+// q0), exactly the division), every friction limit a finite value >= +0 (the
+// plain selects are then Math.max/Math.min), and every contact clamped.  The
+// conditions are gathered per lane into FastCheck without a branch; the step
+// checks them once at its end and, if any failed, runs again from its start
+// with row_contact.
+struct FastCheck {
+    unsigned num_hi_max = 0;            // high words of |numerator|
+    unsigned num_hi_min = 0xffffffffu;
+    unsigned limit_hi_max = 0;          // high words of the friction limits (sign included)
+    bool plain = true;                  // stored reciprocals nonzero, contacts clamped
+    AMD_INLINE bool ok() const {
+        return plain && num_hi_min >= 0x30500000u && num_hi_max < 0x4F900000u && limit_hi_max < 0x7FF00000u;
     }
-    if (last) {
-        // Contact.updateContactState (Contact.java:385-398)
-        const double w1 = c.b[1] + row_jdv(c.jc[2] * dv);
-        const double w2 = c.b[2] + row_jdv(c.jc[4] * dv);
-        if (lead) {
-            int st;
-            if (fabs(lam[0]) <= 1e-14) st = AM_STATE_BROKEN;
-            else if (fabs(w1) > P.sliding) st = AM_STATE_ONEDGE;
-            else if (fabs(w2) > P.sliding) st = AM_STATE_ONEDGE;
-            else st = AM_STATE_CLEAR;
-            P.sstate[k] = st;
+};
+AMD_INLINE unsigned hi_word(double v) { return (unsigned)((unsigned long long)__double_as_longlong(v) >> 32); }
+AMD_INLINE void row_contact_fast(const FlowArgs& F, const WaveRec& c, int k, int p, bool last, bool lead, double f,
+                                 double& dv, double& mx, bool& nan, FastCheck& chk) {
+    const PgsArgs& P = F.P;
+    double lam[3] = {c.lam[0], c.lam[1], c.lam[2]};
+    chk.plain = chk.plain && c.clamp != 0.0;
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const double jd = row_jdv(c.jc[2 * r] * dv);
+        const double prev = lam[r];
+        const double num = c.d[r] * lam[r] - P.omega * (c.b[r] + jd);
+        const unsigned h = hi_word(num) & 0x7fffffffu;
+        chk.num_hi_max = max(chk.num_hi_max, h);
+        chk.num_hi_min = min(chk.num_hi_min, h);
+        chk.plain = chk.plain && c.y[r] != 0.0;
+        const double q0 = num * c.y[r];
+        const double rr = __builtin_fma(-c.den[r], q0, num);
+        double l = __builtin_fma(rr, c.y[r], q0);
+        if (r == 0) {
+            l = l <= 0.0 ? 0.0 : l;
+        } else {
+            const double limit = c.mu * lam[0];
+            chk.limit_hi_max = max(chk.limit_hi_max, hi_word(limit));
+            const double a = l < -limit ? -limit : l;
+            l = a > limit ? limit : a;
         }
+        lam[r] = l;
+        const double diff = l - prev;
+        dv = (f * diff) * c.jc[2 * r + 1] + dv;
+        mx = fmax(mx, fabs(diff));
+        nan = nan || diff != diff;
     }
+    contact_done(F, c, k, p, last, lead, lam, dv);
 }
 
 // Steps: up to 4 consecutive sorted groups of one parallel colour (no shared
@@ -1032,16 +1091,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     __shared__ DmaSlot s_dma[kDmaSlots];
     const LaneMap lmap = lane_map(li);
     int chk = 0;  // sweeps 1..chk are known not to have converged
-    // counters of the last step, released once its deltaV stores are done
-    // (the wait overlaps the next step's first poll); lanes 0 and 1 of a row
-    bool pend = false;
-    unsigned* pend_ctr = nullptr;
-    unsigned pend_val = 0;
-    auto release = [&]() {
-        __builtin_amdgcn_s_waitcnt(0);
-        if (pend_ctr) st_cnt(pend_ctr, pend_val);
-        pend = false;
-    };
 
     for (int p = first; p <= lastp; p++) {
         // pass gates: sweep p-2 must be known not converged (p-1 for the last sweep)
@@ -1110,7 +1159,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             if (P.per_scene && p >= 1 && P.check_tolerance) {
                 const int t1 = p - 2, t2 = (p == lastp) ? p - 1 : 0;
                 if (t1 >= 1 || t2 >= 1) {
-                    if (pend) release();  // a group waited on may need this wave's last counters
                     const unsigned want = has_group ? P.scene_groups[sc] : 0u;
                     const unsigned* d1 = &P.scene_done[(size_t)sc * (P.iterations + 1) + (t1 >= 1 ? t1 : t2)];
                     const unsigned* d2 = &P.scene_done[(size_t)sc * (P.iterations + 1) + (t2 >= 1 ? t2 : t1)];
@@ -1185,7 +1233,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 while (true) {
                     unsigned v = tgt;
                     if (need) v = ld_cnt(ctr);
-                    if (pend) release();
                     if (__all(v == tgt)) break;
                     __builtin_amdgcn_s_sleep(1);
                     if (++spins > kFlowSpinLimit ||
@@ -1211,8 +1258,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             // (groups of more than kDmaSlots contacts) read global memory
             double smx = 0.0;  // this step's max |dl| and NaN flag (row-uniform)
             bool snan = false;
-            for (int t = 0; t < nmax; t++) {
-                WaveRec rc;
+            auto read_rec = [&](int t, WaveRec& rc) {
                 if (t < kDmaSlots) {
                     const double* rl = reinterpret_cast<const double*>(&s_dma[t].rec[row >> 1][(row & 1) * 32]);
                     const double* ll = odd_lam ? reinterpret_cast<const double*>(&s_dma[t].lam[row * 16])
@@ -1223,7 +1269,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                     const double* rg = P.rec + (size_t)k * kRecDoubles;
                     read_wave_rec(rg, lam_src(F, k, p), lmap, rc);
                 }
-                if (t < n) row_contact(F, rc, gd.x + t, p, last, lead, f, dv, smx, snan);
+            };
+            // sweeps try the fast update first; the lambdas it stored are
+            // rewritten (same places, same lanes) if the step runs again
+            bool general = p == 0;
+            if (!general) {
+                const double dv_start = dv;
+                FastCheck fc;
+                for (int t = 0; t < nmax; t++) {
+                    WaveRec rc;
+                    read_rec(t, rc);
+                    if (t < n) row_contact_fast(F, rc, gd.x + t, p, last, lead, f, dv, smx, snan, fc);
+                }
+                if (__builtin_expect(!__all(fc.ok()), 0)) {
+                    general = true;
+                    dv = dv_start;
+                    smx = 0.0;
+                    snan = false;
+                    __builtin_amdgcn_s_waitcnt(0);
+                }
+            }
+            if (general) {
+                for (int t = 0; t < nmax; t++) {
+                    WaveRec rc;
+                    read_rec(t, rc);
+                    if (t < n) row_contact(F, rc, gd.x + t, p, last, lead, f, dv, smx, snan);
+                }
             }
             mx = fmax(mx, smx);
             nan = nan || snan;
@@ -1237,20 +1308,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 __hip_atomic_fetch_add(&P.scene_done[(size_t)sc * (P.iterations + 1) + p], 1u, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
             }
-            // publish deltaV now, the counters once the stores are done
+            // publish deltaV, then the counters (lanes 0 and 1 of a row) as
+            // soon as the stores are complete.  Nothing else is in flight
+            // here: the next step's record copies are issued only after this
+            // wait, so they do not delay the counters.
             if (active && mine && dyn) st_coherent(&cur[e * 6 + comp], dv);
-            pend = true;
-            pend_ctr = nullptr;
-            if (active && li == 0 && fd.y) {
-                pend_ctr = &F.cnt[gd.z];
-                pend_val = (unsigned)p * (unsigned)fd.y + (unsigned)fd.x + 1u;
-            }
-            if (active && li == 1 && fd.w) {
-                pend_ctr = &F.cnt[gd.w];
-                pend_val = (unsigned)p * (unsigned)fd.w + (unsigned)fd.z + 1u;
-            }
+            __builtin_amdgcn_s_waitcnt(0);
+            if (active && li == 0 && fd.y) st_cnt(&F.cnt[gd.z], (unsigned)p * (unsigned)fd.y + (unsigned)fd.x + 1u);
+            if (active && li == 1 && fd.w) st_cnt(&F.cnt[gd.w], (unsigned)p * (unsigned)fd.w + (unsigned)fd.z + 1u);
         }
-        if (pend) release();
         if (p >= 1) {
             if (__any(nan)) mx = __longlong_as_double(0x7ff8000000000000ll);
             mx = wave_max(mx);
