@@ -433,8 +433,9 @@ int build_layout(am_ctx* c) {
                          c->g_sizes.p, c->g_pos_start.p, c->s_order.p, c->s_group_desc.p, c->s_pos_a.p,
                          c->g_spos.p, sc + 9, c->d_scan_temp.p);
     AM_HIP(c, c->s_flow.ensure((size_t)std::max(ng, 1) * 4));
+    AM_HIP(c, c->s_pred.ensure((size_t)std::max(ng, 1) * 2));
     launch_flow_ranks(s, c->n_ent, ng, c->d_flags.p, c->inc_start.p, c->inc_count.p, c->inc_items.p, c->g_spos.p,
-                      c->s_group_desc.p, c->s_flow.p);
+                      c->s_group_desc.p, c->s_flow.p, c->s_pred.p);
     AM_HIP(c, hipGetLastError());
     int cs[kMaxColours + 2];
     AM_HIP(c, hipMemcpyAsync(cs, c->s_colour_start.p, sizeof cs, hipMemcpyDeviceToHost, s));
@@ -883,16 +884,19 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
         const size_t n_dv = (size_t)c->n_ent * 6;
         AM_HIP(c, c->s_lam_alt.ensure((size_t)c->n_contacts * 4));
         AM_HIP(c, c->dv_alt.ensure(std::max(n_dv, (size_t)6)));
-        AM_HIP(c, c->f_cnt.ensure(std::max(c->n_ent, 1)));
+        const size_t n_slot = (size_t)2 * 2 * std::max(c->n_groups, 1) * kSlotDoubles;
+        AM_HIP(c, c->f_slot.ensure(n_slot));
         AM_HIP(c, c->d_pass_done.ensure(iterations + 2));
-        fill_int(s, (int*)c->f_cnt.p, c->n_ent, 0);
+        fill_u64(s, c->f_slot.p, n_slot, kSlotEmpty);
         fill_int(s, (int*)c->d_pass_done.p, iterations + 2, 0);
         AM_HIP(c, hipMemcpyAsync(c->dv_alt.p, c->d_dv.p, n_dv * sizeof(double), hipMemcpyDeviceToDevice, s));
         const int fgrid = grid;
         FlowArgs F;
         F.P = G;
         F.flow = c->s_flow.p;
-        F.cnt = c->f_cnt.p;
+        F.pred = c->s_pred.p;
+        F.slot = c->f_slot.p;
+        F.n_groups = c->n_groups;
         F.dv_alt = c->dv_alt.p;
         F.lam_alt = c->s_lam_alt.p;
         F.pass_done = c->d_pass_done.p;

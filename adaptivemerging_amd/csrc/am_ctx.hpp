@@ -146,8 +146,9 @@ struct am_ctx {
     DBuf<int> s_meta, s_state, d_barrier, s_group_desc, s_pos_a;
     DBuf<unsigned long long> s_iter_max;
     // dataflow schedule: sorted position per group, ranks, counters, odd-pass buffers
-    DBuf<int> g_spos, s_flow;
-    DBuf<unsigned> f_cnt, d_pass_done;
+    DBuf<int> g_spos, s_flow, s_pred;
+    DBuf<unsigned> d_pass_done;
+    DBuf<double> f_slot;
     DBuf<double> dv_alt, s_lam_alt;
     int flow_max_grid = 0;
     // poses advanced by dt for the position-level motion metric

@@ -30,6 +30,15 @@ void fill_int(hipStream_t s, int* p, int n, int v) {
 void fill_double(hipStream_t s, double* p, int n, double v) {
     if (n > 0) hipLaunchKernelGGL(k_fill_double, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
 }
+__global__ void k_fill_u64(unsigned long long* p, size_t n, unsigned long long v) {
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = v;
+}
+void fill_u64(hipStream_t s, void* p, size_t n, unsigned long long v) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (unsigned long long*)p, n,
+                           v);
+}
 
 // ---------------------------------------------------------------------------
 // Groups: runs of contacts with the same unordered slot pair (solve bodies are
@@ -245,29 +254,42 @@ __global__ void k_solve_order(int n_groups, const uint64_t* sorted, const int* g
 // Dataflow ranks: for every dynamic body a with deg_a incident groups, the rank
 // of each of its groups among them in sorted (colour-major) order.  flow[q] =
 // {rank_a, deg_a, rank_b, deg_b} for sorted group q with solve bodies A, B
-// (deg 0 for a pinned body: never waited on, never written).
+// (deg 0 for a pinned body: never waited on, never written).  pred[2 q + s]
+// is the incidence (2 q' + s') of the update of that body just before this
+// one: the group ranked one lower, or for rank 0 the body's last group, whose
+// update of the previous pass comes first.
 __global__ void k_flow_ranks(int n_ent, const int* bflags, const int* inc_start, const int* inc_count,
-                             const int* inc_items, const int* spos, const int* group_desc, int* flow) {
+                             const int* inc_items, const int* spos, const int* group_desc, int* flow, int* pred) {
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n_ent || !dynamic_body(bflags, a)) return;
     const int s = inc_start[a], d = inc_count[a];
+    auto incidence = [&](int q) { return 2 * q + (group_desc[4 * q + 2] == a ? 0 : 1); };
     for (int i = 0; i < d; i++) {
         const int qi = spos[inc_items[s + i]];
-        int r = 0;
-        for (int j = 0; j < d; j++) r += spos[inc_items[s + j]] < qi ? 1 : 0;
-        const int side = group_desc[4 * qi + 2] == a ? 0 : 2;
-        flow[4 * qi + side] = r;
-        flow[4 * qi + side + 1] = d;
+        int r = 0, before = -1, last = -1;
+        for (int j = 0; j < d; j++) {
+            const int qj = spos[inc_items[s + j]];
+            if (qj < qi) {
+                r++;
+                before = max(before, qj);
+            }
+            last = max(last, qj);
+        }
+        const int side = group_desc[4 * qi + 2] == a ? 0 : 1;
+        flow[4 * qi + 2 * side] = r;
+        flow[4 * qi + 2 * side + 1] = d;
+        pred[2 * qi + side] = incidence(before >= 0 ? before : last);
     }
 }
 
 void launch_flow_ranks(hipStream_t s, int n_ent, int n_groups, const int* bflags, const int* inc_start,
                        const int* inc_count, const int* inc_items, const int* spos, const int* group_desc,
-                       int* flow) {
+                       int* flow, int* pred) {
     fill_int(s, flow, 4 * n_groups, 0);
+    fill_int(s, pred, 2 * n_groups, 0);
     if (n_ent > 0 && n_groups > 0)
         hipLaunchKernelGGL(k_flow_ranks, dim3((n_ent + 255) / 256), dim3(256), 0, s, n_ent, bflags, inc_start,
-                           inc_count, inc_items, spos, group_desc, flow);
+                           inc_count, inc_items, spos, group_desc, flow, pred);
 }
 
 // The division (D lambda - omega (b + J.dv)) / (D + C) of a row.  The
@@ -836,6 +858,9 @@ hipError_t launch_pgs(hipStream_t s, const PgsArgs& P, int grid, int block, bool
 // are simply not selected.  The last sweep (contact states) waits for the
 // verdict on the one before it, so it never runs speculatively.
 
+AMD_INLINE double* slot_at(const FlowArgs& F, int parity, int incidence) {
+    return F.slot + ((size_t)parity * 2 * F.n_groups + incidence) * kSlotDoubles;
+}
 AMD_INLINE unsigned ld_cnt(const unsigned* p) {
     return __hip_atomic_load(const_cast<unsigned*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -874,7 +899,8 @@ constexpr long kFlowSpinLimit = 1L << 22;  // polls of one wait (each a memory r
 // Per-contact operands of one row: this lane's J entries and update
 // coefficients of the three rows, and the contact's uniform values.
 struct WaveRec {
-    double jc[6];
+    double jc[6];  // as stored: the A.v lanes flip the sign on use (J)
+    unsigned sign;
     double b[3], d[3], den[3], y[3], mu, clamp, lam[3];
 };
 // Where a lane's entries sit in a record: quads hold A.v, A.w, B.v, B.w;
@@ -902,11 +928,13 @@ AMD_INLINE const double* lam_src(const FlowArgs& F, int k, int p) {
     return from_alt ? &F.lam_alt[4 * k] : &F.P.rec[(size_t)k * kRecDoubles + kRecLambda];
 }
 // rec: the row's record (LDS or global); lam: its lambda for this pass
+AMD_INLINE double J(const WaveRec& c, int i) { return flip(c.jc[i], c.sign); }
 AMD_INLINE void read_wave_rec(const double* rec, const double* lam, const LaneMap& m, WaveRec& w) {
+    w.sign = m.sign;
 #pragma unroll
     for (int r = 0; r < 3; r++) {
-        w.jc[2 * r] = flip(rec[r * kRecRow + m.jofs], m.sign);
-        w.jc[2 * r + 1] = flip(rec[r * kRecRow + m.cofs], m.sign);
+        w.jc[2 * r] = rec[r * kRecRow + m.jofs];
+        w.jc[2 * r + 1] = rec[r * kRecRow + m.cofs];
         w.b[r] = rec[kRecB + r];
         w.d[r] = rec[kRecD + r];
         w.den[r] = rec[kRecDen + r];
@@ -931,9 +959,16 @@ struct DmaSlot {
 // A contact's new lambdas go to this pass's buffer; the last sweep also sets
 // its state (Contact.updateContactState, Contact.java:385-398).
 AMD_INLINE void contact_done(const FlowArgs& F, const WaveRec& c, int k, int p, bool last, bool lead,
-                             const double (&lam)[3], double dv) {
+                             const double (&lam)[3], double dv, double* lam_lds = nullptr) {
     const PgsArgs& P = F.P;
-    {
+    if (lam_lds) {
+        // held in LDS until the step has published its deltaV
+        if (lead) {
+            lam_lds[0] = lam[0];
+            lam_lds[1] = lam[1];
+            lam_lds[2] = lam[2];
+        }
+    } else {
         // every lane of the row stores the same values to the same place
         double* dst = (p & 1) ? &F.lam_alt[4 * k] : &P.rec[(size_t)k * kRecDoubles + kRecLambda];
         dst[0] = lam[0];
@@ -941,8 +976,8 @@ AMD_INLINE void contact_done(const FlowArgs& F, const WaveRec& c, int k, int p, 
         dst[2] = lam[2];
     }
     if (last) {
-        const double w1 = c.b[1] + row_jdv(c.jc[2] * dv);
-        const double w2 = c.b[2] + row_jdv(c.jc[4] * dv);
+        const double w1 = c.b[1] + row_jdv(J(c, 2) * dv);
+        const double w2 = c.b[2] + row_jdv(J(c, 4) * dv);
         if (lead) {
             int st;
             if (fabs(lam[0]) <= 1e-14) st = AM_STATE_BROKEN;
@@ -965,12 +1000,12 @@ AMD_INLINE void row_contact(const FlowArgs& F, const WaveRec& c, int k, int p, b
     double lam[3] = {c.lam[0], c.lam[1], c.lam[2]};
     if (p == 0) {
 #pragma unroll
-        for (int r = 0; r < 3; r++) dv = (f * lam[r]) * c.jc[2 * r + 1] + dv;
+        for (int r = 0; r < 3; r++) dv = (f * lam[r]) * J(c, 2 * r + 1) + dv;
         return;
     }
 #pragma unroll
     for (int r = 0; r < 3; r++) {
-        const double jd = row_jdv(c.jc[2 * r] * dv);
+        const double jd = row_jdv(J(c, 2 * r) * dv);
         const double prev = lam[r];
         const double num = c.d[r] * lam[r] - P.omega * (c.b[r] + jd);
         double l = div_row(num, c.den[r], c.y[r]);
@@ -995,7 +1030,7 @@ AMD_INLINE void row_contact(const FlowArgs& F, const WaveRec& c, int k, int p, b
         l = c.clamp != 0.0 ? lc : l;
         lam[r] = l;
         const double diff = l - prev;
-        dv = (f * diff) * c.jc[2 * r + 1] + dv;
+        dv = (f * diff) * J(c, 2 * r + 1) + dv;
         mx = fmax(mx, fabs(diff));
         nan = nan || diff != diff;
     }
@@ -1021,13 +1056,13 @@ struct FastCheck {
 };
 AMD_INLINE unsigned hi_word(double v) { return (unsigned)((unsigned long long)__double_as_longlong(v) >> 32); }
 AMD_INLINE void row_contact_fast(const FlowArgs& F, const WaveRec& c, int k, int p, bool last, bool lead, double f,
-                                 double& dv, double& mx, bool& nan, FastCheck& chk) {
+                                 double& dv, double& mx, bool& nan, FastCheck& chk, double* lam_lds) {
     const PgsArgs& P = F.P;
     double lam[3] = {c.lam[0], c.lam[1], c.lam[2]};
     chk.plain = chk.plain && c.clamp != 0.0;
 #pragma unroll
     for (int r = 0; r < 3; r++) {
-        const double jd = row_jdv(c.jc[2 * r] * dv);
+        const double jd = row_jdv(J(c, 2 * r) * dv);
         const double prev = lam[r];
         const double num = c.d[r] * lam[r] - P.omega * (c.b[r] + jd);
         const unsigned h = hi_word(num) & 0x7fffffffu;
@@ -1047,11 +1082,11 @@ AMD_INLINE void row_contact_fast(const FlowArgs& F, const WaveRec& c, int k, int
         }
         lam[r] = l;
         const double diff = l - prev;
-        dv = (f * diff) * c.jc[2 * r + 1] + dv;
+        dv = (f * diff) * J(c, 2 * r + 1) + dv;
         mx = fmax(mx, fabs(diff));
         nan = nan || diff != diff;
     }
-    contact_done(F, c, k, p, last, lead, lam, dv);
+    contact_done(F, c, k, p, last, lead, lam, dv, lam_lds);
 }
 
 // Steps: up to 4 consecutive sorted groups of one parallel colour (no shared
@@ -1089,6 +1124,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     const int4* gdesc = reinterpret_cast<const int4*>(P.group_desc);
     const int4* fdesc = reinterpret_cast<const int4*>(F.flow);
     __shared__ DmaSlot s_dma[kDmaSlots];
+    __shared__ double s_lam[kRowsPerWave][kDmaSlots][4];  // fast-update lambdas of the step
+    const int2* gpred = reinterpret_cast<const int2*>(F.pred);
     const LaneMap lmap = lane_map(li);
     int chk = 0;  // sweeps 1..chk are known not to have converged
 
@@ -1131,11 +1168,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         };
         bool nx_active = false;
         int4 nx_gd = make_int4(0, 0, 0, 0), nx_fd = make_int4(0, 0, 0, 0);
-        int nx_sc = 0;
+        int2 nx_pr = make_int2(0, 0);
+        int nx_sc = 0, nx_q = 0;
         if (w < n_steps) {
             const int q = step_group(w, nx_active);
+            nx_q = q;
             nx_gd = gdesc[q];
             nx_fd = fdesc[q];
+            nx_pr = gpred[q];
             if (P.per_scene) nx_sc = P.g_scene[q];
         }
         for (int st = w; st < n_steps; st += W) {
@@ -1143,11 +1183,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             const bool has_group = nx_active;
             const int4 gd = nx_gd;
             const int4 fd = nx_fd;
-            const int sc = nx_sc;
+            const int2 pr = nx_pr;
+            const int sc = nx_sc, qg = nx_q;
             if (st + W < n_steps) {
                 const int q = step_group(st + W, nx_active);
+                nx_q = q;
                 nx_gd = gdesc[q];
                 nx_fd = fdesc[q];
+                nx_pr = gpred[q];
                 if (P.per_scene) nx_sc = P.g_scene[q];
             }
             // per-scene exits: a row whose scene was under the tolerance at
@@ -1221,19 +1264,38 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                     }
                 }
             }
-            // wait until both bodies of every row's group have received every
-            // earlier update (lanes 0 and 1 of a row poll A's and B's counter)
+            // Each lane of a dynamic body takes its deltaV component from the
+            // slot of that body's previous update (this pass's, or for the
+            // body's first group the previous pass's last): it polls until
+            // the slot is filled, then empties it for the slot's next writer.
+            // The body's first update of the launch, and pinned bodies, read
+            // the launch buffers instead.
+            const int e = side_b ? gd.w : gd.z;
+            const int rank = side_b ? fd.z : fd.x;
+            const int deg = side_b ? fd.w : fd.y;
+            double dv = 0.0, f = 1.0;
+            bool dyn = false;
+            double* src = nullptr;
+            if (active && mine) {
+                if (vlane) f = P.minv[e];
+                dyn = deg != 0;
+                if (dyn && !(p == first && rank == 0))
+                    src = slot_at(F, rank == 0 ? ((p - 1) & 1) : (p & 1), side_b ? pr.y : pr.x) + comp;
+                else
+                    dv = ld_coherent(&prv[e * 6 + comp]);
+            }
             {
-                const bool lb = li == 1;
-                const unsigned tgt = lb ? (unsigned)p * (unsigned)fd.w + (unsigned)fd.z
-                                        : (unsigned)p * (unsigned)fd.y + (unsigned)fd.x;
-                const bool need = active && li < 2 && (lb ? fd.w != 0 : fd.y != 0);
-                const unsigned* ctr = &F.cnt[lb ? gd.w : gd.z];
+                bool got = src == nullptr;
                 long spins = 0;
                 while (true) {
-                    unsigned v = tgt;
-                    if (need) v = ld_cnt(ctr);
-                    if (__all(v == tgt)) break;
+                    if (!got) {
+                        const double v = ld_coherent(src);
+                        if ((unsigned long long)__double_as_longlong(v) != kSlotEmpty) {
+                            dv = v;
+                            got = true;
+                        }
+                    }
+                    if (__all(got)) break;
                     __builtin_amdgcn_s_sleep(1);
                     if (++spins > kFlowSpinLimit ||
                         ((spins & 255) == 0 && uniform(__hip_atomic_load(P.error, __ATOMIC_RELAXED,
@@ -1242,15 +1304,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                         return;
                     }
                 }
-            }
-            const int e = side_b ? gd.w : gd.z;
-            const int rank = side_b ? fd.z : fd.x;
-            double dv = 0.0, f = 1.0;
-            bool dyn = false;
-            if (active && mine) {
-                dv = ld_coherent(&(rank == 0 ? prv : cur)[e * 6 + comp]);
-                if (vlane) f = P.minv[e];
-                dyn = (P.bflags[e] & AM_BODY_PINNED) == 0;
+                if (src) st_coherent(src, __longlong_as_double((long long)kSlotEmpty));
             }
             const bool last = p == lastp;
             __builtin_amdgcn_s_waitcnt(0);  // LDS copies landed
@@ -1258,28 +1312,42 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             // (groups of more than kDmaSlots contacts) read global memory
             double smx = 0.0;  // this step's max |dl| and NaN flag (row-uniform)
             bool snan = false;
-            auto read_rec = [&](int t, WaveRec& rc) {
-                if (t < kDmaSlots) {
-                    const double* rl = reinterpret_cast<const double*>(&s_dma[t].rec[row >> 1][(row & 1) * 32]);
-                    const double* ll = odd_lam ? reinterpret_cast<const double*>(&s_dma[t].lam[row * 16])
-                                               : rl + kRecLambda;
-                    read_wave_rec(rl, ll, lmap, rc);
-                } else {
-                    const int k = gd.x + min(t, max(n - 1, 0));
-                    const double* rg = P.rec + (size_t)k * kRecDoubles;
-                    read_wave_rec(rg, lam_src(F, k, p), lmap, rc);
-                }
+            // Contacts in the LDS slots and the rest (read from global
+            // memory) run in separate loops, so that the first one waits on
+            // LDS reads only and never on the lambda stores in flight.
+            const int in_lds = min(nmax, kDmaSlots);
+            auto lds_rec = [&](int t, WaveRec& rc) {
+                const double* rl = reinterpret_cast<const double*>(&s_dma[t].rec[row >> 1][(row & 1) * 32]);
+                const double* ll = odd_lam ? reinterpret_cast<const double*>(&s_dma[t].lam[row * 16]) : rl + kRecLambda;
+                read_wave_rec(rl, ll, lmap, rc);
             };
-            // sweeps try the fast update first; the lambdas it stored are
-            // rewritten (same places, same lanes) if the step runs again
+            auto global_rec = [&](int t, WaveRec& rc) {
+                const int k = gd.x + min(t, max(n - 1, 0));
+                read_wave_rec(P.rec + (size_t)k * kRecDoubles, lam_src(F, k, p), lmap, rc);
+            };
+            // sweeps try the fast update first, which keeps the lambdas of the
+            // contacts in the LDS slots in LDS; the general update stores its
+            // lambdas at once (over the few the fast one stored directly, in
+            // the same places from the same lanes, if the step runs again)
             bool general = p == 0;
             if (!general) {
                 const double dv_start = dv;
                 FastCheck fc;
-                for (int t = 0; t < nmax; t++) {
-                    WaveRec rc;
-                    read_rec(t, rc);
-                    if (t < n) row_contact_fast(F, rc, gd.x + t, p, last, lead, f, dv, smx, snan, fc);
+                // the next contact's record is read while this one runs
+                // (two buffers used in turn, so nothing is copied)
+                WaveRec ra, rb;
+                if (in_lds > 0) lds_rec(0, ra);
+                for (int t = 0; t < in_lds; t += 2) {
+                    if (t + 1 < in_lds) lds_rec(t + 1, rb);
+                    if (t < n) row_contact_fast(F, ra, gd.x + t, p, last, lead, f, dv, smx, snan, fc, s_lam[row][t]);
+                    if (t + 1 >= in_lds) break;
+                    if (t + 2 < in_lds) lds_rec(t + 2, ra);
+                    if (t + 1 < n)
+                        row_contact_fast(F, rb, gd.x + t + 1, p, last, lead, f, dv, smx, snan, fc, s_lam[row][t + 1]);
+                }
+                for (int t = kDmaSlots; t < nmax; t++) {
+                    global_rec(t, ra);
+                    if (t < n) row_contact_fast(F, ra, gd.x + t, p, last, lead, f, dv, smx, snan, fc, nullptr);
                 }
                 if (__builtin_expect(!__all(fc.ok()), 0)) {
                     general = true;
@@ -1290,9 +1358,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 }
             }
             if (general) {
-                for (int t = 0; t < nmax; t++) {
+                for (int t = 0; t < in_lds; t++) {
                     WaveRec rc;
-                    read_rec(t, rc);
+                    lds_rec(t, rc);
+                    if (t < n) row_contact(F, rc, gd.x + t, p, last, lead, f, dv, smx, snan);
+                }
+                for (int t = kDmaSlots; t < nmax; t++) {
+                    WaveRec rc;
+                    global_rec(t, rc);
                     if (t < n) row_contact(F, rc, gd.x + t, p, last, lead, f, dv, smx, snan);
                 }
             }
@@ -1308,14 +1381,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 __hip_atomic_fetch_add(&P.scene_done[(size_t)sc * (P.iterations + 1) + p], 1u, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
             }
-            // publish deltaV, then the counters (lanes 0 and 1 of a row) as
-            // soon as the stores are complete.  Nothing else is in flight
-            // here: the next step's record copies are issued only after this
-            // wait, so they do not delay the counters.
-            if (active && mine && dyn) st_coherent(&cur[e * 6 + comp], dv);
+            // Publish deltaV to this group's slots.  The emptied slot must be
+            // seen empty before these values can be seen (the chain of
+            // updates they start leads to that slot's next writer), so the
+            // stores wait for this wave's earlier memory operations; with the
+            // fast update's lambdas still in LDS, only that store is left.
+            // The body's last update of the pass also goes to the pass's
+            // buffer, which holds the result when this is the last sweep.
             __builtin_amdgcn_s_waitcnt(0);
-            if (active && li == 0 && fd.y) st_cnt(&F.cnt[gd.z], (unsigned)p * (unsigned)fd.y + (unsigned)fd.x + 1u);
-            if (active && li == 1 && fd.w) st_cnt(&F.cnt[gd.w], (unsigned)p * (unsigned)fd.w + (unsigned)fd.z + 1u);
+            if (active && mine && dyn) {
+                st_coherent(slot_at(F, p & 1, 2 * qg + (side_b ? 1 : 0)) + comp, dv);
+                if (rank == deg - 1) st_coherent(&cur[e * 6 + comp], dv);
+            }
+            if (!general && active && li < 3) {
+                const int nl = min(n, kDmaSlots);
+                for (int t = 0; t < nl; t++) {
+                    const int k = gd.x + t;
+                    double* dst = (p & 1) ? &F.lam_alt[4 * k] : &P.rec[(size_t)k * kRecDoubles + kRecLambda];
+                    dst[li] = s_lam[row][t][li];
+                }
+            }
         }
         if (p >= 1) {
             if (__any(nan)) mx = __longlong_as_double(0x7ff8000000000000ll);

@@ -66,10 +66,18 @@ struct PgsArgs {
 };
 
 // Inputs of the dataflow PGS kernel (the same sweeps without grid barriers).
+// A group's updated deltaV of each dynamic body goes to its own slot
+// (kSlotDoubles doubles per incidence 2 q + side, one set per pass parity);
+// the body's next update polls that slot until it is no longer empty, takes
+// the value and empties the slot again.
+constexpr int kSlotDoubles = 8;
+constexpr unsigned long long kSlotEmpty = 0xFFF5A5A5A5A5A5A5ull;  // a signalling NaN: no arithmetic result
 struct FlowArgs {
     PgsArgs P;
     const int* flow;       // int4 per sorted group: rank_a, deg_a, rank_b, deg_b
-    unsigned* cnt;         // per entity: group updates applied (zeroed before launch)
+    const int* pred;       // int2 per sorted group: incidence of the previous update of A, of B
+    double* slot;          // 2 parities x 2 n_groups incidences x kSlotDoubles (all empty at launch)
+    int n_groups;
     double* dv_alt;        // deltaV of odd passes (a copy of dv at launch)
     double* lam_alt;       // lambda of odd sweeps, 4 per sweep position (3 used)
     unsigned* pass_done;   // per pass: waves that have finished it (zeroed before launch)
@@ -78,6 +86,7 @@ struct FlowArgs {
 
 void fill_int(hipStream_t s, int* p, int n, int v);
 void fill_double(hipStream_t s, double* p, int n, double v);
+void fill_u64(hipStream_t s, void* p, size_t n, unsigned long long v);
 void launch_groups(hipStream_t s, int cap, const int* n_contacts, const int* cb1, const int* cb2, const int* slot,
                    const int* parent, int* flags, int* scanned, int* group_start, int* group_a, int* group_b,
                    int* d_n_groups, void* scan_temp);
@@ -103,7 +112,7 @@ int pgs_flow_max_blocks(bool* ok);
 hipError_t launch_pgs_flow(hipStream_t s, const FlowArgs& F, int grid, bool plain, int n_dv, const BodiesView& B);
 void launch_flow_ranks(hipStream_t s, int n_ent, int n_groups, const int* bflags, const int* inc_start,
                        const int* inc_count, const int* inc_items, const int* spos, const int* group_desc,
-                       int* flow);
+                       int* flow, int* pred);
 void launch_map_insert(hipStream_t s, const OldContactsView& O, int* table);
 void launch_warm_start(hipStream_t s, BodiesView B, ContactsView C, OldContactsView O, const int* n_groups, int cap,
                        const int* group_start, int post, int* counters);
