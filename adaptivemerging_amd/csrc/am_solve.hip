@@ -1304,10 +1304,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                         return;
                     }
                 }
-                if (src) st_coherent(src, __longlong_as_double((long long)kSlotEmpty));
             }
             const bool last = p == lastp;
             __builtin_amdgcn_s_waitcnt(0);  // LDS copies landed
+            // empty the slot read above only now, so that this wait does not
+            // include the store (the publish below waits for it)
+            if (src) st_coherent(src, __longlong_as_double((long long)kSlotEmpty));
             // rows with fewer contacts idle; contacts past the LDS slots
             // (groups of more than kDmaSlots contacts) read global memory
             double smx = 0.0;  // this step's max |dl| and NaN flag (row-uniform)
