@@ -962,12 +962,11 @@ AMD_INLINE void contact_done(const FlowArgs& F, const WaveRec& c, int k, int p, 
                              const double (&lam)[3], double dv, double* lam_lds = nullptr) {
     const PgsArgs& P = F.P;
     if (lam_lds) {
-        // held in LDS until the step has published its deltaV
-        if (lead) {
-            lam_lds[0] = lam[0];
-            lam_lds[1] = lam[1];
-            lam_lds[2] = lam[2];
-        }
+        // held in LDS until the step has published its deltaV (every lane of
+        // the row writes the same values to the same place)
+        lam_lds[0] = lam[0];
+        lam_lds[1] = lam[1];
+        lam_lds[2] = lam[2];
     } else {
         // every lane of the row stores the same values to the same place
         double* dst = (p & 1) ? &F.lam_alt[4 * k] : &P.rec[(size_t)k * kRecDoubles + kRecLambda];
@@ -1073,7 +1072,9 @@ AMD_INLINE void row_contact_fast(const FlowArgs& F, const WaveRec& c, int k, int
         const double rr = __builtin_fma(-c.den[r], q0, num);
         double l = __builtin_fma(rr, c.y[r], q0);
         if (r == 0) {
-            l = l <= 0.0 ? 0.0 : l;
+            // l is finite and nonzero here (nonzero finite quotient), so
+            // max(l, +0) is Math.max(0, l)
+            l = __builtin_fmax(l, 0.0);
         } else {
             const double limit = c.mu * lam[0];
             chk.limit_hi_max = max(chk.limit_hi_max, hi_word(limit));
@@ -1083,9 +1084,11 @@ AMD_INLINE void row_contact_fast(const FlowArgs& F, const WaveRec& c, int k, int
         lam[r] = l;
         const double diff = l - prev;
         dv = (f * diff) * J(c, 2 * r + 1) + dv;
+        // finite whenever the step's conditions hold (a NaN or infinite
+        // numerator fails them), so no NaN is tracked here
         mx = fmax(mx, fabs(diff));
-        nan = nan || diff != diff;
     }
+    (void)nan;
     contact_done(F, c, k, p, last, lead, lam, dv, lam_lds);
 }
 
