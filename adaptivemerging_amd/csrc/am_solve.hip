@@ -453,13 +453,17 @@ __global__ void k_setup(SetupArgs S) {
     for (int r = 0; r < 3; r++) rec[kRecLambda + r] = C.lam[c * 3 + r];
     bool clamp = !(B.flags[b1] & AM_BODY_MAGNET_ACTIVE) && !(B.flags[b2] & AM_BODY_MAGNET_ACTIVE);
     rec[kRecAB] = __longlong_as_double((long long)(((uint64_t)(uint32_t)eb << 32) | (uint32_t)ea));
-    rec[kRecClamp] = clamp ? 1.0 : 0.0;
+    bool recips = true;
 #pragma unroll
     for (int r = 0; r < 3; r++) {
         const double den = rec[kRecD + r] + S.compliance;
         rec[kRecDen + r] = den;
         rec[kRecY + r] = div_recip(den);
+        recips = recips && rec[kRecY + r] != 0.0;
     }
+    // 0: not clamped; 1: clamped, every row's reciprocal stored (the
+    // dataflow kernel's fast update may take it); 2: clamped otherwise
+    rec[kRecClamp] = !clamp ? 0.0 : recips ? 1.0 : 2.0;
     rec[kRecDoubles - 1] = 0.0;
     reinterpret_cast<int4*>(S.meta)[k] = make_int4(ea, eb, clamp ? 1 : 0, 0);
     S.sstate[k] = C.state[c];
@@ -958,22 +962,12 @@ struct DmaSlot {
 
 // A contact's new lambdas go to this pass's buffer; the last sweep also sets
 // its state (Contact.updateContactState, Contact.java:385-398).
-AMD_INLINE void contact_done(const FlowArgs& F, const WaveRec& c, int k, int p, bool last, bool lead,
-                             const double (&lam)[3], double dv, double* lam_lds = nullptr) {
+AMD_INLINE double* lam_dst(const FlowArgs& F, int k, int p) {
+    return (p & 1) ? &F.lam_alt[4 * k] : &F.P.rec[(size_t)k * kRecDoubles + kRecLambda];
+}
+AMD_INLINE void contact_state(const FlowArgs& F, const WaveRec& c, int k, bool last, bool lead,
+                              const double (&lam)[3], double dv) {
     const PgsArgs& P = F.P;
-    if (lam_lds) {
-        // held in LDS until the step has published its deltaV (every lane of
-        // the row writes the same values to the same place)
-        lam_lds[0] = lam[0];
-        lam_lds[1] = lam[1];
-        lam_lds[2] = lam[2];
-    } else {
-        // every lane of the row stores the same values to the same place
-        double* dst = (p & 1) ? &F.lam_alt[4 * k] : &P.rec[(size_t)k * kRecDoubles + kRecLambda];
-        dst[0] = lam[0];
-        dst[1] = lam[1];
-        dst[2] = lam[2];
-    }
     if (last) {
         const double w1 = c.b[1] + row_jdv(J(c, 2) * dv);
         const double w2 = c.b[2] + row_jdv(J(c, 4) * dv);
@@ -986,6 +980,15 @@ AMD_INLINE void contact_done(const FlowArgs& F, const WaveRec& c, int k, int p, 
             P.sstate[k] = st;
         }
     }
+}
+AMD_INLINE void contact_done(const FlowArgs& F, const WaveRec& c, int k, int p, bool last, bool lead,
+                             const double (&lam)[3], double dv) {
+    // every lane of the row stores the same values to the same place
+    double* dst = lam_dst(F, k, p);
+    dst[0] = lam[0];
+    dst[1] = lam[1];
+    dst[2] = lam[2];
+    contact_state(F, c, k, last, lead, lam, dv);
 }
 
 // One contact of a row-held group in pass p (PGS.java:100-184; pass 0 is
@@ -1040,7 +1043,8 @@ AMD_INLINE void row_contact(const FlowArgs& F, const WaveRec& c, int k, int p, b
 // forms: every row's |numerator| in [2^-250, 2^250) with a nonzero stored
 // reciprocal (the quotient is then q0 = n y, r = fma(-d, q0, n), q = fma(r, y,
 // q0), exactly the division), every friction limit a finite value >= +0 (the
-// plain selects are then Math.max/Math.min), and every contact clamped.  The
+// plain selects are then Math.max/Math.min), and every contact clamped (its
+// record's clamp flag 1, which also says every row's reciprocal is stored).  The
 // conditions are gathered per lane into FastCheck without a branch; the step
 // checks them once at its end and, if any failed, runs again from its start
 // with row_contact.
@@ -1048,17 +1052,19 @@ struct FastCheck {
     unsigned num_hi_max = 0;            // high words of |numerator|
     unsigned num_hi_min = 0xffffffffu;
     unsigned limit_hi_max = 0;          // high words of the friction limits (sign included)
-    bool plain = true;                  // stored reciprocals nonzero, contacts clamped
+    bool plain = true;                  // every contact's clamp flag 1
     AMD_INLINE bool ok() const {
         return plain && num_hi_min >= 0x30500000u && num_hi_max < 0x4F900000u && limit_hi_max < 0x7FF00000u;
     }
 };
 AMD_INLINE unsigned hi_word(double v) { return (unsigned)((unsigned long long)__double_as_longlong(v) >> 32); }
 AMD_INLINE void row_contact_fast(const FlowArgs& F, const WaveRec& c, int k, int p, bool last, bool lead, double f,
-                                 double& dv, double& mx, bool& nan, FastCheck& chk, double* lam_lds) {
+                                 double& dv, double& mx, FastCheck& chk, double (&lam)[3]) {
     const PgsArgs& P = F.P;
-    double lam[3] = {c.lam[0], c.lam[1], c.lam[2]};
-    chk.plain = chk.plain && c.clamp != 0.0;
+    lam[0] = c.lam[0];
+    lam[1] = c.lam[1];
+    lam[2] = c.lam[2];
+    chk.plain = chk.plain && c.clamp == 1.0;
 #pragma unroll
     for (int r = 0; r < 3; r++) {
         const double jd = row_jdv(J(c, 2 * r) * dv);
@@ -1067,7 +1073,6 @@ AMD_INLINE void row_contact_fast(const FlowArgs& F, const WaveRec& c, int k, int
         const unsigned h = hi_word(num) & 0x7fffffffu;
         chk.num_hi_max = max(chk.num_hi_max, h);
         chk.num_hi_min = min(chk.num_hi_min, h);
-        chk.plain = chk.plain && c.y[r] != 0.0;
         const double q0 = num * c.y[r];
         const double rr = __builtin_fma(-c.den[r], q0, num);
         double l = __builtin_fma(rr, c.y[r], q0);
@@ -1088,8 +1093,7 @@ AMD_INLINE void row_contact_fast(const FlowArgs& F, const WaveRec& c, int k, int
         // numerator fails them), so no NaN is tracked here
         mx = fmax(mx, fabs(diff));
     }
-    (void)nan;
-    contact_done(F, c, k, p, last, lead, lam, dv, lam_lds);
+    contact_state(F, c, k, last, lead, lam, dv);
 }
 
 // Steps: up to 4 consecutive sorted groups of one parallel colour (no shared
@@ -1342,17 +1346,34 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 // (two buffers used in turn, so nothing is copied)
                 WaveRec ra, rb;
                 if (in_lds > 0) lds_rec(0, ra);
+                double lam[3];
                 for (int t = 0; t < in_lds; t += 2) {
                     if (t + 1 < in_lds) lds_rec(t + 1, rb);
-                    if (t < n) row_contact_fast(F, ra, gd.x + t, p, last, lead, f, dv, smx, snan, fc, s_lam[row][t]);
+                    if (t < n) {
+                        row_contact_fast(F, ra, gd.x + t, p, last, lead, f, dv, smx, fc, lam);
+                        // every lane of the row writes the same values to the same place
+                        s_lam[row][t][0] = lam[0];
+                        s_lam[row][t][1] = lam[1];
+                        s_lam[row][t][2] = lam[2];
+                    }
                     if (t + 1 >= in_lds) break;
                     if (t + 2 < in_lds) lds_rec(t + 2, ra);
-                    if (t + 1 < n)
-                        row_contact_fast(F, rb, gd.x + t + 1, p, last, lead, f, dv, smx, snan, fc, s_lam[row][t + 1]);
+                    if (t + 1 < n) {
+                        row_contact_fast(F, rb, gd.x + t + 1, p, last, lead, f, dv, smx, fc, lam);
+                        s_lam[row][t + 1][0] = lam[0];
+                        s_lam[row][t + 1][1] = lam[1];
+                        s_lam[row][t + 1][2] = lam[2];
+                    }
                 }
                 for (int t = kDmaSlots; t < nmax; t++) {
                     global_rec(t, ra);
-                    if (t < n) row_contact_fast(F, ra, gd.x + t, p, last, lead, f, dv, smx, snan, fc, nullptr);
+                    if (t < n) {
+                        row_contact_fast(F, ra, gd.x + t, p, last, lead, f, dv, smx, fc, lam);
+                        double* dst = lam_dst(F, gd.x + t, p);
+                        dst[0] = lam[0];
+                        dst[1] = lam[1];
+                        dst[2] = lam[2];
+                    }
                 }
                 if (__builtin_expect(!__all(fc.ok()), 0)) {
                     general = true;
@@ -1401,9 +1422,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             if (!general && active && li < 3) {
                 const int nl = min(n, kDmaSlots);
                 for (int t = 0; t < nl; t++) {
-                    const int k = gd.x + t;
-                    double* dst = (p & 1) ? &F.lam_alt[4 * k] : &P.rec[(size_t)k * kRecDoubles + kRecLambda];
-                    dst[li] = s_lam[row][t][li];
+                    lam_dst(F, gd.x + t, p)[li] = s_lam[row][t][li];
                 }
             }
         }
