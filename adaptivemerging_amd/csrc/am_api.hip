@@ -298,9 +298,10 @@ int rebuild_structure(am_ctx* c) {
     AM_HIP(c, c->d_bucket_start.ensure(tsize + 1));
     AM_HIP(c, c->d_bucket_cursor.ensure(tsize));
     AM_HIP(c, c->d_bucket_items.ensure(std::max(ng, 1)));
-    AM_HIP(c, c->d_pair_counts.ensure(std::max(ng, 1) + 1));
-    AM_HIP(c, c->d_pair_offsets.ensure(std::max(ng, 1) + 1));
-    int r = ensure_scan_temp(c, std::max(tsize, ng) + 1);
+    const int nt = pair_tasks(std::max(ng, 1));
+    AM_HIP(c, c->d_pair_counts.ensure((size_t)nt + 1));
+    AM_HIP(c, c->d_pair_offsets.ensure((size_t)nt + 1));
+    int r = ensure_scan_temp(c, std::max(tsize, nt) + 1);
     if (r) return r;
     // incidence arrays are per entity
     AM_HIP(c, c->inc_count.ensure(n + 1));
@@ -525,7 +526,7 @@ int do_detect(am_ctx* c) {
     A.n_large = (int)c->grid_large.size();
     fill_int(s, sc + 4, 1, 0);
     launch_pairs_impl(s, false, A, c->d_pair_counts.p, nullptr, nullptr, 0, sc + 4);
-    exclusive_scan(s, c->d_pair_counts.p, c->d_pair_offsets.p, ng, sc + 0, c->d_scan_temp.p);
+    exclusive_scan(s, c->d_pair_counts.p, c->d_pair_offsets.p, pair_tasks(ng), sc + 0, c->d_scan_temp.p);
     AM_HIP(c, hipGetLastError());
     int r = read_scalars(c, 0, 1);
     if (r) return r;

@@ -122,107 +122,109 @@ AMD_INLINE bool skip_pair(const PairArgs& A, int slot_a, int slot_b) {
     return false;
 }
 
-// One thread per gridded body: count (write=false) or write its pairs.  A body
-// in the grid emits the pairs with grid bodies of a higher slot found in the
-// 27 cells around its own, the pairs with every large body (those stay out of
-// the grid, whose cells are sized for the others), and all its plane pairs; a
-// large body emits its pairs with large bodies of a higher slot and with the
-// planes.  The keys are sorted afterwards, so the emission order is free.
+// One thread per (gridded body, task): count (write=false) or write pairs.
+// Tasks 0-26 are the 27 cells around the body's own: the pairs with grid
+// bodies of a higher slot found there.  Task 27 holds the pairs with every
+// large body (those stay out of the grid, whose cells are sized for the
+// others) and all the plane pairs; a large body has only task 27, where it
+// pairs with large bodies of a higher slot and with the planes.  Spreading a
+// body's cells over threads keeps each thread's chain of dependent loads
+// short.  The keys are sorted afterwards, so the emission order is free.
+constexpr int kPairTasks = 28;
 template <bool kWrite>
 __global__ void k_pairs(PairArgs A, int* counts, const int* offsets, uint64_t* keys, int cap, int* overflow) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= A.n_grid) return;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)A.n_grid * kPairTasks) return;
+    const int k = (int)(t / kPairTasks), task = (int)(t % kPairTasks);
     const BodiesView& B = A.B;
-    int bi = A.grid_order[k];
-    int slot_i = B.slot[bi], sub_i = B.sub[bi];
+    const int bi = A.grid_order[k];
+    const int slot_i = B.slot[bi], sub_i = B.sub[bi];
     const am_body_desc& di = B.desc[bi];
-    double ci[4] = {A.sphere[k * 4 + 0], A.sphere[k * 4 + 1], A.sphere[k * 4 + 2], A.sphere[k * 4 + 3]};
-    int cx = A.cell[k * 4 + 0], cy = A.cell[k * 4 + 1], cz = A.cell[k * 4 + 2];
+    const double ci[4] = {A.sphere[k * 4 + 0], A.sphere[k * 4 + 1], A.sphere[k * 4 + 2], A.sphere[k * 4 + 3]};
     const bool large_i = A.cell[k * 4 + 3] < 0;
     int n = 0;
-    int base = kWrite ? offsets[k] : 0;
-    for (int q = 0; q < A.n_large; q++) {
-        int kj = A.large[q];
-        if (kj == k) continue;
-        int bj = A.grid_order[kj];
-        if (B.desc[bj].scene != di.scene) continue;
-        int slot_j = B.slot[bj];
-        if (slot_j == slot_i || (large_i && slot_j < slot_i)) continue;
-        if (skip_pair(A, slot_i, slot_j)) continue;
-        double ddx = A.sphere[kj * 4 + 0] - ci[0];
-        double ddy = A.sphere[kj * 4 + 1] - ci[1];
-        double ddz = A.sphere[kj * 4 + 2] - ci[2];
-        double rr = A.sphere[kj * 4 + 3] + ci[3];
-        double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-        if (d2 > rr * rr * (1.0 + 1e-9) + 1e-300) continue;
+    const int base = kWrite ? offsets[t] : 0;
+    auto emit = [&](int slot_j, int sub_j) {
         if (kWrite) {
-            int pos = base + n;
-            if (pos < cap) keys[pos] = pair_key(slot_i, sub_i, slot_j, B.sub[bj]);
+            const int pos = base + n;
+            if (pos < cap) keys[pos] = pair_key(slot_i, sub_i, slot_j, sub_j);
             else atomicOr(overflow, 1);
         }
         n++;
-    }
-    // a large body scans no cells: every grid body pairs with it above
-    for (int dz = -1; dz <= (large_i ? -2 : 1); dz++)
-        for (int dy = -1; dy <= 1; dy++)
-            for (int dx = -1; dx <= 1; dx++) {
-                int nx = cx + dx, ny = cy + dy, nz = cz + dz;
-                int h = cell_hash(nx, ny, nz, di.scene, A.mask);
-                int s0 = A.bucket_start[h], cnt = A.bucket_count[h];
-                for (int q = 0; q < cnt; q++) {
-                    int kj = A.bucket_items[s0 + q];
-                    if (A.cell[kj * 4 + 0] != nx || A.cell[kj * 4 + 1] != ny || A.cell[kj * 4 + 2] != nz) continue;
-                    int bj = A.grid_order[kj];
-                    if (B.desc[bj].scene != di.scene) continue;
-                    int slot_j = B.slot[bj];
-                    if (slot_j <= slot_i) continue;
-                    if (skip_pair(A, slot_i, slot_j)) continue;
-                    double ddx = A.sphere[kj * 4 + 0] - ci[0];
-                    double ddy = A.sphere[kj * 4 + 1] - ci[1];
-                    double ddz = A.sphere[kj * 4 + 2] - ci[2];
-                    double rr = A.sphere[kj * 4 + 3] + ci[3];
-                    double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                    // superset of every narrow-phase first test (<= with slack)
-                    if (d2 > rr * rr * (1.0 + 1e-9) + 1e-300) continue;
-                    if (kWrite) {
-                        int pos = base + n;
-                        if (pos < cap) keys[pos] = pair_key(slot_i, sub_i, slot_j, B.sub[bj]);
-                        else atomicOr(overflow, 1);
-                    }
-                    n++;
-                }
+    };
+    if (task < 27) {
+        // a large body scans no cells: every grid body pairs with it in its task 27
+        if (!large_i) {
+            const int nx = A.cell[k * 4 + 0] + task % 3 - 1;
+            const int ny = A.cell[k * 4 + 1] + (task / 3) % 3 - 1;
+            const int nz = A.cell[k * 4 + 2] + task / 9 - 1;
+            const int h = cell_hash(nx, ny, nz, di.scene, A.mask);
+            const int s0 = A.bucket_start[h], cnt = A.bucket_count[h];
+            for (int q = 0; q < cnt; q++) {
+                const int kj = A.bucket_items[s0 + q];
+                if (A.cell[kj * 4 + 0] != nx || A.cell[kj * 4 + 1] != ny || A.cell[kj * 4 + 2] != nz) continue;
+                const int bj = A.grid_order[kj];
+                if (B.desc[bj].scene != di.scene) continue;
+                const int slot_j = B.slot[bj];
+                if (slot_j <= slot_i) continue;
+                if (skip_pair(A, slot_i, slot_j)) continue;
+                const double ddx = A.sphere[kj * 4 + 0] - ci[0];
+                const double ddy = A.sphere[kj * 4 + 1] - ci[1];
+                const double ddz = A.sphere[kj * 4 + 2] - ci[2];
+                const double rr = A.sphere[kj * 4 + 3] + ci[3];
+                const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                // superset of every narrow-phase first test (<= with slack)
+                if (d2 > rr * rr * (1.0 + 1e-9) + 1e-300) continue;
+                emit(slot_j, B.sub[bj]);
             }
-    for (int q = 0; q < A.n_planes; q++) {
-        int p = A.planes[q];
-        const am_body_desc& pd = B.desc[p];
-        if (pd.scene != di.scene) continue;
-        int slot_p = B.slot[p];
-        if (slot_p == slot_i) continue;
-        if (skip_pair(A, slot_i, slot_p)) continue;
-        d3 nrm = ld3(pd.plane_n);
-        if (di.geom == AM_GEOM_BOX || di.geom == AM_GEOM_COMPOSITE) {
-            // first test of BoxPlane.dBoxPlane (for a composite: its enclosing sphere)
-            if (dot(ld3(&B.x[bi * 3]), nrm) + pd.plane_d > di.radius) continue;
-        } else {
-            // first (root) test of collideSphereTreeAndPlane
-            d3 c = ld3(&A.T.cw[di.tree_first * 3]);
-            double d = nrm.x * c.x + nrm.y * c.y + nrm.z * c.z + pd.plane_d - A.T.nodes[di.tree_first].r;
-            if (!(d < 0)) continue;
         }
-        if (kWrite) {
-            int pos = base + n;
-            if (pos < cap) keys[pos] = pair_key(slot_i, sub_i, slot_p, B.sub[p]);
-            else atomicOr(overflow, 1);
+    } else {
+        for (int q = 0; q < A.n_large; q++) {
+            const int kj = A.large[q];
+            if (kj == k) continue;
+            const int bj = A.grid_order[kj];
+            if (B.desc[bj].scene != di.scene) continue;
+            const int slot_j = B.slot[bj];
+            if (slot_j == slot_i || (large_i && slot_j < slot_i)) continue;
+            if (skip_pair(A, slot_i, slot_j)) continue;
+            const double ddx = A.sphere[kj * 4 + 0] - ci[0];
+            const double ddy = A.sphere[kj * 4 + 1] - ci[1];
+            const double ddz = A.sphere[kj * 4 + 2] - ci[2];
+            const double rr = A.sphere[kj * 4 + 3] + ci[3];
+            const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            if (d2 > rr * rr * (1.0 + 1e-9) + 1e-300) continue;
+            emit(slot_j, B.sub[bj]);
         }
-        n++;
+        for (int q = 0; q < A.n_planes; q++) {
+            const int p = A.planes[q];
+            const am_body_desc& pd = B.desc[p];
+            if (pd.scene != di.scene) continue;
+            const int slot_p = B.slot[p];
+            if (slot_p == slot_i) continue;
+            if (skip_pair(A, slot_i, slot_p)) continue;
+            const d3 nrm = ld3(pd.plane_n);
+            if (di.geom == AM_GEOM_BOX || di.geom == AM_GEOM_COMPOSITE) {
+                // first test of BoxPlane.dBoxPlane (for a composite: its enclosing sphere)
+                if (dot(ld3(&B.x[bi * 3]), nrm) + pd.plane_d > di.radius) continue;
+            } else {
+                // first (root) test of collideSphereTreeAndPlane
+                const d3 c = ld3(&A.T.cw[di.tree_first * 3]);
+                const double d = nrm.x * c.x + nrm.y * c.y + nrm.z * c.z + pd.plane_d - A.T.nodes[di.tree_first].r;
+                if (!(d < 0)) continue;
+            }
+            emit(slot_p, B.sub[p]);
+        }
     }
-    if (!kWrite) counts[k] = n;
+    if (!kWrite) counts[t] = n;
 }
+
+int pair_tasks(int n_grid) { return n_grid * kPairTasks; }
 
 void launch_pairs_impl(hipStream_t s, bool write, const PairArgs& A, int* counts, const int* offsets, uint64_t* keys,
                        int cap, int* overflow) {
     if (A.n_grid <= 0) return;
-    dim3 g((A.n_grid + 127) / 128), b(128);
+    const long n = (long)A.n_grid * kPairTasks;
+    dim3 g((unsigned)((n + 255) / 256)), b(256);
     if (write) hipLaunchKernelGGL(k_pairs<true>, g, b, 0, s, A, counts, offsets, keys, cap, overflow);
     else hipLaunchKernelGGL(k_pairs<false>, g, b, 0, s, A, counts, offsets, keys, cap, overflow);
 }

@@ -190,6 +190,7 @@ void launch_tree_world_centres(hipStream_t s, BodiesView B, TreeView T, const in
                                int n_nodes);
 void launch_bounds(hipStream_t s, BodiesView B, TreeView T, const int* order, int n, double* sphere,
                    double cell, int* cell_key, int hash_mask, const int* is_large);
+int pair_tasks(int n_grid);  // threads of the pair passes: one per (grid body, task)
 void launch_pairs_impl(hipStream_t s, bool write, const PairArgs& A, int* counts, const int* offsets, uint64_t* keys,
                        int cap, int* overflow);
 void launch_narrow(hipStream_t s, bool write, BodiesView B, TreeView T, const uint64_t* keys,
