@@ -19,7 +19,9 @@ Phases (only the last is timed):
      replays phases 1-2 (the GPU path is deterministic, checked) and runs the
      timed steps with the learned start and its plain-start fallback;
   5. timed: --steps steps with the plain warm start (the headline), then the
-     same steps from the same state with the learned start.
+     same steps from the same state with the learned start;
+  6. merging leg: the dominoes-on-platforms configuration with merging on,
+     timed on its own after its push (reported under config.merging).
 
 Prints one JSON line: contact-updates/s (N_contacts x sweeps / solve time)
 and ms/step of the plain run as the headline, both runs' figures (sweeps,
@@ -197,6 +199,52 @@ def settle_and_warm(ctx, args, params_kw, label):
     return st
 
 
+def run_merging_leg(args) -> dict:
+    """The dominoes-on-platforms configuration with merging on (the
+    reference's adaptive-merging path): 2,000 dominoes in 20 rows on 10
+    kinematic platforms, 50 sweeps at the default tolerance, merging and
+    unmerging at the reference defaults, sleeping off.  --merging-settle
+    untimed steps, the push on each row's first domino (5 m/s), then
+    --merging-steps timed steps through the start of the cascade."""
+    from adaptivemerging_amd import workloads
+    from adaptivemerging_amd.context import Context
+
+    scene = workloads.domino_platforms(seed=args.seed)
+    n_dominoes = int(np.sum((scene.arrays()[0]["flags"] & 1) == 0))
+    ctx = Context()
+    ctx.upload(scene)
+    ctx.set_params(gravity_amount=9.81, iterations=50, enable_sleeping=0,
+                   pgs_plain_launch=int(args.pgs_launch == "plain"))
+    t0 = time.perf_counter()
+    for _ in range(args.merging_settle):
+        ctx.step(args.dt)
+    for body, point, force in workloads.domino_push(scene, ctx.pull_state()[0], args.dt):
+        ctx.impulse_add(body, point, force)
+    stats, wall = run_steps(ctx, args.merging_steps, args.dt)
+    ctx.close()
+    out = {
+        "workload": f"{n_dominoes} dominoes (0.05 x 0.5 x 0.25 m) in 20 rows on 10 kinematic platforms, dt "
+                    f"{args.dt}, 50 sweeps, merging and unmerging on at the reference defaults; "
+                    f"{args.merging_settle} untimed steps, a 5 m/s push on each row's first domino, then "
+                    f"{args.merging_steps} timed steps",
+        "ms_per_step": wall * 1e3 / max(len(stats), 1),
+        "ms_merging": float(np.mean([s["ms_merging"] for s in stats])),
+        "ms_detect": float(np.mean([s["ms_detect"] for s in stats])),
+        "ms_solve": float(np.mean([s["ms_solve"] for s in stats])),
+        "contacts_mean": float(np.mean([s["n_contacts"] for s in stats])),
+        "internal_contacts_mean": float(np.mean([s["n_internal_contacts"] for s in stats])),
+        "collections_max": int(max(s["n_collections"] for s in stats)),
+        "merged": int(sum(s["n_merged"] for s in stats)),
+        "unmerged": int(sum(s["n_unmerged"] for s in stats)),
+        "sweeps_mean": float(np.mean([s["iterations_used"] for s in stats])),
+        "untimed_s": time.perf_counter() - t0 - wall,
+    }
+    print(f"# merging: {out['ms_per_step']:.2f} ms/step (merging {out['ms_merging']:.2f} ms), "
+          f"{out['contacts_mean']:.0f} contacts, {out['internal_contacts_mean']:.0f} internal, "
+          f"{out['merged']} merged, {out['unmerged']} unmerged", file=sys.stderr, flush=True)
+    return out
+
+
 def small_pile(args, k: int):
     """Scene k of the small piles the learned start is trained (k >= 0) and
     evaluated (k < 0) on: the configuration's boxes and densities, few
@@ -320,6 +368,10 @@ def main(argv=None) -> int:
     ap.add_argument("--train-steps", type=int, default=30)
     ap.add_argument("--train-iterations", type=int, default=3000)
     ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--merging-leg", choices=["on", "off"], default="on",
+                    help="also time the dominoes configuration with merging on (reported under config.merging)")
+    ap.add_argument("--merging-settle", type=int, default=300, help="untimed domino steps before the push")
+    ap.add_argument("--merging-steps", type=int, default=100, help="timed domino steps from the push on")
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
     ap.add_argument("--cpu-sweeps", type=int, default=40, help="sequential sweeps of the CPU baseline sample")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
@@ -414,6 +466,8 @@ def main(argv=None) -> int:
     print(f"# plain: {res_p['ms_per_step']:.2f} ms/step, {res_p['contacts_mean']:.0f} contacts, sweeps "
           f"{res_p['sweeps_mean']:.0f}, residual {res_p['residual_final_mean']:.3g}", file=sys.stderr, flush=True)
 
+    merging = run_merging_leg(args) if args.merging_leg == "on" else None
+
     cpu = None
     if args.cpu_baseline == "on":
         cpu = run_cpu_baseline(scene, plain.pull_state(), params_kw, args.dt, args.cpu_sweeps, args.cpu_budget)
@@ -485,6 +539,7 @@ def main(argv=None) -> int:
             "learned": res_l,
             "parallelism": "none (1 GPU)",
             "launch_shape": tuned,
+            "merging": merging,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -1,5 +1,6 @@
 """bench.py end to end at a small size: every phase (settle, warm-up, tuning,
-training on held-out piles, plain and learned timed steps, CPU baseline) runs
+training on held-out piles, plain and learned timed steps, the dominoes leg
+with merging on, CPU baseline) runs
 and the one JSON line carries the fields the measurement needs.  It runs in
 this process: a second process on the GPU would share it with the test
 runner's own contexts."""
@@ -23,7 +24,8 @@ def test_bench_small_end_to_end(gpu_lib):
 
     argv = ["--bodies", "400", "--settle", "40", "--warmup", "1", "--steps", "3", "--iterations", "200",
             "--train-scenes", "1", "--eval-scenes", "1", "--train-bodies", "40", "--train-settle", "40",
-            "--train-steps", "8", "--epochs", "2", "--cpu-sweeps", "5"]
+            "--train-steps", "8", "--epochs", "2", "--cpu-sweeps", "5", "--merging-settle", "20",
+            "--merging-steps", "5"]
     buf = io.StringIO()
     with contextlib.redirect_stdout(buf):
         rc = bench.main(argv)
@@ -39,3 +41,5 @@ def test_bench_small_end_to_end(gpu_lib):
     assert lrn["replay_identical"] is True
     assert lrn["held_out"]["sweeps_plain"] > 0 and lrn["held_out"]["sweeps_learned"] > 0
     assert rec["config"]["plain"]["contacts_mean"] > 0
+    mrg = rec["config"]["merging"]
+    assert mrg["ms_per_step"] > 0 and mrg["contacts_mean"] > 1000 and mrg["sweeps_mean"] > 0
