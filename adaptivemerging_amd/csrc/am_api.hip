@@ -692,13 +692,23 @@ int train_append(am_ctx* c) {
 }
 
 // Start residual of the lambdas currently in the records (sweep order).
+void pgs_shape(am_ctx* c, const am_params& P, int* grid, int* block);
+
 int start_residual(am_ctx* c, PgsArgs G, double* out) {
     hipStream_t s = c->stream;
     AM_HIP(c, hipMemcpyAsync(c->d_dv.p, c->dv_save.p, (size_t)c->n_ent * 48, hipMemcpyDeviceToDevice, s));
     G.iterations = 0;
     G.warm_start = 1;
     fill_int(s, c->d_barrier.p, 2, 0);
-    AM_HIP(c, launch_pgs(s, G, 1, 256, false));
+    // the warm-start pass alone, by the coloured kernel at its automatic
+    // shape (its bits do not depend on the shape)
+    am_params pc = c->params;
+    pc.pgs_schedule = 1;
+    pc.pgs_grid = 0;
+    pc.pgs_block = 0;
+    int grid = 1, block = 256;
+    pgs_shape(c, pc, &grid, &block);
+    AM_HIP(c, launch_pgs(s, G, grid, block, c->params.pgs_plain_launch != 0));
     AM_HIP(c, c->d_resid.ensure(1));
     fill_int(s, (int*)c->d_resid.p, 2, 0);
     launch_start_residual(s, c->s_rec.p, c->s_meta.p, c->d_dv.p, c->d_minv.p, c->d_jinv.p, c->n_contacts, G.omega,
