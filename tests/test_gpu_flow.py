@@ -95,3 +95,19 @@ def test_flow_groups_beyond_the_lds_slots(gpu_lib):
     assert it_g == it_o
     assert_contacts_equal(ctx.contacts(), orc.contacts())
     assert_bits_equal("deltaV", ctx.pull_deltav(), orc.pull_deltav())
+
+
+@pytest.mark.parametrize("schedule", [0, 1])
+def test_flow_unclamped_contacts_take_the_general_update(gpu_lib, schedule):
+    """Contacts of magnet-active bodies are not clamped (PGS.java:119-120).
+    The dataflow kernel's fast update covers clamped contacts only, so every
+    step holding one runs again with the general update; results stay
+    bit-exact against the oracle's sequential sweep, at every sweep."""
+    from adaptivemerging_amd._lib import AM_BODY_MAGNET_ACTIVE
+
+    scene = workloads.jenga_tower(seed=3, layers=8, overlap=0.002)
+    dyn = [i for i in range(scene.n) if not int(scene.bodies[i]["flags"]) & 1]
+    for i in dyn[::5]:
+        scene.bodies[i]["flags"] = int(scene.bodies[i]["flags"]) | AM_BODY_MAGNET_ACTIVE
+    for iterations in (1, 2, 9):
+        _solve_pair(scene, 7, iterations=iterations, tolerance=0.0, gravity_amount=9.81, pgs_schedule=schedule)
