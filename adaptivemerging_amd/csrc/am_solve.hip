@@ -1097,27 +1097,33 @@ AMD_INLINE void row_contact_fast(const FlowArgs& F, const WaveRec& c, int k, int
 }
 
 // Steps: up to 4 consecutive sorted groups of one parallel colour (no shared
-// dynamic body), one per 16-lane row of a wave; the sequential tail colour
-// gives one group per step.  Wave w runs steps w, w + W, ... of every pass in
-// order.
+// dynamic body), one per 16-lane row of a wave.  A colour with fewer groups
+// than there are waves, and the sequential tail colour, give one group per
+// step: a step starts when all its groups' inputs are there, and the long
+// chains of dependent updates run through the small late colours, where a
+// group would otherwise wait for its step-mates' inputs.  Wave w runs steps
+// w, w + W, ... of every pass in order.
 constexpr int kRowsPerWave = 4;
 
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_pgs_flow(FlowArgs F) {
     const PgsArgs& P = F.P;
     const int* cs = P.colour_start;
     __shared__ int s_sp[kMaxColours + 2];  // first step of each colour, then the total
+    __shared__ int s_per[kMaxColours + 1];  // groups per step of each colour
+    const int W = gridDim.x;
     if (threadIdx.x == 0) {
         int n = 0;
         for (int c = 0; c <= kMaxColours; c++) {
             s_sp[c] = n;
             const int g = cs[c + 1] - cs[c];
-            n += c < kMaxColours ? (g + kRowsPerWave - 1) / kRowsPerWave : g;
+            const int per = c < kMaxColours && g >= W ? kRowsPerWave : 1;
+            s_per[c] = per;
+            n += (g + per - 1) / per;
         }
         s_sp[kMaxColours + 1] = n;
     }
     __syncthreads();
     const int n_steps = s_sp[kMaxColours + 1];
-    const int W = gridDim.x;
     const int w = blockIdx.x;
     const int lane = threadIdx.x;
     const int row = lane >> 4, li = lane & 15, quad = li >> 2, jj = li & 3;
@@ -1167,7 +1173,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         // this row's group of a step (the colour cursor advances with st)
         auto step_group = [&](int st, bool& act) {
             while (s_sp[c + 1] <= st) c++;
-            const int per = c < kMaxColours ? kRowsPerWave : 1;
+            const int per = s_per[c];
             const int q0 = cs[c] + (st - s_sp[c]) * per;
             const int count = min(per, cs[c + 1] - q0);
             act = row < count;
