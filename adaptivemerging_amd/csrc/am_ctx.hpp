@@ -145,7 +145,8 @@ struct am_ctx {
     DBuf<double> s_rec, s_lam;
     DBuf<int> s_meta, s_state, d_barrier, s_group_desc, s_pos_a;
     DBuf<unsigned long long> s_iter_max;
-    // dataflow schedule: sorted position per group, ranks, counters, odd-pass buffers
+    // dataflow schedule: sorted position per group, ranks and degrees, previous-update
+    // incidences, pass completion counts, deltaV slots, odd-pass buffers
     DBuf<int> g_spos, s_flow, s_pred;
     DBuf<unsigned> d_pass_done;
     DBuf<double> f_slot;

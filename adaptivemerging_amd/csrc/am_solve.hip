@@ -835,25 +835,26 @@ hipError_t launch_pgs(hipStream_t s, const PgsArgs& P, int grid, int block, bool
 // ---------------------------------------------------------------------------
 // Dataflow schedule of the same sweeps.
 //
-// Every dynamic body e has a counter cnt[e] of the group updates applied to it.
-// Sorted group q is the rank_a-th of the deg_a groups on its body A (and the
-// rank_b-th of deg_b on B), so in pass p it may start once cnt[A] == p*deg_a +
-// rank_a and cnt[B] == p*deg_b + rank_b, and it sets them one higher when it is
-// done.  Each body therefore receives its updates in sorted (colour-major)
-// order, the order the coloured schedule and the exported sweep order define,
-// and groups on disjoint bodies commute: the result is bit-identical to the
-// coloured schedule, with no grid-wide barrier.  Pass 0 is confidentWarmStart,
-// passes 1..iterations are sweeps.
+// Each dynamic body receives its group updates in sorted (colour-major) order,
+// the order the coloured schedule and the exported sweep order define, and
+// groups on disjoint bodies commute: the result is bit-identical to the
+// coloured schedule, with no grid-wide barrier.  The order is kept by handing
+// a body's deltaV from each update to the next through slots: sorted group q
+// writes its updated deltaV of each dynamic body to its own slot (incidence
+// 2 q + side, one set of slots per pass parity), and the body's next update
+// (one rank higher, or for rank 0 the body's last group of the previous pass:
+// pred[]) polls that slot until it no longer holds the empty pattern, takes
+// the value and empties the slot for its next writer.  Pass 0 is
+// confidentWarmStart, passes 1..iterations are sweeps.
 //
-// One wave runs one group at a time.  Lanes 0..11 hold one deltaV component
-// each (lanes 0-5: body A v.xyz w.xyz, lanes 6-11: body B), so a row's J.dv is
-// a product per lane and a fixed-shape DPP reduction in the reference's
-// summation order, ((p0 + p1) + p2) + ((p3 + p4) + p5) per body and then A + B,
-// and the lambda update runs once per wave on uniform values.  Wave w owns the
-// sorted groups w, w + W, ... and walks them pass by pass in order; the global
-// minimum (pass, group) not yet done always has its dependencies met, so with
-// every wave resident the schedule cannot deadlock, and bounded spins turn a
-// wave that stops making progress into an error flag instead of a hang.
+// A wave runs steps of up to four groups of one colour, one group per 16-lane
+// row (lanes hold one deltaV component each, quads A.v, A.w, B.v, B.w), so a
+// row's J.dv is a product per lane and a fixed-shape DPP reduction in the
+// reference's summation order.  Wave w runs steps w, w + W, ... pass by pass
+// in order; the global minimum (pass, step) not yet done always has its
+// inputs, so with every wave resident the schedule cannot deadlock, and
+// bounded spins turn a wave that stops making progress into an error flag
+// instead of a hang.
 //
 // Early exit (PGS.java:190) needs the max |dl| of sweep t before sweep t+1 is
 // final.  Sweeps double-buffer lambda (odd sweeps in lam_alt) and deltaV (odd
@@ -868,7 +869,6 @@ AMD_INLINE double* slot_at(const FlowArgs& F, int parity, int incidence) {
 AMD_INLINE unsigned ld_cnt(const unsigned* p) {
     return __hip_atomic_load(const_cast<unsigned*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-AMD_INLINE void st_cnt(unsigned* p, unsigned v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 AMD_INLINE int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 AMD_INLINE double uniform_d(double v) {
     const long long b = __double_as_longlong(v);

@@ -381,7 +381,7 @@ def main(argv=None) -> int:
                     help="plain: ordinary launch of the persistent kernel (for profilers without cooperative-launch "
                          "support)")
     ap.add_argument("--pgs-schedule", choices=["dataflow", "coloured"], default=None,
-                    help="dataflow: per-body update counters; coloured: a grid barrier between colours "
+                    help="dataflow: each body's deltaV handed from update to update; coloured: a grid barrier between colours "
                          "(bit-identical results)")
     ap.add_argument("--autotune", choices=["auto", "on", "off"], default="auto",
                     help="after the warm-up, time the PGS kernel's launch shapes on the live contacts (both "

@@ -168,7 +168,7 @@ typedef struct am_params {
     int32_t pgs_plain_launch;         /* 1: launch the PGS kernel with an ordinary (not cooperative) launch,
                                          grid clamped to the co-resident limit; for profilers that do not
                                          support cooperative launches (0 = cooperative) */
-    int32_t pgs_schedule;             /* 0: dataflow (per-body update counters, no grid barriers);
+    int32_t pgs_schedule;             /* 0: dataflow (each body's deltaV handed from update to update, no grid barriers);
                                          1: colour by colour with a grid barrier between colours.
                                          Both give bit-identical results (the exported sweep order) */
     /* learned warm start (no reference counterpart): 0 plain previous-frame
