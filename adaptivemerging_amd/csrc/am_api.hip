@@ -416,11 +416,17 @@ int build_layout(am_ctx* c) {
         fill_int(s, c->g_colour_a.p, ng, -1);
         int* cin = c->g_colour_a.p;
         int* cout = c->g_colour_b.p;
-        for (int round = 0; round < 4096; round++) {
-            fill_int(s, sc + 5, 1, 0);
-            launch_colour_round(s, ng, c->g_a.p, c->g_b.p, c->d_flags.p, c->inc_start.p, c->inc_count.p,
-                                c->inc_items.p, cin, cout, sc + 5, c->layout_layer);
-            std::swap(cin, cout);
+        // Rounds run in batches between reads of the uncoloured count: a
+        // round after the last one only copies the colours, so the colouring
+        // does not depend on the batch size.
+        constexpr int kRoundsPerCheck = 8;
+        for (int round = 0; round < 4096; round += kRoundsPerCheck) {
+            for (int k = 0; k < kRoundsPerCheck; k++) {
+                fill_int(s, sc + 5, 1, 0);
+                launch_colour_round(s, ng, c->g_a.p, c->g_b.p, c->d_flags.p, c->inc_start.p, c->inc_count.p,
+                                    c->inc_items.p, cin, cout, sc + 5, c->layout_layer);
+                std::swap(cin, cout);
+            }
             int r = read_scalars(c, 5, 1);
             if (r) return r;
             if (c->h_scalars[5] == 0) break;

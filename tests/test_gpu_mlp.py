@@ -157,3 +157,41 @@ def test_learned_start_gain_on_held_out_pile(gpu_lib):
     ratio = sweeps[0] / sweeps[1]
     print(f"sweeps to 1e-6 over 20 steps: plain {sweeps[0]}, learned {sweeps[1]}, ratio {ratio:.3f}", flush=True)
     assert ratio >= 0.95
+
+
+def test_learned_start_fallback_on_full_pile(gpu_lib):
+    """The learned leg of the 20,000-box pile configuration with an untrained
+    network: after the pile has settled, each step evaluates the prediction's
+    start residual against the plain start's and keeps the plain start unless
+    the prediction is better.  Steps that keep it equal a plain context's
+    steps bit for bit (positions, rotations, velocities); at least one step
+    must keep it, and every state stays finite."""
+    scene = workloads.box_pile(seed=0, n=20000)
+    kw = dict(iterations=100, tolerance=1e-6, gravity_amount=9.81, enable_merging=0, enable_unmerging=0,
+              enable_sleeping=0)
+    plain, lrn = Context(), Context()
+    for c in (plain, lrn):
+        c.upload(scene)
+        c.set_params(**kw)
+        for _ in range(300):
+            c.step(0.01)
+    lrn.mlp_init(11)
+    lrn.set_params(learned_warm_start=1)
+    kept = 0
+    for step in range(4):
+        sp = plain.step(0.01)
+        sl = lrn.step(0.01)
+        print(f"step {step}: contacts {sl['n_contacts']} used_learned {sl['used_learned']} start residual plain "
+              f"{sl['start_residual_plain']:.4g} learned {sl['start_residual_learned']:.4g}", flush=True)
+        for name, a, b in zip("xRvw", plain.pull_state(), lrn.pull_state()):
+            assert np.all(np.isfinite(b)), name
+        if sl["used_learned"]:
+            assert sl["start_residual_learned"] < sl["start_residual_plain"]
+            break
+        kept += 1
+        assert sl["iterations_used"] == sp["iterations_used"]
+        for name, a, b in zip("xRvw", plain.pull_state(), lrn.pull_state()):
+            assert_bits_equal(f"step {step} {name}", a, b)
+    assert kept >= 1
+    plain.close()
+    lrn.close()
