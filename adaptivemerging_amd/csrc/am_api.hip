@@ -206,13 +206,15 @@ void default_params(am_params* p) {
 }
 
 // Derived host lists from the structure (parent/slot/sub/members).
-int rebuild_structure(am_ctx* c) {
+int rebuild_structure(am_ctx* c, bool bodies_changed) {
     const int n = c->n_ent;
-    c->planes.clear();
-    c->grid_bodies.clear();
-    for (int i = 0; i < c->n_phys; i++) {
-        if (c->desc[i].geom == AM_GEOM_PLANE) c->planes.push_back(i);
-        else c->grid_bodies.push_back(i);
+    if (bodies_changed) {
+        c->planes.clear();
+        c->grid_bodies.clear();
+        for (int i = 0; i < c->n_phys; i++) {
+            if (c->desc[i].geom == AM_GEOM_PLANE) c->planes.push_back(i);
+            else c->grid_bodies.push_back(i);
+        }
     }
     int n_slots = (int)c->slot_entity.size();
     if (n_slots >= kMaxSlots) return fail(c, AM_ERR_INVALID, "too many top-level bodies for the pair key");
@@ -237,8 +239,10 @@ int rebuild_structure(am_ctx* c) {
     AM_HIP(c, upload(c, c->d_slot_entity, c->slot_entity));
     AM_HIP(c, upload(c, c->d_member_first, c->member_first));
     AM_HIP(c, upload(c, c->d_members, c->members));
-    AM_HIP(c, upload(c, c->d_planes, c->planes));
-    AM_HIP(c, upload(c, c->d_grid, c->grid_bodies));
+    if (bodies_changed) {
+        AM_HIP(c, upload(c, c->d_planes, c->planes));
+        AM_HIP(c, upload(c, c->d_grid, c->grid_bodies));
+    }
     AM_HIP(c, upload(c, c->d_top_pinned, c->top_pinned));
     AM_HIP(c, upload(c, c->d_top_sleeping, c->top_sleeping));
     AM_HIP(c, upload(c, c->d_gravity, c->gravity_targets));
@@ -258,6 +262,13 @@ int rebuild_structure(am_ctx* c) {
             }
     AM_HIP(c, upload(c, c->d_member_list, c->member_list));
     AM_HIP(c, upload(c, c->d_member_adv, c->member_adv));
+    if (!bodies_changed) {
+        AM_HIP(c, c->inc_count.ensure(n + 1));
+        AM_HIP(c, c->inc_start.ensure(n + 1));
+        AM_HIP(c, c->inc_cursor.ensure(n + 1));
+        c->layout_valid = false;
+        return AM_OK;
+    }
     if (c->b2c.size() < (size_t)n * 12) c->b2c.resize((size_t)n * 12, 0.0);
     AM_HIP(c, upload(c, c->d_b2c, c->b2c));
     // grid: cells sized for the bodies up to kLargeRatio times the median

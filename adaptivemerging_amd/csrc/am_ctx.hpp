@@ -3,6 +3,7 @@
 #pragma once
 
 #include <algorithm>
+#include <new>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -38,6 +39,28 @@ struct DBuf {
     }
 };
 
+
+// Page-locked host memory: device copies to and from it run at full speed.
+template <class T>
+struct PinnedAlloc {
+    using value_type = T;
+    PinnedAlloc() = default;
+    template <class U>
+    PinnedAlloc(const PinnedAlloc<U>&) {}
+    T* allocate(size_t n) {
+        void* p = nullptr;
+        if (hipHostMalloc(&p, std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault) != hipSuccess)
+            throw std::bad_alloc();
+        return static_cast<T*>(p);
+    }
+    void deallocate(T* p, size_t) { (void)hipHostFree(p); }
+    template <class U>
+    bool operator==(const PinnedAlloc<U>&) const { return true; }
+    template <class U>
+    bool operator!=(const PinnedAlloc<U>&) const { return false; }
+};
+template <class T>
+using PinnedVec = std::vector<T, PinnedAlloc<T>>;
 
 // Grows b to cap elements of `stride` values, keeping the first `keep` elements.
 template <class T>
@@ -212,6 +235,12 @@ struct am_ctx {
     std::vector<int> free_entities;     // collection entity ids not in use
     std::vector<double> b2c;            // per entity: body-to-collection transform, R (9) then t (3)
     DBuf<double> d_b2c;
+    DBuf<int> ev_list;                  // staging of an event's changed entities
+    DBuf<double> ev_rec;
+    // host mirrors of the entity state kept between events (an event takes
+    // them and gives them back when it commits)
+    am_rt::PinnedVec<double> ev_x, ev_R, ev_v, ev_w, ev_jinv;
+    std::vector<std::vector<int>> ev_members;
     std::vector<int> member_list;       // every collection member (physical body)
     std::vector<int> member_adv;        // members that the single sweep integrates (not pinned)
     DBuf<int> d_member_list, d_member_adv;
@@ -251,7 +280,10 @@ am::ContactsView internal_view(am_ctx* c);
 am::MergeParams merge_params(const am_params& p);
 int read_scalars(am_ctx* c, int first, int n);
 int ensure_contact_capacity(am_ctx* c, int want, int keep = 0);
-int rebuild_structure(am_ctx* c);
+// bodies_changed = false: only the collection structure changed (an event),
+// so the lists derived from the physical bodies' geometry, the broad-phase
+// grid and the member transforms (written by the event) are left as they are.
+int rebuild_structure(am_ctx* c, bool bodies_changed = true);
 int refresh_jinv(am_ctx* c);
 int groups_after_contacts(am_ctx* c);
 int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double* last);

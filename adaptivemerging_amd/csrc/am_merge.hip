@@ -14,6 +14,7 @@
 #include <cmath>
 #include <functional>
 #include <new>
+#include <unordered_map>
 
 #include "am_ctx.hpp"
 
@@ -109,6 +110,33 @@ AMD_INLINE uint32_t pair_hash(uint64_t z) {
 // clearBodyPairContacts (CollisionProcessor.java:213-226), pass 1: a
 // contact is active when |lambda0| > 1e-14; a pair survives with one active
 // contact or more.
+// Entity records an event changed on the host, one per listed entity:
+// x, v, w (3 each), R, jinv (9 each), minv, then the descriptor (60 doubles).
+constexpr int kEntRecDoubles = 3 + 3 + 3 + 9 + 9 + 1 + (int)(sizeof(am_body_desc) / 8);
+static_assert(sizeof(am_body_desc) % 8 == 0, "descriptor is copied as doubles");
+
+__global__ void k_scatter_entities(double* x, double* v, double* w, double* R, double* jinv, double* minv,
+                                   am_body_desc* desc, const int* list, const double* rec, int n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)n * kEntRecDoubles) return;
+    const int k = (int)(i / kEntRecDoubles), q = (int)(i % kEntRecDoubles), e = list[k];
+    const double val = rec[i];
+    if (q < 3) x[(size_t)e * 3 + q] = val;
+    else if (q < 6) v[(size_t)e * 3 + q - 3] = val;
+    else if (q < 9) w[(size_t)e * 3 + q - 6] = val;
+    else if (q < 18) R[(size_t)e * 9 + q - 9] = val;
+    else if (q < 27) jinv[(size_t)e * 9 + q - 18] = val;
+    else if (q < 28) minv[e] = val;
+    else reinterpret_cast<double*>(desc + e)[q - 28] = val;
+}
+
+// Body-to-collection transforms (12 doubles) of the listed bodies.
+__global__ void k_scatter_b2c(double* b2c, const int* list, const double* rec, int n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)n * 12) return;
+    b2c[(size_t)list[i / 12] * 12 + i % 12] = rec[i];
+}
+
 __global__ void k_bpc_keep(ContactsView C, const int* n_groups, const int* group_start, unsigned char* active,
                            int* keep, int cap_groups) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -550,6 +578,29 @@ M3 get_op(V3 bx, double mass, V3 com) {
     return op;
 }
 
+// Items listed per body in item order (compressed rows): for each item i,
+// first(i) and second(i) name its bodies (-1 for none).
+struct Adjacency {
+    std::vector<int> start, item;
+    template <class F1, class F2>
+    void build(int n_bodies, int n_items, F1 first, F2 second) {
+        start.assign(n_bodies + 1, 0);
+        for (int i = 0; i < n_items; i++) {
+            const int a = first(i), b = second(i);
+            if (a >= 0) start[a + 1]++;
+            if (b >= 0) start[b + 1]++;
+        }
+        for (int k = 0; k < n_bodies; k++) start[k + 1] += start[k];
+        item.resize(start[n_bodies]);
+        std::vector<int> at(start.begin(), start.end() - 1);
+        for (int i = 0; i < n_items; i++) {
+            const int a = first(i), b = second(i);
+            if (a >= 0) item[at[a]++] = i;
+            if (b >= 0) item[at[b]++] = i;
+        }
+    }
+};
+
 // Host mirror of one entity for the duration of a merge or unmerge event.
 struct Ent {
     V3 x, v, w;
@@ -562,10 +613,43 @@ struct Ent {
 
 struct Event {
     am_ctx* c;
-    std::vector<double> x, R, v, w, jinv;  // device state of every entity
+    PinnedVec<double> x, R, v, w, jinv;    // device state of every entity
     std::vector<char> touched;             // entity state changed on the host
     std::vector<char> dv_zero;             // deltaV to clear (unmergeBody)
     std::vector<std::vector<int>> members; // per collection entity: RigidCollection.bodies
+    // Top-level list edits.  A removal (ArrayList.remove, the others keep
+    // their order) marks the entity's position; the list is compacted once,
+    // when the event is committed.
+    std::vector<int> top_pos;    // entity -> position in c->slot_entity, -1 if not listed
+    std::vector<char> top_gone;  // position removed
+    // Collections whose member transforms are recomputed at commit from the
+    // collection's stored pose (members keep their world poses during an
+    // event, so the last pose a collection stores gives the same transforms
+    // as recomputing them after each change).
+    std::vector<char> b2c_dirty;
+    std::vector<int> b2c_list;
+    std::vector<char> b2c_changed;  // body transforms to upload
+    int n_begin = 0;                // entities when the event began
+
+    void top_remove(int e) {
+        if (e < (int)top_pos.size() && top_pos[e] >= 0) {
+            top_gone[top_pos[e]] = 1;
+            top_pos[e] = -1;
+        }
+    }
+    void top_append(int e) {
+        if (e >= (int)top_pos.size()) top_pos.resize(e + 1, -1);
+        top_pos[e] = (int)c->slot_entity.size();
+        c->slot_entity.push_back(e);
+        top_gone.push_back(0);
+    }
+    void mark_b2c(int e) {
+        if (e >= (int)b2c_dirty.size()) b2c_dirty.resize(e + 1, 0);
+        if (!b2c_dirty[e]) {
+            b2c_dirty[e] = 1;
+            b2c_list.push_back(e);
+        }
+    }
 
     Ent load(int e) const {
         const am_body_desc& d = c->desc[e];
@@ -703,9 +787,9 @@ void finish_inertia(Ent& C) {
 
 // RigidTransform3D.multAinvB (:201-215): body-to-collection transform of
 // member b, t = R_c^T (x_b - x_c), R = R_c^T R_b.
-void body_to_collection(const Ent& C, V3 xb, const M3& Rb, double* out12) {
-    V3 t = V3{xb.x - C.x.x, xb.y - C.x.y, xb.z - C.x.z};
-    M3 Rt = transpose(C.R);
+void body_to_collection(V3 xc, const M3& Rc, V3 xb, const M3& Rb, double* out12) {
+    V3 t = V3{xb.x - xc.x, xb.y - xc.y, xb.z - xc.z};
+    M3 Rt = transpose(Rc);
     t = mulv(Rt, t);
     M3 R = mmul(Rt, Rb);
     for (int i = 0; i < 9; i++) out12[i] = R.m[i];
@@ -715,13 +799,26 @@ void body_to_collection(const Ent& C, V3 xb, const M3& Rb, double* out12) {
 // RigidCollection.updateBodiesTransformations (:589-592).
 void update_bodies_transformations(Event& E, int ce, const Ent& C) {
     am_ctx* c = E.c;
-    for (int b : E.members[ce]) body_to_collection(C, v3(&E.x[b * 3]), m3(&E.R[b * 9]), &c->b2c[(size_t)b * 12]);
+    for (int b : E.members[ce]) {
+        body_to_collection(C.x, C.R, v3(&E.x[b * 3]), m3(&E.R[b * 9]), &c->b2c[(size_t)b * 12]);
+        E.b2c_changed[b] = 1;
+    }
 }
 
-void top_remove(am_ctx* c, int e) {
-    auto& t = c->slot_entity;
-    auto it = std::find(t.begin(), t.end(), e);
-    if (it != t.end()) t.erase(it);
+// The deferred updateBodiesTransformations of the marked collections, from
+// the pose each one stored last.
+void flush_transformations(Event& E) {
+    am_ctx* c = E.c;
+    for (int ce : E.b2c_list) {
+        E.b2c_dirty[ce] = 0;
+        const V3 xc = v3(&E.x[ce * 3]);
+        const M3 Rc = m3(&E.R[ce * 9]);
+        for (int b : E.members[ce]) {
+            body_to_collection(xc, Rc, v3(&E.x[b * 3]), m3(&E.R[b * 9]), &c->b2c[(size_t)b * 12]);
+            E.b2c_changed[b] = 1;
+        }
+    }
+    E.b2c_list.clear();
 }
 
 int alloc_entity(Event& E) {
@@ -748,6 +845,9 @@ int alloc_entity(Event& E) {
         E.touched.resize(n, 0);
         E.dv_zero.resize(n, 0);
         E.members.resize(n);
+        E.top_pos.resize(n, -1);
+        E.b2c_dirty.resize(n, 0);
+        E.b2c_changed.resize(n, 0);
         for (int e = n - 1; e >= old; e--) c->free_entities.push_back(e);
         c->n_ent = n;
     }
@@ -788,7 +888,7 @@ void add_body(Event& E, int ce, Ent& C, int b) {
     collection_state(c, C, b, B);
     add_body_internal(C, B);
     finish_inertia(C);
-    update_bodies_transformations(E, ce, C);
+    E.mark_b2c(ce);  // updateBodiesTransformations
 }
 
 // RigidCollection.addBodies (:153-166).
@@ -802,7 +902,7 @@ void add_bodies(Event& E, int ce, Ent& C, const std::vector<int>& list) {
         add_body_internal(C, B);
     }
     finish_inertia(C);
-    update_bodies_transformations(E, ce, C);
+    E.mark_b2c(ce);
 }
 
 // RigidCollection.addCollection (:172-186): the other collection's members
@@ -817,7 +917,7 @@ void add_collection(Event& E, int ce, Ent& C, int other) {
     collection_state(c, C, other, O);
     add_body_internal(C, O);
     finish_inertia(C);
-    update_bodies_transformations(E, ce, C);
+    E.mark_b2c(ce);
 }
 
 // new RigidCollection(body1, body2) (:55-76): starts as a copy of the first
@@ -880,9 +980,11 @@ void update_bb_members(Event& E, int ce, Ent& C) {
 void remove_bodies(Event& E, int ce, Ent& C, const std::vector<int>& gone) {
     am_ctx* c = E.c;
     auto& mem = E.members[ce];
+    std::vector<int> sorted_gone = gone;
+    std::sort(sorted_gone.begin(), sorted_gone.end());
     std::vector<int> keep;
     for (int b : mem)
-        if (std::find(gone.begin(), gone.end(), b) == gone.end()) keep.push_back(b);
+        if (!std::binary_search(sorted_gone.begin(), sorted_gone.end(), b)) keep.push_back(b);
     mem = keep;
     bool was_pinned = C.pinned;
     C.pinned = false;
@@ -949,6 +1051,12 @@ void remove_bodies(Event& E, int ce, Ent& C, const std::vector<int>& gone) {
 int begin_event(am_ctx* c, Event& E) {
     E.c = c;
     size_t n = (size_t)c->n_ent;
+    E.x.swap(c->ev_x);
+    E.v.swap(c->ev_v);
+    E.w.swap(c->ev_w);
+    E.R.swap(c->ev_R);
+    E.jinv.swap(c->ev_jinv);
+    E.members.swap(c->ev_members);
     E.x.resize(n * 3);
     E.v.resize(n * 3);
     E.w.resize(n * 3);
@@ -963,7 +1071,15 @@ int begin_event(am_ctx* c, Event& E) {
     AM_HIP(c, hipStreamSynchronize(c->stream));
     E.touched.assign(n, 0);
     E.dv_zero.assign(n, 0);
-    E.members.assign(n, std::vector<int>());
+    for (auto& m : E.members) m.clear();
+    E.members.resize(n);
+    E.top_pos.assign(n, -1);
+    E.top_gone.assign(c->slot_entity.size(), 0);
+    for (int q = 0; q < (int)c->slot_entity.size(); q++) E.top_pos[c->slot_entity[q]] = q;
+    E.b2c_dirty.assign(n, 0);
+    E.b2c_list.clear();
+    E.b2c_changed.assign(n, 0);
+    E.n_begin = (int)n;
     for (int e : c->slot_entity)
         if (c->desc[e].geom == AM_GEOM_COLLECTION)
             for (int q = 0; q < c->member_count[e]; q++) E.members[e].push_back(c->members[c->member_first[e] + q]);
@@ -974,34 +1090,93 @@ int begin_event(am_ctx* c, Event& E) {
 int commit_event(am_ctx* c, Event& E) {
     hipStream_t s = c->stream;
     const int n = c->n_ent;
-    const int old_n = (int)(c->d_flags.n);
-    // per-entity device arrays sized by n_ent
-    if (c->d_x.n < (size_t)n * 3) {
-        int keep = old_n;
-        AM_HIP(c, grow_keep(s, c->d_f, n, 3, keep));
-        AM_HIP(c, grow_keep(s, c->d_t, n, 3, keep));
-        AM_HIP(c, grow_keep(s, c->d_dv, n, 6, keep));
-        AM_HIP(c, c->d_x.ensure((size_t)n * 3));
-        AM_HIP(c, c->d_v.ensure((size_t)n * 3));
-        AM_HIP(c, c->d_w.ensure((size_t)n * 3));
-        AM_HIP(c, c->d_R.ensure((size_t)n * 9));
-        AM_HIP(c, c->d_jinv.ensure((size_t)n * 9));
-        AM_HIP(c, c->d_minv.ensure(n));
-        // clear the new tail of the accumulators
-        fill_double(s, c->d_f.p + (size_t)keep * 3, (n - keep) * 3, 0.0);
-        fill_double(s, c->d_t.p + (size_t)keep * 3, (n - keep) * 3, 0.0);
-        fill_double(s, c->d_dv.p + (size_t)keep * 6, (n - keep) * 6, 0.0);
+    flush_transformations(E);
+    {
+        auto& t = c->slot_entity;
+        int k = 0;
+        for (int q = 0; q < (int)t.size(); q++)
+            if (!E.top_gone[q]) t[k++] = t[q];
+        t.resize(k);
     }
-    AM_HIP(c, hipMemcpyAsync(c->d_x.p, E.x.data(), (size_t)n * 24, hipMemcpyHostToDevice, s));
-    AM_HIP(c, hipMemcpyAsync(c->d_v.p, E.v.data(), (size_t)n * 24, hipMemcpyHostToDevice, s));
-    AM_HIP(c, hipMemcpyAsync(c->d_w.p, E.w.data(), (size_t)n * 24, hipMemcpyHostToDevice, s));
-    AM_HIP(c, hipMemcpyAsync(c->d_R.p, E.R.data(), (size_t)n * 72, hipMemcpyHostToDevice, s));
-    AM_HIP(c, hipMemcpyAsync(c->d_jinv.p, E.jinv.data(), (size_t)n * 72, hipMemcpyHostToDevice, s));
-    std::vector<double> minv(n);
-    for (int i = 0; i < n; i++) minv[i] = (c->flags[i] & AM_BODY_PINNED) ? 0.0 : c->desc[i].minv;
-    AM_HIP(c, upload(c, c->d_minv, minv));
-    AM_HIP(c, upload(c, c->d_desc, c->desc));
-    AM_HIP(c, upload(c, c->d_b2c, c->b2c));
+    const int old_n = (int)(c->d_flags.n);
+    // Only the entities the event changed are written back while the entity
+    // count stays; a grown count reallocates and uploads every array.
+    const bool sparse = n == E.n_begin && c->d_x.n >= (size_t)n * 3 && c->d_desc.n >= (size_t)n &&
+                        c->d_minv.n >= (size_t)n && c->d_b2c.n >= (size_t)n * 12;
+    if (sparse) {
+        std::vector<int> list;
+        for (int e = 0; e < n; e++)
+            if (E.touched[e]) list.push_back(e);
+        if (!list.empty()) {
+            std::vector<double> rec(list.size() * kEntRecDoubles);
+            for (size_t k = 0; k < list.size(); k++) {
+                const int e = list[k];
+                double* o = &rec[k * kEntRecDoubles];
+                for (int q = 0; q < 3; q++) {
+                    o[q] = E.x[(size_t)e * 3 + q];
+                    o[3 + q] = E.v[(size_t)e * 3 + q];
+                    o[6 + q] = E.w[(size_t)e * 3 + q];
+                }
+                for (int q = 0; q < 9; q++) {
+                    o[9 + q] = E.R[(size_t)e * 9 + q];
+                    o[18 + q] = E.jinv[(size_t)e * 9 + q];
+                }
+                o[27] = (c->flags[e] & AM_BODY_PINNED) ? 0.0 : c->desc[e].minv;
+                std::memcpy(o + 28, &c->desc[e], sizeof(am_body_desc));
+            }
+            AM_HIP(c, upload(c, c->ev_list, list));
+            AM_HIP(c, upload(c, c->ev_rec, rec));
+            const long m = (long)list.size() * kEntRecDoubles;
+            hipLaunchKernelGGL(k_scatter_entities, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, c->d_x.p,
+                               c->d_v.p, c->d_w.p, c->d_R.p, c->d_jinv.p, c->d_minv.p, c->d_desc.p, c->ev_list.p,
+                               c->ev_rec.p, (int)list.size());
+            AM_HIP(c, hipGetLastError());
+            AM_HIP(c, hipStreamSynchronize(s));  // the staging buffers are reused below
+        }
+        std::vector<int> blist;
+        for (int e = 0; e < n; e++)
+            if (E.b2c_changed[e]) blist.push_back(e);
+        if (!blist.empty()) {
+            std::vector<double> rec(blist.size() * 12);
+            for (size_t k = 0; k < blist.size(); k++)
+                std::memcpy(&rec[k * 12], &c->b2c[(size_t)blist[k] * 12], 12 * sizeof(double));
+            AM_HIP(c, upload(c, c->ev_list, blist));
+            AM_HIP(c, upload(c, c->ev_rec, rec));
+            const long m = (long)blist.size() * 12;
+            hipLaunchKernelGGL(k_scatter_b2c, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, c->d_b2c.p,
+                               c->ev_list.p, c->ev_rec.p, (int)blist.size());
+            AM_HIP(c, hipGetLastError());
+            AM_HIP(c, hipStreamSynchronize(s));
+        }
+    } else {
+        // per-entity device arrays sized by n_ent
+        if (c->d_x.n < (size_t)n * 3) {
+            int keep = old_n;
+            AM_HIP(c, grow_keep(s, c->d_f, n, 3, keep));
+            AM_HIP(c, grow_keep(s, c->d_t, n, 3, keep));
+            AM_HIP(c, grow_keep(s, c->d_dv, n, 6, keep));
+            AM_HIP(c, c->d_x.ensure((size_t)n * 3));
+            AM_HIP(c, c->d_v.ensure((size_t)n * 3));
+            AM_HIP(c, c->d_w.ensure((size_t)n * 3));
+            AM_HIP(c, c->d_R.ensure((size_t)n * 9));
+            AM_HIP(c, c->d_jinv.ensure((size_t)n * 9));
+            AM_HIP(c, c->d_minv.ensure(n));
+            // clear the new tail of the accumulators
+            fill_double(s, c->d_f.p + (size_t)keep * 3, (n - keep) * 3, 0.0);
+            fill_double(s, c->d_t.p + (size_t)keep * 3, (n - keep) * 3, 0.0);
+            fill_double(s, c->d_dv.p + (size_t)keep * 6, (n - keep) * 6, 0.0);
+        }
+        AM_HIP(c, hipMemcpyAsync(c->d_x.p, E.x.data(), (size_t)n * 24, hipMemcpyHostToDevice, s));
+        AM_HIP(c, hipMemcpyAsync(c->d_v.p, E.v.data(), (size_t)n * 24, hipMemcpyHostToDevice, s));
+        AM_HIP(c, hipMemcpyAsync(c->d_w.p, E.w.data(), (size_t)n * 24, hipMemcpyHostToDevice, s));
+        AM_HIP(c, hipMemcpyAsync(c->d_R.p, E.R.data(), (size_t)n * 72, hipMemcpyHostToDevice, s));
+        AM_HIP(c, hipMemcpyAsync(c->d_jinv.p, E.jinv.data(), (size_t)n * 72, hipMemcpyHostToDevice, s));
+        std::vector<double> minv(n);
+        for (int i = 0; i < n; i++) minv[i] = (c->flags[i] & AM_BODY_PINNED) ? 0.0 : c->desc[i].minv;
+        AM_HIP(c, upload(c, c->d_minv, minv));
+        AM_HIP(c, upload(c, c->d_desc, c->desc));
+        AM_HIP(c, upload(c, c->d_b2c, c->b2c));
+    }
     std::vector<int> dz;
     for (int i = 0; i < n; i++)
         if (E.dv_zero[i]) dz.push_back(i);
@@ -1035,7 +1210,13 @@ int commit_event(am_ctx* c, Event& E) {
             }
         }
     }
-    return rebuild_structure(c);
+    E.x.swap(c->ev_x);
+    E.v.swap(c->ev_v);
+    E.w.swap(c->ev_w);
+    E.R.swap(c->ev_R);
+    E.jinv.swap(c->ev_jinv);
+    E.members.swap(c->ev_members);
+    return rebuild_structure(c, false);
 }
 
 }  // namespace
@@ -1356,11 +1537,9 @@ int merge_merge(am_ctx* c, int* n_merged) {
     int r = begin_event(c, E);
     if (r) return r;
     // bodyPairContacts of each physical body (external pairs of this step)
-    std::vector<std::vector<int>> adj(c->n_phys);
-    for (int i = 0; i < (int)T.size(); i++) {
-        adj[T[i].b1].push_back(i);
-        if (T[i].b2 != T[i].b1) adj[T[i].b2].push_back(i);
-    }
+    Adjacency adj;
+    adj.build(c->n_phys, (int)T.size(), [&](int i) { return T[i].b1; },
+              [&](int i) { return T[i].b2 != T[i].b1 ? T[i].b2 : -1; });
     std::vector<char> internal(T.size(), 0);
     std::vector<int> newly;
     auto internalize = [&](int i) {
@@ -1369,21 +1548,18 @@ int merge_merge(am_ctx* c, int* n_merged) {
     };
     // addIncompleteContacts (RigidCollection.java:1003-1015)
     auto add_incomplete = [&](int body) {
-        for (int i : adj[body])
+        for (int q = adj.start[body]; q < adj.start[body + 1]; q++) {
+            const int i = adj.item[q];
             if (!internal[i] && same_collection(c, T[i].b1, T[i].b2)) internalize(i);
+        }
     };
-    std::vector<Ent> cstate(c->n_ent);
-    std::vector<char> cloaded(c->n_ent, 0);
+    // state of the collections this event changes (node-based: references
+    // stay valid while others are added)
+    std::unordered_map<int, Ent> cstate;
     auto coll = [&](int e) -> Ent& {
-        if ((int)cstate.size() < c->n_ent) {
-            cstate.resize(c->n_ent);
-            cloaded.resize(c->n_ent, 0);
-        }
-        if (!cloaded[e]) {
-            cstate[e] = E.load(e);
-            cloaded[e] = 1;
-        }
-        return cstate[e];
+        auto it = cstate.find(e);
+        if (it == cstate.end()) it = cstate.emplace(e, E.load(e)).first;
+        return it->second;
     };
     auto save = [&](int e) { E.store_collection(e, coll(e)); };
     for (int i = 0; i < (int)T.size(); i++) {
@@ -1396,22 +1572,17 @@ int merge_merge(am_ctx* c, int* n_merged) {
         internalize(i);
         int p1 = c->parent[b.b1], p2 = c->parent[b.b2];
         if (p1 < 0 && p2 < 0) {
-            top_remove(c, b.b1);
-            top_remove(c, b.b2);
+            E.top_remove(b.b1);
+            E.top_remove(b.b2);
             Ent C;
             int ce = new_collection(E, b.b1, b.b2, C);
-            if ((int)cstate.size() < c->n_ent) {
-                cstate.resize(c->n_ent);
-                cloaded.resize(c->n_ent, 0);
-            }
             cstate[ce] = C;
-            cloaded[ce] = 1;
             save(ce);
-            c->slot_entity.push_back(ce);
+            E.top_append(ce);
         } else if (p1 >= 0 && p2 >= 0) {
             int keep = p1, gone = p2;
             if (!(E.members[p1].size() > E.members[p2].size())) std::swap(keep, gone);
-            top_remove(c, gone);
+            E.top_remove(gone);
             std::vector<int> gone_bodies = E.members[gone];
             // the absorbed collection's own state is averaged in as one body
             Ent& K = coll(keep);
@@ -1419,10 +1590,10 @@ int merge_merge(am_ctx* c, int* n_merged) {
             save(keep);
             for (int body : gone_bodies) add_incomplete(body);
             free_entity(E, gone);
-            cloaded[gone] = 0;
+            cstate.erase(gone);
         } else {
             int host = p1 >= 0 ? p1 : p2, body = p1 >= 0 ? b.b2 : b.b1;
-            top_remove(c, body);
+            E.top_remove(body);
             Ent& K = coll(host);
             add_body(E, host, K, body);
             save(host);
@@ -1487,35 +1658,34 @@ int merge_unmerge(am_ctx* c, double dt, int* n_events) {
     std::vector<int> reexpose;  // internal pairs going back to the external list, in order
     std::vector<int> additions, removals;
     // internal pairs of each body (BodyPairContact lists of the members)
-    std::vector<std::vector<int>> adj(c->n_phys);
-    for (int i = 0; i < nib; i++) {
-        adj[c->ibpc[i].b1].push_back(i);
-        adj[c->ibpc[i].b2].push_back(i);
-    }
+    Adjacency adj;
+    adj.build(c->n_phys, nib, [&](int i) { return c->ibpc[i].b1; }, [&](int i) { return c->ibpc[i].b2; });
+    // flagged pairs of each collection, in pair order
+    std::unordered_map<int, std::vector<int>> cuts;
+    for (int i = 0; i < nib; i++)
+        if (c->ibpc[i].flag) cuts[c->ibpc[i].coll].push_back(i);
+    // per-body marks, cleared after each collection (only its members are set)
+    std::vector<char> handled(c->n_phys, 0), remained(c->n_phys, 0), in_sub(c->n_phys, 0);
     const std::vector<int> top0 = c->slot_entity;
     for (int ce : top0) {
         if (c->desc[ce].geom != AM_GEOM_COLLECTION || (c->flags[ce] & AM_BODY_SLEEPING)) continue;
-        std::vector<int> cut;
-        for (int i = 0; i < nib; i++)
-            if (c->ibpc[i].coll == ce && c->ibpc[i].flag) cut.push_back(i);
-        if (cut.empty()) continue;
+        auto found = cuts.find(ce);
+        if (found == cuts.end()) continue;
+        const std::vector<int>& cut = found->second;
         for (int i : cut) incoll[i] = 0;
         Ent C = E.load(ce);
         const std::vector<int> bodies = E.members[ce];
-        std::vector<char> handled(c->n_phys, 0), remained(c->n_phys, 0);
         std::vector<int> new_bodies;
         bool remove_coll = true;
         for (int body : bodies) {
             if (handled[body]) continue;
             // buildNeighborBody (:381-393), depth first over pairs still inside
             std::vector<int> sub{body};
-            std::vector<char> in_sub(c->n_phys, 0);
             in_sub[body] = 1;
-            std::vector<int> stack{body};
-            std::vector<int> order;
             // recursive order: visit neighbours in pair order, descending immediately
             std::function<void(int)> dfs = [&](int x) {
-                for (int i : adj[x]) {
+                for (int q = adj.start[x]; q < adj.start[x + 1]; q++) {
+                    const int i = adj.item[q];
                     if (!incoll[i]) continue;
                     int o = c->ibpc[i].b1 == x ? c->ibpc[i].b2 : c->ibpc[i].b1;
                     if (!in_sub[o] && !handled[o]) {
@@ -1526,7 +1696,10 @@ int merge_unmerge(am_ctx* c, double dt, int* n_events) {
                 }
             };
             dfs(body);
-            for (int b : sub) handled[b] = 1;
+            for (int b : sub) {
+                handled[b] = 1;
+                in_sub[b] = 0;
+            }
             if (bodies.size() != sub.size()) {
                 if (sub.size() < bodies.size() / 2 + 1) {
                     for (int b : sub) unmerge_body(E, C, b);
@@ -1564,6 +1737,7 @@ int merge_unmerge(am_ctx* c, double dt, int* n_events) {
                 E.store_collection(ce, C);
             }
         }
+        for (int b : bodies) handled[b] = remained[b] = 0;
         if (!new_bodies.empty()) {
             c->merging_event = true;
             (*n_events)++;
@@ -1571,9 +1745,9 @@ int merge_unmerge(am_ctx* c, double dt, int* n_events) {
             if (remove_coll) removals.push_back(ce);
         }
     }
-    for (int e : additions) c->slot_entity.push_back(e);
+    for (int e : additions) E.top_append(e);
     for (int e : removals) {
-        top_remove(c, e);
+        E.top_remove(e);
         free_entity(E, e);
     }
     // pairs whose bodies no longer share a collection go back to the contact list
@@ -1678,10 +1852,10 @@ int merge_unmerge_all(am_ctx* c, int* n_dissolved) {
         }
     }
     for (int ce : colls) {
-        top_remove(c, ce);
+        E.top_remove(ce);
         free_entity(E, ce);
     }
-    for (int b : additions) c->slot_entity.push_back(b);
+    for (int b : additions) E.top_append(b);
     *n_dissolved = (int)colls.size();
     if ((r = commit_event(c, E))) return r;
     return upload_ibpc(c);
