@@ -21,7 +21,10 @@ Phases (only the last is timed):
   5. timed: --steps steps with the plain warm start (the headline), then the
      same steps from the same state with the learned start;
   6. merging leg: the dominoes-on-platforms configuration with merging on,
-     timed on its own after its push (reported under config.merging).
+     timed on its own after its push (reported under config.merging);
+  7. batched leg: 256 independent piles of boxes and spheres in one context
+     with merging on, 50 sweeps, timed on its own after they settle
+     (reported under config.batched).
 
 Prints one JSON line: contact-updates/s (N_contacts x sweeps / solve time)
 and ms/step of the plain run as the headline, both runs' figures (sweeps,
@@ -245,6 +248,70 @@ def run_merging_leg(args) -> dict:
     return out
 
 
+def run_batched_leg(args) -> dict:
+    """The batched configuration: 256 independent scenes held by one context
+    (one detection, one colouring and one persistent PGS launch per step over
+    all of them), body counts log-uniform 50-2,000, each a pile of boxes
+    (edges 0.1-0.5 m) and spheres (radius 0.05-0.3 m) dropped on its own
+    plane; 50 sweeps at the default tolerance with per-scene exits, merging
+    and unmerging on at the reference defaults, sleeping off.
+    --batched-settle untimed steps, the launch-shape tuner (with --autotune),
+    then --batched-steps timed steps."""
+    from adaptivemerging_amd import workloads
+    from adaptivemerging_amd.context import Context
+
+    scene = workloads.batched_piles(seed=args.seed)
+    n_scenes = int(np.max(scene.arrays()[0]["scene"])) + 1
+    ctx = Context()
+    ctx.upload(scene)
+    ctx.set_params(gravity_amount=9.81, iterations=50, enable_sleeping=0,
+                   pgs_plain_launch=int(args.pgs_launch == "plain"))
+    t0 = time.perf_counter()
+    for _ in range(args.batched_settle):
+        ctx.step(args.dt)
+    shape = None
+    if args.autotune == "on":
+        trials, best = ctx.autotune(args.dt, sweeps=20, reps=3)
+        b = trials[best]
+        shape = {"schedule": "dataflow" if b["pgs_schedule"] == 0 else "coloured", "grid": b["grid_used"],
+                 "block": b["block_used"], "ms_per_sweep": b["ms_per_sweep"]}
+    stats, wall = run_steps(ctx, args.batched_steps, args.dt)
+    used, _ = ctx.scene_iterations()
+    ctx.close()
+    n_c = np.array([s["n_contacts"] for s in stats], dtype=np.float64)
+    sweeps = np.array([s["iterations_used"] for s in stats], dtype=np.float64)
+    solve_ms = np.array([s["ms_solve"] for s in stats])
+    out = {
+        "workload": f"{n_scenes} independent scenes ({scene.n} bodies), body counts log-uniform 50-2000, piles of "
+                    f"boxes and spheres on a plane each, dt {args.dt}, 50 sweeps, merging and unmerging on at the "
+                    f"reference defaults; {args.batched_settle} untimed steps, then {args.batched_steps} timed steps",
+        "scenes": n_scenes,
+        "bodies": scene.n,
+        "ms_per_step": wall * 1e3 / max(len(stats), 1),
+        "contact_updates_per_s": float(np.sum(n_c * sweeps)) / (float(np.sum(solve_ms)) * 1e-3)
+        if np.sum(solve_ms) > 0 else 0.0,
+        "ms_merging": float(np.mean([s["ms_merging"] for s in stats])),
+        "ms_detect": float(np.mean([s["ms_detect"] for s in stats])),
+        "ms_solve": float(np.mean(solve_ms)),
+        "ms_pgs_kernel": float(np.mean([s["ms_pgs_kernel"] for s in stats])),
+        "contacts_mean": float(np.mean(n_c)),
+        "internal_contacts_mean": float(np.mean([s["n_internal_contacts"] for s in stats])),
+        "collections_max": int(max(s["n_collections"] for s in stats)),
+        "merged": int(sum(s["n_merged"] for s in stats)),
+        "unmerged": int(sum(s["n_unmerged"] for s in stats)),
+        "colours_mean": float(np.mean([s["n_colours"] for s in stats])),
+        "sweeps_mean": float(np.mean(sweeps)),
+        "scene_sweeps_last_step": {"min": int(np.min(used)), "mean": float(np.mean(used)), "max": int(np.max(used))},
+        "launch_shape": shape,
+        "untimed_s": time.perf_counter() - t0 - wall,
+    }
+    print(f"# batched: {out['ms_per_step']:.2f} ms/step (solve {out['ms_solve']:.2f}, merging "
+          f"{out['ms_merging']:.2f}, detect {out['ms_detect']:.2f}), {out['contacts_mean']:.0f} contacts, "
+          f"{out['internal_contacts_mean']:.0f} internal, {out['merged']} merged, {out['unmerged']} unmerged",
+          file=sys.stderr, flush=True)
+    return out
+
+
 def small_pile(args, k: int):
     """Scene k of the small piles the learned start is trained (k >= 0) and
     evaluated (k < 0) on: the configuration's boxes and densities, few
@@ -372,6 +439,10 @@ def main(argv=None) -> int:
                     help="also time the dominoes configuration with merging on (reported under config.merging)")
     ap.add_argument("--merging-settle", type=int, default=300, help="untimed domino steps before the push")
     ap.add_argument("--merging-steps", type=int, default=100, help="timed domino steps from the push on")
+    ap.add_argument("--batched-leg", choices=["on", "off"], default="on",
+                    help="also time 256 batched scenes with merging on (reported under config.batched)")
+    ap.add_argument("--batched-settle", type=int, default=200, help="untimed steps of the batched scenes")
+    ap.add_argument("--batched-steps", type=int, default=50, help="timed steps of the batched scenes")
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
     ap.add_argument("--cpu-sweeps", type=int, default=40, help="sequential sweeps of the CPU baseline sample")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
@@ -467,6 +538,7 @@ def main(argv=None) -> int:
           f"{res_p['sweeps_mean']:.0f}, residual {res_p['residual_final_mean']:.3g}", file=sys.stderr, flush=True)
 
     merging = run_merging_leg(args) if args.merging_leg == "on" else None
+    batched = run_batched_leg(args) if args.batched_leg == "on" else None
 
     cpu = None
     if args.cpu_baseline == "on":
@@ -540,6 +612,7 @@ def main(argv=None) -> int:
             "parallelism": "none (1 GPU)",
             "launch_shape": tuned,
             "merging": merging,
+            "batched": batched,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -25,7 +25,7 @@ def test_bench_small_end_to_end(gpu_lib):
     argv = ["--bodies", "400", "--settle", "40", "--warmup", "1", "--steps", "3", "--iterations", "200",
             "--train-scenes", "1", "--eval-scenes", "1", "--train-bodies", "40", "--train-settle", "40",
             "--train-steps", "8", "--epochs", "2", "--cpu-sweeps", "5", "--merging-settle", "20",
-            "--merging-steps", "5"]
+            "--merging-steps", "5", "--batched-settle", "30", "--batched-steps", "3"]
     buf = io.StringIO()
     with contextlib.redirect_stdout(buf):
         rc = bench.main(argv)
@@ -43,3 +43,6 @@ def test_bench_small_end_to_end(gpu_lib):
     assert rec["config"]["plain"]["contacts_mean"] > 0
     mrg = rec["config"]["merging"]
     assert mrg["ms_per_step"] > 0 and mrg["contacts_mean"] > 1000 and mrg["sweeps_mean"] > 0
+    bat = rec["config"]["batched"]
+    assert bat["scenes"] == 256 and bat["ms_per_step"] > 0 and bat["contacts_mean"] > 1000
+    assert bat["sweeps_mean"] > 0 and bat["contact_updates_per_s"] > 0 and bat["launch_shape"] is not None
