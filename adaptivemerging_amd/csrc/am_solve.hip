@@ -1206,46 +1206,23 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 nx_pr = gpred[q];
                 if (P.per_scene) nx_sc = P.g_scene[q];
             }
-            // per-scene exits: a row whose scene was under the tolerance at
-            // sweep p-2 (or p-1 before the last sweep) skips this pass; lane
-            // 0 of the row waits until every group of the scene has finished
-            // those sweeps.  Every group they wait for comes earlier in the
-            // (pass, step) order, so the earliest undone step can always run.
+            // Per-scene exits: a row whose scene was under the tolerance at
+            // sweep p-2 or p-3 (or p-1 before the last sweep) skips this
+            // pass.  A skipped sweep leaves the scene's max at 0, so with
+            // p-3 checked too a scene that skipped once skips every later
+            // pass (its first sweep under the tolerance is final; a later
+            // sweep over it must not bring back a scene whose groups
+            // published nothing for the pass before).  Lane 0 of the row
+            // learns it while the wave polls for its inputs below: it waits
+            // until every group of the scene has finished those sweeps, then
+            // reads the scene's max.  Every group it waits for comes earlier
+            // in the (pass, step) order, so the earliest undone step can
+            // always run.  Until then the row counts as active (its records
+            // are copied and its inputs polled for).
+            const int t1 = p - 2, t2 = (p == lastp) ? p - 1 : 0;
+            const bool check = P.per_scene && p >= 1 && P.check_tolerance && (t1 >= 1 || t2 >= 1);
             bool active = has_group;
-            if (P.per_scene && p >= 1 && P.check_tolerance) {
-                const int t1 = p - 2, t2 = (p == lastp) ? p - 1 : 0;
-                if (t1 >= 1 || t2 >= 1) {
-                    const unsigned want = has_group ? P.scene_groups[sc] : 0u;
-                    const unsigned* d1 = &P.scene_done[(size_t)sc * (P.iterations + 1) + (t1 >= 1 ? t1 : t2)];
-                    const unsigned* d2 = &P.scene_done[(size_t)sc * (P.iterations + 1) + (t2 >= 1 ? t2 : t1)];
-                    long spins = 0;
-                    while (true) {
-                        bool ok = true;
-                        if (has_group && li == 0) ok = ld_cnt(d1) == want && ld_cnt(d2) == want;
-                        if (__all(ok)) break;
-                        __builtin_amdgcn_s_sleep(1);
-                        if (++spins > kFlowSpinLimit ||
-                            ((spins & 255) == 0 && uniform(__hip_atomic_load(P.error, __ATOMIC_RELAXED,
-                                                                              __HIP_MEMORY_SCOPE_AGENT)))) {
-                            if (lane == 0) atomicOr(P.error, 1);
-                            return;
-                        }
-                    }
-                    bool skip = false;
-                    if (has_group && li == 0) {
-                        const unsigned long long* m = &P.scene_max[(size_t)sc * P.iterations];
-                        auto under = [&](int t) {
-                            return __longlong_as_double((long long)__hip_atomic_load(
-                                       const_cast<unsigned long long*>(&m[t - 1]), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT)) < P.tolerance;
-                        };
-                        skip = (t1 >= 1 && under(t1)) || (t2 >= 1 && under(t2));
-                    }
-                    skip = __shfl(skip, row * 16, 64);
-                    active = has_group && !skip;
-                }
-            }
-            const int n = active ? gd.y - gd.x : 0;
+            int n = active ? gd.y - gd.x : 0;
             int nmax = n;
             nmax = max(nmax, __shfl_xor(nmax, 16, 64));
             nmax = max(nmax, __shfl_xor(nmax, 32, 64));
@@ -1299,8 +1276,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             }
             {
                 bool got = src == nullptr;
+                // exit check of this row's scene: 0 waiting for the scene's
+                // groups, 1 resolved (lane 0 of each row; the others resolved)
+                bool resolved = !(check && has_group && li == 0), skip = false;
+                const unsigned want = check && has_group ? P.scene_groups[sc] : 0u;
+                const size_t base = (size_t)sc * (P.iterations + 1);
                 long spins = 0;
                 while (true) {
+                    if (!resolved) {
+                        const unsigned c1 = ld_cnt(&P.scene_done[base + (t1 >= 1 ? t1 : t2)]);
+                        const unsigned c2 = ld_cnt(&P.scene_done[base + (t2 >= 1 ? t2 : t1)]);
+                        if (c1 == want && c2 == want) {
+                            const unsigned long long* m = &P.scene_max[(size_t)sc * P.iterations];
+                            auto under = [&](int t) {
+                                return __longlong_as_double((long long)__hip_atomic_load(
+                                           const_cast<unsigned long long*>(&m[t - 1]), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT)) < P.tolerance;
+                            };
+                            skip = (t1 >= 1 && under(t1)) || (t1 >= 2 && under(t1 - 1)) || (t2 >= 1 && under(t2));
+                            resolved = true;
+                        }
+                    }
                     if (!got) {
                         const double v = ld_coherent(src);
                         if ((unsigned long long)__double_as_longlong(v) != kSlotEmpty) {
@@ -1308,7 +1304,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                             got = true;
                         }
                     }
-                    if (__all(got)) break;
+                    bool done = got;
+                    if (check) {
+                        // a skipped row reads no inputs (its scene's other
+                        // groups skip this pass too and publish none)
+                        const bool row_resolved = __shfl(resolved, row * 16, 64);
+                        const bool row_skip = __shfl(skip, row * 16, 64);
+                        done = row_resolved && (row_skip || got);
+                    }
+                    if (__all(done)) break;
                     __builtin_amdgcn_s_sleep(1);
                     if (++spins > kFlowSpinLimit ||
                         ((spins & 255) == 0 && uniform(__hip_atomic_load(P.error, __ATOMIC_RELAXED,
@@ -1316,6 +1320,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                         if (lane == 0) atomicOr(P.error, 1);
                         return;
                     }
+                }
+                if (check && __shfl(skip, row * 16, 64) && has_group) {
+                    active = false;
+                    n = 0;
+                    src = nullptr;
                 }
             }
             const bool last = p == lastp;
@@ -1403,16 +1412,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             }
             mx = fmax(mx, smx);
             nan = nan || snan;
-            if (P.per_scene && p >= 1 && has_group && lead) {
-                // the scene's max for this sweep, then this group counts as done
-                if (active && !(smx <= 0))
-                    atomicMax(&P.scene_max[(size_t)sc * P.iterations + p - 1],
-                              (unsigned long long)__double_as_longlong(
-                                  snan ? __longlong_as_double(0x7ff8000000000000ll) : smx));
-                __builtin_amdgcn_s_waitcnt(0);
-                __hip_atomic_fetch_add(&P.scene_done[(size_t)sc * (P.iterations + 1) + p], 1u, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-            }
             // Publish deltaV to this group's slots.  The emptied slot must be
             // seen empty before these values can be seen (the chain of
             // updates they start leads to that slot's next writer), so the
@@ -1430,6 +1429,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 for (int t = 0; t < nl; t++) {
                     lam_dst(F, gd.x + t, p)[li] = s_lam[row][t][li];
                 }
+            }
+            // per-scene exits: the scene's max for this sweep, then this group
+            // counts as done (after the publish, which is what later steps wait for)
+            if (P.per_scene && p >= 1 && has_group && lead) {
+                if (active && !(smx <= 0))
+                    atomicMax(&P.scene_max[(size_t)sc * P.iterations + p - 1],
+                              (unsigned long long)__double_as_longlong(
+                                  snan ? __longlong_as_double(0x7ff8000000000000ll) : smx));
+                __builtin_amdgcn_s_waitcnt(0);
+                __hip_atomic_fetch_add(&P.scene_done[(size_t)sc * (P.iterations + 1) + p], 1u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
             }
         }
         if (p >= 1) {

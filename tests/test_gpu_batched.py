@@ -108,3 +108,72 @@ def test_batched_merging_matches_oracle(gpu_lib, schedule):
         assert_bits_equal(f"step {step} per-scene residual", last_g, last_o[:len(last_g)])
         sweeps.update(int(u) for u in used_g)
     assert merges >= 1 and len(sweeps) >= 2
+
+
+@pytest.mark.parametrize("schedule", [0, 1])
+def test_batched_scene_stays_stopped_when_a_later_sweep_rises(gpu_lib, schedule):
+    """Scene 0's tolerance is set between its first sweep under it and the
+    next sweep, which rises over it again (found on the oracle running that
+    scene alone in the GPU's order).  The scene stops at that first sweep,
+    as it does alone, and stays stopped: the dataflow kernel learns a scene's
+    max two sweeps late, so its passes after the first skipped one must not
+    see the higher sweep and resume (their inputs were never published).
+    Steps without such a sweep run with tolerance 0 on both sides."""
+    subs = [workloads.box_cluster(seed=6, n=64, spheres=10), workloads.box_cluster(seed=2, n=27, spheres=4)]
+    scene = workloads.batch(subs)
+    params = dict(PARAMS, tolerance=0.0, pgs_schedule=schedule)
+    ctx = Context()
+    ctx.upload(scene)
+    ctx.set_params(**params)
+    probe, ref = Oracle(subs[0]), Oracle(subs[0])
+    probe.set_params(**params)
+    ref.set_params(**params)
+    dt = 0.01
+    sl = slice(0, subs[0].n)
+    tested = False
+    for step in range(10):
+        ctx.clear_bodies(); ctx.apply_gravity(); ctx.update_contacts_map(); ctx.detect(dt)
+        ctx.warm_start(); ctx.redo_warm_start()
+        order = ctx.solve_order()
+        g0 = ctx.contacts()
+        mine = np.nonzero(g0["body1"] < subs[0].n)[0]
+        local = -np.ones(len(g0["body1"]), dtype=np.int64)
+        local[mine] = np.arange(len(mine))
+        order0 = np.array([local[c] for c in order if local[c] >= 0], dtype=np.int32)
+        for o in (probe, ref):
+            o.clear_bodies(); o.apply_gravity(); o.update_contacts_map(); o.detect()
+            o.warm_start(); o.redo_warm_start()
+        _, _, ch = probe.solve(order0, dt)  # log of each sweep's max |dl|
+        assert len(ch) == params["iterations"]
+        # sweep t + 1 (ch[t]) is under every earlier one and the next is
+        # higher; a few sweeps in, with passes left for the kernel to skip
+        cands = [t for t in range(3, len(ch) - 6) if ch[t] < np.min(ch[:t]) and ch[t + 1] > ch[t]]
+        tol = 0.0
+        if cands and not tested:
+            t = cands[0]
+            tol = float(np.exp(0.5 * (ch[t] + min(ch[t + 1], np.min(ch[:t])))))
+            ctx.set_params(tolerance=tol)
+            ref.set_params(tolerance=tol)
+        it_g, _ = ctx.solve(dt)
+        it_o, _, _ = ref.solve(order0, dt)
+        if tol > 0:
+            used_g, _ = ctx.scene_iterations()
+            assert it_o == t + 1
+            assert used_g[0] == it_o, f"step {step}: scene 0 {used_g[0]} sweeps on the GPU, {it_o} alone"
+            assert it_g == max(used_g)
+            ctx.set_params(tolerance=0.0)
+            ref.set_params(tolerance=0.0)
+            tested = True
+        g = ctx.contacts()
+        gk = {key: val[mine] for key, val in g.items()}
+        assert_contacts_equal(gk, ref.contacts(), keys=["body1", "body2", "lambda", "state"])
+        ctx.advance_velocities(dt); ctx.advance_positions(dt)
+        for o in (probe, ref):
+            o.advance_velocities(dt); o.advance_positions(dt)
+        x, R, v, w = ctx.pull_state()
+        xo, Ro, vo, wo = ref.pull_state()
+        for name, a, b in (("x", x[sl], xo), ("R", R[sl], Ro), ("v", v[sl], vo), ("w", w[sl], wo)):
+            assert_bits_equal(f"step {step} scene 0 {name}", a, b)
+        if tested:
+            break
+    assert tested, "no step had a sweep under all earlier ones followed by a rise"
