@@ -14,6 +14,7 @@
 #include <cmath>
 #include <functional>
 #include <new>
+#include <thread>
 #include <unordered_map>
 
 #include "am_ctx.hpp"
@@ -578,6 +579,39 @@ M3 get_op(V3 bx, double mass, V3 com) {
     return op;
 }
 
+// Runs body(k) for every k in [0, n) on up to kEventThreads host threads, in
+// contiguous ranges of about equal total weight (serially when the work is
+// small).  The jobs of an event write disjoint entities, so the result is
+// the serial one.
+constexpr int kEventThreads = 16;
+constexpr long kWorkPerThread = 4096;  // weight units (bodies) worth a thread
+template <class W, class F>
+void parallel_jobs(int n, W weight, F body) {
+    long total = 0;
+    for (int k = 0; k < n; k++) total += weight(k);
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nt = (int)std::min<long>(std::min<long>(hw, kEventThreads), std::max(total / kWorkPerThread, 1L));
+    if (nt <= 1 || n <= 1) {
+        for (int k = 0; k < n; k++) body(k);
+        return;
+    }
+    std::vector<int> start(nt + 1, n);
+    start[0] = 0;
+    long acc = 0;
+    for (int k = 0, t = 1; k < n && t < nt; k++) {
+        acc += weight(k);
+        while (t < nt && acc * nt >= total * t) start[t++] = k + 1;
+    }
+    std::vector<std::thread> pool;
+    pool.reserve(nt - 1);
+    for (int t = 1; t < nt; t++)
+        pool.emplace_back([&, t] {
+            for (int k = start[t]; k < start[t + 1]; k++) body(k);
+        });
+    for (int k = start[0]; k < start[1]; k++) body(k);
+    for (auto& th : pool) th.join();
+}
+
 // Items listed per body in item order (compressed rows): for each item i,
 // first(i) and second(i) name its bodies (-1 for none).
 struct Adjacency {
@@ -809,7 +843,9 @@ void update_bodies_transformations(Event& E, int ce, const Ent& C) {
 // the pose each one stored last.
 void flush_transformations(Event& E) {
     am_ctx* c = E.c;
-    for (int ce : E.b2c_list) {
+    const std::vector<int>& L = E.b2c_list;
+    parallel_jobs((int)L.size(), [&](int k) { return (long)E.members[L[k]].size(); }, [&](int k) {
+        const int ce = L[k];
         E.b2c_dirty[ce] = 0;
         const V3 xc = v3(&E.x[ce * 3]);
         const M3 Rc = m3(&E.R[ce * 9]);
@@ -817,7 +853,7 @@ void flush_transformations(Event& E) {
             body_to_collection(xc, Rc, v3(&E.x[b * 3]), m3(&E.R[b * 9]), &c->b2c[(size_t)b * 12]);
             E.b2c_changed[b] = 1;
         }
-    }
+    });
     E.b2c_list.clear();
 }
 
@@ -1666,12 +1702,25 @@ int merge_unmerge(am_ctx* c, double dt, int* n_events) {
         if (c->ibpc[i].flag) cuts[c->ibpc[i].coll].push_back(i);
     // per-body marks, cleared after each collection (only its members are set)
     std::vector<char> handled(c->n_phys, 0), remained(c->n_phys, 0), in_sub(c->n_phys, 0);
-    const std::vector<int> top0 = c->slot_entity;
-    for (int ce : top0) {
-        if (c->desc[ce].geom != AM_GEOM_COLLECTION || (c->flags[ce] & AM_BODY_SLEEPING)) continue;
-        auto found = cuts.find(ce);
-        if (found == cuts.end()) continue;
-        const std::vector<int>& cut = found->second;
+    // the collections with flagged pairs, in top-level list order
+    std::vector<int> order;
+    for (const auto& kv : cuts) {
+        const int ce = kv.first;
+        if (ce >= 0 && ce < (int)E.top_pos.size() && E.top_pos[ce] >= 0 && c->desc[ce].geom == AM_GEOM_COLLECTION &&
+            !(c->flags[ce] & AM_BODY_SLEEPING))
+            order.push_back(ce);
+    }
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return E.top_pos[a] < E.top_pos[b]; });
+    // removeBodies of the collections that stay, run after the walk: they
+    // touch only their own collection and members
+    struct Removal {
+        int ce;
+        Ent C;
+        std::vector<int> gone;
+    };
+    std::vector<Removal> removals_left;
+    for (int ce : order) {
+        const std::vector<int>& cut = cuts.find(ce)->second;
         for (int i : cut) incoll[i] = 0;
         Ent C = E.load(ce);
         const std::vector<int> bodies = E.members[ce];
@@ -1732,10 +1781,7 @@ int merge_unmerge(am_ctx* c, double dt, int* n_events) {
             std::vector<int> gone;
             for (int b : bodies)
                 if (handled[b] && !remained[b]) gone.push_back(b);
-            if (!gone.empty()) {
-                remove_bodies(E, ce, C, gone);
-                E.store_collection(ce, C);
-            }
+            if (!gone.empty()) removals_left.push_back(Removal{ce, C, std::move(gone)});
         }
         for (int b : bodies) handled[b] = remained[b] = 0;
         if (!new_bodies.empty()) {
@@ -1745,6 +1791,12 @@ int merge_unmerge(am_ctx* c, double dt, int* n_events) {
             if (remove_coll) removals.push_back(ce);
         }
     }
+    parallel_jobs((int)removals_left.size(), [&](int k) { return (long)E.members[removals_left[k].ce].size(); },
+                  [&](int k) {
+                      Removal& R = removals_left[k];
+                      remove_bodies(E, R.ce, R.C, R.gone);
+                      E.store_collection(R.ce, R.C);
+                  });
     for (int e : additions) E.top_append(e);
     for (int e : removals) {
         E.top_remove(e);
