@@ -1228,8 +1228,6 @@ int commit_event(am_ctx* c, Event& E) {
     c->members.clear();
     c->slot.assign(n, -1);
     c->sub.assign(n, 0);
-    for (int e = 0; e < n; e++)
-        if (c->desc[e].geom != AM_GEOM_COLLECTION) c->parent[e] = c->parent[e];
     for (int s_ = 0; s_ < (int)c->slot_entity.size(); s_++) {
         int e = c->slot_entity[s_];
         c->slot[e] = s_;

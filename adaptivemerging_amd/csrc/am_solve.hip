@@ -1105,6 +1105,9 @@ AMD_INLINE void row_contact_fast(const FlowArgs& F, const WaveRec& c, int k, int
 // w, w + W, ... of every pass in order.
 constexpr int kRowsPerWave = 4;
 
+// kScenes: a batched solve with per-scene exits (P.per_scene set); the
+// single-scene kernel carries none of their code.
+template <bool kScenes>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_pgs_flow(FlowArgs F) {
     const PgsArgs& P = F.P;
     const int* cs = P.colour_start;
@@ -1189,7 +1192,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             nx_gd = gdesc[q];
             nx_fd = fdesc[q];
             nx_pr = gpred[q];
-            if (P.per_scene) nx_sc = P.g_scene[q];
+            if (kScenes) nx_sc = P.g_scene[q];
         }
         for (int st = w; st < n_steps; st += W) {
             // descriptors of this step were loaded during the previous one
@@ -1204,7 +1207,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 nx_gd = gdesc[q];
                 nx_fd = fdesc[q];
                 nx_pr = gpred[q];
-                if (P.per_scene) nx_sc = P.g_scene[q];
+                if (kScenes) nx_sc = P.g_scene[q];
             }
             // Per-scene exits: a row whose scene was under the tolerance at
             // sweep p-2 or p-3 (or p-1 before the last sweep) skips this
@@ -1219,8 +1222,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             // in the (pass, step) order, so the earliest undone step can
             // always run.  Until then the row counts as active (its records
             // are copied and its inputs polled for).
-            const int t1 = p - 2, t2 = (p == lastp) ? p - 1 : 0;
-            const bool check = P.per_scene && p >= 1 && P.check_tolerance && (t1 >= 1 || t2 >= 1);
             bool active = has_group;
             int n = active ? gd.y - gd.x : 0;
             int nmax = n;
@@ -1276,27 +1277,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             }
             {
                 bool got = src == nullptr;
-                // exit check of this row's scene: 0 waiting for the scene's
-                // groups, 1 resolved (lane 0 of each row; the others resolved)
-                bool resolved = !(check && has_group && li == 0), skip = false;
-                const unsigned want = check && has_group ? P.scene_groups[sc] : 0u;
-                const size_t base = (size_t)sc * (P.iterations + 1);
                 long spins = 0;
-                while (true) {
-                    if (!resolved) {
-                        const unsigned c1 = ld_cnt(&P.scene_done[base + (t1 >= 1 ? t1 : t2)]);
-                        const unsigned c2 = ld_cnt(&P.scene_done[base + (t2 >= 1 ? t2 : t1)]);
-                        if (c1 == want && c2 == want) {
-                            const unsigned long long* m = &P.scene_max[(size_t)sc * P.iterations];
-                            auto under = [&](int t) {
-                                return __longlong_as_double((long long)__hip_atomic_load(
-                                           const_cast<unsigned long long*>(&m[t - 1]), __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT)) < P.tolerance;
-                            };
-                            skip = (t1 >= 1 && under(t1)) || (t1 >= 2 && under(t1 - 1)) || (t2 >= 1 && under(t2));
-                            resolved = true;
-                        }
+                auto spin = [&]() -> bool {  // false: give up (error flag set)
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > kFlowSpinLimit ||
+                        ((spins & 255) == 0 && uniform(__hip_atomic_load(P.error, __ATOMIC_RELAXED,
+                                                                          __HIP_MEMORY_SCOPE_AGENT)))) {
+                        if (lane == 0) atomicOr(P.error, 1);
+                        return false;
                     }
+                    return true;
+                };
+                auto poll = [&]() {
                     if (!got) {
                         const double v = ld_coherent(src);
                         if ((unsigned long long)__double_as_longlong(v) != kSlotEmpty) {
@@ -1304,27 +1296,47 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                             got = true;
                         }
                     }
-                    bool done = got;
-                    if (check) {
-                        // a skipped row reads no inputs (its scene's other
-                        // groups skip this pass too and publish none)
+                };
+                const int t1 = p - 2, t2 = (p == lastp) ? p - 1 : 0;
+                if (!(kScenes && p >= 1 && P.check_tolerance && (t1 >= 1 || t2 >= 1))) {
+                    while (true) {
+                        poll();
+                        if (__all(got)) break;
+                        if (!spin()) return;
+                    }
+                } else {
+                    // lane 0 of each row resolves its scene's exit check
+                    // while the row polls; a skipped row reads no inputs (its
+                    // scene's other groups skip this pass too and publish none)
+                    bool resolved = !(has_group && li == 0), skip = false;
+                    const unsigned want = has_group ? P.scene_groups[sc] : 0u;
+                    const size_t base = (size_t)sc * (P.iterations + 1);
+                    while (true) {
+                        if (!resolved) {
+                            const unsigned c1 = ld_cnt(&P.scene_done[base + (t1 >= 1 ? t1 : t2)]);
+                            const unsigned c2 = ld_cnt(&P.scene_done[base + (t2 >= 1 ? t2 : t1)]);
+                            if (c1 == want && c2 == want) {
+                                const unsigned long long* m = &P.scene_max[(size_t)sc * P.iterations];
+                                auto under = [&](int t) {
+                                    return __longlong_as_double((long long)__hip_atomic_load(
+                                               const_cast<unsigned long long*>(&m[t - 1]), __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT)) < P.tolerance;
+                                };
+                                skip = (t1 >= 1 && under(t1)) || (t1 >= 2 && under(t1 - 1)) || (t2 >= 1 && under(t2));
+                                resolved = true;
+                            }
+                        }
+                        poll();
                         const bool row_resolved = __shfl(resolved, row * 16, 64);
                         const bool row_skip = __shfl(skip, row * 16, 64);
-                        done = row_resolved && (row_skip || got);
+                        if (__all(row_resolved && (row_skip || got))) break;
+                        if (!spin()) return;
                     }
-                    if (__all(done)) break;
-                    __builtin_amdgcn_s_sleep(1);
-                    if (++spins > kFlowSpinLimit ||
-                        ((spins & 255) == 0 && uniform(__hip_atomic_load(P.error, __ATOMIC_RELAXED,
-                                                                          __HIP_MEMORY_SCOPE_AGENT)))) {
-                        if (lane == 0) atomicOr(P.error, 1);
-                        return;
+                    if (__shfl(skip, row * 16, 64) && has_group) {
+                        active = false;
+                        n = 0;
+                        src = nullptr;
                     }
-                }
-                if (check && __shfl(skip, row * 16, 64) && has_group) {
-                    active = false;
-                    n = 0;
-                    src = nullptr;
                 }
             }
             const bool last = p == lastp;
@@ -1412,6 +1424,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             }
             mx = fmax(mx, smx);
             nan = nan || snan;
+            // per-scene exits: the scene's max for this sweep (the wait
+            // before the publish below makes it visible before the group
+            // counts as done)
+            if (kScenes && p >= 1 && has_group && lead && active && !(smx <= 0))
+                atomicMax(&P.scene_max[(size_t)sc * P.iterations + p - 1],
+                          (unsigned long long)__double_as_longlong(
+                              snan ? __longlong_as_double(0x7ff8000000000000ll) : smx));
             // Publish deltaV to this group's slots.  The emptied slot must be
             // seen empty before these values can be seen (the chain of
             // updates they start leads to that slot's next writer), so the
@@ -1430,17 +1449,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                     lam_dst(F, gd.x + t, p)[li] = s_lam[row][t][li];
                 }
             }
-            // per-scene exits: the scene's max for this sweep, then this group
-            // counts as done (after the publish, which is what later steps wait for)
-            if (P.per_scene && p >= 1 && has_group && lead) {
-                if (active && !(smx <= 0))
-                    atomicMax(&P.scene_max[(size_t)sc * P.iterations + p - 1],
-                              (unsigned long long)__double_as_longlong(
-                                  snan ? __longlong_as_double(0x7ff8000000000000ll) : smx));
-                __builtin_amdgcn_s_waitcnt(0);
+            // this group counts as done for its scene, after the publish
+            // (which is what later steps wait for)
+            if (kScenes && p >= 1 && has_group && lead)
                 __hip_atomic_fetch_add(&P.scene_done[(size_t)sc * (P.iterations + 1) + p], 1u, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
-            }
         }
         if (p >= 1) {
             if (__any(nan)) mx = __longlong_as_double(0x7ff8000000000000ll);
@@ -1514,17 +1527,22 @@ int pgs_flow_max_blocks(bool* ok) {
     int dev = 0, n_cu = 0, per_cu = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pgs_flow, 64, 0);
+    int per_cu_scenes = 0;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pgs_flow<false>, 64, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_scenes, k_pgs_flow<true>, 64, 0);
+    per_cu = std::min(per_cu, per_cu_scenes);
     *ok = per_cu > 0;
     return n_cu * (per_cu > 0 ? per_cu : 1);
 }
 
 hipError_t launch_pgs_flow(hipStream_t s, const FlowArgs& F, int grid, bool plain, int n_dv, const BodiesView& B) {
     if (plain) {
-        hipLaunchKernelGGL(k_pgs_flow, dim3(grid), dim3(64), 0, s, F);
+        if (F.P.per_scene) hipLaunchKernelGGL(k_pgs_flow<true>, dim3(grid), dim3(64), 0, s, F);
+        else hipLaunchKernelGGL(k_pgs_flow<false>, dim3(grid), dim3(64), 0, s, F);
     } else {
+        const void* kernel = F.P.per_scene ? (const void*)k_pgs_flow<true> : (const void*)k_pgs_flow<false>;
         void* args[] = {(void*)&F};
-        hipError_t e = hipLaunchCooperativeKernel((const void*)k_pgs_flow, dim3(grid), dim3(64), args, 0, s);
+        hipError_t e = hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(64), args, 0, s);
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_flow_finish, dim3(1), dim3(64), 0, s, F.P);
