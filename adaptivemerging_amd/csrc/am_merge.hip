@@ -1323,16 +1323,13 @@ int merge_member_transforms(am_ctx* c) {
 // then the external pairs never reached, then the internal ones.  Groups
 // (contiguous same-pair runs of the combined list, external first) stand for
 // the pairs; the layers order the colouring of the single sweep.
-__global__ void k_bfs_incidence(int n_groups, const int* ga, const int* gb, int* count, const int* start,
-                                int* cursor, int* items) {
+// the bodies of the groups of layer k get mark k (every writer of a body
+// writes the same value)
+__global__ void k_bfs_mark(int n_groups, const int* ga, const int* gb, const int* layer, int k, int* mark) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n_groups) return;
-    const int e2[2] = {ga[g], gb[g]};
-    for (int s = 0; s < 2; s++) {
-        if (s == 1 && e2[1] == e2[0]) break;
-        if (!start) atomicAdd(&count[e2[s]], 1);
-        else items[start[e2[s]] + atomicAdd(&cursor[e2[s]], 1)] = g;
-    }
+    if (g >= n_groups || layer[g] != k) return;
+    mark[ga[g]] = k;
+    mark[gb[g]] = k;
 }
 __global__ void k_bfs_seed(int n_groups, const int* group_start, const int* is_new, int n_ext, int* layer,
                            int* n_seeded) {
@@ -1349,21 +1346,16 @@ __global__ void k_bfs_seed(int n_groups, const int* group_start, const int* is_n
     if (l == 0) atomicAdd(n_seeded, 1);
 }
 // one breadth-first layer: a group not yet reached joins layer k + 1 when a
-// pair of layer k shares one of its bodies (a neighbour set this round reads
-// k + 1, never k, so the result does not depend on scheduling)
-__global__ void k_bfs_round(int n_groups, const int* ga, const int* gb, const int* start, const int* count,
-                            const int* items, int* layer, int k, int* changed) {
+// pair of layer k shares one of its bodies, i.e. when one of its bodies has
+// mark k (groups set this round read k + 1, never k, so the result does not
+// depend on scheduling)
+__global__ void k_bfs_round(int n_groups, const int* ga, const int* gb, const int* mark, int* layer, int k,
+                            int* changed) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n_groups || layer[g] >= 0) return;
-    const int e2[2] = {ga[g], gb[g]};
-    for (int s = 0; s < 2; s++) {
-        const int e = e2[s];
-        for (int q = start[e]; q < start[e] + count[e]; q++)
-            if (layer[items[q]] == k) {
-                layer[g] = k + 1;
-                atomicAdd(changed, 1);
-                return;
-            }
+    if (mark[ga[g]] == k || mark[gb[g]] == k) {
+        layer[g] = k + 1;
+        atomicAdd(changed, 1);
     }
 }
 __global__ void k_bfs_rest(int n_groups, const int* group_start, int n_ext, int* layer, int lmax) {
@@ -1377,19 +1369,10 @@ static int organized_layers(am_ctx* c, int n_ext) {
     hipStream_t s = c->stream;
     const int ng = c->n_groups, ne = c->n_ent;
     AM_HIP(c, c->g_layer.ensure(std::max(ng, 1)));
-    AM_HIP(c, c->bfs_count.ensure(ne + 1));
-    AM_HIP(c, c->bfs_start.ensure(ne + 1));
-    AM_HIP(c, c->bfs_cursor.ensure(ne + 1));
-    AM_HIP(c, c->bfs_items.ensure(2 * (size_t)std::max(ng, 1)));
+    AM_HIP(c, c->bfs_mark.ensure(ne + 1));
     int* sc = c->d_scalars.p;
-    fill_int(s, c->bfs_count.p, ne + 1, 0);
-    fill_int(s, c->bfs_cursor.p, ne + 1, 0);
+    fill_int(s, c->bfs_mark.p, ne + 1, -1);
     dim3 grid((ng + 255) / 256), block(256);
-    hipLaunchKernelGGL(k_bfs_incidence, grid, block, 0, s, ng, c->g_a.p, c->g_b.p, c->bfs_count.p, nullptr, nullptr,
-                       nullptr);
-    exclusive_scan(s, c->bfs_count.p, c->bfs_start.p, ne + 1, sc + 3, c->d_scan_temp.p);
-    hipLaunchKernelGGL(k_bfs_incidence, grid, block, 0, s, ng, c->g_a.p, c->g_b.p, c->bfs_count.p, c->bfs_start.p,
-                       c->bfs_cursor.p, c->bfs_items.p);
     fill_int(s, sc + 12, 1, 0);
     hipLaunchKernelGGL(k_bfs_seed, grid, block, 0, s, ng, c->g_start.p, c->c_new.p, n_ext, c->g_layer.p, sc + 12);
     AM_HIP(c, hipGetLastError());
@@ -1400,8 +1383,9 @@ static int organized_layers(am_ctx* c, int n_ext) {
         lmax = 0;
         for (int k = 0; k < ng; k++) {
             fill_int(s, sc + 12, 1, 0);
-            hipLaunchKernelGGL(k_bfs_round, grid, block, 0, s, ng, c->g_a.p, c->g_b.p, c->bfs_start.p,
-                               c->bfs_count.p, c->bfs_items.p, c->g_layer.p, k, sc + 12);
+            hipLaunchKernelGGL(k_bfs_mark, grid, block, 0, s, ng, c->g_a.p, c->g_b.p, c->g_layer.p, k, c->bfs_mark.p);
+            hipLaunchKernelGGL(k_bfs_round, grid, block, 0, s, ng, c->g_a.p, c->g_b.p, c->bfs_mark.p, c->g_layer.p,
+                               k, sc + 12);
             AM_HIP(c, hipGetLastError());
             if ((r = read_scalars(c, 12, 1))) return r;
             if (c->h_scalars[12] == 0) break;

@@ -1507,20 +1507,34 @@ void launch_scene_finish(hipStream_t s, const PgsArgs& P) {
         hipLaunchKernelGGL(k_scene_finish, dim3((P.n_scenes + 63) / 64), dim3(64), 0, s, P);
 }
 
-__global__ void k_group_scene(const int* group_desc, int n_groups, const int* order, const int* cb1,
-                              const am_body_desc* desc, int* g_scene, unsigned* scene_groups) {
-    int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n_groups) return;
-    const int first = group_desc[q * 4];
-    const int sc = desc[cb1[order[first]]].scene;
-    g_scene[q] = sc;
-    atomicAdd(&scene_groups[sc], 1u);
+// Groups are counted per scene in an LDS histogram per workgroup (when the
+// scenes fit), so each workgroup adds to a scene's global count once instead
+// of every group contending for its scene's counter.
+constexpr int kSceneHist = 4096;
+__global__ __launch_bounds__(1024) void k_group_scene(const int* group_desc, int n_groups, const int* order,
+                                                      const int* cb1, const am_body_desc* desc, int n_scenes,
+                                                      int* g_scene, unsigned* scene_groups) {
+    __shared__ unsigned hist[kSceneHist];
+    const bool local = n_scenes <= kSceneHist;
+    if (local)
+        for (int i = threadIdx.x; i < n_scenes; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n_groups; q += gridDim.x * blockDim.x) {
+        const int sc = desc[cb1[order[group_desc[q * 4]]]].scene;
+        g_scene[q] = sc;
+        if (local) atomicAdd(&hist[sc], 1u);
+        else atomicAdd(&scene_groups[sc], 1u);
+    }
+    __syncthreads();
+    if (local)
+        for (int i = threadIdx.x; i < n_scenes; i += blockDim.x)
+            if (hist[i]) atomicAdd(&scene_groups[i], hist[i]);
 }
 void launch_group_scene(hipStream_t s, const int* group_desc, int n_groups, const int* order, const int* cb1,
-                        const am_body_desc* desc, int* g_scene, unsigned* scene_groups) {
+                        const am_body_desc* desc, int n_scenes, int* g_scene, unsigned* scene_groups) {
     if (n_groups > 0)
-        hipLaunchKernelGGL(k_group_scene, dim3((n_groups + 255) / 256), dim3(256), 0, s, group_desc, n_groups, order,
-                           cb1, desc, g_scene, scene_groups);
+        hipLaunchKernelGGL(k_group_scene, dim3(std::min((n_groups + 4095) / 4096, 256)), dim3(1024), 0, s, group_desc,
+                           n_groups, order, cb1, desc, n_scenes, g_scene, scene_groups);
 }
 
 int pgs_flow_max_blocks(bool* ok) {

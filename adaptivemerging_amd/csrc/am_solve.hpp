@@ -102,7 +102,7 @@ void launch_colour_layout(hipStream_t s, int n_groups, const uint64_t* sorted, c
 void launch_setup(hipStream_t s, const SetupArgs& S, int cap);
 // scene of every sorted group (from its first contact's body 1) and the group count per scene
 void launch_group_scene(hipStream_t s, const int* group_desc, int n_groups, const int* order, const int* cb1,
-                        const am_body_desc* desc, int* g_scene, unsigned* scene_groups);
+                        const am_body_desc* desc, int n_scenes, int* g_scene, unsigned* scene_groups);
 // sweeps used per scene (first sweep under the tolerance, else all)
 void launch_scene_finish(hipStream_t s, const PgsArgs& P);
 void launch_unpack(hipStream_t s, const SetupArgs& S, int cap);
