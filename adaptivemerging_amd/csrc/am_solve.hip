@@ -1145,6 +1145,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     const LaneMap lmap = lane_map(li);
     int chk = 0;  // sweeps 1..chk are known not to have converged
 
+    // per-scene exits: the scene counter this lane's row still has to add
+    // its group to (added once the step after it has waited for its memory
+    // operations, or at the end of the pass)
+    unsigned* pend_done = nullptr;
     for (int p = first; p <= lastp; p++) {
         // pass gates: sweep p-2 must be known not converged (p-1 for the last sweep)
         if (P.check_tolerance) {
@@ -1312,21 +1316,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                     const unsigned want = has_group ? P.scene_groups[sc] : 0u;
                     const size_t base = (size_t)sc * (P.iterations + 1);
                     while (true) {
+                        // the slot and the scene's counters are read together
+                        double v = 0.0;
+                        if (!got) v = ld_coherent(src);
+                        unsigned c1 = 0, c2 = 0;
                         if (!resolved) {
-                            const unsigned c1 = ld_cnt(&P.scene_done[base + (t1 >= 1 ? t1 : t2)]);
-                            const unsigned c2 = ld_cnt(&P.scene_done[base + (t2 >= 1 ? t2 : t1)]);
-                            if (c1 == want && c2 == want) {
-                                const unsigned long long* m = &P.scene_max[(size_t)sc * P.iterations];
-                                auto under = [&](int t) {
-                                    return __longlong_as_double((long long)__hip_atomic_load(
-                                               const_cast<unsigned long long*>(&m[t - 1]), __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT)) < P.tolerance;
-                                };
-                                skip = (t1 >= 1 && under(t1)) || (t1 >= 2 && under(t1 - 1)) || (t2 >= 1 && under(t2));
-                                resolved = true;
-                            }
+                            c1 = ld_cnt(&P.scene_done[base + (t1 >= 1 ? t1 : t2)]);
+                            c2 = ld_cnt(&P.scene_done[base + (t2 >= 1 ? t2 : t1)]);
                         }
-                        poll();
+                        if (!got && (unsigned long long)__double_as_longlong(v) != kSlotEmpty) {
+                            dv = v;
+                            got = true;
+                        }
+                        // the scene's maxima are read only once its counters
+                        // are complete (they are final then)
+                        if (!resolved && c1 == want && c2 == want) {
+                            const unsigned long long* m = &P.scene_max[(size_t)sc * P.iterations];
+                            auto under = [&](int t) {
+                                return __longlong_as_double((long long)__hip_atomic_load(
+                                           const_cast<unsigned long long*>(&m[t - 1]), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT)) < P.tolerance;
+                            };
+                            skip = (t1 >= 1 && under(t1)) || (t1 >= 2 && under(t1 - 1)) || (t2 >= 1 && under(t2));
+                            resolved = true;
+                        }
                         const bool row_resolved = __shfl(resolved, row * 16, 64);
                         const bool row_skip = __shfl(skip, row * 16, 64);
                         if (__all(row_resolved && (row_skip || got))) break;
@@ -1341,6 +1354,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             }
             const bool last = p == lastp;
             __builtin_amdgcn_s_waitcnt(0);  // LDS copies landed
+            if (kScenes && pend_done) {
+                // the previous step's scene max has landed (the wait above)
+                __hip_atomic_fetch_add(pend_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                pend_done = nullptr;
+            }
             // empty the slot read above only now, so that this wait does not
             // include the store (the publish below waits for it)
             if (src) st_coherent(src, __longlong_as_double((long long)kSlotEmpty));
@@ -1424,13 +1442,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             }
             mx = fmax(mx, smx);
             nan = nan || snan;
-            // per-scene exits: the scene's max for this sweep (the wait
-            // before the publish below makes it visible before the group
-            // counts as done)
-            if (kScenes && p >= 1 && has_group && lead && active && !(smx <= 0))
-                atomicMax(&P.scene_max[(size_t)sc * P.iterations + p - 1],
-                          (unsigned long long)__double_as_longlong(
-                              snan ? __longlong_as_double(0x7ff8000000000000ll) : smx));
             // Publish deltaV to this group's slots.  The emptied slot must be
             // seen empty before these values can be seen (the chain of
             // updates they start leads to that slot's next writer), so the
@@ -1449,11 +1460,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                     lam_dst(F, gd.x + t, p)[li] = s_lam[row][t][li];
                 }
             }
-            // this group counts as done for its scene, after the publish
-            // (which is what later steps wait for)
-            if (kScenes && p >= 1 && has_group && lead)
-                __hip_atomic_fetch_add(&P.scene_done[(size_t)sc * (P.iterations + 1) + p], 1u, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
+            // per-scene exits, after the publish (which is what later steps
+            // wait for): the scene's max for this sweep now, the group's
+            // count once a later wait has seen the max land
+            if (kScenes && p >= 1 && has_group && lead) {
+                if (active && !(smx <= 0))
+                    atomicMax(&P.scene_max[(size_t)sc * P.iterations + p - 1],
+                              (unsigned long long)__double_as_longlong(
+                                  snan ? __longlong_as_double(0x7ff8000000000000ll) : smx));
+                pend_done = &P.scene_done[(size_t)sc * (P.iterations + 1) + p];
+            }
+        }
+        if (kScenes && pend_done) {
+            __builtin_amdgcn_s_waitcnt(0);
+            __hip_atomic_fetch_add(pend_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            pend_done = nullptr;
         }
         if (p >= 1) {
             if (__any(nan)) mx = __longlong_as_double(0x7ff8000000000000ll);
