@@ -868,7 +868,7 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
         AM_HIP(c, c->scene_done.ensure(ns * (iterations + 1)));
         fill_int(s, (int*)c->scene_groups.p, (int)ns, 0);
         fill_int(s, (int*)c->scene_max.p, (int)(2 * ns * iterations), 0);
-        fill_int(s, (int*)c->scene_done.p, (int)(ns * (iterations + 1)), 0);
+        fill_int(s, (int*)c->scene_done.p, (int)(2 * ns * (iterations + 1)), 0);
         launch_group_scene(s, c->s_group_desc.p, c->n_groups, c->s_order.p, c->c_b1.p, c->d_desc.p, c->n_scenes,
                            c->g_scene.p, c->scene_groups.p);
         G.g_scene = c->g_scene.p;

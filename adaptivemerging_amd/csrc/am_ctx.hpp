@@ -192,7 +192,8 @@ struct am_ctx {
     bool scene_solve_valid = false;
     int scene_iters = 1;  // sweep cap of that solve (the stride of scene_max)
     DBuf<int> g_scene, scene_used;
-    DBuf<unsigned> scene_done, scene_groups;
+    DBuf<unsigned long long> scene_done;
+    DBuf<unsigned> scene_groups;
     DBuf<unsigned long long> scene_max;
     // sleeping (Sleeping.java): per-entity metric histories
     DBuf<double> d_sleep_hist;          // kSleepHistStride per entity, oldest first

@@ -866,6 +866,9 @@ hipError_t launch_pgs(hipStream_t s, const PgsArgs& P, int grid, int block, bool
 AMD_INLINE double* slot_at(const FlowArgs& F, int parity, int incidence) {
     return F.slot + ((size_t)parity * 2 * F.n_groups + incidence) * kSlotDoubles;
 }
+AMD_INLINE unsigned long long ld_cnt64(const unsigned long long* p) {
+    return __hip_atomic_load(const_cast<unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 AMD_INLINE unsigned ld_cnt(const unsigned* p) {
     return __hip_atomic_load(const_cast<unsigned*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -1148,7 +1151,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     // per-scene exits: the scene counter this lane's row still has to add
     // its group to (added once the step after it has waited for its memory
     // operations, or at the end of the pass)
-    unsigned* pend_done = nullptr;
+    unsigned long long* pend_done = nullptr;
+    unsigned long long pend_add = 0;
     for (int p = first; p <= lastp; p++) {
         // pass gates: sweep p-2 must be known not converged (p-1 for the last sweep)
         if (P.check_tolerance) {
@@ -1314,30 +1318,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                     // scene's other groups skip this pass too and publish none)
                     bool resolved = !(has_group && li == 0), skip = false;
                     const unsigned want = has_group ? P.scene_groups[sc] : 0u;
-                    const size_t base = (size_t)sc * (P.iterations + 1);
+                    const unsigned long long* cnt = &P.scene_done[(size_t)sc * (P.iterations + 1)];
+                    // sweeps checked: t1, t1 - 1 and t2 where they exist (a
+                    // missing one repeats t1 or t2)
+                    const int ta = t1 >= 1 ? t1 : t2, tb = t1 >= 2 ? t1 - 1 : ta, tc = t2 >= 1 ? t2 : ta;
                     while (true) {
-                        // the slot and the scene's counters are read together
+                        // the slot and the scene's words are read together; a
+                        // sweep's word is final once all the scene's groups
+                        // have counted themselves in it, and the scene was
+                        // under the tolerance there if none was over it
                         double v = 0.0;
                         if (!got) v = ld_coherent(src);
-                        unsigned c1 = 0, c2 = 0;
+                        unsigned long long wa = 0, wb = 0, wc = 0;
                         if (!resolved) {
-                            c1 = ld_cnt(&P.scene_done[base + (t1 >= 1 ? t1 : t2)]);
-                            c2 = ld_cnt(&P.scene_done[base + (t2 >= 1 ? t2 : t1)]);
+                            wa = ld_cnt64(&cnt[ta]);
+                            wb = ld_cnt64(&cnt[tb]);
+                            wc = ld_cnt64(&cnt[tc]);
                         }
                         if (!got && (unsigned long long)__double_as_longlong(v) != kSlotEmpty) {
                             dv = v;
                             got = true;
                         }
-                        // the scene's maxima are read only once its counters
-                        // are complete (they are final then)
-                        if (!resolved && c1 == want && c2 == want) {
-                            const unsigned long long* m = &P.scene_max[(size_t)sc * P.iterations];
-                            auto under = [&](int t) {
-                                return __longlong_as_double((long long)__hip_atomic_load(
-                                           const_cast<unsigned long long*>(&m[t - 1]), __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT)) < P.tolerance;
-                            };
-                            skip = (t1 >= 1 && under(t1)) || (t1 >= 2 && under(t1 - 1)) || (t2 >= 1 && under(t2));
+                        if (!resolved && (unsigned)(wa >> 32) == want && (unsigned)(wb >> 32) == want &&
+                            (unsigned)(wc >> 32) == want) {
+                            skip = (unsigned)wa == 0u || (unsigned)wb == 0u || (unsigned)wc == 0u;
                             resolved = true;
                         }
                         const bool row_resolved = __shfl(resolved, row * 16, 64);
@@ -1356,7 +1360,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             __builtin_amdgcn_s_waitcnt(0);  // LDS copies landed
             if (kScenes && pend_done) {
                 // the previous step's scene max has landed (the wait above)
-                __hip_atomic_fetch_add(pend_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(pend_done, pend_add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 pend_done = nullptr;
             }
             // empty the slot read above only now, so that this wait does not
@@ -1469,11 +1473,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                               (unsigned long long)__double_as_longlong(
                                   snan ? __longlong_as_double(0x7ff8000000000000ll) : smx));
                 pend_done = &P.scene_done[(size_t)sc * (P.iterations + 1) + p];
+                // over the tolerance (NaN included), as the scene's max would be
+                const bool over = active && !(smx <= 0) && (snan || !(smx < P.tolerance));
+                pend_add = (1ull << 32) | (over ? 1ull : 0ull);
             }
         }
         if (kScenes && pend_done) {
             __builtin_amdgcn_s_waitcnt(0);
-            __hip_atomic_fetch_add(pend_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(pend_done, pend_add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             pend_done = nullptr;
         }
         if (p >= 1) {

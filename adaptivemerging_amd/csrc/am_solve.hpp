@@ -60,7 +60,9 @@ struct PgsArgs {
     int n_scenes = 1;
     const int* g_scene = nullptr;              // scene of each sorted group
     unsigned long long* scene_max = nullptr;
-    unsigned* scene_done = nullptr;
+    // per scene and sweep (dataflow schedule): groups done in the high 32
+    // bits, those whose max |dl| was not under the tolerance in the low 32
+    unsigned long long* scene_done = nullptr;
     const unsigned* scene_groups = nullptr;    // groups per scene
     int* scene_used = nullptr;                 // sweeps each scene used (written after the solve)
 };
