@@ -570,7 +570,7 @@ def main(argv=None) -> int:
     achieved = res_p["_pgs_bytes"] / (res_p["_pgs_ms"] * 1e-3) / 1e9 if res_p["_pgs_ms"] > 0 else 0.0
     roofline = {
         "bound": "hbm",
-        "kernel": ("am::k_pgs_flow (persistent dataflow PGS)" if args.pgs_schedule == "dataflow"
+        "kernel": ("am::k_pgs_flow<false> (persistent dataflow PGS)" if args.pgs_schedule == "dataflow"
                    else "am::k_pgs<false> (persistent coloured PGS)"),
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
