@@ -227,7 +227,7 @@ int rebuild_structure(am_ctx* c, bool bodies_changed) {
         c->top_pinned[s] = (c->flags[e] & AM_BODY_PINNED) ? 1 : 0;
         c->top_sleeping[s] = (c->flags[e] & AM_BODY_SLEEPING) ? 1 : 0;
         c->gravity_targets.push_back(e);
-        if (c->desc[e].geom == AM_GEOM_COLLECTION)
+        if (is_collection(c, e))
             for (int q = 0; q < c->member_count[e]; q++) c->gravity_targets.push_back(c->members[c->member_first[e] + q]);
         if (!(c->flags[e] & (AM_BODY_PINNED | AM_BODY_SLEEPING))) c->integrate_targets.push_back(e);
     }
@@ -254,7 +254,7 @@ int rebuild_structure(am_ctx* c, bool bodies_changed) {
     c->member_list.clear();
     c->member_adv.clear();
     for (int e : c->slot_entity)
-        if (c->desc[e].geom == AM_GEOM_COLLECTION)
+        if (is_collection(c, e))
             for (int q = 0; q < c->member_count[e]; q++) {
                 int b = c->members[c->member_first[e] + q];
                 c->member_list.push_back(b);
@@ -2154,7 +2154,7 @@ int am_step(am_ctx* c, double dt, am_step_stats* st) {
         st->used_learned = c->used_learned;
         int nc = 0;
         for (int e : c->slot_entity)
-            if (c->desc[e].geom == AM_GEOM_COLLECTION) nc++;
+            if (is_collection(c, e)) nc++;
         st->n_collections = nc;
         st->n_merged = merged;
         st->n_unmerged = cut;

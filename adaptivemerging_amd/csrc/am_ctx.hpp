@@ -312,6 +312,11 @@ int merge_push_down(am_ctx* c, bool integrated_only);
 int merge_member_transforms(am_ctx* c);
 int pull_contacts(am_ctx* c, const am::ContactsView& V, int count, am_contact_arrays* o, int32_t capacity);
 bool has_collections(const am_ctx* c);
+// Physical bodies hold ids [0, n_phys) (uploads never hold collections);
+// collection entities are allocated after them, so a listed entity is a
+// collection exactly when its id is n_phys or more.  Loops over every
+// top-level entity use this instead of reading each descriptor.
+inline bool is_collection(const am_ctx* c, int e) { return e >= c->n_phys; }
 // sleeping phases (am_merge.hip)
 int ensure_sleep_state(am_ctx* c);
 int sleep_wake(am_ctx* c, int* n_woken);

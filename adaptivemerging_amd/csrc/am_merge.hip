@@ -1117,7 +1117,7 @@ int begin_event(am_ctx* c, Event& E) {
     E.b2c_changed.assign(n, 0);
     E.n_begin = (int)n;
     for (int e : c->slot_entity)
-        if (c->desc[e].geom == AM_GEOM_COLLECTION)
+        if (is_collection(c, e))
             for (int q = 0; q < c->member_count[e]; q++) E.members[e].push_back(c->members[c->member_first[e] + q]);
     return AM_OK;
 }
@@ -1232,7 +1232,7 @@ int commit_event(am_ctx* c, Event& E) {
         int e = c->slot_entity[s_];
         c->slot[e] = s_;
         c->parent[e] = -1;
-        if (c->desc[e].geom == AM_GEOM_COLLECTION) {
+        if (is_collection(c, e)) {
             c->member_first[e] = (int)c->members.size();
             c->member_count[e] = (int)E.members[e].size();
             int q = 0;
