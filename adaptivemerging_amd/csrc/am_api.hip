@@ -425,17 +425,19 @@ int build_layout(am_ctx* c) {
         launch_incidence(s, true, ng, c->g_a.p, c->g_b.p, c->d_flags.p, c->inc_count.p, c->inc_start.p,
                          c->inc_cursor.p, c->inc_items.p);
         fill_int(s, c->g_colour_a.p, ng, -1);
+        AM_HIP(c, c->g_blocker.ensure(std::max(ng, 1)));
+        fill_int(s, c->g_blocker.p, ng, -1);
         int* cin = c->g_colour_a.p;
         int* cout = c->g_colour_b.p;
-        // Rounds run in batches between reads of the uncoloured count: a
-        // round after the last one only copies the colours, so the colouring
-        // does not depend on the batch size.
+        // Rounds run in batches between reads of a flag that some group is
+        // still uncoloured: a round after the last one only copies the
+        // colours, so the colouring does not depend on the batch size.
         constexpr int kRoundsPerCheck = 8;
         for (int round = 0; round < 4096; round += kRoundsPerCheck) {
             for (int k = 0; k < kRoundsPerCheck; k++) {
                 fill_int(s, sc + 5, 1, 0);
                 launch_colour_round(s, ng, c->g_a.p, c->g_b.p, c->d_flags.p, c->inc_start.p, c->inc_count.p,
-                                    c->inc_items.p, cin, cout, sc + 5, c->layout_layer);
+                                    c->inc_items.p, cin, cout, sc + 5, c->layout_layer, c->g_blocker.p);
                 std::swap(cin, cout);
             }
             int r = read_scalars(c, 5, 1);

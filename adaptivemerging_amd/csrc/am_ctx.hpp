@@ -258,6 +258,7 @@ struct am_ctx {
     int last_merged = 0, last_unmerged = 0;
     // breadth-first layers of the single sweep's groups (getOrganizedContacts)
     DBuf<int> g_layer, bfs_mark;
+    DBuf<int> g_blocker;  // colouring: the neighbour a waiting group waits for
     const int* layout_layer = nullptr;  // set while the single sweep builds its layout
     DBuf<int> sweep_order;              // order of the last single sweep inside collections
     int sweep_order_n = -1;             // -1: none since the last solve or detection

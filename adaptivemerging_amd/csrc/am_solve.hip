@@ -133,7 +133,7 @@ AMD_INLINE bool higher(uint32_t h, uint32_t g) {
 // the parallel ones: those groups run in the sequential tail in colour order.
 __global__ void k_colour_round(int n_groups, const int* ga, const int* gb, const int* bflags, const int* inc_start,
                                const int* inc_count, const int* inc_items, const int* col_in, int* col_out,
-                               int* remaining, const int* layer) {
+                               int* remaining, const int* layer, int* blocker) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n_groups) return;
     int c = col_in[g];
@@ -143,7 +143,11 @@ __global__ void k_colour_round(int n_groups, const int* ga, const int* gb, const
     }
     const int lg = layer ? layer[g] : 0;
     int lo = 0;  // the colour must be at least lo (above every lower-layer neighbour)
-    bool ready = true;
+    // The neighbour that kept this group waiting in an earlier round keeps it
+    // waiting while it is uncoloured (priorities and layers do not change),
+    // so the incidence lists are scanned again only once it has its colour.
+    const int bl = blocker[g];
+    bool ready = !(bl >= 0 && col_in[bl] < 0);
     const int bodies[2] = {ga[g], gb[g]};
     if (layer) {
         for (int s = 0; s < 2 && ready; s++) {
@@ -156,6 +160,7 @@ __global__ void k_colour_round(int n_groups, const int* ga, const int* gb, const
                 int ch = col_in[h];
                 if (ch < 0) {
                     ready = false;
+                    blocker[g] = h;
                     break;
                 }
                 lo = max(lo, ch + 1);
@@ -174,6 +179,7 @@ __global__ void k_colour_round(int n_groups, const int* ga, const int* gb, const
             if (ch < 0) {
                 if (higher((uint32_t)h, (uint32_t)g)) {
                     ready = false;
+                    blocker[g] = h;
                     break;
                 }
             } else if (ch >= lo && ch - lo < 64) {
@@ -181,19 +187,19 @@ __global__ void k_colour_round(int n_groups, const int* ga, const int* gb, const
             }
         }
     }
-    if (!ready) {
-        col_out[g] = -1;
-        atomicAdd(remaining, 1);
-        return;
-    }
-    int pick;
-    if (!layer) {
-        pick = kMaxColours;  // sequential tail if all 64 parallel colours are taken
-        if (~used != 0) pick = __ffsll((long long)~used) - 1;
-    } else {
-        pick = lo + (~used != 0 ? __ffsll((long long)~used) - 1 : 64);
+    int pick = -1;
+    if (ready) {
+        if (!layer) {
+            pick = kMaxColours;  // sequential tail if all 64 parallel colours are taken
+            if (~used != 0) pick = __ffsll((long long)~used) - 1;
+        } else {
+            pick = lo + (~used != 0 ? __ffsll((long long)~used) - 1 : 64);
+        }
     }
     col_out[g] = pick;
+    // some group is still uncoloured: one flag store per wave
+    const unsigned long long waiting = __ballot(!ready);
+    if (waiting && (int)(threadIdx.x & 63) == __ffsll((long long)__ballot(1)) - 1) *remaining = 1;
 }
 
 // Sort key of a group: colour, then (inside a parallel colour, where groups
@@ -2187,10 +2193,10 @@ void launch_incidence(hipStream_t s, bool fill, int n_groups, const int* ga, con
 }
 void launch_colour_round(hipStream_t s, int n_groups, const int* ga, const int* gb, const int* bflags,
                          const int* inc_start, const int* inc_count, const int* inc_items, const int* col_in,
-                         int* col_out, int* remaining, const int* layer) {
+                         int* col_out, int* remaining, const int* layer, int* blocker) {
     if (n_groups <= 0) return;
     hipLaunchKernelGGL(k_colour_round, dim3((n_groups + 255) / 256), dim3(256), 0, s, n_groups, ga, gb, bflags,
-                       inc_start, inc_count, inc_items, col_in, col_out, remaining, layer);
+                       inc_start, inc_count, inc_items, col_in, col_out, remaining, layer, blocker);
 }
 void launch_colour_keys(hipStream_t s, int n_groups, const int* colour, const int* group_start, uint64_t* keys) {
     if (n_groups > 0)

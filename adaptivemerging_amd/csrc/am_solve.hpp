@@ -96,7 +96,7 @@ void launch_incidence(hipStream_t s, bool fill, int n_groups, const int* ga, con
                       int* inc_count, const int* inc_start, int* inc_cursor, int* inc_items);
 void launch_colour_round(hipStream_t s, int n_groups, const int* ga, const int* gb, const int* bflags,
                          const int* inc_start, const int* inc_count, const int* inc_items, const int* col_in,
-                         int* col_out, int* remaining, const int* layer = nullptr);
+                         int* col_out, int* remaining, const int* layer, int* blocker);
 void launch_colour_keys(hipStream_t s, int n_groups, const int* colour, const int* group_start, uint64_t* keys);
 void launch_colour_layout(hipStream_t s, int n_groups, const uint64_t* sorted, const int* group_start,
                           const int* group_a, const int* group_b, int* colour_start, int* sizes, int* pos_start,
