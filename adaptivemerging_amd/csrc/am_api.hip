@@ -215,6 +215,12 @@ int rebuild_structure(am_ctx* c, bool bodies_changed) {
             if (c->desc[i].geom == AM_GEOM_PLANE) c->planes.push_back(i);
             else c->grid_bodies.push_back(i);
         }
+        // planes by scene, so a body's pair test visits only its scene's
+        std::stable_sort(c->planes.begin(), c->planes.end(),
+                         [&](int a, int b) { return c->desc[a].scene < c->desc[b].scene; });
+        c->plane_start.assign(c->n_scenes + 1, 0);
+        for (int p : c->planes) c->plane_start[c->desc[p].scene + 1]++;
+        for (int k = 0; k < c->n_scenes; k++) c->plane_start[k + 1] += c->plane_start[k];
     }
     int n_slots = (int)c->slot_entity.size();
     if (n_slots >= kMaxSlots) return fail(c, AM_ERR_INVALID, "too many top-level bodies for the pair key");
@@ -241,6 +247,7 @@ int rebuild_structure(am_ctx* c, bool bodies_changed) {
     AM_HIP(c, upload(c, c->d_members, c->members));
     if (bodies_changed) {
         AM_HIP(c, upload(c, c->d_planes, c->planes));
+        AM_HIP(c, upload(c, c->d_plane_start, c->plane_start));
         AM_HIP(c, upload(c, c->d_grid, c->grid_bodies));
     }
     AM_HIP(c, upload(c, c->d_top_pinned, c->top_pinned));
@@ -538,7 +545,7 @@ int do_detect(am_ctx* c) {
     A.bucket_items = c->d_bucket_items.p;
     A.mask = c->hash_mask;
     A.planes = c->d_planes.p;
-    A.n_planes = (int)c->planes.size();
+    A.plane_start = c->d_plane_start.p;
     A.top_pinned = c->d_top_pinned.p;
     A.top_sleeping = c->d_top_sleeping.p;
     A.large = c->d_grid_large.p;

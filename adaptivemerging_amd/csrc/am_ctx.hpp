@@ -130,6 +130,8 @@ struct am_ctx {
     std::vector<int> node_sub;          // per node: sub-body (inside its composite) owning it, -1 otherwise
     std::vector<int> parent, slot, sub, slot_entity, member_first, member_count, members, flags;
     std::vector<int> planes, grid_bodies, node_body, top_pinned, top_sleeping, gravity_targets, integrate_targets;
+    std::vector<int> plane_start;  // per scene: first of its planes in `planes` (planes grouped by scene)
+    DBuf<int> d_plane_start;
     std::vector<int> grid_is_large, grid_large;  // per gridded body: too large for the grid; their gridded indices
     double cell_size = 1.0;
     int hash_mask = 0;

@@ -195,10 +195,9 @@ __global__ void k_pairs(PairArgs A, int* counts, const int* offsets, uint64_t* k
             if (d2 > rr * rr * (1.0 + 1e-9) + 1e-300) continue;
             emit(slot_j, B.sub[bj]);
         }
-        for (int q = 0; q < A.n_planes; q++) {
+        for (int q = A.plane_start[di.scene]; q < A.plane_start[di.scene + 1]; q++) {
             const int p = A.planes[q];
             const am_body_desc& pd = B.desc[p];
-            if (pd.scene != di.scene) continue;
             const int slot_p = B.slot[p];
             if (slot_p == slot_i) continue;
             if (skip_pair(A, slot_i, slot_p)) continue;

@@ -179,8 +179,8 @@ struct PairArgs {
     const int* bucket_count;
     const int* bucket_items;
     int mask;
-    const int* planes;
-    int n_planes;
+    const int* planes;       // grouped by scene, each group in body order
+    const int* plane_start;  // per scene: its planes are planes[plane_start[s] .. plane_start[s + 1])
     const int* top_pinned;   // per slot
     const int* top_sleeping; // per slot
     const int* large;        // gridded indices of the bodies too large for the grid (not in its buckets)
