@@ -925,7 +925,7 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
         AM_HIP(c, c->f_slot.ensure(n_slot));
         AM_HIP(c, c->d_pass_done.ensure(iterations + 2));
         fill_u64(s, c->f_slot.p, n_slot, kSlotEmpty);
-        fill_int(s, (int*)c->d_pass_done.p, iterations + 2, 0);
+        fill_int(s, (int*)c->d_pass_done.p, 2 * (iterations + 2), 0);
         AM_HIP(c, hipMemcpyAsync(c->dv_alt.p, c->d_dv.p, n_dv * sizeof(double), hipMemcpyDeviceToDevice, s));
         const int fgrid = grid;
         FlowArgs F;

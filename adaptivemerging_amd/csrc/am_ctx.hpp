@@ -173,7 +173,7 @@ struct am_ctx {
     // dataflow schedule: sorted position per group, ranks and degrees, previous-update
     // incidences, pass completion counts, deltaV slots, odd-pass buffers
     DBuf<int> g_spos, s_flow, s_pred;
-    DBuf<unsigned> d_pass_done;
+    DBuf<unsigned long long> d_pass_done;
     DBuf<double> f_slot;
     DBuf<double> dv_alt, s_lam_alt;
     int flow_max_grid = 0;

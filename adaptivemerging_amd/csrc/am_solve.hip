@@ -1165,11 +1165,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             const int need = (p == lastp && p >= 2) ? p - 1 : p - 2;
             long spins = 0;
             while (chk < need) {
-                const unsigned d = (unsigned)uniform((int)ld_cnt(&F.pass_done[chk + 1]));
+                // one read: the waves done with sweep chk + 1 and those over
+                // the tolerance there (the sweep converged if none was)
+                const unsigned long long wd = ld_cnt64(&F.pass_done[chk + 1]);
+                const unsigned d = (unsigned)uniform((int)(unsigned)(wd >> 32));
                 if (d == F.n_waves) {
-                    const double m = uniform_d(__longlong_as_double((long long)__hip_atomic_load(
-                        &P.iter_max[chk], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
-                    if (m < P.tolerance) return;  // converged at sweep chk + 1
+                    if (uniform((int)(unsigned)wd) == 0) return;  // converged at sweep chk + 1
                     chk++;
                     spins = 0;
                     continue;
@@ -1495,8 +1496,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             if (lane == 0) {
                 if (!(mx <= 0))  // NaN included
                     atomicMax(&P.iter_max[p - 1], (unsigned long long)__double_as_longlong(mx));
+                // the wait also lands this wave's lambda stores before its next
+                // pass reads them back
                 __builtin_amdgcn_s_waitcnt(0);
-                __hip_atomic_fetch_add(&F.pass_done[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const bool over = !(mx <= 0) && !(mx < P.tolerance);  // NaN over
+                __hip_atomic_fetch_add(&F.pass_done[p], (1ull << 32) | (over ? 1ull : 0ull), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
             }
         }
     }

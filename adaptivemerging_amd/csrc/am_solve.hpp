@@ -82,7 +82,9 @@ struct FlowArgs {
     int n_groups;
     double* dv_alt;        // deltaV of odd passes (a copy of dv at launch)
     double* lam_alt;       // lambda of odd sweeps, 4 per sweep position (3 used)
-    unsigned* pass_done;   // per pass: waves that have finished it (zeroed before launch)
+    // per pass (zeroed before launch): waves that have finished it in the
+    // high 32 bits, those whose max |dl| was not under the tolerance in the low
+    unsigned long long* pass_done;
     unsigned n_waves;
 };
 
