@@ -1299,6 +1299,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                         ((spins & 255) == 0 && uniform(__hip_atomic_load(P.error, __ATOMIC_RELAXED,
                                                                           __HIP_MEMORY_SCOPE_AGENT)))) {
                         if (lane == 0) atomicOr(P.error, 1);
+                        __builtin_amdgcn_s_waitcnt(0);  // this step's LDS copies land before the wave ends
                         return false;
                     }
                     return true;
