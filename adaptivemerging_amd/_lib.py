@@ -90,7 +90,7 @@ class Params(C.Structure):
         ("pgs_block", C.c_int32), ("pgs_grid", C.c_int32), ("pgs_plain_launch", C.c_int32),
         ("pgs_schedule", C.c_int32),
         ("learned_warm_start", C.c_int32), ("collect_training", C.c_int32),
-        ("organize_contacts", C.c_int32), ("reserved_params", C.c_int32),
+        ("organize_contacts", C.c_int32), ("host_threads", C.c_int32),
     ]
 
 
@@ -305,7 +305,7 @@ _DEFAULTS = dict(
     update_contacts_in_collections=1, step_accum_merging=3, step_accum_unmerging=3, steps_between_merge_events=10,
     metric_position_level=0,
     threshold_merge=1e-2, threshold_unmerge=2e-2, threshold_breath=1e-5, pgs_block=0, pgs_grid=0, pgs_plain_launch=0, pgs_schedule=0,
-    learned_warm_start=0, collect_training=0, organize_contacts=1,
+    learned_warm_start=0, collect_training=0, organize_contacts=1, host_threads=0,
 )
 
 

@@ -203,6 +203,7 @@ void default_params(am_params* p) {
     p->pgs_plain_launch = 0;
     p->pgs_schedule = 0;
     p->organize_contacts = 1;
+    p->host_threads = 0;
 }
 
 // Derived host lists from the structure (parent/slot/sub/members).
@@ -394,6 +395,7 @@ int am_params_set(am_ctx* c, const am_params* p) {
     if (p->pgs_schedule != 0 && p->pgs_schedule != 1) return fail(c, AM_ERR_INVALID, "pgs_schedule must be 0 or 1");
     if (p->sleep_step_accum < 0 || p->sleep_step_accum > am::kMaxSleepAccum)
         return fail(c, AM_ERR_INVALID, "sleep_step_accum must be in 0..200");
+    if (p->host_threads < 0) return fail(c, AM_ERR_INVALID, "host_threads must be >= 0");
     c->params = *p;
     return AM_OK;
 }

@@ -579,18 +579,19 @@ M3 get_op(V3 bx, double mass, V3 com) {
     return op;
 }
 
-// Runs body(k) for every k in [0, n) on up to kEventThreads host threads, in
-// contiguous ranges of about equal total weight (serially when the work is
-// small).  The jobs of an event write disjoint entities, so the result is
-// the serial one.
+// Runs body(k) for every k in [0, n) on up to kEventThreads host threads
+// (at most max_threads when that is positive), in contiguous ranges of about
+// equal total weight (serially when the work is small).  The jobs of an
+// event write disjoint entities, so the result is the serial one.
 constexpr int kEventThreads = 16;
 constexpr long kWorkPerThread = 4096;  // weight units (bodies) worth a thread
 template <class W, class F>
-void parallel_jobs(int n, W weight, F body) {
+void parallel_jobs(int max_threads, int n, W weight, F body) {
     long total = 0;
     for (int k = 0; k < n; k++) total += weight(k);
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const int nt = (int)std::min<long>(std::min<long>(hw, kEventThreads), std::max(total / kWorkPerThread, 1L));
+    const int cap = max_threads > 0 ? std::min(max_threads, kEventThreads) : kEventThreads;
+    const int nt = (int)std::min<long>(std::min<long>(hw, cap), std::max(total / kWorkPerThread, 1L));
     if (nt <= 1 || n <= 1) {
         for (int k = 0; k < n; k++) body(k);
         return;
@@ -844,7 +845,8 @@ void update_bodies_transformations(Event& E, int ce, const Ent& C) {
 void flush_transformations(Event& E) {
     am_ctx* c = E.c;
     const std::vector<int>& L = E.b2c_list;
-    parallel_jobs((int)L.size(), [&](int k) { return (long)E.members[L[k]].size(); }, [&](int k) {
+    auto weight = [&](int k) { return (long)E.members[L[k]].size(); };
+    parallel_jobs(c->params.host_threads, (int)L.size(), weight, [&](int k) {
         const int ce = L[k];
         E.b2c_dirty[ce] = 0;
         const V3 xc = v3(&E.x[ce * 3]);
@@ -1773,7 +1775,8 @@ int merge_unmerge(am_ctx* c, double dt, int* n_events) {
             if (remove_coll) removals.push_back(ce);
         }
     }
-    parallel_jobs((int)removals_left.size(), [&](int k) { return (long)E.members[removals_left[k].ce].size(); },
+    parallel_jobs(c->params.host_threads, (int)removals_left.size(),
+                  [&](int k) { return (long)E.members[removals_left[k].ce].size(); },
                   [&](int k) {
                       Removal& R = removals_left[k];
                       remove_bodies(E, R.ce, R.C, R.gone);

@@ -180,7 +180,9 @@ typedef struct am_params {
     int32_t organize_contacts;        /* 1 (Merging.organizeContacts): the single sweep inside collections
                                          visits the body pairs breadth first from those holding a new
                                          contact (getOrganizedContacts); 0: colour order only */
-    int32_t reserved_params;
+    int32_t host_threads;             /* host threads a merge or unmerge event may split its per-collection
+                                         work over (0 = up to 16; 1 = the calling thread only); the
+                                         results do not depend on it (no reference counterpart) */
 } am_params;
 
 /* A spring (Spring.java:10-104).  ZERO pulls body1's point pb1 towards the

@@ -68,6 +68,7 @@ int main(void) {
   F(am_spring, type) F(am_spring, body2) F(am_spring, pb1) F(am_spring, pb2) F(am_spring, pw) F(am_spring, l0)
   F(am_spring, k) F(am_spring, d) F(am_spring, ls)
   F(am_params, spring_stiffness_mod) F(am_params, spring_damping_mod) F(am_params, pgs_schedule) F(am_params, organize_contacts)
+  F(am_params, host_threads)
   F(am_tune_trial, grid_used) F(am_tune_trial, block_used) F(am_tune_trial, ms_per_sweep)
   return 0;
 }

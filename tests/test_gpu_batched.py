@@ -177,3 +177,34 @@ def test_batched_scene_stays_stopped_when_a_later_sweep_rises(gpu_lib, schedule)
         if tested:
             break
     assert tested, "no step had a sweep under all earlier ones followed by a rise"
+
+
+def test_batched_merging_host_threads_do_not_change_results(gpu_lib):
+    """The 256 batched scenes with merging on, 150 steps (merge events of
+    thousands of pairs every tenth step, unmerge events cutting hundreds of
+    collections at most steps): the events' per-collection work split over
+    the host's threads (host_threads 0) gives the states, the collection
+    structure and the contacts of the calling thread alone (host_threads 1),
+    bit for bit."""
+    scene = workloads.batched_piles(seed=0)
+    runs = []
+    for threads in (1, 0):
+        ctx = Context()
+        ctx.upload(scene)
+        ctx.set_params(gravity_amount=9.81, iterations=50, enable_sleeping=0, host_threads=threads)
+        merged = 0
+        for _ in range(150):
+            merged += ctx.step(0.01)["n_merged"]
+        runs.append((ctx.pull_state(), ctx.structure(), ctx.contacts(), merged, ctx.entities()))
+        print("host_threads", threads, "merged", merged, flush=True)
+        ctx.close()
+    (s1, t1, c1, m1, e1), (s0, t0, c0, m0, e0) = runs
+    assert m1 == m0 and m1 > 10000
+    # collections' mass properties and member boxes (from the member transforms)
+    assert e1.tobytes() == e0.tobytes()
+    for name, a, b in zip(("x", "R", "v", "w"), s1, s0):
+        assert_bits_equal(name, a, b)
+    for a, b in zip(t1, t0):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+    for key in ("body1", "body2", "lambda"):
+        assert_bits_equal(key, c1[key], c0[key])
