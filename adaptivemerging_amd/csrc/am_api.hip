@@ -925,9 +925,9 @@ int do_solve(am_ctx* c, double dt, int post, bool main_solve, int* iters, double
         AM_HIP(c, c->dv_alt.ensure(std::max(n_dv, (size_t)6)));
         const size_t n_slot = (size_t)2 * 2 * std::max(c->n_groups, 1) * kSlotDoubles;
         AM_HIP(c, c->f_slot.ensure(n_slot));
-        AM_HIP(c, c->d_pass_done.ensure(iterations + 2));
+        AM_HIP(c, c->d_pass_done.ensure((size_t)(iterations + 2) * kPassWords));
         fill_u64(s, c->f_slot.p, n_slot, kSlotEmpty);
-        fill_int(s, (int*)c->d_pass_done.p, 2 * (iterations + 2), 0);
+        fill_int(s, (int*)c->d_pass_done.p, 2 * (iterations + 2) * kPassWords, 0);
         AM_HIP(c, hipMemcpyAsync(c->dv_alt.p, c->d_dv.p, n_dv * sizeof(double), hipMemcpyDeviceToDevice, s));
         const int fgrid = grid;
         FlowArgs F;

@@ -1165,9 +1165,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             const int need = (p == lastp && p >= 2) ? p - 1 : p - 2;
             long spins = 0;
             while (chk < need) {
-                // one read: the waves done with sweep chk + 1 and those over
-                // the tolerance there (the sweep converged if none was)
-                const unsigned long long wd = ld_cnt64(&F.pass_done[chk + 1]);
+                // the waves done with sweep chk + 1 and those over the
+                // tolerance there (the sweep converged if none was), summed
+                // over the sweep's words (read together)
+                unsigned long long wd = 0;
+                const unsigned long long* pw = &F.pass_done[(size_t)(chk + 1) * kPassWords];
+#pragma unroll
+                for (int q = 0; q < kPassWords; q++) wd += ld_cnt64(&pw[q]);
                 const unsigned d = (unsigned)uniform((int)(unsigned)(wd >> 32));
                 if (d == F.n_waves) {
                     if (uniform((int)(unsigned)wd) == 0) return;  // converged at sweep chk + 1
@@ -1501,8 +1505,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 // pass reads them back
                 __builtin_amdgcn_s_waitcnt(0);
                 const bool over = !(mx <= 0) && !(mx < P.tolerance);  // NaN over
-                __hip_atomic_fetch_add(&F.pass_done[p], (1ull << 32) | (over ? 1ull : 0ull), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(&F.pass_done[(size_t)p * kPassWords + (w % kPassWords)],
+                                       (1ull << 32) | (over ? 1ull : 0ull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
     }

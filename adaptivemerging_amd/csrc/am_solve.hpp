@@ -74,6 +74,8 @@ struct PgsArgs {
 // the value and empties the slot again.
 constexpr int kSlotDoubles = 8;
 constexpr unsigned long long kSlotEmpty = 0xFFF5A5A5A5A5A5A5ull;  // a signalling NaN: no arithmetic result
+constexpr int kPassWords = 8;  // the pass words spread the waves' adds over this many addresses
+
 struct FlowArgs {
     PgsArgs P;
     const int* flow;       // int4 per sorted group: rank_a, deg_a, rank_b, deg_b
@@ -82,8 +84,9 @@ struct FlowArgs {
     int n_groups;
     double* dv_alt;        // deltaV of odd passes (a copy of dv at launch)
     double* lam_alt;       // lambda of odd sweeps, 4 per sweep position (3 used)
-    // per pass (zeroed before launch): waves that have finished it in the
-    // high 32 bits, those whose max |dl| was not under the tolerance in the low
+    // per pass, kPassWords words (zeroed before launch): waves that have
+    // finished it in the high 32 bits, those whose max |dl| was not under the
+    // tolerance in the low; wave w adds to word w % kPassWords
     unsigned long long* pass_done;
     unsigned n_waves;
 };
