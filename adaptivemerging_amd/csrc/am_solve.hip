@@ -875,15 +875,7 @@ AMD_INLINE double* slot_at(const FlowArgs& F, int parity, int incidence) {
 AMD_INLINE unsigned long long ld_cnt64(const unsigned long long* p) {
     return __hip_atomic_load(const_cast<unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-AMD_INLINE unsigned ld_cnt(const unsigned* p) {
-    return __hip_atomic_load(const_cast<unsigned*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 AMD_INLINE int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
-AMD_INLINE double uniform_d(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readfirstlane((int)b), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
 // DPP move of a double (bound_ctrl: lanes without a source read 0)
 template <int kCtrl>
 AMD_INLINE double dpp_d(double v) {
